@@ -1,0 +1,106 @@
+/*
+ * CPU oracle for the kernel-PC conditional-independence path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library restates, in plain serial C, the
+ * reference algorithm (kpcalg 1.0.1 + mgcv 1.9-1 + pcalg stable skeleton, as
+ * replicated by fastkpc's C++ sources) so that tests, the smoke check and the
+ * benchmark's cpu_baseline leg can check the HIP product against it.  The
+ * product library never links, loads or calls it.
+ */
+#ifndef KPC_ORACLE_H
+#define KPC_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- dcov.gamma (kpcalg/R/dcovgamma.R:56-82) ---- */
+double orc_pgamma_lower(double x, double a);
+double orc_gamma_pvalue(double stat, double mean, double var);
+int orc_dcov_component(const double* x, int n, int k, double* values, double* centered,
+                       double* dsum, double* moment);
+double orc_weighted_cross_sum(const double* cx, const double* lx, const double* cy,
+                              const double* ly, int n, int k);
+void orc_dcov_pair_from_components(int n, int k, const double* cx, const double* lx,
+                                   double dsum_x, double mom_x, const double* cy,
+                                   const double* ly, double dsum_y, double mom_y,
+                                   double out[5]);
+/* out = {p.value, nV2, mean, variance, statistic} */
+int orc_dcov_gamma(const double* x, const double* y, int n, int num_col, double out[5]);
+
+/* ---- regrXonS (kpcalg/R/regrXonS.R:33-61, mgcv gam GCV.Cp) ---- */
+typedef struct {
+  int n, q, m;  /* rows, coefficients, penalties */
+  double* X;    /* n x q column-major, first column the intercept */
+  double** S;   /* m penalty blocks, dim[i] x dim[i] column-major */
+  int* off;     /* 0-based first coefficient of each block */
+  int* dim;
+  int* rank;
+} orc_setup;
+
+typedef struct {
+  double tol;
+  int max_step_halving_single;
+  int max_step_halving_multi;
+  int max_iterations;
+  double max_newton_step;
+  double boundary_probe_step;
+  int max_boundary_probes;
+  double rank_tol;
+} orc_magic_control;
+
+typedef struct {
+  double score, rss, edf;
+  double log_sp[64];
+  int iterations, evaluations, step_failed;
+} orc_magic_result;
+
+void orc_magic_default_control(orc_magic_control* c);
+int orc_regr_setup(const double* data, int n, int ld, const int* S, int s_len, orc_setup* out);
+void orc_setup_free(orc_setup* s);
+int orc_magic_fit(const orc_setup* s, const double* y, const orc_magic_control* ctl,
+                  double* resid, orc_magic_result* res);
+int orc_magic_eval(const orc_setup* s, const double* y, const double* lsp, double rank_tol,
+                   double* score, double* grad, double* hess);
+int orc_regrxons(const double* data, int n, int ld, int target, const int* S, int s_len,
+                 const orc_magic_control* ctl, double* resid, orc_magic_result* res);
+
+/* ---- pcalg stable skeleton (fastkpc/src/skeleton_engine.cpp:96-227) ---- */
+enum { ORC_DCC_GAMMA = 0, ORC_HSIC_GAMMA = 1, ORC_DCC_PERM = 2, ORC_HSIC_PERM = 3 };
+
+typedef struct {
+  int ci_method;
+  double alpha;
+  int m_max;        /* resolved maximum conditioning size */
+  int num_col;
+  double hsic_sig;
+  int replicates;   /* permutation methods */
+  unsigned long long perm_seed;
+  int threads;      /* OpenMP threads for the per-level precompute (0 = serial) */
+} orc_skeleton_options;
+
+typedef struct {
+  int p, levels;
+  unsigned char* adjacency; /* p x p row-major */
+  double* pmax;             /* p x p row-major */
+  int* sepset_len;          /* p x p; -1 = none */
+  int* sepset;              /* p x p x 64 */
+  int n_edgetests[64];
+  int trace_len, trace_cap;
+  int* trace_level; int* trace_x; int* trace_y; int* trace_slen; int* trace_S; /* S: cap x 64 */
+  double* trace_p;
+  unsigned char* trace_del;
+} orc_skeleton_result;
+
+int orc_skeleton(const double* data, int n, int p, const orc_skeleton_options* opt,
+                 orc_skeleton_result* res);
+void orc_skeleton_result_free(orc_skeleton_result* r);
+
+/* single CI test p-value (kernelCItest semantics) */
+int orc_ci_test(const double* data, int n, int p, int x, int y, const int* S, int s_len,
+                const orc_skeleton_options* opt, double out[5]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,326 @@
+/*
+ * pcalg "stable" skeleton driven by kernelCItest, for the CPU oracle (test
+ * infrastructure only).
+ *
+ * Follows fastkpc/src/skeleton_engine.cpp:96-227 (CPU twin of pcalg
+ * skeleton(method="stable")) and fastkpc/src/skeleton_task_scheduler.cpp:
+ * 82-121 (task order): per level take an adjacency snapshot; for every edge
+ * x<y test x against subsets of adj(x)\{y} in lexicographic order, stop at the
+ * first p >= alpha (delete, sepset = S), otherwise test y against subsets of
+ * adj(y)\{x}; deletions apply at level end; pcalg's `done` flag ends the
+ * search when no visited orientation had more than `level` neighbours.
+ * kernelCItest semantics: kpcalg/R/kernelCItest.R:110-128 (raw columns at
+ * |S| = 0, regrXonS residuals otherwise).
+ */
+#include "kpc_oracle.h"
+#include "orc_internal.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef struct {
+  uint64_t key_mask;
+  int target;
+  int used;
+  double* values;
+  double* vecs;
+  double dsum, moment;
+} comp_entry;
+
+typedef struct {
+  comp_entry* slots;
+  size_t cap;
+  size_t count;
+} comp_cache;
+
+static uint64_t mix(uint64_t x) {
+  x ^= x >> 33; x *= 0xff51afd7ed558ccdULL; x ^= x >> 33; x *= 0xc4ceb9fe1a85ec53ULL; x ^= x >> 33;
+  return x;
+}
+
+static comp_entry* cache_find(comp_cache* c, int target, uint64_t mask, int insert) {
+  if (insert && (c->count + 1) * 2 > c->cap) {
+    size_t ncap = c->cap ? c->cap * 2 : 1024;
+    comp_entry* ns = (comp_entry*)calloc(ncap, sizeof(comp_entry));
+    for (size_t i = 0; i < c->cap; ++i) {
+      if (!c->slots[i].used) continue;
+      size_t h = mix(c->slots[i].key_mask * 131ULL + (uint64_t)c->slots[i].target) & (ncap - 1);
+      while (ns[h].used) h = (h + 1) & (ncap - 1);
+      ns[h] = c->slots[i];
+    }
+    free(c->slots);
+    c->slots = ns;
+    c->cap = ncap;
+  }
+  if (!c->cap) return NULL;
+  size_t h = mix(mask * 131ULL + (uint64_t)target) & (c->cap - 1);
+  while (c->slots[h].used) {
+    if (c->slots[h].target == target && c->slots[h].key_mask == mask) return &c->slots[h];
+    h = (h + 1) & (c->cap - 1);
+  }
+  if (!insert) return NULL;
+  c->slots[h].used = 1;
+  c->slots[h].target = target;
+  c->slots[h].key_mask = mask;
+  c->count++;
+  return &c->slots[h];
+}
+
+static void cache_free(comp_cache* c) {
+  for (size_t i = 0; i < c->cap; ++i)
+    if (c->slots[i].used) { free(c->slots[i].values); free(c->slots[i].vecs); }
+  free(c->slots);
+}
+
+static uint64_t mask_of(const int* S, int s_len) {
+  uint64_t m = 0;
+  for (int i = 0; i < s_len; ++i) m |= 1ULL << S[i];
+  return m;
+}
+
+/* residual of `target` on S (raw column when S is empty) */
+static int residual_for(const double* data, int n, int target, const int* S, int s_len,
+                        double* out) {
+  if (s_len == 0) {
+    memcpy(out, data + (size_t)target * n, sizeof(double) * n);
+    return 0;
+  }
+  orc_magic_control ctl;
+  orc_magic_default_control(&ctl);
+  orc_magic_result res;
+  return orc_regrxons(data, n, n, target, S, s_len, &ctl, out, &res);
+}
+
+typedef struct {
+  double* values;
+  double* vecs;
+  double dsum, moment;
+  int rc;
+} comp_val;
+
+static comp_val compute_component(const double* data, int n, int target, const int* S,
+                                  int s_len, int k) {
+  comp_val v;
+  v.values = (double*)malloc(sizeof(double) * k);
+  v.vecs = (double*)malloc(sizeof(double) * (size_t)n * k);
+  double* r = (double*)malloc(sizeof(double) * n);
+  v.rc = residual_for(data, n, target, S, s_len, r);
+  if (v.rc == 0) v.rc = orc_dcov_component(r, n, k, v.values, v.vecs, &v.dsum, &v.moment);
+  free(r);
+  return v;
+}
+
+static int get_component(comp_cache* c, const double* data, int n, int target, const int* S,
+                         int s_len, int k, comp_entry** out) {
+  uint64_t m = mask_of(S, s_len);
+  comp_entry* e = cache_find(c, target, m, 0);
+  if (!e) {
+    comp_val v = compute_component(data, n, target, S, s_len, k);
+    if (v.rc != 0) { free(v.values); free(v.vecs); return v.rc; }
+    e = cache_find(c, target, m, 1);
+    e->values = v.values; e->vecs = v.vecs; e->dsum = v.dsum; e->moment = v.moment;
+  }
+  *out = e;
+  return 0;
+}
+
+static int ci_pvalue(comp_cache* c, const double* data, int n, int x, int y, const int* S,
+                     int s_len, const orc_skeleton_options* opt, double out[5]) {
+  if (opt->ci_method != ORC_DCC_GAMMA) return -20;
+  comp_entry *ex, *ey;
+  int rc = get_component(c, data, n, x, S, s_len, opt->num_col, &ex);
+  if (rc) return rc;
+  rc = get_component(c, data, n, y, S, s_len, opt->num_col, &ey);
+  if (rc) return rc;
+  orc_dcov_pair_from_components(n, opt->num_col, ex->vecs, ex->values, ex->dsum, ex->moment,
+                                ey->vecs, ey->values, ey->dsum, ey->moment, out);
+  return 0;
+}
+
+int orc_ci_test(const double* data, int n, int p, int x, int y, const int* S, int s_len,
+                const orc_skeleton_options* opt, double out[5]) {
+  (void)p;
+  comp_cache c = {0};
+  int rc = ci_pvalue(&c, data, n, x, y, S, s_len, opt, out);
+  cache_free(&c);
+  return rc;
+}
+
+/* lexicographic k-subsets of nb[0..len-1] */
+static int next_comb(int* idx, int k, int len) {
+  int i = k - 1;
+  while (i >= 0 && idx[i] == len - k + i) --i;
+  if (i < 0) return 0;
+  ++idx[i];
+  for (int j = i + 1; j < k; ++j) idx[j] = idx[j - 1] + 1;
+  return 1;
+}
+
+static void trace_push(orc_skeleton_result* r, int level, int x, int y, const int* S, int s_len,
+                       double pv, int del) {
+  if (r->trace_len == r->trace_cap) {
+    int nc = r->trace_cap ? r->trace_cap * 2 : 4096;
+    r->trace_level = (int*)realloc(r->trace_level, sizeof(int) * nc);
+    r->trace_x = (int*)realloc(r->trace_x, sizeof(int) * nc);
+    r->trace_y = (int*)realloc(r->trace_y, sizeof(int) * nc);
+    r->trace_slen = (int*)realloc(r->trace_slen, sizeof(int) * nc);
+    r->trace_S = (int*)realloc(r->trace_S, sizeof(int) * (size_t)nc * 64);
+    r->trace_p = (double*)realloc(r->trace_p, sizeof(double) * nc);
+    r->trace_del = (unsigned char*)realloc(r->trace_del, nc);
+    r->trace_cap = nc;
+  }
+  int t = r->trace_len++;
+  r->trace_level[t] = level;
+  r->trace_x[t] = x;
+  r->trace_y[t] = y;
+  r->trace_slen[t] = s_len;
+  for (int i = 0; i < s_len; ++i) r->trace_S[(size_t)t * 64 + i] = S[i];
+  r->trace_p[t] = pv;
+  r->trace_del[t] = (unsigned char)del;
+}
+
+/* Fill the component cache for every (target, S) the level could request. */
+static int prefetch_level(comp_cache* c, const double* data, int n, int p, const unsigned char* snap,
+                          int level, const orc_skeleton_options* opt) {
+  int cap = 1024, cnt = 0;
+  int* tg = (int*)malloc(sizeof(int) * cap);
+  int* sl = (int*)malloc(sizeof(int) * (size_t)cap * 64);
+  int nb[64], idx[64], S[64];
+  for (int x = 0; x < p; ++x)
+    for (int y = 0; y < p; ++y) {
+      if (x == y || !snap[x * p + y]) continue;
+      int len = 0;
+      for (int i = 0; i < p; ++i) if (i != y && snap[i * p + x]) nb[len++] = i;
+      if (len < level) continue;
+      for (int i = 0; i < level; ++i) idx[i] = i;
+      do {
+        for (int i = 0; i < level; ++i) S[i] = nb[idx[i]];
+        for (int side = 0; side < 2; ++side) {
+          int t = side ? y : x;
+          if (cache_find(c, t, mask_of(S, level), 0)) continue;
+          int dup = 0;
+          uint64_t m = mask_of(S, level);
+          for (int q = cnt - 1; q >= 0 && q >= cnt - 8; --q)
+            if (tg[q] == t && mask_of(sl + (size_t)q * 64, level) == m) dup = 1;
+          if (dup) continue;
+          if (cnt == cap) {
+            cap *= 2;
+            tg = (int*)realloc(tg, sizeof(int) * cap);
+            sl = (int*)realloc(sl, sizeof(int) * (size_t)cap * 64);
+          }
+          tg[cnt] = t;
+          memcpy(sl + (size_t)cnt * 64, S, sizeof(int) * level);
+          ++cnt;
+          /* reserve the key so later duplicates are skipped */
+          comp_entry* e = cache_find(c, t, m, 1);
+          e->values = NULL;
+          e->vecs = NULL;
+        }
+      } while (level > 0 && next_comb(idx, level, len));
+    }
+  int rc = 0;
+  comp_val* vals = (comp_val*)calloc(cnt ? cnt : 1, sizeof(comp_val));
+#pragma omp parallel for schedule(dynamic, 1) num_threads(opt->threads > 0 ? opt->threads : 1)
+  for (int i = 0; i < cnt; ++i)
+    vals[i] = compute_component(data, n, tg[i], sl + (size_t)i * 64, level, opt->num_col);
+  for (int i = 0; i < cnt; ++i) {
+    comp_entry* e = cache_find(c, tg[i], mask_of(sl + (size_t)i * 64, level), 0);
+    if (vals[i].rc != 0) { rc = vals[i].rc; free(vals[i].values); free(vals[i].vecs); continue; }
+    e->values = vals[i].values;
+    e->vecs = vals[i].vecs;
+    e->dsum = vals[i].dsum;
+    e->moment = vals[i].moment;
+  }
+  free(vals); free(tg); free(sl);
+  return rc;
+}
+
+int orc_skeleton(const double* data, int n, int p, const orc_skeleton_options* opt,
+                 orc_skeleton_result* r) {
+  memset(r, 0, sizeof(*r));
+  if (p < 2 || p > 64) return -30;
+  r->p = p;
+  r->adjacency = (unsigned char*)malloc((size_t)p * p);
+  r->pmax = (double*)malloc(sizeof(double) * (size_t)p * p);
+  r->sepset_len = (int*)malloc(sizeof(int) * (size_t)p * p);
+  r->sepset = (int*)calloc((size_t)p * p * 64, sizeof(int));
+  for (int i = 0; i < p * p; ++i) { r->adjacency[i] = 1; r->pmax[i] = -INFINITY; r->sepset_len[i] = -1; }
+  for (int i = 0; i < p; ++i) { r->adjacency[i * p + i] = 0; r->pmax[i * p + i] = 1.0; }
+  unsigned char* snap = (unsigned char*)malloc((size_t)p * p);
+  unsigned char* del = (unsigned char*)malloc((size_t)p * p);
+  comp_cache cache = {0};
+  int nb[64], idx[64], S[64];
+  int done = 0, ord = 0, rc = 0;
+  while (!done && ord <= opt->m_max) {
+    int any = 0;
+    for (int i = 0; i < p * p; ++i) any |= r->adjacency[i];
+    if (!any) break;
+    done = 1;
+    memcpy(snap, r->adjacency, (size_t)p * p);
+    memset(del, 0, (size_t)p * p);
+    if (opt->threads > 0) {
+      rc = prefetch_level(&cache, data, n, p, snap, ord, opt);
+      if (rc) break;
+    }
+    int tests = 0;
+    for (int a = 0; a < p - 1 && !rc; ++a)
+      for (int b = a + 1; b < p && !rc; ++b) {
+        if (!snap[a * p + b]) continue;
+        for (int side = 0; side < 2 && !rc; ++side) {
+          int x = side ? b : a, y = side ? a : b;
+          if (del[a * p + b]) break;
+          int len = 0;
+          for (int i = 0; i < p; ++i) if (i != y && snap[i * p + x]) nb[len++] = i;
+          if (len < ord) continue;
+          if (len > ord) done = 0;
+          for (int i = 0; i < ord; ++i) idx[i] = i;
+          do {
+            for (int i = 0; i < ord; ++i) S[i] = nb[idx[i]];
+            double out[5];
+            rc = ci_pvalue(&cache, data, n, x, y, S, ord, opt, out);
+            if (rc) break;
+            double pv = out[0];
+            if (isnan(pv)) pv = 1.0;
+            ++tests;
+            if (r->pmax[x * p + y] < pv) r->pmax[x * p + y] = pv;
+            int deleted = pv >= opt->alpha;
+            trace_push(r, ord, x, y, S, ord, pv, deleted);
+            if (deleted) {
+              del[a * p + b] = del[b * p + a] = 1;
+              r->sepset_len[a * p + b] = r->sepset_len[b * p + a] = ord;
+              for (int i = 0; i < ord; ++i) {
+                r->sepset[((size_t)a * p + b) * 64 + i] = S[i];
+                r->sepset[((size_t)b * p + a) * 64 + i] = S[i];
+              }
+              break;
+            }
+          } while (ord > 0 && next_comb(idx, ord, len));
+        }
+      }
+    for (int i = 0; i < p * p; ++i) if (del[i]) r->adjacency[i] = 0;
+    r->n_edgetests[ord] = tests;
+    ++ord;
+  }
+  r->levels = ord;
+  for (int i = 0; i < p - 1; ++i)
+    for (int j = i + 1; j < p; ++j) {
+      double m = r->pmax[i * p + j] > r->pmax[j * p + i] ? r->pmax[i * p + j] : r->pmax[j * p + i];
+      r->pmax[i * p + j] = r->pmax[j * p + i] = m;
+    }
+  cache_free(&cache);
+  free(snap);
+  free(del);
+  return rc;
+}
+
+void orc_skeleton_result_free(orc_skeleton_result* r) {
+  free(r->adjacency); free(r->pmax); free(r->sepset_len); free(r->sepset);
+  free(r->trace_level); free(r->trace_x); free(r->trace_y); free(r->trace_slen);
+  free(r->trace_S); free(r->trace_p); free(r->trace_del);
+  memset(r, 0, sizeof(*r));
+}
