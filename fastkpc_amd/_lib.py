@@ -1,0 +1,113 @@
+"""ctypes binding of libkpcmi.so (the gfx950 kernel-PC CI library).
+
+The library is the product: importing this module never falls back to a CPU
+implementation.  If the shared library is missing, `lib()` raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libkpcmi.so")
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+_U8 = ctypes.POINTER(ctypes.c_ubyte)
+
+
+class KpcOptions(ctypes.Structure):
+    _fields_ = [
+        ("ci_method", ctypes.c_int),
+        ("alpha", ctypes.c_double),
+        ("max_conditioning_size", ctypes.c_int),
+        ("index", ctypes.c_double),
+        ("num_col", ctypes.c_int),
+        ("hsic_sig", ctypes.c_double),
+        ("replicates", ctypes.c_int),
+        ("include_observed", ctypes.c_int),
+        ("has_seed", ctypes.c_int),
+        ("seed", ctypes.c_ulonglong),
+        ("mode", ctypes.c_int),
+        ("plain_test_limit", ctypes.c_int),
+    ]
+
+
+class KpcResult(ctypes.Structure):
+    _fields_ = [
+        ("p", ctypes.c_int),
+        ("levels", ctypes.c_int),
+        ("adjacency", _U8),
+        ("pmax", _D),
+        ("sepset_len", _I),
+        ("sepset", _I),
+        ("n_edgetests", ctypes.c_int * 64),
+        ("n_tasks", ctypes.c_int),
+        ("task_level", _I),
+        ("task_x", _I),
+        ("task_y", _I),
+        ("task_slen", _I),
+        ("task_S", _I),
+        ("task_p", _D),
+        ("task_deleted", _U8),
+        ("wall_seconds", ctypes.c_double),
+        ("stage_seconds", ctypes.c_double * 8),
+        ("physical_tests", ctypes.c_longlong),
+        ("residual_fits", ctypes.c_longlong),
+        ("components", ctypes.c_longlong),
+        ("setups", ctypes.c_longlong),
+        ("magic_evaluations", ctypes.c_longlong),
+        ("truncated", ctypes.c_int),
+    ]
+
+
+EXPORTS = (
+    "kpc_default_options", "kpc_skeleton_run", "kpc_result_free", "kpc_ci_batch",
+    "kpc_regrxons_batch", "kpc_gcv_grad_hess", "kpc_dcov_components", "kpc_last_error",
+    "kpc_device_count",
+)
+
+_lib = None
+
+
+class KpcLibraryError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise KpcLibraryError(
+            "libkpcmi.so is not built; run `python -m fastkpc_amd.build` (no CPU fallback exists)")
+    L = ctypes.CDLL(LIB_PATH)
+    L.kpc_default_options.argtypes = [ctypes.POINTER(KpcOptions)]
+    L.kpc_default_options.restype = None
+    L.kpc_skeleton_run.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.POINTER(KpcOptions),
+                                   ctypes.POINTER(ctypes.POINTER(KpcResult))]
+    L.kpc_result_free.argtypes = [ctypes.POINTER(KpcResult)]
+    L.kpc_result_free.restype = None
+    L.kpc_ci_batch.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.POINTER(KpcOptions),
+                               ctypes.c_int, _I, _I, ctypes.c_int, _I, _D]
+    L.kpc_regrxons_batch.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I,
+                                     ctypes.c_int, _I, _D, _D]
+    L.kpc_gcv_grad_hess.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I, ctypes.c_int,
+                                    _I, _D, _D, _D, _D]
+    L.kpc_dcov_components.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D, _D, _D]
+    L.kpc_last_error.restype = ctypes.c_char_p
+    L.kpc_device_count.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().kpc_last_error().decode("utf-8", "replace")
+        raise KpcLibraryError("kpcmi error %d: %s" % (rc, msg))
+
+
+def dptr(a):
+    return a.ctypes.data_as(_D)
+
+
+def iptr(a):
+    return a.ctypes.data_as(_I)

@@ -1,0 +1,222 @@
+"""Python mirror of the reference's R-facing API for the kernel-PC CI path.
+
+`fastkpc_compatible_cuda_skeleton` mirrors fastkpc/R/fast_kpc.R:56-130 (same
+option names, m.max resolution fastkpc/R/cuda_native.R:7-27, result fields
+fastkpc/src/full_cuda_ci_one_call.cpp:5597-5640, 6008-6030); `kernelCItest`
+mirrors kpcalg/R/kernelCItest.R:68-129 for one test; `regrXonS` mirrors
+kpcalg/R/regrXonS.R:33-61.  All computation runs in libkpcmi.so on the GPU.
+"""
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import KpcOptions, KpcResult, check, dptr, iptr
+
+CI_METHODS = {"dcc.gamma": 0, "hsic.gamma": 1, "dcc.perm": 2, "hsic.perm": 3}
+ALLOWED_OPTIONS = ("max_conditioning_size", "index", "numCol", "trace_level", "dcov_batch",
+                   "mgcv_residual_backend", "mgcv_residual_backend_native_s_size_limit",
+                   "mgcv_residual_backend_condition_threshold", "route",
+                   "compatible_cuda_strict", "ci_method", "hsic_params", "permutation_params",
+                   "mode", "plain_test_limit")
+
+
+def resolve_max_conditioning_size(max_conditioning_size, p):
+    """fastkpc/R/cuda_native.R:7-27: Inf -> p - 2, else min(floor(m), p - 2)."""
+    m = max_conditioning_size
+    if (not isinstance(m, (int, float)) or isinstance(m, bool) or (isinstance(m, float) and math.isnan(m))
+            or m < 0 or not isinstance(p, int) or p < 2):
+        raise ValueError("max_conditioning_size must be one non-negative number and p must be an integer >= 2")
+    if math.isinf(m):
+        return p - 2
+    return int(min(math.floor(m), p - 2))
+
+
+def _options(n, p, alpha, options):
+    unknown = [k for k in options if k not in ALLOWED_OPTIONS]
+    if unknown:
+        raise ValueError("unknown compatible CUDA option(s): " + ",".join(sorted(unknown)))
+    o = KpcOptions()
+    _lib.lib().kpc_default_options(o)
+    o.alpha = float(alpha)
+    o.max_conditioning_size = resolve_max_conditioning_size(
+        options.get("max_conditioning_size", float("inf")), p)
+    o.index = float(options.get("index", 1))
+    o.num_col = int(options.get("numCol", n // 10))
+    method = options.get("ci_method", "dcc.gamma")
+    if method not in CI_METHODS:
+        raise ValueError("ci_method must be one of " + ", ".join(CI_METHODS))
+    o.ci_method = CI_METHODS[method]
+    o.hsic_sig = float(options.get("hsic_params", {}).get("sig", 1.0))
+    perm = options.get("permutation_params", {})
+    o.replicates = int(perm.get("replicates", 100))
+    o.include_observed = 1 if perm.get("include_observed", True) else 0
+    if perm.get("seed") is not None:
+        o.has_seed = 1
+        o.seed = int(perm["seed"])
+    mode = options.get("mode", "batched")
+    if mode not in ("batched", "plain"):
+        raise ValueError("mode must be 'batched' or 'plain'")
+    o.mode = 0 if mode == "batched" else 1
+    o.plain_test_limit = int(options.get("plain_test_limit", 0))
+    return o
+
+
+def _as_data(data):
+    d = np.asarray(data, dtype=np.float64)
+    if d.ndim != 2:
+        raise ValueError("data must be a 2-D matrix")
+    return np.asfortranarray(d)
+
+
+def fastkpc_compatible_cuda_skeleton(data, alpha, labels=None, options=None):
+    options = dict(options or {})
+    d = _as_data(data)
+    n, p = d.shape
+    if labels is None:
+        labels = ["V%d" % (i + 1) for i in range(p)]
+    if len(labels) != p:
+        raise ValueError("labels length must match number of columns")
+    route = options.get("route", "full_cuda")
+    if route != "full_cuda":
+        raise ValueError("only route='full_cuda' is provided by this library")
+    o = _options(n, p, alpha, options)
+    res = _lib.ctypes.POINTER(KpcResult)()
+    check(_lib.lib().kpc_skeleton_run(dptr(d), n, p, o, _lib.ctypes.byref(res)))
+    try:
+        r = res.contents
+        adj = np.ctypeslib.as_array(r.adjacency, (p * p,)).reshape(p, p).astype(bool).copy()
+        pmax = np.ctypeslib.as_array(r.pmax, (p * p,)).reshape(p, p).copy()
+        slen = np.ctypeslib.as_array(r.sepset_len, (p * p,)).reshape(p, p).copy()
+        sep = np.ctypeslib.as_array(r.sepset, (p * p * 64,)).reshape(p, p, 64)
+        sepsets = [[None if slen[i, j] < 0 else [int(v) + 1 for v in sep[i, j, :slen[i, j]]]
+                    for j in range(p)] for i in range(p)]
+        L = r.n_tasks
+        if L > 0:
+            tS = np.ctypeslib.as_array(r.task_S, (L * 64,)).reshape(L, 64).copy()
+            ts = np.ctypeslib.as_array(r.task_slen, (L,)).copy()
+            tasks = {
+                "level": np.ctypeslib.as_array(r.task_level, (L,)).copy(),
+                "x": np.ctypeslib.as_array(r.task_x, (L,)).copy() + 1,
+                "y": np.ctypeslib.as_array(r.task_y, (L,)).copy() + 1,
+                "S_key": ["|".join(str(v + 1) for v in tS[i, :ts[i]]) for i in range(L)],
+                "conditioning_size": ts,
+                "p_used": np.ctypeslib.as_array(r.task_p, (L,)).copy(),
+                "native_edge_deleted": np.ctypeslib.as_array(r.task_deleted, (L,)).astype(bool).copy(),
+            }
+        else:
+            tasks = {"level": np.zeros(0, int), "x": np.zeros(0, int), "y": np.zeros(0, int),
+                     "S_key": [], "conditioning_size": np.zeros(0, int), "p_used": np.zeros(0),
+                     "native_edge_deleted": np.zeros(0, bool)}
+        levels = r.levels
+        summary = {
+            "entrypoint": "kpc_skeleton_run",
+            "n": n, "p": p, "alpha": float(alpha),
+            "ci_method": options.get("ci_method", "dcc.gamma"),
+            "mode": options.get("mode", "batched"),
+            "max_conditioning_size_resolved": o.max_conditioning_size,
+            "wall_seconds": r.wall_seconds,
+            "stage_seconds": dict(zip(["smooth_setup", "geometry", "gcv", "dcov_component",
+                                       "dcov_pair", "host"], list(r.stage_seconds)[:6])),
+            "physical_tests": r.physical_tests,
+            "residual_fits": r.residual_fits,
+            "components": r.components,
+            "setups": r.setups,
+            "magic_evaluations": r.magic_evaluations,
+            "truncated": bool(r.truncated),
+            "labels": list(labels),
+        }
+        return {
+            "adjacency": adj,
+            "sepsets": sepsets,
+            "pMax": pmax,
+            "n.edgetests": [int(v) for v in list(r.n_edgetests)[:levels]],
+            "tasks": tasks,
+            "levels": levels,
+            "summary": summary,
+        }
+    finally:
+        _lib.lib().kpc_result_free(res)
+
+
+def ci_batch(data, x, y, S, num_col=35, ci_method="dcc.gamma", alpha=0.1):
+    """Batch of CI tests sharing |S|: returns (count, 4) {p, statistic, mean, variance}."""
+    d = _as_data(data)
+    n, p = d.shape
+    x = np.ascontiguousarray(x, dtype=np.int32)
+    y = np.ascontiguousarray(y, dtype=np.int32)
+    S = np.ascontiguousarray(np.asarray(S, dtype=np.int32).reshape(len(x), -1))
+    o = _options(n, p, alpha, {"numCol": num_col, "ci_method": ci_method})
+    out = np.zeros((len(x), 4))
+    check(_lib.lib().kpc_ci_batch(dptr(d), n, p, o, len(x), iptr(x), iptr(y), S.shape[1],
+                                  iptr(S), dptr(out)))
+    return out
+
+
+def kernelCItest(x, y, S, suffStat):
+    """kpcalg/R/kernelCItest.R:68-129 with 1-based x, y, S; suffStat holds data,
+    ic.method, index, numCol, sig, p."""
+    data = suffStat["data"]
+    n = np.asarray(data).shape[0]
+    method = suffStat.get("ic.method", "dcc.perm")
+    num_col = suffStat.get("numCol", n // 10)
+    S0 = [int(s) - 1 for s in (S or [])]
+    out = ci_batch(data, [int(x) - 1], [int(y) - 1], [S0], num_col=num_col, ci_method=method)
+    return float(out[0, 0])
+
+
+def regrXonS(data, targets, S):
+    """Residuals of each target (0-based) on conditioning set S (0-based, shared)."""
+    d = _as_data(data)
+    n, p = d.shape
+    t = np.ascontiguousarray(targets, dtype=np.int32)
+    Sm = np.ascontiguousarray(np.tile(np.asarray(sorted(S), dtype=np.int32), (len(t), 1)))
+    resid = np.zeros((len(t), n))
+    info = np.zeros((len(t), 32))
+    check(_lib.lib().kpc_regrxons_batch(dptr(d), n, p, len(t), iptr(t), Sm.shape[1], iptr(Sm),
+                                        dptr(resid), dptr(info)))
+    return resid, info
+
+
+def regrxons_batch(data, targets, S):
+    """Residuals for per-fit conditioning sets S (count x s_len, 0-based)."""
+    d = _as_data(data)
+    n, p = d.shape
+    t = np.ascontiguousarray(targets, dtype=np.int32)
+    Sm = np.ascontiguousarray(np.asarray(S, dtype=np.int32).reshape(len(t), -1))
+    resid = np.zeros((len(t), n))
+    info = np.zeros((len(t), 32))
+    check(_lib.lib().kpc_regrxons_batch(dptr(d), n, p, len(t), iptr(t), Sm.shape[1], iptr(Sm),
+                                        dptr(resid), dptr(info)))
+    return resid, info
+
+
+def gcv_grad_hess(data, targets, S, log_sp):
+    """GCV score, gradient and Hessian in log(sp) (hand-written derivative)."""
+    d = _as_data(data)
+    n, p = d.shape
+    t = np.ascontiguousarray(targets, dtype=np.int32)
+    Sm = np.ascontiguousarray(np.asarray(S, dtype=np.int32).reshape(len(t), -1))
+    lsp = np.ascontiguousarray(np.asarray(log_sp, dtype=np.float64).reshape(len(t), -1))
+    m = lsp.shape[1]
+    score = np.zeros(len(t))
+    grad = np.zeros((len(t), m))
+    hess = np.zeros((len(t), m, m))
+    check(_lib.lib().kpc_gcv_grad_hess(dptr(d), n, p, len(t), iptr(t), Sm.shape[1], iptr(Sm),
+                                       dptr(lsp), dptr(score), dptr(grad), dptr(hess)))
+    return score, grad, hess
+
+
+def dcov_components(x, k):
+    """dcov.gamma low-rank components of the columns of x (n x count)."""
+    xx = np.asfortranarray(np.asarray(x, dtype=np.float64))
+    if xx.ndim == 1:
+        xx = xx[:, None]
+    n, c = xx.shape
+    vals = np.zeros((c, k))
+    vecs = np.zeros((c, k, n))
+    dsum = np.zeros(c)
+    mom = np.zeros(c)
+    check(_lib.lib().kpc_dcov_components(dptr(xx), n, c, k, dptr(vals), dptr(vecs), dptr(dsum),
+                                         dptr(mom)))
+    return vals, vecs.transpose(0, 2, 1), dsum, mom

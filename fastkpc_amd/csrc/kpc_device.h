@@ -1,0 +1,74 @@
+// Launch interfaces of the gfx950 kernels (host side of the device code).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace kpc {
+
+constexpr int kMagicInfo = 32;
+
+struct DesignLayout {
+  long long X, QR, tau, R, piv, rS, Sp, lsp0, status, stride;
+};
+
+struct GeometryItem {
+  int design_slot;
+  int smooth_slot[16];
+};
+
+struct MagicItem {
+  int design_slot;
+  int target;
+  int resid_slot;
+  int pad;
+};
+
+struct MagicControl {
+  double tol;
+  int max_half_single;
+  int max_half_multi;
+  int max_iterations;
+  double max_newton_step;
+  double probe_step;
+  int max_probes;
+  double rank_tol;
+};
+
+DesignLayout host_design_layout(int n, int q, int m, int rk);
+
+long long dcov_component_scratch_doubles(int n, int k);
+size_t dcov_component_lds_bytes(int n);
+hipError_t launch_dcov_components(const double* src_base, const long long* src_off,
+                                  const int* dst_slot, int count, int n, int k, double* vals,
+                                  double* vecs, double* dsum, double* moment, double* scratch,
+                                  int grid, hipStream_t stream);
+hipError_t launch_dcov_pairs(const int* px, const int* py, int count, int n, int k,
+                             const double* vals, const double* vecs, const double* dsum,
+                             const double* moment, double* out, int grid, hipStream_t stream);
+
+long long smooth_scratch_doubles(int n);
+size_t smooth_lds_bytes(int n);
+hipError_t launch_smooth_setup(const double* data, int n, int p, const int* items, int count,
+                               double* out_X, double* out_S, int* out_info, double* scratch,
+                               int grid, hipStream_t stream);
+
+long long geometry_scratch_doubles(int q);
+hipError_t launch_geometry(const GeometryItem* items, int count, int n, int q, int m, int dim,
+                           int rk, const double* smooth_X, const double* smooth_S, double* design,
+                           long long smooth_x_stride, double* scratch, int grid,
+                           hipStream_t stream);
+
+long long magic_scratch_doubles(int q, int m, int rk);
+size_t magic_lds_bytes(int n, int q, bool aug_in_lds);
+bool magic_aug_fits_lds(int n, int q);
+hipError_t launch_magic(const MagicItem* items, int count, int n, int q, int m, int rk,
+                        const double* data, const double* design, double* resid, double* info,
+                        double* scratch, int grid, const MagicControl& ctl, hipStream_t stream);
+
+}  // namespace kpc
+
+namespace kpc {
+hipError_t launch_gcv_eval(const MagicItem* items, int count, int n, int q, int m, int rk,
+                           const double* data, const double* design, const double* lsp,
+                           double* out, double* scratch, int grid, double rank_tol,
+                           hipStream_t stream);
+}  // namespace kpc

@@ -1,0 +1,967 @@
+// Host controller and C ABI of the MI355X kernel-PC CI path.
+//
+// Skeleton semantics follow pcalg skeleton(method = "stable") as replicated by
+// fastkpc/src/skeleton_task_scheduler.cpp:70-132 (task order) and
+// fastkpc/src/full_cuda_ci_one_call.cpp:5229-5580 (level loop, per-edge early
+// stop, canonical replay, pMax, sepsets, n.edgetests, `done` rule).
+//
+// Execution (KPC_BATCHED): each level is evaluated in frontier waves.  A wave
+// takes, for every undecided edge, the next `chunk` tasks of its current
+// orientation (chunk doubles per wave), evaluates all of them in one batch of
+// launches -- smooth setups, design geometries, GCV fits, dcov components,
+// pair statistics -- and replays them in canonical order.  Device-resident
+// caches keyed by S (designs), (target, S) (residuals, components) and by
+// column (univariate smooths) make every residual and component computed at
+// most once per level.  KPC_PLAIN runs the same kernels once per test with
+// no cross-test caching (one launch sequence per CI test).
+#include "../../include/kpcmi.h"
+#include "kpc_common.h"
+#include "kpc_device.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct KpcError : std::runtime_error {
+  int code;
+  KpcError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess)
+    throw KpcError(-2, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename T>
+struct DevArray {
+  T* ptr = nullptr;
+  size_t cap = 0;
+  void reserve(size_t count) {
+    if (count <= cap) return;
+    if (ptr) hip_check(hipFree(ptr), "hipFree");
+    size_t c = count < 64 ? 64 : count;
+    hip_check(hipMalloc((void**)&ptr, c * sizeof(T)), "hipMalloc");
+    cap = c;
+  }
+  void upload(const std::vector<T>& v, hipStream_t s) {
+    reserve(v.size());
+    if (!v.empty())
+      hip_check(hipMemcpyAsync(ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s),
+                "hipMemcpyAsync H2D");
+  }
+  ~DevArray() {
+    if (ptr) (void)hipFree(ptr);
+  }
+};
+
+struct KeyHash {
+  size_t operator()(const std::pair<int, uint64_t>& k) const {
+    uint64_t x = k.second * 0x9E3779B97F4A7C15ULL ^ (uint64_t)(k.first + 1) * 0xC2B2AE3D27D4EB4FULL;
+    x ^= x >> 29;
+    return (size_t)x;
+  }
+};
+
+struct Task {
+  int x, y, slen;
+  int S[64];
+  uint64_t mask;
+  double p = 0.0, stat = 0.0, mean = 0.0, var = 0.0;
+  bool computed = false;
+  bool consumed = false;
+  bool deleted = false;
+};
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+class Engine {
+ public:
+  Engine(const double* h_data, int n, int p, const kpc_options& opt) : n_(n), p_(p), opt_(opt) {
+    if (n < 6) throw KpcError(-1, "n must be > 5");
+    if (n > kpc::kMaxN) throw KpcError(-1, "n exceeds the supported maximum (2048)");
+    if (p < 2 || p > 64) throw KpcError(-1, "p must be in [2, 64]");
+    if (opt.num_col <= 0 || opt.num_col >= n || opt.num_col > kpc::kMaxNumCol)
+      throw KpcError(-1, "num_col must be positive, < n and <= 64");
+    if (opt.ci_method != KPC_DCC_GAMMA)
+      throw KpcError(-3, "only ci_method dcc.gamma is available in this build");
+    for (long long i = 0; i < (long long)n * p; ++i)
+      if (!std::isfinite(h_data[i])) throw KpcError(-1, "data must be finite");
+    hip_check(hipStreamCreate(&stream_), "hipStreamCreate");
+    hip_check(hipMalloc((void**)&d_data_, sizeof(double) * (size_t)n * p), "hipMalloc data");
+    hip_check(hipMemcpy(d_data_, h_data, sizeof(double) * (size_t)n * p, hipMemcpyHostToDevice),
+              "upload data");
+    k_ = opt.num_col;
+    ctl_.tol = 1e-7;
+    ctl_.max_half_single = 15;
+    ctl_.max_half_multi = 25;
+    ctl_.max_iterations = 400;
+    ctl_.max_newton_step = 5.0;
+    ctl_.probe_step = 2.0;
+    ctl_.max_probes = 5;
+    ctl_.rank_tol = 1.4901161193847656e-8;
+    for (int e = 0; e < 8; ++e) hip_check(hipEventCreate(&ev_[e]), "hipEventCreate");
+  }
+  ~Engine() {
+    if (d_data_) (void)hipFree(d_data_);
+    for (int e = 0; e < 8; ++e) (void)hipEventDestroy(ev_[e]);
+    if (stream_) (void)hipStreamDestroy(stream_);
+  }
+
+  // ---- per-level arenas / caches -------------------------------------
+  void begin_level(int level) {
+    level_ = level;
+    design_map_.clear();
+    resid_map_.clear();
+    comp_map_.clear();
+    n_design_ = n_resid_ = n_comp_ = n_lsmooth_ = 0;
+    if (level <= 2) {
+      m_ = 1;
+      dim_ = level == 2 ? 29 : 9;
+      rk_ = level == 2 ? 27 : 8;
+      q_ = level == 2 ? 30 : 10;
+    } else {
+      m_ = level;
+      dim_ = 9;
+      rk_ = 8;
+      q_ = 1 + 9 * level;
+    }
+    if (level > 0) {
+      if (q_ >= n_) throw KpcError(-4, "model coefficient dimension must be less than n");
+      if (q_ > kpc::kMaxQ - 1 || m_ > kpc::kMaxPen)
+        throw KpcError(-4, "conditioning set too large for this build (|S| <= 14)");
+      layout_ = kpc::host_design_layout(n_, q_, m_, rk_);
+    }
+  }
+
+  void reserve_level(size_t max_designs, size_t max_resid) {
+    const size_t nx = (size_t)n_ * 29;
+    if (level_ > 0 && level_ <= 2) {
+      lsmooth_X_.reserve(max_designs * nx);
+      lsmooth_S_.reserve(max_designs * 29 * 29);
+      lsmooth_info_.reserve(max_designs * 4);
+    }
+    if (level_ > 2) {
+      usmooth_X_.reserve((size_t)p_ * nx);
+      usmooth_S_.reserve((size_t)p_ * 29 * 29);
+      usmooth_info_.reserve((size_t)p_ * 4);
+    }
+    if (level_ > 0) {
+      design_.reserve(max_designs * (size_t)layout_.stride);
+      resid_.reserve(max_resid * (size_t)n_);
+      rinfo_.reserve(max_resid * kpc::kMagicInfo);
+    }
+    comp_vals_.reserve(max_resid * (size_t)k_);
+    comp_vecs_.reserve(max_resid * (size_t)n_ * k_);
+    comp_dsum_.reserve(max_resid);
+    comp_mom_.reserve(max_resid);
+  }
+
+  // Evaluate p-values for the given tasks (all with |S| == level_).
+  void evaluate(std::vector<Task*>& tasks) {
+    if (tasks.empty()) return;
+    const double t0 = now_s();
+    std::vector<int> smooth_items;     // d, c0, c1, slot
+    std::vector<kpc::GeometryItem> geo_items;
+    std::vector<kpc::MagicItem> fit_items;
+    std::vector<long long> comp_src;
+    std::vector<int> comp_dst;
+    std::vector<int> pair_x, pair_y;
+    for (Task* t : tasks) {
+      int slots[2];
+      for (int side = 0; side < 2; ++side) {
+        const int tgt = side ? t->y : t->x;
+        auto key = std::make_pair(tgt, t->mask);
+        auto it = comp_map_.find(key);
+        if (it != comp_map_.end()) { slots[side] = it->second; continue; }
+        const int cs = n_comp_++;
+        comp_map_[key] = cs;
+        slots[side] = cs;
+        comp_dst.push_back(cs);
+        if (level_ == 0) {
+          comp_src.push_back((long long)tgt * n_);
+          continue;
+        }
+        int dslot = ensure_design(*t, smooth_items, geo_items);
+        const int rs = n_resid_++;
+        resid_map_[key] = rs;
+        kpc::MagicItem mi;
+        mi.design_slot = dslot;
+        mi.target = tgt;
+        mi.resid_slot = rs;
+        mi.pad = 0;
+        fit_items.push_back(mi);
+        comp_src.push_back((long long)rs * n_);
+      }
+      pair_x.push_back(slots[0]);
+      pair_y.push_back(slots[1]);
+    }
+    stats_.physical_tests += (long long)tasks.size();
+    stats_.fits += (long long)fit_items.size();
+    stats_.components += (long long)comp_dst.size();
+    stats_.setups += (long long)geo_items.size();
+    // uploads
+    d_smooth_items_.upload(smooth_items, stream_);
+    d_geo_items_.upload(geo_items, stream_);
+    d_fit_items_.upload(fit_items, stream_);
+    d_comp_src_.upload(comp_src, stream_);
+    d_comp_dst_.upload(comp_dst, stream_);
+    d_pair_x_.upload(pair_x, stream_);
+    d_pair_y_.upload(pair_y, stream_);
+    d_pair_out_.reserve(tasks.size() * 4);
+    stage_host_ += now_s() - t0;
+    hip_check(hipEventRecord(ev_[0], stream_), "event");
+    const int nsm = (int)smooth_items.size() / 4;
+    if (nsm > 0) {
+      const long long stride = kpc::smooth_scratch_doubles(n_);
+      int grid = std::min(nsm, grid_limit(stride, 512));
+      scratch_.reserve((size_t)grid * stride);
+      const bool uni = level_ > 2;
+      hip_check(kpc::launch_smooth_setup(d_data_, n_, p_, d_smooth_items_.ptr, nsm,
+                                         uni ? usmooth_X_.ptr : lsmooth_X_.ptr,
+                                         uni ? usmooth_S_.ptr : lsmooth_S_.ptr,
+                                         uni ? usmooth_info_.ptr : lsmooth_info_.ptr,
+                                         scratch_.ptr, grid, stream_),
+                "smooth setup launch");
+      pending_smooth_checks_ = true;
+    }
+    hip_check(hipEventRecord(ev_[1], stream_), "event");
+    if (!geo_items.empty()) {
+      const long long stride = kpc::geometry_scratch_doubles(q_);
+      int grid = std::min((int)geo_items.size(), 2048);
+      scratch2_.reserve((size_t)grid * stride);
+      const bool uni = level_ > 2;
+      hip_check(kpc::launch_geometry(d_geo_items_.ptr, (int)geo_items.size(), n_, q_, m_, dim_, rk_,
+                                     uni ? usmooth_X_.ptr : lsmooth_X_.ptr,
+                                     uni ? usmooth_S_.ptr : lsmooth_S_.ptr, design_.ptr,
+                                     (long long)n_ * 29, scratch2_.ptr, grid, stream_),
+                "geometry launch");
+    }
+    hip_check(hipEventRecord(ev_[2], stream_), "event");
+    if (!fit_items.empty()) {
+      const long long stride = kpc::magic_scratch_doubles(q_, m_, rk_);
+      int grid = std::min((int)fit_items.size(), 8192);
+      scratch3_.reserve((size_t)grid * stride);
+      hip_check(kpc::launch_magic(d_fit_items_.ptr, (int)fit_items.size(), n_, q_, m_, rk_,
+                                  d_data_, design_.ptr, resid_.ptr, rinfo_.ptr, scratch3_.ptr,
+                                  grid, ctl_, stream_),
+                "magic launch");
+    }
+    hip_check(hipEventRecord(ev_[3], stream_), "event");
+    if (!comp_dst.empty()) {
+      const long long stride = kpc::dcov_component_scratch_doubles(n_, k_);
+      int grid = std::min((int)comp_dst.size(), grid_limit(stride, 2048));
+      scratch4_.reserve((size_t)grid * stride);
+      hip_check(kpc::launch_dcov_components(level_ == 0 ? d_data_ : resid_.ptr, d_comp_src_.ptr,
+                                            d_comp_dst_.ptr, (int)comp_dst.size(), n_, k_,
+                                            comp_vals_.ptr, comp_vecs_.ptr, comp_dsum_.ptr,
+                                            comp_mom_.ptr, scratch4_.ptr, grid, stream_),
+                "dcov component launch");
+    }
+    hip_check(hipEventRecord(ev_[4], stream_), "event");
+    {
+      int grid = std::min((int)tasks.size(), 65535);
+      hip_check(kpc::launch_dcov_pairs(d_pair_x_.ptr, d_pair_y_.ptr, (int)tasks.size(), n_, k_,
+                                       comp_vals_.ptr, comp_vecs_.ptr, comp_dsum_.ptr,
+                                       comp_mom_.ptr, d_pair_out_.ptr, grid, stream_),
+                "dcov pair launch");
+    }
+    hip_check(hipEventRecord(ev_[5], stream_), "event");
+    std::vector<double> out(tasks.size() * 4);
+    hip_check(hipMemcpyAsync(out.data(), d_pair_out_.ptr, sizeof(double) * out.size(),
+                             hipMemcpyDeviceToHost, stream_),
+              "D2H p-values");
+    hip_check(hipStreamSynchronize(stream_), "wave synchronize");
+    float ms;
+    for (int s = 0; s < 5; ++s) {
+      hip_check(hipEventElapsedTime(&ms, ev_[s], ev_[s + 1]), "event elapsed");
+      stage_[s] += ms * 1e-3;
+    }
+    check_device_status(fit_items, geo_items, nsm);
+    for (size_t i = 0; i < tasks.size(); ++i) {
+      Task* t = tasks[i];
+      t->p = out[4 * i + 0];
+      t->stat = out[4 * i + 1];
+      t->mean = out[4 * i + 2];
+      t->var = out[4 * i + 3];
+      if (std::isnan(t->p)) t->p = 1.0;  // NAdelete = TRUE
+      t->computed = true;
+    }
+  }
+
+  int level() const { return level_; }
+  int n() const { return n_; }
+  int p() const { return p_; }
+  double stage(int i) const { return stage_[i]; }
+  double stage_host() const { return stage_host_; }
+  struct Stats {
+    long long physical_tests = 0, fits = 0, components = 0, setups = 0, evals = 0;
+  } stats_;
+
+  // direct access for the regrXonS / gcv entry points
+  double* data_ptr() { return d_data_; }
+  hipStream_t stream() { return stream_; }
+  const kpc::MagicControl& ctl() const { return ctl_; }
+
+  // Build designs for arbitrary S sets (all |S| == level_); returns slots.
+  std::vector<int> build_designs(const std::vector<Task>& sets) {
+    std::vector<int> smooth_items;
+    std::vector<kpc::GeometryItem> geo_items;
+    std::vector<int> slots;
+    for (const Task& t : sets) slots.push_back(ensure_design(t, smooth_items, geo_items));
+    d_smooth_items_.upload(smooth_items, stream_);
+    d_geo_items_.upload(geo_items, stream_);
+    const int nsm = (int)smooth_items.size() / 4;
+    const bool uni = level_ > 2;
+    if (nsm > 0) {
+      const long long stride = kpc::smooth_scratch_doubles(n_);
+      int grid = std::min(nsm, grid_limit(stride, 512));
+      scratch_.reserve((size_t)grid * stride);
+      hip_check(kpc::launch_smooth_setup(d_data_, n_, p_, d_smooth_items_.ptr, nsm,
+                                         uni ? usmooth_X_.ptr : lsmooth_X_.ptr,
+                                         uni ? usmooth_S_.ptr : lsmooth_S_.ptr,
+                                         uni ? usmooth_info_.ptr : lsmooth_info_.ptr,
+                                         scratch_.ptr, grid, stream_),
+                "smooth setup launch");
+    }
+    if (!geo_items.empty()) {
+      const long long stride = kpc::geometry_scratch_doubles(q_);
+      int grid = std::min((int)geo_items.size(), 2048);
+      scratch2_.reserve((size_t)grid * stride);
+      hip_check(kpc::launch_geometry(d_geo_items_.ptr, (int)geo_items.size(), n_, q_, m_, dim_, rk_,
+                                     uni ? usmooth_X_.ptr : lsmooth_X_.ptr,
+                                     uni ? usmooth_S_.ptr : lsmooth_S_.ptr, design_.ptr,
+                                     (long long)n_ * 29, scratch2_.ptr, grid, stream_),
+                "geometry launch");
+    }
+    hip_check(hipStreamSynchronize(stream_), "setup synchronize");
+    check_device_status({}, geo_items, nsm);
+    return slots;
+  }
+  double* design_ptr() { return design_.ptr; }
+  int q() const { return q_; }
+  int m() const { return m_; }
+  int rk() const { return rk_; }
+  DevArray<double>& resid_arena() { return resid_; }
+  DevArray<double>& rinfo_arena() { return rinfo_; }
+  DevArray<double>& scratch3() { return scratch3_; }
+
+ private:
+  int grid_limit(long long stride, int cap) const {
+    // keep any one scratch slab under ~8 GiB
+    long long lim = (8LL << 30) / 8 / std::max(1LL, stride);
+    return (int)std::max(1LL, std::min((long long)cap, lim));
+  }
+
+  int ensure_design(const Task& t, std::vector<int>& smooth_items,
+                    std::vector<kpc::GeometryItem>& geo_items) {
+    auto it = design_map_.find(t.mask);
+    if (it != design_map_.end()) return it->second;
+    const int ds = n_design_++;
+    design_map_[t.mask] = ds;
+    kpc::GeometryItem gi;
+    std::memset(&gi, 0, sizeof(gi));
+    gi.design_slot = ds;
+    if (level_ <= 2) {
+      const int ss = n_lsmooth_++;
+      smooth_items.push_back(level_);
+      smooth_items.push_back(t.S[0]);
+      smooth_items.push_back(level_ == 2 ? t.S[1] : t.S[0]);
+      smooth_items.push_back(ss);
+      gi.smooth_slot[0] = ss;
+    } else {
+      for (int i = 0; i < t.slen; ++i) {
+        const int col = t.S[i];
+        if (uni_slot_.empty()) uni_slot_.assign(p_, -1);
+        if (uni_slot_[col] < 0) {
+          uni_slot_[col] = col;
+          smooth_items.push_back(1);
+          smooth_items.push_back(col);
+          smooth_items.push_back(col);
+          smooth_items.push_back(col);
+        }
+        gi.smooth_slot[i] = col;
+      }
+    }
+    geo_items.push_back(gi);
+    return ds;
+  }
+
+  void check_device_status(const std::vector<kpc::MagicItem>& fits,
+                           const std::vector<kpc::GeometryItem>& geos, int nsm) {
+    if (nsm > 0) {
+      const bool uni = level_ > 2;
+      DevArray<int>& info = uni ? usmooth_info_ : lsmooth_info_;
+      const size_t cnt = uni ? (size_t)p_ : (size_t)n_lsmooth_;
+      std::vector<int> h(cnt * 4);
+      hip_check(hipMemcpy(h.data(), info.ptr, sizeof(int) * h.size(), hipMemcpyDeviceToHost),
+                "smooth info");
+      for (size_t i = 0; i < cnt; ++i) {
+        if (uni && uni_slot_[i] < 0) continue;
+        if (h[4 * i] != 0)
+          throw KpcError(-5, "thin-plate setup failed (fewer unique locations than basis size)");
+      }
+    }
+    for (const auto& g : geos) {
+      double st = 0.0;
+      hip_check(hipMemcpy(&st, design_.ptr + (size_t)g.design_slot * layout_.stride + layout_.status,
+                          sizeof(double), hipMemcpyDeviceToHost),
+                "design status");
+      if (st != 0.0) throw KpcError(-5, "design geometry failed (penalty rank or initial.sp)");
+    }
+    if (!fits.empty()) {
+      std::vector<double> h((size_t)n_resid_ * kpc::kMagicInfo);
+      hip_check(hipMemcpy(h.data(), rinfo_.ptr, sizeof(double) * h.size(), hipMemcpyDeviceToHost),
+                "fit info");
+      for (const auto& f : fits) {
+        const double* o = h.data() + (size_t)f.resid_slot * kpc::kMagicInfo;
+        if (o[0] != 0.0) throw KpcError(-6, "GCV optimiser did not converge");
+        stats_.evals += (long long)o[5];
+      }
+    }
+  }
+
+  int n_, p_, k_;
+  kpc_options opt_;
+  hipStream_t stream_ = nullptr;
+  double* d_data_ = nullptr;
+  hipEvent_t ev_[8];
+  kpc::MagicControl ctl_;
+  int level_ = -1, q_ = 0, m_ = 0, dim_ = 0, rk_ = 0;
+  kpc::DesignLayout layout_{};
+  std::unordered_map<uint64_t, int> design_map_;
+  std::unordered_map<std::pair<int, uint64_t>, int, KeyHash> resid_map_, comp_map_;
+  std::vector<int> uni_slot_;
+  int n_design_ = 0, n_resid_ = 0, n_comp_ = 0, n_lsmooth_ = 0;
+  DevArray<double> lsmooth_X_, lsmooth_S_, usmooth_X_, usmooth_S_;
+  DevArray<int> lsmooth_info_, usmooth_info_;
+  DevArray<double> design_, resid_, rinfo_;
+  DevArray<double> comp_vals_, comp_vecs_, comp_dsum_, comp_mom_;
+  DevArray<double> scratch_, scratch2_, scratch3_, scratch4_;
+  DevArray<int> d_smooth_items_;
+  DevArray<kpc::GeometryItem> d_geo_items_;
+  DevArray<kpc::MagicItem> d_fit_items_;
+  DevArray<long long> d_comp_src_;
+  DevArray<int> d_comp_dst_, d_pair_x_, d_pair_y_;
+  DevArray<double> d_pair_out_;
+  double stage_[5] = {0, 0, 0, 0, 0};
+  double stage_host_ = 0.0;
+  bool pending_smooth_checks_ = false;
+};
+
+uint64_t mask_of(const int* S, int len) {
+  uint64_t m = 0;
+  for (int i = 0; i < len; ++i) m |= 1ULL << S[i];
+  return m;
+}
+
+bool next_comb(int* idx, int k, int len) {
+  int i = k - 1;
+  while (i >= 0 && idx[i] == len - k + i) --i;
+  if (i < 0) return false;
+  ++idx[i];
+  for (int j = i + 1; j < k; ++j) idx[j] = idx[j - 1] + 1;
+  return true;
+}
+
+struct Orientation {
+  int x, y;
+  int first = 0, count = 0;
+  int nbrs = 0;
+};
+
+struct EdgeRun {
+  int a, b;
+  Orientation o[2];
+  int side = 0, cursor = 0;
+  int state = 0;  // 0 active, 1 deleted, 2 kept
+};
+
+int fill_error(const std::exception& e) {
+  g_last_error = e.what();
+  if (auto* k = dynamic_cast<const KpcError*>(&e)) return k->code;
+  return -100;
+}
+
+kpc_result* new_result(int p) {
+  kpc_result* r = (kpc_result*)std::calloc(1, sizeof(kpc_result));
+  r->p = p;
+  r->adjacency = (unsigned char*)std::malloc((size_t)p * p);
+  r->pmax = (double*)std::malloc(sizeof(double) * (size_t)p * p);
+  r->sepset_len = (int*)std::malloc(sizeof(int) * (size_t)p * p);
+  r->sepset = (int*)std::calloc((size_t)p * p * 64, sizeof(int));
+  for (int i = 0; i < p * p; ++i) {
+    r->adjacency[i] = 1;
+    r->pmax[i] = -INFINITY;
+    r->sepset_len[i] = -1;
+  }
+  for (int i = 0; i < p; ++i) {
+    r->adjacency[i * p + i] = 0;
+    r->pmax[i * p + i] = 1.0;
+  }
+  return r;
+}
+
+void push_trace(kpc_result* r, int& cap, const Task& t, int level) {
+  if (r->n_tasks == cap) {
+    cap = cap ? cap * 2 : 4096;
+    r->task_level = (int*)std::realloc(r->task_level, sizeof(int) * cap);
+    r->task_x = (int*)std::realloc(r->task_x, sizeof(int) * cap);
+    r->task_y = (int*)std::realloc(r->task_y, sizeof(int) * cap);
+    r->task_slen = (int*)std::realloc(r->task_slen, sizeof(int) * cap);
+    r->task_S = (int*)std::realloc(r->task_S, sizeof(int) * (size_t)cap * 64);
+    r->task_p = (double*)std::realloc(r->task_p, sizeof(double) * cap);
+    r->task_deleted = (unsigned char*)std::realloc(r->task_deleted, cap);
+  }
+  const int i = r->n_tasks++;
+  r->task_level[i] = level;
+  r->task_x[i] = t.x;
+  r->task_y[i] = t.y;
+  r->task_slen[i] = t.slen;
+  for (int j = 0; j < 64; ++j) r->task_S[(size_t)i * 64 + j] = j < t.slen ? t.S[j] : 0;
+  r->task_p[i] = t.p;
+  r->task_deleted[i] = t.deleted ? 1 : 0;
+}
+
+void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
+  const int p = eng.p();
+  const int mmax = opt.max_conditioning_size < 0 ? p - 2 : std::min(opt.max_conditioning_size, p - 2);
+  std::vector<unsigned char> snap((size_t)p * p);
+  int cap = 0;
+  bool done = false;
+  int ord = 0;
+  const bool plain = opt.mode == KPC_PLAIN;
+  long long plain_tests = 0;
+  while (!done && ord <= mmax) {
+    bool any = false;
+    for (int i = 0; i < p * p; ++i) any |= r->adjacency[i] != 0;
+    if (!any) break;
+    done = true;
+    std::memcpy(snap.data(), r->adjacency, (size_t)p * p);
+    // ---- level plan
+    std::vector<Task> tasks;
+    std::vector<EdgeRun> edges;
+    int nb[64], idx[64];
+    for (int a = 0; a < p - 1; ++a)
+      for (int b = a + 1; b < p; ++b) {
+        if (!snap[(size_t)a * p + b]) continue;
+        EdgeRun er;
+        er.a = a;
+        er.b = b;
+        for (int side = 0; side < 2; ++side) {
+          const int x = side ? b : a, y = side ? a : b;
+          Orientation& o = er.o[side];
+          o.x = x;
+          o.y = y;
+          o.first = (int)tasks.size();
+          int len = 0;
+          for (int i = 0; i < p; ++i)
+            if (i != y && snap[(size_t)i * p + x]) nb[len++] = i;
+          o.nbrs = len;
+          if (len >= ord) {
+            for (int i = 0; i < ord; ++i) idx[i] = i;
+            do {
+              Task t;
+              t.x = x;
+              t.y = y;
+              t.slen = ord;
+              for (int i = 0; i < ord; ++i) t.S[i] = nb[idx[i]];
+              t.mask = mask_of(t.S, ord);
+              tasks.push_back(t);
+            } while (ord > 0 && next_comb(idx, ord, len));
+          }
+          o.count = (int)tasks.size() - o.first;
+        }
+        edges.push_back(er);
+      }
+    eng.begin_level(ord);
+    {
+      std::unordered_map<uint64_t, int> ud;
+      std::unordered_map<std::pair<int, uint64_t>, int, KeyHash> ur;
+      for (const Task& t : tasks) {
+        ud.emplace(t.mask, 0);
+        ur.emplace(std::make_pair(t.x, t.mask), 0);
+        ur.emplace(std::make_pair(t.y, t.mask), 0);
+      }
+      if (plain) eng.reserve_level(1, 2);
+      else eng.reserve_level(ud.size(), ur.size());
+    }
+    // ---- frontier waves
+    for (EdgeRun& e : edges) {
+      e.side = 0;
+      e.cursor = 0;
+      if (e.o[0].nbrs > ord) done = false;
+      if (e.o[0].count == 0) {
+        e.side = 1;
+        if (e.o[1].nbrs > ord) done = false;
+        if (e.o[1].count == 0) e.state = 2;
+      }
+    }
+    int chunk = 1;
+    bool active = true;
+    std::vector<Task*> batch;
+    bool truncated = false;
+    while (active) {
+      batch.clear();
+      for (EdgeRun& e : edges) {
+        if (e.state != 0) continue;
+        const Orientation& o = e.o[e.side];
+        int taken = 0;
+        for (int c = e.cursor; c < o.count && taken < chunk; ++c) {
+          Task& t = tasks[(size_t)o.first + c];
+          if (!t.computed) {
+            batch.push_back(&t);
+            ++taken;
+          }
+        }
+        if (plain && !batch.empty()) break;
+      }
+      if (plain) {
+        if (opt.plain_test_limit > 0 && plain_tests >= opt.plain_test_limit) {
+          truncated = true;
+          break;
+        }
+        eng.begin_level(ord);  // fresh caches: one launch sequence per test
+        eng.reserve_level(1, 2);
+        plain_tests += (long long)batch.size();
+      }
+      eng.evaluate(batch);
+      // replay in canonical order
+      active = false;
+      for (EdgeRun& e : edges) {
+        while (e.state == 0) {
+          const Orientation& o = e.o[e.side];
+          if (e.cursor < o.count) {
+            Task& t = tasks[(size_t)o.first + e.cursor];
+            if (!t.computed) break;
+            t.consumed = true;
+            ++r->n_edgetests[ord];
+            double& pm = r->pmax[(size_t)t.x * p + t.y];
+            if (pm < t.p) pm = t.p;
+            if (t.p >= opt.alpha) {
+              t.deleted = true;
+              e.state = 1;
+              r->adjacency[(size_t)e.a * p + e.b] = 0;  // applied after the level via snap
+              r->sepset_len[(size_t)e.a * p + e.b] = r->sepset_len[(size_t)e.b * p + e.a] = ord;
+              for (int i = 0; i < ord; ++i) {
+                r->sepset[((size_t)e.a * p + e.b) * 64 + i] = t.S[i];
+                r->sepset[((size_t)e.b * p + e.a) * 64 + i] = t.S[i];
+              }
+              break;
+            }
+            ++e.cursor;
+          } else if (e.side == 0) {
+            e.side = 1;
+            e.cursor = 0;
+            if (e.o[1].nbrs > ord) done = false;
+            if (e.o[1].count == 0) e.state = 2;
+          } else {
+            e.state = 2;
+          }
+        }
+        if (e.state == 0) active = true;
+      }
+      if (!plain) chunk = std::min(chunk * 2, 1 << 20);
+    }
+    for (int a = 0; a < p; ++a)
+      for (int b = 0; b < p; ++b)
+        if (!r->adjacency[(size_t)a * p + b] || !r->adjacency[(size_t)b * p + a]) {
+          r->adjacency[(size_t)a * p + b] = 0;
+          r->adjacency[(size_t)b * p + a] = 0;
+        }
+    for (const Task& t : tasks)
+      if (t.consumed) push_trace(r, cap, t, ord);
+    ++ord;
+    if (truncated) {
+      r->truncated = 1;
+      break;
+    }
+  }
+  r->levels = ord;
+  for (int i = 0; i < p - 1; ++i)
+    for (int j = i + 1; j < p; ++j) {
+      const double m = std::max(r->pmax[(size_t)i * p + j], r->pmax[(size_t)j * p + i]);
+      r->pmax[(size_t)i * p + j] = r->pmax[(size_t)j * p + i] = m;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void kpc_default_options(kpc_options* o) {
+  std::memset(o, 0, sizeof(*o));
+  o->ci_method = KPC_DCC_GAMMA;
+  o->alpha = 0.1;
+  o->max_conditioning_size = -1;
+  o->index = 1.0;
+  o->num_col = 35;
+  o->hsic_sig = 1.0;
+  o->replicates = 100;
+  o->include_observed = 1;
+  o->mode = KPC_BATCHED;
+}
+
+int kpc_skeleton_run(const double* data, int n, int p, const kpc_options* opt, kpc_result** out) {
+  try {
+    if (!data || !opt || !out) throw KpcError(-1, "null argument");
+    if (!(opt->alpha > 0.0 && opt->alpha < 1.0)) throw KpcError(-1, "alpha must be in (0, 1)");
+    const double t0 = now_s();
+    Engine eng(data, n, p, *opt);
+    kpc_result* r = new_result(p);
+    try {
+      run_skeleton(eng, *opt, r);
+    } catch (...) {
+      kpc_result_free(r);
+      throw;
+    }
+    r->wall_seconds = now_s() - t0;
+    for (int s = 0; s < 5; ++s) r->stage_seconds[s] = eng.stage(s);
+    r->stage_seconds[5] = eng.stage_host();
+    r->physical_tests = eng.stats_.physical_tests;
+    r->residual_fits = eng.stats_.fits;
+    r->components = eng.stats_.components;
+    r->setups = eng.stats_.setups;
+    r->magic_evaluations = eng.stats_.evals;
+    *out = r;
+    return 0;
+  } catch (const std::exception& e) {
+    return fill_error(e);
+  }
+}
+
+void kpc_result_free(kpc_result* r) {
+  if (!r) return;
+  std::free(r->adjacency);
+  std::free(r->pmax);
+  std::free(r->sepset_len);
+  std::free(r->sepset);
+  std::free(r->task_level);
+  std::free(r->task_x);
+  std::free(r->task_y);
+  std::free(r->task_slen);
+  std::free(r->task_S);
+  std::free(r->task_p);
+  std::free(r->task_deleted);
+  std::free(r);
+}
+
+int kpc_ci_batch(const double* data, int n, int p, const kpc_options* opt, int count, const int* x,
+                 const int* y, int s_len, const int* S, double* out) {
+  try {
+    if (!data || !opt || !out || count < 0 || s_len < 0 || s_len > 62)
+      throw KpcError(-1, "invalid arguments");
+    Engine eng(data, n, p, *opt);
+    eng.begin_level(s_len);
+    std::vector<Task> tasks((size_t)count);
+    for (int i = 0; i < count; ++i) {
+      Task& t = tasks[i];
+      t.x = x[i];
+      t.y = y[i];
+      t.slen = s_len;
+      if (t.x < 0 || t.x >= p || t.y < 0 || t.y >= p || t.x == t.y)
+        throw KpcError(-1, "x/y out of range");
+      for (int j = 0; j < s_len; ++j) {
+        t.S[j] = S[(size_t)i * s_len + j];
+        if (t.S[j] < 0 || t.S[j] >= p || t.S[j] == t.x || t.S[j] == t.y)
+          throw KpcError(-1, "S out of range or overlaps x/y");
+        if (j > 0 && t.S[j] <= t.S[j - 1]) throw KpcError(-1, "S must be strictly increasing");
+      }
+      t.mask = mask_of(t.S, s_len);
+    }
+    eng.reserve_level((size_t)count, (size_t)count * 2);
+    std::vector<Task*> ptrs;
+    for (Task& t : tasks) ptrs.push_back(&t);
+    eng.evaluate(ptrs);
+    for (int i = 0; i < count; ++i) {
+      out[4 * i + 0] = tasks[i].p;
+      out[4 * i + 1] = tasks[i].stat;
+      out[4 * i + 2] = tasks[i].mean;
+      out[4 * i + 3] = tasks[i].var;
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fill_error(e);
+  }
+}
+
+int kpc_regrxons_batch(const double* data, int n, int p, int count, const int* targets, int s_len,
+                       const int* S, double* resid, double* info) {
+  try {
+    if (!data || !targets || !resid || count <= 0 || s_len < 1 || s_len > 62)
+      throw KpcError(-1, "invalid arguments");
+    kpc_options o;
+    kpc_default_options(&o);
+    o.num_col = std::min(35, n - 1);
+    Engine eng(data, n, p, o);
+    eng.begin_level(s_len);
+    std::vector<Task> sets((size_t)count);
+    for (int i = 0; i < count; ++i) {
+      sets[i].slen = s_len;
+      sets[i].x = targets[i];
+      if (targets[i] < 0 || targets[i] >= p) throw KpcError(-1, "target out of range");
+      for (int j = 0; j < s_len; ++j) {
+        sets[i].S[j] = S[(size_t)i * s_len + j];
+        if (sets[i].S[j] < 0 || sets[i].S[j] >= p) throw KpcError(-1, "S out of range");
+        if (j > 0 && sets[i].S[j] <= sets[i].S[j - 1]) throw KpcError(-1, "S must be increasing");
+      }
+      sets[i].mask = mask_of(sets[i].S, s_len);
+    }
+    eng.reserve_level((size_t)count, (size_t)count);
+    std::vector<int> dslots = eng.build_designs(sets);
+    std::vector<kpc::MagicItem> items((size_t)count);
+    for (int i = 0; i < count; ++i) {
+      items[i].design_slot = dslots[i];
+      items[i].target = targets[i];
+      items[i].resid_slot = i;
+      items[i].pad = 0;
+    }
+    DevArray<kpc::MagicItem> d_items;
+    d_items.upload(items, eng.stream());
+    const long long stride = kpc::magic_scratch_doubles(eng.q(), eng.m(), eng.rk());
+    const int grid = std::min(count, 8192);
+    eng.scratch3().reserve((size_t)grid * stride);
+    hip_check(kpc::launch_magic(d_items.ptr, count, n, eng.q(), eng.m(), eng.rk(), eng.data_ptr(),
+                                eng.design_ptr(), eng.resid_arena().ptr, eng.rinfo_arena().ptr,
+                                eng.scratch3().ptr, grid, eng.ctl(), eng.stream()),
+              "magic launch");
+    hip_check(hipStreamSynchronize(eng.stream()), "magic synchronize");
+    hip_check(hipMemcpy(resid, eng.resid_arena().ptr, sizeof(double) * (size_t)count * n,
+                        hipMemcpyDeviceToHost),
+              "D2H residuals");
+    if (info)
+      hip_check(hipMemcpy(info, eng.rinfo_arena().ptr,
+                          sizeof(double) * (size_t)count * kpc::kMagicInfo, hipMemcpyDeviceToHost),
+                "D2H fit info");
+    return 0;
+  } catch (const std::exception& e) {
+    return fill_error(e);
+  }
+}
+
+int kpc_gcv_grad_hess(const double* data, int n, int p, int count, const int* targets, int s_len,
+                      const int* S, const double* log_sp, double* score, double* grad,
+                      double* hess) {
+  try {
+    if (!data || !targets || !log_sp || count <= 0 || s_len < 1 || s_len > 62)
+      throw KpcError(-1, "invalid arguments");
+    kpc_options o;
+    kpc_default_options(&o);
+    o.num_col = std::min(35, n - 1);
+    Engine eng(data, n, p, o);
+    eng.begin_level(s_len);
+    std::vector<Task> sets((size_t)count);
+    for (int i = 0; i < count; ++i) {
+      sets[i].slen = s_len;
+      sets[i].x = targets[i];
+      if (targets[i] < 0 || targets[i] >= p) throw KpcError(-1, "target out of range");
+      for (int j = 0; j < s_len; ++j) {
+        sets[i].S[j] = S[(size_t)i * s_len + j];
+        if (sets[i].S[j] < 0 || sets[i].S[j] >= p) throw KpcError(-1, "S out of range");
+        if (j > 0 && sets[i].S[j] <= sets[i].S[j - 1]) throw KpcError(-1, "S must be increasing");
+      }
+      sets[i].mask = mask_of(sets[i].S, s_len);
+    }
+    eng.reserve_level((size_t)count, (size_t)count);
+    std::vector<int> dslots = eng.build_designs(sets);
+    const int m = eng.m();
+    std::vector<kpc::MagicItem> items((size_t)count);
+    for (int i = 0; i < count; ++i) {
+      items[i].design_slot = dslots[i];
+      items[i].target = targets[i];
+      items[i].resid_slot = i;
+      items[i].pad = 0;
+    }
+    DevArray<kpc::MagicItem> d_items;
+    d_items.upload(items, eng.stream());
+    std::vector<double> hl(log_sp, log_sp + (size_t)count * m);
+    DevArray<double> d_lsp, d_out;
+    d_lsp.upload(hl, eng.stream());
+    const size_t per = (size_t)(1 + m + m * m);
+    d_out.reserve(per * count);
+    const long long stride = kpc::magic_scratch_doubles(eng.q(), m, eng.rk());
+    const int grid = std::min(count, 8192);
+    eng.scratch3().reserve((size_t)grid * stride);
+    hip_check(kpc::launch_gcv_eval(d_items.ptr, count, n, eng.q(), m, eng.rk(), eng.data_ptr(),
+                                   eng.design_ptr(), d_lsp.ptr, d_out.ptr, eng.scratch3().ptr, grid,
+                                   eng.ctl().rank_tol, eng.stream()),
+              "gcv eval launch");
+    hip_check(hipStreamSynchronize(eng.stream()), "gcv synchronize");
+    std::vector<double> h(per * count);
+    hip_check(hipMemcpy(h.data(), d_out.ptr, sizeof(double) * h.size(), hipMemcpyDeviceToHost),
+              "D2H gcv");
+    for (int i = 0; i < count; ++i) {
+      const double* o2 = h.data() + per * i;
+      if (score) score[i] = o2[0];
+      if (grad) for (int j = 0; j < m; ++j) grad[(size_t)i * m + j] = o2[1 + j];
+      if (hess) for (int j = 0; j < m * m; ++j) hess[(size_t)i * m * m + j] = o2[1 + m + j];
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fill_error(e);
+  }
+}
+
+int kpc_dcov_components(const double* x, int n, int count, int k, double* values, double* vectors,
+                        double* dsum, double* moment) {
+  try {
+    if (!x || count <= 0 || n < 6 || n > kpc::kMaxN || k <= 0 || k >= n || k > kpc::kMaxNumCol)
+      throw KpcError(-1, "invalid arguments");
+    for (long long i = 0; i < (long long)n * count; ++i)
+      if (!std::isfinite(x[i])) throw KpcError(-1, "data must be finite");
+    hipStream_t s;
+    hip_check(hipStreamCreate(&s), "stream");
+    DevArray<double> d_x, d_vals, d_vecs, d_dsum, d_mom, d_scr;
+    DevArray<long long> d_src;
+    DevArray<int> d_dst;
+    std::vector<double> hx(x, x + (size_t)n * count);
+    d_x.upload(hx, s);
+    std::vector<long long> src(count);
+    std::vector<int> dst(count);
+    for (int i = 0; i < count; ++i) { src[i] = (long long)i * n; dst[i] = i; }
+    d_src.upload(src, s);
+    d_dst.upload(dst, s);
+    d_vals.reserve((size_t)count * k);
+    d_vecs.reserve((size_t)count * n * k);
+    d_dsum.reserve(count);
+    d_mom.reserve(count);
+    const long long stride = kpc::dcov_component_scratch_doubles(n, k);
+    const int grid = std::min(count, 1024);
+    d_scr.reserve((size_t)grid * stride);
+    hip_check(kpc::launch_dcov_components(d_x.ptr, d_src.ptr, d_dst.ptr, count, n, k, d_vals.ptr,
+                                          d_vecs.ptr, d_dsum.ptr, d_mom.ptr, d_scr.ptr, grid, s),
+              "component launch");
+    hip_check(hipStreamSynchronize(s), "component synchronize");
+    if (values) hip_check(hipMemcpy(values, d_vals.ptr, sizeof(double) * count * k, hipMemcpyDeviceToHost), "D2H");
+    if (vectors) hip_check(hipMemcpy(vectors, d_vecs.ptr, sizeof(double) * (size_t)count * n * k, hipMemcpyDeviceToHost), "D2H");
+    if (dsum) hip_check(hipMemcpy(dsum, d_dsum.ptr, sizeof(double) * count, hipMemcpyDeviceToHost), "D2H");
+    if (moment) hip_check(hipMemcpy(moment, d_mom.ptr, sizeof(double) * count, hipMemcpyDeviceToHost), "D2H");
+    (void)hipStreamDestroy(s);
+    return 0;
+  } catch (const std::exception& e) {
+    return fill_error(e);
+  }
+}
+
+const char* kpc_last_error(void) { return g_last_error.c_str(); }
+
+int kpc_device_count(void) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+  return c;
+}
+
+}  // extern "C"

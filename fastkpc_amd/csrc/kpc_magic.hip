@@ -1,0 +1,1105 @@
+// regrXonS design geometry and mgcv `magic` GCV smoothing selection on gfx950.
+//
+// geometry_kernel (one workgroup per conditioning set S):
+//   X = [1 | X_1 | ... ] (fastkpc/src/full_cuda_ci_native_setup.cpp:1188-1317),
+//   LAPACK-dgeqp3-style pivoted Householder QR of X, per-block penalty roots
+//   by pivoted Cholesky with tol 0 (mgcv mroot "chol"), roots permuted into QR
+//   order, S_i <- rS_i rS_i', and mgcv initial.sp
+//   (full_cuda_ci_native_setup.cpp:881-1127; mgcv_multi_penalty_cpp.cpp:612-700).
+// magic_kernel (one workgroup per (target, S) fit):
+//   GCV score of [R; chol(sum_i e^rho_i S_i)] by one-sided Jacobi SVD with
+//   rank tolerance sqrt(eps) (mgcv_multi_penalty_cpp.cpp:721-818), gradient and
+//   Hessian in log(sp) (:819-990, evaluated through the factor T_i =
+//   D^-1 V1' rS_i so every product is nr x rank_i), Newton / steepest descent
+//   with step halving, the mgcv convergence test and the final infinity
+//   probes (:1118-1346), then residuals y - X beta.
+#include "kpc_common.h"
+#include "kpc_device.h"
+
+namespace kpc {
+
+constexpr int kMagicThreads = 256;
+constexpr double kEps = 2.220446049250313e-16;
+
+__host__ __device__ inline DesignLayout design_layout(int n, int q, int m, int rk) {
+  DesignLayout L;
+  long long o = 0;
+  L.X = o; o += (long long)n * q;
+  L.QR = o; o += (long long)n * q;
+  L.tau = o; o += q;
+  L.R = o; o += (long long)q * q;
+  L.piv = o; o += q;
+  L.rS = o; o += (long long)q * m * rk;
+  L.Sp = o; o += (long long)m * q * q;
+  L.lsp0 = o; o += m;
+  L.status = o; o += 2;
+  L.stride = (o + 31) / 32 * 32;
+  return L;
+}
+
+DesignLayout host_design_layout(int n, int q, int m, int rk) { return design_layout(n, q, m, rk); }
+
+// R mean of a short vector (serial; long double + correction in R).
+__device__ double serial_r_mean(const double* x, int n) {
+  dd s{0.0, 0.0};
+  for (int i = 0; i < n; ++i) s = dd_add(s, dd{x[i], 0.0});
+  dd m = dd_div_int(s, (double)n);
+  dd c{0.0, 0.0};
+  for (int i = 0; i < n; ++i) c = dd_add(c, dd_add(dd{x[i], 0.0}, dd{-m.hi, -m.lo}));
+  dd r = dd_add(m, dd_div_int(c, (double)n));
+  return r.hi + r.lo;
+}
+
+// Pivoted Cholesky (LAPACK dpstf2, upper) of the q x q symmetric matrix A
+// (column-major, ld q).  work: 2q doubles.  Returns the rank; piv receives the
+// pivot order.  Pivot selection / swaps on lane 0, row updates in parallel.
+__device__ int pstf2_upper(double* A, int q, int* piv, double* work, double tol, int* ictl,
+                           double* dctl) {
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    for (int i = 0; i < q; ++i) { piv[i] = i; work[i] = 0.0; }
+    int pvt = 0;
+    double ajj = A[0];
+    for (int i = 1; i < q; ++i)
+      if (A[(size_t)i * q + i] > ajj) { ajj = A[(size_t)i * q + i]; pvt = i; }
+    ictl[0] = (ajj <= 0.0 || isnan(ajj)) ? 0 : 1;
+    ictl[1] = pvt;
+    dctl[0] = ajj;
+    dctl[1] = tol < 0.0 ? (double)q * kEps * ajj : tol;
+    ictl[2] = q;  // rank
+  }
+  __syncthreads();
+  if (!ictl[0]) return 0;
+  const double dstop = dctl[1];
+  for (int j = 0; j < q; ++j) {
+    if (tid == 0) {
+      for (int i = j; i < q; ++i) {
+        if (j > 0) work[i] += A[(size_t)i * q + j - 1] * A[(size_t)i * q + j - 1];
+        work[q + i] = A[(size_t)i * q + i] - work[i];
+      }
+      int pvt = ictl[1];
+      double ajj = dctl[0];
+      int stop = 0;
+      if (j > 0) {
+        pvt = j;
+        ajj = work[q + j];
+        for (int i = j + 1; i < q; ++i)
+          if (work[q + i] > ajj) { ajj = work[q + i]; pvt = i; }
+        if (ajj <= dstop || isnan(ajj)) {
+          A[(size_t)j * q + j] = ajj;
+          ictl[2] = j;
+          stop = 1;
+        }
+      }
+      if (!stop && j != pvt) {
+        A[(size_t)pvt * q + pvt] = A[(size_t)j * q + j];
+        for (int k = 0; k < j; ++k) {
+          double t = A[(size_t)j * q + k];
+          A[(size_t)j * q + k] = A[(size_t)pvt * q + k];
+          A[(size_t)pvt * q + k] = t;
+        }
+        for (int k = pvt + 1; k < q; ++k) {
+          double t = A[(size_t)k * q + j];
+          A[(size_t)k * q + j] = A[(size_t)k * q + pvt];
+          A[(size_t)k * q + pvt] = t;
+        }
+        for (int k = j + 1; k < pvt; ++k) {
+          double t = A[(size_t)k * q + j];
+          A[(size_t)k * q + j] = A[(size_t)pvt * q + k];
+          A[(size_t)pvt * q + k] = t;
+        }
+        double t = work[j]; work[j] = work[pvt]; work[pvt] = t;
+        int it = piv[pvt]; piv[pvt] = piv[j]; piv[j] = it;
+      }
+      if (!stop) {
+        ajj = sqrt(ajj);
+        A[(size_t)j * q + j] = ajj;
+        dctl[2] = 1.0 / ajj;
+      }
+      ictl[3] = stop;
+    }
+    __syncthreads();
+    if (ictl[3]) break;
+    const double rinv = dctl[2];
+    for (int c = j + 1 + tid; c < q; c += blockDim.x) {
+      double s = 0.0;
+      for (int k = 0; k < j; ++k) s += A[(size_t)c * q + k] * A[(size_t)j * q + k];
+      A[(size_t)c * q + j] = (A[(size_t)c * q + j] - s) * rinv;
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  return ictl[2];
+}
+
+// ------------------------------------------------------------- geometry
+__global__ __launch_bounds__(kMagicThreads) void geometry_kernel(
+    const GeometryItem* __restrict__ items, int count, int n, int q, int m, int dim, int rk,
+    const double* __restrict__ smooth_X, const double* __restrict__ smooth_S,
+    double* __restrict__ design, long long smooth_x_stride, double* __restrict__ scratch,
+    long long scratch_stride) {
+  __shared__ double red[16];
+  __shared__ double vn1[kMaxQ], vn2[kMaxQ], colv[kMaxQ];
+  __shared__ int ipiv[kMaxQ];
+  __shared__ int ictl[8];
+  __shared__ double dctl[8];
+  const int tid = threadIdx.x;
+  const int lane = tid & (KPC_WAVE - 1);
+  const int wid = tid / KPC_WAVE;
+  const int nw = blockDim.x / KPC_WAVE;
+  const DesignLayout DL = design_layout(n, q, m, rk);
+  double* ws = scratch + (size_t)blockIdx.x * scratch_stride;
+  for (int item = blockIdx.x; item < count; item += gridDim.x) {
+    const GeometryItem it = items[item];
+    double* D = design + (size_t)it.design_slot * DL.stride;
+    double* X = D + DL.X;
+    double* QR = D + DL.QR;
+    // assemble X
+    for (int r = tid; r < n; r += blockDim.x) X[r] = 1.0;
+    for (int s = 0; s < m; ++s) {
+      const double* Xs = smooth_X + (size_t)it.smooth_slot[s] * smooth_x_stride;
+      for (int e = tid; e < n * dim; e += blockDim.x) X[(size_t)(1 + s * dim) * n + e] = Xs[e];
+    }
+    __syncthreads();
+    for (int e = tid; e < n * q; e += blockDim.x) QR[e] = X[e];
+    __syncthreads();
+    // pivoted QR (dgeqp3 semantics)
+    for (int j = wid; j < q; j += nw) {
+      double s = 0.0;
+      for (int r = lane; r < n; r += KPC_WAVE) s += QR[(size_t)j * n + r] * QR[(size_t)j * n + r];
+      s = wave_sum(s);
+      if (lane == 0) { vn1[j] = sqrt(s); vn2[j] = sqrt(s); ipiv[j] = j; }
+    }
+    __syncthreads();
+    const double tol3z = sqrt(kEps);
+    for (int i = 0; i < q; ++i) {
+      if (tid == 0) {
+        int pvt = i;
+        for (int j = i + 1; j < q; ++j) if (vn1[j] > vn1[pvt]) pvt = j;
+        ictl[0] = pvt;
+        if (pvt != i) {
+          int t = ipiv[pvt]; ipiv[pvt] = ipiv[i]; ipiv[i] = t;
+          vn1[pvt] = vn1[i];
+          vn2[pvt] = vn2[i];
+        }
+      }
+      __syncthreads();
+      const int pvt = ictl[0];
+      if (pvt != i) {
+        for (int r = tid; r < n; r += blockDim.x) {
+          double t = QR[(size_t)pvt * n + r];
+          QR[(size_t)pvt * n + r] = QR[(size_t)i * n + r];
+          QR[(size_t)i * n + r] = t;
+        }
+        __syncthreads();
+      }
+      // larfg on column i rows i..n-1
+      double part = 0.0;
+      for (int r = i + 1 + tid; r < n; r += blockDim.x) part += QR[(size_t)i * n + r] * QR[(size_t)i * n + r];
+      const double xn2 = block_sum(part, red);
+      const double alpha = QR[(size_t)i * n + i];
+      double tau = 0.0, beta = alpha, scal = 0.0;
+      if (xn2 != 0.0) {
+        beta = -copysign(hypot(alpha, sqrt(xn2)), alpha);
+        tau = (beta - alpha) / beta;
+        scal = 1.0 / (alpha - beta);
+      }
+      for (int r = i + 1 + tid; r < n; r += blockDim.x) QR[(size_t)i * n + r] *= scal;
+      __syncthreads();
+      if (tid == 0) {
+        QR[(size_t)i * n + i] = beta;
+        D[DL.tau + i] = tau;
+      }
+      __syncthreads();
+      // apply H_i to trailing columns
+      if (tau != 0.0) {
+        for (int j = i + 1 + wid; j < q; j += nw) {
+          double* cj = QR + (size_t)j * n;
+          const double* vi = QR + (size_t)i * n;
+          double s = 0.0;
+          for (int r = i + lane; r < n; r += KPC_WAVE) s += (r == i ? 1.0 : vi[r]) * cj[r];
+          s = wave_sum(s) * tau;
+          for (int r = i + lane; r < n; r += KPC_WAVE) cj[r] -= s * (r == i ? 1.0 : vi[r]);
+        }
+      }
+      __syncthreads();
+      // norm downdates
+      for (int j = i + 1 + wid; j < q; j += nw) {
+        if (vn1[j] != 0.0) {
+          double temp = fabs(QR[(size_t)j * n + i]) / vn1[j];
+          temp = 1.0 - temp * temp;
+          if (temp < 0.0) temp = 0.0;
+          double temp2 = temp * (vn1[j] / vn2[j]) * (vn1[j] / vn2[j]);
+          if (temp2 <= tol3z) {
+            double s = 0.0;
+            for (int r = i + 1 + lane; r < n; r += KPC_WAVE) s += QR[(size_t)j * n + r] * QR[(size_t)j * n + r];
+            s = wave_sum(s);
+            if (lane == 0) { vn1[j] = sqrt(s); vn2[j] = vn1[j]; }
+          } else if (lane == 0) {
+            vn1[j] *= sqrt(temp);
+          }
+        }
+      }
+      __syncthreads();
+    }
+    // R and pivots
+    for (int e = tid; e < q * q; e += blockDim.x) {
+      const int r = e % q, c = e / q;
+      D[DL.R + e] = r <= c ? QR[(size_t)c * n + r] : 0.0;
+    }
+    for (int i = tid; i < q; i += blockDim.x) D[DL.piv + i] = (double)ipiv[i];
+    __syncthreads();
+    // penalty roots: per block pivoted Cholesky with tol 0 (mroot)
+    int status = 0;
+    for (int s = 0; s < m; ++s) {
+      const double* Sb = smooth_S + (size_t)it.smooth_slot[s] * 29 * 29;
+      double* F = ws;               // dim x dim
+      double* wk = ws + dim * dim;  // 2 dim
+      int* pv = reinterpret_cast<int*>(ws + dim * dim + 2 * dim + 8);
+      for (int e = tid; e < dim * dim; e += blockDim.x) {
+        const int r = e % dim, c = e / dim;
+        F[e] = r <= c ? Sb[(size_t)c * dim + r] : 0.0;
+      }
+      __syncthreads();
+      const int ar = pstf2_upper(F, dim, pv, wk, 0.0, ictl, dctl);
+      if (ar < rk) status = -5;
+      if (ar < dim) {
+        for (int e = tid; e < dim * dim; e += blockDim.x) {
+          const int r = e % dim, c = e / dim;
+          if (c >= ar && r >= ar) F[e] = 0.0;
+        }
+      }
+      __syncthreads();
+      // inverse pivot -> block root (dim x rk): root[o][c] = F[c][inv[o]]
+      double* rS = D + DL.rS + (size_t)s * q * rk;
+      for (int e = tid; e < q * rk; e += blockDim.x) rS[e] = 0.0;
+      __syncthreads();
+      for (int e = tid; e < dim * rk; e += blockDim.x) {
+        const int pos = e % dim, c = e / dim;  // pos = pivoted position
+        const int orig = pv[pos];
+        // full row index = 1 + s*dim + orig; then permute rows into QR order
+        const int full_row = 1 + s * dim + orig;
+        // find pivoted row r with ipiv[r] == full_row
+        int prow = -1;
+        for (int r = 0; r < q; ++r) if (ipiv[r] == full_row) { prow = r; break; }
+        rS[(size_t)c * q + prow] = F[(size_t)pos * dim + c];
+      }
+      __syncthreads();
+      double* Sp = D + DL.Sp + (size_t)s * q * q;
+      for (int e = tid; e < q * q; e += blockDim.x) {
+        const int r = e % q, c = e / q;
+        double v = 0.0;
+        for (int k2 = 0; k2 < rk; ++k2) v += rS[(size_t)k2 * q + r] * rS[(size_t)k2 * q + c];
+        Sp[e] = v;
+      }
+      __syncthreads();
+    }
+    // initial.sp
+    for (int c = wid; c < q; c += nw) {
+      dd s{0.0, 0.0};
+      for (int r = lane; r < n; r += KPC_WAVE) {
+        const double v = X[(size_t)c * n + r];
+        s = dd_add(s, dd{v * v, 0.0});
+      }
+      s = wave_sum_dd(s);
+      if (lane == 0) colv[c] = s.hi + s.lo;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double* pd = ws;                 // q
+      unsigned char pen[kMaxQ];
+      double sp[kMaxPen];
+      for (int c = 0; c < q; ++c) { pd[c] = 0.0; pen[c] = 0; }
+      const double thr_scale = pow(kEps, 0.8);
+      double sx[32], sp_d[32];
+      for (int s = 0; s < m; ++s) {
+        const double* B = smooth_S + (size_t)it.smooth_slot[s] * 29 * 29;
+        const int off = 1 + s * dim;
+        double mx = 0.0;
+        for (int e = 0; e < dim * dim; ++e) mx = fmax(mx, fabs(B[e]));
+        const double thr = thr_scale * mx;
+        int cnt = 0;
+        for (int r = 0; r < dim; ++r) {
+          dd rt{0.0, 0.0}, ct{0.0, 0.0};
+          for (int o = 0; o < dim; ++o) {
+            rt = dd_add(rt, dd{fabs(B[(size_t)o * dim + r]), 0.0});
+            ct = dd_add(ct, dd{fabs(B[(size_t)r * dim + o]), 0.0});
+          }
+          dd rm = dd_div_int(rt, (double)dim), cmn = dd_div_int(ct, (double)dim);
+          const double rmn = rm.hi + rm.lo, cmv2 = cmn.hi + cmn.lo;
+          const double dg = fabs(B[(size_t)r * dim + r]);
+          if (rmn > thr && cmv2 > thr && dg > thr) {
+            sx[cnt] = colv[off + r];
+            sp_d[cnt] = B[(size_t)r * dim + r];
+            pen[off + r] = 1;
+            ++cnt;
+          }
+        }
+        if (cnt == 0) { status = -8; sp[s] = 1.0; continue; }
+        sp[s] = serial_r_mean(sx, cnt) / serial_r_mean(sp_d, cnt);
+        for (int r = 0; r < dim; ++r) pd[off + r] += sp[s] * B[(size_t)r * dim + r];
+      }
+      double vals[kMaxQ];
+      for (int guard = 0; guard < 4000; ++guard) {
+        int cnt = 0;
+        for (int c = 0; c < q; ++c)
+          if (pd[c] > 0.0 && pen[c] && colv[c] > 0.0) vals[cnt++] = colv[c] / (colv[c] + pd[c]);
+        if (serial_r_mean(vals, cnt) > 0.4) {
+          for (int s = 0; s < m; ++s) sp[s] *= 10.0;
+          for (int c = 0; c < q; ++c) pd[c] *= 10.0;
+        } else {
+          break;
+        }
+      }
+      for (int guard = 0; guard < 4000; ++guard) {
+        int cnt = 0;
+        for (int c = 0; c < q; ++c)
+          if (pd[c] > 0.0 && pen[c] && colv[c] > 0.0) vals[cnt++] = colv[c] / (colv[c] + pd[c]);
+        if (serial_r_mean(vals, cnt) < 0.4) {
+          for (int s = 0; s < m; ++s) sp[s] /= 10.0;
+          for (int c = 0; c < q; ++c) pd[c] /= 10.0;
+        } else {
+          break;
+        }
+      }
+      for (int s = 0; s < m; ++s) D[DL.lsp0 + s] = log(sp[s]);
+      D[DL.status] = (double)status;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- magic
+struct MagicLds {
+  double* y;      // n (Q'y in the first q entries after the transform)
+  double* yorig;  // n
+  double* aug;    // (q + q) x q, column-major with ld = rows
+  double* V;      // q x q
+  double* d;      // q singular values (by column)
+  int* ord;       // q: descending singular value order
+  double* y1;     // q
+  double* lsp;    // kMaxPen
+  double* trial;  // kMaxPen
+  double* grad;   // kMaxPen (gradient used by the convergence test)
+  double* cgrad;  // kMaxPen (gradient at the current point)
+  double* hess;   // kMaxPen^2
+  double* nstep;
+  double* sstep;
+  double* step;
+  double* red;    // 16
+  double* dctl;   // 16
+  int* ictl;      // 32
+  int* piv;       // q
+  double* wk;     // 2q
+};
+
+__host__ __device__ inline size_t magic_lds_doubles(int n, int q, bool aug_in_lds) {
+  size_t s = 2 * (size_t)n + (aug_in_lds ? (size_t)3 * q * q : 0) + 4 * (size_t)q +
+             7 * kMaxPen + kMaxPen * kMaxPen + 32 + (size_t)q + 16;
+  return s;
+}
+
+__device__ MagicLds carve_magic(double* base, int n, int q, bool aug_in_lds, double* gscratch) {
+  MagicLds L;
+  double* p = base;
+  L.y = p; p += n;
+  L.yorig = p; p += n;
+  if (aug_in_lds) {
+    L.aug = p; p += 2 * (size_t)q * q;
+    L.V = p; p += (size_t)q * q;
+  } else {
+    L.aug = gscratch;
+    L.V = gscratch + 2 * (size_t)q * q;
+  }
+  L.d = p; p += q;
+  L.y1 = p; p += q;
+  L.lsp = p; p += kMaxPen;
+  L.trial = p; p += kMaxPen;
+  L.grad = p; p += kMaxPen;
+  L.cgrad = p; p += kMaxPen;
+  L.nstep = p; p += kMaxPen;
+  L.sstep = p; p += kMaxPen;
+  L.step = p; p += kMaxPen;
+  L.hess = p; p += kMaxPen * kMaxPen;
+  L.red = p; p += 16;
+  L.dctl = p; p += 16;
+  L.wk = p; p += 2 * q;
+  int* ip = reinterpret_cast<int*>(p);
+  L.ord = ip; ip += q;
+  L.piv = ip; ip += q;
+  L.ictl = ip;
+  return L;
+}
+
+struct MagicScratch {
+  double* agg;   // q x q
+  double* T;     // nr x (m*rk)
+  double* GT;    // nr x (m*rk)
+  double* G;     // nr x nr
+  double* My;    // m x nr
+  double* Ky;
+  double* yK;
+  double* gy;    // nr
+};
+
+__device__ MagicScratch carve_magic_scratch(double* base, int q, int m, int rk) {
+  MagicScratch S;
+  double* p = base + 3 * (size_t)q * q;  // after the global aug/V fallback
+  S.agg = p; p += (size_t)q * q;
+  S.T = p; p += (size_t)q * m * rk;
+  S.GT = p; p += (size_t)q * m * rk;
+  S.G = p; p += (size_t)q * q;
+  S.My = p; p += (size_t)m * q;
+  S.Ky = p; p += (size_t)m * q;
+  S.yK = p; p += (size_t)m * q;
+  S.gy = p; p += q;
+  return S;
+}
+
+long long magic_scratch_doubles(int q, int m, int rk) {
+  long long s = 3LL * q * q + (long long)q * q + 2LL * q * m * rk + (long long)q * q + 3LL * m * q +
+                q + 64;
+  return (s + 31) / 32 * 32;
+}
+
+// One-sided Jacobi SVD of the rows x q matrix A (column-major, ld rows) with
+// round-robin parallel ordering; V accumulates the rotations.
+__device__ void jacobi_svd(double* A, int rows, int q, double* V, double* d, int* ord,
+                           int* ictl) {
+  const int tid = threadIdx.x;
+  for (int e = tid; e < q * q; e += blockDim.x) V[e] = (e % q == e / q) ? 1.0 : 0.0;
+  const int qe = q + (q & 1);
+  const int npairs = qe / 2;
+  int lpp = 64;
+  while (lpp > 1 && npairs * lpp > (int)blockDim.x) lpp >>= 1;
+  const int pair = tid / lpp;
+  const int sub = tid % lpp;
+  __syncthreads();
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    if (tid == 0) ictl[8] = 0;
+    __syncthreads();
+    for (int rnd = 0; rnd < qe - 1; ++rnd) {
+      int a = -1, b = -1;
+      if (pair < npairs) {
+        if (pair == 0) { a = rnd; b = qe - 1; }
+        else { a = (rnd + pair) % (qe - 1); b = (rnd - pair + (qe - 1)) % (qe - 1); }
+        if (a > b) { int t = a; a = b; b = t; }
+      }
+      const bool active = pair < npairs && b < q;
+      double al = 0.0, be = 0.0, ga = 0.0;
+      if (active) {
+        const double* ca = A + (size_t)a * rows;
+        const double* cb = A + (size_t)b * rows;
+        for (int r = sub; r < rows; r += lpp) {
+          const double x = ca[r], y = cb[r];
+          al += x * x;
+          be += y * y;
+          ga += x * y;
+        }
+      }
+      for (int off = lpp >> 1; off > 0; off >>= 1) {
+        al += __shfl_xor(al, off, KPC_WAVE);
+        be += __shfl_xor(be, off, KPC_WAVE);
+        ga += __shfl_xor(ga, off, KPC_WAVE);
+      }
+      if (active && ga != 0.0 && fabs(ga) > kEps * sqrt(al * be)) {
+        const double zeta = (be - al) / (2.0 * ga);
+        const double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        const double c = 1.0 / sqrt(1.0 + t * t);
+        const double s = c * t;
+        double* ca = A + (size_t)a * rows;
+        double* cb = A + (size_t)b * rows;
+        for (int r = sub; r < rows; r += lpp) {
+          const double x = ca[r], y = cb[r];
+          ca[r] = c * x - s * y;
+          cb[r] = s * x + c * y;
+        }
+        double* va = V + (size_t)a * q;
+        double* vb = V + (size_t)b * q;
+        for (int r = sub; r < q; r += lpp) {
+          const double x = va[r], y = vb[r];
+          va[r] = c * x - s * y;
+          vb[r] = s * x + c * y;
+        }
+        if (sub == 0) ictl[8] = 1;
+      }
+      __syncthreads();
+    }
+    if (!ictl[8]) break;
+    __syncthreads();
+  }
+  // singular values and normalised left vectors
+  const int lane = tid & (KPC_WAVE - 1);
+  const int wid = tid / KPC_WAVE;
+  const int nw = blockDim.x / KPC_WAVE;
+  for (int c = wid; c < q; c += nw) {
+    double* col = A + (size_t)c * rows;
+    double s = 0.0;
+    for (int r = lane; r < rows; r += KPC_WAVE) s += col[r] * col[r];
+    s = sqrt(wave_sum(s));
+    if (lane == 0) d[c] = s;
+    const double inv = s > 0.0 ? 1.0 / s : 0.0;
+    for (int r = lane; r < rows; r += KPC_WAVE) col[r] *= inv;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int i = 0; i < q; ++i) ord[i] = i;
+    for (int i = 1; i < q; ++i) {
+      int oi = ord[i];
+      int j = i - 1;
+      while (j >= 0 && d[ord[j]] < d[oi]) { ord[j + 1] = ord[j]; --j; }
+      ord[j + 1] = oi;
+    }
+  }
+  __syncthreads();
+}
+
+// GCV core at log(sp) = lspv.  Results: dctl[0]=rss dctl[1]=edf dctl[2]=score
+// ictl[0]=nr ictl[1]=rank (aggregate root) ictl[2]=status, y1, d, ord.
+__device__ void magic_eval(const MagicLds& L, const MagicScratch& S, const double* D,
+                           const DesignLayout& DL, int n, int q, int m, const double* lspv,
+                           double ynorm2, double rank_tol) {
+  const int tid = threadIdx.x;
+  const int lane = tid & (KPC_WAVE - 1);
+  const int wid = tid / KPC_WAVE;
+  const int nw = blockDim.x / KPC_WAVE;
+  // aggregate penalty, accumulated penalty by penalty
+  for (int e = tid; e < q * q; e += blockDim.x) {
+    double a = 0.0;
+    for (int i = 0; i < m; ++i) a += D[DL.Sp + (size_t)i * q * q + e] * exp(lspv[i]);
+    S.agg[e] = a;
+  }
+  __syncthreads();
+  const int rank = pstf2_upper(S.agg, q, L.piv, L.wk, -1.0, L.ictl + 16, L.dctl + 10);
+  const int rows = q + rank;
+  // augmented [R; root]
+  for (int e = tid; e < rows * q; e += blockDim.x) {
+    const int r = e % rows, c = e / rows;
+    L.aug[e] = r < q ? D[DL.R + (size_t)c * q + r] : 0.0;
+  }
+  __syncthreads();
+  for (int e = tid; e < rank * q; e += blockDim.x) {
+    const int r = e % rank, col = e / rank;
+    if (r <= col) L.aug[(size_t)L.piv[col] * rows + q + r] = S.agg[(size_t)col * q + r];
+  }
+  __syncthreads();
+  jacobi_svd(L.aug, rows, q, L.V, L.d, L.ord, L.ictl);
+  if (tid == 0) {
+    const double thr = L.d[L.ord[0]] * rank_tol;
+    int nr = 0;
+    for (int i = 0; i < q; ++i) if (L.d[L.ord[i]] >= thr) ++nr;
+    L.ictl[0] = nr;
+    L.ictl[1] = rank;
+  }
+  __syncthreads();
+  const int nr = L.ictl[0];
+  // y1 = U1' y0
+  for (int c = wid; c < nr; c += nw) {
+    const double* u = L.aug + (size_t)L.ord[c] * rows;
+    double s = 0.0;
+    for (int r = lane; r < q; r += KPC_WAVE) s += u[r] * L.y[r];
+    s = wave_sum(s);
+    if (lane == 0) L.y1[c] = s;
+  }
+  __syncthreads();
+  double p_yaay = 0.0, p_edf = 0.0;
+  for (int r = tid; r < q; r += blockDim.x) {
+    double v = 0.0;
+    for (int c = 0; c < nr; ++c) {
+      const double u = L.aug[(size_t)L.ord[c] * rows + r];
+      v += u * L.y1[c];
+      p_edf += u * u;
+    }
+    p_yaay += v * v;
+  }
+  const double yAAy = block_sum(p_yaay, L.red);
+  const double edf = block_sum(p_edf, L.red);
+  if (tid == 0) {
+    double yAy = 0.0;
+    for (int c = 0; c < nr; ++c) yAy += L.y1[c] * L.y1[c];
+    double rss = ynorm2 - 2.0 * yAy + yAAy;
+    if (rss < 0.0) rss = 0.0;
+    const double delta = (double)n - edf;
+    L.dctl[0] = rss;
+    L.dctl[1] = edf;
+    L.dctl[2] = delta > 1e-8 ? (double)n * rss / (delta * delta) : __builtin_inf();
+    L.ictl[2] = 0;
+  }
+  __syncthreads();
+}
+
+// Gradient and Hessian of the GCV score at the point last evaluated.
+__device__ void magic_derivs(const MagicLds& L, const MagicScratch& S, const double* D,
+                             const DesignLayout& DL, int n, int q, int m, int rk,
+                             const double* lspv) {
+  const int tid = threadIdx.x;
+  const int nr = L.ictl[0];
+  const int rows = q + L.ictl[1];
+  const int mr = m * rk;
+  // T (nr x m*rk): T[c][a] = (V1' rS)[c][a] / d_c
+  for (int e = tid; e < nr * mr; e += blockDim.x) {
+    const int c = e % nr, a = e / nr;
+    const double* v = L.V + (size_t)L.ord[c] * q;
+    const double* rs = D + DL.rS + (size_t)a * q;
+    double s = 0.0;
+    for (int k = 0; k < q; ++k) s += v[k] * rs[k];
+    S.T[(size_t)a * nr + c] = s / L.d[L.ord[c]];
+  }
+  // G = U1' U1 (nr x nr)
+  for (int e = tid; e < nr * nr; e += blockDim.x) {
+    const int i = e % nr, j = e / nr;
+    const double* ui = L.aug + (size_t)L.ord[i] * rows;
+    const double* uj = L.aug + (size_t)L.ord[j] * rows;
+    double s = 0.0;
+    for (int r = 0; r < q; ++r) s += ui[r] * uj[r];
+    S.G[e] = s;
+  }
+  __syncthreads();
+  // GT = G T, gy = G y1
+  for (int e = tid; e < nr * mr; e += blockDim.x) {
+    const int c = e % nr, a = e / nr;
+    double s = 0.0;
+    for (int k = 0; k < nr; ++k) s += S.G[(size_t)k * nr + c] * S.T[(size_t)a * nr + k];
+    S.GT[e] = s;
+  }
+  for (int c = tid; c < nr; c += blockDim.x) {
+    double s = 0.0;
+    for (int k = 0; k < nr; ++k) s += S.G[(size_t)k * nr + c] * L.y1[k];
+    S.gy[c] = s;
+  }
+  __syncthreads();
+  // per penalty: My = T_i (T_i' y1), Ky = T_i (T_i' G y1), yK = G My
+  for (int e = tid; e < m * nr; e += blockDim.x) {
+    const int i = e / nr, o = e % nr;
+    double my = 0.0, ky = 0.0;
+    for (int a = 0; a < rk; ++a) {
+      const double* Ta = S.T + (size_t)(i * rk + a) * nr;
+      double ty = 0.0, tg = 0.0;
+      for (int c = 0; c < nr; ++c) { ty += Ta[c] * L.y1[c]; tg += Ta[c] * S.gy[c]; }
+      my += Ta[o] * ty;
+      ky += Ta[o] * tg;
+    }
+    S.My[(size_t)i * nr + o] = my;
+    S.Ky[(size_t)i * nr + o] = ky;
+  }
+  __syncthreads();
+  for (int e = tid; e < m * nr; e += blockDim.x) {
+    const int i = e / nr, o = e % nr;
+    double s = 0.0;
+    for (int k = 0; k < nr; ++k) s += S.G[(size_t)k * nr + o] * S.My[(size_t)i * nr + k];
+    S.yK[(size_t)i * nr + o] = s;
+  }
+  __syncthreads();
+  // pairwise traces tr(M_j K_i) = tr((T_j'T_i)(T_i' G T_j)) -> hess scratch
+  for (int e = tid; e < m * m; e += blockDim.x) {
+    const int i = e % m, j = e / m;
+    if (j > i) continue;
+    double tsum = 0.0;
+    for (int a = 0; a < rk; ++a) {        // column of T_j
+      for (int b = 0; b < rk; ++b) {      // column of T_i
+        const double* Tj = S.T + (size_t)(j * rk + a) * nr;
+        const double* Ti = S.T + (size_t)(i * rk + b) * nr;
+        const double* GTj = S.GT + (size_t)(j * rk + a) * nr;
+        double x = 0.0, y = 0.0;
+        for (int c = 0; c < nr; ++c) { x += Tj[c] * Ti[c]; y += Ti[c] * GTj[c]; }
+        tsum += x * y;
+      }
+    }
+    S.agg[e] = tsum;  // agg is free after the evaluation
+  }
+  for (int i = tid; i < m; i += blockDim.x) {
+    double tr = 0.0;
+    for (int a = 0; a < rk; ++a) {
+      const double* Ta = S.T + (size_t)(i * rk + a) * nr;
+      const double* GTa = S.GT + (size_t)(i * rk + a) * nr;
+      for (int c = 0; c < nr; ++c) tr += Ta[c] * GTa[c];
+    }
+    S.agg[m * m + i] = tr;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double dnorm[kMaxPen], ddelta[kMaxPen], d2n[kMaxPen * kMaxPen], d2d[kMaxPen * kMaxPen];
+    for (int i = 0; i < m; ++i) {
+      const double li = exp(lspv[i]);
+      ddelta[i] = li * S.agg[m * m + i];
+      for (int j = 0; j <= i; ++j) {
+        const double dv = -2.0 * exp(lspv[i] + lspv[j]) * S.agg[j * m + i];
+        d2d[j * m + i] = dv;
+        d2d[i * m + j] = dv;
+      }
+      d2d[i * m + i] += ddelta[i];
+      double nd = 0.0;
+      for (int c = 0; c < nr; ++c) nd += L.y1[c] * (S.My[(size_t)i * nr + c] - S.Ky[(size_t)i * nr + c]);
+      dnorm[i] = 2.0 * li * nd;
+      for (int j = 0; j <= i; ++j) {
+        double nv = 0.0;
+        for (int c = 0; c < nr; ++c) {
+          const double myi = S.My[(size_t)i * nr + c], myj = S.My[(size_t)j * nr + c];
+          nv += myi * S.Ky[(size_t)j * nr + c] + myj * S.Ky[(size_t)i * nr + c] - 2.0 * myi * myj +
+                S.yK[(size_t)i * nr + c] * myj;
+        }
+        nv *= 2.0 * exp(lspv[i] + lspv[j]);
+        d2n[j * m + i] = nv;
+        d2n[i * m + j] = nv;
+      }
+      d2n[i * m + i] += dnorm[i];
+    }
+    const double rss = L.dctl[0], edf = L.dctl[1];
+    const double delta = (double)n - edf;
+    const double ss = (double)n / (delta * delta);
+    const double ds = 2.0 * ss * rss / delta;
+    const double cs = -2.0 * ss / delta;
+    const double cv = 3.0 * ds / delta;
+    for (int i = 0; i < m; ++i) L.cgrad[i] = ss * dnorm[i] - ds * ddelta[i];
+    for (int i = 0; i < m; ++i)
+      for (int j = 0; j <= i; ++j) {
+        const double v = cs * (ddelta[j] * dnorm[i] + ddelta[i] * dnorm[j]) + ss * d2n[j * m + i] +
+                         cv * ddelta[i] * ddelta[j] - ds * d2d[j * m + i];
+        L.hess[j * m + i] = v;
+        L.hess[i * m + j] = v;
+      }
+  }
+  __syncthreads();
+}
+
+// Symmetric eigen of the small Hessian (cyclic Jacobi), lane-serial.
+__device__ void small_syev(const double* Hin, int m, double* w, double* Z) {
+  double A[kMaxPen * kMaxPen];
+  for (int i = 0; i < m * m; ++i) A[i] = Hin[i];
+  for (int i = 0; i < m * m; ++i) Z[i] = (i % m == i / m) ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < m * m; ++i) {
+      tot += A[i] * A[i];
+      if (i % m != i / m) off += A[i] * A[i];
+    }
+    if (off <= 1e-300 || off <= kEps * kEps * tot * 1e-4) break;
+    for (int p = 0; p < m - 1; ++p)
+      for (int q2 = p + 1; q2 < m; ++q2) {
+        const double apq = A[q2 * m + p];
+        if (apq == 0.0) continue;
+        const double app = A[p * m + p], aqq = A[q2 * m + q2];
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double t = copysign(1.0, theta) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < m; ++k) {
+          const double akp = A[p * m + k], akq = A[q2 * m + k];
+          A[p * m + k] = c * akp - s * akq;
+          A[q2 * m + k] = s * akp + c * akq;
+        }
+        for (int k = 0; k < m; ++k) {
+          const double apk = A[k * m + p], aqk = A[k * m + q2];
+          A[k * m + p] = c * apk - s * aqk;
+          A[k * m + q2] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < m; ++k) {
+          const double zkp = Z[p * m + k], zkq = Z[q2 * m + k];
+          Z[p * m + k] = c * zkp - s * zkq;
+          Z[q2 * m + k] = s * zkp + c * zkq;
+        }
+      }
+  }
+  for (int i = 0; i < m; ++i) w[i] = A[i * m + i];
+}
+
+__global__ __launch_bounds__(kMagicThreads) void magic_kernel(
+    const MagicItem* __restrict__ items, int count, int n, int q, int m, int rk,
+    const double* __restrict__ data, const double* __restrict__ design,
+    double* __restrict__ resid, double* __restrict__ info, double* __restrict__ scratch,
+    long long scratch_stride, int aug_in_lds, MagicControl ctl) {
+  extern __shared__ double lds_dyn[];
+  const int tid = threadIdx.x;
+  double* gs = scratch + (size_t)blockIdx.x * scratch_stride;
+  MagicLds L = carve_magic(lds_dyn, n, q, aug_in_lds != 0, gs);
+  MagicScratch S = carve_magic_scratch(gs, q, m, rk);
+  const DesignLayout DL = design_layout(n, q, m, rk);
+  for (int item = blockIdx.x; item < count; item += gridDim.x) {
+    const MagicItem it = items[item];
+    const double* D = design + (size_t)it.design_slot * DL.stride;
+    const double* y = data + (size_t)it.target * n;
+    double* out = info + (size_t)it.resid_slot * kMagicInfo;
+    for (int r = tid; r < n; r += blockDim.x) { L.y[r] = y[r]; L.yorig[r] = y[r]; }
+    __syncthreads();
+    // y0 = Q' y (reflectors in order)
+    for (int i = 0; i < q; ++i) {
+      const double* v = D + DL.QR + (size_t)i * n;
+      const double tau = D[DL.tau + i];
+      double part = 0.0;
+      for (int r = i + tid; r < n; r += blockDim.x) part += (r == i ? 1.0 : v[r]) * L.y[r];
+      const double s = block_sum(part, L.red) * tau;
+      for (int r = i + tid; r < n; r += blockDim.x) L.y[r] -= s * (r == i ? 1.0 : v[r]);
+      __syncthreads();
+    }
+    double part = 0.0;
+    for (int r = tid; r < n; r += blockDim.x) part += L.yorig[r] * L.yorig[r];
+    const double ynorm2 = block_sum(part, L.red);
+    if (tid < m) L.lsp[tid] = D[DL.lsp0 + tid];
+    __syncthreads();
+    const int max_half = m == 1 ? ctl.max_half_single : ctl.max_half_multi;
+    magic_eval(L, S, D, DL, n, q, m, L.lsp, ynorm2, ctl.rank_tol);
+    magic_derivs(L, S, D, DL, n, q, m, rk, L.lsp);
+    // optimizer state (lane 0 decides, LDS broadcasts)
+    double* st = L.dctl;  // st[3]=min_score st[4]=score_red
+    int* ic = L.ictl;     // ic[3]=use_sd ic[4]=converged ic[5]=iter ic[6]=tries ic[7]=step_failed
+    if (tid == 0) {
+      st[3] = L.dctl[2];
+      st[4] = 1e10;
+      ic[3] = 0;
+      ic[4] = 0;
+      ic[7] = 0;
+      ic[9] = 1;  // evaluations
+      for (int i = 0; i < m; ++i) { L.grad[i] = 0.0; L.nstep[i] = 0.0; L.sstep[i] = 0.0; }
+    }
+    __syncthreads();
+    int iter;
+    for (iter = 1; iter <= ctl.max_iterations; ++iter) {
+      if (tid == 0) ic[6] = 0;
+      __syncthreads();
+      if (iter > 1) {
+        if (tid == 0) {
+          for (int i = 0; i < m; ++i) L.step[i] = ic[3] ? L.sstep[i] : L.nstep[i];
+          ic[11] = 1;  // trying
+        }
+        __syncthreads();
+        while (ic[11]) {
+          if (tid == 0) {
+            ic[6] += 1;
+            if (ic[6] == 4 && !ic[3]) {
+              ic[3] = 1;
+              for (int i = 0; i < m; ++i) L.step[i] = L.sstep[i];
+            }
+            for (int i = 0; i < m; ++i) L.trial[i] = L.lsp[i] + L.step[i];
+          }
+          __syncthreads();
+          magic_eval(L, S, D, DL, n, q, m, L.trial, ynorm2, ctl.rank_tol);
+          if (tid == 0) {
+            ic[9] += 1;
+            ic[12] = L.dctl[2] < st[3];
+          }
+          __syncthreads();
+          if (ic[12]) {
+            if (tid == 0) {
+              ic[11] = 0;
+              st[4] = st[3] - L.dctl[2];
+              st[3] = L.dctl[2];
+              for (int i = 0; i < m; ++i) L.lsp[i] = L.trial[i];
+            }
+            __syncthreads();
+            magic_derivs(L, S, D, DL, n, q, m, rk, L.lsp);
+          } else if (tid == 0) {
+            for (int i = 0; i < m; ++i) L.step[i] /= 2.0;
+          }
+          if (tid == 0) {
+            if (ic[6] == max_half - 1 && ic[11]) for (int i = 0; i < m; ++i) L.step[i] = 0.0;
+            if (ic[6] == max_half) ic[11] = 0;
+          }
+          __syncthreads();
+        }
+      }
+      if (tid == 0) {
+        if (iter > 3) {
+          double gn = 0.0;
+          for (int i = 0; i < m; ++i) gn += L.grad[i] * L.grad[i];
+          gn = sqrt(gn);
+          int conv = st[4] <= ctl.tol * (1.0 + st[3]) &&
+                     gn <= pow(ctl.tol, 1.0 / 3.0) * (1.0 + fabs(st[3]));
+          if (ic[6] == max_half) { conv = 1; ic[7] = 1; }
+          ic[4] = conv;
+        }
+        for (int i = 0; i < m; ++i) L.grad[i] = L.cgrad[i];
+        double w[kMaxPen], Z[kMaxPen * kMaxPen];
+        small_syev(L.hess, m, w, Z);
+        int sd = 0;
+        for (int i = 0; i < m; ++i) if (w[i] < 0.0) sd = 1;
+        ic[3] = sd;
+        if (!sd) {
+          for (int r = 0; r < m; ++r) {
+            double v = 0.0;
+            for (int c = 0; c < m; ++c) {
+              double pc = 0.0;
+              for (int k = 0; k < m; ++k) pc += Z[c * m + k] * L.grad[k];
+              v += Z[c * m + r] * (pc / w[c]);
+            }
+            L.nstep[r] = -v;
+          }
+          double mx = fabs(L.nstep[0]);
+          for (int i = 1; i < m; ++i) mx = fmax(mx, fabs(L.nstep[i]));
+          if (mx > ctl.max_newton_step)
+            for (int i = 0; i < m; ++i) L.nstep[i] *= ctl.max_newton_step / mx;
+        }
+        double mg = fabs(L.grad[0]);
+        for (int i = 1; i < m; ++i) mg = fmax(mg, fabs(L.grad[i]));
+        for (int i = 0; i < m; ++i) L.sstep[i] = mg == 0.0 ? 0.0 : -L.grad[i] / mg;
+      }
+      __syncthreads();
+      if (ic[4]) break;
+    }
+    const int converged = ic[4];
+    // infinity probes per coordinate
+    if (converged) {
+      for (int c = 0; c < m; ++c) {
+        const double dir = L.grad[c] < 0.0 ? 1.0 : -1.0;
+        for (int pr = 1; pr <= ctl.max_probes; ++pr) {
+          if (tid == 0) {
+            for (int i = 0; i < m; ++i) L.trial[i] = L.lsp[i];
+            L.trial[c] += dir * ctl.probe_step;
+          }
+          __syncthreads();
+          magic_eval(L, S, D, DL, n, q, m, L.trial, ynorm2, ctl.rank_tol);
+          if (tid == 0) {
+            ic[9] += 1;
+            ic[12] = L.dctl[2] < st[3];
+            if (ic[12]) {
+              for (int i = 0; i < m; ++i) L.lsp[i] = L.trial[i];
+              st[3] = L.dctl[2];
+            }
+          }
+          __syncthreads();
+          if (!ic[12]) break;
+        }
+      }
+    }
+    // final fit at the selected log(sp)
+    magic_eval(L, S, D, DL, n, q, m, L.lsp, ynorm2, ctl.rank_tol);
+    {
+      const int nr = L.ictl[0];
+      double* theta = S.gy;  // q entries (gy sized >= q)
+      for (int r = tid; r < q; r += blockDim.x) {
+        double v = 0.0;
+        for (int c = 0; c < nr; ++c)
+          v += L.V[(size_t)L.ord[c] * q + r] * (L.y1[c] / L.d[L.ord[c]]);
+        theta[r] = v;
+      }
+      __syncthreads();
+      double* beta = S.My;  // q entries
+      for (int i = tid; i < q; i += blockDim.x) beta[(int)D[DL.piv + i]] = theta[i];
+      __syncthreads();
+      double* ro = resid + (size_t)it.resid_slot * n;
+      const double* X = D + DL.X;
+      for (int r = tid; r < n; r += blockDim.x) {
+        double f = 0.0;
+        for (int c = 0; c < q; ++c) f += X[(size_t)c * n + r] * beta[c];
+        ro[r] = L.yorig[r] - f;
+      }
+      if (tid == 0) {
+        out[0] = converged ? 0.0 : -7.0;
+        out[1] = L.dctl[2];
+        out[2] = L.dctl[0];
+        out[3] = L.dctl[1];
+        out[4] = (double)iter;
+        out[5] = (double)ic[9];
+        out[6] = (double)ic[7];
+        out[7] = (double)nr;
+        for (int i = 0; i < m && i < kMagicInfo - 8; ++i) out[8 + i] = L.lsp[i];
+      }
+      __syncthreads();
+    }
+  }
+}
+
+size_t magic_lds_bytes(int n, int q, bool aug_in_lds) {
+  return magic_lds_doubles(n, q, aug_in_lds) * sizeof(double);
+}
+
+bool magic_aug_fits_lds(int n, int q) {
+  return magic_lds_bytes(n, q, true) <= 150 * 1024;
+}
+
+long long geometry_scratch_doubles(int q) {
+  long long s = 29LL * 29 + 4 * 29 + 64 + 2LL * q + 64;
+  return (s + 31) / 32 * 32;
+}
+
+hipError_t launch_geometry(const GeometryItem* items, int count, int n, int q, int m, int dim,
+                           int rk, const double* smooth_X, const double* smooth_S, double* design,
+                           long long smooth_x_stride, double* scratch, int grid,
+                           hipStream_t stream) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(geometry_kernel, dim3(grid), dim3(kMagicThreads), 0, stream, items, count, n,
+                     q, m, dim, rk, smooth_X, smooth_S, design, smooth_x_stride, scratch,
+                     geometry_scratch_doubles(q));
+  return hipGetLastError();
+}
+
+hipError_t launch_magic(const MagicItem* items, int count, int n, int q, int m, int rk,
+                        const double* data, const double* design, double* resid, double* info,
+                        double* scratch, int grid, const MagicControl& ctl, hipStream_t stream) {
+  if (count <= 0) return hipSuccess;
+  const bool in_lds = magic_aug_fits_lds(n, q);
+  const size_t lds = magic_lds_bytes(n, q, in_lds);
+  hipError_t err = hipFuncSetAttribute((const void*)magic_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (err != hipSuccess) return err;
+  hipLaunchKernelGGL(magic_kernel, dim3(grid), dim3(kMagicThreads), lds, stream, items, count, n,
+                     q, m, rk, data, design, resid, info, scratch, magic_scratch_doubles(q, m, rk),
+                     in_lds ? 1 : 0, ctl);
+  return hipGetLastError();
+}
+
+}  // namespace kpc
+
+namespace kpc {
+
+// GCV score, gradient and Hessian at caller-supplied log(sp): the hand-written
+// derivative of the magic objective, exposed on its own for testing against
+// central differences and for external Newton drivers.
+__global__ __launch_bounds__(kMagicThreads) void gcv_eval_kernel(
+    const MagicItem* __restrict__ items, int count, int n, int q, int m, int rk,
+    const double* __restrict__ data, const double* __restrict__ design,
+    const double* __restrict__ lsp_in, double* __restrict__ out, double* __restrict__ scratch,
+    long long scratch_stride, int aug_in_lds, double rank_tol) {
+  extern __shared__ double lds_dyn[];
+  const int tid = threadIdx.x;
+  double* gs = scratch + (size_t)blockIdx.x * scratch_stride;
+  MagicLds L = carve_magic(lds_dyn, n, q, aug_in_lds != 0, gs);
+  MagicScratch S = carve_magic_scratch(gs, q, m, rk);
+  const DesignLayout DL = design_layout(n, q, m, rk);
+  for (int item = blockIdx.x; item < count; item += gridDim.x) {
+    const MagicItem it = items[item];
+    const double* D = design + (size_t)it.design_slot * DL.stride;
+    const double* y = data + (size_t)it.target * n;
+    for (int r = tid; r < n; r += blockDim.x) { L.y[r] = y[r]; L.yorig[r] = y[r]; }
+    __syncthreads();
+    for (int i = 0; i < q; ++i) {
+      const double* v = D + DL.QR + (size_t)i * n;
+      const double tau = D[DL.tau + i];
+      double part = 0.0;
+      for (int r = i + tid; r < n; r += blockDim.x) part += (r == i ? 1.0 : v[r]) * L.y[r];
+      const double s = block_sum(part, L.red) * tau;
+      for (int r = i + tid; r < n; r += blockDim.x) L.y[r] -= s * (r == i ? 1.0 : v[r]);
+      __syncthreads();
+    }
+    double part = 0.0;
+    for (int r = tid; r < n; r += blockDim.x) part += L.yorig[r] * L.yorig[r];
+    const double ynorm2 = block_sum(part, L.red);
+    if (tid < m) L.lsp[tid] = lsp_in[(size_t)item * m + tid];
+    __syncthreads();
+    magic_eval(L, S, D, DL, n, q, m, L.lsp, ynorm2, rank_tol);
+    magic_derivs(L, S, D, DL, n, q, m, rk, L.lsp);
+    double* o = out + (size_t)item * (1 + m + m * m);
+    if (tid == 0) {
+      o[0] = L.dctl[2];
+      for (int i = 0; i < m; ++i) o[1 + i] = L.cgrad[i];
+      for (int i = 0; i < m * m; ++i) o[1 + m + i] = L.hess[i];
+    }
+    __syncthreads();
+  }
+}
+
+hipError_t launch_gcv_eval(const MagicItem* items, int count, int n, int q, int m, int rk,
+                           const double* data, const double* design, const double* lsp,
+                           double* out, double* scratch, int grid, double rank_tol,
+                           hipStream_t stream) {
+  if (count <= 0) return hipSuccess;
+  const bool in_lds = magic_aug_fits_lds(n, q);
+  const size_t lds = magic_lds_bytes(n, q, in_lds);
+  hipError_t err = hipFuncSetAttribute((const void*)gcv_eval_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (err != hipSuccess) return err;
+  hipLaunchKernelGGL(gcv_eval_kernel, dim3(grid), dim3(kMagicThreads), lds, stream, items, count,
+                     n, q, m, rk, data, design, lsp, out, scratch, magic_scratch_doubles(q, m, rk),
+                     in_lds ? 1 : 0, rank_tol);
+  return hipGetLastError();
+}
+
+}  // namespace kpc

@@ -1,0 +1,112 @@
+/*
+ * kpcmi — MI355X (gfx950) kernel-PC conditional-independence path, C ABI.
+ *
+ * Drop-in boundary for the reference's R-facing native entry points:
+ *   kpc_skeleton_run    replaces  C_full_cuda_ci_one_call_skeleton[_method]
+ *                                 (fastkpc/src/r_api_cuda.cpp:9183-9249) and the
+ *                                 C++ fastkpc::full_cuda_ci_one_call_skeleton_method
+ *                                 (fastkpc/src/full_cuda_ci_one_call.hpp:19-36)
+ *   kpc_ci_batch        replaces  one same-S cohort of execute_group
+ *                                 (fastkpc/src/full_cuda_ci_one_call.cpp:4197-5036);
+ *                                 per-test semantics of kpcalg kernelCItest
+ *                                 (kpcalg/R/kernelCItest.R:68-129)
+ *   kpc_regrxons_batch  replaces  the GCV optimiser + fixed-sp residual solve
+ *                                 (mgcv_single_penalty_gcv.hpp / mgcv_multi_penalty_gcv_capacity.hpp /
+ *                                  mgcv_fixed_sp_runtime.hpp:70-75); semantics of
+ *                                 kpcalg regrXonS (kpcalg/R/regrXonS.R:33-61)
+ *   kpc_gcv_grad_hess   exposes   the GCV score gradient/Hessian in log(sp)
+ *                                 (fastkpc/src/mgcv_multi_penalty_cpp.cpp:819-990)
+ *   kpc_last_error      replaces  the std::runtime_error -> R error mapping
+ *                                 (BEGIN_RCPP/END_RCPP in r_api_cuda.cpp)
+ *
+ * Conventions: every entry returns 0 on success and a negative status on
+ * failure (then kpc_last_error() describes it); inputs are host pointers to
+ * column-major n x p doubles unless the name says _device; outputs are caller
+ * allocated except kpc_result, which kpc_result_free releases.  One context
+ * per host thread; calls on one GPU from several threads serialise.
+ */
+#ifndef KPCMI_H
+#define KPCMI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum kpc_ci_method { KPC_DCC_GAMMA = 0, KPC_HSIC_GAMMA = 1, KPC_DCC_PERM = 2, KPC_HSIC_PERM = 3 };
+enum kpc_exec_mode { KPC_BATCHED = 0, KPC_PLAIN = 1 };
+
+typedef struct kpc_options {
+  int ci_method;             /* kpc_ci_method */
+  double alpha;              /* deletion threshold: p >= alpha deletes */
+  int max_conditioning_size; /* m.max; negative means Inf (resolved to p - 2) */
+  double index;              /* dcov index; outside [0, 2] reported as 1 */
+  int num_col;               /* truncation rank, 0 < num_col < n */
+  double hsic_sig;           /* rbfdot sigma = 1 / hsic_sig */
+  int replicates;            /* permutation replicates */
+  int include_observed;
+  int has_seed;
+  unsigned long long seed;
+  int mode;                  /* kpc_exec_mode */
+  int plain_test_limit;      /* KPC_PLAIN only: stop after this many tests (0 = all) */
+} kpc_options;
+
+typedef struct kpc_result {
+  int p;
+  int levels;
+  unsigned char* adjacency; /* p x p, row-major, 1 = edge kept */
+  double* pmax;             /* p x p */
+  int* sepset_len;          /* p x p; -1 = no sepset */
+  int* sepset;              /* p x p x 64, 0-based variable indices */
+  int n_edgetests[64];
+  int n_tasks;              /* canonical trace rows (consumed tests) */
+  int* task_level;
+  int* task_x;
+  int* task_y;
+  int* task_slen;
+  int* task_S;              /* n_tasks x 64 */
+  double* task_p;
+  unsigned char* task_deleted;
+  double wall_seconds;
+  double stage_seconds[8];  /* smooth, geometry, gcv, component, pair, host, h2d/d2h, other */
+  long long physical_tests;
+  long long residual_fits;
+  long long components;
+  long long setups;
+  long long magic_evaluations;
+  int truncated;            /* KPC_PLAIN stopped at plain_test_limit */
+} kpc_result;
+
+void kpc_default_options(kpc_options* opt);
+int kpc_skeleton_run(const double* data, int n, int p, const kpc_options* opt, kpc_result** out);
+void kpc_result_free(kpc_result* r);
+
+/* One batch of CI tests (x_i, y_i | S_i), S given as count x s_len 0-based
+ * indices (all tests share s_len).  out: count x 4 {p.value, statistic, mean,
+ * variance}. */
+int kpc_ci_batch(const double* data, int n, int p, const kpc_options* opt, int count,
+                 const int* x, const int* y, int s_len, const int* S, double* out);
+
+/* regrXonS residuals of `targets` on S (count x s_len).  resid: count x n;
+ * info: count x 32 {status, score, rss, edf, iterations, evaluations,
+ * step_failed, numerical_rank, log_sp[0..]} */
+int kpc_regrxons_batch(const double* data, int n, int p, int count, const int* targets,
+                       int s_len, const int* S, double* resid, double* info);
+
+/* GCV score, gradient (m) and Hessian (m x m) in log(sp) at log_sp (count x m)
+ * for target on S. */
+int kpc_gcv_grad_hess(const double* data, int n, int p, int count, const int* targets, int s_len,
+                      const int* S, const double* log_sp, double* score, double* grad,
+                      double* hess);
+
+/* dcov.gamma low-rank components of `count` vectors (n each): values count x k,
+ * centred vectors count x n x k, dsum, moment. */
+int kpc_dcov_components(const double* x, int n, int count, int k, double* values,
+                        double* vectors, double* dsum, double* moment);
+
+const char* kpc_last_error(void);
+int kpc_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,215 @@
+"""ctypes binding of the CPU oracle liborc.so.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py to check or time against the reference
+algorithm.  The product (fastkpc_amd) never imports this module.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liborc.so")
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+
+
+class OrcSkeletonOptions(ctypes.Structure):
+    _fields_ = [("ci_method", ctypes.c_int), ("alpha", ctypes.c_double), ("m_max", ctypes.c_int),
+                ("num_col", ctypes.c_int), ("hsic_sig", ctypes.c_double),
+                ("replicates", ctypes.c_int), ("perm_seed", ctypes.c_ulonglong),
+                ("threads", ctypes.c_int)]
+
+
+class OrcSkeletonResult(ctypes.Structure):
+    _fields_ = [("p", ctypes.c_int), ("levels", ctypes.c_int),
+                ("adjacency", ctypes.POINTER(ctypes.c_ubyte)), ("pmax", _D),
+                ("sepset_len", _I), ("sepset", _I), ("n_edgetests", ctypes.c_int * 64),
+                ("trace_len", ctypes.c_int), ("trace_cap", ctypes.c_int),
+                ("trace_level", _I), ("trace_x", _I), ("trace_y", _I), ("trace_slen", _I),
+                ("trace_S", _I), ("trace_p", _D), ("trace_del", ctypes.POINTER(ctypes.c_ubyte))]
+
+
+class OrcSetup(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("q", ctypes.c_int), ("m", ctypes.c_int), ("X", _D),
+                ("S", ctypes.POINTER(_D)), ("off", _I), ("dim", _I), ("rank", _I)]
+
+
+class OrcMagicControl(ctypes.Structure):
+    _fields_ = [("tol", ctypes.c_double), ("max_step_halving_single", ctypes.c_int),
+                ("max_step_halving_multi", ctypes.c_int), ("max_iterations", ctypes.c_int),
+                ("max_newton_step", ctypes.c_double), ("boundary_probe_step", ctypes.c_double),
+                ("max_boundary_probes", ctypes.c_int), ("rank_tol", ctypes.c_double)]
+
+
+class OrcMagicResult(ctypes.Structure):
+    _fields_ = [("score", ctypes.c_double), ("rss", ctypes.c_double), ("edf", ctypes.c_double),
+                ("log_sp", ctypes.c_double * 64), ("iterations", ctypes.c_int),
+                ("evaluations", ctypes.c_int), ("step_failed", ctypes.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            subprocess.run(["make", "-s", "-C", HERE, "liborc.so"], check=True)
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_dcov_gamma.argtypes = [_D, _D, ctypes.c_int, ctypes.c_int, _D]
+        L.orc_dcov_component.argtypes = [_D, ctypes.c_int, ctypes.c_int, _D, _D, _D, _D]
+        L.orc_pgamma_lower.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.orc_pgamma_lower.restype = ctypes.c_double
+        L.orc_gamma_pvalue.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.orc_gamma_pvalue.restype = ctypes.c_double
+        L.orc_skeleton.argtypes = [_D, ctypes.c_int, ctypes.c_int,
+                                   ctypes.POINTER(OrcSkeletonOptions),
+                                   ctypes.POINTER(OrcSkeletonResult)]
+        L.orc_skeleton_result_free.argtypes = [ctypes.POINTER(OrcSkeletonResult)]
+        L.orc_magic_default_control.argtypes = [ctypes.POINTER(OrcMagicControl)]
+        L.orc_regr_setup.argtypes = [_D, ctypes.c_int, ctypes.c_int, _I, ctypes.c_int,
+                                     ctypes.POINTER(OrcSetup)]
+        L.orc_setup_free.argtypes = [ctypes.POINTER(OrcSetup)]
+        L.orc_magic_fit.argtypes = [ctypes.POINTER(OrcSetup), _D, ctypes.POINTER(OrcMagicControl),
+                                    _D, ctypes.POINTER(OrcMagicResult)]
+        L.orc_magic_eval.argtypes = [ctypes.POINTER(OrcSetup), _D, _D, ctypes.c_double, _D, _D, _D]
+        L.orc_regrxons.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I, ctypes.c_int,
+                                   ctypes.POINTER(OrcMagicControl), _D,
+                                   ctypes.POINTER(OrcMagicResult)]
+        L.orc_ci_test.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I,
+                                  ctypes.c_int, ctypes.POINTER(OrcSkeletonOptions), _D]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_D)
+
+
+def dcov_gamma(x, y, num_col=35):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.zeros(5)
+    rc = lib().orc_dcov_gamma(_d(x), _d(y), len(x), num_col, _d(out))
+    if rc != 0:
+        raise RuntimeError("orc_dcov_gamma failed %d" % rc)
+    return {"p.value": out[0], "nV2": out[1], "mean": out[2], "variance": out[3]}
+
+
+def dcov_component(x, k=35):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = len(x)
+    vals = np.zeros(k)
+    vecs = np.zeros(n * k)
+    dsum = np.zeros(1)
+    mom = np.zeros(1)
+    rc = lib().orc_dcov_component(_d(x), n, k, _d(vals), _d(vecs), _d(dsum), _d(mom))
+    if rc != 0:
+        raise RuntimeError("orc_dcov_component failed %d" % rc)
+    return vals, vecs.reshape(k, n).T, dsum[0], mom[0]
+
+
+def pgamma_lower(x, a):
+    return lib().orc_pgamma_lower(float(x), float(a))
+
+
+def gamma_pvalue(stat, mean, var):
+    return lib().orc_gamma_pvalue(float(stat), float(mean), float(var))
+
+
+def default_control():
+    c = OrcMagicControl()
+    lib().orc_magic_default_control(ctypes.byref(c))
+    return c
+
+
+def regrxons(data, target, S, control=None):
+    d = np.asfortranarray(data, dtype=np.float64)
+    n, p = d.shape
+    Sa = np.ascontiguousarray(sorted(S), dtype=np.int32)
+    resid = np.zeros(n)
+    res = OrcMagicResult()
+    ctl = control or default_control()
+    rc = lib().orc_regrxons(_d(d), n, n, int(target), Sa.ctypes.data_as(_I), len(Sa),
+                            ctypes.byref(ctl), _d(resid), ctypes.byref(res))
+    if rc != 0:
+        raise RuntimeError("orc_regrxons failed %d" % rc)
+    m = len(Sa) if len(Sa) > 2 else 1
+    return resid, {"score": res.score, "rss": res.rss, "edf": res.edf,
+                   "log_sp": np.array(res.log_sp[:m]), "iterations": res.iterations,
+                   "evaluations": res.evaluations, "step_failed": res.step_failed}
+
+
+def gcv_eval(data, target, S, log_sp, rank_tol=1.4901161193847656e-8):
+    d = np.asfortranarray(data, dtype=np.float64)
+    n, p = d.shape
+    Sa = np.ascontiguousarray(sorted(S), dtype=np.int32)
+    st = OrcSetup()
+    rc = lib().orc_regr_setup(_d(d), n, n, Sa.ctypes.data_as(_I), len(Sa), ctypes.byref(st))
+    if rc != 0:
+        raise RuntimeError("orc_regr_setup failed %d" % rc)
+    try:
+        m = st.m
+        lsp = np.ascontiguousarray(log_sp, dtype=np.float64).reshape(m)
+        y = np.ascontiguousarray(d[:, target])
+        score = np.zeros(1)
+        grad = np.zeros(m)
+        hess = np.zeros(m * m)
+        rc = lib().orc_magic_eval(ctypes.byref(st), _d(y), _d(lsp), rank_tol, _d(score), _d(grad),
+                                  _d(hess))
+        if rc != 0:
+            raise RuntimeError("orc_magic_eval failed %d" % rc)
+        return score[0], grad, hess.reshape(m, m)
+    finally:
+        lib().orc_setup_free(ctypes.byref(st))
+
+
+def ci_test(data, x, y, S, num_col=35):
+    d = np.asfortranarray(data, dtype=np.float64)
+    n, p = d.shape
+    Sa = np.ascontiguousarray(S, dtype=np.int32)
+    opt = OrcSkeletonOptions(0, 0.1, p - 2, num_col, 1.0, 100, 0, 0)
+    out = np.zeros(5)
+    rc = lib().orc_ci_test(_d(d), n, p, int(x), int(y), Sa.ctypes.data_as(_I), len(Sa),
+                           ctypes.byref(opt), _d(out))
+    if rc != 0:
+        raise RuntimeError("orc_ci_test failed %d" % rc)
+    return {"p.value": out[0], "nV2": out[1], "mean": out[2], "variance": out[3]}
+
+
+def skeleton(data, alpha=0.1, m_max=None, num_col=35, threads=8):
+    d = np.asfortranarray(data, dtype=np.float64)
+    n, p = d.shape
+    opt = OrcSkeletonOptions(0, alpha, p - 2 if m_max is None else m_max, num_col, 1.0, 100, 0,
+                             threads)
+    res = OrcSkeletonResult()
+    rc = lib().orc_skeleton(_d(d), n, p, ctypes.byref(opt), ctypes.byref(res))
+    if rc != 0:
+        lib().orc_skeleton_result_free(ctypes.byref(res))
+        raise RuntimeError("orc_skeleton failed %d" % rc)
+    try:
+        L = res.trace_len
+        adj = np.ctypeslib.as_array(res.adjacency, (p * p,)).reshape(p, p).astype(bool).copy()
+        pmax = np.ctypeslib.as_array(res.pmax, (p * p,)).reshape(p, p).copy()
+        slen = np.ctypeslib.as_array(res.sepset_len, (p * p,)).reshape(p, p).copy()
+        sep = np.ctypeslib.as_array(res.sepset, (p * p * 64,)).reshape(p, p, 64)
+        sepsets = [[None if slen[i, j] < 0 else [int(v) + 1 for v in sep[i, j, :slen[i, j]]]
+                    for j in range(p)] for i in range(p)]
+        tS = np.ctypeslib.as_array(res.trace_S, (L * 64,)).reshape(L, 64)
+        ts = np.ctypeslib.as_array(res.trace_slen, (L,))
+        tasks = {
+            "level": np.ctypeslib.as_array(res.trace_level, (L,)).copy(),
+            "x": np.ctypeslib.as_array(res.trace_x, (L,)).copy() + 1,
+            "y": np.ctypeslib.as_array(res.trace_y, (L,)).copy() + 1,
+            "S_key": ["|".join(str(v + 1) for v in tS[i, :ts[i]]) for i in range(L)],
+            "p_used": np.ctypeslib.as_array(res.trace_p, (L,)).copy(),
+            "native_edge_deleted": np.ctypeslib.as_array(res.trace_del, (L,)).astype(bool).copy(),
+        }
+        return {"adjacency": adj, "pMax": pmax, "sepsets": sepsets,
+                "n.edgetests": list(res.n_edgetests)[:res.levels], "tasks": tasks,
+                "levels": res.levels}
+    finally:
+        lib().orc_skeleton_result_free(ctypes.byref(res))
