@@ -1,0 +1,258 @@
+#!/usr/bin/env python3
+"""Benchmark: kernel-PC dcc.gamma skeleton (CI tests/s) on one MI355X.
+
+One step = one full skeleton run (levels, sepsets, deletion trace) of the
+dcc.gamma kernel-PC path over the synthetic nonlinear SEM n=351, p=48
+(edge probability 0.08, per-node noise variance U[0.25, 1], seed 2),
+alpha = 0.1, numCol = 35, index = 1, m.max = Inf.  The data is uploaded
+before the timed region; the skeleton call is synchronous (it returns after
+the last level's p-values are back on the host).
+
+Reported beside the batched form:
+  * plain form -- one launch sequence per CI test, measured on the first
+    --plain-per-level tests of every level and extrapolated with the level's
+    test count;
+  * roofline of the dominant kernel from HIP-event kernel times;
+  * cpu_baseline -- the C oracle (a port of the reference CPU path) timed on a
+    bounded per-level sample of the same workload on this host, 1 thread.
+Prints ONE JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+HBM_PEAK_GBS = 8000.0
+
+
+def design_dims(level):
+    """(q, penalties, rank per penalty) of the regrXonS design at |S| = level."""
+    if level == 1:
+        return 10, 1, 8
+    if level == 2:
+        return 30, 1, 27
+    return 1 + 9 * level, level, 8
+
+
+def gcv_flops(n, level, fits, evals):
+    """Algorithmic FLOPs of the GCV stage: per target Q'y (4nq) and the final fit
+    (2nq), per objective evaluation q^3/3 (pivoted Cholesky of the aggregate
+    penalty) + 4(q+r)q^2 + 8q^3 (SVD of the (q+r) x q augmented system) +
+    q^2 r m (derivatives)."""
+    if level == 0:
+        return 0.0
+    q, m, rk = design_dims(level)
+    r = min(q, m * rk)
+    per_eval = q ** 3 / 3.0 + 4.0 * (q + r) * q * q + 8.0 * q ** 3 + q * q * r * m
+    return fits * (6.0 * n * q) + evals * per_eval
+
+
+def component_flops(n, k, comps):
+    """Algorithmic FLOPs per dcov component: (4/3)n^3 (tridiagonal reduction)
+    + 2n^2 k (eigenvector back-transform) + 3n^2 (distances, centring)."""
+    return comps * (4.0 / 3.0 * n ** 3 + 2.0 * n * n * k + 3.0 * n * n)
+
+
+def pair_flops(n, k, pairs):
+    return pairs * (2.0 * k * k * n + 3.0 * k * k)
+
+
+def read_traffic(kernel):
+    path = os.path.join(ROOT, "profiles", "traffic_%s.json" % kernel)
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        d = json.load(fh)
+    return d.get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(data, res, n, k, seconds_budget):
+    """Time the C oracle on a per-level sample of the workload's residual fits
+    and dcov components; extrapolate with the GPU run's per-level counts."""
+    from oracle import orc
+    s = res["summary"]
+    tasks = res["tasks"]
+    levels = res["levels"]
+    per_level = max(2, int(seconds_budget / max(1, levels) / 0.08 / 2))
+    total = 0.0
+    sampled = []
+    t_start = time.time()
+    for lev in range(levels):
+        idx = np.nonzero(tasks["level"] == lev)[0]
+        if len(idx) == 0:
+            continue
+        take = idx[:per_level]
+        fit_t, comp_t, nf, nc = 0.0, 0.0, 0, 0
+        for i in take:
+            x = int(tasks["x"][i]) - 1
+            S = [int(v) - 1 for v in tasks["S_key"][i].split("|")] if tasks["S_key"][i] else []
+            if S:
+                t0 = time.perf_counter()
+                r, _ = orc.regrxons(data, x, S)
+                fit_t += time.perf_counter() - t0
+                nf += 1
+            else:
+                r = np.ascontiguousarray(data[:, x])
+            t0 = time.perf_counter()
+            orc.dcov_component(r, k)
+            comp_t += time.perf_counter() - t0
+            nc += 1
+            if time.time() - t_start > seconds_budget:
+                break
+        fits = s["level_fits"][lev]
+        comps = s["level_components"][lev]
+        est = (fit_t / nf * fits if nf else 0.0) + comp_t / max(1, nc) * comps
+        total += est
+        sampled.append("L%d:%dfit/%dcomp" % (lev, nf, nc))
+        if time.time() - t_start > seconds_budget:
+            # remaining levels: use this level's per-item cost
+            for lev2 in range(lev + 1, levels):
+                total += (fit_t / max(1, nf)) * s["level_fits"][lev2] + \
+                         (comp_t / max(1, nc)) * s["level_components"][lev2]
+            break
+    tests = int(sum(res["n.edgetests"]))
+    return {
+        "value": tests / total if total > 0 else None,
+        "unit": "CI tests/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": "C oracle (kpcalg/mgcv/pcalg restatement), 1 thread, per-level sample %s of the "
+                  "run's unique residual fits and dcov components, extrapolated with the run's "
+                  "per-level fit/component counts (%.1f s of CPU work; est. skeleton %.1f s)"
+                  % (",".join(sampled), time.time() - t_start, total),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="sem351x48")
+    ap.add_argument("--plain-per-level", type=int, default=30)
+    ap.add_argument("--cpu-baseline-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures the single-GPU path only: this build has no multi-GPU execution "
+              "(one MI355X per box); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import fastkpc_amd as fk
+    from fastkpc_amd.sem import config_data
+
+    data, mask, cfg = config_data(args.config)
+    n, p = data.shape
+    k = 35
+    opts = {"numCol": k, "ci_method": "dcc.gamma", "index": 1}
+    for _ in range(args.warmup):
+        fk.fastkpc_compatible_cuda_skeleton(data, 0.1, options=opts)
+    walls = []
+    res = None
+    stage_tot = {}
+    launch_tot = {}
+    level_evals = None
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        res = fk.fastkpc_compatible_cuda_skeleton(data, 0.1, options=opts)
+        walls.append(time.perf_counter() - t0)
+        s = res["summary"]
+        for key, v in s["stage_seconds"].items():
+            stage_tot[key] = stage_tot.get(key, 0.0) + v
+        for key, v in s["stage_launches"].items():
+            launch_tot[key] = launch_tot.get(key, 0) + v
+    s = res["summary"]
+    tests = int(sum(res["n.edgetests"]))
+    wall = float(np.mean(walls))
+    value = tests / wall
+
+    # ---- roofline of the dominant kernel (HIP events inside the library)
+    kernels = {
+        "gcv": ("magic_kernel", sum(gcv_flops(n, l, s["level_fits"][l], s["level_evaluations"][l])
+                                    for l in range(res["levels"]))),
+        "dcov_component": ("dcov_lanczos_kernel",
+                           component_flops(n, k, sum(s["level_components"]))),
+        "dcov_pair": ("dcov_pair_kernel", pair_flops(n, k, sum(s["level_physical"]))),
+    }
+    dom = max(kernels, key=lambda key: stage_tot.get(key, 0.0))
+    kname, flops_per_run = kernels[dom]
+    ktime = stage_tot[dom] / args.steps
+    launches = launch_tot[dom] / args.steps
+    achieved = flops_per_run / ktime / 1e12
+    roofline = {
+        "bound": "mfma",
+        "kernel": kname,
+        "achieved": round(achieved, 4),
+        "peak": FP64_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / FP64_PEAK_TFLOPS, 6),
+        "traffic": read_traffic(kname),
+        "launches_per_step": launches,
+        "avg_launch_us": round(ktime / max(1, launches) * 1e6, 1),
+        "algorithmic_flops_per_launch": flops_per_run / max(1, launches),
+    }
+
+    # ---- plain form (one launch sequence per test), stratified per level
+    plain = None
+    if not args.no_plain:
+        popts = dict(opts, mode="plain", plain_test_limit=args.plain_per_level)
+        rp = fk.fastkpc_compatible_cuda_skeleton(data, 0.1, options=popts)
+        sp = rp["summary"]
+        est = 0.0
+        for lev in range(rp["levels"]):
+            if sp["plain_tests"][lev] > 0:
+                est += rp["n.edgetests"][lev] * sp["plain_seconds"][lev] / sp["plain_tests"][lev]
+        same = bool(np.array_equal(rp["adjacency"], res["adjacency"]) and
+                    rp["n.edgetests"] == res["n.edgetests"])
+        plain = {
+            "tests_per_s": round(tests / est, 2) if est > 0 else None,
+            "skeleton_wall_s_est": round(est, 3),
+            "sampled_tests": int(sum(sp["plain_tests"])),
+            "same_skeleton": same,
+            "batched_over_plain": round(value / (tests / est), 2) if est > 0 else None,
+        }
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(data, res, n, k, args.cpu_baseline_seconds)
+
+    line = {
+        "metric": "CI tests/s + skeleton wall-s, 351x48 dcc.gamma: batched+adjoint vs plain GPU",
+        "value": round(value, 2),
+        "unit": "CI tests/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic nonlinear SEM (generator fastkpc_amd/sem.py, seed %d)" % cfg["seed"],
+        "config": {
+            "workload": "kernel-PC skeleton, dcc.gamma, sem351x48: n=351 p=48 edge_prob=0.08 "
+                        "noise var U[0.25,1], alpha=0.1 numCol=35 index=1 m.max=Inf",
+            "n": n, "p": p, "ci_method": "dcc.gamma", "alpha": 0.1, "numCol": k,
+            "m_max_resolved": p - 2, "form": "batched (frontier-speculative level waves)",
+            "logical_tests": tests, "n_edgetests": res["n.edgetests"],
+            "physical_tests": int(s["physical_tests"]), "levels": res["levels"],
+            "skeleton_wall_s": round(wall, 4),
+            "stage_seconds_per_step": {kk: round(v / args.steps, 4) for kk, v in stage_tot.items()},
+        },
+        "plain_gpu": plain,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

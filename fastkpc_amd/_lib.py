@@ -56,6 +56,16 @@ class KpcResult(ctypes.Structure):
         ("setups", ctypes.c_longlong),
         ("magic_evaluations", ctypes.c_longlong),
         ("truncated", ctypes.c_int),
+        ("level_tests", ctypes.c_longlong * 64),
+        ("level_physical", ctypes.c_longlong * 64),
+        ("level_fits", ctypes.c_longlong * 64),
+        ("level_components", ctypes.c_longlong * 64),
+        ("level_evaluations", ctypes.c_longlong * 64),
+        ("level_lanczos_steps", ctypes.c_longlong * 64),
+        ("level_seconds", ctypes.c_double * 64),
+        ("plain_tests", ctypes.c_longlong * 64),
+        ("plain_seconds", ctypes.c_double * 64),
+        ("stage_launches", ctypes.c_longlong * 8),
     ]
 
 

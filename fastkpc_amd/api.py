@@ -123,7 +123,17 @@ def fastkpc_compatible_cuda_skeleton(data, alpha, labels=None, options=None):
             "components": r.components,
             "setups": r.setups,
             "magic_evaluations": r.magic_evaluations,
-            "truncated": bool(r.truncated),
+            "level_tests": list(r.level_tests)[:levels],
+            "level_physical": list(r.level_physical)[:levels],
+            "level_fits": list(r.level_fits)[:levels],
+            "level_components": list(r.level_components)[:levels],
+            "level_evaluations": list(r.level_evaluations)[:levels],
+            "level_lanczos_steps": list(r.level_lanczos_steps)[:levels],
+            "level_seconds": list(r.level_seconds)[:levels],
+            "plain_tests": list(r.plain_tests)[:levels],
+            "plain_seconds": list(r.plain_seconds)[:levels],
+            "stage_launches": dict(zip(["smooth_setup", "geometry", "gcv", "dcov_component",
+                                        "dcov_pair"], list(r.stage_launches)[:5])),
             "labels": list(labels),
         }
         return {

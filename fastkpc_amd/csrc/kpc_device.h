@@ -72,3 +72,14 @@ hipError_t launch_gcv_eval(const MagicItem* items, int count, int n, int q, int 
                            double* out, double* scratch, int grid, double rank_tol,
                            hipStream_t stream);
 }  // namespace kpc
+
+namespace kpc {
+size_t dcov_lanczos_lds_bytes(int n, int mmax);
+long long dcov_lanczos_scratch_doubles(int n, int k, int mmax);
+hipError_t launch_dcov_lanczos(const double* src_base, const long long* src_off,
+                               const int* dst_slot, int count, int n, int k, double* vals,
+                               double* vecs, double* dsum, double* moment, int* status,
+                               double* scratch, int grid, int mmax, double tol,
+                               hipStream_t stream);
+int lanczos_mmax(int n);
+}  // namespace kpc

@@ -24,6 +24,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -168,6 +169,7 @@ class Engine {
     comp_vecs_.reserve(max_resid * (size_t)n_ * k_);
     comp_dsum_.reserve(max_resid);
     comp_mom_.reserve(max_resid);
+    comp_status_.reserve(max_resid);
   }
 
   // Evaluate p-values for the given tasks (all with |S| == level_).
@@ -261,16 +263,7 @@ class Engine {
                 "magic launch");
     }
     hip_check(hipEventRecord(ev_[3], stream_), "event");
-    if (!comp_dst.empty()) {
-      const long long stride = kpc::dcov_component_scratch_doubles(n_, k_);
-      int grid = std::min((int)comp_dst.size(), grid_limit(stride, 2048));
-      scratch4_.reserve((size_t)grid * stride);
-      hip_check(kpc::launch_dcov_components(level_ == 0 ? d_data_ : resid_.ptr, d_comp_src_.ptr,
-                                            d_comp_dst_.ptr, (int)comp_dst.size(), n_, k_,
-                                            comp_vals_.ptr, comp_vecs_.ptr, comp_dsum_.ptr,
-                                            comp_mom_.ptr, scratch4_.ptr, grid, stream_),
-                "dcov component launch");
-    }
+    if (!comp_dst.empty()) launch_components(level_ == 0 ? d_data_ : resid_.ptr, comp_dst.size());
     hip_check(hipEventRecord(ev_[4], stream_), "event");
     {
       int grid = std::min((int)tasks.size(), 65535);
@@ -286,11 +279,15 @@ class Engine {
               "D2H p-values");
     hip_check(hipStreamSynchronize(stream_), "wave synchronize");
     float ms;
+    const bool launched[5] = {nsm > 0, !geo_items.empty(), !fit_items.empty(), !comp_dst.empty(),
+                              true};
     for (int s = 0; s < 5; ++s) {
       hip_check(hipEventElapsedTime(&ms, ev_[s], ev_[s + 1]), "event elapsed");
       stage_[s] += ms * 1e-3;
+      if (launched[s]) ++launches_[s];
     }
     check_device_status(fit_items, geo_items, nsm);
+    check_component_status(comp_dst);
     for (size_t i = 0; i < tasks.size(); ++i) {
       Task* t = tasks[i];
       t->p = out[4 * i + 0];
@@ -302,13 +299,37 @@ class Engine {
     }
   }
 
+  void launch_components(const double* src, size_t count) {
+    const int mmax = kpc::lanczos_mmax(n_);
+    if (dense_components_) {
+      const long long stride = kpc::dcov_component_scratch_doubles(n_, k_);
+      int grid = std::min((int)count, grid_limit(stride, 2048));
+      scratch4_.reserve((size_t)grid * stride);
+      hip_check(kpc::launch_dcov_components(src, d_comp_src_.ptr, d_comp_dst_.ptr, (int)count, n_,
+                                            k_, comp_vals_.ptr, comp_vecs_.ptr, comp_dsum_.ptr,
+                                            comp_mom_.ptr, scratch4_.ptr, grid, stream_),
+                "dcov component launch");
+    } else {
+      const long long stride = kpc::dcov_lanczos_scratch_doubles(n_, k_, mmax);
+      int grid = std::min((int)count, grid_limit(stride, 4096));
+      scratch4_.reserve((size_t)grid * stride);
+      hip_check(kpc::launch_dcov_lanczos(src, d_comp_src_.ptr, d_comp_dst_.ptr, (int)count, n_, k_,
+                                         comp_vals_.ptr, comp_vecs_.ptr, comp_dsum_.ptr,
+                                         comp_mom_.ptr, comp_status_.ptr, scratch4_.ptr, grid, mmax,
+                                         1e-13, stream_),
+                "dcov lanczos launch");
+    }
+  }
+
   int level() const { return level_; }
   int n() const { return n_; }
   int p() const { return p_; }
   double stage(int i) const { return stage_[i]; }
+  long long launches(int i) const { return launches_[i]; }
   double stage_host() const { return stage_host_; }
   struct Stats {
-    long long physical_tests = 0, fits = 0, components = 0, setups = 0, evals = 0;
+    long long physical_tests = 0, fits = 0, components = 0, setups = 0, evals = 0,
+              lanczos_steps = 0;
   } stats_;
 
   // direct access for the regrXonS / gcv entry points
@@ -434,6 +455,17 @@ class Engine {
     }
   }
 
+  void check_component_status(const std::vector<int>& dst) {
+    if (dense_components_ || dst.empty()) return;
+    std::vector<int> h((size_t)n_comp_);
+    hip_check(hipMemcpy(h.data(), comp_status_.ptr, sizeof(int) * h.size(), hipMemcpyDeviceToHost),
+              "component status");
+    for (int s : dst) {
+      if (h[s] <= 0) throw KpcError(-7, "dcov component eigensolver did not converge");
+      stats_.lanczos_steps += h[s];
+    }
+  }
+
   int n_, p_, k_;
   kpc_options opt_;
   hipStream_t stream_ = nullptr;
@@ -450,6 +482,8 @@ class Engine {
   DevArray<int> lsmooth_info_, usmooth_info_;
   DevArray<double> design_, resid_, rinfo_;
   DevArray<double> comp_vals_, comp_vecs_, comp_dsum_, comp_mom_;
+  DevArray<int> comp_status_;
+  bool dense_components_ = std::getenv("KPC_DCOV_DENSE") != nullptr;
   DevArray<double> scratch_, scratch2_, scratch3_, scratch4_;
   DevArray<int> d_smooth_items_;
   DevArray<kpc::GeometryItem> d_geo_items_;
@@ -458,6 +492,7 @@ class Engine {
   DevArray<int> d_comp_dst_, d_pair_x_, d_pair_y_;
   DevArray<double> d_pair_out_;
   double stage_[5] = {0, 0, 0, 0, 0};
+  long long launches_[5] = {0, 0, 0, 0, 0};
   double stage_host_ = 0.0;
   bool pending_smooth_checks_ = false;
 };
@@ -545,6 +580,7 @@ void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
   int ord = 0;
   const bool plain = opt.mode == KPC_PLAIN;
   long long plain_tests = 0;
+  (void)plain_tests;
   while (!done && ord <= mmax) {
     bool any = false;
     for (int i = 0; i < p * p; ++i) any |= r->adjacency[i] != 0;
@@ -587,6 +623,14 @@ void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
         }
         edges.push_back(er);
       }
+    const double level_t0 = now_s();
+    // Levels whose whole plan needs at most this many residual fits are
+    // evaluated speculatively in one wave (fewer round trips beat the wasted
+    // fits); larger levels run frontier waves with doubling chunks.
+    size_t level_unique_resid = 0;
+    size_t speculate_limit = 16384;
+    if (const char* env = std::getenv("KPC_SPECULATE_LIMIT")) speculate_limit = std::atol(env);
+    const Engine::Stats st0 = eng.stats_;
     eng.begin_level(ord);
     {
       std::unordered_map<uint64_t, int> ud;
@@ -596,8 +640,8 @@ void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
         ur.emplace(std::make_pair(t.x, t.mask), 0);
         ur.emplace(std::make_pair(t.y, t.mask), 0);
       }
-      if (plain) eng.reserve_level(1, 2);
-      else eng.reserve_level(ud.size(), ur.size());
+      eng.reserve_level(ud.size(), ur.size());
+      level_unique_resid = ur.size();
     }
     // ---- frontier waves
     for (EdgeRun& e : edges) {
@@ -610,35 +654,55 @@ void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
         if (e.o[1].count == 0) e.state = 2;
       }
     }
-    int chunk = 1;
+    // frontier chunk: tasks taken per undecided orientation in the first
+    // wave, doubled every wave (KPC_WAVE_CHUNK overrides; <= 0 evaluates the
+    // whole level speculatively in one wave)
+    int chunk = level_unique_resid <= speculate_limit ? 0 : 4;
+    if (const char* env = std::getenv("KPC_WAVE_CHUNK")) chunk = std::atoi(env);
+    if (chunk <= 0) chunk = 1 << 30;
+    const int batched_chunk = chunk;
+    bool plain_now = plain;
+    if (plain_now) chunk = 1;
+    bool speculate_all = chunk >= (1 << 30);
+    long long level_plain = 0;
     bool active = true;
     std::vector<Task*> batch;
-    bool truncated = false;
     while (active) {
       batch.clear();
       for (EdgeRun& e : edges) {
         if (e.state != 0) continue;
-        const Orientation& o = e.o[e.side];
-        int taken = 0;
-        for (int c = e.cursor; c < o.count && taken < chunk; ++c) {
-          Task& t = tasks[(size_t)o.first + c];
-          if (!t.computed) {
-            batch.push_back(&t);
-            ++taken;
+        for (int side = e.side; side < 2; ++side) {
+          const Orientation& o = e.o[side];
+          int taken = 0;
+          for (int c = side == e.side ? e.cursor : 0; c < o.count && taken < chunk; ++c) {
+            Task& t = tasks[(size_t)o.first + c];
+            if (!t.computed) {
+              batch.push_back(&t);
+              ++taken;
+            }
           }
+          if (!speculate_all) break;
         }
-        if (plain && !batch.empty()) break;
+        if (plain_now && !batch.empty()) break;
       }
-      if (plain) {
-        if (opt.plain_test_limit > 0 && plain_tests >= opt.plain_test_limit) {
-          truncated = true;
-          break;
-        }
-        eng.begin_level(ord);  // fresh caches: one launch sequence per test
-        eng.reserve_level(1, 2);
+      if (plain_now) {
+        // one launch sequence per test: fresh caches for every test
+        eng.begin_level(ord);
+        const double tp0 = now_s();
+        eng.evaluate(batch);
+        r->plain_seconds[ord] += now_s() - tp0;
+        r->plain_tests[ord] += (long long)batch.size();
+        ++level_plain;
         plain_tests += (long long)batch.size();
+        if (opt.plain_test_limit > 0 && level_plain >= opt.plain_test_limit) {
+          plain_now = false;  // rest of the level runs batched
+          chunk = batched_chunk;
+          speculate_all = chunk >= (1 << 30);
+          eng.begin_level(ord);
+        }
+      } else {
+        eng.evaluate(batch);
       }
-      eng.evaluate(batch);
       // replay in canonical order
       active = false;
       for (EdgeRun& e : edges) {
@@ -684,11 +748,14 @@ void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
         }
     for (const Task& t : tasks)
       if (t.consumed) push_trace(r, cap, t, ord);
+    r->level_tests[ord] = r->n_edgetests[ord];
+    r->level_physical[ord] = eng.stats_.physical_tests - st0.physical_tests;
+    r->level_fits[ord] = eng.stats_.fits - st0.fits;
+    r->level_components[ord] = eng.stats_.components - st0.components;
+    r->level_evaluations[ord] = eng.stats_.evals - st0.evals;
+    r->level_lanczos_steps[ord] = eng.stats_.lanczos_steps - st0.lanczos_steps;
+    r->level_seconds[ord] = now_s() - level_t0;
     ++ord;
-    if (truncated) {
-      r->truncated = 1;
-      break;
-    }
   }
   r->levels = ord;
   for (int i = 0; i < p - 1; ++i)
@@ -729,7 +796,10 @@ int kpc_skeleton_run(const double* data, int n, int p, const kpc_options* opt, k
       throw;
     }
     r->wall_seconds = now_s() - t0;
-    for (int s = 0; s < 5; ++s) r->stage_seconds[s] = eng.stage(s);
+    for (int s = 0; s < 5; ++s) {
+      r->stage_seconds[s] = eng.stage(s);
+      r->stage_launches[s] = eng.launches(s);
+    }
     r->stage_seconds[5] = eng.stage_host();
     r->physical_tests = eng.stats_.physical_tests;
     r->residual_fits = eng.stats_.fits;
@@ -938,12 +1008,25 @@ int kpc_dcov_components(const double* x, int n, int count, int k, double* values
     d_vecs.reserve((size_t)count * n * k);
     d_dsum.reserve(count);
     d_mom.reserve(count);
-    const long long stride = kpc::dcov_component_scratch_doubles(n, k);
-    const int grid = std::min(count, 1024);
-    d_scr.reserve((size_t)grid * stride);
-    hip_check(kpc::launch_dcov_components(d_x.ptr, d_src.ptr, d_dst.ptr, count, n, k, d_vals.ptr,
-                                          d_vecs.ptr, d_dsum.ptr, d_mom.ptr, d_scr.ptr, grid, s),
-              "component launch");
+    DevArray<int> d_st;
+    d_st.reserve(count);
+    if (std::getenv("KPC_DCOV_DENSE") != nullptr) {
+      const long long stride = kpc::dcov_component_scratch_doubles(n, k);
+      const int grid = std::min(count, 1024);
+      d_scr.reserve((size_t)grid * stride);
+      hip_check(kpc::launch_dcov_components(d_x.ptr, d_src.ptr, d_dst.ptr, count, n, k, d_vals.ptr,
+                                            d_vecs.ptr, d_dsum.ptr, d_mom.ptr, d_scr.ptr, grid, s),
+                "component launch");
+    } else {
+      const int mmax = kpc::lanczos_mmax(n);
+      const long long stride = kpc::dcov_lanczos_scratch_doubles(n, k, mmax);
+      const int grid = std::min(count, 1024);
+      d_scr.reserve((size_t)grid * stride);
+      hip_check(kpc::launch_dcov_lanczos(d_x.ptr, d_src.ptr, d_dst.ptr, count, n, k, d_vals.ptr,
+                                         d_vecs.ptr, d_dsum.ptr, d_mom.ptr, d_st.ptr, d_scr.ptr,
+                                         grid, mmax, 1e-13, s),
+                "lanczos launch");
+    }
     hip_check(hipStreamSynchronize(s), "component synchronize");
     if (values) hip_check(hipMemcpy(values, d_vals.ptr, sizeof(double) * count * k, hipMemcpyDeviceToHost), "D2H");
     if (vectors) hip_check(hipMemcpy(vectors, d_vecs.ptr, sizeof(double) * (size_t)count * n * k, hipMemcpyDeviceToHost), "D2H");

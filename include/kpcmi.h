@@ -47,7 +47,7 @@ typedef struct kpc_options {
   int has_seed;
   unsigned long long seed;
   int mode;                  /* kpc_exec_mode */
-  int plain_test_limit;      /* KPC_PLAIN only: stop after this many tests (0 = all) */
+  int plain_test_limit;      /* KPC_PLAIN only: per-level cap on plain tests (0 = all) */
 } kpc_options;
 
 typedef struct kpc_result {
@@ -73,7 +73,20 @@ typedef struct kpc_result {
   long long components;
   long long setups;
   long long magic_evaluations;
-  int truncated;            /* KPC_PLAIN stopped at plain_test_limit */
+  int truncated;            /* unused (kept for layout stability) */
+  /* per-level counters (level = |S|) */
+  long long level_tests[64];        /* consumed tests (== n_edgetests) */
+  long long level_physical[64];     /* evaluated tests incl. speculation */
+  long long level_fits[64];         /* GCV fits (unique (target, S)) */
+  long long level_components[64];   /* dcov components */
+  long long level_evaluations[64];  /* GCV objective evaluations */
+  long long level_lanczos_steps[64];/* Lanczos steps summed over components */
+  double level_seconds[64];         /* wall time of the level */
+  /* KPC_PLAIN sampling: first plain_test_limit tests of each level run one
+   * launch sequence per test (timed here); the rest run batched */
+  long long plain_tests[64];
+  double plain_seconds[64];
+  long long stage_launches[8];      /* kernel launches per stage (as stage_seconds) */
 } kpc_result;
 
 void kpc_default_options(kpc_options* opt);

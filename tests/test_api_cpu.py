@@ -1,0 +1,90 @@
+"""CPU-side checks of the product library: it loads without a GPU, exports
+every entry point the C header declares, and validates arguments before it
+touches a device.  Also the host-side API helpers and the SEM generator."""
+import math
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "kpcmi.h")).read()
+    return sorted(set(re.findall(r"\b(kpc_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib_path = os.path.join(ROOT, "fastkpc_amd", "libkpcmi.so")
+    assert os.path.exists(lib_path), "build the library first (python -m fastkpc_amd.build)"
+    out = subprocess.run(["nm", "-D", "--defined-only", lib_path], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r" T (kpc_\w+)", out))
+    declared = header_functions()
+    assert declared, "no declarations parsed"
+    missing = [f for f in declared if f not in exported]
+    assert not missing, missing
+    from fastkpc_amd import _lib
+    L = _lib.lib()
+    for f in declared:
+        assert hasattr(L, f)
+
+
+def test_argument_validation_needs_no_gpu():
+    from fastkpc_amd import _lib
+    from fastkpc_amd._lib import KpcOptions
+    import ctypes
+    L = _lib.lib()
+    o = KpcOptions()
+    L.kpc_default_options(o)
+    assert o.alpha == 0.1 and o.num_col == 35 and o.max_conditioning_size == -1
+    data = np.asfortranarray(np.random.default_rng(0).standard_normal((30, 4)))
+    res = ctypes.POINTER(_lib.KpcResult)()
+    o.num_col = 40  # >= n
+    rc = L.kpc_skeleton_run(_lib.dptr(data), 30, 4, o, ctypes.byref(res))
+    assert rc == -1 and b"num_col" in L.kpc_last_error()
+    o.num_col = 5
+    bad = data.copy()
+    bad[3, 1] = np.nan
+    rc = L.kpc_skeleton_run(_lib.dptr(np.asfortranarray(bad)), 30, 4, o, ctypes.byref(res))
+    assert rc == -1 and b"finite" in L.kpc_last_error()
+    o.ci_method = 7
+    rc = L.kpc_skeleton_run(_lib.dptr(data), 30, 4, o, ctypes.byref(res))
+    assert rc < 0
+
+
+def test_resolve_max_conditioning_size():
+    from fastkpc_amd import resolve_max_conditioning_size as r
+    assert r(float("inf"), 48) == 46
+    assert r(7, 48) == 7
+    assert r(7.9, 48) == 7
+    assert r(100, 10) == 8
+    with pytest.raises(ValueError):
+        r(-1, 10)
+    with pytest.raises(ValueError):
+        r(float("nan"), 10)
+
+
+def test_option_whitelist_and_labels():
+    import fastkpc_amd as fk
+    data = np.zeros((20, 3))
+    with pytest.raises(ValueError, match="unknown compatible CUDA option"):
+        fk.fastkpc_compatible_cuda_skeleton(data, 0.1, options={"bogus": 1})
+    with pytest.raises(ValueError, match="labels length"):
+        fk.fastkpc_compatible_cuda_skeleton(data, 0.1, labels=["a"])
+
+
+def test_sem_generator_is_deterministic_and_acyclic():
+    from fastkpc_amd.sem import config_data, generate
+    d1, m1, c = config_data("sem351x48")
+    d2, m2, _ = config_data("sem351x48")
+    assert d1.shape == (351, 48) and np.array_equal(d1, d2) and np.array_equal(m1, m2)
+    assert not np.any(np.tril(m1))  # parents precede children
+    roots = np.nonzero(~m1.any(axis=0))[0]
+    for j in roots:
+        assert abs(d1[:, j].std() - 1.0) < 0.15
+    d3, _ = generate(n=50, p=6, edge_prob=0.5, seed=11, link="mixed")
+    assert np.isfinite(d3).all()

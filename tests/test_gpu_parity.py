@@ -1,0 +1,207 @@
+"""HIP path vs the oracle (and the reference fixtures) on MI355X.
+
+Tolerances: skeleton adjacency, sepsets, n.edgetests and deletion trace
+exact; p-values within 1e-9 relative of the oracle (absolute 1e-12 below
+1e-3, where 1 - pgamma cancels); residuals within 1e-8 relative L2; GCV
+scores within 1e-9 relative.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, trace_map
+
+pytestmark = pytest.mark.gpu
+
+
+def fk():
+    import fastkpc_amd
+    from fastkpc_amd import _lib
+    assert _lib.lib().kpc_device_count() >= 1, "no GPU visible"
+    return fastkpc_amd
+
+
+def orc():
+    from oracle import orc as o
+    return o
+
+
+def p_close(a, b):
+    return abs(a - b) <= 1e-9 * abs(b) + 1e-12
+
+
+def test_dcov_gamma_reference_fixture(dcov_fixture):
+    f = fk()
+    for c in dcov_fixture:
+        data = np.column_stack([c["rx"], c["ry"]])
+        out = f.ci_batch(data, [0], [1], np.zeros((1, 0), np.int32))
+        assert p_close(out[0, 0], c["p_value"]), (c["case_id"], out[0, 0], c["p_value"])
+        assert abs(out[0, 1] - c["nV2"]) <= 1e-9 * c["nV2"]
+        assert abs(out[0, 2] - c["nV2Mean"]) <= 1e-12 * c["nV2Mean"]
+        assert abs(out[0, 3] - c["nV2Variance"]) <= 1e-9 * c["nV2Variance"]
+
+
+def test_dcov_components_vs_oracle(dcov_fixture):
+    f = fk()
+    o = orc()
+    X = np.column_stack([c["rx"] for c in dcov_fixture])
+    vals, vecs, dsum, mom = f.dcov_components(X, 35)
+    for i in range(X.shape[1]):
+        ov, oc, osum, omom = o.dcov_component(X[:, i], 35)
+        scale = abs(ov[0])
+        np.testing.assert_allclose(vals[i], ov, rtol=0, atol=1e-13 * scale)
+        assert abs(dsum[i] - osum) <= 1e-14 * osum
+        assert abs(mom[i] - omom) <= 1e-10 * abs(omom)
+        # vectors carry the eigenvalue weight in every statistic; compare the
+        # weighted, sign-invariant projectors of the leading pairs
+        for c in range(10):
+            s = np.sign(vecs[i][:, c] @ oc[:, c])
+            np.testing.assert_allclose(s * vecs[i][:, c], oc[:, c], atol=1e-9)
+
+
+def test_dcov_translation_scale_and_swap_invariance():
+    f = fk()
+    rng = np.random.default_rng(21)
+    x = rng.standard_normal(300)
+    y = np.tanh(2 * x) + 0.7 * rng.standard_normal(300)
+    data = np.column_stack([x, y, 4.0 * x - 2.0, -y, y])
+    out = f.ci_batch(data, [0, 2, 1, 0, 3], [1, 1, 0, 3, 2], np.zeros((5, 0), np.int32))
+    base = out[0, 0]
+    for v in out[:, 0]:
+        assert abs(v - base) <= 1e-10
+
+
+def test_dcov_with_tied_values_matches_oracle():
+    f = fk()
+    o = orc()
+    rng = np.random.default_rng(4)
+    x = np.round(rng.standard_normal(200), 1)  # many ties -> repeated eigenvalues
+    y = np.round(x + rng.standard_normal(200), 1)
+    out = f.ci_batch(np.column_stack([x, y]), [0], [1], np.zeros((1, 0), np.int32))
+    ref = o.dcov_gamma(x, y, 20 if False else 35)
+    assert p_close(out[0, 0], ref["p.value"]), (out[0, 0], ref["p.value"])
+
+
+@pytest.mark.parametrize("S", [[2], [3, 7], [1, 5, 9], [4, 8, 12, 16]])
+def test_regrxons_vs_oracle(dev500, S):
+    f = fk()
+    o = orc()
+    data = dev500["data"]
+    targets = [0, 10, 20, 33]
+    resid, info = f.regrxons_batch(data, targets, [S] * len(targets))
+    for j, t in enumerate(targets):
+        ro, oi = o.regrxons(data, t, S)
+        assert info[j, 0] == 0.0
+        rel = np.linalg.norm(resid[j] - ro) / np.linalg.norm(ro)
+        assert rel <= 1e-8, (t, rel)
+        assert abs(info[j, 1] - oi["score"]) <= 1e-9 * oi["score"]
+        m = len(oi["log_sp"])
+        np.testing.assert_allclose(info[j, 8:8 + m], oi["log_sp"], atol=1e-6)
+
+
+@pytest.mark.parametrize("S", [[5], [2, 9], [1, 3, 6]])
+def test_gcv_gradient_hessian_vs_oracle_and_finite_differences(dev500, S):
+    f = fk()
+    o = orc()
+    data = dev500["data"]
+    m = 1 if len(S) <= 2 else len(S)
+    lsp = np.linspace(-1.0, 1.5, m)
+    score, grad, hess = f.gcv_grad_hess(data, [7], [S], lsp[None, :])
+    os_, og, oh = o.gcv_eval(data, 7, S, lsp)
+    assert abs(score[0] - os_) <= 1e-10 * os_
+    np.testing.assert_allclose(grad[0], og, rtol=1e-7, atol=1e-10 * os_)
+    np.testing.assert_allclose(hess[0], oh, rtol=1e-6, atol=1e-9 * os_)
+    h = 1e-5
+    pts = []
+    for i in range(m):
+        e = np.zeros(m)
+        e[i] = h
+        pts += [lsp + e, lsp - e]
+    sc, gr, _ = f.gcv_grad_hess(data, [7] * len(pts), [S] * len(pts), np.array(pts))
+    for i in range(m):
+        fd = (sc[2 * i] - sc[2 * i + 1]) / (2 * h)
+        assert abs(fd - grad[0, i]) <= 1e-6 * abs(grad[0, i]) + 1e-9 * score[0]
+        fdh = (gr[2 * i] - gr[2 * i + 1]) / (2 * h)
+        np.testing.assert_allclose(fdh, hess[0, i], rtol=1e-4, atol=1e-6 * (np.abs(hess).max() + score[0]))
+
+
+def compare_skeleton(res, gold, tol_p):
+    assert res["n.edgetests"] == [int(v) for v in gold["n_edgetests"]]
+    assert np.array_equal(res["adjacency"], gold["adjacency"])
+    t = res["tasks"]
+    mine = trace_map(t["level"], t["x"], t["y"], t["S_key"], t["p_used"])
+    ref = trace_map(gold["level"], gold["x"], gold["y"], gold["S"], gold["p"])
+    assert set(mine) == set(ref)
+    bad = [(k, mine[k], ref[k]) for k in ref if not tol_p(mine[k], ref[k])]
+    assert not bad, bad[:5]
+    dels = [(int(a), int(b), int(c), str(d)) for a, b, c, d, e in
+            zip(t["level"], t["x"], t["y"], t["S_key"], t["native_edge_deleted"]) if e]
+    gdels = [(int(a), int(b), int(c), str(d)) for a, b, c, d, e in
+             zip(gold["level"], gold["x"], gold["y"], gold["S"], gold["deleted"]) if e]
+    assert dels == gdels
+
+
+def test_skeleton_500x50_vs_reference_fixture_and_oracle(dev500):
+    f = fk()
+    d = dev500
+    res = f.fastkpc_compatible_cuda_skeleton(d["data"], 0.1, options={"numCol": 35})
+    assert res["n.edgetests"] == [int(v) for v in d["n_edgetests"]]
+    assert np.array_equal(res["adjacency"], d["adjacency"])
+    for row in d["sepsets"]:
+        i, j, ln = row[0], row[1], row[2]
+        if d["adjacency"][i, j] or i == j:
+            continue
+        assert res["sepsets"][i][j] == [int(v) for v in row[3:3 + ln]]
+    t = res["tasks"]
+    mine = trace_map(t["level"], t["x"], t["y"], t["S_key"], t["p_used"])
+    ref = trace_map(d["lt_level"], d["lt_x"], d["lt_y"], d["lt_S"], d["lt_p"])
+    assert set(mine) == set(ref)
+    assert max(abs(mine[k] - ref[k]) for k in ref) <= 1e-8
+    path = os.path.join(GOLDEN, "oracle_dev500x50.npz")
+    if os.path.exists(path):
+        compare_skeleton(res, np.load(path), p_close)
+
+
+def test_skeleton_sem351x48_vs_oracle():
+    f = fk()
+    from fastkpc_amd.sem import config_data
+    data, _, _ = config_data("sem351x48")
+    gold = np.load(os.path.join(GOLDEN, "oracle_sem351x48.npz"))
+    res = f.fastkpc_compatible_cuda_skeleton(data, 0.1, options={"numCol": 35})
+    compare_skeleton(res, gold, p_close)
+    # pMax: max p-value per edge over the consumed tests
+    np.testing.assert_allclose(res["pMax"][np.isfinite(gold["pmax"])],
+                               gold["pmax"][np.isfinite(gold["pmax"])], rtol=1e-9, atol=1e-12)
+
+
+def test_plain_form_equals_batched_form():
+    f = fk()
+    from fastkpc_amd.sem import generate
+    data, _ = generate(n=150, p=10, edge_prob=0.35, seed=12)
+    rb = f.fastkpc_compatible_cuda_skeleton(data, 0.1, options={"numCol": 15})
+    rp = f.fastkpc_compatible_cuda_skeleton(data, 0.1, options={"numCol": 15, "mode": "plain"})
+    assert rb["n.edgetests"] == rp["n.edgetests"]
+    assert np.array_equal(rb["adjacency"], rp["adjacency"])
+    assert rb["tasks"]["S_key"] == rp["tasks"]["S_key"]
+    np.testing.assert_allclose(rb["tasks"]["p_used"], rp["tasks"]["p_used"], rtol=1e-12, atol=1e-15)
+    assert sum(rp["summary"]["plain_tests"]) == sum(rp["n.edgetests"])
+
+
+def test_ci_batch_vs_oracle_random_sets():
+    f = fk()
+    o = orc()
+    from fastkpc_amd.sem import config_data
+    data, _, _ = config_data("sem351x48")
+    rng = np.random.default_rng(17)
+    for s_len in (0, 1, 2, 3, 5):
+        xs, ys, Ss = [], [], []
+        for _ in range(3):
+            perm = rng.permutation(48)
+            xs.append(int(perm[0]))
+            ys.append(int(perm[1]))
+            Ss.append(sorted(int(v) for v in perm[2:2 + s_len]))
+        out = f.ci_batch(data, xs, ys, np.array(Ss, dtype=np.int32).reshape(3, s_len))
+        for i in range(3):
+            ref = o.ci_test(data, xs[i], ys[i], Ss[i])
+            assert p_close(out[i, 0], ref["p.value"]), (s_len, xs[i], ys[i], Ss[i], out[i, 0], ref)
