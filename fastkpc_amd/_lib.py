@@ -66,6 +66,7 @@ class KpcResult(ctypes.Structure):
         ("plain_tests", ctypes.c_longlong * 64),
         ("plain_seconds", ctypes.c_double * 64),
         ("stage_launches", ctypes.c_longlong * 8),
+        ("task_margin", _D),
     ]
 
 

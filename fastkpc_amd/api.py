@@ -103,11 +103,12 @@ def fastkpc_compatible_cuda_skeleton(data, alpha, labels=None, options=None):
                 "conditioning_size": ts,
                 "p_used": np.ctypeslib.as_array(r.task_p, (L,)).copy(),
                 "native_edge_deleted": np.ctypeslib.as_array(r.task_deleted, (L,)).astype(bool).copy(),
+                "gcv_margin": np.ctypeslib.as_array(r.task_margin, (L,)).copy(),
             }
         else:
             tasks = {"level": np.zeros(0, int), "x": np.zeros(0, int), "y": np.zeros(0, int),
                      "S_key": [], "conditioning_size": np.zeros(0, int), "p_used": np.zeros(0),
-                     "native_edge_deleted": np.zeros(0, bool)}
+                     "native_edge_deleted": np.zeros(0, bool), "gcv_margin": np.zeros(0)}
         levels = r.levels
         summary = {
             "entrypoint": "kpc_skeleton_run",

@@ -80,6 +80,7 @@ struct Task {
   int S[64];
   uint64_t mask;
   double p = 0.0, stat = 0.0, mean = 0.0, var = 0.0;
+  double margin = 1e300;  // smallest optimizer decision margin of the two fits
   bool computed = false;
   bool consumed = false;
   bool deleted = false;
@@ -182,13 +183,20 @@ class Engine {
     std::vector<long long> comp_src;
     std::vector<int> comp_dst;
     std::vector<int> pair_x, pair_y;
+    std::vector<int> fit_x, fit_y;  // residual slot of each task's fits (-1: raw column)
     for (Task* t : tasks) {
       int slots[2];
+      int rslot[2] = {-1, -1};
       for (int side = 0; side < 2; ++side) {
         const int tgt = side ? t->y : t->x;
         auto key = std::make_pair(tgt, t->mask);
         auto it = comp_map_.find(key);
-        if (it != comp_map_.end()) { slots[side] = it->second; continue; }
+        if (it != comp_map_.end()) {
+          slots[side] = it->second;
+          auto rt = resid_map_.find(key);
+          if (rt != resid_map_.end()) rslot[side] = rt->second;
+          continue;
+        }
         const int cs = n_comp_++;
         comp_map_[key] = cs;
         slots[side] = cs;
@@ -200,6 +208,7 @@ class Engine {
         int dslot = ensure_design(*t, smooth_items, geo_items);
         const int rs = n_resid_++;
         resid_map_[key] = rs;
+        rslot[side] = rs;
         kpc::MagicItem mi;
         mi.design_slot = dslot;
         mi.target = tgt;
@@ -210,6 +219,8 @@ class Engine {
       }
       pair_x.push_back(slots[0]);
       pair_y.push_back(slots[1]);
+      fit_x.push_back(rslot[0]);
+      fit_y.push_back(rslot[1]);
     }
     stats_.physical_tests += (long long)tasks.size();
     stats_.fits += (long long)fit_items.size();
@@ -288,6 +299,13 @@ class Engine {
     }
     check_device_status(fit_items, geo_items, nsm);
     check_component_status(comp_dst);
+    std::vector<double> rinfo_h;
+    if (level_ > 0 && n_resid_ > 0) {
+      rinfo_h.resize((size_t)n_resid_ * kpc::kMagicInfo);
+      hip_check(hipMemcpy(rinfo_h.data(), rinfo_.ptr, sizeof(double) * rinfo_h.size(),
+                          hipMemcpyDeviceToHost),
+                "fit margins");
+    }
     for (size_t i = 0; i < tasks.size(); ++i) {
       Task* t = tasks[i];
       t->p = out[4 * i + 0];
@@ -296,6 +314,10 @@ class Engine {
       t->var = out[4 * i + 3];
       if (std::isnan(t->p)) t->p = 1.0;  // NAdelete = TRUE
       t->computed = true;
+      t->margin = 1e300;
+      for (int rs : {fit_x[i], fit_y[i]})
+        if (rs >= 0)
+          t->margin = std::min(t->margin, rinfo_h[(size_t)rs * kpc::kMagicInfo + kpc::kMagicInfo - 2]);
     }
   }
 
@@ -560,6 +582,7 @@ void push_trace(kpc_result* r, int& cap, const Task& t, int level) {
     r->task_S = (int*)std::realloc(r->task_S, sizeof(int) * (size_t)cap * 64);
     r->task_p = (double*)std::realloc(r->task_p, sizeof(double) * cap);
     r->task_deleted = (unsigned char*)std::realloc(r->task_deleted, cap);
+    r->task_margin = (double*)std::realloc(r->task_margin, sizeof(double) * cap);
   }
   const int i = r->n_tasks++;
   r->task_level[i] = level;
@@ -569,6 +592,7 @@ void push_trace(kpc_result* r, int& cap, const Task& t, int level) {
   for (int j = 0; j < 64; ++j) r->task_S[(size_t)i * 64 + j] = j < t.slen ? t.S[j] : 0;
   r->task_p[i] = t.p;
   r->task_deleted[i] = t.deleted ? 1 : 0;
+  r->task_margin[i] = t.margin;
 }
 
 void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
@@ -826,6 +850,7 @@ void kpc_result_free(kpc_result* r) {
   std::free(r->task_S);
   std::free(r->task_p);
   std::free(r->task_deleted);
+  std::free(r->task_margin);
   std::free(r);
 }
 

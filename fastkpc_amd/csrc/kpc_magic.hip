@@ -585,11 +585,18 @@ __device__ void magic_eval(const MagicLds& L, const MagicScratch& S, const doubl
   __syncthreads();
   jacobi_svd(L.aug, rows, q, L.V, L.d, L.ord, L.ictl);
   if (tid == 0) {
-    const double thr = L.d[L.ord[0]] * rank_tol;
+    const double d0 = L.d[L.ord[0]];
+    const double thr = d0 * rank_tol;
     int nr = 0;
-    for (int i = 0; i < q; ++i) if (L.d[L.ord[i]] >= thr) ++nr;
+    double rm = 1e300;
+    for (int i = 0; i < q; ++i) {
+      const double di = L.d[L.ord[i]];
+      if (di >= thr) ++nr;
+      rm = fmin(rm, fabs(di / d0 - rank_tol) / rank_tol);
+    }
     L.ictl[0] = nr;
     L.ictl[1] = rank;
+    L.dctl[6] = rm;
   }
   __syncthreads();
   const int nr = L.ictl[0];
@@ -848,6 +855,7 @@ __global__ __launch_bounds__(kMagicThreads) void magic_kernel(
       ic[4] = 0;
       ic[7] = 0;
       ic[9] = 1;  // evaluations
+      st[5] = L.dctl[6];  // smallest normalised decision margin
       for (int i = 0; i < m; ++i) { L.grad[i] = 0.0; L.nstep[i] = 0.0; L.sstep[i] = 0.0; }
     }
     __syncthreads();
@@ -875,6 +883,7 @@ __global__ __launch_bounds__(kMagicThreads) void magic_kernel(
           if (tid == 0) {
             ic[9] += 1;
             ic[12] = L.dctl[2] < st[3];
+            st[5] = fmin(st[5], fmin(L.dctl[6], fabs(L.dctl[2] - st[3]) / st[3]));
           }
           __syncthreads();
           if (ic[12]) {
@@ -901,8 +910,10 @@ __global__ __launch_bounds__(kMagicThreads) void magic_kernel(
           double gn = 0.0;
           for (int i = 0; i < m; ++i) gn += L.grad[i] * L.grad[i];
           gn = sqrt(gn);
-          int conv = st[4] <= ctl.tol * (1.0 + st[3]) &&
-                     gn <= pow(ctl.tol, 1.0 / 3.0) * (1.0 + fabs(st[3]));
+          const double t1 = ctl.tol * (1.0 + st[3]);
+          const double t2 = pow(ctl.tol, 1.0 / 3.0) * (1.0 + fabs(st[3]));
+          int conv = st[4] <= t1 && gn <= t2;
+          st[5] = fmin(st[5], fmin(fabs(st[4] - t1) / t1, fabs(gn - t2) / t2));
           if (ic[6] == max_half) { conv = 1; ic[7] = 1; }
           ic[4] = conv;
         }
@@ -910,7 +921,10 @@ __global__ __launch_bounds__(kMagicThreads) void magic_kernel(
         double w[kMaxPen], Z[kMaxPen * kMaxPen];
         small_syev(L.hess, m, w, Z);
         int sd = 0;
-        for (int i = 0; i < m; ++i) if (w[i] < 0.0) sd = 1;
+        for (int i = 0; i < m; ++i) {
+          if (w[i] < 0.0) sd = 1;
+          st[5] = fmin(st[5], fabs(w[i]) / st[3]);
+        }
         ic[3] = sd;
         if (!sd) {
           for (int r = 0; r < m; ++r) {
@@ -949,6 +963,7 @@ __global__ __launch_bounds__(kMagicThreads) void magic_kernel(
           if (tid == 0) {
             ic[9] += 1;
             ic[12] = L.dctl[2] < st[3];
+            st[5] = fmin(st[5], fmin(L.dctl[6], fabs(L.dctl[2] - st[3]) / st[3]));
             if (ic[12]) {
               for (int i = 0; i < m; ++i) L.lsp[i] = L.trial[i];
               st[3] = L.dctl[2];
@@ -990,7 +1005,8 @@ __global__ __launch_bounds__(kMagicThreads) void magic_kernel(
         out[5] = (double)ic[9];
         out[6] = (double)ic[7];
         out[7] = (double)nr;
-        for (int i = 0; i < m && i < kMagicInfo - 8; ++i) out[8 + i] = L.lsp[i];
+        for (int i = 0; i < m && i < kMagicInfo - 10; ++i) out[8 + i] = L.lsp[i];
+        out[kMagicInfo - 2] = st[5];
       }
       __syncthreads();
     }

@@ -87,6 +87,10 @@ typedef struct kpc_result {
   long long plain_tests[64];
   double plain_seconds[64];
   long long stage_launches[8];      /* kernel launches per stage (as stage_seconds) */
+  /* per trace row: smallest normalised decision margin of the GCV optimiser
+   * over the test's two fits (1e300 at |S| = 0); below ~1e-12 a rounding
+   * difference can flip an accept/reject decision of the optimiser */
+  double* task_margin;
 } kpc_result;
 
 void kpc_default_options(kpc_options* opt);
@@ -101,7 +105,8 @@ int kpc_ci_batch(const double* data, int n, int p, const kpc_options* opt, int c
 
 /* regrXonS residuals of `targets` on S (count x s_len).  resid: count x n;
  * info: count x 32 {status, score, rss, edf, iterations, evaluations,
- * step_failed, numerical_rank, log_sp[0..]} */
+ * step_failed, numerical_rank, log_sp[0..21], ..., [30] = smallest normalised
+ * optimiser decision margin} */
 int kpc_regrxons_batch(const double* data, int n, int p, int count, const int* targets,
                        int s_len, const int* S, double* resid, double* info);
 

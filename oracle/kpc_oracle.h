@@ -53,6 +53,7 @@ typedef struct {
   double score, rss, edf;
   double log_sp[64];
   int iterations, evaluations, step_failed;
+  double margin; /* smallest normalised decision margin of the optimizer */
 } orc_magic_result;
 
 void orc_magic_default_control(orc_magic_control* c);
@@ -90,6 +91,7 @@ typedef struct {
   int* trace_level; int* trace_x; int* trace_y; int* trace_slen; int* trace_S; /* S: cap x 64 */
   double* trace_p;
   unsigned char* trace_del;
+  double* trace_margin; /* min decision margin of the test's two fits */
 } orc_skeleton_result;
 
 int orc_skeleton(const double* data, int n, int p, const orc_skeleton_options* opt,

@@ -29,7 +29,8 @@ class OrcSkeletonResult(ctypes.Structure):
                 ("sepset_len", _I), ("sepset", _I), ("n_edgetests", ctypes.c_int * 64),
                 ("trace_len", ctypes.c_int), ("trace_cap", ctypes.c_int),
                 ("trace_level", _I), ("trace_x", _I), ("trace_y", _I), ("trace_slen", _I),
-                ("trace_S", _I), ("trace_p", _D), ("trace_del", ctypes.POINTER(ctypes.c_ubyte))]
+                ("trace_S", _I), ("trace_p", _D), ("trace_del", ctypes.POINTER(ctypes.c_ubyte)),
+                ("trace_margin", _D)]
 
 
 class OrcSetup(ctypes.Structure):
@@ -47,7 +48,8 @@ class OrcMagicControl(ctypes.Structure):
 class OrcMagicResult(ctypes.Structure):
     _fields_ = [("score", ctypes.c_double), ("rss", ctypes.c_double), ("edf", ctypes.c_double),
                 ("log_sp", ctypes.c_double * 64), ("iterations", ctypes.c_int),
-                ("evaluations", ctypes.c_int), ("step_failed", ctypes.c_int)]
+                ("evaluations", ctypes.c_int), ("step_failed", ctypes.c_int),
+                ("margin", ctypes.c_double)]
 
 
 _lib = None
@@ -140,7 +142,8 @@ def regrxons(data, target, S, control=None):
     m = len(Sa) if len(Sa) > 2 else 1
     return resid, {"score": res.score, "rss": res.rss, "edf": res.edf,
                    "log_sp": np.array(res.log_sp[:m]), "iterations": res.iterations,
-                   "evaluations": res.evaluations, "step_failed": res.step_failed}
+                   "evaluations": res.evaluations, "step_failed": res.step_failed,
+                   "margin": res.margin}
 
 
 def gcv_eval(data, target, S, log_sp, rank_tol=1.4901161193847656e-8):
@@ -207,6 +210,7 @@ def skeleton(data, alpha=0.1, m_max=None, num_col=35, threads=8):
             "S_key": ["|".join(str(v + 1) for v in tS[i, :ts[i]]) for i in range(L)],
             "p_used": np.ctypeslib.as_array(res.trace_p, (L,)).copy(),
             "native_edge_deleted": np.ctypeslib.as_array(res.trace_del, (L,)).astype(bool).copy(),
+            "margin": np.ctypeslib.as_array(res.trace_margin, (L,)).copy(),
         }
         return {"adjacency": adj, "pMax": pmax, "sepsets": sepsets,
                 "n.edgetests": list(res.n_edgetests)[:res.levels], "tasks": tasks,

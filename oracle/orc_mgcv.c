@@ -571,6 +571,7 @@ typedef struct {
 typedef struct {
   int nr, arank;
   double rss, edf, score, cond;
+  double rank_margin;  /* min_i |d_i/d_0 - rank_tol| / rank_tol */
   double *U1, *V, *d, *y1; /* U1 q x nr, V q x nr */
 } magic_core;
 
@@ -748,7 +749,12 @@ static int magic_core_eval(const magic_problem* P, const double* y0, double ynor
   orc_svd_jacobi(aug, rows, q, U, d, V);
   double thr = d[0] * P->rank_tol;
   int nr = 0;
-  for (int i = 0; i < q; ++i) if (d[i] >= thr) ++nr;
+  C->rank_margin = 1e300;
+  for (int i = 0; i < q; ++i) {
+    if (d[i] >= thr) ++nr;
+    double mg = fabs(d[i] / d[0] - P->rank_tol) / P->rank_tol;
+    if (mg < C->rank_margin) C->rank_margin = mg;
+  }
   C->arank = rank;
   C->nr = nr;
   C->U1 = (double*)malloc(sizeof(double) * (size_t)q * nr);
@@ -944,6 +950,10 @@ int orc_magic_fit(const orc_setup* s, const double* y, const orc_magic_control* 
   if (rc != 0) goto done;
   magic_derivs(&P, &cur, lsp, cgrad, hess);
   double min_score = cur.score, score_red = 1e10;
+  /* smallest normalised distance of any optimizer decision from its
+   * threshold; below ~1e-10 a rounding difference can flip the decision */
+  double margin = cur.rank_margin;
+#define ORC_MARGIN(v) do { double mv_ = (v); if (mv_ < margin) margin = mv_; } while (0)
   int use_sd = 0, converged = 0, iter, step_failed = 0, evals = 1;
   for (iter = 1; iter <= ctl->max_iterations; ++iter) {
     int tries = 0;
@@ -959,6 +969,8 @@ int orc_magic_fit(const orc_setup* s, const double* y, const orc_magic_control* 
         rc = magic_core_eval(&P, y0, yn, trial, &tc);
         ++evals;
         if (rc != 0) goto done;
+        ORC_MARGIN(tc.rank_margin);
+        ORC_MARGIN(fabs(tc.score - min_score) / min_score);
         if (tc.score < min_score) {
           trying = 0;
           score_red = min_score - tc.score;
@@ -981,12 +993,21 @@ int orc_magic_fit(const orc_setup* s, const double* y, const orc_magic_control* 
       gn = sqrt(gn);
       converged = score_red <= ctl->tol * (1.0 + min_score) &&
                   gn <= pow(ctl->tol, 1.0 / 3.0) * (1.0 + fabs(min_score));
+      {
+        double t1 = ctl->tol * (1.0 + min_score);
+        double t2 = pow(ctl->tol, 1.0 / 3.0) * (1.0 + fabs(min_score));
+        ORC_MARGIN(fabs(score_red - t1) / t1);
+        ORC_MARGIN(fabs(gn - t2) / t2);
+      }
       if (tries == max_half) { converged = 1; step_failed = 1; }
     }
     memcpy(grad, cgrad, sizeof(double) * m);
     orc_syev_jacobi(hess, m, ev, evec);
     use_sd = 0;
-    for (int i = 0; i < m; ++i) if (ev[i] < 0.0) use_sd = 1;
+    for (int i = 0; i < m; ++i) {
+      if (ev[i] < 0.0) use_sd = 1;
+      ORC_MARGIN(fabs(ev[i]) / min_score);
+    }
     if (!use_sd) {
       for (int r = 0; r < m; ++r) {
         double v = 0.0;
@@ -1020,6 +1041,8 @@ int orc_magic_fit(const orc_setup* s, const double* y, const orc_magic_control* 
       rc = magic_core_eval(&P, y0, yn, trial, &tc);
       ++evals;
       if (rc != 0) goto done;
+      ORC_MARGIN(tc.rank_margin);
+      ORC_MARGIN(fabs(tc.score - min_score) / min_score);
       if (!(tc.score < min_score)) { core_free(&tc); break; }
       memcpy(lsp, trial, sizeof(double) * m);
       core_free(&cur);
@@ -1041,6 +1064,7 @@ int orc_magic_fit(const orc_setup* s, const double* y, const orc_magic_control* 
   }
   for (int i = 0; i < m && i < 64; ++i) res->log_sp[i] = lsp[i];
   res->evaluations = evals;
+  res->margin = margin;
   rc = 0;
 done:
   core_free(&cur);

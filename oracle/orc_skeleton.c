@@ -29,7 +29,7 @@ typedef struct {
   int used;
   double* values;
   double* vecs;
-  double dsum, moment;
+  double dsum, moment, margin;
 } comp_entry;
 
 typedef struct {
@@ -85,7 +85,8 @@ static uint64_t mask_of(const int* S, int s_len) {
 
 /* residual of `target` on S (raw column when S is empty) */
 static int residual_for(const double* data, int n, int target, const int* S, int s_len,
-                        double* out) {
+                        double* out, double* margin) {
+  *margin = 1e300;
   if (s_len == 0) {
     memcpy(out, data + (size_t)target * n, sizeof(double) * n);
     return 0;
@@ -93,13 +94,15 @@ static int residual_for(const double* data, int n, int target, const int* S, int
   orc_magic_control ctl;
   orc_magic_default_control(&ctl);
   orc_magic_result res;
-  return orc_regrxons(data, n, n, target, S, s_len, &ctl, out, &res);
+  int rc = orc_regrxons(data, n, n, target, S, s_len, &ctl, out, &res);
+  *margin = res.margin;
+  return rc;
 }
 
 typedef struct {
   double* values;
   double* vecs;
-  double dsum, moment;
+  double dsum, moment, margin;
   int rc;
 } comp_val;
 
@@ -109,7 +112,7 @@ static comp_val compute_component(const double* data, int n, int target, const i
   v.values = (double*)malloc(sizeof(double) * k);
   v.vecs = (double*)malloc(sizeof(double) * (size_t)n * k);
   double* r = (double*)malloc(sizeof(double) * n);
-  v.rc = residual_for(data, n, target, S, s_len, r);
+  v.rc = residual_for(data, n, target, S, s_len, r, &v.margin);
   if (v.rc == 0) v.rc = orc_dcov_component(r, n, k, v.values, v.vecs, &v.dsum, &v.moment);
   free(r);
   return v;
@@ -124,13 +127,14 @@ static int get_component(comp_cache* c, const double* data, int n, int target, c
     if (v.rc != 0) { free(v.values); free(v.vecs); return v.rc; }
     e = cache_find(c, target, m, 1);
     e->values = v.values; e->vecs = v.vecs; e->dsum = v.dsum; e->moment = v.moment;
+    e->margin = v.margin;
   }
   *out = e;
   return 0;
 }
 
 static int ci_pvalue(comp_cache* c, const double* data, int n, int x, int y, const int* S,
-                     int s_len, const orc_skeleton_options* opt, double out[5]) {
+                     int s_len, const orc_skeleton_options* opt, double out[6]) {
   if (opt->ci_method != ORC_DCC_GAMMA) return -20;
   comp_entry *ex, *ey;
   int rc = get_component(c, data, n, x, S, s_len, opt->num_col, &ex);
@@ -139,6 +143,7 @@ static int ci_pvalue(comp_cache* c, const double* data, int n, int x, int y, con
   if (rc) return rc;
   orc_dcov_pair_from_components(n, opt->num_col, ex->vecs, ex->values, ex->dsum, ex->moment,
                                 ey->vecs, ey->values, ey->dsum, ey->moment, out);
+  out[5] = ex->margin < ey->margin ? ex->margin : ey->margin;
   return 0;
 }
 
@@ -146,7 +151,9 @@ int orc_ci_test(const double* data, int n, int p, int x, int y, const int* S, in
                 const orc_skeleton_options* opt, double out[5]) {
   (void)p;
   comp_cache c = {0};
-  int rc = ci_pvalue(&c, data, n, x, y, S, s_len, opt, out);
+  double o6[6];
+  int rc = ci_pvalue(&c, data, n, x, y, S, s_len, opt, o6);
+  for (int i = 0; i < 5; ++i) out[i] = o6[i];
   cache_free(&c);
   return rc;
 }
@@ -162,7 +169,7 @@ static int next_comb(int* idx, int k, int len) {
 }
 
 static void trace_push(orc_skeleton_result* r, int level, int x, int y, const int* S, int s_len,
-                       double pv, int del) {
+                       double pv, int del, double margin) {
   if (r->trace_len == r->trace_cap) {
     int nc = r->trace_cap ? r->trace_cap * 2 : 4096;
     r->trace_level = (int*)realloc(r->trace_level, sizeof(int) * nc);
@@ -172,6 +179,7 @@ static void trace_push(orc_skeleton_result* r, int level, int x, int y, const in
     r->trace_S = (int*)realloc(r->trace_S, sizeof(int) * (size_t)nc * 64);
     r->trace_p = (double*)realloc(r->trace_p, sizeof(double) * nc);
     r->trace_del = (unsigned char*)realloc(r->trace_del, nc);
+    r->trace_margin = (double*)realloc(r->trace_margin, sizeof(double) * nc);
     r->trace_cap = nc;
   }
   int t = r->trace_len++;
@@ -182,6 +190,7 @@ static void trace_push(orc_skeleton_result* r, int level, int x, int y, const in
   for (int i = 0; i < s_len; ++i) r->trace_S[(size_t)t * 64 + i] = S[i];
   r->trace_p[t] = pv;
   r->trace_del[t] = (unsigned char)del;
+  r->trace_margin[t] = margin;
 }
 
 /* Fill the component cache for every (target, S) the level could request. */
@@ -235,6 +244,7 @@ static int prefetch_level(comp_cache* c, const double* data, int n, int p, const
     e->vecs = vals[i].vecs;
     e->dsum = vals[i].dsum;
     e->moment = vals[i].moment;
+    e->margin = vals[i].margin;
   }
   free(vals); free(tg); free(sl);
   return rc;
@@ -281,7 +291,7 @@ int orc_skeleton(const double* data, int n, int p, const orc_skeleton_options* o
           for (int i = 0; i < ord; ++i) idx[i] = i;
           do {
             for (int i = 0; i < ord; ++i) S[i] = nb[idx[i]];
-            double out[5];
+            double out[6];
             rc = ci_pvalue(&cache, data, n, x, y, S, ord, opt, out);
             if (rc) break;
             double pv = out[0];
@@ -289,7 +299,7 @@ int orc_skeleton(const double* data, int n, int p, const orc_skeleton_options* o
             ++tests;
             if (r->pmax[x * p + y] < pv) r->pmax[x * p + y] = pv;
             int deleted = pv >= opt->alpha;
-            trace_push(r, ord, x, y, S, ord, pv, deleted);
+            trace_push(r, ord, x, y, S, ord, pv, deleted, out[5]);
             if (deleted) {
               del[a * p + b] = del[b * p + a] = 1;
               r->sepset_len[a * p + b] = r->sepset_len[b * p + a] = ord;
@@ -321,6 +331,6 @@ int orc_skeleton(const double* data, int n, int p, const orc_skeleton_options* o
 void orc_skeleton_result_free(orc_skeleton_result* r) {
   free(r->adjacency); free(r->pmax); free(r->sepset_len); free(r->sepset);
   free(r->trace_level); free(r->trace_x); free(r->trace_y); free(r->trace_slen);
-  free(r->trace_S); free(r->trace_p); free(r->trace_del);
+  free(r->trace_S); free(r->trace_p); free(r->trace_del); free(r->trace_margin);
   memset(r, 0, sizeof(*r));
 }

@@ -35,7 +35,7 @@ def save(name, data, res):
                         adjacency=res["adjacency"], pmax=res["pMax"], sepsets=sep,
                         n_edgetests=np.array(res["n.edgetests"], dtype=np.int32),
                         level=t["level"], x=t["x"], y=t["y"], S=np.array(t["S_key"]),
-                        p=t["p_used"], deleted=t["native_edge_deleted"])
+                        p=t["p_used"], deleted=t["native_edge_deleted"], margin=t["margin"])
 
 
 def main(names):
