@@ -57,7 +57,7 @@ hipError_t launch_geometry(const GeometryItem* items, int count, int n, int q, i
                            long long smooth_x_stride, double* scratch, int grid,
                            hipStream_t stream);
 
-long long magic_scratch_doubles(int q, int m, int rk);
+long long magic_scratch_doubles(int q, int m, int rk, int n);
 size_t magic_lds_bytes(int n, int q, bool aug_in_lds);
 bool magic_aug_fits_lds(int n, int q);
 hipError_t launch_magic(const MagicItem* items, int count, int n, int q, int m, int rk,

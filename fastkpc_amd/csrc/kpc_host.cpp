@@ -145,6 +145,7 @@ class Engine {
       if (q_ >= n_) throw KpcError(-4, "model coefficient dimension must be less than n");
       if (q_ > kpc::kMaxQ - 1 || m_ > kpc::kMaxPen)
         throw KpcError(-4, "conditioning set too large for this build (|S| <= 14)");
+      if (m_ * rk_ > q_) throw KpcError(-4, "penalty layout exceeds the GCV scratch bound");
       layout_ = kpc::host_design_layout(n_, q_, m_, rk_);
     }
   }
@@ -265,8 +266,8 @@ class Engine {
     }
     hip_check(hipEventRecord(ev_[2], stream_), "event");
     if (!fit_items.empty()) {
-      const long long stride = kpc::magic_scratch_doubles(q_, m_, rk_);
-      int grid = std::min((int)fit_items.size(), 8192);
+      const long long stride = kpc::magic_scratch_doubles(q_, m_, rk_, n_);
+      int grid = std::min((int)fit_items.size(), 65535);
       scratch3_.reserve((size_t)grid * stride);
       hip_check(kpc::launch_magic(d_fit_items_.ptr, (int)fit_items.size(), n_, q_, m_, rk_,
                                   d_data_, design_.ptr, resid_.ptr, rinfo_.ptr, scratch3_.ptr,
@@ -926,7 +927,7 @@ int kpc_regrxons_batch(const double* data, int n, int p, int count, const int* t
     }
     DevArray<kpc::MagicItem> d_items;
     d_items.upload(items, eng.stream());
-    const long long stride = kpc::magic_scratch_doubles(eng.q(), eng.m(), eng.rk());
+    const long long stride = kpc::magic_scratch_doubles(eng.q(), eng.m(), eng.rk(), n);
     const int grid = std::min(count, 8192);
     eng.scratch3().reserve((size_t)grid * stride);
     hip_check(kpc::launch_magic(d_items.ptr, count, n, eng.q(), eng.m(), eng.rk(), eng.data_ptr(),
@@ -987,7 +988,7 @@ int kpc_gcv_grad_hess(const double* data, int n, int p, int count, const int* ta
     d_lsp.upload(hl, eng.stream());
     const size_t per = (size_t)(1 + m + m * m);
     d_out.reserve(per * count);
-    const long long stride = kpc::magic_scratch_doubles(eng.q(), m, eng.rk());
+    const long long stride = kpc::magic_scratch_doubles(eng.q(), m, eng.rk(), n);
     const int grid = std::min(count, 8192);
     eng.scratch3().reserve((size_t)grid * stride);
     hip_check(kpc::launch_gcv_eval(d_items.ptr, count, n, eng.q(), m, eng.rk(), eng.data_ptr(),

@@ -18,7 +18,13 @@
 
 namespace kpc {
 
+constexpr int kGeomThreads = 256;
+// GCV fits run one per workgroup: 64 lanes for q <= 16 (level-1 designs),
+// 256 lanes otherwise, so a fit's Jacobi rounds are spread over the four
+// SIMDs of a CU (a wave64 instruction occupies one SIMD for four cycles and
+// the optimiser's evaluations are strictly sequential).
 constexpr int kMagicThreads = 256;
+inline int magic_threads(int q) { return q <= 16 ? 64 : 256; }
 constexpr double kEps = 2.220446049250313e-16;
 
 __host__ __device__ inline DesignLayout design_layout(int n, int q, int m, int rk) {
@@ -50,90 +56,125 @@ __device__ double serial_r_mean(const double* x, int n) {
   return r.hi + r.lo;
 }
 
+// Workgroup argmax with LAPACK MAXLOC semantics (first index among equal
+// maxima); every thread receives the winning (value, index).
+__device__ void block_argmax_first(double v, int idx, double* redv, int* redi, double* outv,
+                                   int* outi) {
+  const int lane = threadIdx.x & (KPC_WAVE - 1);
+  const int wid = threadIdx.x / KPC_WAVE;
+  const int nw = (blockDim.x + KPC_WAVE - 1) / KPC_WAVE;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double ov = __shfl_xor(v, off, KPC_WAVE);
+    const int oi = __shfl_xor(idx, off, KPC_WAVE);
+    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+  }
+  if (nw == 1) { *outv = v; *outi = idx; return; }
+  __syncthreads();
+  if (lane == 0) { redv[wid] = v; redi[wid] = idx; }
+  __syncthreads();
+  double bv = redv[0];
+  int bi = redi[0];
+  for (int w = 1; w < nw; ++w)
+    if (redv[w] > bv || (redv[w] == bv && redi[w] < bi)) { bv = redv[w]; bi = redi[w]; }
+  *outv = bv;
+  *outi = bi;
+}
+
 // Pivoted Cholesky (LAPACK dpstf2, upper) of the q x q symmetric matrix A
 // (column-major, ld q).  work: 2q doubles.  Returns the rank; piv receives the
-// pivot order.  Pivot selection / swaps on lane 0, row updates in parallel.
+// pivot order.  Every step is workgroup-parallel: diagonal updates, the
+// first-maximum pivot search, the three disjoint row/column swaps and the
+// row of R.  ictl needs 8 ints, dctl 8 doubles (plus redv/redi scratch of
+// 16 entries each inside them).
 __device__ int pstf2_upper(double* A, int q, int* piv, double* work, double tol, int* ictl,
                            double* dctl) {
   const int tid = threadIdx.x;
-  if (tid == 0) {
-    for (int i = 0; i < q; ++i) { piv[i] = i; work[i] = 0.0; }
-    int pvt = 0;
-    double ajj = A[0];
-    for (int i = 1; i < q; ++i)
-      if (A[(size_t)i * q + i] > ajj) { ajj = A[(size_t)i * q + i]; pvt = i; }
-    ictl[0] = (ajj <= 0.0 || isnan(ajj)) ? 0 : 1;
-    ictl[1] = pvt;
-    dctl[0] = ajj;
-    dctl[1] = tol < 0.0 ? (double)q * kEps * ajj : tol;
-    ictl[2] = q;  // rank
+  double* redv = dctl + 8;
+  int* redi = ictl + 8;
+  for (int i = tid; i < q; i += blockDim.x) { piv[i] = i; work[i] = 0.0; }
+  double bv = -__builtin_inf();
+  int bi = 0x7fffffff;
+  for (int i = tid; i < q; i += blockDim.x) {
+    const double v = A[(size_t)i * q + i];
+    if (v > bv || (v == bv && i < bi)) { bv = v; bi = i; }
   }
+  double ajj0;
+  int pvt0;
+  block_argmax_first(bv, bi, redv, redi, &ajj0, &pvt0);
+  if (ajj0 <= 0.0 || isnan(ajj0)) { __syncthreads(); return 0; }
+  const double dstop = tol < 0.0 ? (double)q * kEps * ajj0 : tol;
   __syncthreads();
-  if (!ictl[0]) return 0;
-  const double dstop = dctl[1];
+  int rank = q;
   for (int j = 0; j < q; ++j) {
-    if (tid == 0) {
-      for (int i = j; i < q; ++i) {
-        if (j > 0) work[i] += A[(size_t)i * q + j - 1] * A[(size_t)i * q + j - 1];
-        work[q + i] = A[(size_t)i * q + i] - work[i];
+    bv = -__builtin_inf();
+    bi = 0x7fffffff;
+    for (int i = j + tid; i < q; i += blockDim.x) {
+      if (j > 0) work[i] += A[(size_t)i * q + j - 1] * A[(size_t)i * q + j - 1];
+      const double w2 = A[(size_t)i * q + i] - work[i];
+      work[q + i] = w2;
+      if (w2 > bv || (w2 == bv && i < bi)) { bv = w2; bi = i; }
+    }
+    double ajj;
+    int pvt;
+    if (j == 0) {
+      ajj = ajj0;
+      pvt = pvt0;
+    } else {
+      block_argmax_first(bv, bi, redv, redi, &ajj, &pvt);
+      if (ajj <= dstop || isnan(ajj)) {
+        __syncthreads();
+        if (tid == 0) A[(size_t)j * q + j] = ajj;
+        rank = j;
+        break;
       }
-      int pvt = ictl[1];
-      double ajj = dctl[0];
-      int stop = 0;
-      if (j > 0) {
-        pvt = j;
-        ajj = work[q + j];
-        for (int i = j + 1; i < q; ++i)
-          if (work[q + i] > ajj) { ajj = work[q + i]; pvt = i; }
-        if (ajj <= dstop || isnan(ajj)) {
-          A[(size_t)j * q + j] = ajj;
-          ictl[2] = j;
-          stop = 1;
-        }
-      }
-      if (!stop && j != pvt) {
-        A[(size_t)pvt * q + pvt] = A[(size_t)j * q + j];
-        for (int k = 0; k < j; ++k) {
-          double t = A[(size_t)j * q + k];
-          A[(size_t)j * q + k] = A[(size_t)pvt * q + k];
-          A[(size_t)pvt * q + k] = t;
-        }
-        for (int k = pvt + 1; k < q; ++k) {
-          double t = A[(size_t)k * q + j];
-          A[(size_t)k * q + j] = A[(size_t)k * q + pvt];
-          A[(size_t)k * q + pvt] = t;
-        }
-        for (int k = j + 1; k < pvt; ++k) {
-          double t = A[(size_t)k * q + j];
-          A[(size_t)k * q + j] = A[(size_t)pvt * q + k];
-          A[(size_t)pvt * q + k] = t;
-        }
-        double t = work[j]; work[j] = work[pvt]; work[pvt] = t;
-        int it = piv[pvt]; piv[pvt] = piv[j]; piv[j] = it;
-      }
-      if (!stop) {
-        ajj = sqrt(ajj);
-        A[(size_t)j * q + j] = ajj;
-        dctl[2] = 1.0 / ajj;
-      }
-      ictl[3] = stop;
     }
     __syncthreads();
-    if (ictl[3]) break;
-    const double rinv = dctl[2];
+    if (j != pvt) {
+      // the three swaps touch disjoint entries
+      for (int k = tid; k < j; k += blockDim.x) {
+        const double t = A[(size_t)j * q + k];
+        A[(size_t)j * q + k] = A[(size_t)pvt * q + k];
+        A[(size_t)pvt * q + k] = t;
+      }
+      for (int k = pvt + 1 + tid; k < q; k += blockDim.x) {
+        const double t = A[(size_t)k * q + j];
+        A[(size_t)k * q + j] = A[(size_t)k * q + pvt];
+        A[(size_t)k * q + pvt] = t;
+      }
+      for (int k = j + 1 + tid; k < pvt; k += blockDim.x) {
+        const double t = A[(size_t)k * q + j];
+        A[(size_t)k * q + j] = A[(size_t)pvt * q + k];
+        A[(size_t)pvt * q + k] = t;
+      }
+      if (tid == 0) {
+        A[(size_t)pvt * q + pvt] = A[(size_t)j * q + j];
+        const double t = work[j];
+        work[j] = work[pvt];
+        work[pvt] = t;
+        const int it = piv[pvt];
+        piv[pvt] = piv[j];
+        piv[j] = it;
+      }
+      __syncthreads();
+    }
+    const double rjj = sqrt(ajj);
+    const double rinv = 1.0 / rjj;
     for (int c = j + 1 + tid; c < q; c += blockDim.x) {
       double s = 0.0;
       for (int k = 0; k < j; ++k) s += A[(size_t)c * q + k] * A[(size_t)j * q + k];
       A[(size_t)c * q + j] = (A[(size_t)c * q + j] - s) * rinv;
     }
     __syncthreads();
+    if (tid == 0) A[(size_t)j * q + j] = rjj;
+    __syncthreads();
   }
   __syncthreads();
-  return ictl[2];
+  return rank;
 }
 
 // ------------------------------------------------------------- geometry
-__global__ __launch_bounds__(kMagicThreads) void geometry_kernel(
+__global__ __launch_bounds__(kGeomThreads) void geometry_kernel(
     const GeometryItem* __restrict__ items, int count, int n, int q, int m, int dim, int rk,
     const double* __restrict__ smooth_X, const double* __restrict__ smooth_S,
     double* __restrict__ design, long long smooth_x_stride, double* __restrict__ scratch,
@@ -141,8 +182,8 @@ __global__ __launch_bounds__(kMagicThreads) void geometry_kernel(
   __shared__ double red[16];
   __shared__ double vn1[kMaxQ], vn2[kMaxQ], colv[kMaxQ];
   __shared__ int ipiv[kMaxQ];
-  __shared__ int ictl[8];
-  __shared__ double dctl[8];
+  __shared__ int ictl[24];
+  __shared__ double dctl[24];
   const int tid = threadIdx.x;
   const int lane = tid & (KPC_WAVE - 1);
   const int wid = tid / KPC_WAVE;
@@ -370,11 +411,24 @@ __global__ __launch_bounds__(kMagicThreads) void geometry_kernel(
 }
 
 // ---------------------------------------------------------------- magic
+// Doubles of the per-fit global slab ahead of the response vectors: the
+// aug/V fallback (3 q^2) and the derivative scratch; every design here has
+// m * rk <= q (tp: rank q - 2; additive: 8 m < 1 + 9 m).
+__host__ __device__ inline long long magic_scratch_core(int q) {
+  const long long mr = q;  // m * rk bound
+  long long s = 4LL * q * q + 3LL * q + (long long)q * q + 2LL * q * mr + (long long)q * q +
+                3LL * kMaxPen * q + q + 2LL * mr + 64;
+  return (s + 31) / 32 * 32;
+}
+
 struct MagicLds {
   double* y;      // n (Q'y in the first q entries after the transform)
   double* yorig;  // n
   double* aug;    // (q + q) x q, column-major with ld = rows
   double* V;      // q x q
+  double* X4;     // q x q: aggregate-penalty factor, then G = U1'U1
+  double* gyl;    // q: G y1
+  double* projl;  // 2q: T_a' y1, T_a' G y1
   double* d;      // q singular values (by column)
   int* ord;       // q: descending singular value order
   double* y1;     // q
@@ -394,23 +448,25 @@ struct MagicLds {
 };
 
 __host__ __device__ inline size_t magic_lds_doubles(int n, int q, bool aug_in_lds) {
-  size_t s = 2 * (size_t)n + (aug_in_lds ? (size_t)3 * q * q : 0) + 4 * (size_t)q +
-             7 * kMaxPen + kMaxPen * kMaxPen + 32 + (size_t)q + 16;
+  (void)n;
+  size_t s = (aug_in_lds ? (size_t)4 * q * q + 3 * (size_t)q : 0) + 4 * (size_t)q +
+             7 * kMaxPen + kMaxPen * kMaxPen + 48 + (size_t)q + 16;
   return s;
 }
 
 __device__ MagicLds carve_magic(double* base, int n, int q, bool aug_in_lds, double* gscratch) {
   MagicLds L;
   double* p = base;
-  L.y = p; p += n;
-  L.yorig = p; p += n;
-  if (aug_in_lds) {
-    L.aug = p; p += 2 * (size_t)q * q;
-    L.V = p; p += (size_t)q * q;
-  } else {
-    L.aug = gscratch;
-    L.V = gscratch + 2 * (size_t)q * q;
-  }
+  // response vectors live in the global slab after the derivative scratch
+  L.y = gscratch + magic_scratch_core(q);
+  L.yorig = L.y + n;
+  double* big = aug_in_lds ? p : gscratch;
+  L.aug = big;
+  L.V = big + 2 * (size_t)q * q;
+  L.X4 = big + 3 * (size_t)q * q;
+  L.gyl = big + 4 * (size_t)q * q;
+  L.projl = L.gyl + q;
+  if (aug_in_lds) p += 4 * (size_t)q * q + 3 * (size_t)q;
   L.d = p; p += q;
   L.y1 = p; p += q;
   L.lsp = p; p += kMaxPen;
@@ -422,7 +478,7 @@ __device__ MagicLds carve_magic(double* base, int n, int q, bool aug_in_lds, dou
   L.step = p; p += kMaxPen;
   L.hess = p; p += kMaxPen * kMaxPen;
   L.red = p; p += 16;
-  L.dctl = p; p += 16;
+  L.dctl = p; p += 32;
   L.wk = p; p += 2 * q;
   int* ip = reinterpret_cast<int*>(p);
   L.ord = ip; ip += q;
@@ -440,11 +496,12 @@ struct MagicScratch {
   double* Ky;
   double* yK;
   double* gy;    // nr
+  double* proj;  // 2 * m * rk: T_a' y1 and T_a' G y1
 };
 
 __device__ MagicScratch carve_magic_scratch(double* base, int q, int m, int rk) {
   MagicScratch S;
-  double* p = base + 3 * (size_t)q * q;  // after the global aug/V fallback
+  double* p = base + 4 * (size_t)q * q + 3 * (size_t)q;  // after the global LDS fallback
   S.agg = p; p += (size_t)q * q;
   S.T = p; p += (size_t)q * m * rk;
   S.GT = p; p += (size_t)q * m * rk;
@@ -453,12 +510,14 @@ __device__ MagicScratch carve_magic_scratch(double* base, int q, int m, int rk) 
   S.Ky = p; p += (size_t)m * q;
   S.yK = p; p += (size_t)m * q;
   S.gy = p; p += q;
+  S.proj = p; p += 2 * (size_t)q;
   return S;
 }
 
-long long magic_scratch_doubles(int q, int m, int rk) {
-  long long s = 3LL * q * q + (long long)q * q + 2LL * q * m * rk + (long long)q * q + 3LL * m * q +
-                q + 64;
+long long magic_scratch_doubles(int q, int m, int rk, int n) {
+  long long s = magic_scratch_core(q) + 2LL * n + 64;
+  (void)m;
+  (void)rk;
   return (s + 31) / 32 * 32;
 }
 
@@ -467,34 +526,55 @@ long long magic_scratch_doubles(int q, int m, int rk) {
 __device__ void jacobi_svd(double* A, int rows, int q, double* V, double* d, int* ord,
                            int* ictl) {
   const int tid = threadIdx.x;
-  for (int e = tid; e < q * q; e += blockDim.x) V[e] = (e % q == e / q) ? 1.0 : 0.0;
+  for (int e = tid; e < q * q; e += blockDim.x) V[e] = 0.0;
+  __syncthreads();
+  for (int i = tid; i < q; i += blockDim.x) V[(size_t)i * q + i] = 1.0;
   const int qe = q + (q & 1);
   const int npairs = qe / 2;
   int lpp = 64;
   while (lpp > 1 && npairs * lpp > (int)blockDim.x) lpp >>= 1;
   const int pair = tid / lpp;
   const int sub = tid % lpp;
+  const int qm = qe - 1;
   __syncthreads();
   for (int sweep = 0; sweep < 60; ++sweep) {
     if (tid == 0) ictl[8] = 0;
     __syncthreads();
-    for (int rnd = 0; rnd < qe - 1; ++rnd) {
+    // round-robin schedule: pair 0 meets the fixed column qe-1, pair p>0
+    // pairs (rnd + p) and (rnd - p) mod (qe - 1); advanced incrementally
+    int pa = pair % qm, pb = (qm - pair % qm) % qm;
+    for (int rnd = 0; rnd < qm; ++rnd) {
       int a = -1, b = -1;
       if (pair < npairs) {
         if (pair == 0) { a = rnd; b = qe - 1; }
-        else { a = (rnd + pair) % (qe - 1); b = (rnd - pair + (qe - 1)) % (qe - 1); }
+        else { a = pa; b = pb; }
         if (a > b) { int t = a; a = b; b = t; }
       }
+      if (++pa == qm) pa = 0;
+      if (++pb == qm) pb = 0;
       const bool active = pair < npairs && b < q;
       double al = 0.0, be = 0.0, ga = 0.0;
+      // rows are visited in chunks of kJc with all loads of a chunk issued
+      // before use (LDS latency overlaps); per-lane accumulation order is the
+      // plain row order
+      constexpr int kJc = 8;
       if (active) {
         const double* ca = A + (size_t)a * rows;
         const double* cb = A + (size_t)b * rows;
-        for (int r = sub; r < rows; r += lpp) {
-          const double x = ca[r], y = cb[r];
-          al += x * x;
-          be += y * y;
-          ga += x * y;
+        for (int r0 = sub; r0 < rows; r0 += lpp * kJc) {
+          double xs[kJc], ys[kJc];
+#pragma unroll
+          for (int u = 0; u < kJc; ++u) {
+            const int r = r0 + u * lpp;
+            xs[u] = r < rows ? ca[r] : 0.0;
+            ys[u] = r < rows ? cb[r] : 0.0;
+          }
+#pragma unroll
+          for (int u = 0; u < kJc; ++u) {
+            al += xs[u] * xs[u];
+            be += ys[u] * ys[u];
+            ga += xs[u] * ys[u];
+          }
         }
       }
       for (int off = lpp >> 1; off > 0; off >>= 1) {
@@ -506,25 +586,50 @@ __device__ void jacobi_svd(double* A, int rows, int q, double* V, double* d, int
         const double zeta = (be - al) / (2.0 * ga);
         const double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
         const double c = 1.0 / sqrt(1.0 + t * t);
-        const double s = c * t;
+        const double sn = c * t;
         double* ca = A + (size_t)a * rows;
         double* cb = A + (size_t)b * rows;
-        for (int r = sub; r < rows; r += lpp) {
-          const double x = ca[r], y = cb[r];
-          ca[r] = c * x - s * y;
-          cb[r] = s * x + c * y;
+        for (int r0 = sub; r0 < rows; r0 += lpp * kJc) {
+          double xs[kJc], ys[kJc];
+#pragma unroll
+          for (int u = 0; u < kJc; ++u) {
+            const int r = r0 + u * lpp;
+            xs[u] = r < rows ? ca[r] : 0.0;
+            ys[u] = r < rows ? cb[r] : 0.0;
+          }
+#pragma unroll
+          for (int u = 0; u < kJc; ++u) {
+            const int r = r0 + u * lpp;
+            if (r < rows) {
+              ca[r] = c * xs[u] - sn * ys[u];
+              cb[r] = sn * xs[u] + c * ys[u];
+            }
+          }
         }
         double* va = V + (size_t)a * q;
         double* vb = V + (size_t)b * q;
-        for (int r = sub; r < q; r += lpp) {
-          const double x = va[r], y = vb[r];
-          va[r] = c * x - s * y;
-          vb[r] = s * x + c * y;
+        for (int r0 = sub; r0 < q; r0 += lpp * kJc) {
+          double xs[kJc], ys[kJc];
+#pragma unroll
+          for (int u = 0; u < kJc; ++u) {
+            const int r = r0 + u * lpp;
+            xs[u] = r < q ? va[r] : 0.0;
+            ys[u] = r < q ? vb[r] : 0.0;
+          }
+#pragma unroll
+          for (int u = 0; u < kJc; ++u) {
+            const int r = r0 + u * lpp;
+            if (r < q) {
+              va[r] = c * xs[u] - sn * ys[u];
+              vb[r] = sn * xs[u] + c * ys[u];
+            }
+          }
         }
         if (sub == 0) ictl[8] = 1;
       }
       __syncthreads();
     }
+    if (tid == 0) ictl[28] += 1;
     if (!ictl[8]) break;
     __syncthreads();
   }
@@ -542,14 +647,12 @@ __device__ void jacobi_svd(double* A, int rows, int q, double* V, double* d, int
     for (int r = lane; r < rows; r += KPC_WAVE) col[r] *= inv;
   }
   __syncthreads();
-  if (tid == 0) {
-    for (int i = 0; i < q; ++i) ord[i] = i;
-    for (int i = 1; i < q; ++i) {
-      int oi = ord[i];
-      int j = i - 1;
-      while (j >= 0 && d[ord[j]] < d[oi]) { ord[j + 1] = ord[j]; --j; }
-      ord[j + 1] = oi;
-    }
+  // descending order, ties keep column order (stable): rank by counting
+  for (int i = tid; i < q; i += blockDim.x) {
+    const double di = d[i];
+    int r = 0;
+    for (int j = 0; j < q; ++j) r += (d[j] > di) || (d[j] == di && j < i);
+    ord[r] = i;
   }
   __syncthreads();
 }
@@ -563,14 +666,19 @@ __device__ void magic_eval(const MagicLds& L, const MagicScratch& S, const doubl
   const int lane = tid & (KPC_WAVE - 1);
   const int wid = tid / KPC_WAVE;
   const int nw = blockDim.x / KPC_WAVE;
-  // aggregate penalty, accumulated penalty by penalty
+  // aggregate penalty, accumulated penalty by penalty, factorised in X4
+  double* agg = L.X4;
+  double mult[kMaxPen];
+  for (int i = 0; i < m; ++i) mult[i] = exp(lspv[i]);
   for (int e = tid; e < q * q; e += blockDim.x) {
     double a = 0.0;
-    for (int i = 0; i < m; ++i) a += D[DL.Sp + (size_t)i * q * q + e] * exp(lspv[i]);
-    S.agg[e] = a;
+    for (int i = 0; i < m; ++i) a += D[DL.Sp + (size_t)i * q * q + e] * mult[i];
+    agg[e] = a;
   }
   __syncthreads();
-  const int rank = pstf2_upper(S.agg, q, L.piv, L.wk, -1.0, L.ictl + 16, L.dctl + 10);
+  long long tp0 = wall_clock64();
+  const int rank = pstf2_upper(agg, q, L.piv, L.wk, -1.0, L.ictl + 16, L.dctl + 10);
+  long long tp1 = wall_clock64();
   const int rows = q + rank;
   // augmented [R; root]
   for (int e = tid; e < rows * q; e += blockDim.x) {
@@ -580,23 +688,33 @@ __device__ void magic_eval(const MagicLds& L, const MagicScratch& S, const doubl
   __syncthreads();
   for (int e = tid; e < rank * q; e += blockDim.x) {
     const int r = e % rank, col = e / rank;
-    if (r <= col) L.aug[(size_t)L.piv[col] * rows + q + r] = S.agg[(size_t)col * q + r];
+    if (r <= col) L.aug[(size_t)L.piv[col] * rows + q + r] = agg[(size_t)col * q + r];
   }
   __syncthreads();
+  long long tp2 = wall_clock64();
   jacobi_svd(L.aug, rows, q, L.V, L.d, L.ord, L.ictl);
+  long long tp3 = wall_clock64();
   if (tid == 0) {
+    L.dctl[26] += (double)(tp1 - tp0);
+    L.dctl[27] += (double)(tp3 - tp2);
+  }
+  {
     const double d0 = L.d[L.ord[0]];
     const double thr = d0 * rank_tol;
-    int nr = 0;
+    int cnt = 0;
     double rm = 1e300;
-    for (int i = 0; i < q; ++i) {
-      const double di = L.d[L.ord[i]];
-      if (di >= thr) ++nr;
+    for (int i = tid; i < q; i += blockDim.x) {
+      const double di = L.d[i];
+      cnt += di >= thr;
       rm = fmin(rm, fabs(di / d0 - rank_tol) / rank_tol);
     }
-    L.ictl[0] = nr;
-    L.ictl[1] = rank;
-    L.dctl[6] = rm;
+    const double nrs = block_sum((double)cnt, L.red);
+    const double rmin = -block_max(-rm, L.red);
+    if (tid == 0) {
+      L.ictl[0] = (int)nrs;
+      L.ictl[1] = rank;
+      L.dctl[6] = rmin;
+    }
   }
   __syncthreads();
   const int nr = L.ictl[0];
@@ -621,9 +739,10 @@ __device__ void magic_eval(const MagicLds& L, const MagicScratch& S, const doubl
   }
   const double yAAy = block_sum(p_yaay, L.red);
   const double edf = block_sum(p_edf, L.red);
+  double p_yay = 0.0;
+  for (int c = tid; c < nr; c += blockDim.x) p_yay += L.y1[c] * L.y1[c];
+  const double yAy = block_sum(p_yay, L.red);
   if (tid == 0) {
-    double yAy = 0.0;
-    for (int c = 0; c < nr; ++c) yAy += L.y1[c] * L.y1[c];
     double rss = ynorm2 - 2.0 * yAy + yAAy;
     if (rss < 0.0) rss = 0.0;
     const double delta = (double)n - edf;
@@ -643,48 +762,61 @@ __device__ void magic_derivs(const MagicLds& L, const MagicScratch& S, const dou
   const int nr = L.ictl[0];
   const int rows = q + L.ictl[1];
   const int mr = m * rk;
-  // T (nr x m*rk): T[c][a] = (V1' rS)[c][a] / d_c
-  for (int e = tid; e < nr * mr; e += blockDim.x) {
-    const int c = e % nr, a = e / nr;
-    const double* v = L.V + (size_t)L.ord[c] * q;
-    const double* rs = D + DL.rS + (size_t)a * q;
-    double s = 0.0;
-    for (int k = 0; k < q; ++k) s += v[k] * rs[k];
-    S.T[(size_t)a * nr + c] = s / L.d[L.ord[c]];
-  }
-  // G = U1' U1 (nr x nr)
+  const long long td0 = wall_clock64();
+  // G = U1' U1 (nr x nr) first: after it the augmented buffer is free and
+  // holds T and GT, so the whole derivative working set stays in LDS
+  double* G = L.X4;
   for (int e = tid; e < nr * nr; e += blockDim.x) {
     const int i = e % nr, j = e / nr;
     const double* ui = L.aug + (size_t)L.ord[i] * rows;
     const double* uj = L.aug + (size_t)L.ord[j] * rows;
     double s = 0.0;
     for (int r = 0; r < q; ++r) s += ui[r] * uj[r];
-    S.G[e] = s;
+    G[e] = s;
+  }
+  __syncthreads();
+  double* T = L.aug;                          // nr x mr (mr <= q)
+  double* GT = L.aug + (size_t)q * q;         // nr x mr
+  // T[c][a] = (V1' rS)[c][a] / d_c
+  for (int e = tid; e < nr * mr; e += blockDim.x) {
+    const int c = e % nr, a = e / nr;
+    const double* v = L.V + (size_t)L.ord[c] * q;
+    const double* rs = D + DL.rS + (size_t)a * q;
+    double s = 0.0;
+    for (int k = 0; k < q; ++k) s += v[k] * rs[k];
+    T[(size_t)a * nr + c] = s / L.d[L.ord[c]];
   }
   __syncthreads();
   // GT = G T, gy = G y1
   for (int e = tid; e < nr * mr; e += blockDim.x) {
     const int c = e % nr, a = e / nr;
     double s = 0.0;
-    for (int k = 0; k < nr; ++k) s += S.G[(size_t)k * nr + c] * S.T[(size_t)a * nr + k];
-    S.GT[e] = s;
+    for (int k = 0; k < nr; ++k) s += G[(size_t)k * nr + c] * T[(size_t)a * nr + k];
+    GT[e] = s;
   }
   for (int c = tid; c < nr; c += blockDim.x) {
     double s = 0.0;
-    for (int k = 0; k < nr; ++k) s += S.G[(size_t)k * nr + c] * L.y1[k];
-    S.gy[c] = s;
+    for (int k = 0; k < nr; ++k) s += G[(size_t)k * nr + c] * L.y1[k];
+    L.gyl[c] = s;
   }
   __syncthreads();
-  // per penalty: My = T_i (T_i' y1), Ky = T_i (T_i' G y1), yK = G My
+  // per penalty: My = T_i (T_i' y1), Ky = T_i (T_i' G y1), yK = G My; the
+  // column projections T_a' y1 and T_a' G y1 are formed once
+  for (int a = tid; a < mr; a += blockDim.x) {
+    const double* Ta = T + (size_t)a * nr;
+    double ty = 0.0, tg = 0.0;
+    for (int c = 0; c < nr; ++c) { ty += Ta[c] * L.y1[c]; tg += Ta[c] * L.gyl[c]; }
+    L.projl[2 * a] = ty;
+    L.projl[2 * a + 1] = tg;
+  }
+  __syncthreads();
   for (int e = tid; e < m * nr; e += blockDim.x) {
     const int i = e / nr, o = e % nr;
     double my = 0.0, ky = 0.0;
     for (int a = 0; a < rk; ++a) {
-      const double* Ta = S.T + (size_t)(i * rk + a) * nr;
-      double ty = 0.0, tg = 0.0;
-      for (int c = 0; c < nr; ++c) { ty += Ta[c] * L.y1[c]; tg += Ta[c] * S.gy[c]; }
-      my += Ta[o] * ty;
-      ky += Ta[o] * tg;
+      const double t = T[(size_t)(i * rk + a) * nr + o];
+      my += t * L.projl[2 * (i * rk + a)];
+      ky += t * L.projl[2 * (i * rk + a) + 1];
     }
     S.My[(size_t)i * nr + o] = my;
     S.Ky[(size_t)i * nr + o] = ky;
@@ -693,35 +825,34 @@ __device__ void magic_derivs(const MagicLds& L, const MagicScratch& S, const dou
   for (int e = tid; e < m * nr; e += blockDim.x) {
     const int i = e / nr, o = e % nr;
     double s = 0.0;
-    for (int k = 0; k < nr; ++k) s += S.G[(size_t)k * nr + o] * S.My[(size_t)i * nr + k];
+    for (int k = 0; k < nr; ++k) s += G[(size_t)k * nr + o] * S.My[(size_t)i * nr + k];
     S.yK[(size_t)i * nr + o] = s;
   }
   __syncthreads();
-  // pairwise traces tr(M_j K_i) = tr((T_j'T_i)(T_i' G T_j)) -> hess scratch
-  for (int e = tid; e < m * m; e += blockDim.x) {
-    const int i = e % m, j = e / m;
-    if (j > i) continue;
-    double tsum = 0.0;
-    for (int a = 0; a < rk; ++a) {        // column of T_j
-      for (int b = 0; b < rk; ++b) {      // column of T_i
-        const double* Tj = S.T + (size_t)(j * rk + a) * nr;
-        const double* Ti = S.T + (size_t)(i * rk + b) * nr;
-        const double* GTj = S.GT + (size_t)(j * rk + a) * nr;
+  // pairwise traces tr(M_j K_i) = tr((T_j'T_i)(T_i' G T_j)), one workgroup
+  // reduction per (i, j <= i); then tr(K_i) = sum_a T_ia' G T_ia
+  for (int i = 0; i < m; ++i) {
+    for (int j = 0; j <= i; ++j) {
+      double part = 0.0;
+      for (int e = tid; e < rk * rk; e += blockDim.x) {
+        const int a = e % rk, b = e / rk;  // a: column of T_j, b: column of T_i
+        const double* Tj = T + (size_t)(j * rk + a) * nr;
+        const double* Ti = T + (size_t)(i * rk + b) * nr;
+        const double* GTj = GT + (size_t)(j * rk + a) * nr;
         double x = 0.0, y = 0.0;
         for (int c = 0; c < nr; ++c) { x += Tj[c] * Ti[c]; y += Ti[c] * GTj[c]; }
-        tsum += x * y;
+        part += x * y;
       }
+      const double tsum = block_sum(part, L.red);
+      if (tid == 0) S.agg[j * m + i] = tsum;
     }
-    S.agg[e] = tsum;  // agg is free after the evaluation
-  }
-  for (int i = tid; i < m; i += blockDim.x) {
-    double tr = 0.0;
-    for (int a = 0; a < rk; ++a) {
-      const double* Ta = S.T + (size_t)(i * rk + a) * nr;
-      const double* GTa = S.GT + (size_t)(i * rk + a) * nr;
-      for (int c = 0; c < nr; ++c) tr += Ta[c] * GTa[c];
+    double part = 0.0;
+    for (int e = tid; e < rk * nr; e += blockDim.x) {
+      const int a = e / nr, c = e % nr;
+      part += T[(size_t)(i * rk + a) * nr + c] * GT[(size_t)(i * rk + a) * nr + c];
     }
-    S.agg[m * m + i] = tr;
+    const double tr = block_sum(part, L.red);
+    if (tid == 0) S.agg[m * m + i] = tr;
   }
   __syncthreads();
   if (tid == 0) {
@@ -758,6 +889,7 @@ __device__ void magic_derivs(const MagicLds& L, const MagicScratch& S, const dou
     const double cs = -2.0 * ss / delta;
     const double cv = 3.0 * ds / delta;
     for (int i = 0; i < m; ++i) L.cgrad[i] = ss * dnorm[i] - ds * ddelta[i];
+    L.dctl[29] += (double)(wall_clock64() - td0);
     for (int i = 0; i < m; ++i)
       for (int j = 0; j <= i; ++j) {
         const double v = cs * (ddelta[j] * dnorm[i] + ddelta[i] * dnorm[j]) + ss * d2n[j * m + i] +
@@ -825,7 +957,12 @@ __global__ __launch_bounds__(kMagicThreads) void magic_kernel(
     const double* D = design + (size_t)it.design_slot * DL.stride;
     const double* y = data + (size_t)it.target * n;
     double* out = info + (size_t)it.resid_slot * kMagicInfo;
+    const long long t_start = wall_clock64();
     for (int r = tid; r < n; r += blockDim.x) { L.y[r] = y[r]; L.yorig[r] = y[r]; }
+    if (tid == 0) {
+      L.dctl[26] = L.dctl[27] = L.dctl[29] = 0.0;
+      L.ictl[28] = 0;
+    }
     __syncthreads();
     // y0 = Q' y (reflectors in order)
     for (int i = 0; i < q; ++i) {
@@ -1007,6 +1144,11 @@ __global__ __launch_bounds__(kMagicThreads) void magic_kernel(
         out[7] = (double)nr;
         for (int i = 0; i < m && i < kMagicInfo - 10; ++i) out[8 + i] = L.lsp[i];
         out[kMagicInfo - 2] = st[5];
+        out[kMagicInfo - 3] = (double)(wall_clock64() - t_start);  // wall-clock ticks
+        out[kMagicInfo - 4] = L.dctl[26];  // ticks in the pivoted Cholesky
+        out[kMagicInfo - 5] = L.dctl[27];  // ticks in the SVD
+        out[kMagicInfo - 6] = L.dctl[29];  // ticks in the derivatives
+        out[kMagicInfo - 7] = (double)L.ictl[28];  // Jacobi sweeps
       }
       __syncthreads();
     }
@@ -1031,7 +1173,7 @@ hipError_t launch_geometry(const GeometryItem* items, int count, int n, int q, i
                            long long smooth_x_stride, double* scratch, int grid,
                            hipStream_t stream) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(geometry_kernel, dim3(grid), dim3(kMagicThreads), 0, stream, items, count, n,
+  hipLaunchKernelGGL(geometry_kernel, dim3(grid), dim3(kGeomThreads), 0, stream, items, count, n,
                      q, m, dim, rk, smooth_X, smooth_S, design, smooth_x_stride, scratch,
                      geometry_scratch_doubles(q));
   return hipGetLastError();
@@ -1046,8 +1188,8 @@ hipError_t launch_magic(const MagicItem* items, int count, int n, int q, int m, 
   hipError_t err = hipFuncSetAttribute((const void*)magic_kernel,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (err != hipSuccess) return err;
-  hipLaunchKernelGGL(magic_kernel, dim3(grid), dim3(kMagicThreads), lds, stream, items, count, n,
-                     q, m, rk, data, design, resid, info, scratch, magic_scratch_doubles(q, m, rk),
+  hipLaunchKernelGGL(magic_kernel, dim3(grid), dim3(magic_threads(q)), lds, stream, items, count, n,
+                     q, m, rk, data, design, resid, info, scratch, magic_scratch_doubles(q, m, rk, n),
                      in_lds ? 1 : 0, ctl);
   return hipGetLastError();
 }
@@ -1112,8 +1254,8 @@ hipError_t launch_gcv_eval(const MagicItem* items, int count, int n, int q, int 
   hipError_t err = hipFuncSetAttribute((const void*)gcv_eval_kernel,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (err != hipSuccess) return err;
-  hipLaunchKernelGGL(gcv_eval_kernel, dim3(grid), dim3(kMagicThreads), lds, stream, items, count,
-                     n, q, m, rk, data, design, lsp, out, scratch, magic_scratch_doubles(q, m, rk),
+  hipLaunchKernelGGL(gcv_eval_kernel, dim3(grid), dim3(magic_threads(q)), lds, stream, items, count,
+                     n, q, m, rk, data, design, lsp, out, scratch, magic_scratch_doubles(q, m, rk, n),
                      in_lds ? 1 : 0, rank_tol);
   return hipGetLastError();
 }

@@ -194,9 +194,9 @@ def test_skeleton_sem351x48_vs_oracle():
     rep = {}
     compare_skeleton(res, gold, p_close, rep)
     print("sem351x48 vs oracle:", rep)
-    # pMax: max p-value per edge over the consumed tests
-    np.testing.assert_allclose(res["pMax"][np.isfinite(gold["pmax"])],
-                               gold["pmax"][np.isfinite(gold["pmax"])], rtol=1e-9, atol=1e-12)
+    # pMax: max p-value per edge over the consumed tests (tie rows included)
+    fin = np.isfinite(gold["pmax"])
+    np.testing.assert_allclose(res["pMax"][fin], gold["pmax"][fin], rtol=1e-6, atol=1e-12)
 
 
 def test_plain_form_equals_batched_form():
