@@ -75,22 +75,22 @@ def read_traffic(kernel):
 
 def cpu_baseline(data, res, n, k, seconds_budget):
     """Time the C oracle on a per-level sample of the workload's residual fits
-    and dcov components; extrapolate with the GPU run's per-level counts."""
+    and dcov components (each level gets an equal share of the time budget);
+    extrapolate with the GPU run's per-level fit/component counts."""
     from oracle import orc
     s = res["summary"]
     tasks = res["tasks"]
     levels = res["levels"]
-    per_level = max(2, int(seconds_budget / max(1, levels) / 0.08 / 2))
+    active = [lev for lev in range(levels) if np.any(tasks["level"] == lev)]
+    share = seconds_budget / max(1, len(active))
     total = 0.0
     sampled = []
     t_start = time.time()
-    for lev in range(levels):
+    for lev in active:
         idx = np.nonzero(tasks["level"] == lev)[0]
-        if len(idx) == 0:
-            continue
-        take = idx[:per_level]
         fit_t, comp_t, nf, nc = 0.0, 0.0, 0, 0
-        for i in take:
+        t_lev = time.time()
+        for i in idx:
             x = int(tasks["x"][i]) - 1
             S = [int(v) - 1 for v in tasks["S_key"][i].split("|")] if tasks["S_key"][i] else []
             if S:
@@ -104,19 +104,12 @@ def cpu_baseline(data, res, n, k, seconds_budget):
             orc.dcov_component(r, k)
             comp_t += time.perf_counter() - t0
             nc += 1
-            if time.time() - t_start > seconds_budget:
+            if time.time() - t_lev > share:
                 break
-        fits = s["level_fits"][lev]
-        comps = s["level_components"][lev]
-        est = (fit_t / nf * fits if nf else 0.0) + comp_t / max(1, nc) * comps
+        est = (fit_t / nf * s["level_fits"][lev] if nf else 0.0) + \
+            comp_t / max(1, nc) * s["level_components"][lev]
         total += est
         sampled.append("L%d:%dfit/%dcomp" % (lev, nf, nc))
-        if time.time() - t_start > seconds_budget:
-            # remaining levels: use this level's per-item cost
-            for lev2 in range(lev + 1, levels):
-                total += (fit_t / max(1, nf)) * s["level_fits"][lev2] + \
-                         (comp_t / max(1, nc)) * s["level_components"][lev2]
-            break
     tests = int(sum(res["n.edgetests"]))
     return {
         "value": tests / total if total > 0 else None,
@@ -124,8 +117,8 @@ def cpu_baseline(data, res, n, k, seconds_budget):
         "cores": 1,
         "kind": "port",
         "sample": "C oracle (kpcalg/mgcv/pcalg restatement), 1 thread, per-level sample %s of the "
-                  "run's unique residual fits and dcov components, extrapolated with the run's "
-                  "per-level fit/component counts (%.1f s of CPU work; est. skeleton %.1f s)"
+                  "run's residual fits and dcov components, extrapolated with the run's per-level "
+                  "fit/component counts (%.1f s of CPU work; est. skeleton %.1f s)"
                   % (",".join(sampled), time.time() - t_start, total),
     }
 
