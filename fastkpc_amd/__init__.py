@@ -1,8 +1,11 @@
 """MI355X (gfx950) implementation of the kernel-PC conditional-independence
 hot path: regrXonS GAM residualisation with GCV smoothing selection and the
-kpcalg dcov.gamma test, behind the reference's skeleton API."""
+kpcalg dcov.gamma, hsic.gamma, hsic.perm and dcc.perm tests, behind the
+reference's skeleton API."""
 from .api import (ci_batch, dcov_components, fastkpc_compatible_cuda_skeleton, gcv_grad_hess,
-                  kernelCItest, regrXonS, regrxons_batch, resolve_max_conditioning_size)
+                  kernelCItest, permutation_table, regrXonS, regrxons_batch,
+                  resolve_max_conditioning_size, task_seed)
 
 __all__ = ["fastkpc_compatible_cuda_skeleton", "kernelCItest", "regrXonS", "regrxons_batch",
-           "ci_batch", "gcv_grad_hess", "dcov_components", "resolve_max_conditioning_size"]
+           "ci_batch", "gcv_grad_hess", "dcov_components", "resolve_max_conditioning_size",
+           "task_seed", "permutation_table"]

@@ -72,8 +72,8 @@ class KpcResult(ctypes.Structure):
 
 EXPORTS = (
     "kpc_default_options", "kpc_skeleton_run", "kpc_result_free", "kpc_ci_batch",
-    "kpc_regrxons_batch", "kpc_gcv_grad_hess", "kpc_dcov_components", "kpc_last_error",
-    "kpc_device_count",
+    "kpc_regrxons_batch", "kpc_gcv_grad_hess", "kpc_dcov_components", "kpc_task_seed",
+    "kpc_permutation_table", "kpc_last_error", "kpc_device_count",
 )
 
 _lib = None
@@ -104,6 +104,10 @@ def lib():
     L.kpc_gcv_grad_hess.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I, ctypes.c_int,
                                     _I, _D, _D, _D, _D]
     L.kpc_dcov_components.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D, _D, _D]
+    L.kpc_task_seed.argtypes = [ctypes.c_ulonglong, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I,
+                                ctypes.c_int, ctypes.POINTER(ctypes.c_ulonglong)]
+    L.kpc_permutation_table.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_ulonglong), _I]
     L.kpc_last_error.restype = ctypes.c_char_p
     L.kpc_device_count.restype = ctypes.c_int
     _lib = L

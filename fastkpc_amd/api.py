@@ -103,12 +103,12 @@ def fastkpc_compatible_cuda_skeleton(data, alpha, labels=None, options=None):
                 "conditioning_size": ts,
                 "p_used": np.ctypeslib.as_array(r.task_p, (L,)).copy(),
                 "native_edge_deleted": np.ctypeslib.as_array(r.task_deleted, (L,)).astype(bool).copy(),
-                "gcv_margin": np.ctypeslib.as_array(r.task_margin, (L,)).copy(),
+                "decision_margin": np.ctypeslib.as_array(r.task_margin, (L,)).copy(),
             }
         else:
             tasks = {"level": np.zeros(0, int), "x": np.zeros(0, int), "y": np.zeros(0, int),
                      "S_key": [], "conditioning_size": np.zeros(0, int), "p_used": np.zeros(0),
-                     "native_edge_deleted": np.zeros(0, bool), "gcv_margin": np.zeros(0)}
+                     "native_edge_deleted": np.zeros(0, bool), "decision_margin": np.zeros(0)}
         levels = r.levels
         summary = {
             "entrypoint": "kpc_skeleton_run",
@@ -150,14 +150,20 @@ def fastkpc_compatible_cuda_skeleton(data, alpha, labels=None, options=None):
         _lib.lib().kpc_result_free(res)
 
 
-def ci_batch(data, x, y, S, num_col=35, ci_method="dcc.gamma", alpha=0.1):
-    """Batch of CI tests sharing |S|: returns (count, 4) {p, statistic, mean, variance}."""
+def ci_batch(data, x, y, S, num_col=35, ci_method="dcc.gamma", alpha=0.1, sig=1.0,
+             replicates=100, seed=None, index=1.0, include_observed=True):
+    """Batch of CI tests (x_i, y_i | S_i) sharing |S| (0-based indices):
+    returns (count, 4) {p, statistic, mean, variance}."""
     d = _as_data(data)
     n, p = d.shape
     x = np.ascontiguousarray(x, dtype=np.int32)
     y = np.ascontiguousarray(y, dtype=np.int32)
     S = np.ascontiguousarray(np.asarray(S, dtype=np.int32).reshape(len(x), -1))
-    o = _options(n, p, alpha, {"numCol": num_col, "ci_method": ci_method})
+    perm = {"replicates": replicates, "include_observed": include_observed}
+    if seed is not None:
+        perm["seed"] = seed
+    o = _options(n, p, alpha, {"numCol": num_col, "ci_method": ci_method, "index": index,
+                               "hsic_params": {"sig": sig}, "permutation_params": perm})
     out = np.zeros((len(x), 4))
     check(_lib.lib().kpc_ci_batch(dptr(d), n, p, o, len(x), iptr(x), iptr(y), S.shape[1],
                                   iptr(S), dptr(out)))
@@ -166,14 +172,34 @@ def ci_batch(data, x, y, S, num_col=35, ci_method="dcc.gamma", alpha=0.1):
 
 def kernelCItest(x, y, S, suffStat):
     """kpcalg/R/kernelCItest.R:68-129 with 1-based x, y, S; suffStat holds data,
-    ic.method, index, numCol, sig, p."""
+    ic.method, index, numCol, sig, p (permutations) and, for the permutation
+    methods, an optional integer seed of the counter-based index generator."""
     data = suffStat["data"]
     n = np.asarray(data).shape[0]
     method = suffStat.get("ic.method", "dcc.perm")
     num_col = suffStat.get("numCol", n // 10)
-    S0 = [int(s) - 1 for s in (S or [])]
-    out = ci_batch(data, [int(x) - 1], [int(y) - 1], [S0], num_col=num_col, ci_method=method)
+    S0 = sorted(int(s) - 1 for s in (S or []))
+    out = ci_batch(data, [int(x) - 1], [int(y) - 1], [S0], num_col=num_col, ci_method=method,
+                   sig=suffStat.get("sig", 1.0), replicates=suffStat.get("p", 100),
+                   seed=suffStat.get("seed"), index=suffStat.get("index", 1.0))
     return float(out[0, 0])
+
+
+def task_seed(user_seed, ci_method, x, y, S):
+    """Seed of the permutation test ({x, y}, S) (0-based)."""
+    Sa = np.ascontiguousarray(sorted(S), dtype=np.int32)
+    out = _lib.ctypes.c_ulonglong()
+    check(_lib.lib().kpc_task_seed(int(user_seed), CI_METHODS[ci_method], int(x), int(y),
+                                   iptr(Sa), len(Sa), _lib.ctypes.byref(out)))
+    return int(out.value)
+
+
+def permutation_table(n, replicates, seeds):
+    """Device-generated permutation indices: (len(seeds), replicates, n)."""
+    sd = (_lib.ctypes.c_ulonglong * len(seeds))(*[int(s) for s in seeds])
+    out = np.zeros((len(seeds), replicates, n), dtype=np.int32)
+    check(_lib.lib().kpc_permutation_table(int(n), int(replicates), len(seeds), sd, iptr(out)))
+    return out
 
 
 def regrXonS(data, targets, S):

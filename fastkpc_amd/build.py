@@ -17,7 +17,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libkpcmi.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["kpc_dcov.hip", "kpc_dcov_lanczos.hip", "kpc_smooth.hip", "kpc_magic.hip", "kpc_host.cpp"]
+SOURCES = ["kpc_dcov.hip", "kpc_dcov_lanczos.hip", "kpc_smooth.hip", "kpc_magic.hip", "kpc_hsic.hip",
+           "kpc_host.cpp"]
 COMMON = [
     "-std=c++17", "-O3", "--offload-arch=gfx950", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -5,6 +5,9 @@
 namespace kpc {
 
 constexpr int kMagicInfo = 32;
+// Per-test output record of every pair kernel: {p, statistic, mean, variance,
+// decision margin} (margin 1e300 when the statistic has no discrete decision).
+constexpr int kPairOut = 5;
 
 struct DesignLayout {
   long long X, QR, tau, R, piv, rS, Sp, lsp0, status, stride;
@@ -82,4 +85,28 @@ hipError_t launch_dcov_lanczos(const double* src_base, const long long* src_off,
                                double* scratch, int grid, int mmax, double tol,
                                hipStream_t stream);
 int lanczos_mmax(int n);
+}  // namespace kpc
+
+namespace kpc {
+// HSIC (kernlab inchol factors) and the permutation tests.
+hipError_t launch_inchol_components(const double* src_base, const long long* src_off,
+                                    const int* dst_slot, int count, int n, int k, double sigma,
+                                    double* vecs, double* offdiag, double* moment, int* rank,
+                                    double* margin, int grid, hipStream_t stream);
+hipError_t launch_hsic_pairs(const int* pa, const int* pb, const unsigned long long* seeds,
+                             int count, int n, int k, const double* vecs, const int* rank,
+                             const double* offdiag, const double* moment, int permute,
+                             int replicates, int include_observed, double* out, int grid,
+                             hipStream_t stream);
+hipError_t launch_dcc_rowmeans(const double* src_base, const long long* src_off,
+                               const int* dst_slot, int count, int n, int stride, double index,
+                               double* vecs, double* grand, int grid, hipStream_t stream);
+hipError_t launch_dcc_perm_pairs(const int* pa, const int* pb, const unsigned long long* seeds,
+                                 int count, int n, int stride, double index, const double* vecs,
+                                 const double* grand, int replicates, int include_observed,
+                                 double* out, int grid, hipStream_t stream);
+// Device-generated permutation table (the indices the permutation kernels
+// consume), for verification against the oracle.
+hipError_t launch_permutation_table(const unsigned long long* seeds, int count, int n,
+                                    int replicates, int* out, hipStream_t stream);
 }  // namespace kpc

@@ -5,6 +5,13 @@
 // fastkpc/src/full_cuda_ci_one_call.cpp:5229-5580 (level loop, per-edge early
 // stop, canonical replay, pMax, sepsets, n.edgetests, `done` rule).
 //
+// CI methods: dcc.gamma (kpcalg/R/dcovgamma.R), hsic.gamma / hsic.perm
+// (kpcalg/R/hsicgamma.R, hsicperm.R) and dcc.perm (energy::dcov.test).  Every
+// method is symmetric in (x, y), so a test is identified by ({x, y}, S): the
+// second orientation of a test already evaluated in the level reuses its
+// result, and permutation tests draw their indices from the seed of that
+// identity (permuting the variable with the larger index).
+//
 // Execution (KPC_BATCHED): each level is evaluated in frontier waves.  A wave
 // takes, for every undecided edge, the next `chunk` tasks of its current
 // orientation (chunk doubles per wave), evaluates all of them in one batch of
@@ -75,12 +82,45 @@ struct KeyHash {
   }
 };
 
+struct TwinHash {
+  size_t operator()(const std::pair<uint64_t, uint64_t>& k) const {
+    uint64_t x = k.first * 0x9E3779B97F4A7C15ULL ^ (k.second + 0x632BE59BD9B4E019ULL) * 0xC2B2AE3D27D4EB4FULL;
+    x ^= x >> 31;
+    return (size_t)x;
+  }
+};
+
+struct TestResult {
+  double p, stat, mean, var, margin;
+};
+
+// Task seed of the permutation tests: SplitMix64 finaliser folded over (user
+// seed, method, min(x,y), max(x,y), |S|, S ascending).  Replaces the
+// reference's SHA-256 task identity (fastkpc/src/full_cuda_ci_one_call.cpp:
+// 296-325); the oracle computes the same value.
+uint64_t splitmix(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+uint64_t task_seed(uint64_t user_seed, int method, int x, int y, const int* S, int s_len) {
+  const int a = std::min(x, y), b = std::max(x, y);
+  uint64_t h = splitmix(user_seed ^ (uint64_t)(method + 1));
+  h = splitmix(h ^ (uint64_t)(a + 1));
+  h = splitmix(h ^ (uint64_t)(b + 1));
+  h = splitmix(h ^ (uint64_t)(s_len + 1));
+  for (int i = 0; i < s_len; ++i) h = splitmix(h ^ (uint64_t)(S[i] + 1));
+  return h;
+}
+
 struct Task {
   int x, y, slen;
   int S[64];
   uint64_t mask;
   double p = 0.0, stat = 0.0, mean = 0.0, var = 0.0;
-  double margin = 1e300;  // smallest optimizer decision margin of the two fits
+  double margin = 1e300;  // smallest decision margin (fits, inchol pivots, permutation ties)
   bool computed = false;
   bool consumed = false;
   bool deleted = false;
@@ -98,8 +138,19 @@ class Engine {
     if (p < 2 || p > 64) throw KpcError(-1, "p must be in [2, 64]");
     if (opt.num_col <= 0 || opt.num_col >= n || opt.num_col > kpc::kMaxNumCol)
       throw KpcError(-1, "num_col must be positive, < n and <= 64");
-    if (opt.ci_method != KPC_DCC_GAMMA)
-      throw KpcError(-3, "only ci_method dcc.gamma is available in this build");
+    if (opt.ci_method < KPC_DCC_GAMMA || opt.ci_method > KPC_HSIC_PERM)
+      throw KpcError(-3, "ci_method must be dcc.gamma, hsic.gamma, dcc.perm or hsic.perm");
+    method_ = opt.ci_method;
+    if ((method_ == KPC_HSIC_GAMMA || method_ == KPC_HSIC_PERM) &&
+        !(opt.hsic_sig > 0.0 && std::isfinite(opt.hsic_sig)))
+      throw KpcError(-1, "hsic sig must be positive and finite");
+    if ((method_ == KPC_DCC_PERM || method_ == KPC_HSIC_PERM) &&
+        (opt.replicates < 0 || opt.replicates > 1000000))
+      throw KpcError(-1, "replicates must be in [0, 1e6]");
+    // energy::dcov.test falls back to index 1 outside [0, 2]
+    index_ = (opt.index >= 0.0 && opt.index <= 2.0) ? opt.index : 1.0;
+    seed_ = opt.has_seed ? (uint64_t)opt.seed : 0ULL;
+    comp_k_ = method_ == KPC_DCC_PERM ? 2 : opt.num_col;
     for (long long i = 0; i < (long long)n * p; ++i)
       if (!std::isfinite(h_data[i])) throw KpcError(-1, "data must be finite");
     hip_check(hipStreamCreate(&stream_), "hipStreamCreate");
@@ -129,6 +180,7 @@ class Engine {
     design_map_.clear();
     resid_map_.clear();
     comp_map_.clear();
+    twin_map_.clear();
     n_design_ = n_resid_ = n_comp_ = n_lsmooth_ = 0;
     if (level <= 2) {
       m_ = 1;
@@ -168,16 +220,37 @@ class Engine {
       rinfo_.reserve(max_resid * kpc::kMagicInfo);
     }
     comp_vals_.reserve(max_resid * (size_t)k_);
-    comp_vecs_.reserve(max_resid * (size_t)n_ * k_);
+    comp_vecs_.reserve(max_resid * (size_t)n_ * comp_k_);
     comp_dsum_.reserve(max_resid);
     comp_mom_.reserve(max_resid);
     comp_status_.reserve(max_resid);
+    comp_margin_.reserve(max_resid);
   }
 
   // Evaluate p-values for the given tasks (all with |S| == level_).
-  void evaluate(std::vector<Task*>& tasks) {
-    if (tasks.empty()) return;
+  void evaluate(std::vector<Task*>& all) {
+    if (all.empty()) return;
     const double t0 = now_s();
+    // orientation-free identity: reuse a twin's result, evaluate each once
+    std::vector<Task*> tasks;
+    std::vector<std::pair<Task*, size_t>> dups;  // (task, index into tasks)
+    std::unordered_map<std::pair<uint64_t, uint64_t>, size_t, TwinHash> in_batch;
+    for (Task* t : all) {
+      const auto key = twin_key(*t);
+      auto done = twin_map_.find(key);
+      if (done != twin_map_.end()) {
+        apply_result(*t, done->second);
+        continue;
+      }
+      auto it = in_batch.find(key);
+      if (it != in_batch.end()) {
+        dups.emplace_back(t, it->second);
+        continue;
+      }
+      in_batch.emplace(key, tasks.size());
+      tasks.push_back(t);
+    }
+    if (tasks.empty()) return;
     std::vector<int> smooth_items;     // d, c0, c1, slot
     std::vector<kpc::GeometryItem> geo_items;
     std::vector<kpc::MagicItem> fit_items;
@@ -185,11 +258,15 @@ class Engine {
     std::vector<int> comp_dst;
     std::vector<int> pair_x, pair_y;
     std::vector<int> fit_x, fit_y;  // residual slot of each task's fits (-1: raw column)
+    std::vector<unsigned long long> seeds;
     for (Task* t : tasks) {
       int slots[2];
       int rslot[2] = {-1, -1};
+      if (method_ == KPC_DCC_PERM || method_ == KPC_HSIC_PERM)
+        seeds.push_back(task_seed(seed_, method_, t->x, t->y, t->S, t->slen));
       for (int side = 0; side < 2; ++side) {
-        const int tgt = side ? t->y : t->x;
+        // side 0: min(x, y), side 1: max(x, y) (the permuted variable)
+        const int tgt = side ? std::max(t->x, t->y) : std::min(t->x, t->y);
         auto key = std::make_pair(tgt, t->mask);
         auto it = comp_map_.find(key);
         if (it != comp_map_.end()) {
@@ -235,7 +312,8 @@ class Engine {
     d_comp_dst_.upload(comp_dst, stream_);
     d_pair_x_.upload(pair_x, stream_);
     d_pair_y_.upload(pair_y, stream_);
-    d_pair_out_.reserve(tasks.size() * 4);
+    d_seeds_.upload(seeds, stream_);
+    d_pair_out_.reserve(tasks.size() * kpc::kPairOut);
     stage_host_ += now_s() - t0;
     hip_check(hipEventRecord(ev_[0], stream_), "event");
     const int nsm = (int)smooth_items.size() / 4;
@@ -277,15 +355,9 @@ class Engine {
     hip_check(hipEventRecord(ev_[3], stream_), "event");
     if (!comp_dst.empty()) launch_components(level_ == 0 ? d_data_ : resid_.ptr, comp_dst.size());
     hip_check(hipEventRecord(ev_[4], stream_), "event");
-    {
-      int grid = std::min((int)tasks.size(), 65535);
-      hip_check(kpc::launch_dcov_pairs(d_pair_x_.ptr, d_pair_y_.ptr, (int)tasks.size(), n_, k_,
-                                       comp_vals_.ptr, comp_vecs_.ptr, comp_dsum_.ptr,
-                                       comp_mom_.ptr, d_pair_out_.ptr, grid, stream_),
-                "dcov pair launch");
-    }
+    launch_pairs((int)tasks.size());
     hip_check(hipEventRecord(ev_[5], stream_), "event");
-    std::vector<double> out(tasks.size() * 4);
+    std::vector<double> out(tasks.size() * kpc::kPairOut);
     hip_check(hipMemcpyAsync(out.data(), d_pair_out_.ptr, sizeof(double) * out.size(),
                              hipMemcpyDeviceToHost, stream_),
               "D2H p-values");
@@ -300,6 +372,13 @@ class Engine {
     }
     check_device_status(fit_items, geo_items, nsm);
     check_component_status(comp_dst);
+    std::vector<double> cmargin_h;
+    if (method_ == KPC_HSIC_GAMMA || method_ == KPC_HSIC_PERM) {
+      cmargin_h.resize((size_t)n_comp_);
+      hip_check(hipMemcpy(cmargin_h.data(), comp_margin_.ptr, sizeof(double) * cmargin_h.size(),
+                          hipMemcpyDeviceToHost),
+                "component margins");
+    }
     std::vector<double> rinfo_h;
     if (level_ > 0 && n_resid_ > 0) {
       rinfo_h.resize((size_t)n_resid_ * kpc::kMagicInfo);
@@ -308,21 +387,80 @@ class Engine {
                 "fit margins");
     }
     for (size_t i = 0; i < tasks.size(); ++i) {
-      Task* t = tasks[i];
-      t->p = out[4 * i + 0];
-      t->stat = out[4 * i + 1];
-      t->mean = out[4 * i + 2];
-      t->var = out[4 * i + 3];
-      if (std::isnan(t->p)) t->p = 1.0;  // NAdelete = TRUE
-      t->computed = true;
-      t->margin = 1e300;
+      const double* o = out.data() + kpc::kPairOut * i;
+      TestResult res{o[0], o[1], o[2], o[3], o[4]};
+      if (std::isnan(res.p)) res.p = 1.0;  // NAdelete = TRUE
       for (int rs : {fit_x[i], fit_y[i]})
         if (rs >= 0)
-          t->margin = std::min(t->margin, rinfo_h[(size_t)rs * kpc::kMagicInfo + kpc::kMagicInfo - 2]);
+          res.margin = std::min(res.margin, rinfo_h[(size_t)rs * kpc::kMagicInfo + kpc::kMagicInfo - 2]);
+      if (!cmargin_h.empty())
+        res.margin = std::min(res.margin, std::min(cmargin_h[pair_x[i]], cmargin_h[pair_y[i]]));
+      apply_result(*tasks[i], res);
+      twin_map_.emplace(twin_key(*tasks[i]), res);
+    }
+    for (auto& d : dups) apply_result(*d.first, twin_map_.at(twin_key(*tasks[d.second])));
+  }
+
+  static std::pair<uint64_t, uint64_t> twin_key(const Task& t) {
+    const uint64_t a = (uint64_t)std::min(t.x, t.y), b = (uint64_t)std::max(t.x, t.y);
+    return std::make_pair(t.mask, (a << 32) | b);
+  }
+
+  static void apply_result(Task& t, const TestResult& r) {
+    t.p = r.p;
+    t.stat = r.stat;
+    t.mean = r.mean;
+    t.var = r.var;
+    t.margin = r.margin;
+    t.computed = true;
+  }
+
+  void launch_pairs(int count) {
+    const int grid = std::min(count, 65535);
+    switch (method_) {
+      case KPC_DCC_GAMMA:
+        hip_check(kpc::launch_dcov_pairs(d_pair_x_.ptr, d_pair_y_.ptr, count, n_, k_, comp_vals_.ptr,
+                                         comp_vecs_.ptr, comp_dsum_.ptr, comp_mom_.ptr,
+                                         d_pair_out_.ptr, grid, stream_),
+                  "dcov pair launch");
+        break;
+      case KPC_HSIC_GAMMA:
+      case KPC_HSIC_PERM:
+        hip_check(kpc::launch_hsic_pairs(d_pair_x_.ptr, d_pair_y_.ptr, d_seeds_.ptr, count, n_, k_,
+                                         comp_vecs_.ptr, comp_status_.ptr, comp_dsum_.ptr,
+                                         comp_mom_.ptr, method_ == KPC_HSIC_PERM ? 1 : 0,
+                                         opt_.replicates, opt_.include_observed, d_pair_out_.ptr,
+                                         grid, stream_),
+                  "hsic pair launch");
+        break;
+      default:
+        hip_check(kpc::launch_dcc_perm_pairs(d_pair_x_.ptr, d_pair_y_.ptr, d_seeds_.ptr, count, n_,
+                                             n_ * comp_k_, index_, comp_vecs_.ptr, comp_dsum_.ptr,
+                                             opt_.replicates, opt_.include_observed,
+                                             d_pair_out_.ptr, grid, stream_),
+                  "dcc.perm pair launch");
+        break;
     }
   }
 
   void launch_components(const double* src, size_t count) {
+    if (method_ == KPC_HSIC_GAMMA || method_ == KPC_HSIC_PERM) {
+      const int grid = std::min((int)count, 8192);
+      hip_check(kpc::launch_inchol_components(src, d_comp_src_.ptr, d_comp_dst_.ptr, (int)count, n_,
+                                              k_, 1.0 / opt_.hsic_sig, comp_vecs_.ptr,
+                                              comp_dsum_.ptr, comp_mom_.ptr, comp_status_.ptr,
+                                              comp_margin_.ptr, grid, stream_),
+                "inchol launch");
+      return;
+    }
+    if (method_ == KPC_DCC_PERM) {
+      const int grid = std::min((int)count, 8192);
+      hip_check(kpc::launch_dcc_rowmeans(src, d_comp_src_.ptr, d_comp_dst_.ptr, (int)count, n_,
+                                         n_ * comp_k_, index_, comp_vecs_.ptr, comp_dsum_.ptr, grid,
+                                         stream_),
+                "dcc row-mean launch");
+      return;
+    }
     const int mmax = kpc::lanczos_mmax(n_);
     if (dense_components_) {
       const long long stride = kpc::dcov_component_scratch_doubles(n_, k_);
@@ -479,17 +617,26 @@ class Engine {
   }
 
   void check_component_status(const std::vector<int>& dst) {
-    if (dense_components_ || dst.empty()) return;
+    if (dst.empty() || method_ == KPC_DCC_PERM) return;
+    if (method_ == KPC_DCC_GAMMA && dense_components_) return;
     std::vector<int> h((size_t)n_comp_);
     hip_check(hipMemcpy(h.data(), comp_status_.ptr, sizeof(int) * h.size(), hipMemcpyDeviceToHost),
               "component status");
     for (int s : dst) {
+      if (method_ != KPC_DCC_GAMMA) {
+        if (h[s] <= 0 || h[s] > k_) throw KpcError(-7, "incomplete Cholesky produced no factor");
+        continue;
+      }
       if (h[s] <= 0) throw KpcError(-7, "dcov component eigensolver did not converge");
       stats_.lanczos_steps += h[s];
     }
   }
 
   int n_, p_, k_;
+  int method_ = KPC_DCC_GAMMA;
+  int comp_k_ = 0;  // doubles per row of a component slot (n * comp_k_ per slot)
+  double index_ = 1.0;
+  uint64_t seed_ = 0;
   kpc_options opt_;
   hipStream_t stream_ = nullptr;
   double* d_data_ = nullptr;
@@ -499,6 +646,7 @@ class Engine {
   kpc::DesignLayout layout_{};
   std::unordered_map<uint64_t, int> design_map_;
   std::unordered_map<std::pair<int, uint64_t>, int, KeyHash> resid_map_, comp_map_;
+  std::unordered_map<std::pair<uint64_t, uint64_t>, TestResult, TwinHash> twin_map_;
   std::vector<int> uni_slot_;
   int n_design_ = 0, n_resid_ = 0, n_comp_ = 0, n_lsmooth_ = 0;
   DevArray<double> lsmooth_X_, lsmooth_S_, usmooth_X_, usmooth_S_;
@@ -506,6 +654,8 @@ class Engine {
   DevArray<double> design_, resid_, rinfo_;
   DevArray<double> comp_vals_, comp_vecs_, comp_dsum_, comp_mom_;
   DevArray<int> comp_status_;
+  DevArray<double> comp_margin_;
+  DevArray<unsigned long long> d_seeds_;
   bool dense_components_ = std::getenv("KPC_DCOV_DENSE") != nullptr;
   DevArray<double> scratch_, scratch2_, scratch3_, scratch4_;
   DevArray<int> d_smooth_items_;
@@ -1058,6 +1208,40 @@ int kpc_dcov_components(const double* x, int n, int count, int k, double* values
     if (vectors) hip_check(hipMemcpy(vectors, d_vecs.ptr, sizeof(double) * (size_t)count * n * k, hipMemcpyDeviceToHost), "D2H");
     if (dsum) hip_check(hipMemcpy(dsum, d_dsum.ptr, sizeof(double) * count, hipMemcpyDeviceToHost), "D2H");
     if (moment) hip_check(hipMemcpy(moment, d_mom.ptr, sizeof(double) * count, hipMemcpyDeviceToHost), "D2H");
+    (void)hipStreamDestroy(s);
+    return 0;
+  } catch (const std::exception& e) {
+    return fill_error(e);
+  }
+}
+
+int kpc_task_seed(unsigned long long user_seed, int ci_method, int x, int y, const int* S,
+                  int s_len, unsigned long long* seed) {
+  try {
+    if (!seed || s_len < 0 || s_len > 64 || (s_len > 0 && !S)) throw KpcError(-1, "invalid arguments");
+    *seed = task_seed(user_seed, ci_method, x, y, S, s_len);
+    return 0;
+  } catch (const std::exception& e) {
+    return fill_error(e);
+  }
+}
+
+int kpc_permutation_table(int n, int replicates, int count, const unsigned long long* seeds,
+                          int* out) {
+  try {
+    if (!seeds || !out || n < 2 || n > kpc::kMaxN || replicates < 1 || count < 1)
+      throw KpcError(-1, "invalid arguments");
+    const size_t total = (size_t)count * replicates * n;
+    hipStream_t s;
+    hip_check(hipStreamCreate(&s), "stream");
+    DevArray<unsigned long long> d_seeds;
+    DevArray<int> d_out;
+    d_seeds.upload(std::vector<unsigned long long>(seeds, seeds + count), s);
+    d_out.reserve(total);
+    hip_check(kpc::launch_permutation_table(d_seeds.ptr, count, n, replicates, d_out.ptr, s),
+              "permutation launch");
+    hip_check(hipStreamSynchronize(s), "permutation synchronize");
+    hip_check(hipMemcpy(out, d_out.ptr, sizeof(int) * total, hipMemcpyDeviceToHost), "D2H");
     (void)hipStreamDestroy(s);
     return 0;
   } catch (const std::exception& e) {

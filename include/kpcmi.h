@@ -16,6 +16,10 @@
  *                                 kpcalg regrXonS (kpcalg/R/regrXonS.R:33-61)
  *   kpc_gcv_grad_hess   exposes   the GCV score gradient/Hessian in log(sp)
  *                                 (fastkpc/src/mgcv_multi_penalty_cpp.cpp:819-990)
+ *   kpc_permutation_table replaces full_cuda_ci_method_seeded_permutation_table
+ *                                 (fastkpc/src/cuda/full_cuda_ci_method_batch.cu:1681-1717)
+ *   kpc_task_seed       replaces  task_permutation_seed
+ *                                 (fastkpc/src/full_cuda_ci_one_call.cpp:296-325)
  *   kpc_last_error      replaces  the std::runtime_error -> R error mapping
  *                                 (BEGIN_RCPP/END_RCPP in r_api_cuda.cpp)
  *
@@ -87,9 +91,11 @@ typedef struct kpc_result {
   long long plain_tests[64];
   double plain_seconds[64];
   long long stage_launches[8];      /* kernel launches per stage (as stage_seconds) */
-  /* per trace row: smallest normalised decision margin of the GCV optimiser
-   * over the test's two fits (1e300 at |S| = 0); below ~1e-12 a rounding
-   * difference can flip an accept/reject decision of the optimiser */
+  /* per trace row: smallest normalised decision margin of the test -- GCV
+   * optimiser decisions of its two fits, incomplete-Cholesky pivot/stop
+   * decisions (hsic.*), |T_perm - T| / |T| over the replicates (*.perm);
+   * 1e300 when none applies.  Below ~1e-12 a rounding difference can flip a
+   * discrete decision. */
   double* task_margin;
 } kpc_result;
 
@@ -120,6 +126,20 @@ int kpc_gcv_grad_hess(const double* data, int n, int p, int count, const int* ta
  * centred vectors count x n x k, dsum, moment. */
 int kpc_dcov_components(const double* x, int n, int count, int k, double* values,
                         double* vectors, double* dsum, double* moment);
+
+/* Seed of the permutation test ({x, y}, S) for ci_method: SplitMix64 finaliser
+ * folded over (user_seed, ci_method, min(x,y), max(x,y), s_len, S ascending).
+ * Host only. */
+int kpc_task_seed(unsigned long long user_seed, int ci_method, int x, int y, const int* S,
+                  int s_len, unsigned long long* seed);
+
+/* The permutation indices the dcc.perm / hsic.perm kernels consume, generated
+ * on the device: out is count x replicates x n (0-based); replicate r of seed s
+ * is the Durstenfeld shuffle of 0..n-1 (i = n-1 .. 1, j = floor(u (i+1) / 2^64))
+ * with u the 64-bit word (i & 1 ? w3:w2 : w1:w0) of Philox4x32-10 at key s and
+ * counter {i >> 1, r, 0x4b504331, 0}. */
+int kpc_permutation_table(int n, int replicates, int count, const unsigned long long* seeds,
+                          int* out);
 
 const char* kpc_last_error(void);
 int kpc_device_count(void);

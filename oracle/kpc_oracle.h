@@ -28,6 +28,30 @@ void orc_dcov_pair_from_components(int n, int k, const double* cx, const double*
 /* out = {p.value, nV2, mean, variance, statistic} */
 int orc_dcov_gamma(const double* x, const double* y, int n, int num_col, double out[5]);
 
+/* ---- HSIC (kpcalg/R/hsicgamma.R:57-83, hsicperm.R:61-88; kernlab inchol) ---- */
+int orc_inchol(const double* x, int n, double sigma, double tol, int maxr, double* g,
+               int* pivots, double* margin);
+int orc_hsic_component(const double* x, int n, double sig, int maxr, double* gc, int* rank,
+                       double* offdiag, double* moment, double* margin);
+double orc_hsic_cross(const double* gx, int rx, const double* gy, int ry, int n,
+                      const int* perm);
+/* out = {p.value, HSIC, mean, variance, margin} */
+void orc_hsic_gamma_from_factors(const double* gx, int rx, double offx, double momx,
+                                 const double* gy, int ry, double offy, double momy, int n,
+                                 double out[5]);
+void orc_hsic_perm_from_factors(const double* gx, int rx, const double* gy, int ry, int n,
+                                int R, int include_observed, unsigned long long seed,
+                                double out[5]);
+
+/* ---- permutations (counter-based; shared with the GPU) and dcc.perm ---- */
+void orc_philox4x32_10(const unsigned int ctr[4], const unsigned int key[2], unsigned int out[4]);
+unsigned long long orc_task_seed(unsigned long long user_seed, int method, int x, int y,
+                                 const int* S, int s_len);
+void orc_permutation(unsigned long long seed, int replicate, int n, int* perm);
+/* out = {p.value, n V^2, mean(rep), var(rep), margin} */
+int orc_dcc_perm(const double* x, const double* y, int n, double index, int R,
+                 int include_observed, unsigned long long seed, double out[5]);
+
 /* ---- regrXonS (kpcalg/R/regrXonS.R:33-61, mgcv gam GCV.Cp) ---- */
 typedef struct {
   int n, q, m;  /* rows, coefficients, penalties */
@@ -78,6 +102,8 @@ typedef struct {
   int replicates;   /* permutation methods */
   unsigned long long perm_seed;
   int threads;      /* OpenMP threads for the per-level precompute (0 = serial) */
+  double index;     /* dcc.perm distance exponent */
+  int include_observed;
 } orc_skeleton_options;
 
 typedef struct {
@@ -98,7 +124,8 @@ int orc_skeleton(const double* data, int n, int p, const orc_skeleton_options* o
                  orc_skeleton_result* res);
 void orc_skeleton_result_free(orc_skeleton_result* r);
 
-/* single CI test p-value (kernelCItest semantics) */
+/* single CI test (kernelCItest semantics):
+ * out = {p.value, statistic, mean, variance, decision margin} */
 int orc_ci_test(const double* data, int n, int p, int x, int y, const int* S, int s_len,
                 const orc_skeleton_options* opt, double out[5]);
 

@@ -20,7 +20,17 @@ class OrcSkeletonOptions(ctypes.Structure):
     _fields_ = [("ci_method", ctypes.c_int), ("alpha", ctypes.c_double), ("m_max", ctypes.c_int),
                 ("num_col", ctypes.c_int), ("hsic_sig", ctypes.c_double),
                 ("replicates", ctypes.c_int), ("perm_seed", ctypes.c_ulonglong),
-                ("threads", ctypes.c_int)]
+                ("threads", ctypes.c_int), ("index", ctypes.c_double),
+                ("include_observed", ctypes.c_int)]
+
+
+METHODS = {"dcc.gamma": 0, "hsic.gamma": 1, "dcc.perm": 2, "hsic.perm": 3}
+
+
+def options(method="dcc.gamma", alpha=0.1, m_max=62, num_col=35, sig=1.0, replicates=100,
+            seed=0, threads=0, index=1.0, include_observed=True):
+    return OrcSkeletonOptions(METHODS[method], alpha, m_max, num_col, sig, replicates, seed,
+                              threads, index, 1 if include_observed else 0)
 
 
 class OrcSkeletonResult(ctypes.Structure):
@@ -83,6 +93,25 @@ def lib():
                                    ctypes.POINTER(OrcMagicResult)]
         L.orc_ci_test.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I,
                                   ctypes.c_int, ctypes.POINTER(OrcSkeletonOptions), _D]
+        _U = ctypes.POINTER(ctypes.c_uint)
+        L.orc_philox4x32_10.argtypes = [_U, _U, _U]
+        L.orc_task_seed.argtypes = [ctypes.c_ulonglong, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    _I, ctypes.c_int]
+        L.orc_task_seed.restype = ctypes.c_ulonglong
+        L.orc_permutation.argtypes = [ctypes.c_ulonglong, ctypes.c_int, ctypes.c_int, _I]
+        L.orc_inchol.argtypes = [_D, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                 _D, _I, _D]
+        L.orc_hsic_component.argtypes = [_D, ctypes.c_int, ctypes.c_double, ctypes.c_int, _D, _I,
+                                         _D, _D, _D]
+        L.orc_hsic_gamma_from_factors.argtypes = [_D, ctypes.c_int, ctypes.c_double,
+                                                  ctypes.c_double, _D, ctypes.c_int,
+                                                  ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                                  _D]
+        L.orc_hsic_perm_from_factors.argtypes = [_D, ctypes.c_int, _D, ctypes.c_int, ctypes.c_int,
+                                                 ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong,
+                                                 _D]
+        L.orc_dcc_perm.argtypes = [_D, _D, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_ulonglong, _D]
         _lib = L
     return _lib
 
@@ -170,24 +199,113 @@ def gcv_eval(data, target, S, log_sp, rank_tol=1.4901161193847656e-8):
         lib().orc_setup_free(ctypes.byref(st))
 
 
-def ci_test(data, x, y, S, num_col=35):
+def ci_test(data, x, y, S, num_col=35, method="dcc.gamma", sig=1.0, replicates=100, seed=0,
+            index=1.0, include_observed=True):
+    """kernelCItest(x, y, S) on the oracle; 0-based x, y, S."""
     d = np.asfortranarray(data, dtype=np.float64)
     n, p = d.shape
     Sa = np.ascontiguousarray(S, dtype=np.int32)
-    opt = OrcSkeletonOptions(0, 0.1, p - 2, num_col, 1.0, 100, 0, 0)
+    opt = options(method, 0.1, p - 2, num_col, sig, replicates, seed, 0, index, include_observed)
     out = np.zeros(5)
     rc = lib().orc_ci_test(_d(d), n, p, int(x), int(y), Sa.ctypes.data_as(_I), len(Sa),
                            ctypes.byref(opt), _d(out))
     if rc != 0:
         raise RuntimeError("orc_ci_test failed %d" % rc)
-    return {"p.value": out[0], "nV2": out[1], "mean": out[2], "variance": out[3]}
+    return {"p.value": out[0], "statistic": out[1], "nV2": out[1], "mean": out[2],
+            "variance": out[3], "margin": out[4]}
 
 
-def skeleton(data, alpha=0.1, m_max=None, num_col=35, threads=8):
+def philox4x32_10(ctr, key):
+    c = (ctypes.c_uint * 4)(*[int(v) & 0xFFFFFFFF for v in ctr])
+    k = (ctypes.c_uint * 2)(*[int(v) & 0xFFFFFFFF for v in key])
+    o = (ctypes.c_uint * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return [int(v) for v in o]
+
+
+def task_seed(user_seed, method, x, y, S):
+    Sa = np.ascontiguousarray(S, dtype=np.int32)
+    return int(lib().orc_task_seed(int(user_seed), METHODS[method], int(x), int(y),
+                                   Sa.ctypes.data_as(_I), len(Sa)))
+
+
+def permutation(seed, replicate, n):
+    out = np.zeros(n, dtype=np.int32)
+    lib().orc_permutation(int(seed), int(replicate), int(n), out.ctypes.data_as(_I))
+    return out
+
+
+def inchol(x, sig=1.0, tol=0.001, max_rank=35):
+    """Incomplete Cholesky factor (uncentred, rank columns), pivots, margin."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = len(x)
+    g = np.zeros(n * max_rank)
+    piv = np.zeros(max_rank, dtype=np.int32)
+    mg = np.zeros(1)
+    r = lib().orc_inchol(_d(x), n, 1.0 / sig, tol, max_rank, _d(g), piv.ctypes.data_as(_I),
+                         _d(mg))
+    return g.reshape(max_rank, n).T[:, :r].copy(), piv[:r].copy(), mg[0]
+
+
+def hsic_component(x, sig=1.0, max_rank=35):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = len(x)
+    g = np.zeros(n * max_rank)
+    rank = np.zeros(1, dtype=np.int32)
+    off = np.zeros(1)
+    mom = np.zeros(1)
+    mg = np.zeros(1)
+    rc = lib().orc_hsic_component(_d(x), n, float(sig), max_rank, _d(g),
+                                  rank.ctypes.data_as(_I), _d(off), _d(mom), _d(mg))
+    if rc != 0:
+        raise RuntimeError("orc_hsic_component failed %d" % rc)
+    r = int(rank[0])
+    return g.reshape(max_rank, n).T[:, :r].copy(), off[0], mom[0], mg[0]
+
+
+def hsic_gamma(x, y, sig=1.0, num_col=35):
+    gx, ox, mx, _ = hsic_component(x, sig, num_col)
+    gy, oy, my, _ = hsic_component(y, sig, num_col)
+    n = len(x)
+    out = np.zeros(5)
+    fx = np.ascontiguousarray(gx.T).ravel()
+    fy = np.ascontiguousarray(gy.T).ravel()
+    lib().orc_hsic_gamma_from_factors(_d(fx), gx.shape[1], ox, mx, _d(fy), gy.shape[1], oy, my,
+                                      n, _d(out))
+    return {"p.value": out[0], "statistic": out[1], "mean": out[2], "variance": out[3]}
+
+
+def hsic_perm(x, y, sig=1.0, num_col=35, replicates=100, seed=0, include_observed=True):
+    gx, _, _, _ = hsic_component(x, sig, num_col)
+    gy, _, _, _ = hsic_component(y, sig, num_col)
+    n = len(x)
+    out = np.zeros(5)
+    fx = np.ascontiguousarray(gx.T).ravel()
+    fy = np.ascontiguousarray(gy.T).ravel()
+    lib().orc_hsic_perm_from_factors(_d(fx), gx.shape[1], _d(fy), gy.shape[1], n, replicates,
+                                     1 if include_observed else 0, int(seed), _d(out))
+    return {"p.value": out[0], "statistic": out[1], "mean": out[2], "variance": out[3],
+            "margin": out[4]}
+
+
+def dcc_perm(x, y, index=1.0, replicates=100, seed=0, include_observed=True):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.zeros(5)
+    rc = lib().orc_dcc_perm(_d(x), _d(y), len(x), float(index), int(replicates),
+                            1 if include_observed else 0, int(seed), _d(out))
+    if rc != 0:
+        raise RuntimeError("orc_dcc_perm failed %d" % rc)
+    return {"p.value": out[0], "statistic": out[1], "mean": out[2], "variance": out[3],
+            "margin": out[4]}
+
+
+def skeleton(data, alpha=0.1, m_max=None, num_col=35, threads=8, method="dcc.gamma", sig=1.0,
+             replicates=100, seed=0, index=1.0, include_observed=True):
     d = np.asfortranarray(data, dtype=np.float64)
     n, p = d.shape
-    opt = OrcSkeletonOptions(0, alpha, p - 2 if m_max is None else m_max, num_col, 1.0, 100, 0,
-                             threads)
+    opt = options(method, alpha, p - 2 if m_max is None else m_max, num_col, sig, replicates,
+                  seed, threads, index, include_observed)
     res = OrcSkeletonResult()
     rc = lib().orc_skeleton(_d(d), n, p, ctypes.byref(opt), ctypes.byref(res))
     if rc != 0:

@@ -3,6 +3,7 @@
 #define ORC_INTERNAL_H
 
 #include <stddef.h>
+#include <stdint.h>
 
 int orc_pstf2_upper(double* a, int n, int lda, int* piv, double tol, int* rank);
 void orc_larfg(int m, double* alpha, double* x, int incx, double* tau);

@@ -23,6 +23,10 @@
 #include <omp.h>
 #endif
 
+/* Per-(target, S) component.  dcc.gamma: top-k eigenpairs (values, vecs),
+ * dsum, moment.  hsic.*: centred inchol factor in vecs (rank columns),
+ * dsum = off-diagonal kernel sum, moment = self moment.  dcc.perm: the
+ * residual itself in vecs. */
 typedef struct {
   uint64_t key_mask;
   int target;
@@ -30,6 +34,7 @@ typedef struct {
   double* values;
   double* vecs;
   double dsum, moment, margin;
+  int rank;
 } comp_entry;
 
 typedef struct {
@@ -103,47 +108,93 @@ typedef struct {
   double* values;
   double* vecs;
   double dsum, moment, margin;
+  int rank;
   int rc;
 } comp_val;
 
 static comp_val compute_component(const double* data, int n, int target, const int* S,
-                                  int s_len, int k) {
+                                  int s_len, const orc_skeleton_options* opt) {
+  const int k = opt->num_col;
   comp_val v;
+  memset(&v, 0, sizeof(v));
   v.values = (double*)malloc(sizeof(double) * k);
   v.vecs = (double*)malloc(sizeof(double) * (size_t)n * k);
   double* r = (double*)malloc(sizeof(double) * n);
   v.rc = residual_for(data, n, target, S, s_len, r, &v.margin);
-  if (v.rc == 0) v.rc = orc_dcov_component(r, n, k, v.values, v.vecs, &v.dsum, &v.moment);
+  if (v.rc == 0) {
+    if (opt->ci_method == ORC_DCC_GAMMA) {
+      v.rc = orc_dcov_component(r, n, k, v.values, v.vecs, &v.dsum, &v.moment);
+      v.rank = k;
+    } else if (opt->ci_method == ORC_HSIC_GAMMA || opt->ci_method == ORC_HSIC_PERM) {
+      double im = 1e300;
+      v.rc = orc_hsic_component(r, n, opt->hsic_sig, k, v.vecs, &v.rank, &v.dsum, &v.moment,
+                                &im);
+      if (im < v.margin) v.margin = im;
+    } else {
+      memcpy(v.vecs, r, sizeof(double) * n);
+      v.rank = 1;
+    }
+  }
   free(r);
   return v;
 }
 
 static int get_component(comp_cache* c, const double* data, int n, int target, const int* S,
-                         int s_len, int k, comp_entry** out) {
+                         int s_len, const orc_skeleton_options* opt, comp_entry** out) {
   uint64_t m = mask_of(S, s_len);
   comp_entry* e = cache_find(c, target, m, 0);
   if (!e) {
-    comp_val v = compute_component(data, n, target, S, s_len, k);
+    comp_val v = compute_component(data, n, target, S, s_len, opt);
     if (v.rc != 0) { free(v.values); free(v.vecs); return v.rc; }
     e = cache_find(c, target, m, 1);
     e->values = v.values; e->vecs = v.vecs; e->dsum = v.dsum; e->moment = v.moment;
-    e->margin = v.margin;
+    e->margin = v.margin; e->rank = v.rank;
   }
   *out = e;
   return 0;
 }
 
+/* One CI test on cached components; out = {p, statistic, mean, variance,
+ * margin}.  Permutation tests permute the variable with the larger index and
+ * draw their indices from the task seed of (x, y, S). */
 static int ci_pvalue(comp_cache* c, const double* data, int n, int x, int y, const int* S,
-                     int s_len, const orc_skeleton_options* opt, double out[6]) {
-  if (opt->ci_method != ORC_DCC_GAMMA) return -20;
+                     int s_len, const orc_skeleton_options* opt, double out[5]) {
+  if (opt->ci_method < ORC_DCC_GAMMA || opt->ci_method > ORC_HSIC_PERM) return -20;
+  const int lo = x < y ? x : y, hi = x < y ? y : x;
   comp_entry *ex, *ey;
-  int rc = get_component(c, data, n, x, S, s_len, opt->num_col, &ex);
+  int rc = get_component(c, data, n, lo, S, s_len, opt, &ex);
   if (rc) return rc;
-  rc = get_component(c, data, n, y, S, s_len, opt->num_col, &ey);
+  rc = get_component(c, data, n, hi, S, s_len, opt, &ey);
   if (rc) return rc;
-  orc_dcov_pair_from_components(n, opt->num_col, ex->vecs, ex->values, ex->dsum, ex->moment,
-                                ey->vecs, ey->values, ey->dsum, ey->moment, out);
-  out[5] = ex->margin < ey->margin ? ex->margin : ey->margin;
+  const unsigned long long seed =
+      orc_task_seed(opt->perm_seed, opt->ci_method, x, y, S, s_len);
+  double o[5];
+  switch (opt->ci_method) {
+    case ORC_DCC_GAMMA: {
+      orc_dcov_pair_from_components(n, opt->num_col, ex->vecs, ex->values, ex->dsum, ex->moment,
+                                    ey->vecs, ey->values, ey->dsum, ey->moment, o);
+      o[4] = 1e300;
+      break;
+    }
+    case ORC_HSIC_GAMMA:
+      orc_hsic_gamma_from_factors(ex->vecs, ex->rank, ex->dsum, ex->moment, ey->vecs, ey->rank,
+                                  ey->dsum, ey->moment, n, o);
+      break;
+    case ORC_HSIC_PERM:
+      orc_hsic_perm_from_factors(ex->vecs, ex->rank, ey->vecs, ey->rank, n, opt->replicates,
+                                 opt->include_observed, seed, o);
+      break;
+    default:
+      rc = orc_dcc_perm(ex->vecs, ey->vecs, n, opt->index, opt->replicates,
+                        opt->include_observed, seed, o);
+      if (rc) return rc;
+      break;
+  }
+  for (int i = 0; i < 4; ++i) out[i] = o[i];
+  double mg = o[4];
+  if (ex->margin < mg) mg = ex->margin;
+  if (ey->margin < mg) mg = ey->margin;
+  out[4] = mg;
   return 0;
 }
 
@@ -151,9 +202,7 @@ int orc_ci_test(const double* data, int n, int p, int x, int y, const int* S, in
                 const orc_skeleton_options* opt, double out[5]) {
   (void)p;
   comp_cache c = {0};
-  double o6[6];
-  int rc = ci_pvalue(&c, data, n, x, y, S, s_len, opt, o6);
-  for (int i = 0; i < 5; ++i) out[i] = o6[i];
+  int rc = ci_pvalue(&c, data, n, x, y, S, s_len, opt, out);
   cache_free(&c);
   return rc;
 }
@@ -236,7 +285,7 @@ static int prefetch_level(comp_cache* c, const double* data, int n, int p, const
   comp_val* vals = (comp_val*)calloc(cnt ? cnt : 1, sizeof(comp_val));
 #pragma omp parallel for schedule(dynamic, 1) num_threads(opt->threads > 0 ? opt->threads : 1)
   for (int i = 0; i < cnt; ++i)
-    vals[i] = compute_component(data, n, tg[i], sl + (size_t)i * 64, level, opt->num_col);
+    vals[i] = compute_component(data, n, tg[i], sl + (size_t)i * 64, level, opt);
   for (int i = 0; i < cnt; ++i) {
     comp_entry* e = cache_find(c, tg[i], mask_of(sl + (size_t)i * 64, level), 0);
     if (vals[i].rc != 0) { rc = vals[i].rc; free(vals[i].values); free(vals[i].vecs); continue; }
@@ -245,6 +294,7 @@ static int prefetch_level(comp_cache* c, const double* data, int n, int p, const
     e->dsum = vals[i].dsum;
     e->moment = vals[i].moment;
     e->margin = vals[i].margin;
+    e->rank = vals[i].rank;
   }
   free(vals); free(tg); free(sl);
   return rc;
@@ -291,7 +341,7 @@ int orc_skeleton(const double* data, int n, int p, const orc_skeleton_options* o
           for (int i = 0; i < ord; ++i) idx[i] = i;
           do {
             for (int i = 0; i < ord; ++i) S[i] = nb[idx[i]];
-            double out[6];
+            double out[5];
             rc = ci_pvalue(&cache, data, n, x, y, S, ord, opt, out);
             if (rc) break;
             double pv = out[0];
@@ -299,7 +349,7 @@ int orc_skeleton(const double* data, int n, int p, const orc_skeleton_options* o
             ++tests;
             if (r->pmax[x * p + y] < pv) r->pmax[x * p + y] = pv;
             int deleted = pv >= opt->alpha;
-            trace_push(r, ord, x, y, S, ord, pv, deleted, out[5]);
+            trace_push(r, ord, x, y, S, ord, pv, deleted, out[4]);
             if (deleted) {
               del[a * p + b] = del[b * p + a] = 1;
               r->sepset_len[a * p + b] = r->sepset_len[b * p + a] = ord;

@@ -3,7 +3,8 @@
     python tests/golden/make_oracle_goldens.py [name ...]
 
 Writes tests/golden/oracle_<name>.npz (adjacency, sepsets, n.edgetests and the
-canonical trace with p-values).  The GPU parity tests compare the HIP path
+canonical trace with p-values and decision margins) for each workload in
+WORKLOADS (data source, CI method, permutation replicates).  The GPU parity tests compare the HIP path
 against these; tests/test_oracle_goldens.py re-runs the oracle on the small
 workloads to show the stored traces are the oracle's current output.
 """
@@ -38,17 +39,39 @@ def save(name, data, res):
                         p=t["p_used"], deleted=t["native_edge_deleted"], margin=t["margin"])
 
 
+# name -> (data source, ci_method, replicates); permutation workloads use seed 2024
+WORKLOADS = {
+    "dev500x50": ("dev500x50", "dcc.gamma", 0),
+    "sem351x48": ("sem351x48", "dcc.gamma", 0),
+    "sem351x48_hsic_gamma": ("sem351x48", "hsic.gamma", 0),
+    "sem200x8_hsic_gamma": ("sem200x8", "hsic.gamma", 0),
+    "sem200x8_hsic_perm": ("sem200x8", "hsic.perm", 100),
+    "sem200x8_dcc_perm": ("sem200x8", "dcc.perm", 100),
+    "mixed200x14_hsic_perm": ("mixed200x14", "hsic.perm", 100),
+    "mixed200x14_dcc_perm": ("mixed200x14", "dcc.perm", 100),
+}
+PERM_SEED = 2024
+
+
+def workload_data(src):
+    if src == "dev500x50":
+        return np.load(os.path.join(HERE, "dev500x50_v2.npz"))["data"]
+    if src == "mixed200x14":
+        from fastkpc_amd.sem import generate
+        return generate(n=200, p=14, edge_prob=0.2, seed=14, link="mixed")[0]
+    return config_data(src)[0]
+
+
 def main(names):
     for name in names:
         t0 = time.time()
-        if name == "dev500x50":
-            data = np.load(os.path.join(HERE, "dev500x50_v2.npz"))["data"]
-        else:
-            data, _, _ = config_data(name)
-        res = orc.skeleton(data, alpha=0.1, m_max=data.shape[1] - 2, num_col=35, threads=8)
+        src, method, reps = WORKLOADS[name]
+        data = workload_data(src)
+        res = orc.skeleton(data, alpha=0.1, m_max=data.shape[1] - 2, num_col=35, threads=8,
+                           method=method, replicates=reps, seed=PERM_SEED)
         save(name, data, res)
         print(name, "tests", sum(res["n.edgetests"]), res["n.edgetests"], "%.1fs" % (time.time() - t0))
 
 
 if __name__ == "__main__":
-    main(sys.argv[1:] or ["dev500x50", "sem351x48"])
+    main(sys.argv[1:] or list(WORKLOADS))

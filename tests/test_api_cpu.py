@@ -88,3 +88,18 @@ def test_sem_generator_is_deterministic_and_acyclic():
         assert abs(d1[:, j].std() - 1.0) < 0.15
     d3, _ = generate(n=50, p=6, edge_prob=0.5, seed=11, link="mixed")
     assert np.isfinite(d3).all()
+
+
+def test_task_seed_matches_oracle_without_gpu():
+    # kpc_task_seed is host code: the seed the permutation kernels key on
+    import fastkpc_amd as f
+    from oracle import orc
+    rng = np.random.default_rng(1)
+    for method in ("dcc.perm", "hsic.perm"):
+        for _ in range(50):
+            v = rng.permutation(40)
+            S = sorted(int(a) for a in v[2:2 + rng.integers(0, 6)])
+            user = int(rng.integers(0, 2 ** 63))
+            got = f.task_seed(user, method, int(v[0]), int(v[1]), S)
+            assert got == orc.task_seed(user, method, int(v[0]), int(v[1]), S)
+            assert got == f.task_seed(user, method, int(v[1]), int(v[0]), S)

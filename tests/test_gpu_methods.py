@@ -1,0 +1,185 @@
+"""hsic.gamma, hsic.perm and dcc.perm on MI355X vs the oracle.
+
+Tolerances: permutation indices identical; permutation p-values equal as
+exact ratios; statistics, means and variances within 1e-9 relative;
+hsic.gamma p-values within 1e-9 relative (1e-12 absolute); skeleton
+decisions, sepsets and deletion traces exact.  Tests with a discrete decision
+within 1e-12 of its threshold (see conftest.compare_skeleton) are held to
+1e-6 relative.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, TIE_MARGIN, compare_skeleton, p_close
+
+pytestmark = pytest.mark.gpu
+
+
+def fk():
+    import fastkpc_amd
+    from fastkpc_amd import _lib
+    assert _lib.lib().kpc_device_count() >= 1, "no GPU visible"
+    return fastkpc_amd
+
+
+def orc():
+    from oracle import orc as o
+    return o
+
+
+def rel_close(a, b, rtol=1e-9):
+    return abs(a - b) <= rtol * abs(b) + 1e-300
+
+
+@pytest.mark.parametrize("n", [2, 3, 64, 200, 351, 2048])
+def test_device_permutations_equal_oracle_indices(n):
+    f, o = fk(), orc()
+    seeds = [f.task_seed(2024, "dcc.perm", 0, 5, [1, 2]), 0, 2 ** 64 - 1, 123456789]
+    R = 7
+    table = f.permutation_table(n, R, seeds)
+    for i, s in enumerate(seeds):
+        for r in range(R):
+            assert np.array_equal(table[i, r], o.permutation(s, r, n)), (n, s, r)
+
+
+def mixed_data(n=200, p=10, seed=14):
+    from fastkpc_amd.sem import generate
+    return generate(n=n, p=p, edge_prob=0.3, seed=seed, link="mixed")[0]
+
+
+@pytest.mark.parametrize("method", ["hsic.gamma", "hsic.perm", "dcc.perm"])
+@pytest.mark.parametrize("s_len", [0, 1, 2, 3])
+def test_ci_batch_vs_oracle(method, s_len):
+    f, o = fk(), orc()
+    data = mixed_data()
+    p = data.shape[1]
+    rng = np.random.default_rng(100 + s_len)
+    xs, ys, Ss = [], [], []
+    for _ in range(4):
+        perm = rng.permutation(p)
+        xs.append(int(perm[0]))
+        ys.append(int(perm[1]))
+        Ss.append(sorted(int(v) for v in perm[2:2 + s_len]))
+    S = np.array(Ss, dtype=np.int32).reshape(4, s_len)
+    out = f.ci_batch(data, xs, ys, S, num_col=35, ci_method=method, replicates=60, seed=77)
+    for i in range(4):
+        ref = o.ci_test(data, xs[i], ys[i], Ss[i], num_col=35, method=method, replicates=60,
+                        seed=77)
+        tie = ref["margin"] < TIE_MARGIN
+        if s_len:
+            for t in (xs[i], ys[i]):
+                _, gi = f.regrxons_batch(data, [t], [Ss[i]])
+                tie |= gi[0, 30] < TIE_MARGIN
+        ctx = (method, s_len, xs[i], ys[i], Ss[i], out[i], ref)
+        if tie:
+            assert abs(out[i, 1] - ref["statistic"]) <= 1e-6 * abs(ref["statistic"]), ctx
+            continue
+        assert rel_close(out[i, 1], ref["statistic"]), ctx
+        if method.endswith("perm"):
+            assert out[i, 0] == ref["p.value"], ctx
+            assert rel_close(out[i, 2], ref["mean"], 1e-9), ctx
+            assert rel_close(out[i, 3], ref["variance"], 1e-8), ctx
+        else:
+            assert p_close(out[i, 0], ref["p.value"]), ctx
+            assert rel_close(out[i, 2], ref["mean"]) and rel_close(out[i, 3], ref["variance"]), ctx
+
+
+def test_orientation_free_identity_and_twin_reuse():
+    f = fk()
+    data = mixed_data(seed=3)
+    for method in ("hsic.perm", "dcc.perm", "hsic.gamma", "dcc.gamma"):
+        out = f.ci_batch(data, [1, 6, 1], [6, 1, 6], np.array([[2, 4], [2, 4], [2, 4]], np.int32),
+                         num_col=20, ci_method=method, replicates=40, seed=5)
+        assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2]), method
+        alone = f.ci_batch(data, [6], [1], np.array([[2, 4]], np.int32), num_col=20,
+                           ci_method=method, replicates=40, seed=5)
+        assert np.array_equal(alone[0], out[0]), method
+
+
+@pytest.mark.parametrize("n,num_col,sig", [(6, 3, 1.0), (9, 8, 0.3), (120, 5, 1.0), (700, 35, 1.0),
+                                           (2048, 64, 2.0)])
+def test_hsic_gamma_edge_sizes_vs_oracle(n, num_col, sig):
+    f, o = fk(), orc()
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n)
+    y = np.sin(x) + rng.standard_normal(n)
+    data = np.column_stack([x, y])
+    out = f.ci_batch(data, [0], [1], np.zeros((1, 0), np.int32), num_col=num_col,
+                     ci_method="hsic.gamma", sig=sig)
+    ref = o.ci_test(data, 0, 1, [], num_col=num_col, method="hsic.gamma", sig=sig)
+    if ref["margin"] >= TIE_MARGIN:
+        assert rel_close(out[0, 1], ref["statistic"]), (out[0], ref)
+        assert rel_close(out[0, 2], ref["mean"]) and rel_close(out[0, 3], ref["variance"])
+        assert p_close(out[0, 0], ref["p.value"]) or not np.isfinite(ref["p.value"])
+
+
+@pytest.mark.parametrize("n,index,incl,R", [(6, 1.0, True, 10), (150, 0.5, True, 50),
+                                             (150, 1.5, False, 50), (1000, 1.0, True, 20),
+                                             (2048, 1.0, True, 3), (80, 1.0, True, 0)])
+def test_dcc_perm_edge_cases_vs_oracle(n, index, incl, R):
+    f, o = fk(), orc()
+    rng = np.random.default_rng(n + R)
+    x = rng.standard_normal(n)
+    y = 0.5 * x * x + rng.standard_normal(n)
+    data = np.column_stack([x, y])
+    out = f.ci_batch(data, [0], [1], np.zeros((1, 0), np.int32), num_col=5, ci_method="dcc.perm",
+                     replicates=R, seed=9, index=index, include_observed=incl)
+    ref = o.ci_test(data, 0, 1, [], num_col=5, method="dcc.perm", replicates=R, seed=9,
+                    index=index, include_observed=incl)
+    assert rel_close(out[0, 1], ref["statistic"]), (out[0], ref)
+    assert out[0, 0] == ref["p.value"], (out[0], ref)
+
+
+@pytest.mark.parametrize("R,incl", [(0, True), (0, False), (1, True), (257, True)])
+def test_hsic_perm_replicate_counts_vs_oracle(R, incl):
+    f, o = fk(), orc()
+    data = mixed_data(n=160, seed=8)
+    out = f.ci_batch(data, [0, 2], [3, 5], np.zeros((2, 0), np.int32), num_col=35,
+                     ci_method="hsic.perm", replicates=R, seed=31, include_observed=incl)
+    for i, (x, y) in enumerate([(0, 3), (2, 5)]):
+        ref = o.ci_test(data, x, y, [], num_col=35, method="hsic.perm", replicates=R, seed=31,
+                        include_observed=incl)
+        assert out[i, 0] == ref["p.value"], (R, incl, out[i], ref)
+        assert rel_close(out[i, 1], ref["statistic"])
+
+
+@pytest.mark.parametrize("name", ["sem200x8_hsic_gamma", "sem351x48_hsic_gamma",
+                                  "sem200x8_hsic_perm", "sem200x8_dcc_perm",
+                                  "mixed200x14_hsic_perm", "mixed200x14_dcc_perm"])
+def test_skeleton_methods_vs_oracle(name):
+    f = fk()
+    import sys
+    sys.path.insert(0, GOLDEN)
+    from make_oracle_goldens import PERM_SEED, WORKLOADS, workload_data
+    src, method, reps = WORKLOADS[name]
+    data = workload_data(src)
+    gold = np.load(os.path.join(GOLDEN, "oracle_%s.npz" % name))
+    opts = {"numCol": 35, "ci_method": method}
+    if reps:
+        opts["permutation_params"] = {"replicates": reps, "seed": PERM_SEED}
+    res = f.fastkpc_compatible_cuda_skeleton(data, 0.1, options=opts)
+    exact = method.endswith("perm")
+    rep = {}
+    compare_skeleton(res, gold, (lambda a, b: a == b) if exact else p_close, rep)
+    print(name, "vs oracle:", rep)
+    p = data.shape[1]
+    for i in range(p):
+        for j in range(p):
+            ln = gold["sepsets"][i, j, 0]
+            want = None if ln < 0 else [int(v) for v in gold["sepsets"][i, j, 1:1 + ln]]
+            assert res["sepsets"][i][j] == want, (i, j)
+
+
+def test_plain_form_equals_batched_form_permutation():
+    f = fk()
+    data = mixed_data(n=120, p=8, seed=21)
+    opts = {"numCol": 15, "ci_method": "dcc.perm",
+            "permutation_params": {"replicates": 30, "seed": 4}}
+    rb = f.fastkpc_compatible_cuda_skeleton(data, 0.1, options=opts)
+    rp = f.fastkpc_compatible_cuda_skeleton(data, 0.1, options=dict(opts, mode="plain"))
+    assert rb["n.edgetests"] == rp["n.edgetests"]
+    assert np.array_equal(rb["adjacency"], rp["adjacency"])
+    assert rb["tasks"]["S_key"] == rp["tasks"]["S_key"]
+    assert np.array_equal(rb["tasks"]["p_used"], rp["tasks"]["p_used"])

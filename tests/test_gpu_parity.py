@@ -10,7 +10,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, trace_map
+from conftest import GOLDEN, TIE_MARGIN, compare_skeleton, p_close, trace_map
 
 pytestmark = pytest.mark.gpu
 
@@ -25,13 +25,6 @@ def fk():
 def orc():
     from oracle import orc as o
     return o
-
-
-TIE_MARGIN = 1e-12
-
-
-def p_close(a, b):
-    return abs(a - b) <= 1e-9 * abs(b) + 1e-12
 
 
 def test_dcov_gamma_reference_fixture(dcov_fixture):
@@ -128,38 +121,6 @@ def test_gcv_gradient_hessian_vs_oracle_and_finite_differences(dev500, S):
         assert abs(fd - grad[0, i]) <= 1e-6 * abs(grad[0, i]) + 1e-9 * score[0]
         fdh = (gr[2 * i] - gr[2 * i + 1]) / (2 * h)
         np.testing.assert_allclose(fdh, hess[0, i], rtol=1e-4, atol=1e-6 * (np.abs(hess).max() + score[0]))
-
-
-def compare_skeleton(res, gold, tol_p, report=None):
-    """Decisions exact; p-values within tol_p, except tests whose GCV fits
-    took an optimiser decision within TIE_MARGIN (relative) of its threshold
-    on either implementation -- a rounding difference can flip such a
-    decision (e.g. accepting a final Newton step whose score change is
-    ~1e-16), so those are held to 1e-6 relative."""
-    assert res["n.edgetests"] == [int(v) for v in gold["n_edgetests"]]
-    assert np.array_equal(res["adjacency"], gold["adjacency"])
-    t = res["tasks"]
-    mine = trace_map(t["level"], t["x"], t["y"], t["S_key"], t["p_used"])
-    ref = trace_map(gold["level"], gold["x"], gold["y"], gold["S"], gold["p"])
-    gm = trace_map(t["level"], t["x"], t["y"], t["S_key"], t["gcv_margin"])
-    om = trace_map(gold["level"], gold["x"], gold["y"], gold["S"], gold["margin"])
-    assert set(mine) == set(ref)
-    tie = {k for k in ref if min(gm[k], om[k]) < TIE_MARGIN}
-    bad = [(k, mine[k], ref[k]) for k in ref if k not in tie and not tol_p(mine[k], ref[k])]
-    assert not bad, bad[:5]
-    bad_tie = [(k, mine[k], ref[k]) for k in tie if abs(mine[k] - ref[k]) > 1e-6 * abs(ref[k]) + 1e-12]
-    assert not bad_tie, bad_tie[:5]
-    if report is not None:
-        rel = {k: abs(mine[k] - ref[k]) / max(abs(ref[k]), 1e-300) for k in ref}
-        report.update(rows=len(ref), tie_rows=len(tie),
-                      tie_rows_over_1e9=sum(1 for k in tie if rel[k] > 1e-9),
-                      max_rel_non_tie=max([rel[k] for k in ref if k not in tie] or [0.0]),
-                      max_rel_all=max(rel.values()))
-    dels = [(int(a), int(b), int(c), str(d)) for a, b, c, d, e in
-            zip(t["level"], t["x"], t["y"], t["S_key"], t["native_edge_deleted"]) if e]
-    gdels = [(int(a), int(b), int(c), str(d)) for a, b, c, d, e in
-             zip(gold["level"], gold["x"], gold["y"], gold["S"], gold["deleted"]) if e]
-    assert dels == gdels
 
 
 def test_skeleton_500x50_vs_reference_fixture_and_oracle(dev500):
