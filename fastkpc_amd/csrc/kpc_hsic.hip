@@ -190,10 +190,14 @@ __global__ __launch_bounds__(kHsicThreads) void inchol_kernel(
       const bool valid = bi >= 0 && bi < n && !isnan(best);
       residue = valid ? best : -INFINITY;
       piv = valid ? bi : 0;
+      // decision margins relative to the mass rounding can move (a residue is
+      // 1 - sum g^2): stop rule residue > tol, and the pivot choice
       if (rank < k) {
-        const double stop_gap = fabs(residue - kIncholTol) / kIncholTol;
-        mg = fmin(mg, stop_gap);
-        if (residue > kIncholTol) mg = fmin(mg, (best - second) / (fabs(best) > 0.0 ? fabs(best) : 1.0));
+        mg = fmin(mg, fabs(residue - kIncholTol) / fmax(1.0 - residue, kIncholTol));
+        if (residue > kIncholTol) {
+          const double mass = fmax((1.0 - best) + (1.0 - second), 2.220446049250313e-16);
+          mg = fmin(mg, (best - second) / mass);
+        }
       }
       if (residue > kIncholTol && rank < k) {
         for (int c = tid; c < rank; c += blockDim.x) s_row[c] = G[piv + (size_t)n * c];

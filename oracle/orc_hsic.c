@@ -84,11 +84,16 @@ int orc_inchol(const double* x, int n, double sigma, double tol, int maxr, doubl
       if (diag[i] < best && diag[i] > second) second = diag[i];
     residue = best;
     pivot = bi;
+    /* Decision margins, each relative to the mass a rounding difference can
+     * move: a residue is 1 - sum_c g_c^2, so relative errors delta in the
+     * terms move it by delta * (1 - residue).  Residues still ~1 far from
+     * every pivot are therefore robust even when they sit a few ulps apart. */
     if (rank < maxr) {
-      const double stop_gap = fabs(residue - tol) / tol;
+      const double stop_gap = fabs(residue - tol) / fmax(1.0 - residue, tol);
       if (stop_gap < mg) mg = stop_gap;
       if (residue > tol) {
-        const double piv_gap = (best - second) / (fabs(best) > 0.0 ? fabs(best) : 1.0);
+        const double mass = fmax((1.0 - best) + (1.0 - second), 2.220446049250313e-16);
+        const double piv_gap = (best - second) / mass;
         if (piv_gap < mg) mg = piv_gap;
       }
     }
