@@ -147,6 +147,11 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="sem351x48")
+    ap.add_argument("--method", default="dcc.gamma",
+                    choices=["dcc.gamma", "hsic.gamma", "dcc.perm", "hsic.perm"],
+                    help="CI method (the headline metric is dcc.gamma)")
+    ap.add_argument("--replicates", type=int, default=100,
+                    help="permutation replicates for dcc.perm / hsic.perm")
     ap.add_argument("--plain-per-level", type=int, default=30)
     ap.add_argument("--cpu-baseline-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -168,7 +173,9 @@ def main():
     data, mask, cfg = config_data(args.config)
     n, p = data.shape
     k = 35
-    opts = {"numCol": k, "ci_method": "dcc.gamma", "index": 1}
+    opts = {"numCol": k, "ci_method": args.method, "index": 1}
+    if args.method.endswith("perm"):
+        opts["permutation_params"] = {"replicates": args.replicates, "seed": 2024}
     for _ in range(args.warmup):
         fk.fastkpc_compatible_cuda_skeleton(data, 0.1, options=opts)
     walls = []
@@ -194,10 +201,17 @@ def main():
     kernels = {
         "gcv": ("magic_kernel", sum(gcv_flops(n, l, s["level_fits"][l], s["level_evaluations"][l])
                                     for l in range(res["levels"]))),
-        "dcov_component": ("dcov_lanczos_kernel",
-                           component_flops(n, k, sum(s["level_components"]))),
-        "dcov_pair": ("dcov_pair_kernel", pair_flops(n, k, sum(s["level_physical"]))),
     }
+    if args.method == "dcc.gamma":
+        kernels["dcov_component"] = ("dcov_lanczos_kernel",
+                                     component_flops(n, k, sum(s["level_components"])))
+        kernels["dcov_pair"] = ("dcov_pair_kernel", pair_flops(n, k, sum(s["level_physical"])))
+    elif args.method == "dcc.perm":
+        # per replicate (and the observed sum): the upper triangle of 64 x 64
+        # tiles, 10 flops per (i, j) (two centred distances and a product)
+        ns = (n + 63) // 64
+        per = (args.replicates + 1) * ns * (ns + 1) / 2 * 64 * 64 * 10.0
+        kernels["dcov_pair"] = ("dcc_perm_kernel", per * sum(s["level_physical"]))
     dom = max(kernels, key=lambda key: stage_tot.get(key, 0.0))
     kname, flops_per_run = kernels[dom]
     ktime = stage_tot[dom] / args.steps
@@ -237,11 +251,13 @@ def main():
         }
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.method == "dcc.gamma":
         cpu = cpu_baseline(data, res, n, k, args.cpu_baseline_seconds)
 
     line = {
-        "metric": "CI tests/s + skeleton wall-s, 351x48 dcc.gamma: batched+adjoint vs plain GPU",
+        "metric": ("CI tests/s + skeleton wall-s, 351x48 dcc.gamma: batched+adjoint vs plain GPU"
+                   if args.method == "dcc.gamma" and args.config == "sem351x48" else
+                   "CI tests/s, %s %s" % (args.config, args.method)),
         "value": round(value, 2),
         "unit": "CI tests/s",
         "n_gpus": 1,
@@ -254,9 +270,11 @@ def main():
         "dtype": "f64",
         "data": "synthetic nonlinear SEM (generator fastkpc_amd/sem.py, seed %d)" % cfg["seed"],
         "config": {
-            "workload": "kernel-PC skeleton, dcc.gamma, sem351x48: n=351 p=48 edge_prob=0.08 "
-                        "noise var U[0.25,1], alpha=0.1 numCol=35 index=1 m.max=Inf",
-            "n": n, "p": p, "ci_method": "dcc.gamma", "alpha": 0.1, "numCol": k,
+            "workload": "kernel-PC skeleton, %s, %s: n=%d p=%d alpha=0.1 numCol=35 index=1 "
+                        "m.max=Inf%s" % (args.method, args.config, n, p,
+                                         " replicates=%d" % args.replicates
+                                         if args.method.endswith("perm") else ""),
+            "n": n, "p": p, "ci_method": args.method, "alpha": 0.1, "numCol": k,
             "m_max_resolved": p - 2, "form": "batched (frontier-speculative level waves)",
             "logical_tests": tests, "n_edgetests": res["n.edgetests"],
             "physical_tests": int(s["physical_tests"]), "levels": res["levels"],

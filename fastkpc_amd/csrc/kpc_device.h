@@ -10,7 +10,7 @@ constexpr int kMagicInfo = 32;
 constexpr int kPairOut = 5;
 
 struct DesignLayout {
-  long long X, QR, tau, R, piv, rS, Sp, lsp0, status, stride;
+  long long X, QR, tau, R, piv, rS, Sp, lsp0, norms, status, stride;
 };
 
 struct GeometryItem {
@@ -61,8 +61,8 @@ hipError_t launch_geometry(const GeometryItem* items, int count, int n, int q, i
                            hipStream_t stream);
 
 long long magic_scratch_doubles(int q, int m, int rk, int n);
-size_t magic_lds_bytes(int n, int q, bool aug_in_lds);
-bool magic_aug_fits_lds(int n, int q);
+size_t magic_lds_bytes(int n, int q, int m, bool aug_in_lds);
+bool magic_aug_fits_lds(int n, int q, int m);
 hipError_t launch_magic(const MagicItem* items, int count, int n, int q, int m, int rk,
                         const double* data, const double* design, double* resid, double* info,
                         double* scratch, int grid, const MagicControl& ctl, hipStream_t stream);

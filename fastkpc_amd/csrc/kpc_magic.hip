@@ -15,6 +15,7 @@
 //   probes (:1118-1346), then residuals y - X beta.
 #include "kpc_common.h"
 #include "kpc_device.h"
+#include "kpc_wave_linalg.h"
 
 namespace kpc {
 
@@ -38,6 +39,7 @@ __host__ __device__ inline DesignLayout design_layout(int n, int q, int m, int r
   L.rS = o; o += (long long)q * m * rk;
   L.Sp = o; o += (long long)m * q * q;
   L.lsp0 = o; o += m;
+  L.norms = o; o += 2 + 3 * m;
   L.status = o; o += 2;
   L.stride = (o + 31) / 32 * 32;
   return L;
@@ -171,6 +173,31 @@ __device__ int pstf2_upper(double* A, int q, int* piv, double* work, double tol,
   }
   __syncthreads();
   return rank;
+}
+
+// X = R^-1 for the upper-triangular q x q matrix R (column-major, ld ld), X
+// column-major with ld q, dinv: q doubles of scratch.  Right-looking: for
+// k = q-1 .. 0 every (row i < k, column c >= k) pair subtracts R_ik x_kc in
+// parallel, one workgroup barrier per k (the serial back substitution would
+// chain q^2/2 dependent FP64 operations).  Rows are scaled by 1/R_ii at the end.
+__device__ void upper_tri_inverse(const double* R, int ld, int q, double* X, double* dinv) {
+  for (int e = threadIdx.x; e < q * q; e += blockDim.x) X[e] = (e % q == e / q) ? 1.0 : 0.0;
+  for (int i = threadIdx.x; i < q; i += blockDim.x) dinv[i] = 1.0 / R[(size_t)i * ld + i];
+  __syncthreads();
+  for (int k = q - 1; k > 0; --k) {
+    const double dk = dinv[k];
+    const int w = q - k;
+    for (int e = threadIdx.x; e < k * w; e += blockDim.x) {
+      const int i = e % k, c = k + e / k;
+      X[(size_t)c * q + i] -= R[(size_t)k * ld + i] * (X[(size_t)c * q + k] * dk);
+    }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < q * q; e += blockDim.x) {
+    const int i = e % q, c = e / q;
+    if (i <= c) X[e] *= dinv[i];
+  }
+  __syncthreads();
 }
 
 // ------------------------------------------------------------- geometry
@@ -334,6 +361,18 @@ __global__ __launch_bounds__(kGeomThreads) void geometry_kernel(
         Sp[e] = v;
       }
       __syncthreads();
+      // largest diagonal of S_i and its smallest kept Cholesky pivot (the
+      // pivots of chol(sp_i S_i) are sp_i times these): the GCV evaluation
+      // uses them to tell when the total-penalty root would drop pivots
+      double lm = 0.0;
+      for (int r = tid; r < q; r += blockDim.x) lm = fmax(lm, Sp[(size_t)r * q + r]);
+      const double mxd = block_max(lm, red);
+      if (tid == 0) {
+        const double f = F[(size_t)(rk - 1) * dim + (rk - 1)];
+        D[DL.norms + 2 + m + s] = mxd;
+        D[DL.norms + 2 + 2 * m + s] = f * f;
+      }
+      __syncthreads();
     }
     // initial.sp
     for (int c = wid; c < q; c += nw) {
@@ -407,16 +446,46 @@ __global__ __launch_bounds__(kGeomThreads) void geometry_kernel(
       D[DL.status] = (double)status;
     }
     __syncthreads();
+    // norms for the GCV full-rank certificate: |R|_F^2, |R^-1|_F^2, |rS_i|_F^2
+    {
+      double* Rinv = ws + q;  // q x q after the q doubles of initial.sp
+      upper_tri_inverse(D + DL.R, q, q, Rinv, Rinv + (size_t)q * q);
+      double pr = 0.0, pi = 0.0;
+      for (int e = tid; e < q * q; e += blockDim.x) {
+        const int r = e % q, c = e / q;
+        if (r <= c) {
+          const double a = D[DL.R + e];
+          pr += a * a;
+          pi += Rinv[e] * Rinv[e];
+        }
+      }
+      const double nr2 = block_sum(pr, red);
+      const double ni2 = block_sum(pi, red);
+      if (tid == 0) {
+        D[DL.norms] = nr2;
+        D[DL.norms + 1] = ni2;
+      }
+      for (int i = 0; i < m; ++i) {
+        double ps = 0.0;
+        for (int e = tid; e < rk * q; e += blockDim.x) {
+          const double v = D[DL.rS + (size_t)i * rk * q + e];
+          ps += v * v;
+        }
+        const double ns = block_sum(ps, red);
+        if (tid == 0) D[DL.norms + 2 + i] = ns;
+      }
+      __syncthreads();
+    }
   }
 }
 
 // ---------------------------------------------------------------- magic
 // Doubles of the per-fit global slab ahead of the response vectors: the
-// aug/V fallback (3 q^2) and the derivative scratch; every design here has
-// m * rk <= q (tp: rank q - 2; additive: 8 m < 1 + 9 m).
+// aug/V/X4/Qm fallback (6 q^2 + 3 q) and the derivative scratch; every design
+// here has m * rk <= q (tp: rank q - 2; additive: 8 m < 1 + 9 m).
 __host__ __device__ inline long long magic_scratch_core(int q) {
   const long long mr = q;  // m * rk bound
-  long long s = 4LL * q * q + 3LL * q + (long long)q * q + 2LL * q * mr + (long long)q * q +
+  long long s = 6LL * q * q + 3LL * q + (long long)q * q + 2LL * q * mr + (long long)q * q +
                 3LL * kMaxPen * q + q + 2LL * mr + 64;
   return (s + 31) / 32 * 32;
 }
@@ -425,8 +494,9 @@ struct MagicLds {
   double* y;      // n (Q'y in the first q entries after the transform)
   double* yorig;  // n
   double* aug;    // (q + q) x q, column-major with ld = rows
-  double* V;      // q x q
+  double* V;      // q x q: SVD right vectors, or R2^-1 on the QR path
   double* X4;     // q x q: aggregate-penalty factor, then G = U1'U1
+  double* Qm;     // (q + q) x q: orthogonal factor of [R; E] on the QR path
   double* gyl;    // q: G y1
   double* projl;  // 2q: T_a' y1, T_a' G y1
   double* d;      // q singular values (by column)
@@ -445,16 +515,28 @@ struct MagicLds {
   int* ictl;      // 32
   int* piv;       // q
   double* wk;     // 2q
+  double* spv;    // 2 kMaxPen: sp_i = exp(log sp_i), then sqrt(sp_i)
+  double* My;     // m x nr derivative work (LDS-resident: read by lane loops)
+  double* Ky;     // m x nr
+  double* yK;     // m x nr
+  double* tr;     // m^2 + m: pairwise traces, then tr(K_i)
+  double* dn;     // m: d|y - Ay|^2 / d rho_i
+  double* d2n;    // m x m
+  double* syA;    // m x m: Hessian eigen work
+  double* syZ;    // m x m: eigenvectors
+  double* syw;    // m: eigenvalues
 };
 
-__host__ __device__ inline size_t magic_lds_doubles(int n, int q, bool aug_in_lds) {
+__host__ __device__ inline size_t magic_lds_doubles(int n, int q, int m, bool aug_in_lds) {
   (void)n;
-  size_t s = (aug_in_lds ? (size_t)4 * q * q + 3 * (size_t)q : 0) + 4 * (size_t)q +
+  size_t s = (aug_in_lds ? (size_t)6 * q * q + 3 * (size_t)q : 0) + 4 * (size_t)q +
              7 * kMaxPen + kMaxPen * kMaxPen + 48 + (size_t)q + 16;
+  s += 2 * kMaxPen + 3 * (size_t)m * q + 4 * (size_t)m * m + 3 * (size_t)m + 8;
   return s;
 }
 
-__device__ MagicLds carve_magic(double* base, int n, int q, bool aug_in_lds, double* gscratch) {
+__device__ MagicLds carve_magic(double* base, int n, int q, int m, bool aug_in_lds,
+                                double* gscratch) {
   MagicLds L;
   double* p = base;
   // response vectors live in the global slab after the derivative scratch
@@ -464,9 +546,10 @@ __device__ MagicLds carve_magic(double* base, int n, int q, bool aug_in_lds, dou
   L.aug = big;
   L.V = big + 2 * (size_t)q * q;
   L.X4 = big + 3 * (size_t)q * q;
-  L.gyl = big + 4 * (size_t)q * q;
+  L.Qm = big + 4 * (size_t)q * q;
+  L.gyl = big + 6 * (size_t)q * q;
   L.projl = L.gyl + q;
-  if (aug_in_lds) p += 4 * (size_t)q * q + 3 * (size_t)q;
+  if (aug_in_lds) p += 6 * (size_t)q * q + 3 * (size_t)q;
   L.d = p; p += q;
   L.y1 = p; p += q;
   L.lsp = p; p += kMaxPen;
@@ -480,6 +563,16 @@ __device__ MagicLds carve_magic(double* base, int n, int q, bool aug_in_lds, dou
   L.red = p; p += 16;
   L.dctl = p; p += 32;
   L.wk = p; p += 2 * q;
+  L.spv = p; p += 2 * kMaxPen;
+  L.My = p; p += (size_t)m * q;
+  L.Ky = p; p += (size_t)m * q;
+  L.yK = p; p += (size_t)m * q;
+  L.tr = p; p += (size_t)m * m + m;
+  L.dn = p; p += m;
+  L.d2n = p; p += (size_t)m * m;
+  L.syA = p; p += (size_t)m * m;
+  L.syZ = p; p += (size_t)m * m;
+  L.syw = p; p += m + 8;
   int* ip = reinterpret_cast<int*>(p);
   L.ord = ip; ip += q;
   L.piv = ip; ip += q;
@@ -501,7 +594,7 @@ struct MagicScratch {
 
 __device__ MagicScratch carve_magic_scratch(double* base, int q, int m, int rk) {
   MagicScratch S;
-  double* p = base + 4 * (size_t)q * q + 3 * (size_t)q;  // after the global LDS fallback
+  double* p = base + 6 * (size_t)q * q + 3 * (size_t)q;  // after the global LDS fallback
   S.agg = p; p += (size_t)q * q;
   S.T = p; p += (size_t)q * m * rk;
   S.GT = p; p += (size_t)q * m * rk;
@@ -657,30 +750,124 @@ __device__ void jacobi_svd(double* A, int rows, int q, double* V, double* d, int
   __syncthreads();
 }
 
-// GCV core at log(sp) = lspv.  Results: dctl[0]=rss dctl[1]=edf dctl[2]=score
-// ictl[0]=nr ictl[1]=rank (aggregate root) ictl[2]=status, y1, d, ord.
-__device__ void magic_eval(const MagicLds& L, const MagicScratch& S, const double* D,
-                           const DesignLayout& DL, int n, int q, int m, const double* lspv,
-                           double ynorm2, double rank_tol) {
-  const int tid = threadIdx.x;
-  const int lane = tid & (KPC_WAVE - 1);
-  const int wid = tid / KPC_WAVE;
-  const int nw = blockDim.x / KPC_WAVE;
-  // aggregate penalty, accumulated penalty by penalty, factorised in X4
-  double* agg = L.X4;
-  double mult[kMaxPen];
-  for (int i = 0; i < m; ++i) mult[i] = exp(lspv[i]);
-  for (int e = tid; e < q * q; e += blockDim.x) {
-    double a = 0.0;
-    for (int i = 0; i < m; ++i) a += D[DL.Sp + (size_t)i * q * q + e] * mult[i];
-    agg[e] = a;
+// 8-lane sum (aligned groups of 8 lanes) by DPP moves: xor 1, xor 2 inside
+// quads, then the half-row mirror.
+template <int kCtrl>
+__device__ __forceinline__ double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), kCtrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), kCtrl, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double sum8(double v) {
+  v += dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_move<0x141>(v);  // row_half_mirror
+  return v;
+}
+
+// Householder reflector (LAPACK dlarfg) of column jj of W over row jj and the
+// E rows q..q+mr-1, by one aligned group of 8 lanes: v overwrites the E part,
+// beta the diagonal, tau[jj] the scale.
+__device__ __forceinline__ void group_reflector(double* W, int rows, int q, int mr, int jj,
+                                                double* tau, int sub) {
+  double* w = W + (size_t)jj * rows;
+  double s2 = 0.0;
+  for (int k = sub; k < mr; k += 8) s2 += w[q + k] * w[q + k];
+  s2 = sum8(s2);
+  wave_sync();
+  const double alpha = w[jj];
+  double t = 0.0, scal = 0.0, beta = alpha;
+  if (s2 > 0.0) {
+    // r = 1/|(alpha, x)|: beta = -sign(alpha)/r, tau = 1 + |alpha| r,
+    // 1/(alpha - beta) = 1 / (sign(alpha) (|alpha| + 1/r))
+    const double h = fma(alpha, alpha, s2);
+    const double r = nrsq(h);
+    const double nrm = h * r;
+    beta = -copysign(nrm, alpha);
+    t = fma(fabs(alpha), r, 1.0);
+    scal = copysign(nrcp(fabs(alpha) + nrm), alpha);
   }
+  for (int k = sub; k < mr; k += 8) w[q + k] *= scal;
+  if (sub == 0) {
+    w[jj] = beta;
+    tau[jj] = t;
+  }
+}
+
+// Structured Householder QR of W = [R; E] (rows = q + mr, column-major, ld
+// rows; R upper triangular): step j reflects row j of the top block against
+// the E rows only, so the top block stays triangular and every reflector has
+// length 1 + mr.  Groups of 8 lanes own one column each; the group that
+// updates column j + 1 builds its reflector right away, so each step costs
+// one workgroup barrier.  R2 overwrites the top block, reflector tails the E
+// rows, tau[0..q).
+__device__ void stacked_qr(double* W, int rows, int q, int mr, double* tau) {
+  const int tid = threadIdx.x;
+  constexpr int kG = 8;
+  const int ncg = blockDim.x / kG;
+  const int sub = tid % kG, cg = tid / kG;
+  if (cg == 0) group_reflector(W, rows, q, mr, 0, tau, sub);
   __syncthreads();
-  long long tp0 = wall_clock64();
-  const int rank = pstf2_upper(agg, q, L.piv, L.wk, -1.0, L.ictl + 16, L.dctl + 10);
-  long long tp1 = wall_clock64();
+  for (int j = 0; j < q - 1; ++j) {
+    const double tj = tau[j];
+    const double* vj = W + (size_t)j * rows + q;
+    for (int c0 = j + 1; c0 < q; c0 += ncg) {
+      const int c = c0 + cg;
+      double* wc = W + (size_t)(c < q ? c : j) * rows;
+      double dot = 0.0;
+      if (c < q) {
+        if (sub == 0) dot = wc[j];
+        for (int k = sub; k < mr; k += kG) dot += vj[k] * wc[q + k];
+      }
+      dot = sum8(dot) * tj;
+      if (c < q && tj != 0.0) {
+        if (sub == 0) wc[j] -= dot;
+        for (int k = sub; k < mr; k += kG) wc[q + k] -= dot * vj[k];
+      }
+      if (c == j + 1) group_reflector(W, rows, q, mr, j + 1, tau, sub);
+    }
+    __syncthreads();
+  }
+}
+
+// Q1 (top q rows of the orthogonal factor of [R; E]) from the reflectors of
+// stacked_qr: Qm = H_0 ... H_{q-1} [I; 0] (LAPACK dorg2r order).
+__device__ void stacked_q1(const double* W, int rows, int q, int mr, const double* tau,
+                           double* Qm) {
+  const int tid = threadIdx.x;
+  constexpr int kG = 8;
+  const int ncg = blockDim.x / kG;
+  const int sub = tid % kG, cg = tid / kG;
+  for (int e = tid; e < rows * q; e += blockDim.x) Qm[e] = (e % rows == e / rows) ? 1.0 : 0.0;
+  __syncthreads();
+  for (int j = q - 1; j >= 0; --j) {
+    const double tj = tau[j];
+    if (tj == 0.0) continue;
+    const double* v = W + (size_t)j * rows + q;
+    for (int c0 = j; c0 < q; c0 += ncg) {
+      const int c = c0 + cg;
+      double* qc = Qm + (size_t)(c < q ? c : j) * rows;
+      double dot = 0.0;
+      if (c < q) {
+        if (sub == 0) dot = qc[j];
+        for (int k = sub; k < mr; k += kG) dot += v[k] * qc[q + k];
+      }
+      dot = sum8(dot) * tj;
+      if (c < q) {
+        if (sub == 0) qc[j] -= dot;
+        for (int k = sub; k < mr; k += kG) qc[q + k] -= dot * v[k];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Copy [R; root] into the augmented buffer (rows = q + rank).
+__device__ void build_augmented(const MagicLds& L, const double* D, const DesignLayout& DL, int q,
+                                int rank, const double* root) {
+  const int tid = threadIdx.x;
   const int rows = q + rank;
-  // augmented [R; root]
   for (int e = tid; e < rows * q; e += blockDim.x) {
     const int r = e % rows, c = e / rows;
     L.aug[e] = r < q ? D[DL.R + (size_t)c * q + r] : 0.0;
@@ -688,17 +875,142 @@ __device__ void magic_eval(const MagicLds& L, const MagicScratch& S, const doubl
   __syncthreads();
   for (int e = tid; e < rank * q; e += blockDim.x) {
     const int r = e % rank, col = e / rank;
-    if (r <= col) L.aug[(size_t)L.piv[col] * rows + q + r] = agg[(size_t)col * q + r];
+    if (r <= col) L.aug[(size_t)L.piv[col] * rows + q + r] = root[(size_t)col * q + r];
   }
   __syncthreads();
-  long long tp2 = wall_clock64();
-  jacobi_svd(L.aug, rows, q, L.V, L.d, L.ord, L.ictl);
-  long long tp3 = wall_clock64();
-  if (tid == 0) {
-    L.dctl[26] += (double)(tp1 - tp0);
-    L.dctl[27] += (double)(tp3 - tp2);
+}
+
+// GCV core at log(sp) = lspv.  Results: dctl[0]=rss dctl[1]=edf dctl[2]=score
+// dctl[6]=rank-decision margin, ictl[0]=nr ictl[1]=rank (aggregate root)
+// ictl[2]=status ictl[13]=basis (1: QR, 0: SVD), y1.
+//
+// mgcv magic takes the SVD [R; E] = U D V' and keeps the singular values above
+// d_max sqrt(eps) (the numerical rank nr).  When the rank is certifiably full
+// -- d_min / d_max >= 1 / (|R2|_F |R2^-1|_F) > 2 rank_tol for the triangular
+// factor R2 of [R; E] -- every quantity magic derives from U1 (first q rows of
+// U) and V D^-1 is invariant under the orthogonal change of basis U1 = Q1 W,
+// V D^-1 = R2^-1 W, so the structured QR (q reflectors of length 1 + rank)
+// replaces the Jacobi SVD; otherwise the SVD decides the rank exactly.
+__device__ void magic_eval(const MagicLds& L, const MagicScratch& S, const double* D,
+                           const DesignLayout& DL, int n, int q, int m, int rk,
+                           const double* lspv, double ynorm2, double rank_tol) {
+  const int tid = threadIdx.x;
+  const int lane = tid & (KPC_WAVE - 1);
+  const int wid = tid / KPC_WAVE;
+  const int nw = blockDim.x / KPC_WAVE;
+  const double* spv = L.spv;
+  if (tid < m) {
+    const double e = exp(lspv[tid]);
+    L.spv[tid] = e;
+    L.spv[kMaxPen + tid] = sqrt(e);
   }
-  {
+  __syncthreads();
+  // The total-penalty root of mgcv (pivoted Cholesky, default tolerance
+  // q eps max diag) drops every pivot <= dstop.  The penalties act on disjoint
+  // coefficient blocks, so the pivots of block i are sp_i times the pivots of
+  // chol(S_i); when even the smallest of them clears dstop by a wide margin
+  // the root has full rank m rk and [R; sqrt(sp_i) rS_i'] has the same Gram
+  // matrix as [R; root], i.e. the same singular values and U1, V, D.
+  const int mr = m * rk;
+  double dmax = 0.0;
+  for (int i = 0; i < m; ++i) dmax = fmax(dmax, spv[i] * D[DL.norms + 2 + m + i]);
+  const double dstop = (double)q * kEps * dmax;
+  bool stacked = true;
+  for (int i = 0; i < m; ++i) stacked = stacked && spv[i] * D[DL.norms + 2 + 2 * m + i] > 1e3 * dstop;
+  long long tp0 = wall_clock64(), tp1 = tp0, tp2 = tp0;
+  int rows, erows = mr;
+  if (tid == 0) L.ictl[15] = 0;
+  bool qr_ok = false;
+  double lb = 0.0;
+  if (stacked) {
+    // full-rank certificate, stage 1 from the design norms: sigma_min([R; E])
+    // >= sigma_min(R) >= 1 / |R^-1|_F, sigma_max^2 <= |R|_F^2 + sum_i sp_i |rS_i|_F^2
+    double emax = D[DL.norms];
+    for (int i = 0; i < m; ++i) emax += spv[i] * D[DL.norms + 2 + i];
+    lb = 1.0 / sqrt(D[DL.norms + 1] * emax);
+    qr_ok = isfinite(lb) && lb > 2.0 * rank_tol;
+    rows = q + mr;
+    for (int e = tid; e < rows * q; e += blockDim.x) {
+      const int r = e % rows, c = e / rows;
+      L.aug[e] = r < q ? D[DL.R + (size_t)c * q + r]
+                       : spv[kMaxPen + (r - q) / rk] * D[DL.rS + (size_t)(r - q) * q + c];
+    }
+    __syncthreads();
+  } else {
+    // root with dropped pivots: exactly mgcv's [R; root]
+    double* agg = L.X4;
+    for (int e = tid; e < q * q; e += blockDim.x) {
+      double a = 0.0;
+      for (int i = 0; i < m; ++i) a += D[DL.Sp + (size_t)i * q * q + e] * spv[i];
+      agg[e] = a;
+    }
+    __syncthreads();
+    if (wid == 0) {
+      const int r = pstf2_upper_wave(agg, q, L.piv, L.wk);
+      if (lane == 0) L.ictl[14] = r;
+    }
+    __syncthreads();
+    erows = L.ictl[14];
+    rows = q + erows;
+    build_augmented(L, D, DL, q, erows, agg);
+  }
+  stacked_qr(L.aug, rows, q, erows, L.wk);
+  if (!qr_ok) {
+    // stage 2 from the factor itself: 1 / (|R2|_F |R2^-1|_F)
+    upper_tri_inverse(L.aug, rows, q, L.V, L.wk + q);
+    double pr = 0.0, pi = 0.0;
+    for (int e = tid; e < q * q; e += blockDim.x) {
+      const int r = e % q, c = e / q;
+      if (r <= c) {
+        const double a = L.aug[(size_t)c * rows + r], b = L.V[e];
+        pr += a * a;
+        pi += b * b;
+      }
+    }
+    const double nr2 = block_sum(pr, L.red);
+    const double ni2 = block_sum(pi, L.red);
+    lb = 1.0 / sqrt(nr2 * ni2);
+    qr_ok = isfinite(lb) && lb > 2.0 * rank_tol;
+    if (qr_ok && tid == 0) L.ictl[15] = 1;  // R2 inverted already
+  }
+  if (qr_ok) stacked_q1(L.aug, rows, q, erows, L.wk, L.Qm);
+  if (qr_ok) {
+    // ---- QR path on [R; sqrt(sp_i) rS_i'] (same Gram matrix as [R; E])
+    if (tid == 0) {
+      L.ictl[0] = q;
+      L.ictl[1] = erows;
+      L.ictl[13] = 1;
+      L.dctl[6] = lb / rank_tol - 1.0;
+    }
+    // y1 = Q1' y0
+    for (int c = wid; c < q; c += nw) {
+      const double* u = L.Qm + (size_t)c * rows;
+      double sacc = 0.0;
+      for (int r = lane; r < q; r += KPC_WAVE) sacc += u[r] * L.y[r];
+      sacc = wave_sum(sacc);
+      if (lane == 0) L.y1[c] = sacc;
+    }
+    __syncthreads();
+  } else {
+    // ---- exact path: aggregate penalty, pivoted Cholesky root, Jacobi SVD
+    double* agg = L.X4;
+    for (int e = tid; e < q * q; e += blockDim.x) {
+      double a = 0.0;
+      for (int i = 0; i < m; ++i) a += D[DL.Sp + (size_t)i * q * q + e] * spv[i];
+      agg[e] = a;
+    }
+    __syncthreads();
+    if (wid == 0) {
+      const int r = pstf2_upper_wave(agg, q, L.piv, L.wk);
+      if (lane == 0) L.ictl[14] = r;
+    }
+    __syncthreads();
+    const int rank = L.ictl[14];
+    tp1 = wall_clock64();
+    rows = q + rank;
+    build_augmented(L, D, DL, q, rank, agg);
+    tp2 = wall_clock64();
+    jacobi_svd(L.aug, rows, q, L.V, L.d, L.ord, L.ictl);
     const double d0 = L.d[L.ord[0]];
     const double thr = d0 * rank_tol;
     int cnt = 0;
@@ -713,25 +1025,34 @@ __device__ void magic_eval(const MagicLds& L, const MagicScratch& S, const doubl
     if (tid == 0) {
       L.ictl[0] = (int)nrs;
       L.ictl[1] = rank;
+      L.ictl[13] = 0;
       L.dctl[6] = rmin;
     }
+    __syncthreads();
+    const int nr = L.ictl[0];
+    // y1 = U1' y0
+    for (int c = wid; c < nr; c += nw) {
+      const double* u = L.aug + (size_t)L.ord[c] * rows;
+      double sacc = 0.0;
+      for (int r = lane; r < q; r += KPC_WAVE) sacc += u[r] * L.y[r];
+      sacc = wave_sum(sacc);
+      if (lane == 0) L.y1[c] = sacc;
+    }
+    __syncthreads();
   }
-  __syncthreads();
+  long long tp3 = wall_clock64();
+  if (tid == 0) {
+    L.dctl[26] += (double)(tp1 - tp0);
+    L.dctl[27] += (double)(tp3 - tp2);
+  }
   const int nr = L.ictl[0];
-  // y1 = U1' y0
-  for (int c = wid; c < nr; c += nw) {
-    const double* u = L.aug + (size_t)L.ord[c] * rows;
-    double s = 0.0;
-    for (int r = lane; r < q; r += KPC_WAVE) s += u[r] * L.y[r];
-    s = wave_sum(s);
-    if (lane == 0) L.y1[c] = s;
-  }
-  __syncthreads();
+  const bool qr = L.ictl[13] != 0;
+  rows = q + L.ictl[1];
   double p_yaay = 0.0, p_edf = 0.0;
   for (int r = tid; r < q; r += blockDim.x) {
     double v = 0.0;
     for (int c = 0; c < nr; ++c) {
-      const double u = L.aug[(size_t)L.ord[c] * rows + r];
+      const double u = qr ? L.Qm[(size_t)c * rows + r] : L.aug[(size_t)L.ord[c] * rows + r];
       v += u * L.y1[c];
       p_edf += u * u;
     }
@@ -763,28 +1084,37 @@ __device__ void magic_derivs(const MagicLds& L, const MagicScratch& S, const dou
   const int rows = q + L.ictl[1];
   const int mr = m * rk;
   const long long td0 = wall_clock64();
+  const bool qr = L.ictl[13] != 0;
   // G = U1' U1 (nr x nr) first: after it the augmented buffer is free and
-  // holds T and GT, so the whole derivative working set stays in LDS
+  // holds T and GT, so the whole derivative working set stays in LDS.  On the
+  // QR basis U1 -> Q1 and D^-1 V1' -> R2^-T (same traces and products).
   double* G = L.X4;
   for (int e = tid; e < nr * nr; e += blockDim.x) {
     const int i = e % nr, j = e / nr;
-    const double* ui = L.aug + (size_t)L.ord[i] * rows;
-    const double* uj = L.aug + (size_t)L.ord[j] * rows;
+    const double* ui = qr ? L.Qm + (size_t)i * rows : L.aug + (size_t)L.ord[i] * rows;
+    const double* uj = qr ? L.Qm + (size_t)j * rows : L.aug + (size_t)L.ord[j] * rows;
     double s = 0.0;
     for (int r = 0; r < q; ++r) s += ui[r] * uj[r];
     G[e] = s;
   }
   __syncthreads();
+  if (qr && !L.ictl[15]) upper_tri_inverse(L.aug, rows, q, L.V, L.wk);  // before T overwrites R2
   double* T = L.aug;                          // nr x mr (mr <= q)
   double* GT = L.aug + (size_t)q * q;         // nr x mr
-  // T[c][a] = (V1' rS)[c][a] / d_c
+  // T[c][a] = (V1' rS)[c][a] / d_c, or (R2^-T rS)[c][a]
   for (int e = tid; e < nr * mr; e += blockDim.x) {
     const int c = e % nr, a = e / nr;
-    const double* v = L.V + (size_t)L.ord[c] * q;
     const double* rs = D + DL.rS + (size_t)a * q;
     double s = 0.0;
-    for (int k = 0; k < q; ++k) s += v[k] * rs[k];
-    T[(size_t)a * nr + c] = s / L.d[L.ord[c]];
+    if (qr) {
+      const double* x = L.V + (size_t)c * q;  // column c of R2^-1
+      for (int k = 0; k <= c; ++k) s += x[k] * rs[k];
+      T[(size_t)a * nr + c] = s;
+    } else {
+      const double* v = L.V + (size_t)L.ord[c] * q;
+      for (int k = 0; k < q; ++k) s += v[k] * rs[k];
+      T[(size_t)a * nr + c] = s / L.d[L.ord[c]];
+    }
   }
   __syncthreads();
   // GT = G T, gy = G y1
@@ -818,15 +1148,15 @@ __device__ void magic_derivs(const MagicLds& L, const MagicScratch& S, const dou
       my += t * L.projl[2 * (i * rk + a)];
       ky += t * L.projl[2 * (i * rk + a) + 1];
     }
-    S.My[(size_t)i * nr + o] = my;
-    S.Ky[(size_t)i * nr + o] = ky;
+    L.My[(size_t)i * nr + o] = my;
+    L.Ky[(size_t)i * nr + o] = ky;
   }
   __syncthreads();
   for (int e = tid; e < m * nr; e += blockDim.x) {
     const int i = e / nr, o = e % nr;
     double s = 0.0;
-    for (int k = 0; k < nr; ++k) s += G[(size_t)k * nr + o] * S.My[(size_t)i * nr + k];
-    S.yK[(size_t)i * nr + o] = s;
+    for (int k = 0; k < nr; ++k) s += G[(size_t)k * nr + o] * L.My[(size_t)i * nr + k];
+    L.yK[(size_t)i * nr + o] = s;
   }
   __syncthreads();
   // pairwise traces tr(M_j K_i) = tr((T_j'T_i)(T_i' G T_j)), one workgroup
@@ -844,7 +1174,7 @@ __device__ void magic_derivs(const MagicLds& L, const MagicScratch& S, const dou
         part += x * y;
       }
       const double tsum = block_sum(part, L.red);
-      if (tid == 0) S.agg[j * m + i] = tsum;
+      if (tid == 0) L.tr[j * m + i] = tsum;
     }
     double part = 0.0;
     for (int e = tid; e < rk * nr; e += blockDim.x) {
@@ -852,34 +1182,50 @@ __device__ void magic_derivs(const MagicLds& L, const MagicScratch& S, const dou
       part += T[(size_t)(i * rk + a) * nr + c] * GT[(size_t)(i * rk + a) * nr + c];
     }
     const double tr = block_sum(part, L.red);
-    if (tid == 0) S.agg[m * m + i] = tr;
+    if (tid == 0) L.tr[m * m + i] = tr;
+  }
+  __syncthreads();
+  // d|y-Ay|^2 terms: one lane per i (dnorm) and per pair j <= i (d2n)
+  const int npairs = m * (m + 1) / 2;
+  for (int e = tid; e < m + npairs; e += blockDim.x) {
+    if (e < m) {
+      const int i = e;
+      double nd = 0.0;
+      for (int c = 0; c < nr; ++c) nd += L.y1[c] * (L.My[(size_t)i * nr + c] - L.Ky[(size_t)i * nr + c]);
+      L.dn[i] = 2.0 * exp(lspv[i]) * nd;
+    } else {
+      int i = 0, rem = e - m;
+      while (rem > i) {
+        rem -= i + 1;
+        ++i;
+      }
+      const int j = rem;
+      double nv = 0.0;
+      for (int c = 0; c < nr; ++c) {
+        const double myi = L.My[(size_t)i * nr + c], myj = L.My[(size_t)j * nr + c];
+        nv += myi * L.Ky[(size_t)j * nr + c] + myj * L.Ky[(size_t)i * nr + c] - 2.0 * myi * myj +
+              L.yK[(size_t)i * nr + c] * myj;
+      }
+      nv *= 2.0 * exp(lspv[i] + lspv[j]);
+      L.d2n[j * m + i] = nv;
+      L.d2n[i * m + j] = nv;
+    }
   }
   __syncthreads();
   if (tid == 0) {
-    double dnorm[kMaxPen], ddelta[kMaxPen], d2n[kMaxPen * kMaxPen], d2d[kMaxPen * kMaxPen];
+    double* dnorm = L.dn;
+    double* d2n = L.d2n;
+    double* ddelta = L.syw;   // m (free until the eigen-decomposition)
+    double* d2d = L.syA;      // m x m (likewise)
     for (int i = 0; i < m; ++i) {
       const double li = exp(lspv[i]);
-      ddelta[i] = li * S.agg[m * m + i];
+      ddelta[i] = li * L.tr[m * m + i];
       for (int j = 0; j <= i; ++j) {
-        const double dv = -2.0 * exp(lspv[i] + lspv[j]) * S.agg[j * m + i];
+        const double dv = -2.0 * exp(lspv[i] + lspv[j]) * L.tr[j * m + i];
         d2d[j * m + i] = dv;
         d2d[i * m + j] = dv;
       }
       d2d[i * m + i] += ddelta[i];
-      double nd = 0.0;
-      for (int c = 0; c < nr; ++c) nd += L.y1[c] * (S.My[(size_t)i * nr + c] - S.Ky[(size_t)i * nr + c]);
-      dnorm[i] = 2.0 * li * nd;
-      for (int j = 0; j <= i; ++j) {
-        double nv = 0.0;
-        for (int c = 0; c < nr; ++c) {
-          const double myi = S.My[(size_t)i * nr + c], myj = S.My[(size_t)j * nr + c];
-          nv += myi * S.Ky[(size_t)j * nr + c] + myj * S.Ky[(size_t)i * nr + c] - 2.0 * myi * myj +
-                S.yK[(size_t)i * nr + c] * myj;
-        }
-        nv *= 2.0 * exp(lspv[i] + lspv[j]);
-        d2n[j * m + i] = nv;
-        d2n[i * m + j] = nv;
-      }
       d2n[i * m + i] += dnorm[i];
     }
     const double rss = L.dctl[0], edf = L.dctl[1];
@@ -901,9 +1247,9 @@ __device__ void magic_derivs(const MagicLds& L, const MagicScratch& S, const dou
   __syncthreads();
 }
 
-// Symmetric eigen of the small Hessian (cyclic Jacobi), lane-serial.
-__device__ void small_syev(const double* Hin, int m, double* w, double* Z) {
-  double A[kMaxPen * kMaxPen];
+// Symmetric eigen of the small Hessian (cyclic Jacobi), lane-serial on LDS
+// buffers (A: m x m work, Z: eigenvectors, w: eigenvalues).
+__device__ void small_syev(const double* Hin, int m, double* w, double* Z, double* A) {
   for (int i = 0; i < m * m; ++i) A[i] = Hin[i];
   for (int i = 0; i < m * m; ++i) Z[i] = (i % m == i / m) ? 1.0 : 0.0;
   for (int sweep = 0; sweep < 100; ++sweep) {
@@ -949,7 +1295,7 @@ __global__ __launch_bounds__(kMagicThreads) void magic_kernel(
   extern __shared__ double lds_dyn[];
   const int tid = threadIdx.x;
   double* gs = scratch + (size_t)blockIdx.x * scratch_stride;
-  MagicLds L = carve_magic(lds_dyn, n, q, aug_in_lds != 0, gs);
+  MagicLds L = carve_magic(lds_dyn, n, q, m, aug_in_lds != 0, gs);
   MagicScratch S = carve_magic_scratch(gs, q, m, rk);
   const DesignLayout DL = design_layout(n, q, m, rk);
   for (int item = blockIdx.x; item < count; item += gridDim.x) {
@@ -980,7 +1326,7 @@ __global__ __launch_bounds__(kMagicThreads) void magic_kernel(
     if (tid < m) L.lsp[tid] = D[DL.lsp0 + tid];
     __syncthreads();
     const int max_half = m == 1 ? ctl.max_half_single : ctl.max_half_multi;
-    magic_eval(L, S, D, DL, n, q, m, L.lsp, ynorm2, ctl.rank_tol);
+    magic_eval(L, S, D, DL, n, q, m, rk, L.lsp, ynorm2, ctl.rank_tol);
     magic_derivs(L, S, D, DL, n, q, m, rk, L.lsp);
     // optimizer state (lane 0 decides, LDS broadcasts)
     double* st = L.dctl;  // st[3]=min_score st[4]=score_red
@@ -1016,7 +1362,7 @@ __global__ __launch_bounds__(kMagicThreads) void magic_kernel(
             for (int i = 0; i < m; ++i) L.trial[i] = L.lsp[i] + L.step[i];
           }
           __syncthreads();
-          magic_eval(L, S, D, DL, n, q, m, L.trial, ynorm2, ctl.rank_tol);
+          magic_eval(L, S, D, DL, n, q, m, rk, L.trial, ynorm2, ctl.rank_tol);
           if (tid == 0) {
             ic[9] += 1;
             ic[12] = L.dctl[2] < st[3];
@@ -1055,8 +1401,9 @@ __global__ __launch_bounds__(kMagicThreads) void magic_kernel(
           ic[4] = conv;
         }
         for (int i = 0; i < m; ++i) L.grad[i] = L.cgrad[i];
-        double w[kMaxPen], Z[kMaxPen * kMaxPen];
-        small_syev(L.hess, m, w, Z);
+        double* w = L.syw;
+        double* Z = L.syZ;
+        small_syev(L.hess, m, w, Z, L.syA);
         int sd = 0;
         for (int i = 0; i < m; ++i) {
           if (w[i] < 0.0) sd = 1;
@@ -1096,7 +1443,7 @@ __global__ __launch_bounds__(kMagicThreads) void magic_kernel(
             L.trial[c] += dir * ctl.probe_step;
           }
           __syncthreads();
-          magic_eval(L, S, D, DL, n, q, m, L.trial, ynorm2, ctl.rank_tol);
+          magic_eval(L, S, D, DL, n, q, m, rk, L.trial, ynorm2, ctl.rank_tol);
           if (tid == 0) {
             ic[9] += 1;
             ic[12] = L.dctl[2] < st[3];
@@ -1112,14 +1459,20 @@ __global__ __launch_bounds__(kMagicThreads) void magic_kernel(
       }
     }
     // final fit at the selected log(sp)
-    magic_eval(L, S, D, DL, n, q, m, L.lsp, ynorm2, ctl.rank_tol);
+    magic_eval(L, S, D, DL, n, q, m, rk, L.lsp, ynorm2, ctl.rank_tol);
     {
       const int nr = L.ictl[0];
+      const bool qr = L.ictl[13] != 0;
+      if (qr && !L.ictl[15]) upper_tri_inverse(L.aug, q + L.ictl[1], q, L.V, L.wk);
       double* theta = S.gy;  // q entries (gy sized >= q)
       for (int r = tid; r < q; r += blockDim.x) {
         double v = 0.0;
-        for (int c = 0; c < nr; ++c)
-          v += L.V[(size_t)L.ord[c] * q + r] * (L.y1[c] / L.d[L.ord[c]]);
+        if (qr) {
+          for (int c = r; c < q; ++c) v += L.V[(size_t)c * q + r] * L.y1[c];  // R2^-1 y1
+        } else {
+          for (int c = 0; c < nr; ++c)
+            v += L.V[(size_t)L.ord[c] * q + r] * (L.y1[c] / L.d[L.ord[c]]);
+        }
         theta[r] = v;
       }
       __syncthreads();
@@ -1155,16 +1508,16 @@ __global__ __launch_bounds__(kMagicThreads) void magic_kernel(
   }
 }
 
-size_t magic_lds_bytes(int n, int q, bool aug_in_lds) {
-  return magic_lds_doubles(n, q, aug_in_lds) * sizeof(double);
+size_t magic_lds_bytes(int n, int q, int m, bool aug_in_lds) {
+  return magic_lds_doubles(n, q, m, aug_in_lds) * sizeof(double);
 }
 
-bool magic_aug_fits_lds(int n, int q) {
-  return magic_lds_bytes(n, q, true) <= 150 * 1024;
+bool magic_aug_fits_lds(int n, int q, int m) {
+  return magic_lds_bytes(n, q, m, true) <= 150 * 1024;
 }
 
 long long geometry_scratch_doubles(int q) {
-  long long s = 29LL * 29 + 4 * 29 + 64 + 2LL * q + 64;
+  long long s = 29LL * 29 + 4 * 29 + 64 + 2LL * q + 64 + (long long)q * q + 2LL * q;
   return (s + 31) / 32 * 32;
 }
 
@@ -1183,8 +1536,8 @@ hipError_t launch_magic(const MagicItem* items, int count, int n, int q, int m, 
                         const double* data, const double* design, double* resid, double* info,
                         double* scratch, int grid, const MagicControl& ctl, hipStream_t stream) {
   if (count <= 0) return hipSuccess;
-  const bool in_lds = magic_aug_fits_lds(n, q);
-  const size_t lds = magic_lds_bytes(n, q, in_lds);
+  const bool in_lds = magic_aug_fits_lds(n, q, m);
+  const size_t lds = magic_lds_bytes(n, q, m, in_lds);
   hipError_t err = hipFuncSetAttribute((const void*)magic_kernel,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (err != hipSuccess) return err;
@@ -1209,7 +1562,7 @@ __global__ __launch_bounds__(kMagicThreads) void gcv_eval_kernel(
   extern __shared__ double lds_dyn[];
   const int tid = threadIdx.x;
   double* gs = scratch + (size_t)blockIdx.x * scratch_stride;
-  MagicLds L = carve_magic(lds_dyn, n, q, aug_in_lds != 0, gs);
+  MagicLds L = carve_magic(lds_dyn, n, q, m, aug_in_lds != 0, gs);
   MagicScratch S = carve_magic_scratch(gs, q, m, rk);
   const DesignLayout DL = design_layout(n, q, m, rk);
   for (int item = blockIdx.x; item < count; item += gridDim.x) {
@@ -1232,7 +1585,7 @@ __global__ __launch_bounds__(kMagicThreads) void gcv_eval_kernel(
     const double ynorm2 = block_sum(part, L.red);
     if (tid < m) L.lsp[tid] = lsp_in[(size_t)item * m + tid];
     __syncthreads();
-    magic_eval(L, S, D, DL, n, q, m, L.lsp, ynorm2, rank_tol);
+    magic_eval(L, S, D, DL, n, q, m, rk, L.lsp, ynorm2, rank_tol);
     magic_derivs(L, S, D, DL, n, q, m, rk, L.lsp);
     double* o = out + (size_t)item * (1 + m + m * m);
     if (tid == 0) {
@@ -1249,8 +1602,8 @@ hipError_t launch_gcv_eval(const MagicItem* items, int count, int n, int q, int 
                            double* out, double* scratch, int grid, double rank_tol,
                            hipStream_t stream) {
   if (count <= 0) return hipSuccess;
-  const bool in_lds = magic_aug_fits_lds(n, q);
-  const size_t lds = magic_lds_bytes(n, q, in_lds);
+  const bool in_lds = magic_aug_fits_lds(n, q, m);
+  const size_t lds = magic_lds_bytes(n, q, m, in_lds);
   hipError_t err = hipFuncSetAttribute((const void*)gcv_eval_kernel,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (err != hipSuccess) return err;
