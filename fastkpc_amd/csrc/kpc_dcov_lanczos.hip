@@ -9,9 +9,17 @@
 // classical Gram-Schmidt passes) keeps the Krylov basis orthogonal, and the
 // iteration stops when every selected Ritz pair has residual
 // |beta_m s_m| <= tol * ||T||; the extreme eigenvalues of the Lanczos
-// tridiagonal T come from Sturm bisection and their eigenvectors from
-// inverse iteration (one lane per eigenpair).  Kpcalg's own eigs() call
-// (RSpectra, tol 1e-10) is a restarted Lanczos of the same operator.
+// tridiagonal T come from Sturm-count multisection (a group of lanes per
+// eigenvalue, each probing one point of the bracket per round) and their
+// eigenvectors from twisted factorisations (one lane per eigenpair, pivots in
+// LDS).  The convergence checks are scheduled from the observed geometric
+// decay of the worst Ritz residual; a beta below 64 eps |T| is a breakdown
+// (invariant Krylov space, exact Ritz pairs) and ends the iteration.  Kpcalg's own eigs() call (RSpectra, tol 1e-10) is a
+// restarted Lanczos of the same operator.
+//
+// Dependent FP64 operations cost tens of cycles each on CDNA4, so every
+// serial recurrence here uses hardware reciprocal estimates refined by
+// Newton steps and every long dot product runs several accumulators.
 #include "kpc_common.h"
 #include "kpc_device.h"
 
@@ -19,80 +27,168 @@ namespace kpc {
 
 constexpr int kLzThreads = 512;
 
+// Doubles of LDS for the twisted-factorisation pivots (2 m per eigenpair).
+__host__ __device__ inline int lanczos_ivw(int mmax) {
+  return 4096 > 6 * mmax + 64 ? 4096 : 6 * mmax + 64;
+}
+
+// 1/x from the hardware estimate and one (Sturm counts) or two Newton steps.
+__device__ __forceinline__ double lz_rcp1(double x) {
+  const double r = __builtin_amdgcn_rcp(x);
+  return fma(r, fma(-x, r, 1.0), r);
+}
+__device__ __forceinline__ double lz_rcp(double x) {
+  const double r = lz_rcp1(x);
+  return fma(r, fma(-x, r, 1.0), r);
+}
+
+// Number of eigenvalues of the tridiagonal (d, e2 = e^2) strictly below x.
 __device__ int lz_sturm(const double* d, const double* e2, int n, double x, double pivmin) {
   int cnt = 0;
   double q = d[0] - x;
   if (fabs(q) < pivmin) q = -pivmin;
   cnt += q < 0.0;
   for (int i = 1; i < n; ++i) {
-    q = d[i] - x - e2[i - 1] / q;
+    q = (d[i] - x) - e2[i - 1] * lz_rcp1(q);
     if (fabs(q) < pivmin) q = -pivmin;
     cnt += q < 0.0;
   }
   return cnt;
 }
 
-__device__ double lz_bisect(const double* d, const double* e2, int n, int idx, double lo, double hi,
-                            double pivmin) {
-  for (int it = 0; it < 200; ++it) {
-    const double mid = 0.5 * (lo + hi);
-    if (mid <= lo || mid >= hi) break;
-    if (hi - lo <= 2.0 * 2.220446049250313e-16 * fmax(fabs(lo), fabs(hi)) + pivmin) break;
-    if (lz_sturm(d, e2, n, mid, pivmin) > idx) hi = mid; else lo = mid;
+// Eigenvector of the tridiagonal (al, be) for the eigenvalue th by a twisted
+// factorization (as in LAPACK dlar1v / MRRR): forward pivots d_i of
+// T - th I = L D L', backward pivots r_i of U R U', twist index
+// k = argmin |gamma_k| with gamma_k = d_k + r_k - (al_k - th); then x_k = 1,
+// x_i = -be_i x_{i+1} / d_i upward and x_{i+1} = -be_i x_i / r_{i+1} downward,
+// which satisfies every row except k (residual gamma_k).  rd, rr: m doubles
+// of workspace (reciprocal pivots).  Writes the unit vector to z when z is
+// non-null; returns its last component.
+__device__ double lz_twisted(const double* al, const double* be, int m, double th, double pivmin,
+                             double* rd, double* rr, double* z) {
+  // forward and backward pivot sweeps, interleaved for ILP
+  double d = al[0] - th;
+  double r = al[m - 1] - th;
+  for (int i = 0; i < m; ++i) {
+    if (fabs(d) < pivmin) d = -pivmin;
+    if (fabs(r) < pivmin) r = -pivmin;
+    const double id = lz_rcp(d), ir = lz_rcp(r);
+    rd[i] = id;
+    rr[m - 1 - i] = ir;
+    if (i + 1 < m) {
+      d = (al[i + 1] - th) - be[i] * be[i] * id;
+      r = (al[m - 2 - i] - th) - be[m - 2 - i] * be[m - 2 - i] * ir;
+    }
   }
-  return 0.5 * (lo + hi);
+  // twist: |gamma_k| = |1/rd_k + 1/rr_k - (al_k - th)|
+  int kt = 0;
+  double gmin = 1e308;
+  for (int i = 0; i < m; ++i) {
+    const double g = fabs(lz_rcp(rd[i]) + lz_rcp(rr[i]) - (al[i] - th));
+    if (g < gmin) {
+      gmin = g;
+      kt = i;
+    }
+  }
+  double n0 = 1.0, n1 = 0.0, x = 1.0, last = kt == m - 1 ? 1.0 : 0.0;
+  if (z) z[kt] = 1.0;
+  for (int i = kt - 1; i >= 0; --i) {
+    x = -be[i] * x * rd[i];
+    if (z) z[i] = x;
+    n1 += x * x;
+  }
+  x = 1.0;
+  for (int i = kt; i < m - 1; ++i) {
+    x = -be[i] * x * rr[i + 1];
+    if (z) z[i + 1] = x;
+    n0 += x * x;
+  }
+  if (kt < m - 1) last = x;
+  const double inv = 1.0 / sqrt(n0 + n1);
+  if (z)
+    for (int i = 0; i < m; ++i) z[i] *= inv;
+  return last * inv;
 }
 
-// Inverse iteration on the m x m tridiagonal (d, e) for eigenvalue lam;
-// ws: 5m doubles; z (m) receives the unit eigenvector.
-__device__ void lz_invit(const double* d, const double* e, int m, double lam, double tnorm,
-                         double* ws, double* z) {
-  double* dl = ws;
-  double* dm = ws + m;
-  double* du = ws + 2 * m;
-  double* du2 = ws + 3 * m;
-  int* ip = reinterpret_cast<int*>(ws + 4 * m);
-  const double pert = 2.220446049250313e-16 * tnorm;
-  for (int i = 0; i < m; ++i) dm[i] = d[i] - lam;
-  for (int i = 0; i < m - 1; ++i) { dl[i] = e[i]; du[i] = e[i]; du2[i] = 0.0; }
-  for (int i = 0; i < m - 1; ++i) {
-    if (fabs(dm[i]) >= fabs(dl[i])) {
-      ip[i] = 0;
-      if (dm[i] == 0.0) dm[i] = pert;
-      const double f = dl[i] / dm[i];
-      dl[i] = f;
-      dm[i + 1] -= f * du[i];
-    } else {
-      ip[i] = 1;
-      const double f = dm[i] / dl[i];
-      dm[i] = dl[i];
-      dl[i] = f;
-      const double t = du[i];
-      du[i] = dm[i + 1];
-      dm[i + 1] = t - f * dm[i + 1];
-      if (i < m - 2) { du2[i] = du[i + 1]; du[i + 1] = -f * du[i + 1]; }
-    }
+// Sturm multisection for the candidate eigenvalues of the tridiagonal
+// (al, e2): candidate c has ascending index idx(c) = c < nlow ? c : hs + c - nlow.
+// G lanes per candidate probe G interior points of its bracket per round (the
+// bracket shrinks by G + 1); rounds stop when every bracket is below
+// rtol max(|lo|, |hi|) + pivmin.  Brackets persist in clo/chi (init: start
+// from [lo0, hi0]); cval receives the midpoints.  All threads must call it.
+__device__ void lz_multisect(const double* al, const double* e2, int m, double pivmin, int ncand,
+                             int nlow, int hs, double rtol, bool init, double lo0, double hi0,
+                             double* clo, double* chi, double* cval) {
+  const int tid = threadIdx.x;
+  const int lane = tid & (KPC_WAVE - 1);
+  int G = ncand > 0 ? (int)blockDim.x / ncand : 1;
+  G = G > 8 ? 8 : (G < 1 ? 1 : G);
+  int Gp = 1;
+  while (Gp * 2 <= G) Gp *= 2;  // power of two: groups align with lanes
+  G = Gp;
+  const int c = tid / G, g = tid % G;
+  const bool own = c < ncand;
+  const int idx = own ? (c < nlow ? c : hs + (c - nlow)) : 0;
+  double lo = lo0, hi = hi0;
+  if (own && !init) {
+    lo = clo[c];
+    hi = chi[c];
   }
-  if (dm[m - 1] == 0.0) dm[m - 1] = pert;
-  for (int i = 0; i < m; ++i) z[i] = 1.0 + 0.01 * (double)((i * 7919) % 101) / 101.0;
-  for (int it = 0; it < 3; ++it) {
-    for (int i = 0; i < m - 1; ++i) {
-      if (ip[i] == 0) {
-        z[i + 1] -= dl[i] * z[i];
+  for (int round = 0; round < 400; ++round) {
+    const double w = hi - lo;
+    const bool done = !own || w <= rtol * fmax(fabs(lo), fabs(hi)) + pivmin;
+    if (__syncthreads_and(done ? 1 : 0)) break;
+    const double step = w / (double)(G + 1);
+    int flag = 0;
+    if (own && !done) {
+      const double xg = lo + step * (double)(g + 1);
+      flag = lz_sturm(al, e2, m, xg, pivmin) > idx;
+    }
+    // first flagged point of the group (count(x) > idx moves the upper end)
+    const unsigned long long bal = __ballot(flag);
+    const int gbase = lane - g;
+    const unsigned bits = (unsigned)((bal >> gbase) & ((1ULL << G) - 1ULL));
+    if (own && !done) {
+      const int first = bits ? __builtin_ctz(bits) : G;
+      const double nlo = first == 0 ? lo : lo + step * (double)first;
+      const double nhi = first == G ? hi : lo + step * (double)(first + 1);
+      if (nhi > nlo) {
+        lo = nlo;
+        hi = nhi;
       } else {
-        const double t = z[i];
-        z[i] = z[i + 1];
-        z[i + 1] = t - dl[i] * z[i];
+        hi = lo;  // no representable interior point left
       }
     }
-    z[m - 1] /= dm[m - 1];
-    if (m >= 2) z[m - 2] = (z[m - 2] - du[m - 2] * z[m - 1]) / dm[m - 2];
-    for (int i = m - 3; i >= 0; --i) z[i] = (z[i] - du[i] * z[i + 1] - du2[i] * z[i + 2]) / dm[i];
-    double nrm = 0.0;
-    for (int i = 0; i < m; ++i) nrm += z[i] * z[i];
-    nrm = 1.0 / sqrt(nrm);
-    for (int i = 0; i < m; ++i) z[i] *= nrm;
   }
+  if (own && g == 0) {
+    clo[c] = lo;
+    chi[c] = hi;
+    cval[c] = 0.5 * (lo + hi);
+  }
+  __syncthreads();
+}
+
+// Top-kk candidates by |value| (ties toward the higher ascending index),
+// lane 0: sval/sidx receive values and candidate slots.
+__device__ void lz_select(const double* cval, int ncand, int nlow, int hs, int kk, double* sval,
+                          int* sidx, int* used) {
+  if (threadIdx.x == 0) {
+    for (int c = 0; c < ncand; ++c) used[c] = 0;
+    for (int s2 = 0; s2 < kk; ++s2) {
+      int best = -1;
+      for (int c = 0; c < ncand; ++c) {
+        if (used[c]) continue;
+        if (best < 0) { best = c; continue; }
+        const double a = fabs(cval[c]), b = fabs(cval[best]);
+        const int ic = c < nlow ? c : hs + (c - nlow), ib = best < nlow ? best : hs + (best - nlow);
+        if (a > b || (a == b && ic > ib)) best = c;
+      }
+      sval[s2] = cval[best];
+      sidx[s2] = best;
+      used[best] = 1;
+    }
+  }
+  __syncthreads();
 }
 
 __global__ __launch_bounds__(kLzThreads) void dcov_lanczos_kernel(
@@ -101,12 +197,12 @@ __global__ __launch_bounds__(kLzThreads) void dcov_lanczos_kernel(
     double* __restrict__ comp_vecs, double* __restrict__ comp_dsum,
     double* __restrict__ comp_moment, int* __restrict__ comp_status, double* __restrict__ scratch,
     long long scratch_stride, int mmax, double tol) {
-  extern __shared__ double lds_dyn[];
+  extern __shared__ double2 lds_v2[];
+  double* lds_dyn = reinterpret_cast<double*>(lds_v2);
   const int nl = (n + 63) & ~63;
   const int ml = (mmax + 64) & ~63;
-  double* r = lds_dyn;                 // n
-  double* qv = r + nl;                 // current Lanczos vector
-  double* z = qv + nl;                 // residual vector
+  double2* rq = lds_v2;                // n: (residual, current Lanczos vector)
+  double* z = lds_dyn + 2 * nl;        // residual vector
   double* al = z + nl;                 // ml
   double* be = al + ml;                // ml
   double* e2 = be + ml;                // ml
@@ -118,24 +214,27 @@ __global__ __launch_bounds__(kLzThreads) void dcov_lanczos_kernel(
   int* cidx = reinterpret_cast<int*>(sres + kMaxNumCol);  // 2 kMaxNumCol
   int* sidx = cidx + 2 * kMaxNumCol;                      // kMaxNumCol
   int* ictl = sidx + kMaxNumCol;                          // 8
+  double* clo = reinterpret_cast<double*>(ictl + 8);      // 2 kMaxNumCol brackets
+  double* chi = clo + 2 * kMaxNumCol;                     // 2 kMaxNumCol
+  double* ivl = chi + 2 * kMaxNumCol;                     // inverse-iteration workspace
+  const int ivw = lanczos_ivw(mmax);
   const int tid = threadIdx.x;
   const int lane = tid & (KPC_WAVE - 1);
   const int wid = tid / KPC_WAVE;
   const int nw = blockDim.x / KPC_WAVE;
   double* Q = scratch + (size_t)blockIdx.x * scratch_stride;   // n x (mmax + 1)
   double* S = Q + (size_t)n * (mmax + 1);                      // k x mmax (Ritz coefficients)
-  double* ivws = S + (size_t)k * mmax;                         // k x 5 mmax
   for (int item = blockIdx.x; item < count; item += gridDim.x) {
     const double* x = src_base + src_off[item];
     const int slot = dst_slot[item];
-    for (int i = tid; i < n; i += blockDim.x) r[i] = x[i];
+    for (int i = tid; i < n; i += blockDim.x) rq[i].x = x[i];
     __syncthreads();
     // sum(D) in double-double
     dd acc{0.0, 0.0};
     for (int i = tid; i < n; i += blockDim.x) {
-      const double ri = r[i];
+      const double ri = rq[i].x;
       dd rowacc{0.0, 0.0};
-      for (int j = 0; j < n; ++j) rowacc = dd_add(rowacc, dd{fabs(ri - r[j]), 0.0});
+      for (int j = 0; j < n; ++j) rowacc = dd_add(rowacc, dd{fabs(ri - rq[j].x), 0.0});
       acc = dd_add(acc, rowacc);
     }
     acc = block_sum_dd(acc, red);
@@ -147,7 +246,7 @@ __global__ __launch_bounds__(kLzThreads) void dcov_lanczos_kernel(
       for (int i = 0; i < n; ++i) {
         st = (st * 106UL + 1283UL) % 6075UL;
         const double v = (double)st / 6075.0 - 0.5;
-        qv[i] = v;
+        z[i] = v;
         nrm += v * v;
       }
       red[8] = 1.0 / sqrt(nrm);
@@ -156,58 +255,126 @@ __global__ __launch_bounds__(kLzThreads) void dcov_lanczos_kernel(
     const double inv0 = red[8];
     __syncthreads();
     for (int i = tid; i < n; i += blockDim.x) {
-      qv[i] *= inv0;
-      Q[i] = qv[i];
+      const double v = z[i] * inv0;
+      rq[i].y = v;
+      Q[i] = v;
     }
     __syncthreads();
     int m_final = mmax;
+    int next_check = k + 1, prev_m = 0;
+    double prev_worst = 0.0;
+    double tmax = 0.0;  // running max of |alpha_j| + |beta_j| (scale of T)
     if (tid == 0) { ictl[1] = 0; ictl[2] = 2; }
     __syncthreads();
     for (int j = 0; j < mmax; ++j) {
-      // z = D q_j, recomputed from r
+      // z = D q_j recomputed from the residual, eight accumulators per row
       double part = 0.0;
       for (int i = tid; i < n; i += blockDim.x) {
-        const double ri = r[i];
-        double s = 0.0;
-        for (int l = 0; l < n; ++l) s += fabs(ri - r[l]) * qv[l];
-        z[i] = s;
-        part += qv[i] * s;
+        const double ri = rq[i].x;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0, a5 = 0.0, a6 = 0.0, a7 = 0.0;
+        int l = 0;
+        for (; l + 8 <= n; l += 8) {
+          const double2 v0 = rq[l], v1 = rq[l + 1], v2 = rq[l + 2], v3 = rq[l + 3];
+          const double2 v4 = rq[l + 4], v5 = rq[l + 5], v6 = rq[l + 6], v7 = rq[l + 7];
+          a0 += fabs(ri - v0.x) * v0.y;
+          a1 += fabs(ri - v1.x) * v1.y;
+          a2 += fabs(ri - v2.x) * v2.y;
+          a3 += fabs(ri - v3.x) * v3.y;
+          a4 += fabs(ri - v4.x) * v4.y;
+          a5 += fabs(ri - v5.x) * v5.y;
+          a6 += fabs(ri - v6.x) * v6.y;
+          a7 += fabs(ri - v7.x) * v7.y;
+        }
+        for (; l < n; ++l) a0 += fabs(ri - rq[l].x) * rq[l].y;
+        const double sacc = ((a0 + a1) + (a2 + a3)) + ((a4 + a5) + (a6 + a7));
+        z[i] = sacc;
+        part += rq[i].y * sacc;
       }
       const double aj = block_sum(part, red);
       if (tid == 0) al[j] = aj;
       const double bprev = j > 0 ? be[j - 1] : 0.0;
       const double* qp = j > 0 ? Q + (size_t)(j - 1) * n : Q;
-      for (int i = tid; i < n; i += blockDim.x) z[i] -= aj * qv[i] + bprev * qp[i];
-      __syncthreads();
+      part = 0.0;
+      for (int i = tid; i < n; i += blockDim.x) {
+        const double v = z[i] - (aj * rq[i].y + bprev * qp[i]);
+        z[i] = v;
+        part += v * v;
+      }
+      double zn2 = block_sum(part, red);
+      // classical Gram-Schmidt against Q[0..j], repeated while the norm drops
+      // below 1/sqrt(2) of its value before the pass (DGKS, at most twice); a
+      // wave forms eight coefficients at once (lanes over rows, coalesced),
+      // then every row subtracts its combination with four accumulators
       for (int pass = 0; pass < 2; ++pass) {
-        for (int b = wid; b <= j; b += nw) {
-          const double* qb = Q + (size_t)b * n;
-          double s = 0.0;
-          for (int i = lane; i < n; i += KPC_WAVE) s += z[i] * qb[i];
-          s = wave_sum(s);
-          if (lane == 0) cg[b] = s;
+        for (int b0 = wid * 8; b0 <= j; b0 += nw * 8) {
+          double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0, c4 = 0.0, c5 = 0.0, c6 = 0.0, c7 = 0.0;
+          const int nb = min(8, j + 1 - b0);
+          const double* qb = Q + (size_t)b0 * n;
+          for (int i = lane; i < n; i += KPC_WAVE) {
+            const double zi = z[i];
+            c0 += zi * qb[i];
+            if (nb > 1) c1 += zi * qb[(size_t)n + i];
+            if (nb > 2) c2 += zi * qb[(size_t)2 * n + i];
+            if (nb > 3) c3 += zi * qb[(size_t)3 * n + i];
+            if (nb > 4) c4 += zi * qb[(size_t)4 * n + i];
+            if (nb > 5) c5 += zi * qb[(size_t)5 * n + i];
+            if (nb > 6) c6 += zi * qb[(size_t)6 * n + i];
+            if (nb > 7) c7 += zi * qb[(size_t)7 * n + i];
+          }
+          c0 = wave_sum(c0);
+          c1 = wave_sum(c1);
+          c2 = wave_sum(c2);
+          c3 = wave_sum(c3);
+          c4 = wave_sum(c4);
+          c5 = wave_sum(c5);
+          c6 = wave_sum(c6);
+          c7 = wave_sum(c7);
+          if (lane == 0) {
+            cg[b0] = c0;
+            if (nb > 1) cg[b0 + 1] = c1;
+            if (nb > 2) cg[b0 + 2] = c2;
+            if (nb > 3) cg[b0 + 3] = c3;
+            if (nb > 4) cg[b0 + 4] = c4;
+            if (nb > 5) cg[b0 + 5] = c5;
+            if (nb > 6) cg[b0 + 6] = c6;
+            if (nb > 7) cg[b0 + 7] = c7;
+          }
         }
         __syncthreads();
         for (int i = tid; i < n; i += blockDim.x) {
-          double s = 0.0;
-          for (int b = 0; b <= j; ++b) s += cg[b] * Q[(size_t)b * n + i];
-          z[i] -= s;
+          double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+          int b = 0;
+          for (; b + 4 <= j + 1; b += 4) {
+            s0 += cg[b] * Q[(size_t)b * n + i];
+            s1 += cg[b + 1] * Q[(size_t)(b + 1) * n + i];
+            s2 += cg[b + 2] * Q[(size_t)(b + 2) * n + i];
+            s3 += cg[b + 3] * Q[(size_t)(b + 3) * n + i];
+          }
+          for (; b <= j; ++b) s0 += cg[b] * Q[(size_t)b * n + i];
+          z[i] -= (s0 + s1) + (s2 + s3);
         }
-        __syncthreads();
+        part = 0.0;
+        for (int i = tid; i < n; i += blockDim.x) part += z[i] * z[i];
+        const double after = block_sum(part, red);
+        const bool again = after < 0.5 * zn2;
+        zn2 = after;
+        if (!again) break;
       }
-      part = 0.0;
-      for (int i = tid; i < n; i += blockDim.x) part += z[i] * z[i];
-      const double bj = sqrt(block_sum(part, red));
+      const double bj = sqrt(zn2);
       if (tid == 0) be[j] = bj;
-      const double inv = bj > 1e-300 ? 1.0 / bj : 0.0;
+      // breakdown: the Krylov space is invariant to working precision (the
+      // Ritz pairs are exact); continuing would amplify rounding noise by 1/beta
+      tmax = fmax(tmax, fabs(aj) + bj);
+      const bool breakdown = !(bj > 64.0 * 2.220446049250313e-16 * tmax) || bj <= 1e-300;
+      const double inv = breakdown ? 0.0 : 1.0 / bj;
       for (int i = tid; i < n; i += blockDim.x) {
         const double v = z[i] * inv;
-        qv[i] = v;
+        rq[i].y = v;
         Q[(size_t)(j + 1) * n + i] = v;
       }
       __syncthreads();
       const int m = j + 1;
-      const bool check = (m >= k + 1 && ((m - (k + 1)) % 5 == 0)) || m == mmax || bj <= 1e-300;
+      const bool check = m >= next_check || m == mmax || breakdown;
       if (!check) continue;
       // extreme eigenvalues of T (ascending indices {0..k-1} U {max(k, m-k)..m-1})
       for (int i = tid; i < m - 1; i += blockDim.x) e2[i] = be[i] * be[i];
@@ -231,45 +398,63 @@ __global__ __launch_bounds__(kLzThreads) void dcov_lanczos_kernel(
       const int nlow = kk;
       const int hs = (m - kk) > nlow ? (m - kk) : nlow;
       const int ncand = nlow + (m - hs);
-      for (int c = tid; c < ncand; c += blockDim.x) {
-        const int idx = c < nlow ? c : hs + (c - nlow);
-        cidx[c] = idx;
-        cval[c] = lz_bisect(al, e2, m, idx, lo0, hi0, pivmin);
-      }
-      __syncthreads();
-      if (tid == 0) {
-        for (int s = 0; s < kk; ++s) {
-          int best = -1;
-          for (int c = 0; c < ncand; ++c) {
-            if (cidx[c] < 0) continue;
-            if (best < 0) { best = c; continue; }
-            const double a = fabs(cval[c]), b = fabs(cval[best]);
-            if (a > b || (a == b && cidx[c] > cidx[best])) best = c;
+      // Ritz values to full precision, eigenvectors of T by twisted
+      // factorisations (pivot workspace in LDS, batches of eigenpairs), exact
+      // Ritz residuals |beta_m s_m|
+      lz_multisect(al, e2, m, pivmin, ncand, nlow, hs, 2.0 * 2.220446049250313e-16, true, lo0,
+                   hi0, clo, chi, cval);
+      lz_select(cval, ncand, nlow, hs, kk, sval, sidx, cidx);
+      {
+        const int per = 2 * m;
+        const int B = ivw / per > 0 ? ivw / per : 1;
+        for (int c0 = 0; c0 < kk; c0 += B) {
+          const int c = c0 + tid;
+          if (tid < B && c < kk) {
+            double* wsl = ivl + (size_t)tid * per;
+            const double last = lz_twisted(al, be, m, sval[c], pivmin, wsl, wsl + m, nullptr);
+            sres[c] = fabs(bj * last);
           }
-          sval[s] = cval[best];
-          sidx[s] = cidx[best];
-          cidx[best] = -1;
+          __syncthreads();
         }
       }
-      __syncthreads();
-      for (int c = tid; c < kk; c += blockDim.x) {
-        double* sv = S + (size_t)c * mmax;
-        lz_invit(al, be, m, sval[c], tn, ivws + (size_t)c * 5 * mmax, sv);
-        sres[c] = fabs(bj * sv[m - 1]);
-      }
-      __syncthreads();
+      const double tnorm = fmax(fabs(lo0), fabs(hi0));
       if (tid == 0) {
         double worst = 0.0;
         for (int c = 0; c < kk; ++c) worst = fmax(worst, sres[c]);
-        const double tnorm = fmax(fabs(lo0), fabs(hi0));
         const int conv = kk == k && worst <= tol * tnorm;
-        ictl[0] = conv || m == mmax || bj <= 1e-300;
+        ictl[0] = conv || m == mmax || breakdown;
         ictl[1] = kk;
         // status: 0 converged, 1 Krylov space exhausted (exact), 2 step cap hit
-        ictl[2] = conv ? 0 : (bj <= 1e-300 ? 1 : 2);
+        ictl[2] = conv ? 0 : (breakdown ? 1 : 2);
+        red[12] = worst;
+        // next check from the geometric decay of the worst residual
+        const double ratio = worst / (tol * tnorm);
+        int gap = 5;
+        if (prev_m > 0 && prev_worst > worst && worst > 0.0 && ratio > 1.0) {
+          const double rate = log(prev_worst / worst) / (double)(m - prev_m);
+          gap = (int)ceil(log(ratio) / rate);
+          gap = gap < 2 ? 2 : (gap > 25 ? 25 : gap);
+        }
+        ictl[3] = m + gap;
       }
       __syncthreads();
+      if (ictl[0]) {
+        // final Ritz coefficient vectors into S
+        const int per = 2 * m;
+        const int B = ivw / per > 0 ? ivw / per : 1;
+        for (int c0 = 0; c0 < kk; c0 += B) {
+          const int c = c0 + tid;
+          if (tid < B && c < kk) {
+            double* wsl = ivl + (size_t)tid * per;
+            lz_twisted(al, be, m, sval[c], pivmin, wsl, wsl + m, S + (size_t)c * mmax);
+          }
+          __syncthreads();
+        }
+      }
       if (ictl[0]) { m_final = m; break; }
+      prev_m = m;
+      prev_worst = red[12];
+      next_check = ictl[3];
     }
     // fewer than k eigenpairs when the Krylov space was exhausted
     for (int c = ictl[1] + tid; c < k; c += blockDim.x) {
@@ -323,12 +508,13 @@ int lanczos_mmax(int n) { return n < 1024 ? n : 1024; }
 size_t dcov_lanczos_lds_bytes(int n, int mmax) {
   const int nl = (n + 63) & ~63;
   const int ml = (mmax + 64) & ~63;
-  return (size_t)(3 * nl + 4 * ml + 16 + 4 * kMaxNumCol) * sizeof(double) +
+  return (size_t)(3 * nl + 4 * ml + 16 + 4 * kMaxNumCol + 4 * kMaxNumCol + lanczos_ivw(mmax)) *
+             sizeof(double) +
          (size_t)(3 * kMaxNumCol + 8) * sizeof(int);
 }
 
 long long dcov_lanczos_scratch_doubles(int n, int k, int mmax) {
-  long long s = (long long)n * (mmax + 1) + (long long)k * mmax + (long long)k * 5 * mmax + 64;
+  long long s = (long long)n * (mmax + 1) + (long long)k * mmax + 64;
   return (s + 31) / 32 * 32;
 }
 
