@@ -10,8 +10,8 @@ namespace kpc {
 
 constexpr int kMaxN = 2048;      // samples per column
 constexpr int kMaxNumCol = 64;   // dcov/hsic truncation rank
-constexpr int kMaxQ = 128;       // coefficient dimension handled in LDS
-constexpr int kMaxPen = 16;      // penalties per design (|S| for additive)
+constexpr int kMaxQ = 576;       // coefficient dimension (1 + 9 |S| with |S| <= 62)
+constexpr int kMaxPen = 64;      // penalties per design (|S| for additive)
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

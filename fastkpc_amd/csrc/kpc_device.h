@@ -10,12 +10,12 @@ constexpr int kMagicInfo = 32;
 constexpr int kPairOut = 5;
 
 struct DesignLayout {
-  long long X, QR, tau, R, piv, rS, Sp, lsp0, norms, status, stride;
+  long long X, QR, tau, R, piv, rS, lsp0, norms, status, stride;
 };
 
 struct GeometryItem {
   int design_slot;
-  int smooth_slot[16];
+  int smooth_slot[64];
 };
 
 struct MagicItem {

@@ -195,8 +195,8 @@ class Engine {
     }
     if (level > 0) {
       if (q_ >= n_) throw KpcError(-4, "model coefficient dimension must be less than n");
-      if (q_ > kpc::kMaxQ - 1 || m_ > kpc::kMaxPen)
-        throw KpcError(-4, "conditioning set too large for this build (|S| <= 14)");
+      if (q_ > kpc::kMaxQ || m_ > kpc::kMaxPen)
+        throw KpcError(-4, "conditioning set too large (|S| <= 62)");
       if (m_ * rk_ > q_) throw KpcError(-4, "penalty layout exceeds the GCV scratch bound");
       layout_ = kpc::host_design_layout(n_, q_, m_, rk_);
     }
@@ -333,7 +333,7 @@ class Engine {
     hip_check(hipEventRecord(ev_[1], stream_), "event");
     if (!geo_items.empty()) {
       const long long stride = kpc::geometry_scratch_doubles(q_);
-      int grid = std::min((int)geo_items.size(), 2048);
+      int grid = std::min((int)geo_items.size(), grid_limit(stride, 2048));
       scratch2_.reserve((size_t)grid * stride);
       const bool uni = level_ > 2;
       hip_check(kpc::launch_geometry(d_geo_items_.ptr, (int)geo_items.size(), n_, q_, m_, dim_, rk_,
@@ -345,7 +345,7 @@ class Engine {
     hip_check(hipEventRecord(ev_[2], stream_), "event");
     if (!fit_items.empty()) {
       const long long stride = kpc::magic_scratch_doubles(q_, m_, rk_, n_);
-      int grid = std::min((int)fit_items.size(), 65535);
+      int grid = std::min((int)fit_items.size(), grid_limit(stride, 65535));
       scratch3_.reserve((size_t)grid * stride);
       hip_check(kpc::launch_magic(d_fit_items_.ptr, (int)fit_items.size(), n_, q_, m_, rk_,
                                   d_data_, design_.ptr, resid_.ptr, rinfo_.ptr, scratch3_.ptr,
@@ -521,7 +521,7 @@ class Engine {
     }
     if (!geo_items.empty()) {
       const long long stride = kpc::geometry_scratch_doubles(q_);
-      int grid = std::min((int)geo_items.size(), 2048);
+      int grid = std::min((int)geo_items.size(), grid_limit(stride, 2048));
       scratch2_.reserve((size_t)grid * stride);
       hip_check(kpc::launch_geometry(d_geo_items_.ptr, (int)geo_items.size(), n_, q_, m_, dim_, rk_,
                                      uni ? usmooth_X_.ptr : lsmooth_X_.ptr,
@@ -534,6 +534,7 @@ class Engine {
     return slots;
   }
   double* design_ptr() { return design_.ptr; }
+  int slab_grid(long long stride, int cap) const { return grid_limit(stride, cap); }
   int q() const { return q_; }
   int m() const { return m_; }
   int rk() const { return rk_; }
@@ -1078,7 +1079,7 @@ int kpc_regrxons_batch(const double* data, int n, int p, int count, const int* t
     DevArray<kpc::MagicItem> d_items;
     d_items.upload(items, eng.stream());
     const long long stride = kpc::magic_scratch_doubles(eng.q(), eng.m(), eng.rk(), n);
-    const int grid = std::min(count, 8192);
+    const int grid = std::min(count, eng.slab_grid(stride, 8192));
     eng.scratch3().reserve((size_t)grid * stride);
     hip_check(kpc::launch_magic(d_items.ptr, count, n, eng.q(), eng.m(), eng.rk(), eng.data_ptr(),
                                 eng.design_ptr(), eng.resid_arena().ptr, eng.rinfo_arena().ptr,
@@ -1139,7 +1140,7 @@ int kpc_gcv_grad_hess(const double* data, int n, int p, int count, const int* ta
     const size_t per = (size_t)(1 + m + m * m);
     d_out.reserve(per * count);
     const long long stride = kpc::magic_scratch_doubles(eng.q(), m, eng.rk(), n);
-    const int grid = std::min(count, 8192);
+    const int grid = std::min(count, eng.slab_grid(stride, 8192));
     eng.scratch3().reserve((size_t)grid * stride);
     hip_check(kpc::launch_gcv_eval(d_items.ptr, count, n, eng.q(), m, eng.rk(), eng.data_ptr(),
                                    eng.design_ptr(), d_lsp.ptr, d_out.ptr, eng.scratch3().ptr, grid,

@@ -37,7 +37,6 @@ __host__ __device__ inline DesignLayout design_layout(int n, int q, int m, int r
   L.R = o; o += (long long)q * q;
   L.piv = o; o += q;
   L.rS = o; o += (long long)q * m * rk;
-  L.Sp = o; o += (long long)m * q * q;
   L.lsp0 = o; o += m;
   L.norms = o; o += 2 + 3 * m;
   L.status = o; o += 2;
@@ -353,19 +352,16 @@ __global__ __launch_bounds__(kGeomThreads) void geometry_kernel(
         rS[(size_t)c * q + prow] = F[(size_t)pos * dim + c];
       }
       __syncthreads();
-      double* Sp = D + DL.Sp + (size_t)s * q * q;
-      for (int e = tid; e < q * q; e += blockDim.x) {
-        const int r = e % q, c = e / q;
-        double v = 0.0;
-        for (int k2 = 0; k2 < rk; ++k2) v += rS[(size_t)k2 * q + r] * rS[(size_t)k2 * q + c];
-        Sp[e] = v;
-      }
-      __syncthreads();
-      // largest diagonal of S_i and its smallest kept Cholesky pivot (the
-      // pivots of chol(sp_i S_i) are sp_i times these): the GCV evaluation
-      // uses them to tell when the total-penalty root would drop pivots
+      // largest diagonal of S_i = rS_i rS_i' and its smallest kept Cholesky
+      // pivot (the pivots of chol(sp_i S_i) are sp_i times these): the GCV
+      // evaluation uses them to tell when the total-penalty root would drop
+      // pivots
       double lm = 0.0;
-      for (int r = tid; r < q; r += blockDim.x) lm = fmax(lm, Sp[(size_t)r * q + r]);
+      for (int r = tid; r < q; r += blockDim.x) {
+        double v = 0.0;
+        for (int k2 = 0; k2 < rk; ++k2) v += rS[(size_t)k2 * q + r] * rS[(size_t)k2 * q + r];
+        lm = fmax(lm, v);
+      }
       const double mxd = block_max(lm, red);
       if (tid == 0) {
         const double f = F[(size_t)(rk - 1) * dim + (rk - 1)];
@@ -480,13 +476,26 @@ __global__ __launch_bounds__(kGeomThreads) void geometry_kernel(
 }
 
 // ---------------------------------------------------------------- magic
-// Doubles of the per-fit global slab ahead of the response vectors: the
-// aug/V/X4/Qm fallback (6 q^2 + 3 q) and the derivative scratch; every design
-// here has m * rk <= q (tp: rank q - 2; additive: 8 m < 1 + 9 m).
-__host__ __device__ inline long long magic_scratch_core(int q) {
-  const long long mr = q;  // m * rk bound
-  long long s = 6LL * q * q + 3LL * q + (long long)q * q + 2LL * q * mr + (long long)q * q +
-                3LL * kMaxPen * q + q + 2LL * mr + 64;
+// Per-fit working set.  The big arrays (aug, V, X4, Qm: 6 q^2 + 3 q) and the
+// derivative work (3 m q + 4 m^2 + 3 m + 8) live in LDS when they fit and in
+// the fit's global slab otherwise; every design here has m rk <= q (tp: rank
+// q - 2; additive: 8 m < 1 + 9 m).
+__host__ __device__ inline long long magic_big_doubles(int q) {
+  return 6LL * q * q + 3LL * q;
+}
+__host__ __device__ inline long long magic_deriv_doubles(int q, int m) {
+  return 3LL * m * q + 4LL * m * m + 3LL * m + 8;
+}
+__host__ __device__ inline long long magic_small_doubles(int q, int m) {
+  // d, y1, wk (4q), 7 m-vectors + hessian, red, dctl, spv (2m), ord/piv/ictl
+  return 4LL * q + 7LL * m + (long long)m * m + 16 + 32 + 2LL * m + q + 24;
+}
+__host__ __device__ inline bool magic_deriv_in_lds(int q, int m) {
+  return magic_deriv_doubles(q, m) * 8 <= 48 * 1024;
+}
+// Doubles of the per-fit global slab ahead of the response vectors.
+__host__ __device__ inline long long magic_scratch_core(int q, int m) {
+  long long s = magic_big_doubles(q) + magic_deriv_doubles(q, m) + 2LL * q + 64;
   return (s + 31) / 32 * 32;
 }
 
@@ -502,21 +511,21 @@ struct MagicLds {
   double* d;      // q singular values (by column)
   int* ord;       // q: descending singular value order
   double* y1;     // q
-  double* lsp;    // kMaxPen
-  double* trial;  // kMaxPen
-  double* grad;   // kMaxPen (gradient used by the convergence test)
-  double* cgrad;  // kMaxPen (gradient at the current point)
-  double* hess;   // kMaxPen^2
+  double* lsp;    // m
+  double* trial;  // m
+  double* grad;   // m (gradient used by the convergence test)
+  double* cgrad;  // m (gradient at the current point)
+  double* hess;   // m^2
   double* nstep;
   double* sstep;
   double* step;
   double* red;    // 16
-  double* dctl;   // 16
+  double* dctl;   // 32
   int* ictl;      // 32
   int* piv;       // q
   double* wk;     // 2q
-  double* spv;    // 2 kMaxPen: sp_i = exp(log sp_i), then sqrt(sp_i)
-  double* My;     // m x nr derivative work (LDS-resident: read by lane loops)
+  double* spv;    // 2m: sp_i = exp(log sp_i), then sqrt(sp_i) at m + i
+  double* My;     // m x nr derivative work
   double* Ky;     // m x nr
   double* yK;     // m x nr
   double* tr;     // m^2 + m: pairwise traces, then tr(K_i)
@@ -529,9 +538,8 @@ struct MagicLds {
 
 __host__ __device__ inline size_t magic_lds_doubles(int n, int q, int m, bool aug_in_lds) {
   (void)n;
-  size_t s = (aug_in_lds ? (size_t)6 * q * q + 3 * (size_t)q : 0) + 4 * (size_t)q +
-             7 * kMaxPen + kMaxPen * kMaxPen + 48 + (size_t)q + 16;
-  s += 2 * kMaxPen + 3 * (size_t)m * q + 4 * (size_t)m * m + 3 * (size_t)m + 8;
+  size_t s = (aug_in_lds ? (size_t)magic_big_doubles(q) : 0) + (size_t)magic_small_doubles(q, m);
+  if (magic_deriv_in_lds(q, m)) s += (size_t)magic_deriv_doubles(q, m);
   return s;
 }
 
@@ -539,8 +547,8 @@ __device__ MagicLds carve_magic(double* base, int n, int q, int m, bool aug_in_l
                                 double* gscratch) {
   MagicLds L;
   double* p = base;
-  // response vectors live in the global slab after the derivative scratch
-  L.y = gscratch + magic_scratch_core(q);
+  // response vectors live in the global slab after the working set
+  L.y = gscratch + magic_scratch_core(q, m);
   L.yorig = L.y + n;
   double* big = aug_in_lds ? p : gscratch;
   L.aug = big;
@@ -549,30 +557,32 @@ __device__ MagicLds carve_magic(double* base, int n, int q, int m, bool aug_in_l
   L.Qm = big + 4 * (size_t)q * q;
   L.gyl = big + 6 * (size_t)q * q;
   L.projl = L.gyl + q;
-  if (aug_in_lds) p += 6 * (size_t)q * q + 3 * (size_t)q;
+  if (aug_in_lds) p += magic_big_doubles(q);
   L.d = p; p += q;
   L.y1 = p; p += q;
-  L.lsp = p; p += kMaxPen;
-  L.trial = p; p += kMaxPen;
-  L.grad = p; p += kMaxPen;
-  L.cgrad = p; p += kMaxPen;
-  L.nstep = p; p += kMaxPen;
-  L.sstep = p; p += kMaxPen;
-  L.step = p; p += kMaxPen;
-  L.hess = p; p += kMaxPen * kMaxPen;
+  L.lsp = p; p += m;
+  L.trial = p; p += m;
+  L.grad = p; p += m;
+  L.cgrad = p; p += m;
+  L.nstep = p; p += m;
+  L.sstep = p; p += m;
+  L.step = p; p += m;
+  L.hess = p; p += (size_t)m * m;
   L.red = p; p += 16;
   L.dctl = p; p += 32;
   L.wk = p; p += 2 * q;
-  L.spv = p; p += 2 * kMaxPen;
-  L.My = p; p += (size_t)m * q;
-  L.Ky = p; p += (size_t)m * q;
-  L.yK = p; p += (size_t)m * q;
-  L.tr = p; p += (size_t)m * m + m;
-  L.dn = p; p += m;
-  L.d2n = p; p += (size_t)m * m;
-  L.syA = p; p += (size_t)m * m;
-  L.syZ = p; p += (size_t)m * m;
-  L.syw = p; p += m + 8;
+  L.spv = p; p += 2 * m;
+  double* dw = magic_deriv_in_lds(q, m) ? p : gscratch + magic_big_doubles(q);
+  L.My = dw; dw += (size_t)m * q;
+  L.Ky = dw; dw += (size_t)m * q;
+  L.yK = dw; dw += (size_t)m * q;
+  L.tr = dw; dw += (size_t)m * m + m;
+  L.dn = dw; dw += m;
+  L.d2n = dw; dw += (size_t)m * m;
+  L.syA = dw; dw += (size_t)m * m;
+  L.syZ = dw; dw += (size_t)m * m;
+  L.syw = dw; dw += m + 8;
+  if (magic_deriv_in_lds(q, m)) p = dw;
   int* ip = reinterpret_cast<int*>(p);
   L.ord = ip; ip += q;
   L.piv = ip; ip += q;
@@ -581,34 +591,21 @@ __device__ MagicLds carve_magic(double* base, int n, int q, int m, bool aug_in_l
 }
 
 struct MagicScratch {
-  double* agg;   // q x q
-  double* T;     // nr x (m*rk)
-  double* GT;    // nr x (m*rk)
-  double* G;     // nr x nr
-  double* My;    // m x nr
-  double* Ky;
-  double* yK;
-  double* gy;    // nr
-  double* proj;  // 2 * m * rk: T_a' y1 and T_a' G y1
+  double* theta;  // q: coefficients in the QR basis
+  double* beta;   // q: coefficients in the original column order
 };
 
 __device__ MagicScratch carve_magic_scratch(double* base, int q, int m, int rk) {
+  (void)rk;
   MagicScratch S;
-  double* p = base + 6 * (size_t)q * q + 3 * (size_t)q;  // after the global LDS fallback
-  S.agg = p; p += (size_t)q * q;
-  S.T = p; p += (size_t)q * m * rk;
-  S.GT = p; p += (size_t)q * m * rk;
-  S.G = p; p += (size_t)q * q;
-  S.My = p; p += (size_t)m * q;
-  S.Ky = p; p += (size_t)m * q;
-  S.yK = p; p += (size_t)m * q;
-  S.gy = p; p += q;
-  S.proj = p; p += 2 * (size_t)q;
+  double* p = base + magic_big_doubles(q) + magic_deriv_doubles(q, m);
+  S.theta = p; p += q;
+  S.beta = p; p += q;
   return S;
 }
 
 long long magic_scratch_doubles(int q, int m, int rk, int n) {
-  long long s = magic_scratch_core(q) + 2LL * n + 64;
+  long long s = magic_scratch_core(q, m) + 2LL * n + 64;
   (void)m;
   (void)rk;
   return (s + 31) / 32 * 32;
@@ -626,25 +623,26 @@ __device__ void jacobi_svd(double* A, int rows, int q, double* V, double* d, int
   const int npairs = qe / 2;
   int lpp = 64;
   while (lpp > 1 && npairs * lpp > (int)blockDim.x) lpp >>= 1;
-  const int pair = tid / lpp;
+  const int groups = (int)blockDim.x / lpp;
   const int sub = tid % lpp;
   const int qm = qe - 1;
+  // pairs per group (more than one only for q > 2 blockDim), uniform count
+  const int passes = (npairs + groups - 1) / groups;
   __syncthreads();
   for (int sweep = 0; sweep < 60; ++sweep) {
     if (tid == 0) ictl[8] = 0;
     __syncthreads();
     // round-robin schedule: pair 0 meets the fixed column qe-1, pair p>0
-    // pairs (rnd + p) and (rnd - p) mod (qe - 1); advanced incrementally
-    int pa = pair % qm, pb = (qm - pair % qm) % qm;
+    // pairs (rnd + p) and (rnd - p) mod (qe - 1)
     for (int rnd = 0; rnd < qm; ++rnd) {
+     for (int ps = 0; ps < passes; ++ps) {
+      const int pair = tid / lpp + ps * groups;
       int a = -1, b = -1;
       if (pair < npairs) {
         if (pair == 0) { a = rnd; b = qe - 1; }
-        else { a = pa; b = pb; }
+        else { a = (rnd + pair) % qm; b = (rnd - pair % qm + qm) % qm; }
         if (a > b) { int t = a; a = b; b = t; }
       }
-      if (++pa == qm) pa = 0;
-      if (++pb == qm) pb = 0;
       const bool active = pair < npairs && b < q;
       double al = 0.0, be = 0.0, ga = 0.0;
       // rows are visited in chunks of kJc with all loads of a chunk issued
@@ -720,6 +718,7 @@ __device__ void jacobi_svd(double* A, int rows, int q, double* V, double* d, int
         }
         if (sub == 0) ictl[8] = 1;
       }
+     }
       __syncthreads();
     }
     if (tid == 0) ictl[28] += 1;
@@ -863,6 +862,23 @@ __device__ void stacked_q1(const double* W, int rows, int q, int mr, const doubl
   }
 }
 
+// Total penalty sum_i sp_i S_i = sum_i sp_i rS_i rS_i' (q x q, column-major).
+__device__ void aggregate_penalty(const double* D, const DesignLayout& DL, int q, int m, int rk,
+                                  const double* spv, double* agg) {
+  for (int e = threadIdx.x; e < q * q; e += blockDim.x) {
+    const int r = e % q, c = e / q;
+    double a = 0.0;
+    for (int i = 0; i < m; ++i) {
+      const double* R0 = D + DL.rS + (size_t)i * rk * q;
+      double v = 0.0;
+      for (int k2 = 0; k2 < rk; ++k2) v += R0[(size_t)k2 * q + r] * R0[(size_t)k2 * q + c];
+      a += v * spv[i];
+    }
+    agg[e] = a;
+  }
+  __syncthreads();
+}
+
 // Copy [R; root] into the augmented buffer (rows = q + rank).
 __device__ void build_augmented(const MagicLds& L, const double* D, const DesignLayout& DL, int q,
                                 int rank, const double* root) {
@@ -902,7 +918,7 @@ __device__ void magic_eval(const MagicLds& L, const MagicScratch& S, const doubl
   if (tid < m) {
     const double e = exp(lspv[tid]);
     L.spv[tid] = e;
-    L.spv[kMaxPen + tid] = sqrt(e);
+    L.spv[m + tid] = sqrt(e);
   }
   __syncthreads();
   // The total-penalty root of mgcv (pivoted Cholesky, default tolerance
@@ -933,17 +949,13 @@ __device__ void magic_eval(const MagicLds& L, const MagicScratch& S, const doubl
     for (int e = tid; e < rows * q; e += blockDim.x) {
       const int r = e % rows, c = e / rows;
       L.aug[e] = r < q ? D[DL.R + (size_t)c * q + r]
-                       : spv[kMaxPen + (r - q) / rk] * D[DL.rS + (size_t)(r - q) * q + c];
+                       : spv[m + (r - q) / rk] * D[DL.rS + (size_t)(r - q) * q + c];
     }
     __syncthreads();
   } else {
     // root with dropped pivots: exactly mgcv's [R; root]
     double* agg = L.X4;
-    for (int e = tid; e < q * q; e += blockDim.x) {
-      double a = 0.0;
-      for (int i = 0; i < m; ++i) a += D[DL.Sp + (size_t)i * q * q + e] * spv[i];
-      agg[e] = a;
-    }
+    aggregate_penalty(D, DL, q, m, rk, spv, agg);
     __syncthreads();
     if (wid == 0) {
       const int r = pstf2_upper_wave(agg, q, L.piv, L.wk);
@@ -994,11 +1006,7 @@ __device__ void magic_eval(const MagicLds& L, const MagicScratch& S, const doubl
   } else {
     // ---- exact path: aggregate penalty, pivoted Cholesky root, Jacobi SVD
     double* agg = L.X4;
-    for (int e = tid; e < q * q; e += blockDim.x) {
-      double a = 0.0;
-      for (int i = 0; i < m; ++i) a += D[DL.Sp + (size_t)i * q * q + e] * spv[i];
-      agg[e] = a;
-    }
+    aggregate_penalty(D, DL, q, m, rk, spv, agg);
     __syncthreads();
     if (wid == 0) {
       const int r = pstf2_upper_wave(agg, q, L.piv, L.wk);
@@ -1464,7 +1472,7 @@ __global__ __launch_bounds__(kMagicThreads) void magic_kernel(
       const int nr = L.ictl[0];
       const bool qr = L.ictl[13] != 0;
       if (qr && !L.ictl[15]) upper_tri_inverse(L.aug, q + L.ictl[1], q, L.V, L.wk);
-      double* theta = S.gy;  // q entries (gy sized >= q)
+      double* theta = S.theta;
       for (int r = tid; r < q; r += blockDim.x) {
         double v = 0.0;
         if (qr) {
@@ -1476,7 +1484,7 @@ __global__ __launch_bounds__(kMagicThreads) void magic_kernel(
         theta[r] = v;
       }
       __syncthreads();
-      double* beta = S.My;  // q entries
+      double* beta = S.beta;
       for (int i = tid; i < q; i += blockDim.x) beta[(int)D[DL.piv + i]] = theta[i];
       __syncthreads();
       double* ro = resid + (size_t)it.resid_slot * n;

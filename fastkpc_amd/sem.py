@@ -16,6 +16,9 @@ CONFIGS = {
     # n=500, p=50, batched-form workload with 10% Student-t(3) noise nodes
     "sem500x50": dict(n=500, p=50, edge_prob=0.06, noise="t3_mix", t_frac=0.1, var_lo=0.25,
                       var_hi=1.0, seed=3),
+    # n=2000, p=64, permutation / large-additive stress workload, mixed links
+    "sem2000x64": dict(n=2000, p=64, edge_prob=0.05, noise="uniform_var", var_lo=0.25,
+                       var_hi=1.0, seed=5, link="mixed"),
 }
 
 

@@ -198,3 +198,28 @@ def test_ci_batch_vs_oracle_random_sets():
             tol = 1e-6 if tie else 1e-9
             assert abs(out[i, 0] - ref["p.value"]) <= tol * ref["p.value"] + 1e-12, \
                 (s_len, xs[i], ys[i], Ss[i], out[i, 0], ref)
+
+
+def test_large_additive_designs_vs_oracle():
+    # |S| = 20 (q = 181): full GCV fit; |S| = 62 (q = 559, n = 2000): one GCV
+    # evaluation with gradient and Hessian at a fixed log(sp)
+    f = fk()
+    o = orc()
+    from fastkpc_amd.sem import config_data
+    data, _, _ = config_data("sem2000x64")
+    S20 = list(range(2, 22))
+    sub = np.asfortranarray(data[:600])
+    resid, info = f.regrxons_batch(sub, [0], [S20])
+    ro, oi = o.regrxons(sub, 0, S20)
+    tie = min(oi["margin"], info[0, 30]) < TIE_MARGIN
+    assert info[0, 0] == 0.0
+    rel = np.linalg.norm(resid[0] - ro) / np.linalg.norm(ro)
+    assert rel <= (1e-6 if tie else 1e-8), (rel, tie)
+    assert abs(info[0, 1] - oi["score"]) <= 1e-9 * oi["score"]
+    S62 = list(range(1, 63))
+    lsp = np.linspace(-1.0, 2.0, 62)
+    score, grad, hess = f.gcv_grad_hess(data, [0], [S62], lsp[None, :])
+    os_, og, oh = o.gcv_eval(data, 0, S62, lsp)
+    assert abs(score[0] - os_) <= 1e-9 * os_
+    np.testing.assert_allclose(grad[0], og, rtol=1e-6, atol=1e-9 * os_)
+    np.testing.assert_allclose(hess[0], oh, rtol=1e-5, atol=1e-8 * os_)
