@@ -22,6 +22,7 @@
 // Newton steps and every long dot product runs several accumulators.
 #include "kpc_common.h"
 #include "kpc_device.h"
+#include "kpc_tridiag.h"
 
 namespace kpc {
 
@@ -30,142 +31,6 @@ constexpr int kLzThreads = 512;
 // Doubles of LDS for the twisted-factorisation pivots (2 m per eigenpair).
 __host__ __device__ inline int lanczos_ivw(int mmax) {
   return 4096 > 6 * mmax + 64 ? 4096 : 6 * mmax + 64;
-}
-
-// 1/x from the hardware estimate and one (Sturm counts) or two Newton steps.
-__device__ __forceinline__ double lz_rcp1(double x) {
-  const double r = __builtin_amdgcn_rcp(x);
-  return fma(r, fma(-x, r, 1.0), r);
-}
-__device__ __forceinline__ double lz_rcp(double x) {
-  const double r = lz_rcp1(x);
-  return fma(r, fma(-x, r, 1.0), r);
-}
-
-// Number of eigenvalues of the tridiagonal (d, e2 = e^2) strictly below x.
-__device__ int lz_sturm(const double* d, const double* e2, int n, double x, double pivmin) {
-  int cnt = 0;
-  double q = d[0] - x;
-  if (fabs(q) < pivmin) q = -pivmin;
-  cnt += q < 0.0;
-  for (int i = 1; i < n; ++i) {
-    q = (d[i] - x) - e2[i - 1] * lz_rcp1(q);
-    if (fabs(q) < pivmin) q = -pivmin;
-    cnt += q < 0.0;
-  }
-  return cnt;
-}
-
-// Eigenvector of the tridiagonal (al, be) for the eigenvalue th by a twisted
-// factorization (as in LAPACK dlar1v / MRRR): forward pivots d_i of
-// T - th I = L D L', backward pivots r_i of U R U', twist index
-// k = argmin |gamma_k| with gamma_k = d_k + r_k - (al_k - th); then x_k = 1,
-// x_i = -be_i x_{i+1} / d_i upward and x_{i+1} = -be_i x_i / r_{i+1} downward,
-// which satisfies every row except k (residual gamma_k).  rd, rr: m doubles
-// of workspace (reciprocal pivots).  Writes the unit vector to z when z is
-// non-null; returns its last component.
-__device__ double lz_twisted(const double* al, const double* be, int m, double th, double pivmin,
-                             double* rd, double* rr, double* z) {
-  // forward and backward pivot sweeps, interleaved for ILP
-  double d = al[0] - th;
-  double r = al[m - 1] - th;
-  for (int i = 0; i < m; ++i) {
-    if (fabs(d) < pivmin) d = -pivmin;
-    if (fabs(r) < pivmin) r = -pivmin;
-    const double id = lz_rcp(d), ir = lz_rcp(r);
-    rd[i] = id;
-    rr[m - 1 - i] = ir;
-    if (i + 1 < m) {
-      d = (al[i + 1] - th) - be[i] * be[i] * id;
-      r = (al[m - 2 - i] - th) - be[m - 2 - i] * be[m - 2 - i] * ir;
-    }
-  }
-  // twist: |gamma_k| = |1/rd_k + 1/rr_k - (al_k - th)|
-  int kt = 0;
-  double gmin = 1e308;
-  for (int i = 0; i < m; ++i) {
-    const double g = fabs(lz_rcp(rd[i]) + lz_rcp(rr[i]) - (al[i] - th));
-    if (g < gmin) {
-      gmin = g;
-      kt = i;
-    }
-  }
-  double n0 = 1.0, n1 = 0.0, x = 1.0, last = kt == m - 1 ? 1.0 : 0.0;
-  if (z) z[kt] = 1.0;
-  for (int i = kt - 1; i >= 0; --i) {
-    x = -be[i] * x * rd[i];
-    if (z) z[i] = x;
-    n1 += x * x;
-  }
-  x = 1.0;
-  for (int i = kt; i < m - 1; ++i) {
-    x = -be[i] * x * rr[i + 1];
-    if (z) z[i + 1] = x;
-    n0 += x * x;
-  }
-  if (kt < m - 1) last = x;
-  const double inv = 1.0 / sqrt(n0 + n1);
-  if (z)
-    for (int i = 0; i < m; ++i) z[i] *= inv;
-  return last * inv;
-}
-
-// Sturm multisection for the candidate eigenvalues of the tridiagonal
-// (al, e2): candidate c has ascending index idx(c) = c < nlow ? c : hs + c - nlow.
-// G lanes per candidate probe G interior points of its bracket per round (the
-// bracket shrinks by G + 1); rounds stop when every bracket is below
-// rtol max(|lo|, |hi|) + pivmin.  Brackets persist in clo/chi (init: start
-// from [lo0, hi0]); cval receives the midpoints.  All threads must call it.
-__device__ void lz_multisect(const double* al, const double* e2, int m, double pivmin, int ncand,
-                             int nlow, int hs, double rtol, bool init, double lo0, double hi0,
-                             double* clo, double* chi, double* cval) {
-  const int tid = threadIdx.x;
-  const int lane = tid & (KPC_WAVE - 1);
-  int G = ncand > 0 ? (int)blockDim.x / ncand : 1;
-  G = G > 8 ? 8 : (G < 1 ? 1 : G);
-  int Gp = 1;
-  while (Gp * 2 <= G) Gp *= 2;  // power of two: groups align with lanes
-  G = Gp;
-  const int c = tid / G, g = tid % G;
-  const bool own = c < ncand;
-  const int idx = own ? (c < nlow ? c : hs + (c - nlow)) : 0;
-  double lo = lo0, hi = hi0;
-  if (own && !init) {
-    lo = clo[c];
-    hi = chi[c];
-  }
-  for (int round = 0; round < 400; ++round) {
-    const double w = hi - lo;
-    const bool done = !own || w <= rtol * fmax(fabs(lo), fabs(hi)) + pivmin;
-    if (__syncthreads_and(done ? 1 : 0)) break;
-    const double step = w / (double)(G + 1);
-    int flag = 0;
-    if (own && !done) {
-      const double xg = lo + step * (double)(g + 1);
-      flag = lz_sturm(al, e2, m, xg, pivmin) > idx;
-    }
-    // first flagged point of the group (count(x) > idx moves the upper end)
-    const unsigned long long bal = __ballot(flag);
-    const int gbase = lane - g;
-    const unsigned bits = (unsigned)((bal >> gbase) & ((1ULL << G) - 1ULL));
-    if (own && !done) {
-      const int first = bits ? __builtin_ctz(bits) : G;
-      const double nlo = first == 0 ? lo : lo + step * (double)first;
-      const double nhi = first == G ? hi : lo + step * (double)(first + 1);
-      if (nhi > nlo) {
-        lo = nlo;
-        hi = nhi;
-      } else {
-        hi = lo;  // no representable interior point left
-      }
-    }
-  }
-  if (own && g == 0) {
-    clo[c] = lo;
-    chi[c] = hi;
-    cval[c] = 0.5 * (lo + hi);
-  }
-  __syncthreads();
 }
 
 // Top-kk candidates by |value| (ties toward the higher ascending index),

@@ -5,6 +5,7 @@
 //   :96-107  shift = R mean of each covariate (long double + correction)
 //   :114-152 eta constants, :154-198 null space / polynomial powers
 //   :200-239 lexicographically sorted unique locations
+//   :266-301 eigenpairs of the Lanczos tridiagonal, descending order
 //   :303-494 Rlanczos: LCG start vector, Lanczos with two reorthogonalisation
 //            passes, Ritz check every max(10, k/2) (capped at n/10) steps
 //            once the step index reaches k, eps^0.7 * ||T|| stop rule with the
@@ -13,18 +14,40 @@
 //   :621-688 sum-to-zero constraint (LINPACK dqrdc2/dqrqty, one reflector)
 //   :690-858 X = [U D Z | T], penalty Z' D Z, RMS column scaling,
 //            S <- S / (||S||_1 / ||X||_inf^2)
-// Reorthogonalisation is done as two classical Gram-Schmidt passes (the
-// reference uses modified Gram-Schmidt); both give orthogonality to working
-// precision.  Ritz values/vectors of the Lanczos tridiagonal come from
-// implicit QL whose Givens sequences are computed by one lane and applied to
-// the eigenvector rows by the whole workgroup.
+//
+// Device design:
+//  * E_ij = eta(|x_i - x_j|) is never stored: every Lanczos step recomputes
+//    E q from the unique locations held in LDS (nu^2 eta evaluations, no
+//    memory traffic).  Lanes own rows, waves own (row tile, column chunk)
+//    units, four evaluations are in flight per lane, and the chunk partial
+//    sums are added in a fixed order.
+//  * The eta of d = 2 uses tp_log (atanh series, within 2 ulp) instead of
+//    the library logarithm, whose double-double compensation triples the
+//    cost of the product.
+//  * The three-term recurrence and the reference's two modified Gram-Schmidt
+//    passes become two classical Gram-Schmidt passes over the whole basis
+//    (the first one on E q itself, so alpha and the beta term are among its
+//    coefficients); both schemes are orthogonal to working precision.  A
+//    wave forms eight coefficients at once with coalesced basis reads.
+//  * A Ritz check needs the k largest and k smallest eigenvalues of the
+//    Lanczos tridiagonal and the last components of the k leading
+//    eigenvectors: Sturm multisection and twisted factorisations give them
+//    in parallel, and the Ritz vectors of the selected pairs come from
+//    twisted factorisations as well (re-orthogonalised within eigenvalue
+//    clusters).  Eigenvector signs are immaterial: flipping a Ritz vector
+//    flips the matching column of the basis and rows/columns of the penalty,
+//    which leaves the fitted smooth unchanged.
+//  * The penalty transforms run in LDS with one thread per row or column;
+//    each entry is formed with the reference's operation order.
 #include "kpc_common.h"
 #include "kpc_device.h"
+#include "kpc_tridiag.h"
 
 namespace kpc {
 
-constexpr int kSmoothThreads = 256;
+constexpr int kSmoothThreads = 512;
 constexpr int kMaxBasis = 32;
+constexpr int kSmoothWork = 4096;  // doubles of LDS work area
 
 __device__ double eta_constant(int m, int d) {
   const double pi = asin(1.0) * 2.0;
@@ -50,44 +73,91 @@ __device__ double eta_constant(int m, int d) {
   return value;
 }
 
-__device__ double eta_r2(int m, int d, double r2, double eta0) {
-  if (r2 <= 0.0) return 0.0;
-  const int d2 = d / 2;
-  double value = eta0;
-  if (d % 2 == 0) {
-    value *= log(r2) * 0.5;
-    for (int i = 0; i < m - d2; ++i) value *= r2;
-  } else {
-    for (int i = 0; i < m - d2 - 1; ++i) value *= r2;
-    value *= sqrt(r2);
+// Natural logarithm of a finite x > 0 by argument reduction and the atanh
+// series: x = 2^k m with m in [sqrt(1/2), sqrt(2)), s = (m - 1) / (m + 1),
+// log(m) = 2 atanh(s) = 2 s sum_{i=0..11} s^(2i) / (2i + 1) (|s| < 0.1716, so
+// the omitted terms are below 2^-56 relative), log 2 split into a 32-bit head
+// (k log2_head exact) and a tail.  Within 2 ulp of the correctly rounded
+// logarithm; a third of the instructions of the library routine.  x <= 0
+// gives a meaningless finite value: callers mask it.
+__device__ __forceinline__ double tp_log(double x) {
+  double m = __builtin_amdgcn_frexp_mant(x);
+  int k = __builtin_amdgcn_frexp_exp(x);
+  if (m < 0.7071067811865476) {
+    m = m + m;
+    k -= 1;
   }
-  return value;
+  const double f = m - 1.0;
+  const double den = m + 1.0;
+  double r = __builtin_amdgcn_rcp(den);
+  r = fma(r, fma(-den, r, 1.0), r);
+  r = fma(r, fma(-den, r, 1.0), r);
+  double s = f * r;
+  s = fma(r, fma(-den, s, f), s);
+  const double z = s * s;
+  double P = 0.043478260869565216;   // 1/23
+  P = fma(P, z, 0.047619047619047616);  // 1/21
+  P = fma(P, z, 0.05263157894736842);   // 1/19
+  P = fma(P, z, 0.058823529411764705);  // 1/17
+  P = fma(P, z, 0.06666666666666667);   // 1/15
+  P = fma(P, z, 0.07692307692307693);   // 1/13
+  P = fma(P, z, 0.09090909090909091);   // 1/11
+  P = fma(P, z, 0.1111111111111111);    // 1/9
+  P = fma(P, z, 0.14285714285714285);   // 1/7
+  P = fma(P, z, 0.2);                   // 1/5
+  P = fma(P, z, 0.3333333333333333);    // 1/3
+  const double lm = 2.0 * fma(s * z, P, s);
+  const double kd = (double)k;
+  return fma(kd, 0.6931471803691238, fma(kd, 1.9082149292705877e-10, lm));
+}
+
+// Order-2 thin-plate eta from the coordinate differences: d = 1 gives
+// eta0 r^2 |r| (sqrt(fl(dx^2)) = |dx| exactly under correct rounding), d = 2
+// gives eta0 (log(r^2) / 2) r^2 evaluated as (eta0 / 2) log(r^2) r^2 (the
+// halving is exact); zero at r = 0.  eta = eta0 (d = 1) or eta0 / 2 (d = 2).
+template <int D>
+__device__ __forceinline__ double tp_eta(double dx, double dy, double eta) {
+  if (D == 2) {
+    const double r2 = dx * dx + dy * dy;
+    const double v = (eta * tp_log(r2)) * r2;
+    return r2 > 0.0 ? v : 0.0;
+  }
+  const double r2 = dx * dx;
+  const double v = (eta * r2) * fabs(dx);
+  return r2 > 0.0 ? v : 0.0;
 }
 
 // Workgroup LDS, carved from dynamic shared memory with arrays of np2 (the
 // next power of two >= n) entries.
 struct SmoothLds {
-  double* ka;   // sort key 0 -> unique coordinate 0
-  double* kb;   // sort key 1 -> unique coordinate 1
-  int* ki;      // sort index -> original row -> unique id map
-  double* z;    // Lanczos residual; Givens cos; QT reflectors
-  double* al;   // Lanczos alpha; QL diagonal
-  double* be;   // Lanczos beta
-  double* cc;   // CGS coefficients; QL off-diagonal; column means
-  double* ss;   // Givens sin
-  double* tv;   // Ritz values, descending
-  int* perm;    // descending order -> QL column
-  double* red;  // 16
-  int* ctl;     // 8
-  double* dctl; // 8
-  double* zq;   // 3 * kMaxBasis
-  double* cm;   // kMaxBasis
-  double* rms;  // kMaxBasis
+  double* ka;    // sort key 0 -> unique coordinate 0
+  double* kb;    // sort key 1 -> unique coordinate 1
+  double* qv;    // current Lanczos vector
+  double* z;     // Lanczos residual (unique ids while sorting)
+  double* al;    // Lanczos alpha
+  double* be;    // Lanczos beta
+  double* e2;    // beta^2 for the Sturm counts
+  double* w;     // kSmoothWork: product partials, CGS coefficients,
+                 // twisted pivots, penalty work
+  double* red;   // 16
+  double* dctl;  // 8
+  double* zq;    // 3 * kMaxBasis  T'U rows
+  double* hz;    // 3 * kMaxBasis  QT reflectors
+  double* cm;    // kMaxBasis      Ritz values of the basis
+  double* rms;   // kMaxBasis
+  double* clo;   // 2 * kMaxBasis  multisection brackets
+  double* chi;   // 2 * kMaxBasis
+  double* cval;  // 2 * kMaxBasis  candidate eigenvalues (ascending slots)
+  double* errs;  // kMaxBasis      Ritz residual bounds of the leading pairs
+  int* ki;       // sort index -> original row -> unique id
+  int* ctl;      // 16
 };
 
 __host__ __device__ inline size_t smooth_lds_size(int np2) {
-  return (size_t)np2 * (8 * sizeof(double) + 2 * sizeof(int)) +
-         (16 + 8 + 3 * kMaxBasis + 2 * kMaxBasis) * sizeof(double) + 8 * sizeof(int);
+  return (size_t)np2 * (7 * sizeof(double) + sizeof(int)) +
+         (size_t)(kSmoothWork + 16 + 8 + 6 * kMaxBasis + 2 * kMaxBasis + 6 * kMaxBasis +
+                  kMaxBasis) * sizeof(double) +
+         16 * sizeof(int);
 }
 
 __device__ inline SmoothLds carve_smooth_lds(double* base, int np2) {
@@ -95,95 +165,68 @@ __device__ inline SmoothLds carve_smooth_lds(double* base, int np2) {
   double* d = base;
   L.ka = d; d += np2;
   L.kb = d; d += np2;
+  L.qv = d; d += np2;
   L.z = d; d += np2;
   L.al = d; d += np2;
   L.be = d; d += np2;
-  L.cc = d; d += np2;
-  L.ss = d; d += np2;
-  L.tv = d; d += np2;
+  L.e2 = d; d += np2;
+  L.w = d; d += kSmoothWork;
   L.red = d; d += 16;
   L.dctl = d; d += 8;
   L.zq = d; d += 3 * kMaxBasis;
+  L.hz = d; d += 3 * kMaxBasis;
   L.cm = d; d += kMaxBasis;
   L.rms = d; d += kMaxBasis;
+  L.clo = d; d += 2 * kMaxBasis;
+  L.chi = d; d += 2 * kMaxBasis;
+  L.cval = d; d += 2 * kMaxBasis;
+  L.errs = d; d += kMaxBasis;
   int* ip = reinterpret_cast<int*>(d);
   L.ki = ip; ip += np2;
-  L.perm = ip; ip += np2;
   L.ctl = ip;
   return L;
 }
 
-// Implicit QL (EISPACK tql2 ordering) on LDS (d, e) with eigenvectors in the
-// global size x size matrix Zm (column-major).  Lane 0 generates each sweep's
-// Givens rotations, every lane applies them to its rows.
-__device__ void ql_eigen_parallel(const SmoothLds& L, int size, double* Zm) {
-  const int tid = threadIdx.x;
-  double* d = L.al;
-  double* e = L.cc;  // e[i] = T(i, i+1), e[size-1] = 0
-  for (int i = tid; i < size * size; i += blockDim.x) Zm[i] = 0.0;
-  __syncthreads();
-  for (int i = tid; i < size; i += blockDim.x) Zm[(size_t)i * size + i] = 1.0;
-  if (tid == 0) { L.ctl[0] = 0; L.ctl[1] = 0; L.ctl[4] = 0; }
-  __syncthreads();
-  while (true) {
-    if (tid == 0) {
-      int l = L.ctl[0];
-      int found = 0;
-      while (l < size && !found) {
-        int m;
-        for (m = l; m < size - 1; ++m) {
-          double dd2 = fabs(d[m]) + fabs(d[m + 1]);
-          if (fabs(e[m]) <= 2.220446049250313e-16 * dd2) break;
-        }
-        if (m == l) { ++l; L.ctl[4] = 0; continue; }
-        if (L.ctl[4]++ >= 60) { ++l; L.ctl[4] = 0; continue; }
-        double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-        double r = hypot(g, 1.0);
-        g = d[m] - d[l] + e[l] / (g + copysign(r, g));
-        double s = 1.0, c = 1.0, p = 0.0;
-        int i;
-        int early = 0;
-        for (i = m - 1; i >= l; --i) {
-          double f = s * e[i], b = c * e[i];
-          e[i + 1] = (r = hypot(f, g));
-          if (r == 0.0) { d[i + 1] -= p; e[m] = 0.0; early = 1; break; }
-          s = f / r;
-          c = g / r;
-          g = d[i + 1] - p;
-          r = (d[i] - g) * s + 2.0 * c * b;
-          d[i + 1] = g + (p = s * r);
-          g = c * r - b;
-          L.z[i] = c;     // rotation (c, s) acting on columns (i, i+1)
-          L.ss[i] = s;
-        }
-        if (!early) {
-          d[l] -= p;
-          e[l] = g;
-          e[m] = 0.0;
-        }
-        L.ctl[1] = m - 1;          // first rotation index
-        L.ctl[2] = early ? i + 1 : l;  // last rotation index
-        found = 1;
+// Chunk partials of E q (E implicit) into L.w[chunk * nu + row]; columns are
+// read in blocks of eight (LDS broadcasts) ahead of their evaluation.
+template <int D>
+__device__ void tp_product(const SmoothLds& L, int nu, int nchunk, int cs, double eta) {
+  const int lane = threadIdx.x & (KPC_WAVE - 1);
+  const int wid = threadIdx.x / KPC_WAVE;
+  const int nw = blockDim.x / KPC_WAVE;
+  const int ntile = (nu + KPC_WAVE - 1) / KPC_WAVE;
+  const double* __restrict__ ka = L.ka;
+  const double* __restrict__ kbv = L.kb;
+  const double* __restrict__ q = L.qv;
+  for (int u = wid; u < ntile * nchunk; u += nw) {
+    const int tile = u % ntile, chunk = u / ntile;
+    const int i = tile * KPC_WAVE + lane;
+    if (i >= nu) continue;
+    const int c0 = chunk * cs;
+    const int c1 = min(nu, c0 + cs);
+    const double xi = ka[i];
+    const double yi = D == 2 ? kbv[i] : 0.0;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int j = c0;
+    for (; j + 8 <= c1; j += 8) {
+      double xs[8], ys[8], qs[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        xs[k] = ka[j + k];
+        ys[k] = D == 2 ? kbv[j + k] : 0.0;
+        qs[k] = q[j + k];
       }
-      L.ctl[0] = l;
-      L.ctl[3] = found;
-    }
-    __syncthreads();
-    if (!L.ctl[3]) break;
-    const int ihi = L.ctl[1], ilo = L.ctl[2];
-    for (int k = tid; k < size; k += blockDim.x) {
-      for (int i = ihi; i >= ilo; --i) {
-        const double c = L.z[i], s = L.ss[i];
-        double* zi = Zm + (size_t)i * size + k;
-        double* zi1 = Zm + (size_t)(i + 1) * size + k;
-        const double f = *zi1;
-        *zi1 = s * *zi + c * f;
-        *zi = c * *zi - s * f;
+#pragma unroll
+      for (int k = 0; k < 8; k += 4) {
+        a0 += tp_eta<D>(xi - xs[k], yi - ys[k], eta) * qs[k];
+        a1 += tp_eta<D>(xi - xs[k + 1], yi - ys[k + 1], eta) * qs[k + 1];
+        a2 += tp_eta<D>(xi - xs[k + 2], yi - ys[k + 2], eta) * qs[k + 2];
+        a3 += tp_eta<D>(xi - xs[k + 3], yi - ys[k + 3], eta) * qs[k + 3];
       }
     }
-    __syncthreads();
+    for (; j < c1; ++j) a0 += tp_eta<D>(xi - ka[j], D == 2 ? yi - kbv[j] : 0.0, eta) * q[j];
+    L.w[chunk * nu + i] = (a0 + a1) + (a2 + a3);
   }
-  __syncthreads();
 }
 
 __global__ __launch_bounds__(kSmoothThreads) void smooth_setup_kernel(
@@ -191,6 +234,7 @@ __global__ __launch_bounds__(kSmoothThreads) void smooth_setup_kernel(
     double* __restrict__ out_X, double* __restrict__ out_S, int* __restrict__ out_info,
     double* __restrict__ scratch, long long scratch_stride) {
   extern __shared__ double lds_raw[];
+  constexpr double kEps = 2.220446049250313e-16;
   int np2 = 1;
   while (np2 < n) np2 <<= 1;
   SmoothLds L = carve_smooth_lds(lds_raw, np2);
@@ -276,24 +320,11 @@ __global__ __launch_bounds__(kSmoothThreads) void smooth_setup_kernel(
       continue;
     }
     // ---- scratch layout
-    double* E = ws;                                  // nu x nu
-    double* Qb = E + (size_t)nu * nu;                // nu x (nu+1)
-    double* Zm = Qb + (size_t)nu * (nu + 1);         // up to nu x nu (QL vectors)
-    double* U = Zm + (size_t)nu * nu;                // nu x kb
-    double* Xs = U + (size_t)nu * kb;                // n x kb (row-major) before absorption
+    double* Qb = ws;                                 // nu x (nu+1) Lanczos basis
+    double* Zs = Qb + (size_t)nu * (nu + 1);         // kMaxBasis x nu Ritz coefficients
+    double* U = Zs + (size_t)kMaxBasis * nu;         // nu x kb Ritz vectors
+    double* Xs = U + (size_t)nu * kMaxBasis;         // n x kb (row-major) before absorption
     const double eta0 = eta_constant(order, d);
-    for (int j = 0; j < nu; ++j) {
-      for (int i = tid; i < nu; i += blockDim.x) {
-        double s2;
-        double dx = L.ka[i] - L.ka[j];
-        s2 = dx * dx;
-        if (d == 2) {
-          double dy = L.kb[i] - L.kb[j];
-          s2 += dy * dy;
-        }
-        E[(size_t)j * nu + i] = (i == j) ? 0.0 : eta_r2(order, d, s2, eta0);
-      }
-    }
     // ---- Lanczos start vector (mgcv LCG)
     if (tid == 0) {
       unsigned long st = 1;
@@ -301,144 +332,256 @@ __global__ __launch_bounds__(kSmoothThreads) void smooth_setup_kernel(
       for (int r = 0; r < nu; ++r) {
         st = (st * 106UL + 1283UL) % 6075UL;
         double vv = (double)st / 6075.0 - 0.5;
-        Qb[r] = vv;
+        L.z[r] = vv;
         nrm += vv * vv;
       }
       L.dctl[0] = sqrt(nrm);
     }
     __syncthreads();
-    for (int r = tid; r < nu; r += blockDim.x) Qb[r] /= L.dctl[0];
+    for (int r = tid; r < nu; r += blockDim.x) {
+      const double v = L.z[r] / L.dctl[0];
+      L.qv[r] = v;
+      Qb[r] = v;
+    }
     __syncthreads();
-    const double tol = pow(2.220446049250313e-16, 0.7);
+    const double tol = pow(kEps, 0.7);
     int check = kb / 2;
     if (check < 10) check = 10;
     int tenth = (int)floor(nu / 10.0);
     if (tenth < 1) tenth = 1;
     if (tenth < check) check = tenth;
-    int upper = kb, lower = 0, completed = nu, tri_len = 0, converged = 0;
+    // product decomposition: (row tile, column chunk) units spread evenly;
+    // the partials and the Gram-Schmidt coefficients share the work area
+    const int ntile = (nu + KPC_WAVE - 1) / KPC_WAVE;
+    int nchunk = 1;
+    while (nchunk < 8 && (ntile * nchunk) % nw != 0 && (2 * nchunk + 1) * nu <= kSmoothWork)
+      nchunk *= 2;
+    const int cs = ((nu + nchunk - 1) / nchunk + 7) & ~7;
+    double* hc = L.w + (size_t)nchunk * nu;  // Gram-Schmidt coefficients (nu)
+    const double eta = d == 2 ? 0.5 * eta0 : eta0;
+    int upper = kb, lower = 0, completed = nu;
     for (int it = 0; it < nu; ++it) {
-      const double* qi = Qb + (size_t)it * nu;
-      for (int r = tid; r < nu; r += blockDim.x) {
-        double s = 0.0;
-        for (int c = 0; c < nu; ++c) s += E[(size_t)c * nu + r] * qi[c];
-        L.z[r] = s;
-      }
-      double part = 0.0;
-      for (int r = tid; r < nu; r += blockDim.x) part += qi[r] * L.z[r];
-      const double proj = block_sum(part, L.red);
-      if (tid == 0) L.al[it] = proj;
-      if (it == 0) {
-        for (int r = tid; r < nu; r += blockDim.x) L.z[r] -= proj * Qb[r];
-      } else {
-        const double prior = L.be[it - 1];
-        const double* qp = Qb + (size_t)(it - 1) * nu;
-        for (int r = tid; r < nu; r += blockDim.x) L.z[r] -= proj * qi[r] + prior * qp[r];
-        __syncthreads();
-        for (int pass = 0; pass < 2; ++pass) {
-          for (int b = wid; b <= it; b += nw) {
-            const double* qb = Qb + (size_t)b * nu;
-            double s = 0.0;
-            for (int r = lane; r < nu; r += KPC_WAVE) s += L.z[r] * qb[r];
-            s = wave_sum(s);
-            if (lane == 0) L.cc[b] = s;
-          }
-          __syncthreads();
-          for (int r = tid; r < nu; r += blockDim.x) {
-            double s = 0.0;
-            for (int b = 0; b <= it; ++b) s += L.cc[b] * Qb[(size_t)b * nu + r];
-            L.z[r] -= s;
-          }
-          __syncthreads();
-        }
-      }
+      if (d == 2)
+        tp_product<2>(L, nu, nchunk, cs, eta);
+      else
+        tp_product<1>(L, nu, nchunk, cs, eta);
       __syncthreads();
-      part = 0.0;
-      for (int r = tid; r < nu; r += blockDim.x) part += L.z[r] * L.z[r];
-      const double rn = sqrt(block_sum(part, L.red));
+      // Classical Gram-Schmidt against Q[0..it].  Pass 1 acts on E q_it (the
+      // sum of the chunk partials): its coefficient on q_it is the Lanczos
+      // alpha = q_it' E q_it and the one on q_{it-1} takes the place of the
+      // three-term recurrence's beta, so the recurrence and the first
+      // reorthogonalisation are one pass.  Pass 2 (it > 0) reorthogonalises
+      // again, as the reference does; at it = 0 the reference only removes
+      // the alpha component.
+      const int npass = it == 0 ? 1 : 2;
+      double rn = 0.0;
+      for (int pass = 0; pass < npass; ++pass) {
+        // coefficients: a wave takes eight basis vectors (rows past the last
+        // one are clamped to it, so every load is unconditional and all of
+        // them can be in flight together)
+        for (int b0 = wid * 8; b0 <= it; b0 += nw * 8) {
+          double g[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+          const int nb = min(8, it + 1 - b0);
+          const double* qrow[8];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) qrow[k] = Qb + (size_t)(b0 + (k < nb ? k : nb - 1)) * nu;
+#pragma unroll 2
+          for (int i = lane; i < nu; i += KPC_WAVE) {
+            double v[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v[k] = qrow[k][i];
+            double zi;
+            if (pass == 0) {
+              zi = 0.0;
+              for (int c = 0; c < nchunk; ++c) zi += L.w[c * nu + i];
+            } else {
+              zi = L.z[i];
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) g[k] += zi * v[k];
+          }
+#pragma unroll
+          for (int k = 0; k < 8; ++k) g[k] = wave_sum(g[k]);
+          if (lane < nb) {
+            double gv = g[0];
+#pragma unroll
+            for (int k = 1; k < 8; ++k) gv = lane == k ? g[k] : gv;
+            hc[b0 + lane] = gv;
+          }
+        }
+        __syncthreads();
+        double part = 0.0;
+        for (int i = tid; i < nu; i += blockDim.x) {
+          double zi;
+          if (pass == 0) {
+            zi = 0.0;
+            for (int c = 0; c < nchunk; ++c) zi += L.w[c * nu + i];
+          } else {
+            zi = L.z[i];
+          }
+          double sa[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+          int b = 0;
+#pragma unroll 2
+          for (; b + 8 <= it + 1; b += 8) {
+            double v[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v[k] = Qb[(size_t)(b + k) * nu + i];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) sa[k] += hc[b + k] * v[k];
+          }
+          for (; b <= it; ++b) sa[0] += hc[b] * Qb[(size_t)b * nu + i];
+          zi -= ((sa[0] + sa[1]) + (sa[2] + sa[3])) + ((sa[4] + sa[5]) + (sa[6] + sa[7]));
+          L.z[i] = zi;
+          part += zi * zi;
+        }
+        if (pass == 0 && tid == 0) L.al[it] = hc[it];
+        if (pass == npass - 1)
+          rn = sqrt(block_sum(part, L.red));
+        else
+          __syncthreads();
+      }
       if (tid == 0) L.be[it] = rn;
       if (it < nu - 1) {
         double* qn = Qb + (size_t)(it + 1) * nu;
-        for (int r = tid; r < nu; r += blockDim.x) qn[r] = L.z[r] / rn;
+        for (int r = tid; r < nu; r += blockDim.x) {
+          const double v = L.z[r] / rn;
+          L.qv[r] = v;
+          qn[r] = v;
+        }
       }
       __syncthreads();
-      if ((it >= kb && it % check == 0) || it == nu - 1) {
-        const int size = it + 1;
-        // QL on a copy of (alpha, beta)
-        // al is destroyed by QL, so keep the Lanczos alpha in tv temporarily
-        for (int i = tid; i < size; i += blockDim.x) {
-          L.tv[i] = L.al[i];
-          L.cc[i] = (i < size - 1) ? L.be[i] : 0.0;
+      if (!((it >= kb && it % check == 0) || it == nu - 1)) continue;
+      // ---- Ritz check on the tridiagonal T (size x size)
+      const int size = it + 1;
+      double gl = 1e308, gu = -1e308, emx = 0.0;
+      for (int i = tid; i < size; i += blockDim.x) {
+        const double rad = (i > 0 ? fabs(L.be[i - 1]) : 0.0) + (i < size - 1 ? fabs(L.be[i]) : 0.0);
+        gl = fmin(gl, L.al[i] - rad);
+        gu = fmax(gu, L.al[i] + rad);
+        if (i < size - 1) {
+          const double b2 = L.be[i] * L.be[i];
+          L.e2[i] = b2;
+          emx = fmax(emx, b2);
+        }
+      }
+      gl = -block_max(-gl, L.red);
+      gu = block_max(gu, L.red);
+      emx = block_max(emx, L.red);
+      const double pivmin = 2.2250738585072014e-308 * fmax(1.0, emx);
+      const double span = gu - gl;
+      const double lo0 = gl - 2.0 * kEps * span - pivmin;
+      const double hi0 = gu + 2.0 * kEps * span + pivmin;
+      const int kk = kb < size ? kb : size;
+      const int nlow = kk;
+      const int hs = (size - kk) > nlow ? (size - kk) : nlow;
+      const int ncand = nlow + (size - hs);
+      lz_multisect(L.al, L.e2, size, pivmin, ncand, nlow, hs, 2.0 * kEps, true, lo0, hi0, L.clo,
+                   L.chi, L.cval);
+      // eigenvalue with descending index j
+      auto desc_val = [&](int j) {
+        const int a = size - 1 - j;
+        return L.cval[a < nlow ? a : nlow + (a - hs)];
+      };
+      const int per = 2 * size;
+      const int B = kSmoothWork / per > 0 ? kSmoothWork / per : 1;
+      for (int j0 = 0; j0 < kk; j0 += B) {
+        const int j = j0 + tid;
+        if (tid < B && j < kk) {
+          double* wsl = L.w + (size_t)tid * per;
+          const double last = lz_twisted(L.al, L.be, size, desc_val(j), pivmin, wsl, wsl + size,
+                                         nullptr);
+          L.errs[j] = fabs(rn * last);
         }
         __syncthreads();
-        ql_eigen_parallel(L, size, Zm);
-        // order: descending eigenvalues; restore alpha
-        if (tid == 0) {
-          for (int i = 0; i < size; ++i) L.perm[i] = i;
-          for (int i = 1; i < size; ++i) {
-            int pi = L.perm[i];
-            int j = i - 1;
-            while (j >= 0 && L.al[L.perm[j]] < L.al[pi]) { L.perm[j + 1] = L.perm[j]; --j; }
-            L.perm[j + 1] = pi;
-          }
-        }
-        __syncthreads();
-        for (int i = tid; i < size; i += blockDim.x) {
-          const double lam = L.al[L.perm[i]];
-          L.ss[i] = lam;  // descending Ritz values (temporary)
-        }
-        __syncthreads();
-        for (int i = tid; i < size; i += blockDim.x) {
-          const double a = L.tv[i];
-          L.tv[i] = L.ss[i];
-          L.al[i] = a;
-        }
-        __syncthreads();
-        if (tid == 0) {
-          double tnorm = fabs(L.tv[0]);
-          if (fabs(L.tv[it]) > tnorm) tnorm = fabs(L.tv[it]);
-          int conv = 0, pos = 0, neg = 0;
-          if (it >= kb) {
-            const double maxerr = tnorm * tol;
-            conv = 1;
-            while (pos + neg < kb) {
-              if (fabs(L.tv[pos]) >= fabs(L.tv[it - neg])) {
-                double err = fabs(rn * Zm[(size_t)L.perm[pos] * size + it]);
-                if (err > maxerr) { conv = 0; break; }
-                ++pos;
-              } else {
-                double err = fabs(rn * Zm[(size_t)L.perm[neg] * size + it]);  // index quirk
-                if (err > maxerr) { conv = 0; break; }
-                ++neg;
-              }
+      }
+      if (tid == 0) {
+        double tnorm = fabs(desc_val(0));
+        if (fabs(desc_val(it)) > tnorm) tnorm = fabs(desc_val(it));
+        int conv = 0, pos = 0, neg = 0;
+        if (it >= kb) {
+          const double maxerr = tnorm * tol;
+          conv = 1;
+          while (pos + neg < kb) {
+            if (fabs(desc_val(pos)) >= fabs(desc_val(it - neg))) {
+              if (L.errs[pos] > maxerr) { conv = 0; break; }
+              ++pos;
+            } else {
+              if (L.errs[neg] > maxerr) { conv = 0; break; }  // reference index quirk
+              ++neg;
             }
           }
-          L.ctl[6] = conv;
-          L.ctl[7] = pos;
-          L.ctl[5] = neg;
         }
-        __syncthreads();
-        tri_len = size;
-        if (L.ctl[6]) {
-          upper = L.ctl[7];
-          lower = L.ctl[5];
-          completed = it + 1;
-          converged = 1;
-        }
-        __syncthreads();
-        if (converged) break;
+        L.ctl[6] = conv;
+        L.ctl[7] = pos;
+        L.ctl[8] = neg;
       }
+      __syncthreads();
+      if (!L.ctl[6] && it < nu - 1) continue;
+      if (L.ctl[6]) {
+        upper = L.ctl[7];
+        lower = L.ctl[8];
+      }
+      completed = size;
+      // ---- coefficient vectors of the selected Ritz pairs into Zs (K x size)
+      const int K = upper + lower;
+      for (int e0 = 0; e0 < K; e0 += B) {
+        const int e = e0 + tid;
+        if (tid < B && e < K) {
+          const int src = e < upper ? e : size - (upper + lower - e);
+          const double th = desc_val(src);
+          double* wsl = L.w + (size_t)tid * per;
+          lz_twisted(L.al, L.be, size, th, pivmin, wsl, wsl + size, Zs + (size_t)e * size);
+          L.cm[e] = th;
+        }
+        __syncthreads();
+      }
+      // re-orthogonalise within eigenvalue clusters (relative gap < 1e-3)
+      if (wid == 0) {
+        for (int e = 1; e < K; ++e) {
+          const double le = L.cm[e];
+          double* ze = Zs + (size_t)e * size;
+          bool touched = false;
+          for (int f = 0; f < e; ++f) {
+            const double lf = L.cm[f];
+            if (fabs(le - lf) > 1e-3 * fmax(fabs(le), fabs(lf))) continue;
+            const double* zf = Zs + (size_t)f * size;
+            double s = 0.0;
+            for (int i = lane; i < size; i += KPC_WAVE) s += zf[i] * ze[i];
+            s = wave_sum(s);
+            for (int i = lane; i < size; i += KPC_WAVE) ze[i] -= s * zf[i];
+            touched = true;
+          }
+          if (touched) {
+            double s = 0.0;
+            for (int i = lane; i < size; i += KPC_WAVE) s += ze[i] * ze[i];
+            s = 1.0 / sqrt(wave_sum(s));
+            for (int i = lane; i < size; i += KPC_WAVE) ze[i] *= s;
+          }
+        }
+      }
+      __syncthreads();
+      break;
     }
-    (void)converged;
-    // ---- Ritz vectors U (nu x kb) and values
-    for (int e = 0; e < upper + lower; ++e) {
-      const int src = e < upper ? e : completed - (upper + lower - e);
-      const double* sv = Zm + (size_t)L.perm[src] * tri_len;
-      for (int r = tid; r < nu; r += blockDim.x) {
-        double s = 0.0;
-        for (int b = 0; b < completed; ++b) s += Qb[(size_t)b * nu + r] * sv[b];
-        U[(size_t)e * nu + r] = s;
+    // ---- Ritz vectors U = Q Zs (nu x K), eight vectors per thread
+    {
+      const int K = upper + lower;
+      const int ng = (K + 7) / 8;
+      for (int u = tid; u < nu * ng; u += blockDim.x) {
+        const int r = u % nu, g = u / nu;
+        const int e0 = 8 * g;
+        const int ne = min(8, K - e0);
+        double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        const double* zs = Zs + (size_t)e0 * completed;
+        for (int b = 0; b < completed; ++b) {
+          const double qb = Qb[(size_t)b * nu + r];
+#pragma unroll
+          for (int k = 0; k < 8; ++k)
+            if (k < ne) acc[k] += qb * zs[(size_t)k * completed + b];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (k < ne) U[(size_t)(e0 + k) * nu + r] = acc[k];
       }
-      if (tid == 0) L.cm[e] = L.tv[src];
     }
     __syncthreads();
     // ---- T columns [1, x (, y)]; TU = T' U (M x kb)
@@ -454,9 +597,9 @@ __global__ __launch_bounds__(kSmoothThreads) void smooth_setup_kernel(
     }
     __syncthreads();
     if (tid == 0) {
-      // QT householders: zq (input, row-major) -> stored reflectors in z[]
+      // QT householders: zq (input, row-major) -> stored reflectors in hz[]
       double* in = L.zq;
-      double* st = L.z;  // M x kb
+      double* st = L.hz;  // M x kb
       const int cols = kb;
       for (int step = 0; step < M; ++step) {
         double* rs = in + step * cols;
@@ -491,7 +634,7 @@ __global__ __launch_bounds__(kSmoothThreads) void smooth_setup_kernel(
       double row[kMaxBasis];
       for (int c = 0; c < kb; ++c) row[c] = U[(size_t)c * nu + u] * L.cm[c];
       for (int t = 0; t < M; ++t) {
-        const double* ht = L.z + t * kb;
+        const double* ht = L.hz + t * kb;
         double pr = 0.0;
         for (int c = 0; c < kb; ++c) pr += row[c] * ht[c];
         for (int c = 0; c < kb; ++c) row[c] -= pr * ht[c];
@@ -524,6 +667,14 @@ __global__ __launch_bounds__(kSmoothThreads) void smooth_setup_kernel(
       part = fmax(part, s);
     }
     const double xnorm = block_max(part, L.red);
+    // ---- penalty work in LDS: P, SP (row-major kb x kb), left (column-major),
+    // column means; prod in the (now idle) multisection brackets
+    double* P = L.w;
+    double* SP = P + kMaxBasis * kMaxBasis;
+    double* left = SP + kMaxBasis * kMaxBasis;
+    double* cmv = left + kMaxBasis * kMaxBasis;
+    double* prod = L.clo;
+    const int kk2 = kb * kb;
     // ---- column means (long double in the reference) for the constraint
     for (int c = wid; c < kb; c += nw) {
       dd s{0.0, 0.0};
@@ -531,55 +682,61 @@ __global__ __launch_bounds__(kSmoothThreads) void smooth_setup_kernel(
       s = wave_sum_dd(s);
       if (lane == 0) {
         dd m = dd_div_int(s, (double)n);
-        L.cc[c] = m.hi + m.lo;
+        cmv[c] = m.hi + m.lo;
       }
     }
+    // ---- penalty P = Z' D Z: diag(D), hq_post then hq_pre_transpose
+    for (int i = tid; i < kk2; i += blockDim.x) P[i] = (i / kb == i % kb) ? L.cm[i / kb] : 0.0;
     __syncthreads();
+    for (int t = 0; t < M; ++t) {  // P <- P (I - h h')
+      const double* ht = L.hz + t * kb;
+      for (int r = tid; r < kb; r += blockDim.x) {
+        double v = 0.0;
+        for (int c = 0; c < kb; ++c) v += P[r * kb + c] * ht[c];
+        prod[r] = v;
+      }
+      __syncthreads();
+      for (int i = tid; i < kk2; i += blockDim.x) P[i] -= prod[i / kb] * ht[i % kb];
+      __syncthreads();
+    }
+    for (int t = 0; t < M; ++t) {  // P <- (I - h h') P
+      const double* ht = L.hz + t * kb;
+      for (int c = tid; c < kb; c += blockDim.x) {
+        double v = 0.0;
+        for (int r = 0; r < kb; ++r) v += P[r * kb + c] * ht[r];
+        prod[c] = v;
+      }
+      __syncthreads();
+      for (int i = tid; i < kk2; i += blockDim.x) P[i] -= prod[i % kb] * ht[i / kb];
+      __syncthreads();
+    }
+    // zero null-space rows/columns; RMS scaling (row c then column c for
+    // ascending c: an entry is divided by the smaller index's rms first)
+    for (int i = tid; i < kk2; i += blockDim.x) {
+      const int r = i / kb, c = i % kb;
+      double v = (r >= kb - M || c >= kb - M) ? 0.0 : P[i];
+      const int a = r < c ? r : c, b = r < c ? c : r;
+      v /= L.rms[a];
+      v /= L.rms[b];
+      P[i] = v;
+    }
+    __syncthreads();
+    for (int i = tid; i < kk2; i += blockDim.x) {
+      const int r = i / kb, c = i % kb;
+      SP[i] = (P[r * kb + c] + P[c * kb + r]) / 2.0;
+    }
+    __syncthreads();
+    double cpart = 0.0;
+    for (int c = tid; c < kb; c += blockDim.x) {
+      double s = 0.0;
+      for (int r = 0; r < kb; ++r) s += fabs(SP[r * kb + c]);
+      cpart = s;
+    }
+    const double pnorm = block_max(cpart, L.red);
+    const double pscale = pnorm / (xnorm * xnorm);
+    for (int i = tid; i < kk2; i += blockDim.x) SP[i] /= pscale;
     if (tid == 0) {
-      // penalty P = Z' D Z with the reflectors, zero null space, RMS scaling
-      double* P = Xs + (size_t)n * kb;  // kb x kb row-major in scratch
-      for (int i = 0; i < kb * kb; ++i) P[i] = 0.0;
-      for (int i = 0; i < kb; ++i) P[i * kb + i] = L.cm[i];
-      double prod[kMaxBasis];
-      for (int t = 0; t < M; ++t) {  // hq_post
-        const double* ht = L.z + t * kb;
-        for (int r = 0; r < kb; ++r) {
-          double v = 0.0;
-          for (int c = 0; c < kb; ++c) v += P[r * kb + c] * ht[c];
-          prod[r] = v;
-        }
-        for (int r = 0; r < kb; ++r)
-          for (int c = 0; c < kb; ++c) P[r * kb + c] -= prod[r] * ht[c];
-      }
-      for (int t = 0; t < M; ++t) {  // hq_pre_transpose
-        const double* ht = L.z + t * kb;
-        for (int c = 0; c < kb; ++c) {
-          double v = 0.0;
-          for (int r = 0; r < kb; ++r) v += P[r * kb + c] * ht[r];
-          prod[c] = v;
-        }
-        for (int r = 0; r < kb; ++r)
-          for (int c = 0; c < kb; ++c) P[r * kb + c] -= prod[c] * ht[r];
-      }
-      for (int r = 0; r < kb; ++r)
-        for (int c = kb - M; c < kb; ++c) { P[r * kb + c] = 0.0; P[c * kb + r] = 0.0; }
-      for (int c = 0; c < kb; ++c) {
-        for (int i = 0; i < kb; ++i) P[c * kb + i] /= L.rms[c];
-        for (int i = 0; i < kb; ++i) P[i * kb + c] /= L.rms[c];
-      }
-      double* SP = P + kb * kb;
-      for (int r = 0; r < kb; ++r)
-        for (int c = 0; c < kb; ++c) SP[r * kb + c] = (P[r * kb + c] + P[c * kb + r]) / 2.0;
-      double pnorm = 0.0;
-      for (int c = 0; c < kb; ++c) {
-        double s = 0.0;
-        for (int r = 0; r < kb; ++r) s += fabs(SP[r * kb + c]);
-        if (s > pnorm) pnorm = s;
-      }
-      const double pscale = pnorm / (xnorm * xnorm);
-      for (int i = 0; i < kb * kb; ++i) SP[i] /= pscale;
       // LINPACK single-column Householder on the column means
-      double* cmv = L.cc;
       double nrm = 0.0;
       for (int i = 0; i < kb; ++i) nrm += cmv[i] * cmv[i];
       nrm = sqrt(nrm);
@@ -593,42 +750,46 @@ __global__ __launch_bounds__(kSmoothThreads) void smooth_setup_kernel(
         cmv[0] = -nrm;
       }
       L.dctl[1] = qraux;
-      // absorbed penalty: Q' S Q without first row/col
-      double* left = SP + kb * kb;  // column-major kb x kb
-      double yv[kMaxBasis];
-      for (int c = 0; c < kb; ++c) {
-        for (int r = 0; r < kb; ++r) yv[r] = SP[r * kb + c];
-        if (qraux != 0.0) {
-          double t = qraux * yv[0];
-          for (int i = 1; i < kb; ++i) t += cmv[i] * yv[i];
-          t = -t / qraux;
-          yv[0] += t * qraux;
-          for (int i = 1; i < kb; ++i) yv[i] += t * cmv[i];
-        }
-        for (int r = 0; r < kb; ++r) left[c * kb + r] = yv[r];
+    }
+    __syncthreads();
+    const double qraux = L.dctl[1];
+    // absorbed penalty: Q' S Q without the first row/column; left = Q' SP
+    for (int c = tid; c < kb; c += blockDim.x) {
+      double* yv = left + c * kb;
+      for (int r = 0; r < kb; ++r) yv[r] = SP[r * kb + c];
+      if (qraux != 0.0) {
+        double t = qraux * yv[0];
+        for (int i = 1; i < kb; ++i) t += cmv[i] * yv[i];
+        t = -t / qraux;
+        yv[0] += t * qraux;
+        for (int i = 1; i < kb; ++i) yv[i] += t * cmv[i];
       }
+    }
+    __syncthreads();
+    {
       double* So = out_S + (size_t)slot * 29 * 29;
-      for (int fr = 0; fr < kb - 1; ++fr) {
-        for (int oc = 0; oc < kb; ++oc) yv[oc] = left[oc * kb + fr + 1];
+      for (int fr = tid; fr < kb - 1; fr += blockDim.x) {
+        double t = 0.0;
         if (qraux != 0.0) {
-          double t = qraux * yv[0];
-          for (int i = 1; i < kb; ++i) t += cmv[i] * yv[i];
+          t = qraux * left[fr + 1];
+          for (int i = 1; i < kb; ++i) t += cmv[i] * left[i * kb + fr + 1];
           t = -t / qraux;
-          yv[0] += t * qraux;
-          for (int i = 1; i < kb; ++i) yv[i] += t * cmv[i];
         }
-        for (int c = 0; c < kb - 1; ++c) So[c * (kb - 1) + fr] = yv[c + 1];
+        for (int c = 0; c < kb - 1; ++c) {
+          double v = left[(c + 1) * kb + fr + 1];
+          if (qraux != 0.0) v += t * cmv[c + 1];
+          So[c * (kb - 1) + fr] = v;
+        }
       }
+    }
+    if (tid == 0) {
       info[0] = 0;
       info[1] = nu;
       info[2] = completed;
       info[3] = kb - M;
     }
-    __syncthreads();
     // ---- absorbed X rows -> column-major n x (kb-1)
     {
-      const double qraux = L.dctl[1];
-      const double* cmv = L.cc;
       double* Xo = out_X + (size_t)slot * n * 29;
       for (int r = tid; r < n; r += blockDim.x) {
         double yv[kMaxBasis];
@@ -645,13 +806,14 @@ __global__ __launch_bounds__(kSmoothThreads) void smooth_setup_kernel(
     }
     __syncthreads();
   }
+  (void)p;
 }
 
 long long smooth_scratch_doubles(int n) {
-  // E + Q basis + QL vectors + U + X (row-major) + penalty work
+  // Q basis + Ritz coefficient vectors + U + X (row-major)
   long long nu = n;
-  long long s = nu * nu + nu * (nu + 1) + nu * nu + nu * kMaxBasis + (long long)n * kMaxBasis +
-                3LL * kMaxBasis * kMaxBasis + 64;
+  long long s = nu * (nu + 1) + (long long)kMaxBasis * nu + nu * kMaxBasis +
+                (long long)n * kMaxBasis + 64;
   if (s < 2LL * n) s = 2LL * n;
   return (s + 31) / 32 * 32;
 }
