@@ -40,6 +40,29 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return t;
 }
 
+// Two workgroup sums at once (one pair of barriers); `red` must hold
+// 2 * (blockDim.x / 64) doubles.
+__device__ __forceinline__ void block_sum2(double& a, double& b, double* red) {
+  const int lane = threadIdx.x & (KPC_WAVE - 1);
+  const int wid = threadIdx.x / KPC_WAVE;
+  const int nw = (blockDim.x + KPC_WAVE - 1) / KPC_WAVE;
+  a = wave_sum(a);
+  b = wave_sum(b);
+  __syncthreads();
+  if (lane == 0) {
+    red[2 * wid] = a;
+    red[2 * wid + 1] = b;
+  }
+  __syncthreads();
+  double ta = 0.0, tb = 0.0;
+  for (int w = 0; w < nw; ++w) {
+    ta += red[2 * w];
+    tb += red[2 * w + 1];
+  }
+  a = ta;
+  b = tb;
+}
+
 __device__ __forceinline__ double block_max(double v, double* red) {
   const int lane = threadIdx.x & (KPC_WAVE - 1);
   const int wid = threadIdx.x / KPC_WAVE;

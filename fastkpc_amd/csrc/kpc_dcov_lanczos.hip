@@ -33,30 +33,54 @@ __host__ __device__ inline int lanczos_ivw(int mmax) {
   return 4096 > 6 * mmax + 64 ? 4096 : 6 * mmax + 64;
 }
 
-// Top-kk candidates by |value| (ties toward the higher ascending index),
-// lane 0: sval/sidx receive values and candidate slots.
+// Top-kk candidates by |value| (ties toward the higher ascending index): a
+// candidate's rank is the number of candidates ahead of it in that order, and
+// ranks below kk fill sval/sidx in order (one thread per candidate).
 __device__ void lz_select(const double* cval, int ncand, int nlow, int hs, int kk, double* sval,
-                          int* sidx, int* used) {
-  if (threadIdx.x == 0) {
-    for (int c = 0; c < ncand; ++c) used[c] = 0;
-    for (int s2 = 0; s2 < kk; ++s2) {
-      int best = -1;
-      for (int c = 0; c < ncand; ++c) {
-        if (used[c]) continue;
-        if (best < 0) { best = c; continue; }
-        const double a = fabs(cval[c]), b = fabs(cval[best]);
-        const int ic = c < nlow ? c : hs + (c - nlow), ib = best < nlow ? best : hs + (best - nlow);
-        if (a > b || (a == b && ic > ib)) best = c;
-      }
-      sval[s2] = cval[best];
-      sidx[s2] = best;
-      used[best] = 1;
+                          int* sidx) {
+  for (int c = threadIdx.x; c < ncand; c += blockDim.x) {
+    const double a = fabs(cval[c]);
+    const int ic = c < nlow ? c : hs + (c - nlow);
+    int rank = 0;
+    for (int o = 0; o < ncand; ++o) {
+      const double b = fabs(cval[o]);
+      const int io = o < nlow ? o : hs + (o - nlow);
+      rank += (b > a) || (b == a && io > ic);
+    }
+    if (rank < kk) {
+      sval[rank] = cval[c];
+      sidx[rank] = c;
     }
   }
   __syncthreads();
 }
 
-__global__ __launch_bounds__(kLzThreads) void dcov_lanczos_kernel(
+// Z[:, e] = Q S[:, e] for e < k (n x m basis, coefficient vectors at stride
+// sst): a thread takes one row and eight vectors, so every basis entry is
+// loaded once per group; the coefficient sums run over the basis in order.
+__device__ inline void ritz_assemble(const double* __restrict__ Q, const double* Sv, int sst,
+                                     int n, int m, int k, double* __restrict__ Z) {
+  const int ng = (k + 7) / 8;
+  for (int u = threadIdx.x; u < n * ng; u += blockDim.x) {
+    const int i = u % n, g = u / n;
+    const int e0 = 8 * g;
+    const int ne = min(8, k - e0);
+    const double* sv = Sv + (size_t)e0 * sst;
+    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int b = 0; b < m; ++b) {
+      const double qb = Q[(size_t)b * n + i];
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+        if (c < ne) acc[c] += qb * sv[(size_t)c * sst + b];
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+      if (c < ne) Z[(size_t)(e0 + c) * n + i] = acc[c];
+  }
+}
+
+__global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
     const double* __restrict__ src_base, const long long* __restrict__ src_off,
     const int* __restrict__ dst_slot, int count, int n, int k, double* __restrict__ comp_vals,
     double* __restrict__ comp_vecs, double* __restrict__ comp_dsum,
@@ -151,14 +175,14 @@ __global__ __launch_bounds__(kLzThreads) void dcov_lanczos_kernel(
           a7 += fabs(ri - v7.x) * v7.y;
         }
         for (; l < n; ++l) a0 += fabs(ri - rq[l].x) * rq[l].y;
-        const double sacc = ((a0 + a1) + (a2 + a3)) + ((a4 + a5) + (a6 + a7));
-        z[i] = sacc;
-        part += rq[i].y * sacc;
+        const double zi = ((a0 + a1) + (a2 + a3)) + ((a4 + a5) + (a6 + a7));
+        z[i] = zi;
+        part += rq[i].y * zi;
       }
       const double aj = block_sum(part, red);
-      if (tid == 0) al[j] = aj;
+      // three-term recurrence
       const double bprev = j > 0 ? be[j - 1] : 0.0;
-      const double* qp = j > 0 ? Q + (size_t)(j - 1) * n : Q;
+      const double* qp = Q + (size_t)(j > 0 ? j - 1 : 0) * n;
       part = 0.0;
       for (int i = tid; i < n; i += blockDim.x) {
         const double v = z[i] - (aj * rq[i].y + bprev * qp[i]);
@@ -166,65 +190,61 @@ __global__ __launch_bounds__(kLzThreads) void dcov_lanczos_kernel(
         part += v * v;
       }
       double zn2 = block_sum(part, red);
-      // classical Gram-Schmidt against Q[0..j], repeated while the norm drops
-      // below 1/sqrt(2) of its value before the pass (DGKS, at most twice); a
-      // wave forms eight coefficients at once (lanes over rows, coalesced),
-      // then every row subtracts its combination with four accumulators
+      // Classical Gram-Schmidt against Q[0..j], repeated once more when a pass
+      // leaves less than half the squared norm it started from (DGKS).  A
+      // wave forms eight coefficients at once (lanes over rows, coalesced;
+      // basis rows past j are clamped to j so every load is unconditional),
+      // then each row subtracts its combination with eight accumulators.
       for (int pass = 0; pass < 2; ++pass) {
         for (int b0 = wid * 8; b0 <= j; b0 += nw * 8) {
-          double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0, c4 = 0.0, c5 = 0.0, c6 = 0.0, c7 = 0.0;
+          double g[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
           const int nb = min(8, j + 1 - b0);
-          const double* qb = Q + (size_t)b0 * n;
+          const double* qrow[8];
+#pragma unroll
+          for (int c = 0; c < 8; ++c) qrow[c] = Q + (size_t)(b0 + (c < nb ? c : nb - 1)) * n;
+#pragma unroll 2
           for (int i = lane; i < n; i += KPC_WAVE) {
+            double v[8];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) v[c] = qrow[c][i];
             const double zi = z[i];
-            c0 += zi * qb[i];
-            if (nb > 1) c1 += zi * qb[(size_t)n + i];
-            if (nb > 2) c2 += zi * qb[(size_t)2 * n + i];
-            if (nb > 3) c3 += zi * qb[(size_t)3 * n + i];
-            if (nb > 4) c4 += zi * qb[(size_t)4 * n + i];
-            if (nb > 5) c5 += zi * qb[(size_t)5 * n + i];
-            if (nb > 6) c6 += zi * qb[(size_t)6 * n + i];
-            if (nb > 7) c7 += zi * qb[(size_t)7 * n + i];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) g[c] += zi * v[c];
           }
-          c0 = wave_sum(c0);
-          c1 = wave_sum(c1);
-          c2 = wave_sum(c2);
-          c3 = wave_sum(c3);
-          c4 = wave_sum(c4);
-          c5 = wave_sum(c5);
-          c6 = wave_sum(c6);
-          c7 = wave_sum(c7);
-          if (lane == 0) {
-            cg[b0] = c0;
-            if (nb > 1) cg[b0 + 1] = c1;
-            if (nb > 2) cg[b0 + 2] = c2;
-            if (nb > 3) cg[b0 + 3] = c3;
-            if (nb > 4) cg[b0 + 4] = c4;
-            if (nb > 5) cg[b0 + 5] = c5;
-            if (nb > 6) cg[b0 + 6] = c6;
-            if (nb > 7) cg[b0 + 7] = c7;
+#pragma unroll
+          for (int c = 0; c < 8; ++c) g[c] = wave_sum(g[c]);
+          if (lane < nb) {
+            double gv = g[0];
+#pragma unroll
+            for (int c = 1; c < 8; ++c) gv = lane == c ? g[c] : gv;
+            cg[b0 + lane] = gv;
           }
         }
         __syncthreads();
-        for (int i = tid; i < n; i += blockDim.x) {
-          double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-          int b = 0;
-          for (; b + 4 <= j + 1; b += 4) {
-            s0 += cg[b] * Q[(size_t)b * n + i];
-            s1 += cg[b + 1] * Q[(size_t)(b + 1) * n + i];
-            s2 += cg[b + 2] * Q[(size_t)(b + 2) * n + i];
-            s3 += cg[b + 3] * Q[(size_t)(b + 3) * n + i];
-          }
-          for (; b <= j; ++b) s0 += cg[b] * Q[(size_t)b * n + i];
-          z[i] -= (s0 + s1) + (s2 + s3);
-        }
         part = 0.0;
-        for (int i = tid; i < n; i += blockDim.x) part += z[i] * z[i];
+        for (int i = tid; i < n; i += blockDim.x) {
+          double sa[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+          int b = 0;
+#pragma unroll 2
+          for (; b + 8 <= j + 1; b += 8) {
+            double v[8];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) v[c] = Q[(size_t)(b + c) * n + i];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) sa[c] += cg[b + c] * v[c];
+          }
+          for (; b <= j; ++b) sa[0] += cg[b] * Q[(size_t)b * n + i];
+          const double zi =
+              z[i] - (((sa[0] + sa[1]) + (sa[2] + sa[3])) + ((sa[4] + sa[5]) + (sa[6] + sa[7])));
+          z[i] = zi;
+          part += zi * zi;
+        }
         const double after = block_sum(part, red);
         const bool again = after < 0.5 * zn2;
         zn2 = after;
         if (!again) break;
       }
+      if (tid == 0) al[j] = aj;
       const double bj = sqrt(zn2);
       if (tid == 0) be[j] = bj;
       // breakdown: the Krylov space is invariant to working precision (the
@@ -241,7 +261,7 @@ __global__ __launch_bounds__(kLzThreads) void dcov_lanczos_kernel(
       const int m = j + 1;
       const bool check = m >= next_check || m == mmax || breakdown;
       if (!check) continue;
-      // extreme eigenvalues of T (ascending indices {0..k-1} U {max(k, m-k)..m-1})
+      // Gershgorin bounds of T
       for (int i = tid; i < m - 1; i += blockDim.x) e2[i] = be[i] * be[i];
       double gl = 1e308, gu = -1e308, tn = 0.0, emx = 0.0;
       for (int i = tid; i < m; i += blockDim.x) {
@@ -261,14 +281,29 @@ __global__ __launch_bounds__(kLzThreads) void dcov_lanczos_kernel(
       const double hi0 = gu + 2.0 * 2.220446049250313e-16 * span + pivmin;
       const int kk = k < m ? k : m;
       const int nlow = kk;
-      const int hs = (m - kk) > nlow ? (m - kk) : nlow;
-      const int ncand = nlow + (m - hs);
-      // Ritz values to full precision, eigenvectors of T by twisted
-      // factorisations (pivot workspace in LDS, batches of eigenpairs), exact
-      // Ritz residuals |beta_m s_m|
-      lz_multisect(al, e2, m, pivmin, ncand, nlow, hs, 2.0 * 2.220446049250313e-16, true, lo0,
-                   hi0, clo, chi, cval);
-      lz_select(cval, ncand, nlow, hs, kk, sval, sidx, cidx);
+      // Ritz values to full precision: the kk lowest and the hcnt highest
+      // eigenvalues of T.  When some computed high candidate is left out of
+      // the top kk by |value|, every uncomputed middle eigenvalue is too (it
+      // lies between the lowest-kk and the highest-hcnt sets, so either the
+      // unselected high candidate or all kk low ones dominate it); otherwise
+      // hcnt doubles.  A distance matrix has one positive eigenvalue, so the
+      // first pass usually decides.
+      int hcnt = kk < 4 ? kk : 4;
+      int hs = 0;
+      for (;;) {
+        hs = (m - hcnt) > nlow ? (m - hcnt) : nlow;
+        const int ncand = nlow + (m - hs);
+        lz_multisect(al, e2, m, pivmin, ncand, nlow, hs, 2.0 * 2.220446049250313e-16,
+                     2.0 * 2.220446049250313e-16 * tn, true, lo0, hi0, clo, chi, cval);
+        lz_select(cval, ncand, nlow, hs, kk, sval, sidx);
+        int high = 0;
+        for (int c = 0; c < kk; ++c) high += sidx[c] >= nlow;
+        if (high < m - hs || hs == nlow || hcnt >= kk) break;
+        hcnt = 2 * hcnt < kk ? 2 * hcnt : kk;
+        __syncthreads();
+      }
+      // eigenvectors of T by twisted factorisations (pivot workspace in LDS,
+      // batches of eigenpairs), exact Ritz residuals |beta_m s_m|
       {
         const int per = 2 * m;
         const int B = ivw / per > 0 ? ivw / per : 1;
@@ -328,14 +363,15 @@ __global__ __launch_bounds__(kLzThreads) void dcov_lanczos_kernel(
       for (int b = 0; b < m_final; ++b) sv[b] = 0.0;
     }
     __syncthreads();
-    // Ritz vectors U = Q S -> output
+    // Ritz vectors U = Q S -> output (coefficients staged in LDS when they fit)
     double* Z = comp_vecs + (size_t)slot * n * k;
-    for (int e = tid; e < n * k; e += blockDim.x) {
-      const int i = e % n, c = e / n;
-      const double* sv = S + (size_t)c * mmax;
-      double s = 0.0;
-      for (int b = 0; b < m_final; ++b) s += Q[(size_t)b * n + i] * sv[b];
-      Z[(size_t)c * n + i] = s;
+    if ((long long)k * m_final <= ivw) {
+      for (int e = tid; e < k * m_final; e += blockDim.x)
+        ivl[e] = S[(size_t)(e / m_final) * mmax + e % m_final];
+      __syncthreads();
+      ritz_assemble(Q, ivl, m_final, n, m_final, k, Z);
+    } else {
+      ritz_assemble(Q, S, mmax, n, m_final, k, Z);
     }
     __syncthreads();
     for (int c = wid; c < k; c += nw) {
@@ -346,14 +382,48 @@ __global__ __launch_bounds__(kLzThreads) void dcov_lanczos_kernel(
       for (int i = lane; i < n; i += KPC_WAVE) zc[i] -= s;
     }
     __syncthreads();
+    // self moment sum_ij lambda_i lambda_j (z_i . z_j)^2 over the centred
+    // vectors: the Gram entries (i <= j) accumulate over blocks of 64 rows
+    // staged in LDS (row-major 64 x k), in row order
+    double gacc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const int npairs = k * (k + 1) / 2;
+    for (int r0 = 0; r0 < n; r0 += 64) {
+      const int rows = min(64, n - r0);
+      for (int e = tid; e < rows * k; e += blockDim.x) {
+        const int c = e / rows, r = e % rows;
+        ivl[r * k + c] = Z[(size_t)c * n + r0 + r];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < 5; ++t) {
+        int pidx = tid + t * (int)blockDim.x;
+        if (pidx < npairs) {
+          int i = 0;
+          while (pidx >= k - i) {
+            pidx -= k - i;
+            ++i;
+          }
+          const int jj = i + pidx;
+          double sacc = gacc[t];
+          for (int r = 0; r < rows; ++r) sacc += ivl[r * k + i] * ivl[r * k + jj];
+          gacc[t] = sacc;
+        }
+      }
+      __syncthreads();
+    }
     double mpart = 0.0;
-    for (int e = tid; e < k * k; e += blockDim.x) {
-      const int i = e % k, jj = e / k;
-      const double* ci = Z + (size_t)i * n;
-      const double* cj = Z + (size_t)jj * n;
-      double s = 0.0;
-      for (int rr = 0; rr < n; ++rr) s += ci[rr] * cj[rr];
-      mpart += sval[i] * sval[jj] * s * s;
+#pragma unroll
+    for (int t = 0; t < 5; ++t) {
+      int pidx = tid + t * (int)blockDim.x;
+      if (pidx < npairs) {
+        int i = 0;
+        while (pidx >= k - i) {
+          pidx -= k - i;
+          ++i;
+        }
+        const int jj = i + pidx;
+        mpart += (jj == i ? 1.0 : 2.0) * (sval[i] * sval[jj] * gacc[t] * gacc[t]);
+      }
     }
     const double moment = block_sum(mpart, red);
     if (tid == 0) {

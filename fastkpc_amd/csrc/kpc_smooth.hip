@@ -24,11 +24,11 @@
 //  * The eta of d = 2 uses tp_log (atanh series, within 2 ulp) instead of
 //    the library logarithm, whose double-double compensation triples the
 //    cost of the product.
-//  * The three-term recurrence and the reference's two modified Gram-Schmidt
-//    passes become two classical Gram-Schmidt passes over the whole basis
-//    (the first one on E q itself, so alpha and the beta term are among its
-//    coefficients); both schemes are orthogonal to working precision.  A
-//    wave forms eight coefficients at once with coalesced basis reads.
+//  * The reference's two modified Gram-Schmidt passes after the three-term
+//    recurrence become classical Gram-Schmidt passes, the second one only
+//    when the first leaves less than half the squared norm (DGKS); both
+//    schemes are orthogonal to working precision.  A wave forms eight
+//    coefficients at once with coalesced, unconditional basis reads.
 //  * A Ritz check needs the k largest and k smallest eigenvalues of the
 //    Lanczos tridiagonal and the last components of the k leading
 //    eigenvectors: Sturm multisection and twisted factorisations give them
@@ -366,19 +366,32 @@ __global__ __launch_bounds__(kSmoothThreads) void smooth_setup_kernel(
       else
         tp_product<1>(L, nu, nchunk, cs, eta);
       __syncthreads();
-      // Classical Gram-Schmidt against Q[0..it].  Pass 1 acts on E q_it (the
-      // sum of the chunk partials): its coefficient on q_it is the Lanczos
-      // alpha = q_it' E q_it and the one on q_{it-1} takes the place of the
-      // three-term recurrence's beta, so the recurrence and the first
-      // reorthogonalisation are one pass.  Pass 2 (it > 0) reorthogonalises
-      // again, as the reference does; at it = 0 the reference only removes
-      // the alpha component.
-      const int npass = it == 0 ? 1 : 2;
-      double rn = 0.0;
-      for (int pass = 0; pass < npass; ++pass) {
-        // coefficients: a wave takes eight basis vectors (rows past the last
-        // one are clamped to it, so every load is unconditional and all of
-        // them can be in flight together)
+      // z = E q_it (sum of the chunk partials), alpha, three-term recurrence
+      double part = 0.0;
+      for (int r = tid; r < nu; r += blockDim.x) {
+        double sz = 0.0;
+        for (int c = 0; c < nchunk; ++c) sz += L.w[c * nu + r];
+        L.z[r] = sz;
+        part += L.qv[r] * sz;
+      }
+      const double proj = block_sum(part, L.red);
+      if (tid == 0) L.al[it] = proj;
+      const double prior = it > 0 ? L.be[it - 1] : 0.0;
+      const double* qp = Qb + (size_t)(it > 0 ? it - 1 : 0) * nu;
+      part = 0.0;
+      for (int r = tid; r < nu; r += blockDim.x) {
+        const double v = L.z[r] - (proj * L.qv[r] + prior * qp[r]);
+        L.z[r] = v;
+        part += v * v;
+      }
+      double zn2 = block_sum(part, L.red);
+      // Reorthogonalisation (it > 0; at it = 0 the reference only removes the
+      // alpha component): classical Gram-Schmidt against Q[0..it], repeated
+      // once more when a pass leaves less than half the squared norm it
+      // started from (DGKS).  A wave forms eight coefficients at once (rows
+      // past the last basis vector are clamped to it, so every load is
+      // unconditional and all of them can be in flight together).
+      for (int pass = 0; pass < 2 && it > 0; ++pass) {
         for (int b0 = wid * 8; b0 <= it; b0 += nw * 8) {
           double g[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
           const int nb = min(8, it + 1 - b0);
@@ -390,13 +403,7 @@ __global__ __launch_bounds__(kSmoothThreads) void smooth_setup_kernel(
             double v[8];
 #pragma unroll
             for (int k = 0; k < 8; ++k) v[k] = qrow[k][i];
-            double zi;
-            if (pass == 0) {
-              zi = 0.0;
-              for (int c = 0; c < nchunk; ++c) zi += L.w[c * nu + i];
-            } else {
-              zi = L.z[i];
-            }
+            const double zi = L.z[i];
 #pragma unroll
             for (int k = 0; k < 8; ++k) g[k] += zi * v[k];
           }
@@ -410,15 +417,8 @@ __global__ __launch_bounds__(kSmoothThreads) void smooth_setup_kernel(
           }
         }
         __syncthreads();
-        double part = 0.0;
+        part = 0.0;
         for (int i = tid; i < nu; i += blockDim.x) {
-          double zi;
-          if (pass == 0) {
-            zi = 0.0;
-            for (int c = 0; c < nchunk; ++c) zi += L.w[c * nu + i];
-          } else {
-            zi = L.z[i];
-          }
           double sa[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
           int b = 0;
 #pragma unroll 2
@@ -430,16 +430,17 @@ __global__ __launch_bounds__(kSmoothThreads) void smooth_setup_kernel(
             for (int k = 0; k < 8; ++k) sa[k] += hc[b + k] * v[k];
           }
           for (; b <= it; ++b) sa[0] += hc[b] * Qb[(size_t)b * nu + i];
-          zi -= ((sa[0] + sa[1]) + (sa[2] + sa[3])) + ((sa[4] + sa[5]) + (sa[6] + sa[7]));
+          const double zi =
+              L.z[i] - (((sa[0] + sa[1]) + (sa[2] + sa[3])) + ((sa[4] + sa[5]) + (sa[6] + sa[7])));
           L.z[i] = zi;
           part += zi * zi;
         }
-        if (pass == 0 && tid == 0) L.al[it] = hc[it];
-        if (pass == npass - 1)
-          rn = sqrt(block_sum(part, L.red));
-        else
-          __syncthreads();
+        const double after = block_sum(part, L.red);
+        const bool again = after < 0.5 * zn2;
+        zn2 = after;
+        if (!again) break;
       }
+      const double rn = sqrt(zn2);
       if (tid == 0) L.be[it] = rn;
       if (it < nu - 1) {
         double* qn = Qb + (size_t)(it + 1) * nu;
@@ -475,8 +476,8 @@ __global__ __launch_bounds__(kSmoothThreads) void smooth_setup_kernel(
       const int nlow = kk;
       const int hs = (size - kk) > nlow ? (size - kk) : nlow;
       const int ncand = nlow + (size - hs);
-      lz_multisect(L.al, L.e2, size, pivmin, ncand, nlow, hs, 2.0 * kEps, true, lo0, hi0, L.clo,
-                   L.chi, L.cval);
+      lz_multisect(L.al, L.e2, size, pivmin, ncand, nlow, hs, 2.0 * kEps,
+                   2.0 * kEps * fmax(fabs(lo0), fabs(hi0)), true, lo0, hi0, L.clo, L.chi, L.cval);
       // eigenvalue with descending index j
       auto desc_val = [&](int j) {
         const int a = size - 1 - j;

@@ -90,11 +90,14 @@ __device__ inline double lz_twisted(const double* al, const double* be, int m, d
 // (al, e2): candidate c has ascending index idx(c) = c < nlow ? c : hs + c - nlow.
 // G lanes per candidate probe G interior points of its bracket per round (the
 // bracket shrinks by G + 1); rounds stop when every bracket is below
-// rtol max(|lo|, |hi|) + pivmin.  Brackets persist in clo/chi (init: start
-// from [lo0, hi0]); cval receives the midpoints.  All threads must call it.
+// rtol max(|lo|, |hi|) + atol + pivmin (Sturm counts are exact only for a
+// perturbation of T of order eps |T|, so atol ~ eps |T| costs no accuracy and
+// bounds the rounds for eigenvalues near zero).  Brackets persist in clo/chi
+// (init: start from [lo0, hi0]); cval receives the midpoints.  All threads
+// must call it.
 __device__ inline void lz_multisect(const double* al, const double* e2, int m, double pivmin, int ncand,
-                             int nlow, int hs, double rtol, bool init, double lo0, double hi0,
-                             double* clo, double* chi, double* cval) {
+                             int nlow, int hs, double rtol, double atol, bool init, double lo0,
+                             double hi0, double* clo, double* chi, double* cval) {
   const int tid = threadIdx.x;
   const int lane = tid & (KPC_WAVE - 1);
   int G = ncand > 0 ? (int)blockDim.x / ncand : 1;
@@ -112,7 +115,7 @@ __device__ inline void lz_multisect(const double* al, const double* e2, int m, d
   }
   for (int round = 0; round < 400; ++round) {
     const double w = hi - lo;
-    const bool done = !own || w <= rtol * fmax(fabs(lo), fabs(hi)) + pivmin;
+    const bool done = !own || w <= rtol * fmax(fabs(lo), fabs(hi)) + atol + pivmin;
     if (__syncthreads_and(done ? 1 : 0)) break;
     const double step = w / (double)(G + 1);
     int flag = 0;
