@@ -303,28 +303,81 @@ __global__ __launch_bounds__(kDcovThreads) void dcov_component_kernel(
 // ------------------------------------------------------------------ pair
 // One workgroup per pair: nV2 = sum_ij lx_i ly_j (Cx'Cy)_ij^2 / n, kpcalg
 // moments and the gamma tail (fastkpc/src/legacy_dcov_gamma_cpp.cpp:420-455).
-__global__ __launch_bounds__(kDcovThreads) void dcov_pair_kernel(
+// dcov.gamma statistic of one test from the two components: with centred
+// vectors X (n x k), Y (n x k) and values lx, ly,
+// n V^2 = sum_ij lx_i ly_j (X_i . Y_j)^2 / n.  C = X'Y is a small GEMM: rows
+// are staged through LDS in chunks of kPairRows (row-major, k padded to kp),
+// each thread accumulates one or two 4 x 4 blocks of C in registers, and every
+// C_ij is summed over the rows in order (as a plain dot product would).
+constexpr int kPairThreads = 128;
+constexpr int kPairRows = 64;
+
+__device__ __forceinline__ void pair_block_accumulate(const double* Xc, const double* Yc, int kp,
+                                                      int rows, int bi, int bj, double* acc) {
+  for (int r = 0; r < rows; ++r) {
+    const double4 a = *reinterpret_cast<const double4*>(Xc + r * kp + 4 * bi);
+    const double4 b = *reinterpret_cast<const double4*>(Yc + r * kp + 4 * bj);
+    const double av[4] = {a.x, a.y, a.z, a.w};
+    const double bv[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[u * 4 + v] += av[u] * bv[v];
+  }
+}
+
+__global__ __launch_bounds__(kPairThreads) void dcov_pair_kernel(
     const int* __restrict__ pair_x, const int* __restrict__ pair_y, int count, int n, int k,
     const double* __restrict__ comp_vals, const double* __restrict__ comp_vecs,
     const double* __restrict__ comp_dsum, const double* __restrict__ comp_moment,
     double* __restrict__ out) {
+  extern __shared__ double lds_pair[];
   __shared__ double red[16];
   const int tid = threadIdx.x;
+  const int kp = (k + 3) & ~3;
+  const int nb = kp / 4;
+  double* Xc = lds_pair;
+  double* Yc = lds_pair + kPairRows * kp;
+  const int b0 = tid, b1 = tid + (int)blockDim.x;
+  const bool has0 = b0 < nb * nb, has1 = b1 < nb * nb;
   for (int item = blockIdx.x; item < count; item += gridDim.x) {
     const int cx = pair_x[item], cy = pair_y[item];
     const double* X = comp_vecs + (size_t)cx * n * k;
     const double* Y = comp_vecs + (size_t)cy * n * k;
     const double* lx = comp_vals + (size_t)cx * k;
     const double* ly = comp_vals + (size_t)cy * k;
-    double part = 0.0;
-    for (int e = tid; e < k * k; e += blockDim.x) {
-      const int i = e % k, j = e / k;
-      const double* a = X + (size_t)i * n;
-      const double* b = Y + (size_t)j * n;
-      double s = 0.0;
-      for (int r = 0; r < n; ++r) s += a[r] * b[r];
-      part += lx[i] * ly[j] * s * s;
+    double acc0[16], acc1[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      acc0[u] = 0.0;
+      acc1[u] = 0.0;
     }
+    for (int r0 = 0; r0 < n; r0 += kPairRows) {
+      const int rows = min(kPairRows, n - r0);
+      __syncthreads();
+      for (int e = tid; e < rows * kp; e += blockDim.x) {
+        const int i = e / rows, r = e % rows;
+        Xc[r * kp + i] = i < k ? X[(size_t)i * n + r0 + r] : 0.0;
+        Yc[r * kp + i] = i < k ? Y[(size_t)i * n + r0 + r] : 0.0;
+      }
+      __syncthreads();
+      if (has0) pair_block_accumulate(Xc, Yc, kp, rows, b0 % nb, b0 / nb, acc0);
+      if (has1) pair_block_accumulate(Xc, Yc, kp, rows, b1 % nb, b1 / nb, acc1);
+    }
+    double part = 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        if (has0) {
+          const int i = 4 * (b0 % nb) + u, j = 4 * (b0 / nb) + v;
+          if (i < k && j < k) part += lx[i] * ly[j] * acc0[u * 4 + v] * acc0[u * 4 + v];
+        }
+        if (has1) {
+          const int i = 4 * (b1 % nb) + u, j = 4 * (b1 / nb) + v;
+          if (i < k && j < k) part += lx[i] * ly[j] * acc1[u * 4 + v] * acc1[u * 4 + v];
+        }
+      }
     const double total = block_sum(part, red);
     if (tid == 0) {
       const double nd = (double)n;
@@ -339,7 +392,6 @@ __global__ __launch_bounds__(kDcovThreads) void dcov_pair_kernel(
       o[3] = var;
       o[4] = 1e300;
     }
-    __syncthreads();
   }
 }
 
@@ -373,7 +425,10 @@ hipError_t launch_dcov_pairs(const int* px, const int* py, int count, int n, int
                              const double* vals, const double* vecs, const double* dsum,
                              const double* moment, double* out, int grid, hipStream_t stream) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(dcov_pair_kernel, dim3(grid), dim3(kDcovThreads), 0, stream, px, py, count,
+  const int kp = (k + 3) & ~3;
+  if ((kp / 4) * (kp / 4) > 2 * kPairThreads) return hipErrorInvalidValue;
+  const size_t lds = (size_t)2 * kPairRows * kp * sizeof(double);
+  hipLaunchKernelGGL(dcov_pair_kernel, dim3(grid), dim3(kPairThreads), lds, stream, px, py, count,
                      n, k, vals, vecs, dsum, moment, out);
   return hipGetLastError();
 }
