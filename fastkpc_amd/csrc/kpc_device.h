@@ -92,19 +92,44 @@ namespace kpc {
 hipError_t launch_inchol_components(const double* src_base, const long long* src_off,
                                     const int* dst_slot, int count, int n, int k, double sigma,
                                     double* vecs, double* offdiag, double* moment, int* rank,
-                                    double* margin, int grid, hipStream_t stream);
-hipError_t launch_hsic_pairs(const int* pa, const int* pb, const unsigned long long* seeds,
-                             int count, int n, int k, const double* vecs, const int* rank,
-                             const double* offdiag, const double* moment, int permute,
-                             int replicates, int include_observed, double* out, int grid,
-                             hipStream_t stream);
+                                    double* margin, double* rows, int kp, int grid,
+                                    hipStream_t stream);
+hipError_t launch_hsic_pairs(const int* pa, const int* pb, int count, int n, int k,
+                             const double* vecs, const int* rank, const double* offdiag,
+                             const double* moment, double* out, int grid, hipStream_t stream);
 hipError_t launch_dcc_rowmeans(const double* src_base, const long long* src_off,
                                const int* dst_slot, int count, int n, int stride, double index,
                                double* vecs, double* grand, int grid, hipStream_t stream);
-hipError_t launch_dcc_perm_pairs(const int* pa, const int* pb, const unsigned long long* seeds,
-                                 int count, int n, int stride, double index, const double* vecs,
-                                 const double* grand, int replicates, int include_observed,
-                                 double* out, int grid, hipStream_t stream);
+
+// Permutation tests (kpc_perm.hip).  A unit is `nr` consecutive replicates
+// r0 .. r0 + nr of test `test`; its permutations and values live at slots
+// slot .. slot + nr of the chunk's scratch.
+struct PermUnit {
+  int test;
+  int r0;
+  int nr;
+  int slot;
+};
+constexpr int kHsicPermBlock = 16;  // replicates per hsic.perm unit (4 per wave)
+int perm_generate_width(int n);     // replicates per generation unit
+size_t dcc_perm_lds_bytes(int n, int rb);
+hipError_t launch_perm_generate(const PermUnit* units, int nunits,
+                                const unsigned long long* seeds, int n, unsigned short* perms,
+                                int grid, hipStream_t stream);
+// perms == nullptr: identity permutation (the observed statistic)
+hipError_t launch_hsic_perm_values(const PermUnit* units, int nunits, const int* pa,
+                                   const int* pb, int n, int kt, const double* rows,
+                                   const unsigned short* perms, double* vals, int grid,
+                                   hipStream_t stream);
+hipError_t launch_dcc_perm_values(const PermUnit* units, int nunits, const int* pa, const int* pb,
+                                  int n, int stride, double index, const double* vecs,
+                                  const double* grand, const unsigned short* perms, double* vals,
+                                  int rb, int grid, hipStream_t stream);
+hipError_t launch_perm_tally(const PermUnit* segs, int nsegs, const double* vals,
+                             const double* obs, double scale, double* state, hipStream_t stream);
+hipError_t launch_perm_finalize(int count, const double* obs, const double* state, double scale,
+                                int replicates, int include_observed, int one_if_nonpositive,
+                                double* out, hipStream_t stream);
 // Device-generated permutation table (the indices the permutation kernels
 // consume), for verification against the oracle.
 hipError_t launch_permutation_table(const unsigned long long* seeds, int count, int n,

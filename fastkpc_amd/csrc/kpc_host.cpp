@@ -225,6 +225,7 @@ class Engine {
     comp_mom_.reserve(max_resid);
     comp_status_.reserve(max_resid);
     comp_margin_.reserve(max_resid);
+    if (method_ == KPC_HSIC_PERM) comp_rows_.reserve(max_resid * (size_t)n_ * 16 * hsic_kt());
   }
 
   // Evaluate p-values for the given tasks (all with |S| == level_).
@@ -425,22 +426,114 @@ class Engine {
                   "dcov pair launch");
         break;
       case KPC_HSIC_GAMMA:
-      case KPC_HSIC_PERM:
-        hip_check(kpc::launch_hsic_pairs(d_pair_x_.ptr, d_pair_y_.ptr, d_seeds_.ptr, count, n_, k_,
+        hip_check(kpc::launch_hsic_pairs(d_pair_x_.ptr, d_pair_y_.ptr, count, n_, k_,
                                          comp_vecs_.ptr, comp_status_.ptr, comp_dsum_.ptr,
-                                         comp_mom_.ptr, method_ == KPC_HSIC_PERM ? 1 : 0,
-                                         opt_.replicates, opt_.include_observed, d_pair_out_.ptr,
-                                         grid, stream_),
+                                         comp_mom_.ptr, d_pair_out_.ptr, grid, stream_),
                   "hsic pair launch");
         break;
       default:
-        hip_check(kpc::launch_dcc_perm_pairs(d_pair_x_.ptr, d_pair_y_.ptr, d_seeds_.ptr, count, n_,
-                                             n_ * comp_k_, index_, comp_vecs_.ptr, comp_dsum_.ptr,
-                                             opt_.replicates, opt_.include_observed,
-                                             d_pair_out_.ptr, grid, stream_),
-                  "dcc.perm pair launch");
+        perm_pairs(count);
         break;
     }
+  }
+
+  int hsic_kt() const { return (k_ + 15) / 16; }
+
+  // Permutation tests of `count` pairs: observed statistics (identity
+  // permutation), then chunks of at most perm_chunk_replicates() replicates:
+  // generate the index arrays, evaluate every replicate, tally per test in
+  // replicate order; finally p, statistic, mean, variance and margin.
+  long long perm_chunk_replicates() const {
+    const long long b = kPermScratchBytes / (2LL * n_);
+    return b < 1 ? 1 : b;
+  }
+
+  void perm_values(const kpc::PermUnit* units, int nunits, const unsigned short* perms,
+                   double* vals) {
+    const int grid = std::min(nunits, 65535);
+    if (method_ == KPC_HSIC_PERM)
+      hip_check(kpc::launch_hsic_perm_values(units, nunits, d_pair_x_.ptr, d_pair_y_.ptr, n_,
+                                             hsic_kt(), comp_rows_.ptr, perms, vals, grid,
+                                             stream_),
+                "hsic.perm replicate launch");
+    else
+      hip_check(kpc::launch_dcc_perm_values(units, nunits, d_pair_x_.ptr, d_pair_y_.ptr, n_,
+                                            n_ * comp_k_, index_, comp_vecs_.ptr, comp_dsum_.ptr,
+                                            perms, vals, dcc_perm_rb_, grid, stream_),
+                "dcc.perm replicate launch");
+  }
+
+  void perm_pairs(int count) {
+    const bool hsic = method_ == KPC_HSIC_PERM;
+    const int R = opt_.replicates;
+    const double scale = hsic ? 1.0 : 1.0 / (double)n_;
+    const int vblock = hsic ? kpc::kHsicPermBlock : dcc_perm_rb_;
+    const int gw = kpc::perm_generate_width(n_);
+    const long long budget = perm_chunk_replicates();
+    std::vector<kpc::PermUnit> obs_units((size_t)count);
+    std::vector<double> state((size_t)4 * count);
+    for (int t = 0; t < count; ++t) {
+      obs_units[t] = {t, 0, 1, t};
+      state[4 * t] = opt_.include_observed ? 1.0 : 0.0;
+      state[4 * t + 1] = 0.0;
+      state[4 * t + 2] = 0.0;
+      state[4 * t + 3] = 1e300;
+    }
+    d_perm_units_.upload(obs_units, stream_);
+    perm_state_.upload(state, stream_);
+    perm_obs_.reserve((size_t)count);
+    perm_values(d_perm_units_.ptr, count, nullptr, perm_obs_.ptr);
+    hip_check(hipStreamSynchronize(stream_), "permutation synchronize");
+    if (R > 0) {
+      const long long chunk_reps = std::min(budget, (long long)count * R);
+      perm_scratch_.reserve((size_t)chunk_reps * n_);
+      perm_vals_.reserve((size_t)chunk_reps);
+      // one chunk: [segments | generation units | value units]; the host list
+      // is the source of an asynchronous copy, so each chunk ends with a
+      // stream synchronisation before the list is rebuilt
+      std::vector<kpc::PermUnit> segs, gen, val;
+      long long used = 0;
+      auto flush = [&]() {
+        if (segs.empty()) return;
+        const size_t ns = segs.size(), ng = gen.size(), nv = val.size();
+        segs.insert(segs.end(), gen.begin(), gen.end());
+        segs.insert(segs.end(), val.begin(), val.end());
+        d_perm_units_.upload(segs, stream_);
+        const kpc::PermUnit* U = d_perm_units_.ptr;
+        hip_check(kpc::launch_perm_generate(U + ns, (int)ng, d_seeds_.ptr, n_, perm_scratch_.ptr,
+                                            (int)std::min<size_t>(ng, 8192), stream_),
+                  "permutation generate launch");
+        perm_values(U + ns + ng, (int)nv, perm_scratch_.ptr, perm_vals_.ptr);
+        hip_check(kpc::launch_perm_tally(U, (int)ns, perm_vals_.ptr, perm_obs_.ptr, scale,
+                                         perm_state_.ptr, stream_),
+                  "permutation tally launch");
+        hip_check(hipStreamSynchronize(stream_), "permutation synchronize");
+        segs.clear();
+        gen.clear();
+        val.clear();
+        used = 0;
+      };
+      for (int t = 0; t < count; ++t) {
+        int r = 0;
+        while (r < R) {
+          const int take = (int)std::min<long long>(R - r, budget - used);
+          const int slot0 = (int)used;
+          segs.push_back({t, r, take, slot0});
+          for (int b = 0; b < take; b += gw)
+            gen.push_back({t, r + b, std::min(gw, take - b), slot0 + b});
+          for (int b = 0; b < take; b += vblock)
+            val.push_back({t, r + b, std::min(vblock, take - b), slot0 + b});
+          used += take;
+          r += take;
+          if (used == budget) flush();
+        }
+      }
+      flush();
+    }
+    hip_check(kpc::launch_perm_finalize(count, perm_obs_.ptr, perm_state_.ptr, scale, R,
+                                        opt_.include_observed, hsic ? 0 : 1, d_pair_out_.ptr,
+                                        stream_),
+              "permutation finalize launch");
   }
 
   void launch_components(const double* src, size_t count) {
@@ -449,7 +542,9 @@ class Engine {
       hip_check(kpc::launch_inchol_components(src, d_comp_src_.ptr, d_comp_dst_.ptr, (int)count, n_,
                                               k_, 1.0 / opt_.hsic_sig, comp_vecs_.ptr,
                                               comp_dsum_.ptr, comp_mom_.ptr, comp_status_.ptr,
-                                              comp_margin_.ptr, grid, stream_),
+                                              comp_margin_.ptr,
+                                              method_ == KPC_HSIC_PERM ? comp_rows_.ptr : nullptr,
+                                              16 * hsic_kt(), grid, stream_),
                 "inchol launch");
       return;
     }
@@ -665,6 +760,12 @@ class Engine {
   DevArray<long long> d_comp_src_;
   DevArray<int> d_comp_dst_, d_pair_x_, d_pair_y_;
   DevArray<double> d_pair_out_;
+  // permutation tests
+  static constexpr long long kPermScratchBytes = 1LL << 30;
+  int dcc_perm_rb_ = 16;
+  DevArray<double> comp_rows_, perm_vals_, perm_obs_, perm_state_;
+  DevArray<unsigned short> perm_scratch_;
+  DevArray<kpc::PermUnit> d_perm_units_;
   double stage_[5] = {0, 0, 0, 0, 0};
   long long launches_[5] = {0, 0, 0, 0, 0};
   double stage_host_ = 0.0;

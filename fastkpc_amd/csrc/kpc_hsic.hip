@@ -16,19 +16,11 @@
 //            (fastkpc/src/cuda/full_cuda_ci_method_batch.cu:284-339); the
 //            double-centred matrix is never materialised.
 //
-// Pairs (one workgroup per test, strided):
-//   hsic     HSIC = |Gcx^T Gcy|_F^2 / n^2; gamma moments and tail
-//            (kpcalg/R/hsicgamma.R:74-83), or R permutation replicates of the
-//            second factor's rows (kpcalg/R/hsicperm.R:77-88).
-//   dcc.perm T = sum_ij A_ij B_ij and replicates sum_ij A_ij B_{pi(i) pi(j)},
-//            A and B double-centred distances evaluated on the fly from LDS
-//            over the upper triangle of 64 x 64 tiles (off-diagonal tiles
-//            count twice); statistic T / n, p = exceed / (R + include_observed)
-//            (fastkpc/src/cuda/full_cuda_ci_method_batch.cu:341-443).
-//
-// Permutations are generated in LDS by the workgroup itself, one replicate per
-// lane: Durstenfeld shuffle driven by Philox4x32-10 (key = task seed, counter
-// {i >> 1, replicate, 0x4b504331, 0}), the same index arrays the oracle draws.
+// Pairs (hsic.gamma; one workgroup per test, strided):
+//   HSIC = |Gcx^T Gcy|_F^2 / n^2; gamma moments and tail
+//   (kpcalg/R/hsicgamma.R:74-83).  The permutation tests (hsic.perm, dcc.perm)
+//   are in kpc_perm.hip; for hsic.perm the inchol kernel also writes the
+//   centred factor row-major, zero-padded to 16 KT columns.
 #include "kpc_common.h"
 #include "kpc_device.h"
 #include "kpc_gamma.h"
@@ -38,59 +30,6 @@ namespace kpc {
 constexpr int kHsicThreads = 256;
 constexpr int kRowsPerThread = kMaxN / kHsicThreads;
 constexpr double kIncholTol = 0.001;
-constexpr int kPermLdsBytes = 48 * 1024;
-
-// ------------------------------------------------------------ permutations
-__device__ __forceinline__ void philox4x32_10(uint32_t& c0, uint32_t& c1, uint32_t& c2,
-                                              uint32_t& c3, uint32_t k0, uint32_t k1) {
-#pragma unroll
-  for (int r = 0; r < 10; ++r) {
-    if (r > 0) {
-      k0 += 0x9E3779B9u;
-      k1 += 0xBB67AE85u;
-    }
-    const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-    const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
-    const uint32_t n0 = hi1 ^ c1 ^ k0;
-    const uint32_t n2 = hi0 ^ c3 ^ k1;
-    c0 = n0;
-    c1 = lo1;
-    c2 = n2;
-    c3 = lo0;
-  }
-}
-
-template <typename T>
-__device__ void make_permutation(unsigned long long seed, int rep, int n, T* perm) {
-  for (int i = 0; i < n; ++i) perm[i] = (T)i;
-  const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
-  uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
-  int blk = -1;
-  for (int i = n - 1; i >= 1; --i) {
-    if ((i >> 1) != blk) {
-      blk = i >> 1;
-      w0 = (uint32_t)blk;
-      w1 = (uint32_t)rep;
-      w2 = 0x4b504331u;
-      w3 = 0u;
-      philox4x32_10(w0, w1, w2, w3, k0, k1);
-    }
-    const unsigned long long u = (i & 1) ? ((unsigned long long)w3 << 32 | w2)
-                                         : ((unsigned long long)w1 << 32 | w0);
-    const int j = (int)__umul64hi(u, (unsigned long long)(i + 1));
-    const T t = perm[i];
-    perm[i] = perm[j];
-    perm[j] = t;
-  }
-}
-
-__global__ void permutation_table_kernel(const unsigned long long* __restrict__ seeds, int count,
-                                         int n, int replicates, int* __restrict__ out) {
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= (long long)count * replicates) return;
-  const int item = (int)(g / replicates), rep = (int)(g % replicates);
-  make_permutation<int>(seeds[item], rep, n, out + g * n);
-}
 
 // First maximum over the workgroup: largest value, smallest index on ties.
 __device__ __forceinline__ void block_argmax_first(double& v, int& idx, double* rv, int* ri) {
@@ -126,7 +65,8 @@ __global__ __launch_bounds__(kHsicThreads) void inchol_kernel(
     const double* __restrict__ src_base, const long long* __restrict__ src_off,
     const int* __restrict__ dst_slot, int count, int n, int k, double sigma,
     double* __restrict__ vecs, double* __restrict__ offdiag, double* __restrict__ moment,
-    int* __restrict__ rank_out, double* __restrict__ margin_out) {
+    int* __restrict__ rank_out, double* __restrict__ margin_out, double* __restrict__ rows,
+    int kp) {
   __shared__ double s_row[kMaxNumCol];
   __shared__ double red[16];
   __shared__ int redi[16];
@@ -258,16 +198,23 @@ __global__ __launch_bounds__(kHsicThreads) void inchol_kernel(
       rank_out[slot] = rank;
       margin_out[slot] = mg;
     }
+    // row-major copy of the centred factor, zero-padded to kp columns (the
+    // operand layout of the hsic.perm replicate kernel)
+    if (rows) {
+      double* R = rows + (size_t)slot * n * kp;
+      for (int e = tid; e < n * kp; e += blockDim.x) {
+        const int i = e / kp, c = e - (e / kp) * kp;
+        R[e] = c < k ? G[i + (size_t)n * c] : 0.0;
+      }
+    }
     __syncthreads();
   }
 }
 
 // ------------------------------------------------------------- hsic pairs
-// Sum over this thread's (a, b) entries of (sum_i A[i,a] B[perm(i),b])^2.
-template <bool kPerm>
+// Sum over this thread's (a, b) entries of (sum_i A[i,a] B[i,b])^2.
 __device__ __forceinline__ double hsic_cross_part(const double* __restrict__ A, int ra,
-                                                  const double* __restrict__ B, int rb, int n,
-                                                  const unsigned short* perm) {
+                                                  const double* __restrict__ B, int rb, int n) {
   double part = 0.0;
   const int ne = ra * rb;
   for (int e = threadIdx.x; e < ne; e += blockDim.x) {
@@ -275,48 +222,17 @@ __device__ __forceinline__ double hsic_cross_part(const double* __restrict__ A, 
     const double* ga = A + (size_t)n * a;
     const double* gb = B + (size_t)n * b;
     double dot = 0.0;
-    for (int i = 0; i < n; ++i) dot += ga[i] * gb[kPerm ? (int)perm[i] : i];
+    for (int i = 0; i < n; ++i) dot += ga[i] * gb[i];
     part += dot * dot;
   }
   return part;
 }
 
-struct PermTally {
-  int exceed;
-  double mean, m2, margin;
-};
-
-__device__ __forceinline__ void tally(PermTally& t, double v, double observed, double scale,
-                                      int count_before) {
-  if (v >= observed) ++t.exceed;
-  const double g = fabs(v - observed) / (fabs(observed) > 0.0 ? fabs(observed) : 1.0);
-  t.margin = fmin(t.margin, g);
-  const double s = v * scale;
-  const double d = s - t.mean;
-  t.mean += d / (double)(count_before + 1);
-  t.m2 += d * (s - t.mean);
-}
-
-__device__ __forceinline__ void perm_record(double* o, const PermTally& t, double observed,
-                                            double scale, int replicates, int include_observed,
-                                            bool one_if_nonpositive) {
-  const int denom = replicates + (include_observed ? 1 : 0);
-  double p = denom > 0 ? (double)t.exceed / (double)denom : 1.0;
-  if (one_if_nonpositive && !(observed > 0.0)) p = 1.0;
-  o[0] = p;
-  o[1] = observed * scale;
-  o[2] = t.mean;
-  o[3] = replicates > 1 ? t.m2 / (double)(replicates - 1) : 0.0;
-  o[4] = t.margin;
-}
-
 __global__ __launch_bounds__(kHsicThreads) void hsic_pair_kernel(
-    const int* __restrict__ pa, const int* __restrict__ pb,
-    const unsigned long long* __restrict__ seeds, int count, int n, int k,
+    const int* __restrict__ pa, const int* __restrict__ pb, int count, int n, int k,
     const double* __restrict__ vecs, const int* __restrict__ rank,
-    const double* __restrict__ offdiag, const double* __restrict__ moment, int permute,
-    int replicates, int include_observed, int rbatch, double* __restrict__ out) {
-  extern __shared__ unsigned short perm_lds[];
+    const double* __restrict__ offdiag, const double* __restrict__ moment,
+    double* __restrict__ out) {
   __shared__ double red[16];
   const int tid = threadIdx.x;
   const double nd = (double)n;
@@ -325,43 +241,23 @@ __global__ __launch_bounds__(kHsicThreads) void hsic_pair_kernel(
     const int ra = rank[ca], rb = rank[cb];
     const double* A = vecs + (size_t)ca * n * k;
     const double* B = vecs + (size_t)cb * n * k;
-    const double observed = block_sum(hsic_cross_part<false>(A, ra, B, rb, n, nullptr), red) /
-                            (nd * nd);
+    const double observed = block_sum(hsic_cross_part(A, ra, B, rb, n), red) / (nd * nd);
     double* o = out + (size_t)item * kPairOut;
-    if (!permute) {
-      if (tid == 0) {
-        const double mux = offdiag[ca] / (nd * (nd - 1.0));
-        const double muy = offdiag[cb] / (nd * (nd - 1.0));
-        const double mean = (1.0 + mux * muy - mux - muy) / nd;
-        const double f =
-            2.0 * (nd - 4.0) * (nd - 5.0) / (nd * (nd - 1.0) * (nd - 2.0) * (nd - 3.0));
-        const double var = f * moment[ca] * moment[cb] / (nd * nd * nd * nd);
-        const bool ok = isfinite(observed) && isfinite(mean) && isfinite(var) && mean > 0.0 &&
-                        var > 0.0;
-        o[0] = ok ? gamma_pvalue(observed, mean, var) : 1.0;
-        o[1] = observed;
-        o[2] = mean;
-        o[3] = var;
-        o[4] = 1e300;
-      }
-      __syncthreads();
-      continue;
+    if (tid == 0) {
+      const double mux = offdiag[ca] / (nd * (nd - 1.0));
+      const double muy = offdiag[cb] / (nd * (nd - 1.0));
+      const double mean = (1.0 + mux * muy - mux - muy) / nd;
+      const double f =
+          2.0 * (nd - 4.0) * (nd - 5.0) / (nd * (nd - 1.0) * (nd - 2.0) * (nd - 3.0));
+      const double var = f * moment[ca] * moment[cb] / (nd * nd * nd * nd);
+      const bool ok = isfinite(observed) && isfinite(mean) && isfinite(var) && mean > 0.0 &&
+                      var > 0.0;
+      o[0] = ok ? gamma_pvalue(observed, mean, var) : 1.0;
+      o[1] = observed;
+      o[2] = mean;
+      o[3] = var;
+      o[4] = 1e300;
     }
-    PermTally t{include_observed ? 1 : 0, 0.0, 0.0, 1e300};
-    const unsigned long long seed = seeds[item];
-    for (int base = 0; base < replicates; base += rbatch) {
-      const int nb = min(rbatch, replicates - base);
-      __syncthreads();
-      if (tid < nb) make_permutation<unsigned short>(seed, base + tid, n, perm_lds + (size_t)tid * n);
-      __syncthreads();
-      for (int r = 0; r < nb; ++r) {
-        const double v =
-            block_sum(hsic_cross_part<true>(A, ra, B, rb, n, perm_lds + (size_t)r * n), red) /
-            (nd * nd);
-        if (tid == 0) tally(t, v, observed, 1.0, base + r);
-      }
-    }
-    if (tid == 0) perm_record(o, t, observed, 1.0, replicates, include_observed, false);
     __syncthreads();
   }
 }
@@ -401,122 +297,25 @@ __global__ __launch_bounds__(kHsicThreads) void dcc_rowmean_kernel(
   }
 }
 
-// sum_ij A_ij B_ij over the upper triangle of 64 x 64 tiles, A_ij =
-// ((D_ij - a_i) - a_j) + abar from XA = {x, a} and likewise B from YB.
-template <bool kUnitIndex>
-__device__ __forceinline__ double dcc_tile_sum(const double2* XA, const double2* YB, int n,
-                                               double abar, double bbar, double index) {
-  const int lane = threadIdx.x & (KPC_WAVE - 1);
-  const int wid = threadIdx.x / KPC_WAVE;
-  const int nw = blockDim.x / KPC_WAVE;
-  const int ns = (n + KPC_WAVE - 1) / KPC_WAVE;
-  const int npair = ns * (ns + 1) / 2;
-  double acc = 0.0;
-  for (int sp = wid; sp < npair; sp += nw) {
-    int s = 0, rem = sp;
-    while (rem >= ns - s) {
-      rem -= ns - s;
-      ++s;
-    }
-    const int s2 = s + rem;
-    const int i = s * KPC_WAVE + lane;
-    double tile = 0.0;
-    if (i < n) {
-      const double2 xa = XA[i];
-      const double2 yb = YB[i];
-      const int j0 = s2 * KPC_WAVE;
-      const int j1 = min(n, j0 + KPC_WAVE);
-      for (int j = j0; j < j1; ++j) {
-        const double2 xj = XA[j];
-        const double2 yj = YB[j];
-        const double a = ((dist_pow<kUnitIndex>(xa.x - xj.x, index) - xa.y) - xj.y) + abar;
-        const double b = ((dist_pow<kUnitIndex>(yb.x - yj.x, index) - yb.y) - yj.y) + bbar;
-        tile += a * b;
-      }
-    }
-    acc += s == s2 ? tile : 2.0 * tile;
-  }
-  return acc;
-}
-
-template <bool kUnitIndex>
-__global__ __launch_bounds__(kHsicThreads) void dcc_perm_kernel(
-    const int* __restrict__ pa, const int* __restrict__ pb,
-    const unsigned long long* __restrict__ seeds, int count, int n, int stride, double index,
-    const double* __restrict__ vecs, const double* __restrict__ grand, int replicates,
-    int include_observed, int rbatch, double* __restrict__ out) {
-  extern __shared__ double2 dlds[];
-  double2* XA = dlds;
-  double2* YB = XA + n;
-  double2* YP = YB + n;
-  unsigned short* perm = (unsigned short*)(YP + n);
-  __shared__ double red[16];
-  const int tid = threadIdx.x;
-  const double inv_n = 1.0 / (double)n;
-  for (int item = blockIdx.x; item < count; item += gridDim.x) {
-    const int ca = pa[item], cb = pb[item];
-    const double* VA = vecs + (size_t)ca * stride;
-    const double* VB = vecs + (size_t)cb * stride;
-    const double abar = grand[ca], bbar = grand[cb];
-    for (int i = tid; i < n; i += blockDim.x) {
-      XA[i] = make_double2(VA[i], VA[n + i]);
-      YB[i] = make_double2(VB[i], VB[n + i]);
-    }
-    __syncthreads();
-    const double observed =
-        block_sum(dcc_tile_sum<kUnitIndex>(XA, YB, n, abar, bbar, index), red);
-    PermTally t{include_observed ? 1 : 0, 0.0, 0.0, 1e300};
-    const unsigned long long seed = seeds[item];
-    for (int base = 0; base < replicates; base += rbatch) {
-      const int nb = min(rbatch, replicates - base);
-      __syncthreads();
-      if (tid < nb) make_permutation<unsigned short>(seed, base + tid, n, perm + (size_t)tid * n);
-      __syncthreads();
-      for (int r = 0; r < nb; ++r) {
-        const unsigned short* pr = perm + (size_t)r * n;
-        for (int i = tid; i < n; i += blockDim.x) YP[i] = YB[pr[i]];
-        __syncthreads();
-        const double v = block_sum(dcc_tile_sum<kUnitIndex>(XA, YP, n, abar, bbar, index), red);
-        if (tid == 0) tally(t, v, observed, inv_n, base + r);
-      }
-    }
-    if (tid == 0)
-      perm_record(out + (size_t)item * kPairOut, t, observed, inv_n, replicates, include_observed,
-                  true);
-    __syncthreads();
-  }
-}
-
 // ---------------------------------------------------------------- launchers
-static int perm_batch(int n, int replicates) {
-  int rb = kPermLdsBytes / (2 * n);
-  rb = rb < 1 ? 1 : rb;
-  rb = rb > kHsicThreads ? kHsicThreads : rb;
-  return replicates < rb ? (replicates > 0 ? replicates : 1) : rb;
-}
-
 hipError_t launch_inchol_components(const double* src_base, const long long* src_off,
                                     const int* dst_slot, int count, int n, int k, double sigma,
                                     double* vecs, double* offdiag, double* moment, int* rank,
-                                    double* margin, int grid, hipStream_t stream) {
+                                    double* margin, double* rows, int kp, int grid,
+                                    hipStream_t stream) {
   if (count <= 0) return hipSuccess;
-  if (n > kMaxN || k > kMaxNumCol) return hipErrorInvalidValue;
+  if (n > kMaxN || k > kMaxNumCol || (rows && kp < k)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(inchol_kernel, dim3(grid), dim3(kHsicThreads), 0, stream, src_base, src_off,
-                     dst_slot, count, n, k, sigma, vecs, offdiag, moment, rank, margin);
+                     dst_slot, count, n, k, sigma, vecs, offdiag, moment, rank, margin, rows, kp);
   return hipGetLastError();
 }
 
-hipError_t launch_hsic_pairs(const int* pa, const int* pb, const unsigned long long* seeds,
-                             int count, int n, int k, const double* vecs, const int* rank,
-                             const double* offdiag, const double* moment, int permute,
-                             int replicates, int include_observed, double* out, int grid,
-                             hipStream_t stream) {
+hipError_t launch_hsic_pairs(const int* pa, const int* pb, int count, int n, int k,
+                             const double* vecs, const int* rank, const double* offdiag,
+                             const double* moment, double* out, int grid, hipStream_t stream) {
   if (count <= 0) return hipSuccess;
-  const int rb = perm_batch(n, replicates);
-  const size_t lds = permute && replicates > 0 ? (size_t)rb * n * sizeof(unsigned short) : 0;
-  hipLaunchKernelGGL(hsic_pair_kernel, dim3(grid), dim3(kHsicThreads), lds, stream, pa, pb, seeds,
-                     count, n, k, vecs, rank, offdiag, moment, permute, replicates,
-                     include_observed, rb, out);
+  hipLaunchKernelGGL(hsic_pair_kernel, dim3(grid), dim3(kHsicThreads), 0, stream, pa, pb, count,
+                     n, k, vecs, rank, offdiag, moment, out);
   return hipGetLastError();
 }
 
@@ -531,40 +330,6 @@ hipError_t launch_dcc_rowmeans(const double* src_base, const long long* src_off,
   else
     hipLaunchKernelGGL(dcc_rowmean_kernel<false>, dim3(grid), dim3(kHsicThreads), lds, stream,
                        src_base, src_off, dst_slot, count, n, stride, index, vecs, grand);
-  return hipGetLastError();
-}
-
-hipError_t launch_dcc_perm_pairs(const int* pa, const int* pb, const unsigned long long* seeds,
-                                 int count, int n, int stride, double index, const double* vecs,
-                                 const double* grand, int replicates, int include_observed,
-                                 double* out, int grid, hipStream_t stream) {
-  if (count <= 0) return hipSuccess;
-  const int rb = perm_batch(n, replicates);
-  const size_t lds = (size_t)3 * n * sizeof(double2) + (size_t)rb * n * sizeof(unsigned short);
-  const void* fn = index == 1.0 ? (const void*)dcc_perm_kernel<true>
-                                : (const void*)dcc_perm_kernel<false>;
-  hipError_t err =
-      hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (err != hipSuccess) return err;
-  if (index == 1.0)
-    hipLaunchKernelGGL(dcc_perm_kernel<true>, dim3(grid), dim3(kHsicThreads), lds, stream, pa, pb,
-                       seeds, count, n, stride, index, vecs, grand, replicates, include_observed,
-                       rb, out);
-  else
-    hipLaunchKernelGGL(dcc_perm_kernel<false>, dim3(grid), dim3(kHsicThreads), lds, stream, pa,
-                       pb, seeds, count, n, stride, index, vecs, grand, replicates,
-                       include_observed, rb, out);
-  return hipGetLastError();
-}
-
-hipError_t launch_permutation_table(const unsigned long long* seeds, int count, int n,
-                                    int replicates, int* out, hipStream_t stream) {
-  const long long total = (long long)count * replicates;
-  if (total <= 0) return hipSuccess;
-  const int threads = 256;
-  const long long blocks = (total + threads - 1) / threads;
-  hipLaunchKernelGGL(permutation_table_kernel, dim3((unsigned)blocks), dim3(threads), 0, stream,
-                     seeds, count, n, replicates, out);
   return hipGetLastError();
 }
 
