@@ -112,7 +112,8 @@ struct PermUnit {
 };
 constexpr int kHsicPermBlock = 16;  // replicates per hsic.perm unit (4 per wave)
 int perm_generate_width(int n);     // replicates per generation unit
-size_t dcc_perm_lds_bytes(int n, int rb);
+size_t dcc_perm_lds_bytes(int n, int rb, int jt);
+bool dcc_perm_shape_supported(int jt, int rb);
 hipError_t launch_perm_generate(const PermUnit* units, int nunits,
                                 const unsigned long long* seeds, int n, unsigned short* perms,
                                 int grid, hipStream_t stream);
@@ -124,7 +125,7 @@ hipError_t launch_hsic_perm_values(const PermUnit* units, int nunits, const int*
 hipError_t launch_dcc_perm_values(const PermUnit* units, int nunits, const int* pa, const int* pb,
                                   int n, int stride, double index, const double* vecs,
                                   const double* grand, const unsigned short* perms, double* vals,
-                                  int rb, int grid, hipStream_t stream);
+                                  int jt, int rb, int grid, hipStream_t stream);
 hipError_t launch_perm_tally(const PermUnit* segs, int nsegs, const double* vals,
                              const double* obs, double scale, double* state, hipStream_t stream);
 hipError_t launch_perm_finalize(int count, const double* obs, const double* state, double scale,

@@ -30,6 +30,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -167,6 +168,14 @@ class Engine {
     ctl_.max_probes = 5;
     ctl_.rank_tol = 1.4901161193847656e-8;
     for (int e = 0; e < 8; ++e) hip_check(hipEventCreate(&ev_[e]), "hipEventCreate");
+    if (const char* sh = std::getenv("KPC_DCC_SHAPE")) {
+      int jt = 0, rb = 0;
+      if (std::sscanf(sh, "%dx%d", &jt, &rb) == 2 && kpc::dcc_perm_shape_supported(jt, rb) &&
+          rb > 1) {
+        dcc_perm_jt_ = jt;
+        dcc_perm_rb_ = rb;
+      }
+    }
   }
   ~Engine() {
     if (d_data_) (void)hipFree(d_data_);
@@ -459,7 +468,8 @@ class Engine {
     else
       hip_check(kpc::launch_dcc_perm_values(units, nunits, d_pair_x_.ptr, d_pair_y_.ptr, n_,
                                             n_ * comp_k_, index_, comp_vecs_.ptr, comp_dsum_.ptr,
-                                            perms, vals, dcc_perm_rb_, grid, stream_),
+                                            perms, vals, dcc_perm_jt_, perms ? dcc_perm_rb_ : 1,
+                                            grid, stream_),
                 "dcc.perm replicate launch");
   }
 
@@ -762,7 +772,7 @@ class Engine {
   DevArray<double> d_pair_out_;
   // permutation tests
   static constexpr long long kPermScratchBytes = 1LL << 30;
-  int dcc_perm_rb_ = 16;
+  int dcc_perm_jt_ = 16, dcc_perm_rb_ = 8;
   DevArray<double> comp_rows_, perm_vals_, perm_obs_, perm_state_;
   DevArray<unsigned short> perm_scratch_;
   DevArray<kpc::PermUnit> d_perm_units_;

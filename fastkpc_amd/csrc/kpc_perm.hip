@@ -31,11 +31,12 @@
 // matrix of x (zero row and column sums), sum_ij A_ij B'_ij equals
 // sum_ij A_ij |y'_i - y'_j|^index for the permuted y' -- the centring of B
 // drops out.  A workgroup owns one test and a block of RB replicates; a wave
-// takes 128 x 32 tiles of the upper triangle (lane = rows i and i + 64), keeps
+// takes 128 x JT tiles of the upper triangle (lane = rows i and i + 64), keeps
 // the weighted A entries (2 off the diagonal, 1 on it, 0 below) in registers,
-// stages the 32 permuted y_j of every replicate in wave-private LDS and
+// stages the JT permuted y_j of every replicate in wave-private LDS and
 // accumulates a_ij |y'_i - y'_j| with one subtract and one FMA per entry and
-// replicate.
+// replicate.  The permutation indices of the next tile are loaded while the
+// current tile computes.  (JT, RB) is a launch-time tile shape.
 #include "kpc_common.h"
 #include "kpc_device.h"
 
@@ -200,101 +201,161 @@ __device__ __forceinline__ double dist_pow(double d, double index) {
   return kUnitIndex ? fabs(d) : pow(fabs(d), index);
 }
 
-// Tiles of the upper triangle: row blocks of 128, column blocks of 32 with
-// 32 cb + 31 >= 128 rb.
+// Tiles of the upper triangle: row blocks of 128 (two rows per lane), column
+// blocks of JT with JT cb + JT - 1 >= 128 rb.
+template <int JT>
 __device__ __forceinline__ int dcc_tile_count(int n) {
-  const int nrb = (n + 127) / 128, ncb = (n + 31) / 32;
+  constexpr int kc = 128 / JT;
+  const int nrb = (n + 127) / 128, ncb = (n + JT - 1) / JT;
   int t = 0;
-  for (int rb = 0; rb < nrb; ++rb) t += max(0, ncb - 4 * rb);
+  for (int rb = 0; rb < nrb; ++rb) t += max(0, ncb - kc * rb);
   return t;
 }
 
-template <bool kUnitIndex, int RB>
+// Replicate values sum_ij a_ij |y'_i - y'_j|^index for the RB replicates of
+// each unit.  LDS: XR = {x, row mean} per sample (both doubled for index 1,
+// so the doubled off-diagonal weight is exact: (|2x_i - 2x_j| + (2abar -
+// 2r_i)) - 2r_j), the second variable y, and per wave RB x JT staged y'_j.
+// Replicate r of a unit only ever meets replicate r's own values, in the
+// same order for every RB, so RB = 1 (the observed pass) reproduces a
+// replicate with the identity permutation bit for bit.
+template <bool kUnitIndex, int JT, int RB>
 __global__ __launch_bounds__(256) void dcc_perm_kernel(
     const PermUnit* __restrict__ units, int nunits, const int* __restrict__ pa,
     const int* __restrict__ pb, int n, int stride, double index, const double* __restrict__ vecs,
     const double* __restrict__ grand, const unsigned short* __restrict__ perms,
     double* __restrict__ vals) {
+  constexpr int kc = 128 / JT;
+  constexpr int kStage = (RB * JT + KPC_WAVE - 1) / KPC_WAVE;  // staged entries per lane
+  constexpr unsigned short kNone = 0xFFFF;                      // entry outside the data: 0
   extern __shared__ double dl[];
   __shared__ double red[4 * RB];
   const int lane = threadIdx.x & (KPC_WAVE - 1);
   const int wid = threadIdx.x / KPC_WAVE;
   const int nw = blockDim.x / KPC_WAVE;
-  double2* XA = (double2*)dl;                          // {x_i, row mean a_i}
-  double* Y = dl + 2 * n;                              // y_i
-  double* stage = dl + ((3 * n + 1) & ~1) + (size_t)wid * RB * 32;
+  double2* XR = (double2*)dl;
+  double* Y = dl + 2 * n;
+  double* stage = dl + ((3 * n + 1) & ~1) + (size_t)wid * RB * JT;
   const double2* stage2 = (const double2*)stage;
-  const int ncb = (n + 31) / 32;
-  const int ntiles = dcc_tile_count(n);
+  const double sc = kUnitIndex ? 2.0 : 1.0;
+  const int ncb = (n + JT - 1) / JT;
+  const int ntiles = dcc_tile_count<JT>(n);
   for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
     const PermUnit U = units[u];
     const double* VA = vecs + (size_t)pa[U.test] * stride;
     const double* VB = vecs + (size_t)pb[U.test] * stride;
     const double abar = grand[pa[U.test]];
+    const double abar_s = sc * abar;
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
-      XA[i] = make_double2(VA[i], VA[n + i]);
+      XR[i] = make_double2(sc * VA[i], sc * VA[n + i]);
       Y[i] = VB[i];
     }
     __syncthreads();
+    auto decode = [&](int t, int& rb, int& cb) {
+      rb = 0;
+      int rem = t;
+      while (rem >= ncb - kc * rb) {
+        rem -= ncb - kc * rb;
+        ++rb;
+      }
+      cb = kc * rb + rem;
+    };
+    // permutation indices of one tile: rows i0, i0 + 64 and the staged columns
+    unsigned short ix0[RB], ix1[RB], ixs[kStage];
+    auto fetch = [&](int t) {
+      int rb, cb;
+      decode(t, rb, cb);
+      const int i0 = 128 * rb + lane, i1 = i0 + 64, j0 = JT * cb;
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        const unsigned short* pr = perms ? perms + (size_t)(U.slot + r) * n : nullptr;
+        const bool live = r < U.nr;
+        ix0[r] = (live && i0 < n) ? (pr ? pr[i0] : (unsigned short)i0) : kNone;
+        ix1[r] = (live && i1 < n) ? (pr ? pr[i1] : (unsigned short)i1) : kNone;
+      }
+#pragma unroll
+      for (int k = 0; k < kStage; ++k) {
+        const int e = lane + KPC_WAVE * k;
+        const int r = e / JT, j = j0 + (e - (e / JT) * JT);
+        unsigned short v = kNone;
+        if (e < RB * JT && r < U.nr && j < n)
+          v = perms ? perms[(size_t)(U.slot + r) * n + j] : (unsigned short)j;
+        ixs[k] = v;
+      }
+    };
     double acc0[RB], acc1[RB];
 #pragma unroll
     for (int r = 0; r < RB; ++r) acc0[r] = acc1[r] = 0.0;
+    if (wid < ntiles) fetch(wid);
     for (int t = wid; t < ntiles; t += nw) {
-      int rb = 0, rem = t;
-      while (rem >= ncb - 4 * rb) {
-        rem -= ncb - 4 * rb;
-        ++rb;
-      }
-      const int cb = 4 * rb + rem;
-      const int i0 = 128 * rb + lane, i1 = i0 + 64, j0 = 32 * cb;
-      const bool v0 = i0 < n, v1 = i1 < n;
-      const double2 x0 = XA[v0 ? i0 : 0];
-      const double2 x1 = XA[v1 ? i1 : 0];
-      double a0[32], a1[32];
-#pragma unroll
-      for (int jj = 0; jj < 32; ++jj) {
-        const int j = j0 + jj;
-        const bool vj = j < n;
-        const double2 xj = XA[vj ? j : 0];
-        const double e0 = ((dist_pow<kUnitIndex>(x0.x - xj.x, index) - x0.y) - xj.y) + abar;
-        const double e1 = ((dist_pow<kUnitIndex>(x1.x - xj.x, index) - x1.y) - xj.y) + abar;
-        a0[jj] = (v0 && vj && j >= i0) ? (j > i0 ? 2.0 * e0 : e0) : 0.0;
-        a1[jj] = (v1 && vj && j >= i1) ? (j > i1 ? 2.0 * e1 : e1) : 0.0;
-      }
-      // permuted y of the tile's 32 columns, one row of 32 per replicate
-      for (int e = lane; e < U.nr * 32; e += KPC_WAVE) {
-        const int r = e >> 5, j = j0 + (e & 31);
-        double v = 0.0;
-        if (j < n) v = Y[perms ? (int)perms[(size_t)(U.slot + r) * n + j] : j];
-        stage[e] = v;
-      }
+      int rb, cb;
+      decode(t, rb, cb);
+      const int i0 = 128 * rb + lane, i1 = i0 + 64, j0 = JT * cb;
+      // stage the permuted values of this tile, then prefetch the next tile's indices
       double y0[RB], y1[RB];
 #pragma unroll
       for (int r = 0; r < RB; ++r) {
-        y0[r] = y1[r] = 0.0;
-        if (r < U.nr) {
-          const unsigned short* pr = perms ? perms + (size_t)(U.slot + r) * n : nullptr;
-          if (v0) y0[r] = Y[pr ? (int)pr[i0] : i0];
-          if (v1) y1[r] = Y[pr ? (int)pr[i1] : i1];
+        y0[r] = ix0[r] == kNone ? 0.0 : Y[ix0[r]];
+        y1[r] = ix1[r] == kNone ? 0.0 : Y[ix1[r]];
+      }
+#pragma unroll
+      for (int k = 0; k < kStage; ++k) {
+        const int e = lane + KPC_WAVE * k;
+        if (e < RB * JT) stage[e] = ixs[k] == kNone ? 0.0 : Y[ixs[k]];
+      }
+      if (t + nw < ntiles) fetch(t + nw);
+      // weighted centred distances of rows i0, i1 against the tile's columns
+      const bool v0 = i0 < n, v1 = i1 < n;
+      const double2 x0 = XR[v0 ? i0 : 0];
+      const double2 x1 = XR[v1 ? i1 : 0];
+      const double c0 = abar_s - x0.y, c1 = abar_s - x1.y;
+      double a0[JT], a1[JT];
+      const bool interior = 128 * rb + 127 < n && j0 + JT - 1 < n && j0 > 128 * rb + 127;
+      if (interior) {
+#pragma unroll
+        for (int jj = 0; jj < JT; ++jj) {
+          const double2 xj = XR[j0 + jj];
+          if (kUnitIndex) {
+            a0[jj] = (fabs(x0.x - xj.x) + c0) - xj.y;
+            a1[jj] = (fabs(x1.x - xj.x) + c1) - xj.y;
+          } else {
+            a0[jj] = 2.0 * ((dist_pow<false>(x0.x - xj.x, index) + c0) - xj.y);
+            a1[jj] = 2.0 * ((dist_pow<false>(x1.x - xj.x, index) + c1) - xj.y);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int jj = 0; jj < JT; ++jj) {
+          const int j = j0 + jj;
+          const bool vj = j < n;
+          const double2 xj = XR[vj ? j : 0];
+          double e0 = (dist_pow<kUnitIndex>(x0.x - xj.x, index) + c0) - xj.y;
+          double e1 = (dist_pow<kUnitIndex>(x1.x - xj.x, index) + c1) - xj.y;
+          if (!kUnitIndex) {
+            e0 *= 2.0;
+            e1 *= 2.0;
+          }
+          a0[jj] = (v0 && vj && j >= i0) ? (j > i0 ? e0 : 0.5 * e0) : 0.0;
+          a1[jj] = (v1 && vj && j >= i1) ? (j > i1 ? e1 : 0.5 * e1) : 0.0;
         }
       }
       wave_lds_sync();
 #pragma unroll
-      for (int jp = 0; jp < 16; ++jp) {
+      for (int jp = 0; jp < JT / 2; ++jp) {
 #pragma unroll
         for (int r = 0; r < RB; ++r) {
-          if (r < U.nr) {
-            const double2 yj = stage2[r * 16 + jp];
-            acc0[r] = fma(a0[2 * jp], dist_pow<kUnitIndex>(y0[r] - yj.x, index), acc0[r]);
-            acc1[r] = fma(a1[2 * jp], dist_pow<kUnitIndex>(y1[r] - yj.x, index), acc1[r]);
-            acc0[r] = fma(a0[2 * jp + 1], dist_pow<kUnitIndex>(y0[r] - yj.y, index), acc0[r]);
-            acc1[r] = fma(a1[2 * jp + 1], dist_pow<kUnitIndex>(y1[r] - yj.y, index), acc1[r]);
-          }
+          const double2 yj = stage2[r * (JT / 2) + jp];
+          acc0[r] = fma(a0[2 * jp], dist_pow<kUnitIndex>(y0[r] - yj.x, index), acc0[r]);
+          acc1[r] = fma(a1[2 * jp], dist_pow<kUnitIndex>(y1[r] - yj.x, index), acc1[r]);
+          acc0[r] = fma(a0[2 * jp + 1], dist_pow<kUnitIndex>(y0[r] - yj.y, index), acc0[r]);
+          acc1[r] = fma(a1[2 * jp + 1], dist_pow<kUnitIndex>(y1[r] - yj.y, index), acc1[r]);
         }
       }
       wave_lds_sync();
     }
+    // weight 2 off the diagonal covers (i, j) and (j, i): vals hold the sum
+    // over the full matrix
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
       const double s = wave_sum(acc0[r] + acc1[r]);
@@ -413,40 +474,61 @@ hipError_t launch_hsic_perm_values(const PermUnit* units, int nunits, const int*
   return hipGetLastError();
 }
 
-size_t dcc_perm_lds_bytes(int n, int rb) {
-  return ((size_t)((3 * n + 1) & ~1) + (size_t)4 * rb * 32) * sizeof(double);
+size_t dcc_perm_lds_bytes(int n, int rb, int jt) {
+  return ((size_t)((3 * n + 1) & ~1) + (size_t)4 * rb * jt) * sizeof(double);
 }
 
-template <bool kUnit, int RB>
+template <bool kUnit, int JT, int RB>
 static hipError_t dcc_launch(const PermUnit* units, int nunits, const int* pa, const int* pb,
                              int n, int stride, double index, const double* vecs,
                              const double* grand, const unsigned short* perms, double* vals,
                              int grid, hipStream_t stream) {
-  const size_t lds = dcc_perm_lds_bytes(n, RB);
-  hipError_t err = hipFuncSetAttribute((const void*)dcc_perm_kernel<kUnit, RB>,
+  const size_t lds = dcc_perm_lds_bytes(n, RB, JT);
+  hipError_t err = hipFuncSetAttribute((const void*)dcc_perm_kernel<kUnit, JT, RB>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (err != hipSuccess) return err;
-  hipLaunchKernelGGL((dcc_perm_kernel<kUnit, RB>), dim3(grid), dim3(256), lds, stream, units,
+  hipLaunchKernelGGL((dcc_perm_kernel<kUnit, JT, RB>), dim3(grid), dim3(256), lds, stream, units,
                      nunits, pa, pb, n, stride, index, vecs, grand, perms, vals);
   return hipGetLastError();
+}
+
+template <int JT, int RB>
+static hipError_t dcc_dispatch(bool unit, const PermUnit* units, int nunits, const int* pa,
+                               const int* pb, int n, int stride, double index,
+                               const double* vecs, const double* grand,
+                               const unsigned short* perms, double* vals, int grid,
+                               hipStream_t stream) {
+  return unit ? dcc_launch<true, JT, RB>(units, nunits, pa, pb, n, stride, index, vecs, grand,
+                                         perms, vals, grid, stream)
+              : dcc_launch<false, JT, RB>(units, nunits, pa, pb, n, stride, index, vecs, grand,
+                                          perms, vals, grid, stream);
+}
+
+// Tile shapes (columns per tile JT, replicates per unit RB); RB = 1 is the
+// observed pass of any JT.
+bool dcc_perm_shape_supported(int jt, int rb) {
+  return (rb == 1 && (jt == 8 || jt == 16 || jt == 32)) ||
+         (jt == 16 && (rb == 8 || rb == 12)) || (jt == 8 && rb == 16) || (jt == 32 && rb == 8);
 }
 
 hipError_t launch_dcc_perm_values(const PermUnit* units, int nunits, const int* pa, const int* pb,
                                   int n, int stride, double index, const double* vecs,
                                   const double* grand, const unsigned short* perms, double* vals,
-                                  int rb, int grid, hipStream_t stream) {
+                                  int jt, int rb, int grid, hipStream_t stream) {
   if (nunits <= 0) return hipSuccess;
   const bool unit = index == 1.0;
-  if (rb == 8)
-    return unit ? dcc_launch<true, 8>(units, nunits, pa, pb, n, stride, index, vecs, grand, perms,
-                                      vals, grid, stream)
-                : dcc_launch<false, 8>(units, nunits, pa, pb, n, stride, index, vecs, grand,
-                                       perms, vals, grid, stream);
-  if (rb == 16)
-    return unit ? dcc_launch<true, 16>(units, nunits, pa, pb, n, stride, index, vecs, grand,
-                                       perms, vals, grid, stream)
-                : dcc_launch<false, 16>(units, nunits, pa, pb, n, stride, index, vecs, grand,
-                                        perms, vals, grid, stream);
+#define KPC_DCC_SHAPE(J, B)                                                                    \
+  if (jt == J && rb == B)                                                                      \
+    return dcc_dispatch<J, B>(unit, units, nunits, pa, pb, n, stride, index, vecs, grand, perms, \
+                              vals, grid, stream);
+  KPC_DCC_SHAPE(8, 1)
+  KPC_DCC_SHAPE(16, 1)
+  KPC_DCC_SHAPE(32, 1)
+  KPC_DCC_SHAPE(16, 8)
+  KPC_DCC_SHAPE(16, 12)
+  KPC_DCC_SHAPE(8, 16)
+  KPC_DCC_SHAPE(32, 8)
+#undef KPC_DCC_SHAPE
   return hipErrorInvalidValue;
 }
 
