@@ -123,6 +123,101 @@ def cpu_baseline(data, res, n, k, seconds_budget):
     }
 
 
+def bench_regression(args):
+    """The standalone regrXonS batch: 1024 fits, n = 500, |S| ~ U{1..6},
+    x ~ U[-2, 2], y = sum_k sin(2 x_k) + 0.3 x_k^2 + N(0, 0.3^2); lambda by the
+    batched in-kernel Newton on the GCV score with the hand-written
+    gradient/Hessian.  One step = one kpc_regrxons_mixed call (data upload,
+    designs, one GCV launch per design width, residual download)."""
+    import fastkpc_amd as fk
+    from fastkpc_amd.sem import regression_batch
+    data, targets, sets = regression_batch()
+    n = data.shape[0]
+    for _ in range(args.warmup):
+        fk.regrxons_mixed(data, targets, sets)
+    walls, setup_s, gcv_s, launches = [], 0.0, 0.0, 0
+    info = None
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        _, info = fk.regrxons_mixed(data, targets, sets)
+        walls.append(time.perf_counter() - t0)
+        t = fk.last_fit_timing()
+        setup_s += t["setup"]
+        gcv_s += t["gcv"]
+        launches += t["gcv_launches"]
+    wall = float(np.mean(walls))
+    fits = len(targets)
+    lens = np.array([len(x) for x in sets])
+    flops = sum(gcv_flops(n, int(lens[i]), 1, info[i, 5]) for i in range(fits))
+    ktime = gcv_s / args.steps
+    achieved = flops / ktime / 1e12
+    # plain form: one call per fit on that fit's own columns
+    plain = None
+    if not args.no_plain:
+        pick = [i for L in range(1, 7) for i in np.nonzero(lens == L)[0][:5]]
+        t_plain = {}
+        for i in pick:
+            cols = sets[i] + [int(targets[i])]
+            sub = np.asfortranarray(data[:, cols])
+            t0 = time.perf_counter()
+            fk.regrxons_batch(sub, [len(cols) - 1], [list(range(len(cols) - 1))])
+            t_plain.setdefault(int(lens[i]), []).append(time.perf_counter() - t0)
+        est = sum(np.mean(t_plain[L]) * np.sum(lens == L) for L in t_plain)
+        plain = {"fits_per_s": round(fits / est, 2), "batch_wall_s_est": round(est, 3),
+                 "sampled_fits": len(pick), "batched_over_plain": round(est / wall, 2)}
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import orc
+        share = args.cpu_baseline_seconds / 6.0
+        est, sampled = 0.0, []
+        t_start = time.time()
+        for L in range(1, 7):
+            idx = np.nonzero(lens == L)[0]
+            tot, k = 0.0, 0
+            for i in idx:
+                t0 = time.perf_counter()
+                orc.regrxons(data, int(targets[i]), sets[i])
+                tot += time.perf_counter() - t0
+                k += 1
+                if tot > share:
+                    break
+            est += tot / k * len(idx)
+            sampled.append("|S|=%d:%d" % (L, k))
+        cpu = {"value": round(fits / est, 2), "unit": "fits/s", "cores": 1, "kind": "port",
+               "sample": "C oracle (mgcv magic restatement), 1 thread, %s fits timed (%.1f s), "
+                         "extrapolated per |S| (est. %.1f s for the batch)"
+                         % (",".join(sampled), time.time() - t_start, est)}
+    line = {
+        "metric": "regrXonS fits/s, 1024 fits n=500 |S|~U{1..6}: batched adjoint-Newton GCV",
+        "value": round(fits / wall, 2),
+        "unit": "fits/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (fastkpc_amd/sem.py regression_batch, seed 4)",
+        "config": {"workload": "regrXonS batch: 1024 fits, n=500, |S|~U{1..6}, x~U[-2,2], "
+                               "y=sum sin(2x)+0.3x^2+N(0,0.3^2)",
+                   "fits": fits, "n": n, "fits_per_S": np.bincount(lens)[1:].tolist(),
+                   "evaluations": int(info[:, 5].sum()),
+                   "setup_s_per_step": round(setup_s / args.steps, 4),
+                   "gcv_s_per_step": round(ktime, 4)},
+        "plain_gpu": plain,
+        "roofline": {"bound": "mfma", "kernel": "magic_kernel", "achieved": round(achieved, 4),
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 6),
+                     "traffic": None, "launches_per_step": launches / args.steps,
+                     "avg_launch_us": round(ktime / max(1, launches / args.steps) * 1e6, 1),
+                     "algorithmic_flops_per_launch": flops / max(1, launches / args.steps)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
 def copy_calibration(nbytes):
     """Device-to-device copy of nbytes through the HIP runtime: the reference
     kernel for scaling FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -146,7 +241,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="sem351x48")
+    ap.add_argument("--config", default="sem351x48",
+                    help="sem351x48 (headline), sem200x8, sem500x50, sem2000x64, or regr1024 "
+                         "(the standalone 1024-fit regrXonS batch)")
     ap.add_argument("--method", default="dcc.gamma",
                     choices=["dcc.gamma", "hsic.gamma", "dcc.perm", "hsic.perm"],
                     help="CI method (the headline metric is dcc.gamma)")
@@ -166,6 +263,10 @@ def main():
         print("bench.py measures the single-GPU path only: this build has no multi-GPU execution "
               "(one MI355X per box); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.config == "regr1024":
+        bench_regression(args)
+        return
 
     import fastkpc_amd as fk
     from fastkpc_amd.sem import config_data

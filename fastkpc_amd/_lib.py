@@ -72,8 +72,9 @@ class KpcResult(ctypes.Structure):
 
 EXPORTS = (
     "kpc_default_options", "kpc_skeleton_run", "kpc_result_free", "kpc_ci_batch",
-    "kpc_regrxons_batch", "kpc_gcv_grad_hess", "kpc_dcov_components", "kpc_task_seed",
-    "kpc_permutation_table", "kpc_last_error", "kpc_device_count",
+    "kpc_regrxons_batch", "kpc_regrxons_mixed", "kpc_gcv_grad_hess", "kpc_dcov_components",
+    "kpc_last_fit_timing", "kpc_task_seed", "kpc_permutation_table", "kpc_last_error",
+    "kpc_device_count",
 )
 
 _lib = None
@@ -101,6 +102,9 @@ def lib():
                                ctypes.c_int, _I, _I, ctypes.c_int, _I, _D]
     L.kpc_regrxons_batch.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I,
                                      ctypes.c_int, _I, _D, _D]
+    L.kpc_regrxons_mixed.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I, _I, _I, _I,
+                                     _D, _D]
+    L.kpc_last_fit_timing.argtypes = [_D]
     L.kpc_gcv_grad_hess.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I, ctypes.c_int,
                                     _I, _D, _D, _D, _D]
     L.kpc_dcov_components.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D, _D, _D]

@@ -228,6 +228,32 @@ def regrxons_batch(data, targets, S):
     return resid, info
 
 
+def regrxons_mixed(data, targets, S_list):
+    """Residuals and fit info of fits with per-fit conditioning sets of any
+    size (0-based; each S sorted ascending): (count, n), (count, 32)."""
+    d = _as_data(data)
+    n, p = d.shape
+    t = np.ascontiguousarray(targets, dtype=np.int32)
+    if len(S_list) != len(t):
+        raise ValueError("one conditioning set per target")
+    lens = np.ascontiguousarray([len(s) for s in S_list], dtype=np.int32)
+    offs = np.ascontiguousarray(np.concatenate([[0], np.cumsum(lens)[:-1]]), dtype=np.int32)
+    flat = np.ascontiguousarray(np.concatenate([np.asarray(s, dtype=np.int32) for s in S_list]),
+                                dtype=np.int32)
+    resid = np.zeros((len(t), n))
+    info = np.zeros((len(t), 32))
+    check(_lib.lib().kpc_regrxons_mixed(dptr(d), n, p, len(t), iptr(t), iptr(lens), iptr(offs),
+                                        iptr(flat), dptr(resid), dptr(info)))
+    return resid, info
+
+
+def last_fit_timing():
+    """Device seconds of the last regrXonS call: {"setup", "gcv", "gcv_launches"}."""
+    out = np.zeros(3)
+    check(_lib.lib().kpc_last_fit_timing(dptr(out)))
+    return {"setup": float(out[0]), "gcv": float(out[1]), "gcv_launches": int(out[2])}
+
+
 def gcv_grad_hess(data, targets, S, log_sp):
     """GCV score, gradient and Hessian in log(sp) (hand-written derivative)."""
     d = _as_data(data)

@@ -36,6 +36,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <map>
 #include <unordered_map>
 #include <vector>
 
@@ -133,10 +134,14 @@ double now_s() {
 
 class Engine {
  public:
-  Engine(const double* h_data, int n, int p, const kpc_options& opt) : n_(n), p_(p), opt_(opt) {
+  // skeleton engines index variables with 64-bit sets (p <= 64); the
+  // regrXonS / GCV entry points key designs by id and take any p >= 2
+  Engine(const double* h_data, int n, int p, const kpc_options& opt, bool skeleton = true)
+      : n_(n), p_(p), opt_(opt) {
+    static_assert(sizeof(Task::S) / sizeof(int) >= 62, "conditioning sets up to 62");
     if (n < 6) throw KpcError(-1, "n must be > 5");
     if (n > kpc::kMaxN) throw KpcError(-1, "n exceeds the supported maximum (2048)");
-    if (p < 2 || p > 64) throw KpcError(-1, "p must be in [2, 64]");
+    if (p < 2 || (skeleton && p > 64)) throw KpcError(-1, "p must be in [2, 64]");
     if (opt.num_col <= 0 || opt.num_col >= n || opt.num_col > kpc::kMaxNumCol)
       throw KpcError(-1, "num_col must be positive, < n and <= 64");
     if (opt.ci_method < KPC_DCC_GAMMA || opt.ci_method > KPC_HSIC_PERM)
@@ -177,6 +182,8 @@ class Engine {
       }
     }
   }
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
   ~Engine() {
     if (d_data_) (void)hipFree(d_data_);
     for (int e = 0; e < 8; ++e) (void)hipEventDestroy(ev_[e]);
@@ -1156,83 +1163,167 @@ int kpc_ci_batch(const double* data, int n, int p, const kpc_options* opt, int c
   }
 }
 
+namespace {
+
+// Conditioning sets of one |S| group, keyed by id (distinct sorted S lists
+// share a design), validated against p.
+std::vector<Task> fit_sets(int p, int count, const int* targets, int s_len,
+                           const int* const* S_rows) {
+  std::vector<Task> sets((size_t)count);
+  std::map<std::vector<int>, uint64_t> ids;
+  for (int i = 0; i < count; ++i) {
+    Task& t = sets[i];
+    t.slen = s_len;
+    t.x = targets[i];
+    if (targets[i] < 0 || targets[i] >= p) throw KpcError(-1, "target out of range");
+    std::vector<int> key((size_t)s_len);
+    for (int j = 0; j < s_len; ++j) {
+      t.S[j] = S_rows[i][j];
+      if (t.S[j] < 0 || t.S[j] >= p) throw KpcError(-1, "S out of range");
+      if (j > 0 && t.S[j] <= t.S[j - 1]) throw KpcError(-1, "S must be increasing");
+      if (t.S[j] == t.x) throw KpcError(-1, "target must not be in S");
+      key[j] = t.S[j];
+    }
+    auto it = ids.emplace(key, (uint64_t)ids.size()).first;
+    t.mask = it->second;
+  }
+  return sets;
+}
+
+// HIP-event seconds of the last regrXonS call on this thread:
+// {design setup (smooth + geometry), GCV fits, launches of the GCV kernel}.
+thread_local double g_fit_timing[3] = {0.0, 0.0, 0.0};
+
+// GCV fits of `count` targets sharing |S| == s_len on an engine whose data
+// is resident: designs (smooth setups + geometry), then one magic launch.
+// resid / info rows go to out_index[i] (count x n, count x 32).
+void run_fit_group(Engine& eng, int count, const int* targets, int s_len,
+                   const int* const* S_rows, const int* out_index, double* resid, double* info) {
+  const int n = eng.n(), p = eng.p();
+  eng.begin_level(s_len);
+  std::vector<Task> sets = fit_sets(p, count, targets, s_len, S_rows);
+  eng.reserve_level((size_t)count, (size_t)count);
+  hipEvent_t ev[3];
+  for (auto& e : ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+  hip_check(hipEventRecord(ev[0], eng.stream()), "event");
+  std::vector<int> dslots = eng.build_designs(sets);
+  std::vector<kpc::MagicItem> items((size_t)count);
+  for (int i = 0; i < count; ++i) {
+    items[i].design_slot = dslots[i];
+    items[i].target = targets[i];
+    items[i].resid_slot = i;
+    items[i].pad = 0;
+  }
+  DevArray<kpc::MagicItem> d_items;
+  d_items.upload(items, eng.stream());
+  const long long stride = kpc::magic_scratch_doubles(eng.q(), eng.m(), eng.rk(), n);
+  const int grid = std::min(count, eng.slab_grid(stride, 8192));
+  eng.scratch3().reserve((size_t)grid * stride);
+  hip_check(hipEventRecord(ev[1], eng.stream()), "event");
+  hip_check(kpc::launch_magic(d_items.ptr, count, n, eng.q(), eng.m(), eng.rk(), eng.data_ptr(),
+                              eng.design_ptr(), eng.resid_arena().ptr, eng.rinfo_arena().ptr,
+                              eng.scratch3().ptr, grid, eng.ctl(), eng.stream()),
+            "magic launch");
+  hip_check(hipEventRecord(ev[2], eng.stream()), "event");
+  hip_check(hipStreamSynchronize(eng.stream()), "magic synchronize");
+  float ms0 = 0.f, ms1 = 0.f;
+  hip_check(hipEventElapsedTime(&ms0, ev[0], ev[1]), "event elapsed");
+  hip_check(hipEventElapsedTime(&ms1, ev[1], ev[2]), "event elapsed");
+  g_fit_timing[0] += ms0 * 1e-3;
+  g_fit_timing[1] += ms1 * 1e-3;
+  g_fit_timing[2] += 1.0;
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  std::vector<double> r((size_t)count * n), f((size_t)count * kpc::kMagicInfo);
+  hip_check(hipMemcpy(r.data(), eng.resid_arena().ptr, sizeof(double) * r.size(),
+                      hipMemcpyDeviceToHost),
+            "D2H residuals");
+  hip_check(hipMemcpy(f.data(), eng.rinfo_arena().ptr, sizeof(double) * f.size(),
+                      hipMemcpyDeviceToHost),
+            "D2H fit info");
+  for (int i = 0; i < count; ++i) {
+    const size_t o = (size_t)out_index[i];
+    std::memcpy(resid + o * n, r.data() + (size_t)i * n, sizeof(double) * n);
+    if (info)
+      std::memcpy(info + o * kpc::kMagicInfo, f.data() + (size_t)i * kpc::kMagicInfo,
+                  sizeof(double) * kpc::kMagicInfo);
+  }
+}
+
+Engine make_fit_engine(const double* data, int n, int p) {
+  kpc_options o;
+  kpc_default_options(&o);
+  o.num_col = std::min(35, n - 1);
+  return Engine(data, n, p, o, false);
+}
+
+}  // namespace
+
 int kpc_regrxons_batch(const double* data, int n, int p, int count, const int* targets, int s_len,
                        const int* S, double* resid, double* info) {
   try {
-    if (!data || !targets || !resid || count <= 0 || s_len < 1 || s_len > 62)
+    if (!data || !targets || !S || !resid || count <= 0 || s_len < 1 || s_len > 62)
       throw KpcError(-1, "invalid arguments");
-    kpc_options o;
-    kpc_default_options(&o);
-    o.num_col = std::min(35, n - 1);
-    Engine eng(data, n, p, o);
-    eng.begin_level(s_len);
-    std::vector<Task> sets((size_t)count);
+    Engine eng = make_fit_engine(data, n, p);
+    g_fit_timing[0] = g_fit_timing[1] = g_fit_timing[2] = 0.0;
+    std::vector<const int*> rows((size_t)count);
+    std::vector<int> order((size_t)count);
     for (int i = 0; i < count; ++i) {
-      sets[i].slen = s_len;
-      sets[i].x = targets[i];
-      if (targets[i] < 0 || targets[i] >= p) throw KpcError(-1, "target out of range");
-      for (int j = 0; j < s_len; ++j) {
-        sets[i].S[j] = S[(size_t)i * s_len + j];
-        if (sets[i].S[j] < 0 || sets[i].S[j] >= p) throw KpcError(-1, "S out of range");
-        if (j > 0 && sets[i].S[j] <= sets[i].S[j - 1]) throw KpcError(-1, "S must be increasing");
-      }
-      sets[i].mask = mask_of(sets[i].S, s_len);
+      rows[i] = S + (size_t)i * s_len;
+      order[i] = i;
     }
-    eng.reserve_level((size_t)count, (size_t)count);
-    std::vector<int> dslots = eng.build_designs(sets);
-    std::vector<kpc::MagicItem> items((size_t)count);
-    for (int i = 0; i < count; ++i) {
-      items[i].design_slot = dslots[i];
-      items[i].target = targets[i];
-      items[i].resid_slot = i;
-      items[i].pad = 0;
-    }
-    DevArray<kpc::MagicItem> d_items;
-    d_items.upload(items, eng.stream());
-    const long long stride = kpc::magic_scratch_doubles(eng.q(), eng.m(), eng.rk(), n);
-    const int grid = std::min(count, eng.slab_grid(stride, 8192));
-    eng.scratch3().reserve((size_t)grid * stride);
-    hip_check(kpc::launch_magic(d_items.ptr, count, n, eng.q(), eng.m(), eng.rk(), eng.data_ptr(),
-                                eng.design_ptr(), eng.resid_arena().ptr, eng.rinfo_arena().ptr,
-                                eng.scratch3().ptr, grid, eng.ctl(), eng.stream()),
-              "magic launch");
-    hip_check(hipStreamSynchronize(eng.stream()), "magic synchronize");
-    hip_check(hipMemcpy(resid, eng.resid_arena().ptr, sizeof(double) * (size_t)count * n,
-                        hipMemcpyDeviceToHost),
-              "D2H residuals");
-    if (info)
-      hip_check(hipMemcpy(info, eng.rinfo_arena().ptr,
-                          sizeof(double) * (size_t)count * kpc::kMagicInfo, hipMemcpyDeviceToHost),
-                "D2H fit info");
+    run_fit_group(eng, count, targets, s_len, rows.data(), order.data(), resid, info);
     return 0;
   } catch (const std::exception& e) {
     return fill_error(e);
   }
 }
 
+int kpc_regrxons_mixed(const double* data, int n, int p, int count, const int* targets,
+                       const int* s_len, const int* s_off, const int* S, double* resid,
+                       double* info) {
+  try {
+    if (!data || !targets || !s_len || !s_off || !S || !resid || count <= 0)
+      throw KpcError(-1, "invalid arguments");
+    for (int i = 0; i < count; ++i)
+      if (s_len[i] < 1 || s_len[i] > 62 || s_off[i] < 0)
+        throw KpcError(-1, "each |S| must be in [1, 62] with a non-negative offset");
+    Engine eng = make_fit_engine(data, n, p);
+    g_fit_timing[0] = g_fit_timing[1] = g_fit_timing[2] = 0.0;
+    for (int L = 1; L <= 62; ++L) {
+      std::vector<int> idx, tg;
+      std::vector<const int*> rows;
+      for (int i = 0; i < count; ++i)
+        if (s_len[i] == L) {
+          idx.push_back(i);
+          tg.push_back(targets[i]);
+          rows.push_back(S + s_off[i]);
+        }
+      if (idx.empty()) continue;
+      run_fit_group(eng, (int)idx.size(), tg.data(), L, rows.data(), idx.data(), resid, info);
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fill_error(e);
+  }
+}
+
+int kpc_last_fit_timing(double* out) {
+  if (!out) return -1;
+  for (int i = 0; i < 3; ++i) out[i] = g_fit_timing[i];
+  return 0;
+}
+
 int kpc_gcv_grad_hess(const double* data, int n, int p, int count, const int* targets, int s_len,
                       const int* S, const double* log_sp, double* score, double* grad,
                       double* hess) {
   try {
-    if (!data || !targets || !log_sp || count <= 0 || s_len < 1 || s_len > 62)
+    if (!data || !targets || !S || !log_sp || count <= 0 || s_len < 1 || s_len > 62)
       throw KpcError(-1, "invalid arguments");
-    kpc_options o;
-    kpc_default_options(&o);
-    o.num_col = std::min(35, n - 1);
-    Engine eng(data, n, p, o);
+    Engine eng = make_fit_engine(data, n, p);
     eng.begin_level(s_len);
-    std::vector<Task> sets((size_t)count);
-    for (int i = 0; i < count; ++i) {
-      sets[i].slen = s_len;
-      sets[i].x = targets[i];
-      if (targets[i] < 0 || targets[i] >= p) throw KpcError(-1, "target out of range");
-      for (int j = 0; j < s_len; ++j) {
-        sets[i].S[j] = S[(size_t)i * s_len + j];
-        if (sets[i].S[j] < 0 || sets[i].S[j] >= p) throw KpcError(-1, "S out of range");
-        if (j > 0 && sets[i].S[j] <= sets[i].S[j - 1]) throw KpcError(-1, "S must be increasing");
-      }
-      sets[i].mask = mask_of(sets[i].S, s_len);
-    }
+    std::vector<const int*> rows((size_t)count);
+    for (int i = 0; i < count; ++i) rows[i] = S + (size_t)i * s_len;
+    std::vector<Task> sets = fit_sets(p, count, targets, s_len, rows.data());
     eng.reserve_level((size_t)count, (size_t)count);
     std::vector<int> dslots = eng.build_designs(sets);
     const int m = eng.m();

@@ -73,3 +73,22 @@ def config_data(name):
     c = dict(CONFIGS[name])
     data, mask = generate(**c)
     return data, mask, c
+
+
+def regression_batch(fits=1024, n=500, s_lo=1, s_hi=6, seed=4):
+    """Standalone regrXonS batch: fit f draws |S_f| ~ U{s_lo..s_hi} predictors
+    x_k ~ U[-2, 2] and the target y = sum_k sin(2 x_k) + 0.3 x_k^2 + N(0, 0.3^2).
+    Returns the n x P data matrix (per fit: its predictors, then its target),
+    the target columns and the conditioning sets (0-based, ascending)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    cols, targets, sets = [], [], []
+    for _ in range(fits):
+        s = int(rng.integers(s_lo, s_hi + 1))
+        X = rng.uniform(-2.0, 2.0, size=(n, s))
+        y = np.sum(np.sin(2.0 * X) + 0.3 * X * X, axis=1) + rng.normal(0.0, 0.3, size=n)
+        base = len(cols)
+        cols.extend(X[:, k] for k in range(s))
+        cols.append(y)
+        sets.append(list(range(base, base + s)))
+        targets.append(base + s)
+    return np.asfortranarray(np.column_stack(cols)), np.asarray(targets, dtype=np.int32), sets

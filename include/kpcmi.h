@@ -116,6 +116,24 @@ int kpc_ci_batch(const double* data, int n, int p, const kpc_options* opt, int c
 int kpc_regrxons_batch(const double* data, int n, int p, int count, const int* targets,
                        int s_len, const int* S, double* resid, double* info);
 
+/* regrXonS fits with per-fit conditioning sets of mixed size: fit i regresses
+ * column targets[i] on the s_len[i] (1..62) increasing 0-based columns
+ * S[s_off[i] .. s_off[i] + s_len[i]).  Fits are grouped by |S| (design width
+ * q = 10, 30 or 1 + 9|S|): one design batch and one GCV launch per group, the
+ * data uploaded once.  resid: count x n; info: count x 32 as in
+ * kpc_regrxons_batch.  Any p >= 2.  Replaces one call per fit of
+ * kpcalg regrXonS (kpcalg/R/regrXonS.R:33-61) / the reference's per-cohort GCV
+ * optimiser calls (mgcv_multi_penalty_gcv_capacity.hpp). */
+int kpc_regrxons_mixed(const double* data, int n, int p, int count, const int* targets,
+                       const int* s_len, const int* s_off, const int* S, double* resid,
+                       double* info);
+
+/* Device time of the last kpc_regrxons_batch / kpc_regrxons_mixed call on
+ * this thread (HIP events on the library's stream): out[0] design setup
+ * seconds (smooth setups + geometry), out[1] GCV kernel seconds, out[2] GCV
+ * launches.  No counterpart in the reference (its timers are R-side). */
+int kpc_last_fit_timing(double* out);
+
 /* GCV score, gradient (m) and Hessian (m x m) in log(sp) at log_sp (count x m)
  * for target on S. */
 int kpc_gcv_grad_hess(const double* data, int n, int p, int count, const int* targets, int s_len,
