@@ -103,3 +103,26 @@ def test_task_seed_matches_oracle_without_gpu():
             got = f.task_seed(user, method, int(v[0]), int(v[1]), S)
             assert got == orc.task_seed(user, method, int(v[0]), int(v[1]), S)
             assert got == f.task_seed(user, method, int(v[1]), int(v[0]), S)
+
+
+def test_regrxons_mixed_validates_before_any_device_call():
+    import fastkpc_amd as fk
+    data = np.asfortranarray(np.random.default_rng(2).standard_normal((40, 5)))
+    with pytest.raises(fk._lib.KpcLibraryError, match=r"\|S\| must be in \[1, 62\]"):
+        fk.regrxons_mixed(data, [0, 1], [[2], []])
+    with pytest.raises(ValueError, match="one conditioning set per target"):
+        fk.regrxons_mixed(data, [0, 1], [[2]])
+
+
+def test_regression_batch_generator():
+    from fastkpc_amd.sem import regression_batch
+    d, t, S = regression_batch(fits=64, n=50, seed=9)
+    d2, t2, S2 = regression_batch(fits=64, n=50, seed=9)
+    assert np.array_equal(d, d2) and np.array_equal(t, t2) and S == S2
+    assert d.shape == (50, sum(len(s) + 1 for s in S))
+    for tg, s in zip(t, S):
+        assert 1 <= len(s) <= 6 and tg == s[-1] + 1 and s == sorted(s)
+        X = d[:, s]
+        assert X.min() >= -2.0 and X.max() <= 2.0
+        noise = d[:, tg] - np.sum(np.sin(2 * X) + 0.3 * X * X, axis=1)
+        assert abs(noise.std() - 0.3) < 0.12
