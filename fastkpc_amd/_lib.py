@@ -73,7 +73,7 @@ class KpcResult(ctypes.Structure):
 EXPORTS = (
     "kpc_default_options", "kpc_skeleton_run", "kpc_result_free", "kpc_ci_batch",
     "kpc_regrxons_batch", "kpc_regrxons_mixed", "kpc_gcv_grad_hess", "kpc_dcov_components",
-    "kpc_last_fit_timing", "kpc_task_seed", "kpc_permutation_table", "kpc_last_error",
+    "kpc_last_fit_timing", "kpc_tuning_report", "kpc_task_seed", "kpc_permutation_table", "kpc_last_error",
     "kpc_device_count",
 )
 
@@ -105,6 +105,7 @@ def lib():
     L.kpc_regrxons_mixed.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I, _I, _I, _I,
                                      _D, _D]
     L.kpc_last_fit_timing.argtypes = [_D]
+    L.kpc_tuning_report.argtypes = [ctypes.c_char_p, ctypes.c_int]
     L.kpc_gcv_grad_hess.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I, ctypes.c_int,
                                     _I, _D, _D, _D, _D]
     L.kpc_dcov_components.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D, _D, _D]

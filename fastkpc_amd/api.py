@@ -135,6 +135,8 @@ def fastkpc_compatible_cuda_skeleton(data, alpha, labels=None, options=None):
             "plain_seconds": list(r.plain_seconds)[:levels],
             "stage_launches": dict(zip(["smooth_setup", "geometry", "gcv", "dcov_component",
                                         "dcov_pair"], list(r.stage_launches)[:5])),
+            "graph_seconds": r.stage_seconds[7],
+            "graph_launches": r.stage_launches[7],
             "labels": list(labels),
         }
         return {
@@ -245,6 +247,25 @@ def regrxons_mixed(data, targets, S_list):
     check(_lib.lib().kpc_regrxons_mixed(dptr(d), n, p, len(t), iptr(t), iptr(lens), iptr(offs),
                                         iptr(flat), dptr(resid), dptr(info)))
     return resid, info
+
+
+def tuning_report():
+    """The launch-shape autotuner's table: {key: (chosen value or None,
+    {candidate: seconds per replicate or None})}."""
+    buf = _lib.ctypes.create_string_buffer(1 << 16)
+    _lib.lib().kpc_tuning_report(buf, len(buf))
+    out = {}
+    for line in buf.value.decode().splitlines():
+        parts = line.split()
+        if len(parts) < 2:
+            continue
+        v = parts[1].split("=")[1]
+        cands = {}
+        for c in parts[2:]:
+            a, b = c.split(":")
+            cands[int(a)] = None if float(b) < 0 else float(b)
+        out[parts[0]] = (None if v == "?" else int(v), cands)
+    return out
 
 
 def last_fit_timing():

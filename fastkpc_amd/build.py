@@ -19,7 +19,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["kpc_dcov.hip", "kpc_dcov_lanczos.hip", "kpc_smooth.hip", "kpc_magic.hip", "kpc_hsic.hip",
            "kpc_perm.hip",
-           "kpc_host.cpp"]
+           "kpc_host.cpp", "kpc_tune.cpp"]
 COMMON = [
     "-std=c++17", "-O3", "--offload-arch=gfx950", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

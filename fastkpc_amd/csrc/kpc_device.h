@@ -110,13 +110,14 @@ struct PermUnit {
   int nr;
   int slot;
 };
-constexpr int kHsicPermBlock = 16;  // replicates per hsic.perm unit (4 per wave)
-int perm_generate_width(int n);     // replicates per generation unit
+int perm_generate_width(int n);     // largest generation width for n
 size_t dcc_perm_lds_bytes(int n, int rb, int jt);
 bool dcc_perm_shape_supported(int jt, int rb);
+// width: replicates per generation unit (<= perm_generate_width(n); each
+// unit's lane shuffles one replicate in width * n * 2 bytes of LDS)
 hipError_t launch_perm_generate(const PermUnit* units, int nunits,
-                                const unsigned long long* seeds, int n, unsigned short* perms,
-                                int grid, hipStream_t stream);
+                                const unsigned long long* seeds, int n, int width,
+                                unsigned short* perms, int grid, hipStream_t stream);
 // perms == nullptr: identity permutation (the observed statistic)
 hipError_t launch_hsic_perm_values(const PermUnit* units, int nunits, const int* pa,
                                    const int* pb, int n, int kt, const double* rows,

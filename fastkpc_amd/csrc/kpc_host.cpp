@@ -24,6 +24,7 @@
 #include "../../include/kpcmi.h"
 #include "kpc_common.h"
 #include "kpc_device.h"
+#include "kpc_tune.h"
 
 #include <hip/hip_runtime.h>
 
@@ -173,12 +174,14 @@ class Engine {
     ctl_.max_probes = 5;
     ctl_.rank_tol = 1.4901161193847656e-8;
     for (int e = 0; e < 8; ++e) hip_check(hipEventCreate(&ev_[e]), "hipEventCreate");
+    for (int e = 0; e < 3; ++e) hip_check(hipEventCreate(&tev_[e]), "hipEventCreate");
     if (const char* sh = std::getenv("KPC_DCC_SHAPE")) {
       int jt = 0, rb = 0;
       if (std::sscanf(sh, "%dx%d", &jt, &rb) == 2 && kpc::dcc_perm_shape_supported(jt, rb) &&
           rb > 1) {
         dcc_perm_jt_ = jt;
         dcc_perm_rb_ = rb;
+        dcc_shape_forced_ = true;
       }
     }
   }
@@ -187,6 +190,8 @@ class Engine {
   ~Engine() {
     if (d_data_) (void)hipFree(d_data_);
     for (int e = 0; e < 8; ++e) (void)hipEventDestroy(ev_[e]);
+    for (int e = 0; e < 3; ++e)
+      if (tev_[e]) (void)hipEventDestroy(tev_[e]);
     if (stream_) (void)hipStreamDestroy(stream_);
   }
 
@@ -333,6 +338,13 @@ class Engine {
     d_pair_out_.reserve(tasks.size() * kpc::kPairOut);
     stage_host_ += now_s() - t0;
     hip_check(hipEventRecord(ev_[0], stream_), "event");
+    // KPC_GRAPH=1: the wave's device pipeline (smooth setups .. pair
+    // statistics) is captured into one hipGraph and launched as a unit; the
+    // permutation methods synchronise between replicate chunks and stay eager
+    const bool graph = use_graph_ && method_ != KPC_DCC_PERM && method_ != KPC_HSIC_PERM;
+    hipGraph_t wave_graph = nullptr;
+    if (graph)
+      hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed), "begin capture");
     const int nsm = (int)smooth_items.size() / 4;
     if (nsm > 0) {
       const long long stride = kpc::smooth_scratch_doubles(n_);
@@ -347,7 +359,7 @@ class Engine {
                 "smooth setup launch");
       pending_smooth_checks_ = true;
     }
-    hip_check(hipEventRecord(ev_[1], stream_), "event");
+    if (!graph) hip_check(hipEventRecord(ev_[1], stream_), "event");
     if (!geo_items.empty()) {
       const long long stride = kpc::geometry_scratch_doubles(q_);
       int grid = std::min((int)geo_items.size(), grid_limit(stride, 2048));
@@ -359,7 +371,7 @@ class Engine {
                                      (long long)n_ * 29, scratch2_.ptr, grid, stream_),
                 "geometry launch");
     }
-    hip_check(hipEventRecord(ev_[2], stream_), "event");
+    if (!graph) hip_check(hipEventRecord(ev_[2], stream_), "event");
     if (!fit_items.empty()) {
       const long long stride = kpc::magic_scratch_doubles(q_, m_, rk_, n_);
       int grid = std::min((int)fit_items.size(), grid_limit(stride, 65535));
@@ -369,11 +381,22 @@ class Engine {
                                   grid, ctl_, stream_),
                 "magic launch");
     }
-    hip_check(hipEventRecord(ev_[3], stream_), "event");
+    if (!graph) hip_check(hipEventRecord(ev_[3], stream_), "event");
     if (!comp_dst.empty()) launch_components(level_ == 0 ? d_data_ : resid_.ptr, comp_dst.size());
-    hip_check(hipEventRecord(ev_[4], stream_), "event");
+    if (!graph) hip_check(hipEventRecord(ev_[4], stream_), "event");
     launch_pairs((int)tasks.size());
-    hip_check(hipEventRecord(ev_[5], stream_), "event");
+    if (graph) {
+      hip_check(hipStreamEndCapture(stream_, &wave_graph), "end capture");
+      hipGraphExec_t exec = nullptr;
+      hip_check(hipGraphInstantiate(&exec, wave_graph, nullptr, nullptr, 0), "graph instantiate");
+      hip_check(hipGraphLaunch(exec, stream_), "graph launch");
+      hip_check(hipEventRecord(ev_[5], stream_), "event");
+      hip_check(hipStreamSynchronize(stream_), "graph synchronize");
+      (void)hipGraphExecDestroy(exec);
+      (void)hipGraphDestroy(wave_graph);
+    } else {
+      hip_check(hipEventRecord(ev_[5], stream_), "event");
+    }
     std::vector<double> out(tasks.size() * kpc::kPairOut);
     hip_check(hipMemcpyAsync(out.data(), d_pair_out_.ptr, sizeof(double) * out.size(),
                              hipMemcpyDeviceToHost, stream_),
@@ -382,11 +405,18 @@ class Engine {
     float ms;
     const bool launched[5] = {nsm > 0, !geo_items.empty(), !fit_items.empty(), !comp_dst.empty(),
                               true};
-    for (int s = 0; s < 5; ++s) {
+    if (graph) {
+      // one graph per wave: only the whole pipeline is timed
+      hip_check(hipEventElapsedTime(&ms, ev_[0], ev_[5]), "event elapsed");
+      graph_seconds_ += ms * 1e-3;
+      ++graph_launches_;
+    }
+    for (int s = 0; s < 5 && !graph; ++s) {
       hip_check(hipEventElapsedTime(&ms, ev_[s], ev_[s + 1]), "event elapsed");
       stage_[s] += ms * 1e-3;
-      if (launched[s]) ++launches_[s];
     }
+    for (int s = 0; s < 5; ++s)
+      if (launched[s]) ++launches_[s];
     check_device_status(fit_items, geo_items, nsm);
     check_component_status(comp_dst);
     std::vector<double> cmargin_h;
@@ -465,7 +495,7 @@ class Engine {
   }
 
   void perm_values(const kpc::PermUnit* units, int nunits, const unsigned short* perms,
-                   double* vals) {
+                   double* vals, int block) {
     const int grid = std::min(nunits, 65535);
     if (method_ == KPC_HSIC_PERM)
       hip_check(kpc::launch_hsic_perm_values(units, nunits, d_pair_x_.ptr, d_pair_y_.ptr, n_,
@@ -475,7 +505,7 @@ class Engine {
     else
       hip_check(kpc::launch_dcc_perm_values(units, nunits, d_pair_x_.ptr, d_pair_y_.ptr, n_,
                                             n_ * comp_k_, index_, comp_vecs_.ptr, comp_dsum_.ptr,
-                                            perms, vals, dcc_perm_jt_, perms ? dcc_perm_rb_ : 1,
+                                            perms, vals, dcc_perm_jt_, perms ? block : 1,
                                             grid, stream_),
                 "dcc.perm replicate launch");
   }
@@ -484,8 +514,6 @@ class Engine {
     const bool hsic = method_ == KPC_HSIC_PERM;
     const int R = opt_.replicates;
     const double scale = hsic ? 1.0 : 1.0 / (double)n_;
-    const int vblock = hsic ? kpc::kHsicPermBlock : dcc_perm_rb_;
-    const int gw = kpc::perm_generate_width(n_);
     const long long budget = perm_chunk_replicates();
     std::vector<kpc::PermUnit> obs_units((size_t)count);
     std::vector<double> state((size_t)4 * count);
@@ -499,12 +527,44 @@ class Engine {
     d_perm_units_.upload(obs_units, stream_);
     perm_state_.upload(state, stream_);
     perm_obs_.reserve((size_t)count);
-    perm_values(d_perm_units_.ptr, count, nullptr, perm_obs_.ptr);
+    perm_values(d_perm_units_.ptr, count, nullptr, perm_obs_.ptr, 1);
     hip_check(hipStreamSynchronize(stream_), "permutation synchronize");
     if (R > 0) {
       const long long chunk_reps = std::min(budget, (long long)count * R);
       perm_scratch_.reserve((size_t)chunk_reps * n_);
       perm_vals_.reserve((size_t)chunk_reps);
+      // tuned launch shapes (bit-identical results for every candidate):
+      // replicates per generation unit and per replicate-kernel unit
+      kpc::Tuner& tuner = kpc::Tuner::instance();
+      const int nb = (n_ + 255) / 256 * 256;
+      const int gmax = kpc::perm_generate_width(n_);
+      const std::string gkey = "perm_generate_width/n" + std::to_string(nb);
+      std::vector<int> gcand = gmax > 1 ? std::vector<int>{gmax, (gmax + 1) / 2}
+                                        : std::vector<int>{1};
+      if (const char* e = std::getenv("KPC_PERM_GEN_WIDTH")) {
+        const int w = std::atoi(e);
+        if (w >= 1 && w <= gmax) gcand = {w};
+      }
+      std::string vkey;
+      std::vector<int> vcand;
+      if (hsic) {
+        vkey = "hsic_perm_block/n" + std::to_string(nb) + "/kt" + std::to_string(hsic_kt());
+        vcand = {16, 8, 32};
+        if (const char* e = std::getenv("KPC_HSIC_PERM_BLOCK")) {
+          const int b = std::atoi(e);
+          if (b >= 1 && b <= 256) vcand = {b};
+        }
+      } else {
+        vkey = "dcc_perm_rb/n" + std::to_string(nb) + "/jt" + std::to_string(dcc_perm_jt_);
+        vcand = dcc_shape_forced_ ? std::vector<int>{dcc_perm_rb_} : std::vector<int>{8, 12};
+      }
+      int gw = 0, vb = 0;
+      bool gtrial = false, vtrial = false;
+      auto pick = [&]() {
+        gw = tuner.choose(gkey, gcand, &gtrial);
+        vb = tuner.choose(vkey, vcand, &vtrial);
+      };
+      pick();
       // one chunk: [segments | generation units | value units]; the host list
       // is the source of an asynchronous copy, so each chunk ends with a
       // stream synchronisation before the list is rebuilt
@@ -517,18 +577,36 @@ class Engine {
         segs.insert(segs.end(), val.begin(), val.end());
         d_perm_units_.upload(segs, stream_);
         const kpc::PermUnit* U = d_perm_units_.ptr;
-        hip_check(kpc::launch_perm_generate(U + ns, (int)ng, d_seeds_.ptr, n_, perm_scratch_.ptr,
-                                            (int)std::min<size_t>(ng, 8192), stream_),
+        hip_check(hipEventRecord(tev_[0], stream_), "event");
+        hip_check(kpc::launch_perm_generate(U + ns, (int)ng, d_seeds_.ptr, n_, gw,
+                                            perm_scratch_.ptr, (int)std::min<size_t>(ng, 8192),
+                                            stream_),
                   "permutation generate launch");
-        perm_values(U + ns + ng, (int)nv, perm_scratch_.ptr, perm_vals_.ptr);
+        hip_check(hipEventRecord(tev_[1], stream_), "event");
+        perm_values(U + ns + ng, (int)nv, perm_scratch_.ptr, perm_vals_.ptr, vb);
+        hip_check(hipEventRecord(tev_[2], stream_), "event");
         hip_check(kpc::launch_perm_tally(U, (int)ns, perm_vals_.ptr, perm_obs_.ptr, scale,
                                          perm_state_.ptr, stream_),
                   "permutation tally launch");
         hip_check(hipStreamSynchronize(stream_), "permutation synchronize");
+        // a trial counts when the chunk carried enough work to time
+        const double reps = (double)used;
+        if ((gtrial || vtrial) && reps * n_ >= 4e6) {
+          float ms = 0.f;
+          if (gtrial) {
+            hip_check(hipEventElapsedTime(&ms, tev_[0], tev_[1]), "event elapsed");
+            tuner.record(gkey, gw, ms * 1e-3, reps);
+          }
+          if (vtrial) {
+            hip_check(hipEventElapsedTime(&ms, tev_[1], tev_[2]), "event elapsed");
+            tuner.record(vkey, vb, ms * 1e-3, reps);
+          }
+        }
         segs.clear();
         gen.clear();
         val.clear();
         used = 0;
+        pick();
       };
       for (int t = 0; t < count; ++t) {
         int r = 0;
@@ -538,8 +616,8 @@ class Engine {
           segs.push_back({t, r, take, slot0});
           for (int b = 0; b < take; b += gw)
             gen.push_back({t, r + b, std::min(gw, take - b), slot0 + b});
-          for (int b = 0; b < take; b += vblock)
-            val.push_back({t, r + b, std::min(vblock, take - b), slot0 + b});
+          for (int b = 0; b < take; b += vb)
+            val.push_back({t, r + b, std::min(vb, take - b), slot0 + b});
           used += take;
           r += take;
           if (used == budget) flush();
@@ -600,6 +678,8 @@ class Engine {
   double stage(int i) const { return stage_[i]; }
   long long launches(int i) const { return launches_[i]; }
   double stage_host() const { return stage_host_; }
+  double graph_seconds() const { return graph_seconds_; }
+  long long graph_launches() const { return graph_launches_; }
   struct Stats {
     long long physical_tests = 0, fits = 0, components = 0, setups = 0, evals = 0,
               lanczos_steps = 0;
@@ -780,12 +860,17 @@ class Engine {
   // permutation tests
   static constexpr long long kPermScratchBytes = 1LL << 30;
   int dcc_perm_jt_ = 16, dcc_perm_rb_ = 8;
+  bool dcc_shape_forced_ = false;
+  hipEvent_t tev_[3] = {nullptr, nullptr, nullptr};
   DevArray<double> comp_rows_, perm_vals_, perm_obs_, perm_state_;
   DevArray<unsigned short> perm_scratch_;
   DevArray<kpc::PermUnit> d_perm_units_;
   double stage_[5] = {0, 0, 0, 0, 0};
   long long launches_[5] = {0, 0, 0, 0, 0};
   double stage_host_ = 0.0;
+  bool use_graph_ = std::getenv("KPC_GRAPH") != nullptr && std::atoi(std::getenv("KPC_GRAPH")) != 0;
+  double graph_seconds_ = 0.0;
+  long long graph_launches_ = 0;
   bool pending_smooth_checks_ = false;
 };
 
@@ -1095,6 +1180,8 @@ int kpc_skeleton_run(const double* data, int n, int p, const kpc_options* opt, k
       r->stage_launches[s] = eng.launches(s);
     }
     r->stage_seconds[5] = eng.stage_host();
+    r->stage_seconds[7] = eng.graph_seconds();
+    r->stage_launches[7] = eng.graph_launches();
     r->physical_tests = eng.stats_.physical_tests;
     r->residual_fits = eng.stats_.fits;
     r->components = eng.stats_.components;
@@ -1305,6 +1392,13 @@ int kpc_regrxons_mixed(const double* data, int n, int p, int count, const int* t
   } catch (const std::exception& e) {
     return fill_error(e);
   }
+}
+
+int kpc_tuning_report(char* buf, int cap) {
+  if (!buf || cap <= 0) return -1;
+  const std::string r = kpc::Tuner::instance().report();
+  std::snprintf(buf, (size_t)cap, "%s", r.c_str());
+  return (int)r.size();
 }
 
 int kpc_last_fit_timing(double* out) {

@@ -434,10 +434,11 @@ int perm_generate_width(int n) {
 }
 
 hipError_t launch_perm_generate(const PermUnit* units, int nunits,
-                                const unsigned long long* seeds, int n, unsigned short* perms,
-                                int grid, hipStream_t stream) {
+                                const unsigned long long* seeds, int n, int width,
+                                unsigned short* perms, int grid, hipStream_t stream) {
   if (nunits <= 0) return hipSuccess;
-  const size_t lds = (size_t)perm_generate_width(n) * n * sizeof(unsigned short);
+  if (width < 1 || width > KPC_WAVE) return hipErrorInvalidValue;
+  const size_t lds = (size_t)width * n * sizeof(unsigned short);
   hipError_t err = hipFuncSetAttribute((const void*)perm_generate_kernel,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (err != hipSuccess) return err;

@@ -71,7 +71,8 @@ typedef struct kpc_result {
   double* task_p;
   unsigned char* task_deleted;
   double wall_seconds;
-  double stage_seconds[8];  /* smooth, geometry, gcv, component, pair, host, h2d/d2h, other */
+  double stage_seconds[8];  /* smooth, geometry, gcv, component, pair, host, h2d/d2h,
+                               [7] whole-wave graphs (KPC_GRAPH=1: stages 0-4 not split) */
   long long physical_tests;
   long long residual_fits;
   long long components;
@@ -133,6 +134,12 @@ int kpc_regrxons_mixed(const double* data, int n, int p, int count, const int* t
  * seconds (smooth setups + geometry), out[1] GCV kernel seconds, out[2] GCV
  * launches.  No counterpart in the reference (its timers are R-side). */
 int kpc_last_fit_timing(double* out);
+
+/* Launch-shape autotuner state, one line per tuned knob ("key value=<v>
+ * <candidate>:<seconds per replicate> ..."), NUL-terminated into buf (cap
+ * bytes); returns the full length.  The tuned knobs change only how work is
+ * split over workgroups, never results.  No counterpart in the reference. */
+int kpc_tuning_report(char* buf, int cap);
 
 /* GCV score, gradient (m) and Hessian (m x m) in log(sp) at log_sp (count x m)
  * for target on S. */

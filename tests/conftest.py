@@ -6,6 +6,8 @@ import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+# the launch-shape autotuner measures afresh in every test process
+os.environ.setdefault("KPC_TUNE_CACHE", "off")
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
