@@ -915,6 +915,7 @@ __device__ void magic_eval(const MagicLds& L, const MagicScratch& S, const doubl
   const int wid = tid / KPC_WAVE;
   const int nw = blockDim.x / KPC_WAVE;
   const double* spv = L.spv;
+  const long long te0 = wall_clock64();
   if (tid < m) {
     const double e = exp(lspv[tid]);
     L.spv[tid] = e;
@@ -1079,6 +1080,7 @@ __device__ void magic_eval(const MagicLds& L, const MagicScratch& S, const doubl
     L.dctl[1] = edf;
     L.dctl[2] = delta > 1e-8 ? (double)n * rss / (delta * delta) : __builtin_inf();
     L.ictl[2] = 0;
+    L.dctl[20] += (double)(wall_clock64() - te0);
   }
   __syncthreads();
 }
@@ -1251,22 +1253,41 @@ __device__ void magic_derivs(const MagicLds& L, const MagicScratch& S, const dou
         L.hess[j * m + i] = v;
         L.hess[i * m + j] = v;
       }
+    L.dctl[21] += (double)(wall_clock64() - td0);
   }
   __syncthreads();
 }
 
-// Symmetric eigen of the small Hessian (cyclic Jacobi), lane-serial on LDS
-// buffers (A: m x m work, Z: eigenvectors, w: eigenvalues).
-__device__ void small_syev(const double* Hin, int m, double* w, double* Z, double* A) {
-  for (int i = 0; i < m * m; ++i) A[i] = Hin[i];
-  for (int i = 0; i < m * m; ++i) Z[i] = (i % m == i / m) ? 1.0 : 0.0;
+// Symmetric eigen of the small Hessian by cyclic Jacobi (the oracle's rotation
+// order and arithmetic), run by one wave: each rotation's element updates are
+// spread over the lanes, the rotation itself is computed redundantly by every
+// lane, and the off/total test is summed by lane 0 in the serial order.  A
+// sweep is a deterministic function of (A, Z), so once a sweep changes no bit
+// every later sweep is the identity and the loop stops with exactly the state
+// the full sweep budget reaches.
+__device__ __forceinline__ bool bits_differ(double x, double y) {
+  return __double_as_longlong(x) != __double_as_longlong(y);
+}
+
+__device__ void small_syev_wave(const double* Hin, int m, double* w, double* Z, double* A) {
+  const int lane = threadIdx.x & (KPC_WAVE - 1);
+  for (int i = lane; i < m * m; i += KPC_WAVE) {
+    A[i] = Hin[i];
+    Z[i] = (i % m == i / m) ? 1.0 : 0.0;
+  }
+  wave_sync();
   for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0.0, tot = 0.0;
-    for (int i = 0; i < m * m; ++i) {
-      tot += A[i] * A[i];
-      if (i % m != i / m) off += A[i] * A[i];
+    int stop = 0;
+    if (lane == 0) {
+      double off = 0.0, tot = 0.0;
+      for (int i = 0; i < m * m; ++i) {
+        tot += A[i] * A[i];
+        if (i % m != i / m) off += A[i] * A[i];
+      }
+      stop = off <= 1e-300 || off <= kEps * kEps * tot * 1e-4;
     }
-    if (off <= 1e-300 || off <= kEps * kEps * tot * 1e-4) break;
+    if (__shfl(stop, 0, KPC_WAVE)) break;
+    bool changed = false;
     for (int p = 0; p < m - 1; ++p)
       for (int q2 = p + 1; q2 < m; ++q2) {
         const double apq = A[q2 * m + p];
@@ -1275,24 +1296,35 @@ __device__ void small_syev(const double* Hin, int m, double* w, double* Z, doubl
         const double theta = (aqq - app) / (2.0 * apq);
         const double t = copysign(1.0, theta) / (fabs(theta) + sqrt(theta * theta + 1.0));
         const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
-        for (int k = 0; k < m; ++k) {
+        wave_sync();
+        for (int k = lane; k < m; k += KPC_WAVE) {
           const double akp = A[p * m + k], akq = A[q2 * m + k];
-          A[p * m + k] = c * akp - s * akq;
-          A[q2 * m + k] = s * akp + c * akq;
+          const double np = c * akp - s * akq, nq = s * akp + c * akq;
+          changed |= bits_differ(np, akp) || bits_differ(nq, akq);
+          A[p * m + k] = np;
+          A[q2 * m + k] = nq;
         }
-        for (int k = 0; k < m; ++k) {
+        wave_sync();
+        for (int k = lane; k < m; k += KPC_WAVE) {
           const double apk = A[k * m + p], aqk = A[k * m + q2];
-          A[k * m + p] = c * apk - s * aqk;
-          A[k * m + q2] = s * apk + c * aqk;
+          const double np = c * apk - s * aqk, nq = s * apk + c * aqk;
+          changed |= bits_differ(np, apk) || bits_differ(nq, aqk);
+          A[k * m + p] = np;
+          A[k * m + q2] = nq;
         }
-        for (int k = 0; k < m; ++k) {
+        for (int k = lane; k < m; k += KPC_WAVE) {
           const double zkp = Z[p * m + k], zkq = Z[q2 * m + k];
-          Z[p * m + k] = c * zkp - s * zkq;
-          Z[q2 * m + k] = s * zkp + c * zkq;
+          const double np = c * zkp - s * zkq, nq = s * zkp + c * zkq;
+          changed |= bits_differ(np, zkp) || bits_differ(nq, zkq);
+          Z[p * m + k] = np;
+          Z[q2 * m + k] = nq;
         }
+        wave_sync();
       }
+    if (!__any(changed)) break;
   }
-  for (int i = 0; i < m; ++i) w[i] = A[i * m + i];
+  for (int i = lane; i < m; i += KPC_WAVE) w[i] = A[i * m + i];
+  wave_sync();
 }
 
 __global__ __launch_bounds__(kMagicThreads) void magic_kernel(
@@ -1315,6 +1347,7 @@ __global__ __launch_bounds__(kMagicThreads) void magic_kernel(
     for (int r = tid; r < n; r += blockDim.x) { L.y[r] = y[r]; L.yorig[r] = y[r]; }
     if (tid == 0) {
       L.dctl[26] = L.dctl[27] = L.dctl[29] = 0.0;
+      L.dctl[20] = L.dctl[21] = L.dctl[22] = L.dctl[23] = 0.0;
       L.ictl[28] = 0;
     }
     __syncthreads();
@@ -1332,6 +1365,7 @@ __global__ __launch_bounds__(kMagicThreads) void magic_kernel(
     for (int r = tid; r < n; r += blockDim.x) part += L.yorig[r] * L.yorig[r];
     const double ynorm2 = block_sum(part, L.red);
     if (tid < m) L.lsp[tid] = D[DL.lsp0 + tid];
+    if (tid == 0) L.dctl[22] = (double)(wall_clock64() - t_start);  // Q'y prologue
     __syncthreads();
     const int max_half = m == 1 ? ctl.max_half_single : ctl.max_half_multi;
     magic_eval(L, S, D, DL, n, q, m, rk, L.lsp, ynorm2, ctl.rank_tol);
@@ -1409,9 +1443,13 @@ __global__ __launch_bounds__(kMagicThreads) void magic_kernel(
           ic[4] = conv;
         }
         for (int i = 0; i < m; ++i) L.grad[i] = L.cgrad[i];
+      }
+      __syncthreads();
+      if (tid < KPC_WAVE) small_syev_wave(L.hess, m, L.syw, L.syZ, L.syA);
+      __syncthreads();
+      if (tid == 0) {
         double* w = L.syw;
         double* Z = L.syZ;
-        small_syev(L.hess, m, w, Z, L.syA);
         int sd = 0;
         for (int i = 0; i < m; ++i) {
           if (w[i] < 0.0) sd = 1;
@@ -1467,6 +1505,7 @@ __global__ __launch_bounds__(kMagicThreads) void magic_kernel(
       }
     }
     // final fit at the selected log(sp)
+    const long long t_final = wall_clock64();
     magic_eval(L, S, D, DL, n, q, m, rk, L.lsp, ynorm2, ctl.rank_tol);
     {
       const int nr = L.ictl[0];
@@ -1510,6 +1549,12 @@ __global__ __launch_bounds__(kMagicThreads) void magic_kernel(
         out[kMagicInfo - 5] = L.dctl[27];  // ticks in the SVD
         out[kMagicInfo - 6] = L.dctl[29];  // ticks in the derivatives
         out[kMagicInfo - 7] = (double)L.ictl[28];  // Jacobi sweeps
+        if (8 + m <= 20) {  // phase ticks where the log(sp) slots leave room
+          out[20] = L.dctl[20];  // all objective evaluations
+          out[21] = L.dctl[21];  // all derivative evaluations
+          out[22] = L.dctl[22];  // Q'y prologue
+          out[23] = (double)(wall_clock64() - t_final);  // final fit and residuals
+        }
       }
       __syncthreads();
     }

@@ -228,7 +228,17 @@ int orc_svd_jacobi(const double* a, int m, int n, double* u, double* s, double* 
 }
 
 /* ---------------------------------------------------------------- syev */
-/* Symmetric eigen by cyclic Jacobi; values ascending, vectors column-major. */
+/* Symmetric eigen by cyclic Jacobi; values ascending, vectors column-major.
+ * A sweep is a deterministic function of (a, z): once a sweep changes no bit
+ * of either, every later sweep is the identity, so the loop stops there with
+ * exactly the state the full sweep budget would reach. */
+static int bits_differ(double x, double y) {
+  unsigned long long a, b;
+  memcpy(&a, &x, sizeof a);
+  memcpy(&b, &y, sizeof b);
+  return a != b;
+}
+
 int orc_syev_jacobi(const double* a_in, int n, double* w, double* z) {
   double* a = (double*)malloc(sizeof(double) * (size_t)n * n);
   memcpy(a, a_in, sizeof(double) * (size_t)n * n);
@@ -244,6 +254,7 @@ int orc_syev_jacobi(const double* a_in, int n, double* w, double* z) {
         if (i != j) off += x;
       }
     if (off <= 1e-300 || off <= DBL_EPSILON * DBL_EPSILON * tot * 1e-4) break;
+    int changed = 0;
     for (int p = 0; p < n - 1; ++p)
       for (int q = p + 1; q < n; ++q) {
         double apq = a[(size_t)q * n + p];
@@ -254,20 +265,27 @@ int orc_syev_jacobi(const double* a_in, int n, double* w, double* z) {
         double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
         for (int k = 0; k < n; ++k) {
           double akp = a[(size_t)p * n + k], akq = a[(size_t)q * n + k];
-          a[(size_t)p * n + k] = c * akp - s * akq;
-          a[(size_t)q * n + k] = s * akp + c * akq;
+          double np = c * akp - s * akq, nq = s * akp + c * akq;
+          changed |= bits_differ(np, akp) | bits_differ(nq, akq);
+          a[(size_t)p * n + k] = np;
+          a[(size_t)q * n + k] = nq;
         }
         for (int k = 0; k < n; ++k) {
           double apk = a[(size_t)k * n + p], aqk = a[(size_t)k * n + q];
-          a[(size_t)k * n + p] = c * apk - s * aqk;
-          a[(size_t)k * n + q] = s * apk + c * aqk;
+          double np = c * apk - s * aqk, nq = s * apk + c * aqk;
+          changed |= bits_differ(np, apk) | bits_differ(nq, aqk);
+          a[(size_t)k * n + p] = np;
+          a[(size_t)k * n + q] = nq;
         }
         for (int k = 0; k < n; ++k) {
           double zkp = z[(size_t)p * n + k], zkq = z[(size_t)q * n + k];
-          z[(size_t)p * n + k] = c * zkp - s * zkq;
-          z[(size_t)q * n + k] = s * zkp + c * zkq;
+          double np = c * zkp - s * zkq, nq = s * zkp + c * zkq;
+          changed |= bits_differ(np, zkp) | bits_differ(nq, zkq);
+          z[(size_t)p * n + k] = np;
+          z[(size_t)q * n + k] = nq;
         }
       }
+    if (!changed) break;
   }
   for (int i = 0; i < n; ++i) w[i] = a[(size_t)i * n + i];
   for (int j = 0; j < n; ++j) {
