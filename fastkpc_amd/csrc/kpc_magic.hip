@@ -199,6 +199,41 @@ __device__ void upper_tri_inverse(const double* R, int ld, int q, double* X, dou
   __syncthreads();
 }
 
+typedef double v4d_t __attribute__((ext_vector_type(4)));
+
+// C (M x N, column-major, ldc) = A' B for A (K x M, lda) and B (K x N, ldb),
+// column-major, on FP64 matrix cores: the 16 x 16 output tiles are spread
+// over the waves, each accumulating v_mfma_f64_16x16x4f64 over K in steps of
+// 4 (lane (g, c) feeds A[k0 + g][i0 + c] and B[k0 + g][j0 + c]; entries past
+// K, M or N read as zero).  No barrier inside: the caller orders phases.
+__device__ void wg_gemm_tn(const double* A, int lda, const double* B, int ldb, double* C, int ldc,
+                           int M, int N, int K) {
+  const int lane = threadIdx.x & (KPC_WAVE - 1);
+  const int wid = threadIdx.x / KPC_WAVE;
+  const int nw = blockDim.x / KPC_WAVE;
+  const int g = lane >> 4, c = lane & 15;
+  const int tm = (M + 15) / 16, tn = (N + 15) / 16;
+  for (int t = wid; t < tm * tn; t += nw) {
+    const int i0 = (t % tm) * 16, j0 = (t / tm) * 16;
+    const int ia = i0 + c, jb = j0 + c;
+    const double* a = A + (size_t)(ia < M ? ia : 0) * lda;
+    const double* b = B + (size_t)(jb < N ? jb : 0) * ldb;
+    v4d_t acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < K; k0 += 4) {
+      const int k = k0 + g;
+      const int kk = k < K ? k : 0;
+      const double av = a[kk], bv = b[kk];
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64((k < K && ia < M) ? av : 0.0,
+                                                (k < K && jb < N) ? bv : 0.0, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = i0 + g + 4 * v, j = j0 + c;
+      if (i < M && j < N) C[(size_t)j * ldc + i] = acc[v];
+    }
+  }
+}
+
 // ------------------------------------------------------------- geometry
 __global__ __launch_bounds__(kGeomThreads) void geometry_kernel(
     const GeometryItem* __restrict__ items, int count, int n, int q, int m, int dim, int rk,
@@ -967,7 +1002,9 @@ __device__ void magic_eval(const MagicLds& L, const MagicScratch& S, const doubl
     rows = q + erows;
     build_augmented(L, D, DL, q, erows, agg);
   }
+  const long long tq0 = wall_clock64();
   stacked_qr(L.aug, rows, q, erows, L.wk);
+  const long long tq1 = wall_clock64();
   if (!qr_ok) {
     // stage 2 from the factor itself: 1 / (|R2|_F |R2^-1|_F)
     upper_tri_inverse(L.aug, rows, q, L.V, L.wk + q);
@@ -986,7 +1023,15 @@ __device__ void magic_eval(const MagicLds& L, const MagicScratch& S, const doubl
     qr_ok = isfinite(lb) && lb > 2.0 * rank_tol;
     if (qr_ok && tid == 0) L.ictl[15] = 1;  // R2 inverted already
   }
+  const long long tq2 = wall_clock64();
   if (qr_ok) stacked_q1(L.aug, rows, q, erows, L.wk, L.Qm);
+  const long long tq3 = wall_clock64();
+  if (tid == 0) {
+    L.dctl[8] += (double)(tq0 - te0);   // penalty scaling + augmented build
+    L.dctl[9] += (double)(tq1 - tq0);   // stacked QR
+    L.dctl[10] += (double)(tq2 - tq1);  // rank certificate from the factor
+    L.dctl[11] += (double)(tq3 - tq2);  // Q1
+  }
   if (qr_ok) {
     // ---- QR path on [R; sqrt(sp_i) rS_i'] (same Gram matrix as [R; E])
     if (tid == 0) {
@@ -1097,49 +1142,52 @@ __device__ void magic_derivs(const MagicLds& L, const MagicScratch& S, const dou
   const bool qr = L.ictl[13] != 0;
   // G = U1' U1 (nr x nr) first: after it the augmented buffer is free and
   // holds T and GT, so the whole derivative working set stays in LDS.  On the
-  // QR basis U1 -> Q1 and D^-1 V1' -> R2^-T (same traces and products).
+  // QR basis U1 -> Q1 and D^-1 V1' -> R2^-T (same traces and products).  The
+  // dense products run on FP64 matrix cores (wg_gemm_tn).
   double* G = L.X4;
-  for (int e = tid; e < nr * nr; e += blockDim.x) {
-    const int i = e % nr, j = e / nr;
-    const double* ui = qr ? L.Qm + (size_t)i * rows : L.aug + (size_t)L.ord[i] * rows;
-    const double* uj = qr ? L.Qm + (size_t)j * rows : L.aug + (size_t)L.ord[j] * rows;
-    double s = 0.0;
-    for (int r = 0; r < q; ++r) s += ui[r] * uj[r];
-    G[e] = s;
-  }
-  __syncthreads();
-  if (qr && !L.ictl[15]) upper_tri_inverse(L.aug, rows, q, L.V, L.wk);  // before T overwrites R2
   double* T = L.aug;                          // nr x mr (mr <= q)
   double* GT = L.aug + (size_t)q * q;         // nr x mr
-  // T[c][a] = (V1' rS)[c][a] / d_c, or (R2^-T rS)[c][a]
-  for (int e = tid; e < nr * mr; e += blockDim.x) {
-    const int c = e % nr, a = e / nr;
-    const double* rs = D + DL.rS + (size_t)a * q;
-    double s = 0.0;
-    if (qr) {
-      const double* x = L.V + (size_t)c * q;  // column c of R2^-1
-      for (int k = 0; k <= c; ++k) s += x[k] * rs[k];
-      T[(size_t)a * nr + c] = s;
-    } else {
+  if (qr) {
+    wg_gemm_tn(L.Qm, rows, L.Qm, rows, G, nr, nr, nr, q);
+    __syncthreads();
+    if (!L.ictl[15]) upper_tri_inverse(L.aug, rows, q, L.V, L.wk);  // before T overwrites R2
+    // T = R2^-T rS (R2^-1 is upper triangular with exact zeros below)
+    wg_gemm_tn(L.V, q, D + DL.rS, q, T, nr, nr, mr, q);
+  } else {
+    for (int e = tid; e < nr * nr; e += blockDim.x) {
+      const int i = e % nr, j = e / nr;
+      const double* ui = L.aug + (size_t)L.ord[i] * rows;
+      const double* uj = L.aug + (size_t)L.ord[j] * rows;
+      double s = 0.0;
+      for (int r = 0; r < q; ++r) s += ui[r] * uj[r];
+      G[e] = s;
+    }
+    __syncthreads();
+    // T[c][a] = (V1' rS)[c][a] / d_c
+    for (int e = tid; e < nr * mr; e += blockDim.x) {
+      const int c = e % nr, a = e / nr;
+      const double* rs = D + DL.rS + (size_t)a * q;
       const double* v = L.V + (size_t)L.ord[c] * q;
+      double s = 0.0;
       for (int k = 0; k < q; ++k) s += v[k] * rs[k];
       T[(size_t)a * nr + c] = s / L.d[L.ord[c]];
     }
   }
   __syncthreads();
-  // GT = G T, gy = G y1
-  for (int e = tid; e < nr * mr; e += blockDim.x) {
-    const int c = e % nr, a = e / nr;
-    double s = 0.0;
-    for (int k = 0; k < nr; ++k) s += G[(size_t)k * nr + c] * T[(size_t)a * nr + k];
-    GT[e] = s;
-  }
+  // GT = G T (G symmetric), gy = G y1
+  wg_gemm_tn(G, nr, T, nr, GT, nr, nr, mr, nr);
   for (int c = tid; c < nr; c += blockDim.x) {
     double s = 0.0;
     for (int k = 0; k < nr; ++k) s += G[(size_t)k * nr + c] * L.y1[k];
     L.gyl[c] = s;
   }
   __syncthreads();
+  // P = T'T and Qx = T' G T (mr x mr) in the Q1 area (free from here on):
+  // tr(M_j K_i) = sum_{a in j, b in i} P[a][b] Qx[b][a], tr(K_i) = sum_{a in i} Qx[a][a]
+  double* P = L.Qm;
+  double* Qx = L.Qm + (size_t)mr * mr;
+  wg_gemm_tn(T, nr, T, nr, P, mr, mr, mr, nr);
+  wg_gemm_tn(T, nr, GT, nr, Qx, mr, mr, mr, nr);
   // per penalty: My = T_i (T_i' y1), Ky = T_i (T_i' G y1), yK = G My; the
   // column projections T_a' y1 and T_a' G y1 are formed once
   for (int a = tid; a < mr; a += blockDim.x) {
@@ -1168,33 +1216,49 @@ __device__ void magic_derivs(const MagicLds& L, const MagicScratch& S, const dou
     for (int k = 0; k < nr; ++k) s += G[(size_t)k * nr + o] * L.My[(size_t)i * nr + k];
     L.yK[(size_t)i * nr + o] = s;
   }
-  __syncthreads();
-  // pairwise traces tr(M_j K_i) = tr((T_j'T_i)(T_i' G T_j)), one workgroup
-  // reduction per (i, j <= i); then tr(K_i) = sum_a T_ia' G T_ia
-  for (int i = 0; i < m; ++i) {
-    for (int j = 0; j <= i; ++j) {
+  const long long td1 = wall_clock64();
+  // traces: one wave per (i, j <= i) pair and per tr(K_i), no workgroup barrier
+  {
+    const int lane = tid & (KPC_WAVE - 1);
+    const int wid = tid / KPC_WAVE;
+    const int nw = blockDim.x / KPC_WAVE;
+    const int npair = m * (m + 1) / 2;
+    for (int e = wid; e < npair + m; e += nw) {
       double part = 0.0;
-      for (int e = tid; e < rk * rk; e += blockDim.x) {
-        const int a = e % rk, b = e / rk;  // a: column of T_j, b: column of T_i
-        const double* Tj = T + (size_t)(j * rk + a) * nr;
-        const double* Ti = T + (size_t)(i * rk + b) * nr;
-        const double* GTj = GT + (size_t)(j * rk + a) * nr;
-        double x = 0.0, y = 0.0;
-        for (int c = 0; c < nr; ++c) { x += Tj[c] * Ti[c]; y += Ti[c] * GTj[c]; }
-        part += x * y;
+      int i = 0, j = 0;
+      if (e < npair) {
+        int rem = e;
+        while (rem > i) {
+          rem -= i + 1;
+          ++i;
+        }
+        j = rem;
+        for (int u = lane; u < rk * rk; u += KPC_WAVE) {
+          const int ja = j * rk + u % rk, ib = i * rk + u / rk;
+          part += P[(size_t)ib * mr + ja] * Qx[(size_t)ja * mr + ib];
+        }
+      } else {
+        i = e - npair;
+        for (int a = lane; a < rk; a += KPC_WAVE) {
+          const int ia = i * rk + a;
+          part += Qx[(size_t)ia * mr + ia];
+        }
       }
-      const double tsum = block_sum(part, L.red);
-      if (tid == 0) L.tr[j * m + i] = tsum;
+      part = wave_sum(part);
+      if (lane == 0) {
+        if (e < npair)
+          L.tr[j * m + i] = part;
+        else
+          L.tr[m * m + i] = part;
+      }
     }
-    double part = 0.0;
-    for (int e = tid; e < rk * nr; e += blockDim.x) {
-      const int a = e / nr, c = e % nr;
-      part += T[(size_t)(i * rk + a) * nr + c] * GT[(size_t)(i * rk + a) * nr + c];
-    }
-    const double tr = block_sum(part, L.red);
-    if (tid == 0) L.tr[m * m + i] = tr;
   }
   __syncthreads();
+  const long long td2 = wall_clock64();
+  if (tid == 0) {
+    L.dctl[12] += (double)(td1 - td0);  // G, T, G T, P, Qx, projections
+    L.dctl[13] += (double)(td2 - td1);  // traces
+  }
   // d|y-Ay|^2 terms: one lane per i (dnorm) and per pair j <= i (d2n)
   const int npairs = m * (m + 1) / 2;
   for (int e = tid; e < m + npairs; e += blockDim.x) {
@@ -1348,6 +1412,8 @@ __global__ __launch_bounds__(kMagicThreads) void magic_kernel(
     if (tid == 0) {
       L.dctl[26] = L.dctl[27] = L.dctl[29] = 0.0;
       L.dctl[20] = L.dctl[21] = L.dctl[22] = L.dctl[23] = 0.0;
+      L.dctl[8] = L.dctl[9] = L.dctl[10] = L.dctl[11] = 0.0;
+      L.dctl[12] = L.dctl[13] = 0.0;
       L.ictl[28] = 0;
     }
     __syncthreads();
@@ -1549,6 +1615,14 @@ __global__ __launch_bounds__(kMagicThreads) void magic_kernel(
         out[kMagicInfo - 5] = L.dctl[27];  // ticks in the SVD
         out[kMagicInfo - 6] = L.dctl[29];  // ticks in the derivatives
         out[kMagicInfo - 7] = (double)L.ictl[28];  // Jacobi sweeps
+        if (8 + m <= 14) {  // evaluation sub-phases where the log(sp) slots leave room
+          out[14] = L.dctl[8];
+          out[15] = L.dctl[9];
+          out[16] = L.dctl[10];
+          out[17] = L.dctl[11];
+          out[18] = L.dctl[12];
+          out[19] = L.dctl[13];
+        }
         if (8 + m <= 20) {  // phase ticks where the log(sp) slots leave room
           out[20] = L.dctl[20];  // all objective evaluations
           out[21] = L.dctl[21];  // all derivative evaluations
