@@ -64,12 +64,16 @@ def pair_flops(n, k, pairs):
     return pairs * (2.0 * k * k * n + 3.0 * k * k)
 
 
-def read_traffic(kernel):
+def read_traffic(kernel, workload_tag):
+    """Calibrated HBM bytes per launch of `kernel` from the committed counter
+    profile, when that profile was taken on this workload (else None)."""
     path = os.path.join(ROOT, "profiles", "traffic_%s.json" % kernel)
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         d = json.load(fh)
+    if workload_tag not in d.get("workload", ""):
+        return None
     return d.get("hbm_bytes_per_launch")
 
 
@@ -333,7 +337,7 @@ def main():
         "peak": FP64_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved / FP64_PEAK_TFLOPS, 6),
-        "traffic": read_traffic(kname),
+        "traffic": read_traffic(kname, "%s %s" % (args.method, args.config)),
         "launches_per_step": launches,
         "avg_launch_us": round(ktime / max(1, launches) * 1e6, 1),
         "algorithmic_flops_per_launch": flops_per_run / max(1, launches),
