@@ -67,6 +67,7 @@ class KpcResult(ctypes.Structure):
         ("plain_seconds", ctypes.c_double * 64),
         ("stage_launches", ctypes.c_longlong * 8),
         ("task_margin", _D),
+        ("level_stage_seconds", (ctypes.c_double * 5) * 64),
     ]
 
 

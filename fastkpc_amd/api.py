@@ -131,6 +131,9 @@ def fastkpc_compatible_cuda_skeleton(data, alpha, labels=None, options=None):
             "level_evaluations": list(r.level_evaluations)[:levels],
             "level_lanczos_steps": list(r.level_lanczos_steps)[:levels],
             "level_seconds": list(r.level_seconds)[:levels],
+            "level_stage_seconds": [dict(zip(["smooth_setup", "geometry", "gcv", "dcov_component",
+                                              "dcov_pair"], list(r.level_stage_seconds[l])))
+                                    for l in range(levels)],
             "plain_tests": list(r.plain_tests)[:levels],
             "plain_seconds": list(r.plain_seconds)[:levels],
             "stage_launches": dict(zip(["smooth_setup", "geometry", "gcv", "dcov_component",

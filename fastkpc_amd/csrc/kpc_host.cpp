@@ -1003,6 +1003,8 @@ void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
         edges.push_back(er);
       }
     const double level_t0 = now_s();
+    double stage0[5];
+    for (int st = 0; st < 5; ++st) stage0[st] = eng.stage(st);
     // Levels whose whole plan needs at most this many residual fits are
     // evaluated speculatively in one wave (fewer round trips beat the wasted
     // fits); larger levels run frontier waves with doubling chunks.
@@ -1134,6 +1136,7 @@ void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
     r->level_evaluations[ord] = eng.stats_.evals - st0.evals;
     r->level_lanczos_steps[ord] = eng.stats_.lanczos_steps - st0.lanczos_steps;
     r->level_seconds[ord] = now_s() - level_t0;
+    for (int st = 0; st < 5; ++st) r->level_stage_seconds[ord][st] = eng.stage(st) - stage0[st];
     ++ord;
   }
   r->levels = ord;

@@ -98,6 +98,8 @@ typedef struct kpc_result {
    * 1e300 when none applies.  Below ~1e-12 a rounding difference can flip a
    * discrete decision. */
   double* task_margin;
+  /* device seconds per level and stage (as stage_seconds[0..4]) */
+  double level_stage_seconds[64][5];
 } kpc_result;
 
 void kpc_default_options(kpc_options* opt);

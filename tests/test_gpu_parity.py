@@ -56,6 +56,24 @@ def test_dcov_components_vs_oracle(dcov_fixture):
             np.testing.assert_allclose(s * vecs[i][:, c], oc[:, c], atol=1e-9)
 
 
+@pytest.mark.parametrize("n", [700, 1100])
+def test_dcov_components_large_n_vs_oracle(n):
+    # Lanczos components past n = 1024 (basis capped at 1024 vectors, matvec
+    # work items looping over the workgroup) against the oracle's dense eigen
+    f = fk()
+    o = orc()
+    rng = np.random.default_rng(n)
+    x = np.sin(rng.uniform(-3, 3, n)) + 0.4 * rng.standard_normal(n)
+    vals, vecs, dsum, mom = f.dcov_components(x[:, None], 35)
+    ov, oc, osum, omom = o.dcov_component(x, 35)
+    np.testing.assert_allclose(vals[0], ov, rtol=0, atol=1e-12 * abs(ov[0]))
+    assert abs(dsum[0] - osum) <= 1e-13 * osum
+    assert abs(mom[0] - omom) <= 1e-9 * abs(omom)
+    for c in range(10):
+        s = np.sign(vecs[0][:, c] @ oc[:, c])
+        np.testing.assert_allclose(s * vecs[0][:, c], oc[:, c], atol=1e-8)
+
+
 def test_dcov_translation_scale_and_swap_invariance():
     f = fk()
     rng = np.random.default_rng(21)
