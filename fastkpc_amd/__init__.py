@@ -4,9 +4,9 @@ kpcalg dcov.gamma, hsic.gamma, hsic.perm and dcc.perm tests, behind the
 reference's skeleton API."""
 from .api import (ci_batch, dcov_components, fastkpc_compatible_cuda_skeleton, gcv_grad_hess,
                   kernelCItest, last_fit_timing, permutation_table, regrXonS, regrxons_batch, regrxons_mixed,
-                  resolve_max_conditioning_size, task_seed, tuning_report)
+                  resolve_max_conditioning_size, task_seed, tuning_report, empty_cache)
 
 __all__ = ["fastkpc_compatible_cuda_skeleton", "kernelCItest", "regrXonS", "regrxons_batch",
-           "regrxons_mixed", "last_fit_timing", "tuning_report",
+           "regrxons_mixed", "last_fit_timing", "tuning_report", "empty_cache",
            "ci_batch", "gcv_grad_hess", "dcov_components", "resolve_max_conditioning_size",
            "task_seed", "permutation_table"]

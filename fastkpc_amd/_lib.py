@@ -75,7 +75,7 @@ EXPORTS = (
     "kpc_default_options", "kpc_skeleton_run", "kpc_result_free", "kpc_ci_batch",
     "kpc_regrxons_batch", "kpc_regrxons_mixed", "kpc_gcv_grad_hess", "kpc_dcov_components",
     "kpc_last_fit_timing", "kpc_tuning_report", "kpc_task_seed", "kpc_permutation_table", "kpc_last_error",
-    "kpc_device_count",
+    "kpc_device_count", "kpc_empty_cache",
 )
 
 _lib = None
@@ -116,6 +116,7 @@ def lib():
                                         ctypes.POINTER(ctypes.c_ulonglong), _I]
     L.kpc_last_error.restype = ctypes.c_char_p
     L.kpc_device_count.restype = ctypes.c_int
+    L.kpc_empty_cache.restype = ctypes.c_int
     _lib = L
     return L
 

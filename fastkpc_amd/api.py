@@ -271,6 +271,11 @@ def tuning_report():
     return out
 
 
+def empty_cache():
+    """Free the device blocks the library caches between calls."""
+    check(_lib.lib().kpc_empty_cache())
+
+
 def last_fit_timing():
     """Device seconds of the last regrXonS call: {"setup", "gcv", "gcv_launches"}."""
     out = np.zeros(3)

@@ -38,6 +38,8 @@
 #include <stdexcept>
 #include <string>
 #include <map>
+#include <mutex>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -55,16 +57,107 @@ void hip_check(hipError_t e, const char* what) {
     throw KpcError(-2, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// Process-wide cache of device blocks, best fit by size.  A released block
+// is handed out again only to the thread (and device) that released it: a
+// thread's calls run one after another, every stream that used the block is
+// either that same stream (stream order protects the reuse) or was
+// synchronised before its arenas were released (engine destructor, stream
+// guard), so a reused block is never still in use by an earlier kernel.
+struct PoolBlock {
+  void* ptr;
+  size_t bytes;
+  int device;
+  std::thread::id owner;
+};
+std::mutex g_pool_mutex;
+std::vector<PoolBlock> g_pool;
+
+void pool_empty();
+
+void* pool_acquire(size_t bytes, size_t* got = nullptr) {
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  {
+    std::lock_guard<std::mutex> lock(g_pool_mutex);
+    size_t best = (size_t)-1;
+    const std::thread::id me = std::this_thread::get_id();
+    for (size_t i = 0; i < g_pool.size(); ++i)
+      if (g_pool[i].device == dev && g_pool[i].owner == me && g_pool[i].bytes >= bytes && g_pool[i].bytes <= 4 * bytes + (1 << 20) &&
+          (best == (size_t)-1 || g_pool[i].bytes < g_pool[best].bytes))
+        best = i;
+    if (best != (size_t)-1) {
+      void* p = g_pool[best].ptr;
+      if (got) *got = g_pool[best].bytes;
+      g_pool.erase(g_pool.begin() + (long)best);
+      return p;
+    }
+  }
+  // round up so slightly larger later requests reuse the block
+  size_t rounded = bytes < (1 << 20) ? bytes : (bytes + (bytes >> 3) + 255) & ~(size_t)255;
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, rounded);
+  if (e != hipSuccess) {
+    // out of memory with cached blocks: give them back and retry once
+    pool_empty();
+    hip_check(hipMalloc(&p, rounded), "hipMalloc");
+  }
+  if (got) *got = rounded;
+  return p;
+}
+
+// Frees every cached block of every thread: blocks are taken out of the pool
+// first, then the device drains, so work enqueued before a block's release
+// has finished when it is freed.
+void pool_empty() {
+  std::vector<PoolBlock> drop;
+  {
+    std::lock_guard<std::mutex> lock(g_pool_mutex);
+    drop.swap(g_pool);
+  }
+  if (drop.empty()) return;
+  (void)hipDeviceSynchronize();
+  for (const PoolBlock& b : drop) (void)hipFree(b.ptr);
+}
+
+void pool_release(void* p, size_t bytes) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lock(g_pool_mutex);
+  g_pool.push_back({p, bytes, dev, std::this_thread::get_id()});
+}
+
+// A stream that is synchronised and destroyed when the guard goes out of
+// scope.  Declared after the arrays its work uses, it is destroyed first, so
+// no kernel is still running when they return their blocks to the pool.
+struct StreamGuard {
+  hipStream_t s = nullptr;
+  StreamGuard() { hip_check(hipStreamCreate(&s), "hipStreamCreate"); }
+  StreamGuard(const StreamGuard&) = delete;
+  StreamGuard& operator=(const StreamGuard&) = delete;
+  ~StreamGuard() {
+    if (s) {
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamDestroy(s);
+    }
+  }
+};
+
 template <typename T>
 struct DevArray {
   T* ptr = nullptr;
   size_t cap = 0;
+  // Blocks come from the process-wide device pool: growing an arena returns
+  // the old block to the pool (stream-ordered reuse by the same stream of a
+  // later call), so repeated calls allocate nothing once warm.
   void reserve(size_t count) {
     if (count <= cap) return;
-    if (ptr) hip_check(hipFree(ptr), "hipFree");
     size_t c = count < 64 ? 64 : count;
-    hip_check(hipMalloc((void**)&ptr, c * sizeof(T)), "hipMalloc");
-    cap = c;
+    size_t bytes = c * sizeof(T);
+    size_t got = bytes;
+    void* fresh = pool_acquire(bytes, &got);
+    if (ptr) pool_release(ptr, cap * sizeof(T));
+    ptr = (T*)fresh;
+    cap = got / sizeof(T);
   }
   void upload(const std::vector<T>& v, hipStream_t s) {
     reserve(v.size());
@@ -73,7 +166,7 @@ struct DevArray {
                 "hipMemcpyAsync H2D");
   }
   ~DevArray() {
-    if (ptr) (void)hipFree(ptr);
+    if (ptr) pool_release(ptr, cap * sizeof(T));
   }
 };
 
@@ -161,7 +254,8 @@ class Engine {
     for (long long i = 0; i < (long long)n * p; ++i)
       if (!std::isfinite(h_data[i])) throw KpcError(-1, "data must be finite");
     hip_check(hipStreamCreate(&stream_), "hipStreamCreate");
-    hip_check(hipMalloc((void**)&d_data_, sizeof(double) * (size_t)n * p), "hipMalloc data");
+    data_bytes_ = sizeof(double) * (size_t)n * p;
+    d_data_ = (double*)pool_acquire(data_bytes_);
     hip_check(hipMemcpy(d_data_, h_data, sizeof(double) * (size_t)n * p, hipMemcpyHostToDevice),
               "upload data");
     k_ = opt.num_col;
@@ -188,7 +282,9 @@ class Engine {
   Engine(const Engine&) = delete;
   Engine& operator=(const Engine&) = delete;
   ~Engine() {
-    if (d_data_) (void)hipFree(d_data_);
+    // arenas go back to the pool after this body: nothing may still use them
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (d_data_) pool_release(d_data_, data_bytes_);
     for (int e = 0; e < 8; ++e) (void)hipEventDestroy(ev_[e]);
     for (int e = 0; e < 3; ++e)
       if (tev_[e]) (void)hipEventDestroy(tev_[e]);
@@ -833,6 +929,7 @@ class Engine {
   kpc_options opt_;
   hipStream_t stream_ = nullptr;
   double* d_data_ = nullptr;
+  size_t data_bytes_ = 0;
   hipEvent_t ev_[8];
   kpc::MagicControl ctl_;
   int level_ = -1, q_ = 0, m_ = 0, dim_ = 0, rk_ = 0;
@@ -1468,11 +1565,11 @@ int kpc_dcov_components(const double* x, int n, int count, int k, double* values
       throw KpcError(-1, "invalid arguments");
     for (long long i = 0; i < (long long)n * count; ++i)
       if (!std::isfinite(x[i])) throw KpcError(-1, "data must be finite");
-    hipStream_t s;
-    hip_check(hipStreamCreate(&s), "stream");
     DevArray<double> d_x, d_vals, d_vecs, d_dsum, d_mom, d_scr;
     DevArray<long long> d_src;
-    DevArray<int> d_dst;
+    DevArray<int> d_dst, d_st;
+    StreamGuard guard;
+    hipStream_t s = guard.s;
     std::vector<double> hx(x, x + (size_t)n * count);
     d_x.upload(hx, s);
     std::vector<long long> src(count);
@@ -1484,7 +1581,6 @@ int kpc_dcov_components(const double* x, int n, int count, int k, double* values
     d_vecs.reserve((size_t)count * n * k);
     d_dsum.reserve(count);
     d_mom.reserve(count);
-    DevArray<int> d_st;
     d_st.reserve(count);
     if (std::getenv("KPC_DCOV_DENSE") != nullptr) {
       const long long stride = kpc::dcov_component_scratch_doubles(n, k);
@@ -1508,7 +1604,6 @@ int kpc_dcov_components(const double* x, int n, int count, int k, double* values
     if (vectors) hip_check(hipMemcpy(vectors, d_vecs.ptr, sizeof(double) * (size_t)count * n * k, hipMemcpyDeviceToHost), "D2H");
     if (dsum) hip_check(hipMemcpy(dsum, d_dsum.ptr, sizeof(double) * count, hipMemcpyDeviceToHost), "D2H");
     if (moment) hip_check(hipMemcpy(moment, d_mom.ptr, sizeof(double) * count, hipMemcpyDeviceToHost), "D2H");
-    (void)hipStreamDestroy(s);
     return 0;
   } catch (const std::exception& e) {
     return fill_error(e);
@@ -1532,17 +1627,16 @@ int kpc_permutation_table(int n, int replicates, int count, const unsigned long 
     if (!seeds || !out || n < 2 || n > kpc::kMaxN || replicates < 1 || count < 1)
       throw KpcError(-1, "invalid arguments");
     const size_t total = (size_t)count * replicates * n;
-    hipStream_t s;
-    hip_check(hipStreamCreate(&s), "stream");
     DevArray<unsigned long long> d_seeds;
     DevArray<int> d_out;
+    StreamGuard guard;
+    hipStream_t s = guard.s;
     d_seeds.upload(std::vector<unsigned long long>(seeds, seeds + count), s);
     d_out.reserve(total);
     hip_check(kpc::launch_permutation_table(d_seeds.ptr, count, n, replicates, d_out.ptr, s),
               "permutation launch");
     hip_check(hipStreamSynchronize(s), "permutation synchronize");
     hip_check(hipMemcpy(out, d_out.ptr, sizeof(int) * total, hipMemcpyDeviceToHost), "D2H");
-    (void)hipStreamDestroy(s);
     return 0;
   } catch (const std::exception& e) {
     return fill_error(e);
@@ -1550,6 +1644,15 @@ int kpc_permutation_table(int n, int replicates, int count, const unsigned long 
 }
 
 const char* kpc_last_error(void) { return g_last_error.c_str(); }
+
+int kpc_empty_cache(void) {
+  try {
+    pool_empty();
+    return 0;
+  } catch (const std::exception& e) {
+    return fill_error(e);
+  }
+}
 
 int kpc_device_count(void) {
   int c = 0;

@@ -168,6 +168,11 @@ int kpc_task_seed(unsigned long long user_seed, int ci_method, int x, int y, con
 int kpc_permutation_table(int n, int replicates, int count, const unsigned long long* seeds,
                           int* out);
 
+/* Frees the device blocks the library keeps cached between calls (its arenas
+ * are reused by later calls of the same thread instead of being freed).
+ * Like torch.cuda.empty_cache; no counterpart in the reference. */
+int kpc_empty_cache(void);
+
 const char* kpc_last_error(void);
 int kpc_device_count(void);
 
