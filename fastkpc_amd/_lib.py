@@ -68,6 +68,7 @@ class KpcResult(ctypes.Structure):
         ("stage_launches", ctypes.c_longlong * 8),
         ("task_margin", _D),
         ("level_stage_seconds", (ctypes.c_double * 5) * 64),
+        ("task_keys", ctypes.c_void_p),
     ]
 
 

@@ -89,17 +89,18 @@ def fastkpc_compatible_cuda_skeleton(data, alpha, labels=None, options=None):
         pmax = np.ctypeslib.as_array(r.pmax, (p * p,)).reshape(p, p).copy()
         slen = np.ctypeslib.as_array(r.sepset_len, (p * p,)).reshape(p, p).copy()
         sep = np.ctypeslib.as_array(r.sepset, (p * p * 64,)).reshape(p, p, 64)
-        sepsets = [[None if slen[i, j] < 0 else [int(v) + 1 for v in sep[i, j, :slen[i, j]]]
-                    for j in range(p)] for i in range(p)]
+        smax = max(int(slen.max()), 0)
+        sp1 = (sep[:, :, :smax] + 1).tolist()
+        sepsets = [[None if k < 0 else s[:k] for s, k in zip(srow, krow)]
+                   for srow, krow in zip(sp1, slen.tolist())]
         L = r.n_tasks
         if L > 0:
-            tS = np.ctypeslib.as_array(r.task_S, (L * 64,)).reshape(L, 64).copy()
             ts = np.ctypeslib.as_array(r.task_slen, (L,)).copy()
             tasks = {
                 "level": np.ctypeslib.as_array(r.task_level, (L,)).copy(),
                 "x": np.ctypeslib.as_array(r.task_x, (L,)).copy() + 1,
                 "y": np.ctypeslib.as_array(r.task_y, (L,)).copy() + 1,
-                "S_key": ["|".join(str(v + 1) for v in tS[i, :ts[i]]) for i in range(L)],
+                "S_key": _lib.ctypes.string_at(r.task_keys).decode().split("\n"),
                 "conditioning_size": ts,
                 "p_used": np.ctypeslib.as_array(r.task_p, (L,)).copy(),
                 "native_edge_deleted": np.ctypeslib.as_array(r.task_deleted, (L,)).astype(bool).copy(),

@@ -513,7 +513,8 @@ class Engine {
     }
     for (int s = 0; s < 5; ++s)
       if (launched[s]) ++launches_[s];
-    check_device_status(fit_items, geo_items, nsm);
+    std::vector<double> rinfo_h;
+    check_device_status(fit_items, geo_items, nsm, &rinfo_h);
     check_component_status(comp_dst);
     std::vector<double> cmargin_h;
     if (method_ == KPC_HSIC_GAMMA || method_ == KPC_HSIC_PERM) {
@@ -521,13 +522,6 @@ class Engine {
       hip_check(hipMemcpy(cmargin_h.data(), comp_margin_.ptr, sizeof(double) * cmargin_h.size(),
                           hipMemcpyDeviceToHost),
                 "component margins");
-    }
-    std::vector<double> rinfo_h;
-    if (level_ > 0 && n_resid_ > 0) {
-      rinfo_h.resize((size_t)n_resid_ * kpc::kMagicInfo);
-      hip_check(hipMemcpy(rinfo_h.data(), rinfo_.ptr, sizeof(double) * rinfo_h.size(),
-                          hipMemcpyDeviceToHost),
-                "fit margins");
     }
     for (size_t i = 0; i < tasks.size(); ++i) {
       const double* o = out.data() + kpc::kPairOut * i;
@@ -871,8 +865,12 @@ class Engine {
     return ds;
   }
 
+  // Status words of a wave's smooth setups, designs and fits, each read in
+  // one copy; the fit info of every residual slot of the level is returned
+  // through rinfo_out when given.
   void check_device_status(const std::vector<kpc::MagicItem>& fits,
-                           const std::vector<kpc::GeometryItem>& geos, int nsm) {
+                           const std::vector<kpc::GeometryItem>& geos, int nsm,
+                           std::vector<double>* rinfo_out = nullptr) {
     if (nsm > 0) {
       const bool uni = level_ > 2;
       DevArray<int>& info = uni ? usmooth_info_ : lsmooth_info_;
@@ -886,14 +884,25 @@ class Engine {
           throw KpcError(-5, "thin-plate setup failed (fewer unique locations than basis size)");
       }
     }
-    for (const auto& g : geos) {
-      double st = 0.0;
-      hip_check(hipMemcpy(&st, design_.ptr + (size_t)g.design_slot * layout_.stride + layout_.status,
-                          sizeof(double), hipMemcpyDeviceToHost),
+    if (!geos.empty()) {
+      // one strided copy of the status word of every design slot in range
+      int lo = geos[0].design_slot, hi = lo;
+      for (const auto& g : geos) {
+        lo = std::min(lo, g.design_slot);
+        hi = std::max(hi, g.design_slot);
+      }
+      std::vector<double> st((size_t)(hi - lo + 1));
+      hip_check(hipMemcpy2D(st.data(), sizeof(double),
+                            design_.ptr + (size_t)lo * layout_.stride + layout_.status,
+                            sizeof(double) * (size_t)layout_.stride, sizeof(double), st.size(),
+                            hipMemcpyDeviceToHost),
                 "design status");
-      if (st != 0.0) throw KpcError(-5, "design geometry failed (penalty rank or initial.sp)");
+      for (const auto& g : geos)
+        if (st[(size_t)(g.design_slot - lo)] != 0.0)
+          throw KpcError(-5, "design geometry failed (penalty rank or initial.sp)");
     }
-    if (!fits.empty()) {
+    const bool want_info = rinfo_out && level_ > 0 && n_resid_ > 0;
+    if (!fits.empty() || want_info) {
       std::vector<double> h((size_t)n_resid_ * kpc::kMagicInfo);
       hip_check(hipMemcpy(h.data(), rinfo_.ptr, sizeof(double) * h.size(), hipMemcpyDeviceToHost),
                 "fit info");
@@ -902,6 +911,7 @@ class Engine {
         if (o[0] != 0.0) throw KpcError(-6, "GCV optimiser did not converge");
         stats_.evals += (long long)o[5];
       }
+      if (want_info) rinfo_out->swap(h);
     }
   }
 
@@ -1237,6 +1247,21 @@ void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
     ++ord;
   }
   r->levels = ord;
+  {
+    std::string keys;
+    keys.reserve((size_t)r->n_tasks * 8 + 1);
+    char num[16];
+    for (int i = 0; i < r->n_tasks; ++i) {
+      if (i) keys.push_back('\n');
+      for (int j = 0; j < r->task_slen[i]; ++j) {
+        if (j) keys.push_back('|');
+        const int len = std::snprintf(num, sizeof(num), "%d", r->task_S[(size_t)i * 64 + j] + 1);
+        keys.append(num, (size_t)len);
+      }
+    }
+    r->task_keys = (char*)std::malloc(keys.size() + 1);
+    std::memcpy(r->task_keys, keys.c_str(), keys.size() + 1);
+  }
   for (int i = 0; i < p - 1; ++i)
     for (int j = i + 1; j < p; ++j) {
       const double m = std::max(r->pmax[(size_t)i * p + j], r->pmax[(size_t)j * p + i]);
@@ -1308,6 +1333,7 @@ void kpc_result_free(kpc_result* r) {
   std::free(r->task_p);
   std::free(r->task_deleted);
   std::free(r->task_margin);
+  std::free(r->task_keys);
   std::free(r);
 }
 

@@ -47,7 +47,7 @@ __host__ __device__ inline DesignLayout design_layout(int n, int q, int m, int r
 DesignLayout host_design_layout(int n, int q, int m, int rk) { return design_layout(n, q, m, rk); }
 
 // R mean of a short vector (serial; long double + correction in R).
-__device__ double serial_r_mean(const double* x, int n) {
+__device__ __attribute__((always_inline)) double serial_r_mean(const double* x, int n) {
   dd s{0.0, 0.0};
   for (int i = 0; i < n; ++i) s = dd_add(s, dd{x[i], 0.0});
   dd m = dd_div_int(s, (double)n);
@@ -59,7 +59,7 @@ __device__ double serial_r_mean(const double* x, int n) {
 
 // Workgroup argmax with LAPACK MAXLOC semantics (first index among equal
 // maxima); every thread receives the winning (value, index).
-__device__ void block_argmax_first(double v, int idx, double* redv, int* redi, double* outv,
+__device__ __attribute__((always_inline)) void block_argmax_first(double v, int idx, double* redv, int* redi, double* outv,
                                    int* outi) {
   const int lane = threadIdx.x & (KPC_WAVE - 1);
   const int wid = threadIdx.x / KPC_WAVE;
@@ -88,7 +88,7 @@ __device__ void block_argmax_first(double v, int idx, double* redv, int* redi, d
 // first-maximum pivot search, the three disjoint row/column swaps and the
 // row of R.  ictl needs 8 ints, dctl 8 doubles (plus redv/redi scratch of
 // 16 entries each inside them).
-__device__ int pstf2_upper(double* A, int q, int* piv, double* work, double tol, int* ictl,
+__device__ __attribute__((always_inline)) int pstf2_upper(double* A, int q, int* piv, double* work, double tol, int* ictl,
                            double* dctl) {
   const int tid = threadIdx.x;
   double* redv = dctl + 8;
@@ -179,7 +179,7 @@ __device__ int pstf2_upper(double* A, int q, int* piv, double* work, double tol,
 // k = q-1 .. 0 every (row i < k, column c >= k) pair subtracts R_ik x_kc in
 // parallel, one workgroup barrier per k (the serial back substitution would
 // chain q^2/2 dependent FP64 operations).  Rows are scaled by 1/R_ii at the end.
-__device__ void upper_tri_inverse(const double* R, int ld, int q, double* X, double* dinv) {
+__device__ __attribute__((always_inline)) void upper_tri_inverse(const double* R, int ld, int q, double* X, double* dinv) {
   for (int e = threadIdx.x; e < q * q; e += blockDim.x) X[e] = (e % q == e / q) ? 1.0 : 0.0;
   for (int i = threadIdx.x; i < q; i += blockDim.x) dinv[i] = 1.0 / R[(size_t)i * ld + i];
   __syncthreads();
@@ -206,7 +206,7 @@ typedef double v4d_t __attribute__((ext_vector_type(4)));
 // over the waves, each accumulating v_mfma_f64_16x16x4f64 over K in steps of
 // 4 (lane (g, c) feeds A[k0 + g][i0 + c] and B[k0 + g][j0 + c]; entries past
 // K, M or N read as zero).  No barrier inside: the caller orders phases.
-__device__ void wg_gemm_tn(const double* A, int lda, const double* B, int ldb, double* C, int ldc,
+__device__ __attribute__((always_inline)) void wg_gemm_tn(const double* A, int lda, const double* B, int ldb, double* C, int ldc,
                            int M, int N, int K) {
   const int lane = threadIdx.x & (KPC_WAVE - 1);
   const int wid = threadIdx.x / KPC_WAVE;
@@ -522,8 +522,9 @@ __host__ __device__ inline long long magic_deriv_doubles(int q, int m) {
   return 3LL * m * q + 4LL * m * m + 3LL * m + 8;
 }
 __host__ __device__ inline long long magic_small_doubles(int q, int m) {
-  // d, y1, wk (4q), 7 m-vectors + hessian, red, dctl, spv (2m), ord/piv/ictl
-  return 4LL * q + 7LL * m + (long long)m * m + 16 + 32 + 2LL * m + q + 24;
+  // d, y1, wk (4q), 7 m-vectors + hessian, red, dctl, spv (2m), design norms
+  // (2 + 3m), ord/piv/ictl
+  return 4LL * q + 7LL * m + (long long)m * m + 16 + 32 + 2LL * m + (2 + 3LL * m) + q + 24;
 }
 __host__ __device__ inline bool magic_deriv_in_lds(int q, int m) {
   return magic_deriv_doubles(q, m) * 8 <= 48 * 1024;
@@ -560,6 +561,7 @@ struct MagicLds {
   int* piv;       // q
   double* wk;     // 2q
   double* spv;    // 2m: sp_i = exp(log sp_i), then sqrt(sp_i) at m + i
+  double* nrm;    // 2 + 3m: the design's norms block, read once per fit
   double* My;     // m x nr derivative work
   double* Ky;     // m x nr
   double* yK;     // m x nr
@@ -578,8 +580,8 @@ __host__ __device__ inline size_t magic_lds_doubles(int n, int q, int m, bool au
   return s;
 }
 
-__device__ MagicLds carve_magic(double* base, int n, int q, int m, bool aug_in_lds,
-                                double* gscratch) {
+__device__ __attribute__((always_inline)) MagicLds carve_magic(double* base, int n, int q, int m, bool aug_in_lds,
+                                double* gscratch, bool deriv_in_lds) {
   MagicLds L;
   double* p = base;
   // response vectors live in the global slab after the working set
@@ -607,7 +609,8 @@ __device__ MagicLds carve_magic(double* base, int n, int q, int m, bool aug_in_l
   L.dctl = p; p += 32;
   L.wk = p; p += 2 * q;
   L.spv = p; p += 2 * m;
-  double* dw = magic_deriv_in_lds(q, m) ? p : gscratch + magic_big_doubles(q);
+  L.nrm = p; p += 2 + 3 * m;
+  double* dw = deriv_in_lds ? p : gscratch + magic_big_doubles(q);
   L.My = dw; dw += (size_t)m * q;
   L.Ky = dw; dw += (size_t)m * q;
   L.yK = dw; dw += (size_t)m * q;
@@ -617,7 +620,7 @@ __device__ MagicLds carve_magic(double* base, int n, int q, int m, bool aug_in_l
   L.syA = dw; dw += (size_t)m * m;
   L.syZ = dw; dw += (size_t)m * m;
   L.syw = dw; dw += m + 8;
-  if (magic_deriv_in_lds(q, m)) p = dw;
+  if (deriv_in_lds) p = dw;
   int* ip = reinterpret_cast<int*>(p);
   L.ord = ip; ip += q;
   L.piv = ip; ip += q;
@@ -630,7 +633,7 @@ struct MagicScratch {
   double* beta;   // q: coefficients in the original column order
 };
 
-__device__ MagicScratch carve_magic_scratch(double* base, int q, int m, int rk) {
+__device__ __attribute__((always_inline)) MagicScratch carve_magic_scratch(double* base, int q, int m, int rk) {
   (void)rk;
   MagicScratch S;
   double* p = base + magic_big_doubles(q) + magic_deriv_doubles(q, m);
@@ -648,7 +651,7 @@ long long magic_scratch_doubles(int q, int m, int rk, int n) {
 
 // One-sided Jacobi SVD of the rows x q matrix A (column-major, ld rows) with
 // round-robin parallel ordering; V accumulates the rotations.
-__device__ void jacobi_svd(double* A, int rows, int q, double* V, double* d, int* ord,
+__device__ __attribute__((always_inline)) void jacobi_svd(double* A, int rows, int q, double* V, double* d, int* ord,
                            int* ictl) {
   const int tid = threadIdx.x;
   for (int e = tid; e < q * q; e += blockDim.x) V[e] = 0.0;
@@ -836,7 +839,7 @@ __device__ __forceinline__ void group_reflector(double* W, int rows, int q, int 
 // updates column j + 1 builds its reflector right away, so each step costs
 // one workgroup barrier.  R2 overwrites the top block, reflector tails the E
 // rows, tau[0..q).
-__device__ void stacked_qr(double* W, int rows, int q, int mr, double* tau) {
+__device__ __attribute__((always_inline)) void stacked_qr(double* W, int rows, int q, int mr, double* tau) {
   const int tid = threadIdx.x;
   constexpr int kG = 8;
   const int ncg = blockDim.x / kG;
@@ -867,7 +870,7 @@ __device__ void stacked_qr(double* W, int rows, int q, int mr, double* tau) {
 
 // Q1 (top q rows of the orthogonal factor of [R; E]) from the reflectors of
 // stacked_qr: Qm = H_0 ... H_{q-1} [I; 0] (LAPACK dorg2r order).
-__device__ void stacked_q1(const double* W, int rows, int q, int mr, const double* tau,
+__device__ __attribute__((always_inline)) void stacked_q1(const double* W, int rows, int q, int mr, const double* tau,
                            double* Qm) {
   const int tid = threadIdx.x;
   constexpr int kG = 8;
@@ -898,7 +901,7 @@ __device__ void stacked_q1(const double* W, int rows, int q, int mr, const doubl
 }
 
 // Total penalty sum_i sp_i S_i = sum_i sp_i rS_i rS_i' (q x q, column-major).
-__device__ void aggregate_penalty(const double* D, const DesignLayout& DL, int q, int m, int rk,
+__device__ __attribute__((always_inline)) void aggregate_penalty(const double* D, const DesignLayout& DL, int q, int m, int rk,
                                   const double* spv, double* agg) {
   for (int e = threadIdx.x; e < q * q; e += blockDim.x) {
     const int r = e % q, c = e / q;
@@ -915,7 +918,7 @@ __device__ void aggregate_penalty(const double* D, const DesignLayout& DL, int q
 }
 
 // Copy [R; root] into the augmented buffer (rows = q + rank).
-__device__ void build_augmented(const MagicLds& L, const double* D, const DesignLayout& DL, int q,
+__device__ __attribute__((always_inline)) void build_augmented(const MagicLds& L, const double* D, const DesignLayout& DL, int q,
                                 int rank, const double* root) {
   const int tid = threadIdx.x;
   const int rows = q + rank;
@@ -942,7 +945,7 @@ __device__ void build_augmented(const MagicLds& L, const double* D, const Design
 // U) and V D^-1 is invariant under the orthogonal change of basis U1 = Q1 W,
 // V D^-1 = R2^-1 W, so the structured QR (q reflectors of length 1 + rank)
 // replaces the Jacobi SVD; otherwise the SVD decides the rank exactly.
-__device__ void magic_eval(const MagicLds& L, const MagicScratch& S, const double* D,
+__device__ __attribute__((always_inline)) void magic_eval(const MagicLds& L, const MagicScratch& S, const double* D,
                            const DesignLayout& DL, int n, int q, int m, int rk,
                            const double* lspv, double ynorm2, double rank_tol) {
   const int tid = threadIdx.x;
@@ -965,10 +968,10 @@ __device__ void magic_eval(const MagicLds& L, const MagicScratch& S, const doubl
   // matrix as [R; root], i.e. the same singular values and U1, V, D.
   const int mr = m * rk;
   double dmax = 0.0;
-  for (int i = 0; i < m; ++i) dmax = fmax(dmax, spv[i] * D[DL.norms + 2 + m + i]);
+  for (int i = 0; i < m; ++i) dmax = fmax(dmax, spv[i] * L.nrm[2 + m + i]);
   const double dstop = (double)q * kEps * dmax;
   bool stacked = true;
-  for (int i = 0; i < m; ++i) stacked = stacked && spv[i] * D[DL.norms + 2 + 2 * m + i] > 1e3 * dstop;
+  for (int i = 0; i < m; ++i) stacked = stacked && spv[i] * L.nrm[2 + 2 * m + i] > 1e3 * dstop;
   long long tp0 = wall_clock64(), tp1 = tp0, tp2 = tp0;
   int rows, erows = mr;
   if (tid == 0) L.ictl[15] = 0;
@@ -977,9 +980,9 @@ __device__ void magic_eval(const MagicLds& L, const MagicScratch& S, const doubl
   if (stacked) {
     // full-rank certificate, stage 1 from the design norms: sigma_min([R; E])
     // >= sigma_min(R) >= 1 / |R^-1|_F, sigma_max^2 <= |R|_F^2 + sum_i sp_i |rS_i|_F^2
-    double emax = D[DL.norms];
-    for (int i = 0; i < m; ++i) emax += spv[i] * D[DL.norms + 2 + i];
-    lb = 1.0 / sqrt(D[DL.norms + 1] * emax);
+    double emax = L.nrm[0];
+    for (int i = 0; i < m; ++i) emax += spv[i] * L.nrm[2 + i];
+    lb = 1.0 / sqrt(L.nrm[1] * emax);
     qr_ok = isfinite(lb) && lb > 2.0 * rank_tol;
     rows = q + mr;
     for (int e = tid; e < rows * q; e += blockDim.x) {
@@ -1131,7 +1134,7 @@ __device__ void magic_eval(const MagicLds& L, const MagicScratch& S, const doubl
 }
 
 // Gradient and Hessian of the GCV score at the point last evaluated.
-__device__ void magic_derivs(const MagicLds& L, const MagicScratch& S, const double* D,
+__device__ __attribute__((always_inline)) void magic_derivs(const MagicLds& L, const MagicScratch& S, const double* D,
                              const DesignLayout& DL, int n, int q, int m, int rk,
                              const double* lspv) {
   const int tid = threadIdx.x;
@@ -1333,7 +1336,7 @@ __device__ __forceinline__ bool bits_differ(double x, double y) {
   return __double_as_longlong(x) != __double_as_longlong(y);
 }
 
-__device__ void small_syev_wave(const double* Hin, int m, double* w, double* Z, double* A) {
+__device__ __attribute__((always_inline)) void small_syev_wave(const double* Hin, int m, double* w, double* Z, double* A) {
   const int lane = threadIdx.x & (KPC_WAVE - 1);
   for (int i = lane; i < m * m; i += KPC_WAVE) {
     A[i] = Hin[i];
@@ -1391,6 +1394,11 @@ __device__ void small_syev_wave(const double* Hin, int m, double* w, double* Z, 
   wave_sync();
 }
 
+// kLds: the whole working set (big arrays and derivative work) is in LDS.
+// Every LDS pointer then derives from the dynamic shared array on every path,
+// so the compiler addresses it as LDS (ds_read / ds_write with immediate
+// offsets) instead of through generic flat pointers.
+template <bool kLds>
 __global__ __launch_bounds__(kMagicThreads) void magic_kernel(
     const MagicItem* __restrict__ items, int count, int n, int q, int m, int rk,
     const double* __restrict__ data, const double* __restrict__ design,
@@ -1399,7 +1407,8 @@ __global__ __launch_bounds__(kMagicThreads) void magic_kernel(
   extern __shared__ double lds_dyn[];
   const int tid = threadIdx.x;
   double* gs = scratch + (size_t)blockIdx.x * scratch_stride;
-  MagicLds L = carve_magic(lds_dyn, n, q, m, aug_in_lds != 0, gs);
+  MagicLds L = carve_magic(lds_dyn, n, q, m, kLds || aug_in_lds != 0, gs,
+                           kLds || magic_deriv_in_lds(q, m));
   MagicScratch S = carve_magic_scratch(gs, q, m, rk);
   const DesignLayout DL = design_layout(n, q, m, rk);
   for (int item = blockIdx.x; item < count; item += gridDim.x) {
@@ -1431,6 +1440,7 @@ __global__ __launch_bounds__(kMagicThreads) void magic_kernel(
     for (int r = tid; r < n; r += blockDim.x) part += L.yorig[r] * L.yorig[r];
     const double ynorm2 = block_sum(part, L.red);
     if (tid < m) L.lsp[tid] = D[DL.lsp0 + tid];
+    for (int i = tid; i < 2 + 3 * m; i += blockDim.x) L.nrm[i] = D[DL.norms + i];
     if (tid == 0) L.dctl[22] = (double)(wall_clock64() - t_start);  // Q'y prologue
     __syncthreads();
     const int max_half = m == 1 ? ctl.max_half_single : ctl.max_half_multi;
@@ -1665,12 +1675,18 @@ hipError_t launch_magic(const MagicItem* items, int count, int n, int q, int m, 
   if (count <= 0) return hipSuccess;
   const bool in_lds = magic_aug_fits_lds(n, q, m);
   const size_t lds = magic_lds_bytes(n, q, m, in_lds);
-  hipError_t err = hipFuncSetAttribute((const void*)magic_kernel,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  const bool all_lds = in_lds && magic_deriv_in_lds(q, m);
+  const void* fn = all_lds ? (const void*)magic_kernel<true> : (const void*)magic_kernel<false>;
+  hipError_t err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (err != hipSuccess) return err;
-  hipLaunchKernelGGL(magic_kernel, dim3(grid), dim3(magic_threads(q)), lds, stream, items, count, n,
-                     q, m, rk, data, design, resid, info, scratch, magic_scratch_doubles(q, m, rk, n),
-                     in_lds ? 1 : 0, ctl);
+  if (all_lds)
+    hipLaunchKernelGGL(magic_kernel<true>, dim3(grid), dim3(magic_threads(q)), lds, stream, items,
+                       count, n, q, m, rk, data, design, resid, info, scratch,
+                       magic_scratch_doubles(q, m, rk, n), 1, ctl);
+  else
+    hipLaunchKernelGGL(magic_kernel<false>, dim3(grid), dim3(magic_threads(q)), lds, stream, items,
+                       count, n, q, m, rk, data, design, resid, info, scratch,
+                       magic_scratch_doubles(q, m, rk, n), in_lds ? 1 : 0, ctl);
   return hipGetLastError();
 }
 
@@ -1681,6 +1697,7 @@ namespace kpc {
 // GCV score, gradient and Hessian at caller-supplied log(sp): the hand-written
 // derivative of the magic objective, exposed on its own for testing against
 // central differences and for external Newton drivers.
+template <bool kLds>
 __global__ __launch_bounds__(kMagicThreads) void gcv_eval_kernel(
     const MagicItem* __restrict__ items, int count, int n, int q, int m, int rk,
     const double* __restrict__ data, const double* __restrict__ design,
@@ -1689,7 +1706,8 @@ __global__ __launch_bounds__(kMagicThreads) void gcv_eval_kernel(
   extern __shared__ double lds_dyn[];
   const int tid = threadIdx.x;
   double* gs = scratch + (size_t)blockIdx.x * scratch_stride;
-  MagicLds L = carve_magic(lds_dyn, n, q, m, aug_in_lds != 0, gs);
+  MagicLds L = carve_magic(lds_dyn, n, q, m, kLds || aug_in_lds != 0, gs,
+                           kLds || magic_deriv_in_lds(q, m));
   MagicScratch S = carve_magic_scratch(gs, q, m, rk);
   const DesignLayout DL = design_layout(n, q, m, rk);
   for (int item = blockIdx.x; item < count; item += gridDim.x) {
@@ -1711,6 +1729,7 @@ __global__ __launch_bounds__(kMagicThreads) void gcv_eval_kernel(
     for (int r = tid; r < n; r += blockDim.x) part += L.yorig[r] * L.yorig[r];
     const double ynorm2 = block_sum(part, L.red);
     if (tid < m) L.lsp[tid] = lsp_in[(size_t)item * m + tid];
+    for (int i = tid; i < 2 + 3 * m; i += blockDim.x) L.nrm[i] = D[DL.norms + i];
     __syncthreads();
     magic_eval(L, S, D, DL, n, q, m, rk, L.lsp, ynorm2, rank_tol);
     magic_derivs(L, S, D, DL, n, q, m, rk, L.lsp);
@@ -1731,12 +1750,18 @@ hipError_t launch_gcv_eval(const MagicItem* items, int count, int n, int q, int 
   if (count <= 0) return hipSuccess;
   const bool in_lds = magic_aug_fits_lds(n, q, m);
   const size_t lds = magic_lds_bytes(n, q, m, in_lds);
-  hipError_t err = hipFuncSetAttribute((const void*)gcv_eval_kernel,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  const bool all_lds = in_lds && magic_deriv_in_lds(q, m);
+  const void* fn = all_lds ? (const void*)gcv_eval_kernel<true> : (const void*)gcv_eval_kernel<false>;
+  hipError_t err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (err != hipSuccess) return err;
-  hipLaunchKernelGGL(gcv_eval_kernel, dim3(grid), dim3(magic_threads(q)), lds, stream, items, count,
-                     n, q, m, rk, data, design, lsp, out, scratch, magic_scratch_doubles(q, m, rk, n),
-                     in_lds ? 1 : 0, rank_tol);
+  if (all_lds)
+    hipLaunchKernelGGL(gcv_eval_kernel<true>, dim3(grid), dim3(magic_threads(q)), lds, stream, items,
+                       count, n, q, m, rk, data, design, lsp, out, scratch,
+                       magic_scratch_doubles(q, m, rk, n), 1, rank_tol);
+  else
+    hipLaunchKernelGGL(gcv_eval_kernel<false>, dim3(grid), dim3(magic_threads(q)), lds, stream, items,
+                       count, n, q, m, rk, data, design, lsp, out, scratch,
+                       magic_scratch_doubles(q, m, rk, n), in_lds ? 1 : 0, rank_tol);
   return hipGetLastError();
 }
 

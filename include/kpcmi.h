@@ -100,6 +100,10 @@ typedef struct kpc_result {
   double* task_margin;
   /* device seconds per level and stage (as stage_seconds[0..4]) */
   double level_stage_seconds[64][5];
+  /* per trace row, the conditioning set as 1-based indices joined by '|'
+   * (the reference trace's S_key; "" for the empty set), rows separated by
+   * '\n', NUL-terminated */
+  char* task_keys;
 } kpc_result;
 
 void kpc_default_options(kpc_options* opt);
