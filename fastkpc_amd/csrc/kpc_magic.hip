@@ -900,6 +900,262 @@ __device__ __attribute__((always_inline)) void stacked_q1(const double* W, int r
   }
 }
 
+// ---- register-resident forms (q, mr <= 8 K): the same arithmetic, in the
+// same order per column, as stacked_qr / stacked_q1 / upper_tri_inverse, with
+// each column held by one aligned group of 8 lanes (lane sub holds rows
+// sub, sub + 8, ...).  Only the exchange of data changes: no workgroup
+// barrier sits on the dependency chain, so results are bit-identical.
+
+// Structured QR by wave 0: group g holds the E rows of columns g, g + 8, ...
+// in registers; reflector j is built by the group holding column j and handed
+// to the other groups by cross-lane moves.
+template <int K>
+__device__ __attribute__((always_inline)) void stacked_qr_regs(double* W, int rows, int q, int mr,
+                                                               double* tau) {
+  const int lane = threadIdx.x & (KPC_WAVE - 1);
+  const int g = lane >> 3, sub = lane & 7;
+  double E[K][K];
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    const int c = g + 8 * i;
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+      const int k = sub + 8 * t;
+      E[i][t] = (c < q && k < mr) ? W[(size_t)c * rows + q + k] : 0.0;
+    }
+  }
+  for (int j = 0; j < q; ++j) {
+    const int gj = j & 7, ij = j >> 3;
+    double u[K];
+#pragma unroll
+    for (int t = 0; t < K; ++t) u[t] = E[0][t];
+#pragma unroll
+    for (int i = 1; i < K; ++i)
+      if (i == ij)
+#pragma unroll
+        for (int t = 0; t < K; ++t) u[t] = E[i][t];
+    double tj = 0.0;
+    if (g == gj) {
+      // dlarfg of column j over row j and the E rows (as group_reflector)
+      double s2 = 0.0;
+#pragma unroll
+      for (int t = 0; t < K; ++t)
+        if (sub + 8 * t < mr) s2 += u[t] * u[t];
+      s2 = sum8(s2);
+      const double alpha = W[(size_t)j * rows + j];
+      double scal = 0.0, beta = alpha;
+      if (s2 > 0.0) {
+        const double h = fma(alpha, alpha, s2);
+        const double r = nrsq(h);
+        const double nrm = h * r;
+        beta = -copysign(nrm, alpha);
+        tj = fma(fabs(alpha), r, 1.0);
+        scal = copysign(nrcp(fabs(alpha) + nrm), alpha);
+      }
+#pragma unroll
+      for (int t = 0; t < K; ++t)
+        if (sub + 8 * t < mr) u[t] *= scal;
+#pragma unroll
+      for (int i = 0; i < K; ++i)
+        if (i == ij)
+#pragma unroll
+          for (int t = 0; t < K; ++t) E[i][t] = u[t];
+      if (sub == 0) {
+        W[(size_t)j * rows + j] = beta;
+        tau[j] = tj;
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < K; ++t) u[t] = __shfl(u[t], gj * 8 + sub, KPC_WAVE);
+    tj = __shfl(tj, gj * 8, KPC_WAVE);
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      const int c = g + 8 * i;
+      if (c > j && c < q) {
+        double* wc = W + (size_t)c * rows;
+        double dot = 0.0;
+        if (sub == 0) dot = wc[j];
+#pragma unroll
+        for (int t = 0; t < K; ++t)
+          if (sub + 8 * t < mr) dot += u[t] * E[i][t];
+        dot = sum8(dot) * tj;
+        if (tj != 0.0) {
+          if (sub == 0) wc[j] -= dot;
+#pragma unroll
+          for (int t = 0; t < K; ++t)
+            if (sub + 8 * t < mr) E[i][t] -= dot * u[t];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    const int c = g + 8 * i;
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+      const int k = sub + 8 * t;
+      if (c < q && k < mr) W[(size_t)c * rows + q + k] = E[i][t];
+    }
+  }
+}
+
+// Q1 column by column: column c only ever combines with itself, so every
+// group runs its columns (cg, cg + ng) through reflectors q-1 .. 0 alone.
+// Needs q <= 2 ng.
+template <int K>
+__device__ __attribute__((always_inline)) void stacked_q1_regs(const double* W, int rows, int q,
+                                                               int mr, const double* tau,
+                                                               double* Qm) {
+  const int tid = threadIdx.x;
+  const int ng = (int)blockDim.x >> 3, cg = tid >> 3, sub = tid & 7;
+  for (int e = tid; e < rows * q; e += blockDim.x) Qm[e] = (e % rows == e / rows) ? 1.0 : 0.0;
+  __syncthreads();
+  double E[2][K];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int t = 0; t < K; ++t) E[i][t] = 0.0;
+  double vn[K];
+  double tn = tau[q - 1];
+#pragma unroll
+  for (int t = 0; t < K; ++t)
+    vn[t] = sub + 8 * t < mr ? W[(size_t)(q - 1) * rows + q + sub + 8 * t] : 0.0;
+  for (int j = q - 1; j >= 0; --j) {
+    double v[K];
+#pragma unroll
+    for (int t = 0; t < K; ++t) v[t] = vn[t];
+    const double tj = tn;
+    if (j > 0) {
+      tn = tau[j - 1];
+#pragma unroll
+      for (int t = 0; t < K; ++t)
+        vn[t] = sub + 8 * t < mr ? W[(size_t)(j - 1) * rows + q + sub + 8 * t] : 0.0;
+    }
+    if (tj == 0.0) continue;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int c = cg + ng * i;
+      if (c < q && c >= j) {
+        const double top = c == j ? 1.0 : 0.0;
+        double dot = 0.0;
+        if (sub == 0) dot = top;
+#pragma unroll
+        for (int t = 0; t < K; ++t)
+          if (sub + 8 * t < mr) dot += v[t] * E[i][t];
+        dot = sum8(dot) * tj;
+        if (sub == 0) Qm[(size_t)c * rows + j] = top - dot;
+#pragma unroll
+        for (int t = 0; t < K; ++t)
+          if (sub + 8 * t < mr) E[i][t] -= dot * v[t];
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int c = cg + ng * i;
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+      const int k = sub + 8 * t;
+      if (c < q && k < mr) Qm[(size_t)c * rows + q + k] = E[i][t];
+    }
+  }
+  __syncthreads();
+}
+
+// R^-1 column by column (as upper_tri_inverse: for k = q-1 .. 1 every row
+// i < k subtracts R_ik (x_k d_k), then rows scale by d_i = 1 / R_ii); steps
+// k > c leave column c unchanged.  Needs q <= 2 ng and q <= 8 K.
+template <int K>
+__device__ __attribute__((always_inline)) void upper_tri_inverse_regs(const double* R, int ld,
+                                                                      int q, double* X,
+                                                                      double* dinv) {
+  const int tid = threadIdx.x;
+  const int ng = (int)blockDim.x >> 3, cg = tid >> 3, sub = tid & 7;
+  const int base = (tid & (KPC_WAVE - 1)) & ~7;
+  for (int i = tid; i < q; i += blockDim.x) dinv[i] = 1.0 / R[(size_t)i * ld + i];
+  __syncthreads();
+  double x[2][K];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int c = cg + ng * i;
+#pragma unroll
+    for (int t = 0; t < K; ++t) x[i][t] = (sub + 8 * t == c) ? 1.0 : 0.0;
+  }
+  for (int k = q - 1; k > 0; --k) {
+    const double dk = dinv[k];
+    const int tk = k >> 3, sk = k & 7;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int c = cg + ng * i;
+      double xk = x[i][0];
+#pragma unroll
+      for (int t = 1; t < K; ++t)
+        if (t == tk) xk = x[i][t];
+      xk = __shfl(xk, base + sk, KPC_WAVE);
+      if (c < q && k <= c) {
+        const double s = xk * dk;
+#pragma unroll
+        for (int t = 0; t < K; ++t) {
+          const int r = sub + 8 * t;
+          if (r < k) x[i][t] -= R[(size_t)k * ld + r] * s;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int c = cg + ng * i;
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+      const int r = sub + 8 * t;
+      if (c < q && r < q) X[(size_t)c * q + r] = r <= c ? x[i][t] * dinv[r] : x[i][t];
+    }
+  }
+  __syncthreads();
+}
+
+// Register width for (q, mr): 0 when the LDS forms must be used.
+__device__ __forceinline__ int regs_width(int q, int mr) {
+  const int k = max((q + 7) >> 3, (mr + 7) >> 3);
+  if (2 * ((int)blockDim.x >> 3) < q) return 0;
+  return k <= 2 ? 2 : k <= 4 ? 4 : 0;
+}
+
+__device__ __attribute__((always_inline)) void stacked_qr_fast(double* W, int rows, int q, int mr,
+                                                               double* tau) {
+  switch (regs_width(q, mr)) {
+    case 2:
+      if (threadIdx.x < KPC_WAVE) stacked_qr_regs<2>(W, rows, q, mr, tau);
+      __syncthreads();
+      break;
+    case 4:
+      if (threadIdx.x < KPC_WAVE) stacked_qr_regs<4>(W, rows, q, mr, tau);
+      __syncthreads();
+      break;
+    default:
+      stacked_qr(W, rows, q, mr, tau);
+  }
+}
+
+__device__ __attribute__((always_inline)) void stacked_q1_fast(const double* W, int rows, int q,
+                                                               int mr, const double* tau,
+                                                               double* Qm) {
+  switch (regs_width(q, mr)) {
+    case 2: stacked_q1_regs<2>(W, rows, q, mr, tau, Qm); break;
+    case 4: stacked_q1_regs<4>(W, rows, q, mr, tau, Qm); break;
+    default: stacked_q1(W, rows, q, mr, tau, Qm);
+  }
+}
+
+__device__ __attribute__((always_inline)) void upper_tri_inverse_fast(const double* R, int ld, int q,
+                                                                      double* X, double* dinv) {
+  switch (regs_width(q, 0)) {
+    case 2: upper_tri_inverse_regs<2>(R, ld, q, X, dinv); break;
+    case 4: upper_tri_inverse_regs<4>(R, ld, q, X, dinv); break;
+    default: upper_tri_inverse(R, ld, q, X, dinv);
+  }
+}
+
 // Total penalty sum_i sp_i S_i = sum_i sp_i rS_i rS_i' (q x q, column-major).
 __device__ __attribute__((always_inline)) void aggregate_penalty(const double* D, const DesignLayout& DL, int q, int m, int rk,
                                   const double* spv, double* agg) {
@@ -1006,11 +1262,11 @@ __device__ __attribute__((always_inline)) void magic_eval(const MagicLds& L, con
     build_augmented(L, D, DL, q, erows, agg);
   }
   const long long tq0 = wall_clock64();
-  stacked_qr(L.aug, rows, q, erows, L.wk);
+  stacked_qr_fast(L.aug, rows, q, erows, L.wk);
   const long long tq1 = wall_clock64();
   if (!qr_ok) {
     // stage 2 from the factor itself: 1 / (|R2|_F |R2^-1|_F)
-    upper_tri_inverse(L.aug, rows, q, L.V, L.wk + q);
+    upper_tri_inverse_fast(L.aug, rows, q, L.V, L.wk + q);
     double pr = 0.0, pi = 0.0;
     for (int e = tid; e < q * q; e += blockDim.x) {
       const int r = e % q, c = e / q;
@@ -1027,7 +1283,7 @@ __device__ __attribute__((always_inline)) void magic_eval(const MagicLds& L, con
     if (qr_ok && tid == 0) L.ictl[15] = 1;  // R2 inverted already
   }
   const long long tq2 = wall_clock64();
-  if (qr_ok) stacked_q1(L.aug, rows, q, erows, L.wk, L.Qm);
+  if (qr_ok) stacked_q1_fast(L.aug, rows, q, erows, L.wk, L.Qm);
   const long long tq3 = wall_clock64();
   if (tid == 0) {
     L.dctl[8] += (double)(tq0 - te0);   // penalty scaling + augmented build
@@ -1153,7 +1409,7 @@ __device__ __attribute__((always_inline)) void magic_derivs(const MagicLds& L, c
   if (qr) {
     wg_gemm_tn(L.Qm, rows, L.Qm, rows, G, nr, nr, nr, q);
     __syncthreads();
-    if (!L.ictl[15]) upper_tri_inverse(L.aug, rows, q, L.V, L.wk);  // before T overwrites R2
+    if (!L.ictl[15]) upper_tri_inverse_fast(L.aug, rows, q, L.V, L.wk);  // before T overwrites R2
     // T = R2^-T rS (R2^-1 is upper triangular with exact zeros below)
     wg_gemm_tn(L.V, q, D + DL.rS, q, T, nr, nr, mr, q);
   } else {
@@ -1586,7 +1842,7 @@ __global__ __launch_bounds__(kMagicThreads) void magic_kernel(
     {
       const int nr = L.ictl[0];
       const bool qr = L.ictl[13] != 0;
-      if (qr && !L.ictl[15]) upper_tri_inverse(L.aug, q + L.ictl[1], q, L.V, L.wk);
+      if (qr && !L.ictl[15]) upper_tri_inverse_fast(L.aug, q + L.ictl[1], q, L.V, L.wk);
       double* theta = S.theta;
       for (int r = tid; r < q; r += blockDim.x) {
         double v = 0.0;
