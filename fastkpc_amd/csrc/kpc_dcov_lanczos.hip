@@ -4,8 +4,10 @@
 // of D_ij = |r_i - r_j| selected with the higher-index tie break, centred
 // vectors, sum(D) and the self moment; kpcalg/R/dcovgamma.R:67-79,
 // fastkpc/src/legacy_dcov_gamma_cpp.cpp:60-78, :135, :283-300), but D is
-// never stored: every Lanczos step recomputes D v from the residual held in
-// LDS (n^2 FMAs, no memory traffic).  Full reorthogonalisation (two
+// never stored: the residual is sorted once, and on sorted values
+// (D v)_i = x_i (2 P_i - sum v) - 2 X_i + sum x v with P, X the exclusive
+// prefix sums of v and x v, so a Lanczos step costs one block scan (O(n))
+// instead of n^2 products.  Full reorthogonalisation (two
 // classical Gram-Schmidt passes) keeps the Krylov basis orthogonal, and the
 // iteration stops when every selected Ritz pair has residual
 // |beta_m s_m| <= tol * ||T||; the extreme eigenvalues of the Lanczos
@@ -59,7 +61,8 @@ __device__ void lz_select(const double* cval, int ncand, int nlow, int hs, int k
 // sst): a thread takes one row and eight vectors, so every basis entry is
 // loaded once per group; the coefficient sums run over the basis in order.
 __device__ inline void ritz_assemble(const double* __restrict__ Q, const double* Sv, int sst,
-                                     int n, int m, int k, double* __restrict__ Z) {
+                                     int n, int m, int k, const int* perm,
+                                     double* __restrict__ Z) {
   const int ng = (k + 7) / 8;
   for (int u = threadIdx.x; u < n * ng; u += blockDim.x) {
     const int i = u % n, g = u / n;
@@ -76,7 +79,7 @@ __device__ inline void ritz_assemble(const double* __restrict__ Q, const double*
     }
 #pragma unroll
     for (int c = 0; c < 8; ++c)
-      if (c < ne) Z[(size_t)(e0 + c) * n + i] = acc[c];
+      if (c < ne) Z[(size_t)(e0 + c) * n + perm[i]] = acc[c];
   }
 }
 
@@ -107,7 +110,7 @@ __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
   double* chi = clo + 2 * kMaxNumCol;                     // 2 kMaxNumCol
   double* ivl = chi + 2 * kMaxNumCol;                     // inverse-iteration workspace
   const int ivw = lanczos_ivw(mmax);
-  double* z2 = ivl + ivw;                                 // nl: second-half matvec partials
+  int* perm = reinterpret_cast<int*>(ivl + ivw);          // n: sorted position -> row
   const int tid = threadIdx.x;
   const int lane = tid & (KPC_WAVE - 1);
   const int wid = tid / KPC_WAVE;
@@ -129,6 +132,43 @@ __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
     }
     acc = block_sum_dd(acc, red);
     const double dsum = acc.hi + acc.lo;
+    // Sort the residual once (bitonic, ties by row index): every Lanczos
+    // vector lives in ascending-residual order, where D v is a prefix-sum
+    // expression, and the Ritz vectors are permuted back on output.  The
+    // sorted values are shifted by their median so the prefix form works on
+    // centred values (D is shift invariant).
+    {
+      int P = 1;
+      while (P < n) P <<= 1;
+      double* key = ivl;      // P keys, then P row indices (ivw >= 2 kMaxN)
+      double* kid = ivl + P;
+      for (int i = tid; i < P; i += blockDim.x) {
+        key[i] = i < n ? rq[i].x : __builtin_inf();
+        kid[i] = (double)i;
+      }
+      __syncthreads();
+      for (int kk = 2; kk <= P; kk <<= 1)
+        for (int jj = kk >> 1; jj > 0; jj >>= 1) {
+          for (int t = tid; t < P / 2; t += blockDim.x) {
+            const int i = 2 * jj * (t / jj) + (t % jj), l = i + jj;
+            const double a = key[i], b = key[l], ia = kid[i], ib = kid[l];
+            const bool gt = a > b || (a == b && ia > ib);
+            if (((i & kk) == 0) == gt) {
+              key[i] = b;
+              key[l] = a;
+              kid[i] = ib;
+              kid[l] = ia;
+            }
+          }
+          __syncthreads();
+        }
+      const double mid = key[n / 2];
+      for (int i = tid; i < n; i += blockDim.x) {
+        perm[i] = (int)kid[i];
+        rq[i].x = key[i] - mid;
+      }
+      __syncthreads();
+    }
     // start vector: mgcv-style LCG values minus 0.5, normalised
     if (tid == 0) {
       unsigned long st = 1;
@@ -145,7 +185,7 @@ __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
     const double inv0 = red[8];
     __syncthreads();
     for (int i = tid; i < n; i += blockDim.x) {
-      const double v = z[i] * inv0;
+      const double v = z[perm[i]] * inv0;
       rq[i].y = v;
       Q[i] = v;
     }
@@ -160,48 +200,58 @@ __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
     if (tid == 0) { ictl[1] = 0; ictl[2] = 2; }
     __syncthreads();
     for (int j = 0; j < mmax; ++j) {
-      // z = D q_j recomputed from the residual.  A thread owns two rows and
-      // one half of the columns, so every broadcast LDS read of (x_l, q_l)
-      // feeds two rows (the read traffic, not the arithmetic, bounds this
-      // loop); the halves meet through z and z2.
+      // z = D q_j on the sorted residual: with exclusive prefix sums
+      // P_i = sum_{l<i} q_l and X_i = sum_{l<i} x_l q_l and totals Tq, Tx,
+      // (D q)_i = x_i (2 P_i - Tq) - 2 X_i + Tx.  A thread scans a run of
+      // consecutive rows; waves scan by lane shuffles, the wave totals meet
+      // in LDS.
       {
-        const int npair = (n + 1) >> 1;
-        const int half_w = (npair + KPC_WAVE - 1) / KPC_WAVE * KPC_WAVE;  // items per half
-        for (int t = tid; t < 2 * half_w; t += blockDim.x) {
-          const int h = t / half_w, pr = t - h * half_w;
-          if (pr >= npair) continue;
-          const int i0 = 2 * pr, i1 = min(2 * pr + 1, n - 1);
-          const double r0 = rq[i0].x, r1 = rq[i1].x;
-          const int l0 = h ? (n >> 1) : 0, l1 = h ? n : (n >> 1);
-          double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0;
-          int l = l0;
-          for (; l + 4 <= l1; l += 4) {
-            const double2 v0 = rq[l], v1 = rq[l + 1], v2 = rq[l + 2], v3 = rq[l + 3];
-            a0 += fabs(r0 - v0.x) * v0.y;
-            b0 += fabs(r1 - v0.x) * v0.y;
-            a1 += fabs(r0 - v1.x) * v1.y;
-            b1 += fabs(r1 - v1.x) * v1.y;
-            a2 += fabs(r0 - v2.x) * v2.y;
-            b2 += fabs(r1 - v2.x) * v2.y;
-            a3 += fabs(r0 - v3.x) * v3.y;
-            b3 += fabs(r1 - v3.x) * v3.y;
+        const int per = (n + (int)blockDim.x - 1) / (int)blockDim.x;
+        const int i0 = min(tid * per, n), i1 = min(i0 + per, n);
+        double sv = 0.0, sx = 0.0;
+        for (int i = i0; i < i1; ++i) {
+          const double2 e = rq[i];
+          sv += e.y;
+          sx += e.x * e.y;
+        }
+        double iv = sv, ix = sx;
+#pragma unroll
+        for (int off = 1; off < KPC_WAVE; off <<= 1) {
+          const double tv = __shfl_up(iv, off, KPC_WAVE), tx = __shfl_up(ix, off, KPC_WAVE);
+          if (lane >= off) {
+            iv += tv;
+            ix += tx;
           }
-          for (; l < l1; ++l) {
-            a0 += fabs(r0 - rq[l].x) * rq[l].y;
-            b0 += fabs(r1 - rq[l].x) * rq[l].y;
+        }
+        double ev = __shfl_up(iv, 1, KPC_WAVE), ex = __shfl_up(ix, 1, KPC_WAVE);
+        if (lane == 0) ev = ex = 0.0;
+        double* scan = ivl;  // free between convergence checks
+        if (lane == KPC_WAVE - 1) {
+          scan[wid] = iv;
+          scan[nw + wid] = ix;
+        }
+        __syncthreads();
+        double bv = 0.0, bx = 0.0, tq = 0.0, tx = 0.0;
+        for (int w = 0; w < nw; ++w) {
+          const double a = scan[w], b = scan[nw + w];
+          if (w < wid) {
+            bv += a;
+            bx += b;
           }
-          double* hz = h ? z2 : z;
-          hz[i0] = (a0 + a1) + (a2 + a3);
-          if (2 * pr + 1 < n) hz[i1] = (b0 + b1) + (b2 + b3);
+          tq += a;
+          tx += b;
+        }
+        double pv = bv + ev, px = bx + ex;
+        for (int i = i0; i < i1; ++i) {
+          const double2 e = rq[i];
+          z[i] = e.x * (2.0 * pv - tq) - 2.0 * px + tx;
+          pv += e.y;
+          px += e.x * e.y;
         }
       }
       __syncthreads();
       double part = 0.0;
-      for (int i = tid; i < n; i += blockDim.x) {
-        const double zi = z[i] + z2[i];
-        z[i] = zi;
-        part += rq[i].y * zi;
-      }
+      for (int i = tid; i < n; i += blockDim.x) part += rq[i].y * z[i];
       const double aj = block_sum(part, red);
       // three-term recurrence
       const double bprev = j > 0 ? be[j - 1] : 0.0;
@@ -392,9 +442,9 @@ __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
       for (int e = tid; e < k * m_final; e += blockDim.x)
         ivl[e] = S[(size_t)(e / m_final) * mmax + e % m_final];
       __syncthreads();
-      ritz_assemble(Q, ivl, m_final, n, m_final, k, Z);
+      ritz_assemble(Q, ivl, m_final, n, m_final, k, perm, Z);
     } else {
-      ritz_assemble(Q, S, mmax, n, m_final, k, Z);
+      ritz_assemble(Q, S, mmax, n, m_final, k, perm, Z);
     }
     __syncthreads();
     for (int c = wid; c < k; c += nw) {
