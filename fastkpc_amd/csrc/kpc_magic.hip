@@ -1114,16 +1114,19 @@ __device__ __attribute__((always_inline)) void upper_tri_inverse_regs(const doub
   __syncthreads();
 }
 
-// Register width for (q, mr): 0 when the LDS forms must be used.
-__device__ __forceinline__ int regs_width(int q, int mr) {
+// Register width for (q, mr): 0 when the LDS forms must be used.  The
+// column-wise forms (Q1, R^-1) hold 2 K values per lane and take K <= 6; the
+// one-wave QR holds K^2 and takes K <= 4.
+__device__ __forceinline__ int regs_width(int q, int mr, int kmax) {
   const int k = max((q + 7) >> 3, (mr + 7) >> 3);
   if (2 * ((int)blockDim.x >> 3) < q) return 0;
-  return k <= 2 ? 2 : k <= 4 ? 4 : 0;
+  const int w = k <= 2 ? 2 : k <= 4 ? 4 : k <= 6 ? 6 : k <= 8 ? 8 : 0;
+  return w <= kmax ? w : 0;
 }
 
 __device__ __attribute__((always_inline)) void stacked_qr_fast(double* W, int rows, int q, int mr,
                                                                double* tau) {
-  switch (regs_width(q, mr)) {
+  switch (regs_width(q, mr, 4)) {
     case 2:
       if (threadIdx.x < KPC_WAVE) stacked_qr_regs<2>(W, rows, q, mr, tau);
       __syncthreads();
@@ -1140,18 +1143,20 @@ __device__ __attribute__((always_inline)) void stacked_qr_fast(double* W, int ro
 __device__ __attribute__((always_inline)) void stacked_q1_fast(const double* W, int rows, int q,
                                                                int mr, const double* tau,
                                                                double* Qm) {
-  switch (regs_width(q, mr)) {
+  switch (regs_width(q, mr, 6)) {
     case 2: stacked_q1_regs<2>(W, rows, q, mr, tau, Qm); break;
     case 4: stacked_q1_regs<4>(W, rows, q, mr, tau, Qm); break;
+    case 6: stacked_q1_regs<6>(W, rows, q, mr, tau, Qm); break;
     default: stacked_q1(W, rows, q, mr, tau, Qm);
   }
 }
 
 __device__ __attribute__((always_inline)) void upper_tri_inverse_fast(const double* R, int ld, int q,
                                                                       double* X, double* dinv) {
-  switch (regs_width(q, 0)) {
+  switch (regs_width(q, 0, 6)) {
     case 2: upper_tri_inverse_regs<2>(R, ld, q, X, dinv); break;
     case 4: upper_tri_inverse_regs<4>(R, ld, q, X, dinv); break;
+    case 6: upper_tri_inverse_regs<6>(R, ld, q, X, dinv); break;
     default: upper_tri_inverse(R, ld, q, X, dinv);
   }
 }
@@ -1655,7 +1660,7 @@ __device__ __attribute__((always_inline)) void small_syev_wave(const double* Hin
 // so the compiler addresses it as LDS (ds_read / ds_write with immediate
 // offsets) instead of through generic flat pointers.
 template <bool kLds>
-__global__ __launch_bounds__(kMagicThreads) void magic_kernel(
+__global__ __launch_bounds__(kMagicThreads, 2) void magic_kernel(
     const MagicItem* __restrict__ items, int count, int n, int q, int m, int rk,
     const double* __restrict__ data, const double* __restrict__ design,
     double* __restrict__ resid, double* __restrict__ info, double* __restrict__ scratch,
@@ -1954,7 +1959,7 @@ namespace kpc {
 // derivative of the magic objective, exposed on its own for testing against
 // central differences and for external Newton drivers.
 template <bool kLds>
-__global__ __launch_bounds__(kMagicThreads) void gcv_eval_kernel(
+__global__ __launch_bounds__(kMagicThreads, 2) void gcv_eval_kernel(
     const MagicItem* __restrict__ items, int count, int n, int q, int m, int rk,
     const double* __restrict__ data, const double* __restrict__ design,
     const double* __restrict__ lsp_in, double* __restrict__ out, double* __restrict__ scratch,
