@@ -903,101 +903,9 @@ __device__ __attribute__((always_inline)) void stacked_q1(const double* W, int r
 // ---- register-resident forms (q, mr <= 8 K): the same arithmetic, in the
 // same order per column, as stacked_qr / stacked_q1 / upper_tri_inverse, with
 // each column held by one aligned group of 8 lanes (lane sub holds rows
-// sub, sub + 8, ...).  Only the exchange of data changes: no workgroup
-// barrier sits on the dependency chain, so results are bit-identical.
-
-// Structured QR by wave 0: group g holds the E rows of columns g, g + 8, ...
-// in registers; reflector j is built by the group holding column j and handed
-// to the other groups by cross-lane moves.
-template <int K>
-__device__ __attribute__((always_inline)) void stacked_qr_regs(double* W, int rows, int q, int mr,
-                                                               double* tau) {
-  const int lane = threadIdx.x & (KPC_WAVE - 1);
-  const int g = lane >> 3, sub = lane & 7;
-  double E[K][K];
-#pragma unroll
-  for (int i = 0; i < K; ++i) {
-    const int c = g + 8 * i;
-#pragma unroll
-    for (int t = 0; t < K; ++t) {
-      const int k = sub + 8 * t;
-      E[i][t] = (c < q && k < mr) ? W[(size_t)c * rows + q + k] : 0.0;
-    }
-  }
-  for (int j = 0; j < q; ++j) {
-    const int gj = j & 7, ij = j >> 3;
-    double u[K];
-#pragma unroll
-    for (int t = 0; t < K; ++t) u[t] = E[0][t];
-#pragma unroll
-    for (int i = 1; i < K; ++i)
-      if (i == ij)
-#pragma unroll
-        for (int t = 0; t < K; ++t) u[t] = E[i][t];
-    double tj = 0.0;
-    if (g == gj) {
-      // dlarfg of column j over row j and the E rows (as group_reflector)
-      double s2 = 0.0;
-#pragma unroll
-      for (int t = 0; t < K; ++t)
-        if (sub + 8 * t < mr) s2 += u[t] * u[t];
-      s2 = sum8(s2);
-      const double alpha = W[(size_t)j * rows + j];
-      double scal = 0.0, beta = alpha;
-      if (s2 > 0.0) {
-        const double h = fma(alpha, alpha, s2);
-        const double r = nrsq(h);
-        const double nrm = h * r;
-        beta = -copysign(nrm, alpha);
-        tj = fma(fabs(alpha), r, 1.0);
-        scal = copysign(nrcp(fabs(alpha) + nrm), alpha);
-      }
-#pragma unroll
-      for (int t = 0; t < K; ++t)
-        if (sub + 8 * t < mr) u[t] *= scal;
-#pragma unroll
-      for (int i = 0; i < K; ++i)
-        if (i == ij)
-#pragma unroll
-          for (int t = 0; t < K; ++t) E[i][t] = u[t];
-      if (sub == 0) {
-        W[(size_t)j * rows + j] = beta;
-        tau[j] = tj;
-      }
-    }
-#pragma unroll
-    for (int t = 0; t < K; ++t) u[t] = __shfl(u[t], gj * 8 + sub, KPC_WAVE);
-    tj = __shfl(tj, gj * 8, KPC_WAVE);
-#pragma unroll
-    for (int i = 0; i < K; ++i) {
-      const int c = g + 8 * i;
-      if (c > j && c < q) {
-        double* wc = W + (size_t)c * rows;
-        double dot = 0.0;
-        if (sub == 0) dot = wc[j];
-#pragma unroll
-        for (int t = 0; t < K; ++t)
-          if (sub + 8 * t < mr) dot += u[t] * E[i][t];
-        dot = sum8(dot) * tj;
-        if (tj != 0.0) {
-          if (sub == 0) wc[j] -= dot;
-#pragma unroll
-          for (int t = 0; t < K; ++t)
-            if (sub + 8 * t < mr) E[i][t] -= dot * u[t];
-        }
-      }
-    }
-  }
-#pragma unroll
-  for (int i = 0; i < K; ++i) {
-    const int c = g + 8 * i;
-#pragma unroll
-    for (int t = 0; t < K; ++t) {
-      const int k = sub + 8 * t;
-      if (c < q && k < mr) W[(size_t)c * rows + q + k] = E[i][t];
-    }
-  }
-}
+// sub, sub + 8, ...).  Only the exchange of data changes, so results are
+// bit-identical: Q1 and R^-1 need no workgroup barrier at all, the QR one per
+// reflector instead of LDS round trips for every column entry.
 
 // Q1 column by column: column c only ever combines with itself, so every
 // group runs its columns (cg, cg + ng) through reflectors q-1 .. 0 alone.
@@ -1062,6 +970,94 @@ __device__ __attribute__((always_inline)) void stacked_q1_regs(const double* W, 
   __syncthreads();
 }
 
+// Structured QR with the columns spread over every 8-lane group of the
+// workgroup (columns cg, cg + ng), their E rows in registers: reflector j is
+// built by the group holding column j right after that column's update and
+// published through W (tail), tau and the diagonal; one workgroup barrier per
+// step.  Needs q <= 2 ng.
+template <int K>
+__device__ __attribute__((always_inline)) void qr_cols_reflector(double (&e)[K], double* W,
+                                                                  int rows, int q, int mr, int jj,
+                                                                  double* tau, int sub) {
+  double s2 = 0.0;
+#pragma unroll
+  for (int t = 0; t < K; ++t)
+    if (sub + 8 * t < mr) s2 += e[t] * e[t];
+  s2 = sum8(s2);
+  const double alpha = W[(size_t)jj * rows + jj];
+  double tj = 0.0, scal = 0.0, beta = alpha;
+  if (s2 > 0.0) {
+    const double h = fma(alpha, alpha, s2);
+    const double r = nrsq(h);
+    const double nrm = h * r;
+    beta = -copysign(nrm, alpha);
+    tj = fma(fabs(alpha), r, 1.0);
+    scal = copysign(nrcp(fabs(alpha) + nrm), alpha);
+  }
+#pragma unroll
+  for (int t = 0; t < K; ++t) {
+    const int k = sub + 8 * t;
+    if (k < mr) {
+      e[t] *= scal;
+      W[(size_t)jj * rows + q + k] = e[t];
+    }
+  }
+  if (sub == 0) {
+    W[(size_t)jj * rows + jj] = beta;
+    tau[jj] = tj;
+  }
+}
+
+template <int K>
+__device__ __attribute__((always_inline)) void stacked_qr_cols(double* W, int rows, int q, int mr,
+                                                               double* tau) {
+  const int tid = threadIdx.x;
+  const int ng = (int)blockDim.x >> 3, cg = tid >> 3, sub = tid & 7;
+  double E[2][K];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int c = cg + ng * i;
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+      const int k = sub + 8 * t;
+      E[i][t] = (c < q && k < mr) ? W[(size_t)c * rows + q + k] : 0.0;
+    }
+  }
+  __syncthreads();
+  if (cg == 0) qr_cols_reflector<K>(E[0], W, rows, q, mr, 0, tau, sub);
+  __syncthreads();
+  for (int j = 0; j < q - 1; ++j) {
+    const double tj = tau[j];
+    double u[K];
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+      const int k = sub + 8 * t;
+      u[t] = k < mr ? W[(size_t)j * rows + q + k] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int c = cg + ng * i;
+      if (c > j && c < q) {
+        double* wc = W + (size_t)c * rows;
+        double dot = 0.0;
+        if (sub == 0) dot = wc[j];
+#pragma unroll
+        for (int t = 0; t < K; ++t)
+          if (sub + 8 * t < mr) dot += u[t] * E[i][t];
+        dot = sum8(dot) * tj;
+        if (tj != 0.0) {
+          if (sub == 0) wc[j] -= dot;
+#pragma unroll
+          for (int t = 0; t < K; ++t)
+            if (sub + 8 * t < mr) E[i][t] -= dot * u[t];
+        }
+        if (c == j + 1) qr_cols_reflector<K>(E[i], W, rows, q, mr, j + 1, tau, sub);
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // R^-1 column by column (as upper_tri_inverse: for k = q-1 .. 1 every row
 // i < k subtracts R_ik (x_k d_k), then rows scale by d_i = 1 / R_ii); steps
 // k > c leave column c unchanged.  Needs q <= 2 ng and q <= 8 K.
@@ -1114,9 +1110,9 @@ __device__ __attribute__((always_inline)) void upper_tri_inverse_regs(const doub
   __syncthreads();
 }
 
-// Register width for (q, mr): 0 when the LDS forms must be used.  The
-// column-wise forms (Q1, R^-1) hold 2 K values per lane and take K <= 6; the
-// one-wave QR holds K^2 and takes K <= 4.
+// Register width for (q, mr): 0 when the LDS forms must be used.  Every
+// register form holds 2 K values per lane (two columns per group) and takes
+// K <= 6.
 __device__ __forceinline__ int regs_width(int q, int mr, int kmax) {
   const int k = max((q + 7) >> 3, (mr + 7) >> 3);
   if (2 * ((int)blockDim.x >> 3) < q) return 0;
@@ -1126,17 +1122,11 @@ __device__ __forceinline__ int regs_width(int q, int mr, int kmax) {
 
 __device__ __attribute__((always_inline)) void stacked_qr_fast(double* W, int rows, int q, int mr,
                                                                double* tau) {
-  switch (regs_width(q, mr, 4)) {
-    case 2:
-      if (threadIdx.x < KPC_WAVE) stacked_qr_regs<2>(W, rows, q, mr, tau);
-      __syncthreads();
-      break;
-    case 4:
-      if (threadIdx.x < KPC_WAVE) stacked_qr_regs<4>(W, rows, q, mr, tau);
-      __syncthreads();
-      break;
-    default:
-      stacked_qr(W, rows, q, mr, tau);
+  switch (regs_width(q, mr, 6)) {
+    case 2: stacked_qr_cols<2>(W, rows, q, mr, tau); break;
+    case 4: stacked_qr_cols<4>(W, rows, q, mr, tau); break;
+    case 6: stacked_qr_cols<6>(W, rows, q, mr, tau); break;
+    default: stacked_qr(W, rows, q, mr, tau);
   }
 }
 
