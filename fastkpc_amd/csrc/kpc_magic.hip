@@ -1304,21 +1304,29 @@ __device__ __attribute__((always_inline)) void magic_eval(const MagicLds& L, con
     }
     __syncthreads();
   } else {
-    // ---- exact path: aggregate penalty, pivoted Cholesky root, Jacobi SVD
-    double* agg = L.X4;
-    aggregate_penalty(D, DL, q, m, rk, spv, agg);
-    __syncthreads();
-    if (wid == 0) {
-      const int r = pstf2_upper_wave(agg, q, L.piv, L.wk);
-      if (lane == 0) L.ictl[14] = r;
+    // ---- exact path: Jacobi SVD of the triangular factor R2 of the matrix
+    // just factored ([R; E] or [R; root], the same Gram matrix).  [R; E] =
+    // Q R2 and R2 = U_R D V' give mgcv's D and V directly and U1 = Q1 U_R
+    // (first q rows of U), so the one-sided Jacobi runs on q rows instead of
+    // q + rank and the root is not rebuilt.
+    const int rank = erows;
+    tp1 = wall_clock64();
+    stacked_q1_fast(L.aug, rows, q, erows, L.wk, L.Qm);
+    for (int e = tid; e < q * q; e += blockDim.x) {
+      const int r = e % q, c = e / q;
+      L.X4[e] = r <= c ? L.aug[(size_t)c * rows + r] : 0.0;
     }
     __syncthreads();
-    const int rank = L.ictl[14];
-    tp1 = wall_clock64();
-    rows = q + rank;
-    build_augmented(L, D, DL, q, rank, agg);
     tp2 = wall_clock64();
-    jacobi_svd(L.aug, rows, q, L.V, L.d, L.ord, L.ictl);
+    jacobi_svd(L.X4, q, q, L.V, L.d, L.ord, L.ictl);
+    for (int e = tid; e < q * q; e += blockDim.x) {
+      const int r = e % q, c = e / q;
+      const double* ur = L.X4 + (size_t)c * q;
+      double v = 0.0;
+      for (int k = 0; k < q; ++k) v += L.Qm[(size_t)k * rows + r] * ur[k];
+      L.aug[(size_t)c * rows + r] = v;
+    }
+    __syncthreads();
     const double d0 = L.d[L.ord[0]];
     const double thr = d0 * rank_tol;
     int cnt = 0;
