@@ -63,6 +63,33 @@ __device__ __forceinline__ void block_sum2(double& a, double& b, double* red) {
   b = tb;
 }
 
+// Three workgroup sums at once (one pair of barriers); `red` must hold
+// 3 * (blockDim.x / 64) doubles.
+__device__ __forceinline__ void block_sum3(double& a, double& b, double& c, double* red) {
+  const int lane = threadIdx.x & (KPC_WAVE - 1);
+  const int wid = threadIdx.x / KPC_WAVE;
+  const int nw = (blockDim.x + KPC_WAVE - 1) / KPC_WAVE;
+  a = wave_sum(a);
+  b = wave_sum(b);
+  c = wave_sum(c);
+  __syncthreads();
+  if (lane == 0) {
+    red[3 * wid] = a;
+    red[3 * wid + 1] = b;
+    red[3 * wid + 2] = c;
+  }
+  __syncthreads();
+  double ta = 0.0, tb = 0.0, tc = 0.0;
+  for (int w = 0; w < nw; ++w) {
+    ta += red[3 * w];
+    tb += red[3 * w + 1];
+    tc += red[3 * w + 2];
+  }
+  a = ta;
+  b = tb;
+  c = tc;
+}
+
 __device__ __forceinline__ double block_max(double v, double* red) {
   const int lane = threadIdx.x & (KPC_WAVE - 1);
   const int wid = threadIdx.x / KPC_WAVE;

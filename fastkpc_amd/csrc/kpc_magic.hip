@@ -1294,13 +1294,16 @@ __device__ __attribute__((always_inline)) void magic_eval(const MagicLds& L, con
       L.ictl[13] = 1;
       L.dctl[6] = lb / rank_tol - 1.0;
     }
-    // y1 = Q1' y0
-    for (int c = wid; c < q; c += nw) {
-      const double* u = L.Qm + (size_t)c * rows;
-      double sacc = 0.0;
-      for (int r = lane; r < q; r += KPC_WAVE) sacc += u[r] * L.y[r];
-      sacc = wave_sum(sacc);
-      if (lane == 0) L.y1[c] = sacc;
+    // y1 = Q1' y0, one aligned 8-lane group per column
+    {
+      const int ng = (int)blockDim.x >> 3, cg = tid >> 3, sub = tid & 7;
+      for (int c = cg; c < q; c += ng) {
+        const double* u = L.Qm + (size_t)c * rows;
+        double sacc = 0.0;
+        for (int r = sub; r < q; r += 8) sacc += u[r] * L.y[r];
+        sacc = sum8(sacc);
+        if (sub == 0) L.y1[c] = sacc;
+      }
     }
     __syncthreads();
   } else {
@@ -1364,21 +1367,34 @@ __device__ __attribute__((always_inline)) void magic_eval(const MagicLds& L, con
   const int nr = L.ictl[0];
   const bool qr = L.ictl[13] != 0;
   rows = q + L.ictl[1];
-  double p_yaay = 0.0, p_edf = 0.0;
-  for (int r = tid; r < q; r += blockDim.x) {
-    double v = 0.0;
-    for (int c = 0; c < nr; ++c) {
-      const double u = qr ? L.Qm[(size_t)c * rows + r] : L.aug[(size_t)L.ord[c] * rows + r];
-      v += u * L.y1[c];
-      p_edf += u * u;
+  double p_yaay = 0.0, p_edf = 0.0, p_yay = 0.0;
+  if (qr) {
+    // (Q1 y1)_r and the squares of row r of Q1, one aligned 8-lane group per row
+    const int ng = (int)blockDim.x >> 3, cg = tid >> 3, sub = tid & 7;
+    for (int r = cg; r < q; r += ng) {
+      double v = 0.0;
+      for (int c = sub; c < nr; c += 8) {
+        const double u = L.Qm[(size_t)c * rows + r];
+        v += u * L.y1[c];
+        p_edf += u * u;
+      }
+      v = sum8(v);
+      if (sub == 0) p_yaay += v * v;
     }
-    p_yaay += v * v;
+  } else {
+    for (int r = tid; r < q; r += blockDim.x) {
+      double v = 0.0;
+      for (int c = 0; c < nr; ++c) {
+        const double u = L.aug[(size_t)L.ord[c] * rows + r];
+        v += u * L.y1[c];
+        p_edf += u * u;
+      }
+      p_yaay += v * v;
+    }
   }
-  const double yAAy = block_sum(p_yaay, L.red);
-  const double edf = block_sum(p_edf, L.red);
-  double p_yay = 0.0;
   for (int c = tid; c < nr; c += blockDim.x) p_yay += L.y1[c] * L.y1[c];
-  const double yAy = block_sum(p_yay, L.red);
+  block_sum3(p_yaay, p_edf, p_yay, L.red);
+  const double yAAy = p_yaay, edf = p_edf, yAy = p_yay;
   if (tid == 0) {
     double rss = ynorm2 - 2.0 * yAy + yAAy;
     if (rss < 0.0) rss = 0.0;
@@ -1438,10 +1454,13 @@ __device__ __attribute__((always_inline)) void magic_derivs(const MagicLds& L, c
   __syncthreads();
   // GT = G T (G symmetric), gy = G y1
   wg_gemm_tn(G, nr, T, nr, GT, nr, nr, mr, nr);
-  for (int c = tid; c < nr; c += blockDim.x) {
+  // short reductions below: one aligned 8-lane group per output (DPP sums)
+  const int ngr = (int)blockDim.x >> 3, cgr = tid >> 3, sub8 = tid & 7;
+  for (int c = cgr; c < nr; c += ngr) {
     double s = 0.0;
-    for (int k = 0; k < nr; ++k) s += G[(size_t)k * nr + c] * L.y1[k];
-    L.gyl[c] = s;
+    for (int k = sub8; k < nr; k += 8) s += G[(size_t)k * nr + c] * L.y1[k];
+    s = sum8(s);
+    if (sub8 == 0) L.gyl[c] = s;
   }
   __syncthreads();
   // P = T'T and Qx = T' G T (mr x mr) in the Q1 area (free from here on):
@@ -1452,12 +1471,16 @@ __device__ __attribute__((always_inline)) void magic_derivs(const MagicLds& L, c
   wg_gemm_tn(T, nr, GT, nr, Qx, mr, mr, mr, nr);
   // per penalty: My = T_i (T_i' y1), Ky = T_i (T_i' G y1), yK = G My; the
   // column projections T_a' y1 and T_a' G y1 are formed once
-  for (int a = tid; a < mr; a += blockDim.x) {
+  for (int a = cgr; a < mr; a += ngr) {
     const double* Ta = T + (size_t)a * nr;
     double ty = 0.0, tg = 0.0;
-    for (int c = 0; c < nr; ++c) { ty += Ta[c] * L.y1[c]; tg += Ta[c] * L.gyl[c]; }
-    L.projl[2 * a] = ty;
-    L.projl[2 * a + 1] = tg;
+    for (int c = sub8; c < nr; c += 8) { ty += Ta[c] * L.y1[c]; tg += Ta[c] * L.gyl[c]; }
+    ty = sum8(ty);
+    tg = sum8(tg);
+    if (sub8 == 0) {
+      L.projl[2 * a] = ty;
+      L.projl[2 * a + 1] = tg;
+    }
   }
   __syncthreads();
   for (int e = tid; e < m * nr; e += blockDim.x) {
@@ -1472,11 +1495,12 @@ __device__ __attribute__((always_inline)) void magic_derivs(const MagicLds& L, c
     L.Ky[(size_t)i * nr + o] = ky;
   }
   __syncthreads();
-  for (int e = tid; e < m * nr; e += blockDim.x) {
+  for (int e = cgr; e < m * nr; e += ngr) {
     const int i = e / nr, o = e % nr;
     double s = 0.0;
-    for (int k = 0; k < nr; ++k) s += G[(size_t)k * nr + o] * L.My[(size_t)i * nr + k];
-    L.yK[(size_t)i * nr + o] = s;
+    for (int k = sub8; k < nr; k += 8) s += G[(size_t)k * nr + o] * L.My[(size_t)i * nr + k];
+    s = sum8(s);
+    if (sub8 == 0) L.yK[(size_t)i * nr + o] = s;
   }
   const long long td1 = wall_clock64();
   // traces: one wave per (i, j <= i) pair and per tr(K_i), no workgroup barrier
@@ -1521,14 +1545,16 @@ __device__ __attribute__((always_inline)) void magic_derivs(const MagicLds& L, c
     L.dctl[12] += (double)(td1 - td0);  // G, T, G T, P, Qx, projections
     L.dctl[13] += (double)(td2 - td1);  // traces
   }
-  // d|y-Ay|^2 terms: one lane per i (dnorm) and per pair j <= i (d2n)
+  // d|y-Ay|^2 terms: one 8-lane group per i (dnorm) and per pair j <= i (d2n)
   const int npairs = m * (m + 1) / 2;
-  for (int e = tid; e < m + npairs; e += blockDim.x) {
+  for (int e = cgr; e < m + npairs; e += ngr) {
     if (e < m) {
       const int i = e;
       double nd = 0.0;
-      for (int c = 0; c < nr; ++c) nd += L.y1[c] * (L.My[(size_t)i * nr + c] - L.Ky[(size_t)i * nr + c]);
-      L.dn[i] = 2.0 * exp(lspv[i]) * nd;
+      for (int c = sub8; c < nr; c += 8)
+        nd += L.y1[c] * (L.My[(size_t)i * nr + c] - L.Ky[(size_t)i * nr + c]);
+      nd = sum8(nd);
+      if (sub8 == 0) L.dn[i] = 2.0 * exp(lspv[i]) * nd;
     } else {
       int i = 0, rem = e - m;
       while (rem > i) {
@@ -1537,14 +1563,17 @@ __device__ __attribute__((always_inline)) void magic_derivs(const MagicLds& L, c
       }
       const int j = rem;
       double nv = 0.0;
-      for (int c = 0; c < nr; ++c) {
+      for (int c = sub8; c < nr; c += 8) {
         const double myi = L.My[(size_t)i * nr + c], myj = L.My[(size_t)j * nr + c];
         nv += myi * L.Ky[(size_t)j * nr + c] + myj * L.Ky[(size_t)i * nr + c] - 2.0 * myi * myj +
               L.yK[(size_t)i * nr + c] * myj;
       }
+      nv = sum8(nv);
       nv *= 2.0 * exp(lspv[i] + lspv[j]);
-      L.d2n[j * m + i] = nv;
-      L.d2n[i * m + j] = nv;
+      if (sub8 == 0) {
+        L.d2n[j * m + i] = nv;
+        L.d2n[i * m + j] = nv;
+      }
     }
   }
   __syncthreads();
