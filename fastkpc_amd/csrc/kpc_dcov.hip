@@ -310,7 +310,7 @@ __global__ __launch_bounds__(kDcovThreads) void dcov_component_kernel(
 // each thread accumulates one or two 4 x 4 blocks of C in registers, and every
 // C_ij is summed over the rows in order (as a plain dot product would).
 constexpr int kPairThreads = 128;
-constexpr int kPairRows = 64;
+constexpr int kPairRows = 32;  // 18 KB of LDS per workgroup at k = 35: eight per CU
 
 __device__ __forceinline__ void pair_block_accumulate(const double* Xc, const double* Yc, int kp,
                                                       int rows, int bi, int bj, double* acc) {
@@ -355,10 +355,26 @@ __global__ __launch_bounds__(kPairThreads) void dcov_pair_kernel(
     for (int r0 = 0; r0 < n; r0 += kPairRows) {
       const int rows = min(kPairRows, n - r0);
       __syncthreads();
-      for (int e = tid; e < rows * kp; e += blockDim.x) {
-        const int i = e / rows, r = e % rows;
-        Xc[r * kp + i] = i < k ? X[(size_t)i * n + r0 + r] : 0.0;
-        Yc[r * kp + i] = i < k ? Y[(size_t)i * n + r0 + r] : 0.0;
+      // four elements of X and Y per thread in flight before any LDS store
+      for (int e0 = tid; e0 < rows * kp; e0 += 4 * (int)blockDim.x) {
+        double xv[4], yv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int e = e0 + u * (int)blockDim.x;
+          const int i = e / rows, r = e % rows;
+          const bool ok = e < rows * kp && i < k;
+          xv[u] = ok ? X[(size_t)i * n + r0 + r] : 0.0;
+          yv[u] = ok ? Y[(size_t)i * n + r0 + r] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int e = e0 + u * (int)blockDim.x;
+          const int i = e / rows, r = e % rows;
+          if (e < rows * kp) {
+            Xc[r * kp + i] = xv[u];
+            Yc[r * kp + i] = yv[u];
+          }
+        }
       }
       __syncthreads();
       if (has0) pair_block_accumulate(Xc, Yc, kp, rows, b0 % nb, b0 / nb, acc0);
