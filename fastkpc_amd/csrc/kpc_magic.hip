@@ -235,6 +235,9 @@ __device__ __attribute__((always_inline)) void wg_gemm_tn(const double* A, int l
 }
 
 // ------------------------------------------------------------- geometry
+__device__ __attribute__((always_inline)) void upper_tri_inverse_fast(const double* R, int ld, int q,
+                                                                      double* X, double* dinv);
+
 __global__ __launch_bounds__(kGeomThreads) void geometry_kernel(
     const GeometryItem* __restrict__ items, int count, int n, int q, int m, int dim, int rk,
     const double* __restrict__ smooth_X, const double* __restrict__ smooth_S,
@@ -245,6 +248,10 @@ __global__ __launch_bounds__(kGeomThreads) void geometry_kernel(
   __shared__ int ipiv[kMaxQ];
   __shared__ int ictl[24];
   __shared__ double dctl[24];
+  // initial.sp work of the serial section (thread 0), in LDS rather than
+  // per-lane scratch memory
+  __shared__ double isp_pd[kMaxQ], isp_vals[kMaxQ], isp_sp[kMaxPen], isp_sx[32], isp_spd[32];
+  __shared__ unsigned char isp_pen[kMaxQ];
   const int tid = threadIdx.x;
   const int lane = tid & (KPC_WAVE - 1);
   const int wid = tid / KPC_WAVE;
@@ -417,12 +424,13 @@ __global__ __launch_bounds__(kGeomThreads) void geometry_kernel(
     }
     __syncthreads();
     if (tid == 0) {
-      double* pd = ws;                 // q
-      unsigned char pen[kMaxQ];
-      double sp[kMaxPen];
+      double* pd = isp_pd;  // q
+      unsigned char* pen = isp_pen;
+      double* sp = isp_sp;
       for (int c = 0; c < q; ++c) { pd[c] = 0.0; pen[c] = 0; }
       const double thr_scale = pow(kEps, 0.8);
-      double sx[32], sp_d[32];
+      double* sx = isp_sx;
+      double* sp_d = isp_spd;
       for (int s = 0; s < m; ++s) {
         const double* B = smooth_S + (size_t)it.smooth_slot[s] * 29 * 29;
         const int off = 1 + s * dim;
@@ -450,7 +458,7 @@ __global__ __launch_bounds__(kGeomThreads) void geometry_kernel(
         sp[s] = serial_r_mean(sx, cnt) / serial_r_mean(sp_d, cnt);
         for (int r = 0; r < dim; ++r) pd[off + r] += sp[s] * B[(size_t)r * dim + r];
       }
-      double vals[kMaxQ];
+      double* vals = isp_vals;
       for (int guard = 0; guard < 4000; ++guard) {
         int cnt = 0;
         for (int c = 0; c < q; ++c)
@@ -480,7 +488,7 @@ __global__ __launch_bounds__(kGeomThreads) void geometry_kernel(
     // norms for the GCV full-rank certificate: |R|_F^2, |R^-1|_F^2, |rS_i|_F^2
     {
       double* Rinv = ws + q;  // q x q after the q doubles of initial.sp
-      upper_tri_inverse(D + DL.R, q, q, Rinv, Rinv + (size_t)q * q);
+      upper_tri_inverse_fast(D + DL.R, q, q, Rinv, Rinv + (size_t)q * q);
       double pr = 0.0, pi = 0.0;
       for (int e = tid; e < q * q; e += blockDim.x) {
         const int r = e % q, c = e / q;
