@@ -186,6 +186,73 @@ struct TwinHash {
   }
 };
 
+// Open-addressing hash map (linear probing, power-of-two capacity, at most
+// half full, Fibonacci hashing of H's value): the per-task lookups of a wave
+// (twins, designs, residual and component slots) without a node allocation per
+// insert.  Capacity is kept across clear().  No iteration order is exposed.
+template <typename K, typename V, typename H>
+class FlatMap {
+ public:
+  V* find(const K& k) {
+    if (size_ == 0) return nullptr;
+    for (size_t i = slot(k);; i = (i + 1) & mask_) {
+      if (!used_[i]) return nullptr;
+      if (keys_[i] == k) return &vals_[i];
+    }
+  }
+  // Inserts (k, v) when k is absent; returns the stored value either way.
+  V& emplace(const K& k, const V& v) {
+    if (2 * (size_ + 1) > used_.size()) grow();
+    size_t i = slot(k);
+    for (; used_[i]; i = (i + 1) & mask_)
+      if (keys_[i] == k) return vals_[i];
+    used_[i] = 1;
+    keys_[i] = k;
+    vals_[i] = v;
+    ++size_;
+    return vals_[i];
+  }
+  void set(const K& k, const V& v) { emplace(k, v) = v; }
+  void clear() {
+    if (size_ == 0) return;
+    std::fill(used_.begin(), used_.end(), (unsigned char)0);
+    size_ = 0;
+  }
+  size_t size() const { return size_; }
+
+ private:
+  size_t slot(const K& k) const {
+    return (size_t)(((uint64_t)H()(k) * 0x9E3779B97F4A7C15ULL) >> shift_);
+  }
+  void grow() {
+    std::vector<K> ok;
+    std::vector<V> ov;
+    std::vector<unsigned char> ou;
+    ok.swap(keys_);
+    ov.swap(vals_);
+    ou.swap(used_);
+    const size_t cap = ou.empty() ? 64 : 2 * ou.size();
+    keys_.resize(cap);
+    vals_.resize(cap);
+    used_.assign(cap, 0);
+    mask_ = cap - 1;
+    shift_ = 64;
+    for (size_t c = cap; c > 1; c >>= 1) --shift_;
+    size_ = 0;
+    for (size_t i = 0; i < ou.size(); ++i)
+      if (ou[i]) emplace(ok[i], ov[i]);
+  }
+  std::vector<K> keys_;
+  std::vector<V> vals_;
+  std::vector<unsigned char> used_;
+  size_t mask_ = 0, size_ = 0;
+  int shift_ = 64;
+};
+
+struct U64Hash {
+  size_t operator()(uint64_t k) const { return (size_t)(k ^ (k >> 31)); }
+};
+
 struct TestResult {
   double p, stat, mean, var, margin;
 };
@@ -352,20 +419,18 @@ class Engine {
     // orientation-free identity: reuse a twin's result, evaluate each once
     std::vector<Task*> tasks;
     std::vector<std::pair<Task*, size_t>> dups;  // (task, index into tasks)
-    std::unordered_map<std::pair<uint64_t, uint64_t>, size_t, TwinHash> in_batch;
+    in_batch_.clear();
     for (Task* t : all) {
       const auto key = twin_key(*t);
-      auto done = twin_map_.find(key);
-      if (done != twin_map_.end()) {
-        apply_result(*t, done->second);
+      if (const TestResult* done = twin_map_.find(key)) {
+        apply_result(*t, *done);
         continue;
       }
-      auto it = in_batch.find(key);
-      if (it != in_batch.end()) {
-        dups.emplace_back(t, it->second);
+      if (const size_t* it = in_batch_.find(key)) {
+        dups.emplace_back(t, *it);
         continue;
       }
-      in_batch.emplace(key, tasks.size());
+      in_batch_.emplace(key, tasks.size());
       tasks.push_back(t);
     }
     if (tasks.empty()) return;
@@ -386,15 +451,13 @@ class Engine {
         // side 0: min(x, y), side 1: max(x, y) (the permuted variable)
         const int tgt = side ? std::max(t->x, t->y) : std::min(t->x, t->y);
         auto key = std::make_pair(tgt, t->mask);
-        auto it = comp_map_.find(key);
-        if (it != comp_map_.end()) {
-          slots[side] = it->second;
-          auto rt = resid_map_.find(key);
-          if (rt != resid_map_.end()) rslot[side] = rt->second;
+        if (const int* it = comp_map_.find(key)) {
+          slots[side] = *it;
+          if (const int* rt = resid_map_.find(key)) rslot[side] = *rt;
           continue;
         }
         const int cs = n_comp_++;
-        comp_map_[key] = cs;
+        comp_map_.set(key, cs);
         slots[side] = cs;
         comp_dst.push_back(cs);
         if (level_ == 0) {
@@ -403,7 +466,7 @@ class Engine {
         }
         int dslot = ensure_design(*t, smooth_items, geo_items);
         const int rs = n_resid_++;
-        resid_map_[key] = rs;
+        resid_map_.set(key, rs);
         rslot[side] = rs;
         kpc::MagicItem mi;
         mi.design_slot = dslot;
@@ -535,7 +598,7 @@ class Engine {
       apply_result(*tasks[i], res);
       twin_map_.emplace(twin_key(*tasks[i]), res);
     }
-    for (auto& d : dups) apply_result(*d.first, twin_map_.at(twin_key(*tasks[d.second])));
+    for (auto& d : dups) apply_result(*d.first, *twin_map_.find(twin_key(*tasks[d.second])));
   }
 
   static std::pair<uint64_t, uint64_t> twin_key(const Task& t) {
@@ -833,10 +896,9 @@ class Engine {
 
   int ensure_design(const Task& t, std::vector<int>& smooth_items,
                     std::vector<kpc::GeometryItem>& geo_items) {
-    auto it = design_map_.find(t.mask);
-    if (it != design_map_.end()) return it->second;
+    if (const int* it = design_map_.find(t.mask)) return *it;
     const int ds = n_design_++;
-    design_map_[t.mask] = ds;
+    design_map_.set(t.mask, ds);
     kpc::GeometryItem gi;
     std::memset(&gi, 0, sizeof(gi));
     gi.design_slot = ds;
@@ -944,9 +1006,10 @@ class Engine {
   kpc::MagicControl ctl_;
   int level_ = -1, q_ = 0, m_ = 0, dim_ = 0, rk_ = 0;
   kpc::DesignLayout layout_{};
-  std::unordered_map<uint64_t, int> design_map_;
-  std::unordered_map<std::pair<int, uint64_t>, int, KeyHash> resid_map_, comp_map_;
-  std::unordered_map<std::pair<uint64_t, uint64_t>, TestResult, TwinHash> twin_map_;
+  FlatMap<uint64_t, int, U64Hash> design_map_;
+  FlatMap<std::pair<int, uint64_t>, int, KeyHash> resid_map_, comp_map_;
+  FlatMap<std::pair<uint64_t, uint64_t>, TestResult, TwinHash> twin_map_;
+  FlatMap<std::pair<uint64_t, uint64_t>, size_t, TwinHash> in_batch_;
   std::vector<int> uni_slot_;
   int n_design_ = 0, n_resid_ = 0, n_comp_ = 0, n_lsmooth_ = 0;
   DevArray<double> lsmooth_X_, lsmooth_S_, usmooth_X_, usmooth_S_;
@@ -1121,8 +1184,8 @@ void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
     const Engine::Stats st0 = eng.stats_;
     eng.begin_level(ord);
     {
-      std::unordered_map<uint64_t, int> ud;
-      std::unordered_map<std::pair<int, uint64_t>, int, KeyHash> ur;
+      FlatMap<uint64_t, int, U64Hash> ud;
+      FlatMap<std::pair<int, uint64_t>, int, KeyHash> ur;
       for (const Task& t : tasks) {
         ud.emplace(t.mask, 0);
         ur.emplace(std::make_pair(t.x, t.mask), 0);
