@@ -55,10 +55,12 @@ hipError_t launch_smooth_setup(const double* data, int n, int p, const int* item
                                int grid, hipStream_t stream);
 
 long long geometry_scratch_doubles(int q);
+// status_out[design_slot] repeats each design's status word in a compact
+// array (one contiguous read for the host).
 hipError_t launch_geometry(const GeometryItem* items, int count, int n, int q, int m, int dim,
                            int rk, const double* smooth_X, const double* smooth_S, double* design,
-                           long long smooth_x_stride, double* scratch, int grid,
-                           hipStream_t stream);
+                           double* status_out, long long smooth_x_stride, double* scratch,
+                           int grid, hipStream_t stream);
 
 long long magic_scratch_doubles(int q, int m, int rk, int n);
 size_t magic_lds_bytes(int n, int q, int m, bool aug_in_lds);

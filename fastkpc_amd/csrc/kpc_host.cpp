@@ -400,6 +400,7 @@ class Engine {
     }
     if (level_ > 0) {
       design_.reserve(max_designs * (size_t)layout_.stride);
+      design_status_.reserve(max_designs);
       resid_.reserve(max_resid * (size_t)n_);
       rinfo_.reserve(max_resid * kpc::kMagicInfo);
     }
@@ -527,7 +528,8 @@ class Engine {
       hip_check(kpc::launch_geometry(d_geo_items_.ptr, (int)geo_items.size(), n_, q_, m_, dim_, rk_,
                                      uni ? usmooth_X_.ptr : lsmooth_X_.ptr,
                                      uni ? usmooth_S_.ptr : lsmooth_S_.ptr, design_.ptr,
-                                     (long long)n_ * 29, scratch2_.ptr, grid, stream_),
+                                     design_status_.ptr, (long long)n_ * 29, scratch2_.ptr, grid,
+                                     stream_),
                 "geometry launch");
     }
     if (!graph) hip_check(hipEventRecord(ev_[2], stream_), "event");
@@ -871,7 +873,8 @@ class Engine {
       hip_check(kpc::launch_geometry(d_geo_items_.ptr, (int)geo_items.size(), n_, q_, m_, dim_, rk_,
                                      uni ? usmooth_X_.ptr : lsmooth_X_.ptr,
                                      uni ? usmooth_S_.ptr : lsmooth_S_.ptr, design_.ptr,
-                                     (long long)n_ * 29, scratch2_.ptr, grid, stream_),
+                                     design_status_.ptr, (long long)n_ * 29, scratch2_.ptr, grid,
+                                     stream_),
                 "geometry launch");
     }
     hip_check(hipStreamSynchronize(stream_), "setup synchronize");
@@ -947,17 +950,15 @@ class Engine {
       }
     }
     if (!geos.empty()) {
-      // one strided copy of the status word of every design slot in range
+      // one copy of the compact status words of every design slot in range
       int lo = geos[0].design_slot, hi = lo;
       for (const auto& g : geos) {
         lo = std::min(lo, g.design_slot);
         hi = std::max(hi, g.design_slot);
       }
       std::vector<double> st((size_t)(hi - lo + 1));
-      hip_check(hipMemcpy2D(st.data(), sizeof(double),
-                            design_.ptr + (size_t)lo * layout_.stride + layout_.status,
-                            sizeof(double) * (size_t)layout_.stride, sizeof(double), st.size(),
-                            hipMemcpyDeviceToHost),
+      hip_check(hipMemcpy(st.data(), design_status_.ptr + lo, sizeof(double) * st.size(),
+                          hipMemcpyDeviceToHost),
                 "design status");
       for (const auto& g : geos)
         if (st[(size_t)(g.design_slot - lo)] != 0.0)
@@ -1006,6 +1007,7 @@ class Engine {
   kpc::MagicControl ctl_;
   int level_ = -1, q_ = 0, m_ = 0, dim_ = 0, rk_ = 0;
   kpc::DesignLayout layout_{};
+  DevArray<double> design_status_;  // status word of each design slot (compact copy)
   FlatMap<uint64_t, int, U64Hash> design_map_;
   FlatMap<std::pair<int, uint64_t>, int, KeyHash> resid_map_, comp_map_;
   FlatMap<std::pair<uint64_t, uint64_t>, TestResult, TwinHash> twin_map_;

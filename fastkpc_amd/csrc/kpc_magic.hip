@@ -241,8 +241,8 @@ __device__ __attribute__((always_inline)) void upper_tri_inverse_fast(const doub
 __global__ __launch_bounds__(kGeomThreads) void geometry_kernel(
     const GeometryItem* __restrict__ items, int count, int n, int q, int m, int dim, int rk,
     const double* __restrict__ smooth_X, const double* __restrict__ smooth_S,
-    double* __restrict__ design, long long smooth_x_stride, double* __restrict__ scratch,
-    long long scratch_stride) {
+    double* __restrict__ design, double* __restrict__ status_out, long long smooth_x_stride,
+    double* __restrict__ scratch, long long scratch_stride) {
   __shared__ double red[16];
   __shared__ double vn1[kMaxQ], vn2[kMaxQ], colv[kMaxQ];
   __shared__ int ipiv[kMaxQ];
@@ -483,6 +483,7 @@ __global__ __launch_bounds__(kGeomThreads) void geometry_kernel(
       }
       for (int s = 0; s < m; ++s) D[DL.lsp0 + s] = log(sp[s]);
       D[DL.status] = (double)status;
+      if (status_out) status_out[it.design_slot] = (double)status;
     }
     __syncthreads();
     // norms for the GCV full-rank certificate: |R|_F^2, |R^-1|_F^2, |rS_i|_F^2
@@ -1956,12 +1957,12 @@ long long geometry_scratch_doubles(int q) {
 
 hipError_t launch_geometry(const GeometryItem* items, int count, int n, int q, int m, int dim,
                            int rk, const double* smooth_X, const double* smooth_S, double* design,
-                           long long smooth_x_stride, double* scratch, int grid,
-                           hipStream_t stream) {
+                           double* status_out, long long smooth_x_stride, double* scratch,
+                           int grid, hipStream_t stream) {
   if (count <= 0) return hipSuccess;
   hipLaunchKernelGGL(geometry_kernel, dim3(grid), dim3(kGeomThreads), 0, stream, items, count, n,
-                     q, m, dim, rk, smooth_X, smooth_S, design, smooth_x_stride, scratch,
-                     geometry_scratch_doubles(q));
+                     q, m, dim, rk, smooth_X, smooth_S, design, status_out, smooth_x_stride,
+                     scratch, geometry_scratch_doubles(q));
   return hipGetLastError();
 }
 
