@@ -37,6 +37,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <deque>
 #include <map>
 #include <mutex>
 #include <thread>
@@ -293,6 +294,36 @@ double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// HIP stream and timing events of an engine, reused by the later engines of
+// the same thread and device (creating and destroying them costs ~0.4 ms per
+// call).  They are kept until the process ends; an engine created while
+// another engine of the same thread holds them gets a set of its own.
+struct EngineHipResources {
+  int device = -1;
+  bool in_use = false;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[11] = {};
+};
+thread_local std::deque<EngineHipResources>* g_engine_res = nullptr;
+
+EngineHipResources* acquire_engine_resources() {
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (!g_engine_res) g_engine_res = new std::deque<EngineHipResources>();
+  for (auto& r : *g_engine_res)
+    if (r.device == dev && !r.in_use) {
+      r.in_use = true;
+      return &r;
+    }
+  EngineHipResources r;
+  r.device = dev;
+  hip_check(hipStreamCreate(&r.stream), "hipStreamCreate");
+  for (int e = 0; e < 11; ++e) hip_check(hipEventCreate(&r.ev[e]), "hipEventCreate");
+  r.in_use = true;
+  g_engine_res->push_back(r);
+  return &g_engine_res->back();
+}
+
 class Engine {
  public:
   // skeleton engines index variables with 64-bit sets (p <= 64); the
@@ -320,7 +351,6 @@ class Engine {
     comp_k_ = method_ == KPC_DCC_PERM ? 2 : opt.num_col;
     for (long long i = 0; i < (long long)n * p; ++i)
       if (!std::isfinite(h_data[i])) throw KpcError(-1, "data must be finite");
-    hip_check(hipStreamCreate(&stream_), "hipStreamCreate");
     data_bytes_ = sizeof(double) * (size_t)n * p;
     d_data_ = (double*)pool_acquire(data_bytes_);
     hip_check(hipMemcpy(d_data_, h_data, sizeof(double) * (size_t)n * p, hipMemcpyHostToDevice),
@@ -334,8 +364,7 @@ class Engine {
     ctl_.probe_step = 2.0;
     ctl_.max_probes = 5;
     ctl_.rank_tol = 1.4901161193847656e-8;
-    for (int e = 0; e < 8; ++e) hip_check(hipEventCreate(&ev_[e]), "hipEventCreate");
-    for (int e = 0; e < 3; ++e) hip_check(hipEventCreate(&tev_[e]), "hipEventCreate");
+
     if (const char* sh = std::getenv("KPC_DCC_SHAPE")) {
       int jt = 0, rb = 0;
       if (std::sscanf(sh, "%dx%d", &jt, &rb) == 2 && kpc::dcc_perm_shape_supported(jt, rb) &&
@@ -345,6 +374,10 @@ class Engine {
         dcc_shape_forced_ = true;
       }
     }
+    res_ = acquire_engine_resources();
+    stream_ = res_->stream;
+    for (int e = 0; e < 8; ++e) ev_[e] = res_->ev[e];
+    for (int e = 0; e < 3; ++e) tev_[e] = res_->ev[8 + e];
   }
   Engine(const Engine&) = delete;
   Engine& operator=(const Engine&) = delete;
@@ -352,10 +385,7 @@ class Engine {
     // arenas go back to the pool after this body: nothing may still use them
     if (stream_) (void)hipStreamSynchronize(stream_);
     if (d_data_) pool_release(d_data_, data_bytes_);
-    for (int e = 0; e < 8; ++e) (void)hipEventDestroy(ev_[e]);
-    for (int e = 0; e < 3; ++e)
-      if (tev_[e]) (void)hipEventDestroy(tev_[e]);
-    if (stream_) (void)hipStreamDestroy(stream_);
+    if (res_) res_->in_use = false;  // stream and events stay with the thread
   }
 
   // ---- per-level arenas / caches -------------------------------------
@@ -1000,6 +1030,7 @@ class Engine {
   double index_ = 1.0;
   uint64_t seed_ = 0;
   kpc_options opt_;
+  EngineHipResources* res_ = nullptr;
   hipStream_t stream_ = nullptr;
   double* d_data_ = nullptr;
   size_t data_bytes_ = 0;
@@ -1315,13 +1346,18 @@ void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
   {
     std::string keys;
     keys.reserve((size_t)r->n_tasks * 8 + 1);
-    char num[16];
+    char num[12];
     for (int i = 0; i < r->n_tasks; ++i) {
       if (i) keys.push_back('\n');
       for (int j = 0; j < r->task_slen[i]; ++j) {
         if (j) keys.push_back('|');
-        const int len = std::snprintf(num, sizeof(num), "%d", r->task_S[(size_t)i * 64 + j] + 1);
-        keys.append(num, (size_t)len);
+        unsigned v = (unsigned)(r->task_S[(size_t)i * 64 + j] + 1);
+        int len = 0;
+        do {
+          num[len++] = (char)('0' + v % 10);
+          v /= 10;
+        } while (v);
+        while (len) keys.push_back(num[--len]);
       }
     }
     r->task_keys = (char*)std::malloc(keys.size() + 1);
