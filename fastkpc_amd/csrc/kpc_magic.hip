@@ -1017,11 +1017,43 @@ __device__ __attribute__((always_inline)) void qr_cols_reflector(double (&e)[K],
   }
 }
 
+// Q1 alongside the QR: X = Q' [I; 0] = H_{q-1} ... H_0 [I; 0] takes every
+// reflector as soon as it is published, and its top block is Q1' (the top
+// entry of column c in row j changes only at step j, so it is written out
+// then; the E rows of X stay in registers).
+template <int K>
+__device__ __forceinline__ void qr_cols_apply_x(double (&X)[2][K], const double (&u)[K], double tj,
+                                                int j, int q, int mr, int rows, int cg, int ng,
+                                                int sub, double* Qm) {
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int c = cg + ng * i;
+    if (c < q) {
+      const double top = c == j ? 1.0 : 0.0;
+      double dot = 0.0;
+      if (sub == 0) dot = top;
+#pragma unroll
+      for (int t = 0; t < K; ++t)
+        if (sub + 8 * t < mr) dot += u[t] * X[i][t];
+      dot = sum8(dot) * tj;
+      if (sub == 0) Qm[(size_t)j * rows + c] = top - dot;  // Q1[c][j]
+#pragma unroll
+      for (int t = 0; t < K; ++t)
+        if (sub + 8 * t < mr) X[i][t] -= dot * u[t];
+    }
+  }
+}
+
 template <int K>
 __device__ __attribute__((always_inline)) void stacked_qr_cols(double* W, int rows, int q, int mr,
-                                                               double* tau) {
+                                                               double* tau, double* Qm) {
   const int tid = threadIdx.x;
   const int ng = (int)blockDim.x >> 3, cg = tid >> 3, sub = tid & 7;
+  double X[2][K];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int t = 0; t < K; ++t) X[i][t] = 0.0;
   double E[2][K];
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
@@ -1063,8 +1095,21 @@ __device__ __attribute__((always_inline)) void stacked_qr_cols(double* W, int ro
         if (c == j + 1) qr_cols_reflector<K>(E[i], W, rows, q, mr, j + 1, tau, sub);
       }
     }
+    qr_cols_apply_x<K>(X, u, tj, j, q, mr, rows, cg, ng, sub, Qm);
     __syncthreads();
   }
+  {
+    const int j = q - 1;
+    const double tj = tau[j];
+    double u[K];
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+      const int k = sub + 8 * t;
+      u[t] = k < mr ? W[(size_t)j * rows + q + k] : 0.0;
+    }
+    qr_cols_apply_x<K>(X, u, tj, j, q, mr, rows, cg, ng, sub, Qm);
+  }
+  __syncthreads();
 }
 
 // R^-1 column by column (as upper_tri_inverse: for k = q-1 .. 1 every row
@@ -1129,13 +1174,15 @@ __device__ __forceinline__ int regs_width(int q, int mr, int kmax) {
   return w <= kmax ? w : 0;
 }
 
-__device__ __attribute__((always_inline)) void stacked_qr_fast(double* W, int rows, int q, int mr,
-                                                               double* tau) {
+// Structured QR; returns true when Q1 (top q rows, ld rows) was formed in Qm
+// along the way (register forms), false when the caller must form it.
+__device__ __attribute__((always_inline)) bool stacked_qr_fast(double* W, int rows, int q, int mr,
+                                                               double* tau, double* Qm) {
   switch (regs_width(q, mr, 6)) {
-    case 2: stacked_qr_cols<2>(W, rows, q, mr, tau); break;
-    case 4: stacked_qr_cols<4>(W, rows, q, mr, tau); break;
-    case 6: stacked_qr_cols<6>(W, rows, q, mr, tau); break;
-    default: stacked_qr(W, rows, q, mr, tau);
+    case 2: stacked_qr_cols<2>(W, rows, q, mr, tau, Qm); return true;
+    case 4: stacked_qr_cols<4>(W, rows, q, mr, tau, Qm); return true;
+    case 6: stacked_qr_cols<6>(W, rows, q, mr, tau, Qm); return true;
+    default: stacked_qr(W, rows, q, mr, tau); return false;
   }
 }
 
@@ -1266,7 +1313,7 @@ __device__ __attribute__((always_inline)) void magic_eval(const MagicLds& L, con
     build_augmented(L, D, DL, q, erows, agg);
   }
   const long long tq0 = wall_clock64();
-  stacked_qr_fast(L.aug, rows, q, erows, L.wk);
+  const bool q1_done = stacked_qr_fast(L.aug, rows, q, erows, L.wk, L.Qm);
   const long long tq1 = wall_clock64();
   if (!qr_ok) {
     // stage 2 from the factor itself: 1 / (|R2|_F |R2^-1|_F)
@@ -1287,7 +1334,7 @@ __device__ __attribute__((always_inline)) void magic_eval(const MagicLds& L, con
     if (qr_ok && tid == 0) L.ictl[15] = 1;  // R2 inverted already
   }
   const long long tq2 = wall_clock64();
-  if (qr_ok) stacked_q1_fast(L.aug, rows, q, erows, L.wk, L.Qm);
+  if (qr_ok && !q1_done) stacked_q1_fast(L.aug, rows, q, erows, L.wk, L.Qm);
   const long long tq3 = wall_clock64();
   if (tid == 0) {
     L.dctl[8] += (double)(tq0 - te0);   // penalty scaling + augmented build
@@ -1323,7 +1370,7 @@ __device__ __attribute__((always_inline)) void magic_eval(const MagicLds& L, con
     // q + rank and the root is not rebuilt.
     const int rank = erows;
     tp1 = wall_clock64();
-    stacked_q1_fast(L.aug, rows, q, erows, L.wk, L.Qm);
+    if (!q1_done) stacked_q1_fast(L.aug, rows, q, erows, L.wk, L.Qm);
     for (int e = tid; e < q * q; e += blockDim.x) {
       const int r = e % q, c = e / q;
       L.X4[e] = r <= c ? L.aug[(size_t)c * rows + r] : 0.0;
