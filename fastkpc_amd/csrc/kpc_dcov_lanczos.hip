@@ -30,9 +30,16 @@ namespace kpc {
 
 constexpr int kLzThreads = 512;
 
-// Doubles of LDS for the twisted-factorisation pivots (2 m per eigenpair).
-__host__ __device__ inline int lanczos_ivw(int mmax) {
-  return 4096 > 6 * mmax + 64 ? 4096 : 6 * mmax + 64;
+// Doubles of LDS for the twisted-factorisation pivots (2 m per eigenpair):
+// enough for all k eigenpairs of the first convergence check (m = 2k + 6) in
+// one batch while two workgroups still share a CU's 160 KB of LDS.
+__host__ __device__ inline int lanczos_ivw(int n, int mmax, int k) {
+  const int nl = (n + 63) & ~63;
+  const int ml = (mmax + 64) & ~63;
+  const int base = 4096 > 6 * mmax + 64 ? 4096 : 6 * mmax + 64;
+  const int room = (81920 - (3 * kMaxNumCol + 8) * 4) / 8 - (4 * nl + 4 * ml + 16 + 8 * kMaxNumCol);
+  const int want = 2 * (2 * k + 6) * k < room ? 2 * (2 * k + 6) * k : room;
+  return base > want ? base : want;
 }
 
 // Top-kk candidates by |value| (ties toward the higher ascending index): a
@@ -58,27 +65,32 @@ __device__ void lz_select(const double* cval, int ncand, int nlow, int hs, int k
 }
 
 // Z[:, e] = Q S[:, e] for e < k (n x m basis, coefficient vectors at stride
-// sst): a thread takes one row and eight vectors, so every basis entry is
-// loaded once per group; the coefficient sums run over the basis in order.
+// sst): a thread takes one row and up to kRitzGroup vectors, so every basis
+// entry is loaded once per group (at k = 35, two groups: two passes of the
+// block over the rows); the basis loop is unrolled sixteen deep so the loads
+// overlap, and each coefficient sum runs over the basis in order.
+constexpr int kRitzGroup = 18;
 __device__ inline void ritz_assemble(const double* __restrict__ Q, const double* Sv, int sst,
                                      int n, int m, int k, const int* perm,
                                      double* __restrict__ Z) {
-  const int ng = (k + 7) / 8;
+  const int ng = (k + kRitzGroup - 1) / kRitzGroup;
   for (int u = threadIdx.x; u < n * ng; u += blockDim.x) {
     const int i = u % n, g = u / n;
-    const int e0 = 8 * g;
-    const int ne = min(8, k - e0);
+    const int e0 = kRitzGroup * g;
+    const int ne = min(kRitzGroup, k - e0);
     const double* sv = Sv + (size_t)e0 * sst;
-    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
+    double acc[kRitzGroup];
+#pragma unroll
+    for (int c = 0; c < kRitzGroup; ++c) acc[c] = 0.0;
+#pragma unroll 16
     for (int b = 0; b < m; ++b) {
       const double qb = Q[(size_t)b * n + i];
 #pragma unroll
-      for (int c = 0; c < 8; ++c)
+      for (int c = 0; c < kRitzGroup; ++c)
         if (c < ne) acc[c] += qb * sv[(size_t)c * sst + b];
     }
 #pragma unroll
-    for (int c = 0; c < 8; ++c)
+    for (int c = 0; c < kRitzGroup; ++c)
       if (c < ne) Z[(size_t)(e0 + c) * n + perm[i]] = acc[c];
   }
 }
@@ -109,7 +121,7 @@ __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
   double* clo = reinterpret_cast<double*>(ictl + 8);      // 2 kMaxNumCol brackets
   double* chi = clo + 2 * kMaxNumCol;                     // 2 kMaxNumCol
   double* ivl = chi + 2 * kMaxNumCol;                     // inverse-iteration workspace
-  const int ivw = lanczos_ivw(mmax);
+  const int ivw = lanczos_ivw(n, mmax, k);
   int* perm = reinterpret_cast<int*>(ivl + ivw);          // n: sorted position -> row
   const int tid = threadIdx.x;
   const int lane = tid & (KPC_WAVE - 1);
@@ -376,7 +388,9 @@ __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
         __syncthreads();
       }
       // eigenvectors of T by twisted factorisations (pivot workspace in LDS,
-      // batches of eigenpairs), exact Ritz residuals |beta_m s_m|
+      // batches of eigenpairs), exact Ritz residuals |beta_m s_m|; the
+      // coefficient vectors go to S at every check, so the last check's are
+      // the final ones
       {
         const int per = 2 * m;
         const int B = ivw / per > 0 ? ivw / per : 1;
@@ -384,7 +398,8 @@ __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
           const int c = c0 + tid;
           if (tid < B && c < kk) {
             double* wsl = ivl + (size_t)tid * per;
-            const double last = lz_twisted(al, be, m, sval[c], pivmin, wsl, wsl + m, nullptr);
+            const double last =
+                lz_twisted(al, be, m, sval[c], pivmin, wsl, wsl + m, S + (size_t)c * mmax);
             sres[c] = fabs(bj * last);
           }
           __syncthreads();
@@ -411,19 +426,6 @@ __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
         ictl[3] = m + gap;
       }
       __syncthreads();
-      if (ictl[0]) {
-        // final Ritz coefficient vectors into S
-        const int per = 2 * m;
-        const int B = ivw / per > 0 ? ivw / per : 1;
-        for (int c0 = 0; c0 < kk; c0 += B) {
-          const int c = c0 + tid;
-          if (tid < B && c < kk) {
-            double* wsl = ivl + (size_t)tid * per;
-            lz_twisted(al, be, m, sval[c], pivmin, wsl, wsl + m, S + (size_t)c * mmax);
-          }
-          __syncthreads();
-        }
-      }
       if (ictl[0]) { m_final = m; break; }
       prev_m = m;
       prev_worst = red[12];
@@ -513,10 +515,10 @@ __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
 
 int lanczos_mmax(int n) { return n < 1024 ? n : 1024; }
 
-size_t dcov_lanczos_lds_bytes(int n, int mmax) {
+size_t dcov_lanczos_lds_bytes(int n, int k, int mmax) {
   const int nl = (n + 63) & ~63;
   const int ml = (mmax + 64) & ~63;
-  return (size_t)(4 * nl + 4 * ml + 16 + 4 * kMaxNumCol + 4 * kMaxNumCol + lanczos_ivw(mmax)) *
+  return (size_t)(4 * nl + 4 * ml + 16 + 4 * kMaxNumCol + 4 * kMaxNumCol + lanczos_ivw(n, mmax, k)) *
              sizeof(double) +
          (size_t)(3 * kMaxNumCol + 8) * sizeof(int);
 }
@@ -532,7 +534,7 @@ hipError_t launch_dcov_lanczos(const double* src_base, const long long* src_off,
                                double* scratch, int grid, int mmax, double tol,
                                hipStream_t stream) {
   if (count <= 0) return hipSuccess;
-  const size_t lds = dcov_lanczos_lds_bytes(n, mmax);
+  const size_t lds = dcov_lanczos_lds_bytes(n, k, mmax);
   hipError_t err = hipFuncSetAttribute((const void*)dcov_lanczos_kernel,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (err != hipSuccess) return err;

@@ -79,7 +79,7 @@ hipError_t launch_gcv_eval(const MagicItem* items, int count, int n, int q, int 
 }  // namespace kpc
 
 namespace kpc {
-size_t dcov_lanczos_lds_bytes(int n, int mmax);
+size_t dcov_lanczos_lds_bytes(int n, int k, int mmax);
 long long dcov_lanczos_scratch_doubles(int n, int k, int mmax);
 hipError_t launch_dcov_lanczos(const double* src_base, const long long* src_off,
                                const int* dst_slot, int count, int n, int k, double* vals,

@@ -67,22 +67,33 @@ __device__ inline double lz_twisted(const double* al, const double* be, int m, d
     }
   }
   double n0 = 1.0, n1 = 0.0, x = 1.0, last = kt == m - 1 ? 1.0 : 0.0;
-  if (z) z[kt] = 1.0;
   for (int i = kt - 1; i >= 0; --i) {
     x = -be[i] * x * rd[i];
-    if (z) z[i] = x;
     n1 += x * x;
   }
   x = 1.0;
   for (int i = kt; i < m - 1; ++i) {
     x = -be[i] * x * rr[i + 1];
-    if (z) z[i + 1] = x;
     n0 += x * x;
   }
   if (kt < m - 1) last = x;
   const double inv = 1.0 / sqrt(n0 + n1);
-  if (z)
-    for (int i = 0; i < m; ++i) z[i] *= inv;
+  if (z) {
+    // the two recurrences again, scaled on the way out (same products, so the
+    // same values as storing them first and scaling in place, without reading
+    // z back)
+    z[kt] = inv;
+    x = 1.0;
+    for (int i = kt - 1; i >= 0; --i) {
+      x = -be[i] * x * rd[i];
+      z[i] = x * inv;
+    }
+    x = 1.0;
+    for (int i = kt; i < m - 1; ++i) {
+      x = -be[i] * x * rr[i + 1];
+      z[i + 1] = x * inv;
+    }
+  }
   return last * inv;
 }
 
