@@ -301,8 +301,9 @@ double now_s() {
 struct EngineHipResources {
   int device = -1;
   bool in_use = false;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr, stream2 = nullptr;
   hipEvent_t ev[11] = {};
+  hipEvent_t ev2[6] = {};
 };
 thread_local std::deque<EngineHipResources>* g_engine_res = nullptr;
 
@@ -318,7 +319,9 @@ EngineHipResources* acquire_engine_resources() {
   EngineHipResources r;
   r.device = dev;
   hip_check(hipStreamCreate(&r.stream), "hipStreamCreate");
+  hip_check(hipStreamCreate(&r.stream2), "hipStreamCreate");
   for (int e = 0; e < 11; ++e) hip_check(hipEventCreate(&r.ev[e]), "hipEventCreate");
+  for (int e = 0; e < 6; ++e) hip_check(hipEventCreate(&r.ev2[e]), "hipEventCreate");
   r.in_use = true;
   g_engine_res->push_back(r);
   return &g_engine_res->back();
@@ -376,14 +379,18 @@ class Engine {
     }
     res_ = acquire_engine_resources();
     stream_ = res_->stream;
+    stream2_ = res_->stream2;
     for (int e = 0; e < 8; ++e) ev_[e] = res_->ev[e];
     for (int e = 0; e < 3; ++e) tev_[e] = res_->ev[8 + e];
+    for (int e = 0; e < 6; ++e) ev2_[e] = res_->ev2[e];
+    if (const char* env = std::getenv("KPC_LANES")) lanes_ = std::atoi(env) == 1 ? 1 : 2;
   }
   Engine(const Engine&) = delete;
   Engine& operator=(const Engine&) = delete;
   ~Engine() {
     // arenas go back to the pool after this body: nothing may still use them
     if (stream_) (void)hipStreamSynchronize(stream_);
+    if (stream2_) (void)hipStreamSynchronize(stream2_);
     if (d_data_) pool_release(d_data_, data_bytes_);
     if (res_) res_->in_use = false;  // stream and events stay with the thread
   }
@@ -465,19 +472,53 @@ class Engine {
       tasks.push_back(t);
     }
     if (tasks.empty()) return;
-    std::vector<int> smooth_items;     // d, c0, c1, slot
-    std::vector<kpc::GeometryItem> geo_items;
-    std::vector<kpc::MagicItem> fit_items;
-    std::vector<long long> comp_src;
-    std::vector<int> comp_dst;
-    std::vector<int> pair_x, pair_y;
-    std::vector<int> fit_x, fit_y;  // residual slot of each task's fits (-1: raw column)
+    const bool perm = method_ == KPC_DCC_PERM || method_ == KPC_HSIC_PERM;
+    // KPC_GRAPH=1: the wave's device pipeline (smooth setups .. pair
+    // statistics) is captured into one hipGraph and launched as a unit; the
+    // permutation methods synchronise between replicate chunks and stay eager
+    const bool graph = use_graph_ && !perm;
+    // distinct conditioning sets alternate between the two lanes
+    std::vector<int> lane_of(tasks.size(), 0);
+    int nlanes = 1;
+    if (lanes_ > 1 && !graph && !perm && level_ > 0) {
+      lane_map_.clear();
+      int distinct = 0;
+      for (size_t i = 0; i < tasks.size(); ++i) {
+        if (const int* l = lane_map_.find(tasks[i]->mask)) {
+          lane_of[i] = *l;
+          continue;
+        }
+        lane_of[i] = distinct & 1;
+        lane_map_.set(tasks[i]->mask, distinct & 1);
+        ++distinct;
+      }
+      if (distinct > 1) nlanes = 2;
+    }
+    struct LaneWork {
+      std::vector<int> smooth_items;  // d, c0, c1, slot
+      std::vector<kpc::GeometryItem> geo_items;
+      std::vector<kpc::MagicItem> fit_items;
+      std::vector<long long> comp_src;
+      std::vector<int> comp_dst;
+      std::vector<int> pair_x, pair_y;
+      std::vector<int> fit_x, fit_y;  // residual slot of each task's fits (-1: raw column)
+      std::vector<size_t> task_idx;
+      std::vector<double> out;
+      int nsm = 0;
+    };
+    LaneWork W[2];
+    // above level 2 the smooths are univariate and shared between sets: they
+    // run on lane 0 first and lane 1 waits for them
+    const bool uni = level_ > 2;
+    std::vector<int> shared_smooth;
     std::vector<unsigned long long> seeds;
-    for (Task* t : tasks) {
+    for (size_t ti = 0; ti < tasks.size(); ++ti) {
+      Task* t = tasks[ti];
+      LaneWork& w = W[lane_of[ti]];
+      std::vector<int>& smooth_items = uni ? shared_smooth : w.smooth_items;
       int slots[2];
       int rslot[2] = {-1, -1};
-      if (method_ == KPC_DCC_PERM || method_ == KPC_HSIC_PERM)
-        seeds.push_back(task_seed(seed_, method_, t->x, t->y, t->S, t->slen));
+      if (perm) seeds.push_back(task_seed(seed_, method_, t->x, t->y, t->S, t->slen));
       for (int side = 0; side < 2; ++side) {
         // side 0: min(x, y), side 1: max(x, y) (the permuted variable)
         const int tgt = side ? std::max(t->x, t->y) : std::min(t->x, t->y);
@@ -490,12 +531,12 @@ class Engine {
         const int cs = n_comp_++;
         comp_map_.set(key, cs);
         slots[side] = cs;
-        comp_dst.push_back(cs);
+        w.comp_dst.push_back(cs);
         if (level_ == 0) {
-          comp_src.push_back((long long)tgt * n_);
+          w.comp_src.push_back((long long)tgt * n_);
           continue;
         }
-        int dslot = ensure_design(*t, smooth_items, geo_items);
+        int dslot = ensure_design(*t, smooth_items, w.geo_items);
         const int rs = n_resid_++;
         resid_map_.set(key, rs);
         rslot[side] = rs;
@@ -504,113 +545,147 @@ class Engine {
         mi.target = tgt;
         mi.resid_slot = rs;
         mi.pad = 0;
-        fit_items.push_back(mi);
-        comp_src.push_back((long long)rs * n_);
+        w.fit_items.push_back(mi);
+        w.comp_src.push_back((long long)rs * n_);
       }
-      pair_x.push_back(slots[0]);
-      pair_y.push_back(slots[1]);
-      fit_x.push_back(rslot[0]);
-      fit_y.push_back(rslot[1]);
+      w.pair_x.push_back(slots[0]);
+      w.pair_y.push_back(slots[1]);
+      w.fit_x.push_back(rslot[0]);
+      w.fit_y.push_back(rslot[1]);
+      w.task_idx.push_back(ti);
+    }
+    if (uni) W[0].smooth_items.swap(shared_smooth);
+    for (int li = 0; li < nlanes; ++li) {
+      W[li].nsm = (int)W[li].smooth_items.size() / 4;
+      stats_.fits += (long long)W[li].fit_items.size();
+      stats_.components += (long long)W[li].comp_dst.size();
+      stats_.setups += (long long)W[li].geo_items.size();
     }
     stats_.physical_tests += (long long)tasks.size();
-    stats_.fits += (long long)fit_items.size();
-    stats_.components += (long long)comp_dst.size();
-    stats_.setups += (long long)geo_items.size();
-    // uploads
-    d_smooth_items_.upload(smooth_items, stream_);
-    d_geo_items_.upload(geo_items, stream_);
-    d_fit_items_.upload(fit_items, stream_);
-    d_comp_src_.upload(comp_src, stream_);
-    d_comp_dst_.upload(comp_dst, stream_);
-    d_pair_x_.upload(pair_x, stream_);
-    d_pair_y_.upload(pair_y, stream_);
-    d_seeds_.upload(seeds, stream_);
-    d_pair_out_.reserve(tasks.size() * kpc::kPairOut);
     stage_host_ += now_s() - t0;
-    hip_check(hipEventRecord(ev_[0], stream_), "event");
-    // KPC_GRAPH=1: the wave's device pipeline (smooth setups .. pair
-    // statistics) is captured into one hipGraph and launched as a unit; the
-    // permutation methods synchronise between replicate chunks and stay eager
-    const bool graph = use_graph_ && method_ != KPC_DCC_PERM && method_ != KPC_HSIC_PERM;
-    hipGraph_t wave_graph = nullptr;
-    if (graph)
-      hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed), "begin capture");
-    const int nsm = (int)smooth_items.size() / 4;
-    if (nsm > 0) {
-      const long long stride = kpc::smooth_scratch_doubles(n_);
-      int grid = std::min(nsm, grid_limit(stride, 512));
-      scratch_.reserve((size_t)grid * stride);
-      const bool uni = level_ > 2;
-      hip_check(kpc::launch_smooth_setup(d_data_, n_, p_, d_smooth_items_.ptr, nsm,
-                                         uni ? usmooth_X_.ptr : lsmooth_X_.ptr,
-                                         uni ? usmooth_S_.ptr : lsmooth_S_.ptr,
-                                         uni ? usmooth_info_.ptr : lsmooth_info_.ptr,
-                                         scratch_.ptr, grid, stream_),
-                "smooth setup launch");
-      pending_smooth_checks_ = true;
+    const double* comp_source = level_ == 0 ? d_data_ : resid_.ptr;
+    // uploads from pageable memory first, while no lane has work queued
+    for (int li = 0; li < nlanes; ++li) {
+      const Lane L = lane(li);
+      const LaneWork& w = W[li];
+      L.smooth_items->upload(w.smooth_items, L.stream);
+      L.geo_items->upload(w.geo_items, L.stream);
+      L.fit_items->upload(w.fit_items, L.stream);
+      L.comp_src->upload(w.comp_src, L.stream);
+      L.comp_dst->upload(w.comp_dst, L.stream);
+      L.pair_x->upload(w.pair_x, L.stream);
+      L.pair_y->upload(w.pair_y, L.stream);
+      if (li == 0) d_seeds_.upload(seeds, stream_);
+      L.pair_out->reserve(w.pair_x.size() * kpc::kPairOut);
     }
-    if (!graph) hip_check(hipEventRecord(ev_[1], stream_), "event");
-    if (!geo_items.empty()) {
-      const long long stride = kpc::geometry_scratch_doubles(q_);
-      int grid = std::min((int)geo_items.size(), grid_limit(stride, 2048));
-      scratch2_.reserve((size_t)grid * stride);
-      const bool uni = level_ > 2;
-      hip_check(kpc::launch_geometry(d_geo_items_.ptr, (int)geo_items.size(), n_, q_, m_, dim_, rk_,
-                                     uni ? usmooth_X_.ptr : lsmooth_X_.ptr,
-                                     uni ? usmooth_S_.ptr : lsmooth_S_.ptr, design_.ptr,
-                                     design_status_.ptr, (long long)n_ * 29, scratch2_.ptr, grid,
-                                     stream_),
-                "geometry launch");
+    for (int li = 0; li < nlanes; ++li) {
+      const Lane L = lane(li);
+      LaneWork& w = W[li];
+      hip_check(hipEventRecord(L.ev[0], L.stream), "event");
+      hipGraph_t wave_graph = nullptr;
+      if (graph)
+        hip_check(hipStreamBeginCapture(L.stream, hipStreamCaptureModeRelaxed), "begin capture");
+      if (w.nsm > 0) {
+        const long long stride = kpc::smooth_scratch_doubles(n_);
+        int grid = std::min(w.nsm, grid_limit(stride, 512));
+        L.scratch->reserve((size_t)grid * stride);
+        hip_check(kpc::launch_smooth_setup(d_data_, n_, p_, L.smooth_items->ptr, w.nsm,
+                                           uni ? usmooth_X_.ptr : lsmooth_X_.ptr,
+                                           uni ? usmooth_S_.ptr : lsmooth_S_.ptr,
+                                           uni ? usmooth_info_.ptr : lsmooth_info_.ptr,
+                                           L.scratch->ptr, grid, L.stream),
+                  "smooth setup launch");
+        pending_smooth_checks_ = true;
+      }
+      if (uni && nlanes > 1) {
+        if (li == 0) hip_check(hipEventRecord(ev_[6], stream_), "event");
+        else hip_check(hipStreamWaitEvent(L.stream, ev_[6], 0), "stream wait");
+      }
+      if (!graph) hip_check(hipEventRecord(L.ev[1], L.stream), "event");
+      if (!w.geo_items.empty()) {
+        const long long stride = kpc::geometry_scratch_doubles(q_);
+        int grid = std::min((int)w.geo_items.size(), grid_limit(stride, 2048));
+        L.scratch2->reserve((size_t)grid * stride);
+        hip_check(kpc::launch_geometry(L.geo_items->ptr, (int)w.geo_items.size(), n_, q_, m_, dim_,
+                                       rk_, uni ? usmooth_X_.ptr : lsmooth_X_.ptr,
+                                       uni ? usmooth_S_.ptr : lsmooth_S_.ptr, design_.ptr,
+                                       design_status_.ptr, (long long)n_ * 29, L.scratch2->ptr,
+                                       grid, L.stream),
+                  "geometry launch");
+      }
+      if (!graph) hip_check(hipEventRecord(L.ev[2], L.stream), "event");
+      if (!w.fit_items.empty()) {
+        const long long stride = kpc::magic_scratch_doubles(q_, m_, rk_, n_);
+        int grid = std::min((int)w.fit_items.size(), grid_limit(stride, 65535));
+        L.scratch3->reserve((size_t)grid * stride);
+        hip_check(kpc::launch_magic(L.fit_items->ptr, (int)w.fit_items.size(), n_, q_, m_, rk_,
+                                    d_data_, design_.ptr, resid_.ptr, rinfo_.ptr,
+                                    L.scratch3->ptr, grid, ctl_, L.stream),
+                  "magic launch");
+      }
+      if (!graph) hip_check(hipEventRecord(L.ev[3], L.stream), "event");
+      if (!w.comp_dst.empty()) launch_components(L, comp_source, w.comp_dst.size());
+      if (!graph) hip_check(hipEventRecord(L.ev[4], L.stream), "event");
+      launch_pairs(L, (int)w.pair_x.size());
+      if (graph) {
+        hip_check(hipStreamEndCapture(L.stream, &wave_graph), "end capture");
+        hipGraphExec_t exec = nullptr;
+        hip_check(hipGraphInstantiate(&exec, wave_graph, nullptr, nullptr, 0), "graph instantiate");
+        hip_check(hipGraphLaunch(exec, L.stream), "graph launch");
+        hip_check(hipEventRecord(L.ev[5], L.stream), "event");
+        hip_check(hipStreamSynchronize(L.stream), "graph synchronize");
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(wave_graph);
+      } else {
+        hip_check(hipEventRecord(L.ev[5], L.stream), "event");
+      }
     }
-    if (!graph) hip_check(hipEventRecord(ev_[2], stream_), "event");
-    if (!fit_items.empty()) {
-      const long long stride = kpc::magic_scratch_doubles(q_, m_, rk_, n_);
-      int grid = std::min((int)fit_items.size(), grid_limit(stride, 65535));
-      scratch3_.reserve((size_t)grid * stride);
-      hip_check(kpc::launch_magic(d_fit_items_.ptr, (int)fit_items.size(), n_, q_, m_, rk_,
-                                  d_data_, design_.ptr, resid_.ptr, rinfo_.ptr, scratch3_.ptr,
-                                  grid, ctl_, stream_),
-                "magic launch");
+    // copies into pageable memory return only when done, so they follow every
+    // lane's launches
+    for (int li = 0; li < nlanes; ++li) {
+      const Lane L = lane(li);
+      LaneWork& w = W[li];
+      w.out.resize(w.pair_x.size() * kpc::kPairOut);
+      hip_check(hipMemcpyAsync(w.out.data(), L.pair_out->ptr, sizeof(double) * w.out.size(),
+                               hipMemcpyDeviceToHost, L.stream),
+                "D2H p-values");
     }
-    if (!graph) hip_check(hipEventRecord(ev_[3], stream_), "event");
-    if (!comp_dst.empty()) launch_components(level_ == 0 ? d_data_ : resid_.ptr, comp_dst.size());
-    if (!graph) hip_check(hipEventRecord(ev_[4], stream_), "event");
-    launch_pairs((int)tasks.size());
-    if (graph) {
-      hip_check(hipStreamEndCapture(stream_, &wave_graph), "end capture");
-      hipGraphExec_t exec = nullptr;
-      hip_check(hipGraphInstantiate(&exec, wave_graph, nullptr, nullptr, 0), "graph instantiate");
-      hip_check(hipGraphLaunch(exec, stream_), "graph launch");
-      hip_check(hipEventRecord(ev_[5], stream_), "event");
-      hip_check(hipStreamSynchronize(stream_), "graph synchronize");
-      (void)hipGraphExecDestroy(exec);
-      (void)hipGraphDestroy(wave_graph);
-    } else {
-      hip_check(hipEventRecord(ev_[5], stream_), "event");
-    }
-    std::vector<double> out(tasks.size() * kpc::kPairOut);
-    hip_check(hipMemcpyAsync(out.data(), d_pair_out_.ptr, sizeof(double) * out.size(),
-                             hipMemcpyDeviceToHost, stream_),
-              "D2H p-values");
-    hip_check(hipStreamSynchronize(stream_), "wave synchronize");
+    for (int li = 0; li < nlanes; ++li)
+      hip_check(hipStreamSynchronize(lane(li).stream), "wave synchronize");
+    // stage times are summed over the lanes (they overlap in wall time)
     float ms;
-    const bool launched[5] = {nsm > 0, !geo_items.empty(), !fit_items.empty(), !comp_dst.empty(),
-                              true};
-    if (graph) {
-      // one graph per wave: only the whole pipeline is timed
-      hip_check(hipEventElapsedTime(&ms, ev_[0], ev_[5]), "event elapsed");
-      graph_seconds_ += ms * 1e-3;
-      ++graph_launches_;
+    for (int li = 0; li < nlanes; ++li) {
+      const Lane L = lane(li);
+      const LaneWork& w = W[li];
+      const bool launched[5] = {w.nsm > 0, !w.geo_items.empty(), !w.fit_items.empty(),
+                                !w.comp_dst.empty(), true};
+      if (graph) {
+        // one graph per wave: only the whole pipeline is timed
+        hip_check(hipEventElapsedTime(&ms, L.ev[0], L.ev[5]), "event elapsed");
+        graph_seconds_ += ms * 1e-3;
+        ++graph_launches_;
+      }
+      for (int s = 0; s < 5 && !graph; ++s) {
+        hip_check(hipEventElapsedTime(&ms, L.ev[s], L.ev[s + 1]), "event elapsed");
+        stage_[s] += ms * 1e-3;
+      }
+      for (int s = 0; s < 5; ++s)
+        if (launched[s]) ++launches_[s];
     }
-    for (int s = 0; s < 5 && !graph; ++s) {
-      hip_check(hipEventElapsedTime(&ms, ev_[s], ev_[s + 1]), "event elapsed");
-      stage_[s] += ms * 1e-3;
-    }
-    for (int s = 0; s < 5; ++s)
-      if (launched[s]) ++launches_[s];
     std::vector<double> rinfo_h;
-    check_device_status(fit_items, geo_items, nsm, &rinfo_h);
-    check_component_status(comp_dst);
+    if (nlanes == 1) {
+      check_device_status(W[0].fit_items, W[0].geo_items, W[0].nsm, &rinfo_h);
+      check_component_status(W[0].comp_dst);
+    } else {
+      std::vector<kpc::MagicItem> fits(W[0].fit_items);
+      fits.insert(fits.end(), W[1].fit_items.begin(), W[1].fit_items.end());
+      std::vector<kpc::GeometryItem> geos(W[0].geo_items);
+      geos.insert(geos.end(), W[1].geo_items.begin(), W[1].geo_items.end());
+      std::vector<int> dst(W[0].comp_dst);
+      dst.insert(dst.end(), W[1].comp_dst.begin(), W[1].comp_dst.end());
+      check_device_status(fits, geos, W[0].nsm + W[1].nsm, &rinfo_h);
+      check_component_status(dst);
+    }
     std::vector<double> cmargin_h;
     if (method_ == KPC_HSIC_GAMMA || method_ == KPC_HSIC_PERM) {
       cmargin_h.resize((size_t)n_comp_);
@@ -618,17 +693,22 @@ class Engine {
                           hipMemcpyDeviceToHost),
                 "component margins");
     }
-    for (size_t i = 0; i < tasks.size(); ++i) {
-      const double* o = out.data() + kpc::kPairOut * i;
-      TestResult res{o[0], o[1], o[2], o[3], o[4]};
-      if (std::isnan(res.p)) res.p = 1.0;  // NAdelete = TRUE
-      for (int rs : {fit_x[i], fit_y[i]})
-        if (rs >= 0)
-          res.margin = std::min(res.margin, rinfo_h[(size_t)rs * kpc::kMagicInfo + kpc::kMagicInfo - 2]);
-      if (!cmargin_h.empty())
-        res.margin = std::min(res.margin, std::min(cmargin_h[pair_x[i]], cmargin_h[pair_y[i]]));
-      apply_result(*tasks[i], res);
-      twin_map_.emplace(twin_key(*tasks[i]), res);
+    for (int li = 0; li < nlanes; ++li) {
+      const LaneWork& w = W[li];
+      for (size_t j = 0; j < w.task_idx.size(); ++j) {
+        Task* t = tasks[w.task_idx[j]];
+        const double* o = w.out.data() + kpc::kPairOut * j;
+        TestResult res{o[0], o[1], o[2], o[3], o[4]};
+        if (std::isnan(res.p)) res.p = 1.0;  // NAdelete = TRUE
+        for (int rs : {w.fit_x[j], w.fit_y[j]})
+          if (rs >= 0)
+            res.margin = std::min(res.margin,
+                                  rinfo_h[(size_t)rs * kpc::kMagicInfo + kpc::kMagicInfo - 2]);
+        if (!cmargin_h.empty())
+          res.margin = std::min(res.margin, std::min(cmargin_h[w.pair_x[j]], cmargin_h[w.pair_y[j]]));
+        apply_result(*t, res);
+        twin_map_.emplace(twin_key(*t), res);
+      }
     }
     for (auto& d : dups) apply_result(*d.first, *twin_map_.find(twin_key(*tasks[d.second])));
   }
@@ -647,19 +727,51 @@ class Engine {
     t.computed = true;
   }
 
-  void launch_pairs(int count) {
+  // One launch lane: a stream, its six stage events, and the work lists and
+  // scratch slabs its kernels use.  Lane 0 is the engine's own stream; tests
+  // split by conditioning set run on lanes 0 and 1 at once (their designs,
+  // fits, components and tests are disjoint), so the last setups or the
+  // slowest fit of one lane overlap the other lane's later stages.
+  struct Lane {
+    hipStream_t stream;
+    hipEvent_t* ev;
+    DevArray<int>* smooth_items;
+    DevArray<kpc::GeometryItem>* geo_items;
+    DevArray<kpc::MagicItem>* fit_items;
+    DevArray<long long>* comp_src;
+    DevArray<int>* comp_dst;
+    DevArray<int>* pair_x;
+    DevArray<int>* pair_y;
+    DevArray<double>* pair_out;
+    DevArray<double>* scratch;   // smooth setups
+    DevArray<double>* scratch2;  // geometry
+    DevArray<double>* scratch3;  // GCV fits
+    DevArray<double>* scratch4;  // components
+  };
+  Lane lane(int i) {
+    if (i == 0)
+      return Lane{stream_,        ev_,          &d_smooth_items_, &d_geo_items_, &d_fit_items_,
+                  &d_comp_src_,   &d_comp_dst_, &d_pair_x_,       &d_pair_y_,    &d_pair_out_,
+                  &scratch_,      &scratch2_,   &scratch3_,       &scratch4_};
+    return Lane{stream2_,        ev2_,          &l1_smooth_items_, &l1_geo_items_, &l1_fit_items_,
+                &l1_comp_src_,   &l1_comp_dst_, &l1_pair_x_,       &l1_pair_y_,    &l1_pair_out_,
+                &l1_scratch_,    &l1_scratch2_, &l1_scratch3_,     &l1_scratch4_};
+  }
+
+  void launch_pairs(int count) { launch_pairs(lane(0), count); }
+  void launch_pairs(const Lane& L, int count) {
     const int grid = std::min(count, 65535);
     switch (method_) {
       case KPC_DCC_GAMMA:
-        hip_check(kpc::launch_dcov_pairs(d_pair_x_.ptr, d_pair_y_.ptr, count, n_, k_, comp_vals_.ptr,
-                                         comp_vecs_.ptr, comp_dsum_.ptr, comp_mom_.ptr,
-                                         d_pair_out_.ptr, grid, stream_),
+        hip_check(kpc::launch_dcov_pairs(L.pair_x->ptr, L.pair_y->ptr, count, n_, k_,
+                                         comp_vals_.ptr, comp_vecs_.ptr, comp_dsum_.ptr,
+                                         comp_mom_.ptr, L.pair_out->ptr, grid, L.stream),
                   "dcov pair launch");
         break;
       case KPC_HSIC_GAMMA:
-        hip_check(kpc::launch_hsic_pairs(d_pair_x_.ptr, d_pair_y_.ptr, count, n_, k_,
+        hip_check(kpc::launch_hsic_pairs(L.pair_x->ptr, L.pair_y->ptr, count, n_, k_,
                                          comp_vecs_.ptr, comp_status_.ptr, comp_dsum_.ptr,
-                                         comp_mom_.ptr, d_pair_out_.ptr, grid, stream_),
+                                         comp_mom_.ptr, L.pair_out->ptr, grid, L.stream),
                   "hsic pair launch");
         break;
       default:
@@ -816,23 +928,24 @@ class Engine {
               "permutation finalize launch");
   }
 
-  void launch_components(const double* src, size_t count) {
+  void launch_components(const Lane& L, const double* src, size_t count) {
+    const long long* csrc = L.comp_src->ptr;
+    const int* cdst = L.comp_dst->ptr;
     if (method_ == KPC_HSIC_GAMMA || method_ == KPC_HSIC_PERM) {
       const int grid = std::min((int)count, 8192);
-      hip_check(kpc::launch_inchol_components(src, d_comp_src_.ptr, d_comp_dst_.ptr, (int)count, n_,
-                                              k_, 1.0 / opt_.hsic_sig, comp_vecs_.ptr,
+      hip_check(kpc::launch_inchol_components(src, csrc, cdst, (int)count, n_, k_,
+                                              1.0 / opt_.hsic_sig, comp_vecs_.ptr,
                                               comp_dsum_.ptr, comp_mom_.ptr, comp_status_.ptr,
                                               comp_margin_.ptr,
                                               method_ == KPC_HSIC_PERM ? comp_rows_.ptr : nullptr,
-                                              16 * hsic_kt(), grid, stream_),
+                                              16 * hsic_kt(), grid, L.stream),
                 "inchol launch");
       return;
     }
     if (method_ == KPC_DCC_PERM) {
       const int grid = std::min((int)count, 8192);
-      hip_check(kpc::launch_dcc_rowmeans(src, d_comp_src_.ptr, d_comp_dst_.ptr, (int)count, n_,
-                                         n_ * comp_k_, index_, comp_vecs_.ptr, comp_dsum_.ptr, grid,
-                                         stream_),
+      hip_check(kpc::launch_dcc_rowmeans(src, csrc, cdst, (int)count, n_, n_ * comp_k_, index_,
+                                         comp_vecs_.ptr, comp_dsum_.ptr, grid, L.stream),
                 "dcc row-mean launch");
       return;
     }
@@ -840,19 +953,19 @@ class Engine {
     if (dense_components_) {
       const long long stride = kpc::dcov_component_scratch_doubles(n_, k_);
       int grid = std::min((int)count, grid_limit(stride, 2048));
-      scratch4_.reserve((size_t)grid * stride);
-      hip_check(kpc::launch_dcov_components(src, d_comp_src_.ptr, d_comp_dst_.ptr, (int)count, n_,
-                                            k_, comp_vals_.ptr, comp_vecs_.ptr, comp_dsum_.ptr,
-                                            comp_mom_.ptr, scratch4_.ptr, grid, stream_),
+      L.scratch4->reserve((size_t)grid * stride);
+      hip_check(kpc::launch_dcov_components(src, csrc, cdst, (int)count, n_, k_, comp_vals_.ptr,
+                                            comp_vecs_.ptr, comp_dsum_.ptr, comp_mom_.ptr,
+                                            L.scratch4->ptr, grid, L.stream),
                 "dcov component launch");
     } else {
       const long long stride = kpc::dcov_lanczos_scratch_doubles(n_, k_, mmax);
       int grid = std::min((int)count, grid_limit(stride, 4096));
-      scratch4_.reserve((size_t)grid * stride);
-      hip_check(kpc::launch_dcov_lanczos(src, d_comp_src_.ptr, d_comp_dst_.ptr, (int)count, n_, k_,
-                                         comp_vals_.ptr, comp_vecs_.ptr, comp_dsum_.ptr,
-                                         comp_mom_.ptr, comp_status_.ptr, scratch4_.ptr, grid, mmax,
-                                         1e-13, stream_),
+      L.scratch4->reserve((size_t)grid * stride);
+      hip_check(kpc::launch_dcov_lanczos(src, csrc, cdst, (int)count, n_, k_, comp_vals_.ptr,
+                                         comp_vecs_.ptr, comp_dsum_.ptr, comp_mom_.ptr,
+                                         comp_status_.ptr, L.scratch4->ptr, grid, mmax, 1e-13,
+                                         L.stream),
                 "dcov lanczos launch");
     }
   }
@@ -1031,7 +1144,8 @@ class Engine {
   uint64_t seed_ = 0;
   kpc_options opt_;
   EngineHipResources* res_ = nullptr;
-  hipStream_t stream_ = nullptr;
+  hipStream_t stream_ = nullptr, stream2_ = nullptr;
+  int lanes_ = 2;
   double* d_data_ = nullptr;
   size_t data_bytes_ = 0;
   hipEvent_t ev_[8];
@@ -1041,6 +1155,7 @@ class Engine {
   DevArray<double> design_status_;  // status word of each design slot (compact copy)
   FlatMap<uint64_t, int, U64Hash> design_map_;
   FlatMap<std::pair<int, uint64_t>, int, KeyHash> resid_map_, comp_map_;
+  FlatMap<uint64_t, int, U64Hash> lane_map_;  // conditioning set -> lane (one wave)
   FlatMap<std::pair<uint64_t, uint64_t>, TestResult, TwinHash> twin_map_;
   FlatMap<std::pair<uint64_t, uint64_t>, size_t, TwinHash> in_batch_;
   std::vector<int> uni_slot_;
@@ -1060,6 +1175,15 @@ class Engine {
   DevArray<long long> d_comp_src_;
   DevArray<int> d_comp_dst_, d_pair_x_, d_pair_y_;
   DevArray<double> d_pair_out_;
+  // the second launch lane's work lists and scratch slabs
+  hipEvent_t ev2_[6];
+  DevArray<int> l1_smooth_items_;
+  DevArray<kpc::GeometryItem> l1_geo_items_;
+  DevArray<kpc::MagicItem> l1_fit_items_;
+  DevArray<long long> l1_comp_src_;
+  DevArray<int> l1_comp_dst_, l1_pair_x_, l1_pair_y_;
+  DevArray<double> l1_pair_out_;
+  DevArray<double> l1_scratch_, l1_scratch2_, l1_scratch3_, l1_scratch4_;
   // permutation tests
   static constexpr long long kPermScratchBytes = 1LL << 30;
   int dcc_perm_jt_ = 16, dcc_perm_rb_ = 8;
