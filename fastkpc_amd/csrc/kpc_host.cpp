@@ -477,10 +477,11 @@ class Engine {
     // statistics) is captured into one hipGraph and launched as a unit; the
     // permutation methods synchronise between replicate chunks and stay eager
     const bool graph = use_graph_ && !perm;
-    // distinct conditioning sets alternate between the two lanes
+    // distinct conditioning sets alternate between the two lanes (waves of
+    // fewer than 64 tests stay on one: a second lane only adds launches there)
     std::vector<int> lane_of(tasks.size(), 0);
     int nlanes = 1;
-    if (lanes_ > 1 && !graph && !perm && level_ > 0) {
+    if (lanes_ > 1 && !graph && !perm && level_ > 0 && tasks.size() >= 64) {
       lane_map_.clear();
       int distinct = 0;
       for (size_t i = 0; i < tasks.size(); ++i) {

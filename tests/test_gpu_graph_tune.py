@@ -3,6 +3,8 @@
 * KPC_GRAPH=1 captures each wave's device pipeline into one hipGraph: the
   skeleton (adjacency, n.edgetests, every trace p-value) must be bit-identical
   to the eager launches, over several graph runs interleaved with eager runs.
+* The default two launch lanes (tests split by conditioning set over two
+  streams) must give the skeleton of one lane (KPC_LANES=1), bit for bit.
 * Every autotuner candidate (dcc.perm replicates per workgroup, hsic.perm
   replicates per unit, permutation generation width) must give bit-identical
   p-values, statistics, means and variances; the tuner's table must hold a
@@ -61,6 +63,27 @@ def test_graph_waves_equal_eager(cfg, method):
         same_skeleton(eager, g)
         again = f.fastkpc_compatible_cuda_skeleton(data, 0.1, options=opts)
         same_skeleton(eager, again)
+
+
+@pytest.mark.parametrize("cfg,method", [("sem351x48", "dcc.gamma"), ("sem200x8", "hsic.gamma"),
+                                        ("sem500x50", "dcc.gamma")])
+def test_two_lanes_equal_one_lane(cfg, method):
+    """A wave's tests split by conditioning set over two streams (the default)
+    give the same skeleton, bit for bit, as one stream (KPC_LANES=1); levels 1-2
+    split per bivariate smooth, levels 3+ share the univariate smooths; waves
+    of fewer than 64 tests stay on one lane."""
+    f = fk()
+    from fastkpc_amd.sem import config_data
+    data, _, _ = config_data(cfg)
+    opts = {"numCol": 35, "ci_method": method}
+    with env(KPC_LANES=1):
+        one = f.fastkpc_compatible_cuda_skeleton(data, 0.1, options=opts)
+    for _ in range(2):
+        two = f.fastkpc_compatible_cuda_skeleton(data, 0.1, options=opts)
+        same_skeleton(one, two)
+        assert two["summary"]["level_fits"] == one["summary"]["level_fits"]
+    if cfg == "sem351x48":
+        assert one["tasks"]["conditioning_size"].max() >= 3  # the shared-smooth levels ran
 
 
 def perm_batch(method, n=600, p=10, seed=21, R=200):
