@@ -69,6 +69,9 @@ class KpcResult(ctypes.Structure):
         ("task_margin", _D),
         ("level_stage_seconds", (ctypes.c_double * 5) * 64),
         ("task_keys", ctypes.c_void_p),
+        ("task_stat", _D),
+        ("task_mean", _D),
+        ("task_var", _D),
     ]
 
 

@@ -106,10 +106,24 @@ def fastkpc_compatible_cuda_skeleton(data, alpha, labels=None, options=None):
                 "native_edge_deleted": np.ctypeslib.as_array(r.task_deleted, (L,)).astype(bool).copy(),
                 "decision_margin": np.ctypeslib.as_array(r.task_margin, (L,)).copy(),
             }
+            stat = np.ctypeslib.as_array(r.task_stat, (L,)).copy()
+            mean = np.ctypeslib.as_array(r.task_mean, (L,)).copy()
+            var = np.ctypeslib.as_array(r.task_var, (L,)).copy()
         else:
             tasks = {"level": np.zeros(0, int), "x": np.zeros(0, int), "y": np.zeros(0, int),
                      "S_key": [], "conditioning_size": np.zeros(0, int), "p_used": np.zeros(0),
                      "native_edge_deleted": np.zeros(0, bool), "decision_margin": np.zeros(0)}
+            stat = mean = var = np.zeros(0)
+        # the tested edge as (smaller, larger) 1-based index, whichever
+        # orientation the test conditioned from (the reference trace's
+        # edge_x / edge_y, fastkpc/src/full_cuda_ci_one_call.cpp:5601-5602)
+        tasks["edge_x"] = np.minimum(tasks["x"], tasks["y"])
+        tasks["edge_y"] = np.maximum(tasks["x"], tasks["y"])
+        if options.get("ci_method", "dcc.gamma") == "hsic.gamma":
+            # fastkpc/src/full_cuda_ci_one_call.cpp:5620-5627
+            tasks["method_statistic"] = stat
+            tasks["method_mean"] = mean
+            tasks["method_variance"] = var
         levels = r.levels
         summary = {
             "entrypoint": "kpc_skeleton_run",

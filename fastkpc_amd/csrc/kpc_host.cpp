@@ -1266,6 +1266,9 @@ void push_trace(kpc_result* r, int& cap, const Task& t, int level) {
     r->task_p = (double*)std::realloc(r->task_p, sizeof(double) * cap);
     r->task_deleted = (unsigned char*)std::realloc(r->task_deleted, cap);
     r->task_margin = (double*)std::realloc(r->task_margin, sizeof(double) * cap);
+    r->task_stat = (double*)std::realloc(r->task_stat, sizeof(double) * cap);
+    r->task_mean = (double*)std::realloc(r->task_mean, sizeof(double) * cap);
+    r->task_var = (double*)std::realloc(r->task_var, sizeof(double) * cap);
   }
   const int i = r->n_tasks++;
   r->task_level[i] = level;
@@ -1276,6 +1279,9 @@ void push_trace(kpc_result* r, int& cap, const Task& t, int level) {
   r->task_p[i] = t.p;
   r->task_deleted[i] = t.deleted ? 1 : 0;
   r->task_margin[i] = t.margin;
+  r->task_stat[i] = t.stat;
+  r->task_mean[i] = t.mean;
+  r->task_var[i] = t.var;
 }
 
 void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
@@ -1560,6 +1566,9 @@ void kpc_result_free(kpc_result* r) {
   std::free(r->task_deleted);
   std::free(r->task_margin);
   std::free(r->task_keys);
+  std::free(r->task_stat);
+  std::free(r->task_mean);
+  std::free(r->task_var);
   std::free(r);
 }
 

@@ -104,6 +104,14 @@ typedef struct kpc_result {
    * (the reference trace's S_key; "" for the empty set), rows separated by
    * '\n', NUL-terminated */
   char* task_keys;
+  /* per trace row, the test statistic and the mean and variance of its null
+   * distribution (gamma approximation; over the replicates for *.perm); the
+   * reference trace carries them as method_statistic / method_mean /
+   * method_variance for hsic.gamma
+   * (fastkpc/src/full_cuda_ci_one_call.cpp:5620-5627) */
+  double* task_stat;
+  double* task_mean;
+  double* task_var;
 } kpc_result;
 
 void kpc_default_options(kpc_options* opt);

@@ -183,3 +183,33 @@ def test_plain_form_equals_batched_form_permutation():
     assert np.array_equal(rb["adjacency"], rp["adjacency"])
     assert rb["tasks"]["S_key"] == rp["tasks"]["S_key"]
     assert np.array_equal(rb["tasks"]["p_used"], rp["tasks"]["p_used"])
+
+
+@pytest.mark.parametrize("method", ["hsic.gamma", "dcc.gamma"])
+def test_skeleton_trace_statistic_and_edge_columns(method):
+    """The skeleton trace's edge_x / edge_y are the 1-based (min, max) of each
+    tested pair; for hsic.gamma it carries method_statistic / method_mean /
+    method_variance, equal bit for bit to ci_batch on the same tests (whose
+    values are checked against the oracle above); dcc.gamma has no such
+    columns, as in the reference trace."""
+    import fastkpc_amd as fk
+    from fastkpc_amd.sem import config_data
+    data, _, _ = config_data("sem200x8")
+    res = fk.fastkpc_compatible_cuda_skeleton(data, 0.1, options={"numCol": 35, "ci_method": method})
+    t = res["tasks"]
+    assert np.array_equal(t["edge_x"], np.minimum(t["x"], t["y"]))
+    assert np.array_equal(t["edge_y"], np.maximum(t["x"], t["y"]))
+    assert np.all(t["edge_x"] < t["edge_y"]) and t["edge_x"].min() >= 1
+    if method != "hsic.gamma":
+        assert "method_statistic" not in t
+        return
+    rows = [i for i in range(len(t["x"])) if t["conditioning_size"][i] == 1]
+    assert len(rows) > 5
+    xs = [int(t["x"][i]) - 1 for i in rows]
+    ys = [int(t["y"][i]) - 1 for i in rows]
+    S = np.array([[int(t["S_key"][i]) - 1] for i in rows], np.int32)
+    out = fk.ci_batch(data, xs, ys, S, num_col=35, ci_method="hsic.gamma")
+    for j, i in enumerate(rows):
+        assert out[j, 1] == t["method_statistic"][i], (i, out[j], t["method_statistic"][i])
+        assert out[j, 2] == t["method_mean"][i] and out[j, 3] == t["method_variance"][i]
+        assert out[j, 0] == t["p_used"][i]
