@@ -72,6 +72,7 @@ class KpcResult(ctypes.Structure):
         ("task_stat", _D),
         ("task_mean", _D),
         ("task_var", _D),
+        ("level_tasks_planned", ctypes.c_longlong * 64),
     ]
 
 

@@ -125,6 +125,19 @@ def fastkpc_compatible_cuda_skeleton(data, alpha, labels=None, options=None):
             tasks["method_mean"] = mean
             tasks["method_variance"] = var
         levels = r.levels
+        # the reference's levels data frame (fastkpc/src/full_cuda_ci_one_call.cpp:5632-5640):
+        # every planned task is either replayed or ignored once its edge is deleted
+        planned = [int(v) for v in list(r.level_tasks_planned)[:levels]]
+        replayed = [int(v) for v in list(r.n_edgetests)[:levels]]
+        lv, dl = tasks["level"], tasks["native_edge_deleted"]
+        level_rows = {
+            "level": list(range(levels)),
+            "tasks_planned": planned,
+            "tests_replayed": replayed,
+            "tasks_ignored_after_delete": [a - b for a, b in zip(planned, replayed)],
+            "deletions": [int(np.count_nonzero(dl[lv == l])) for l in range(levels)],
+            "elapsed_ms": [1e3 * v for v in list(r.level_seconds)[:levels]],
+        }
         summary = {
             "entrypoint": "kpc_skeleton_run",
             "n": n, "p": p, "alpha": float(alpha),
@@ -164,6 +177,7 @@ def fastkpc_compatible_cuda_skeleton(data, alpha, labels=None, options=None):
             "n.edgetests": [int(v) for v in list(r.n_edgetests)[:levels]],
             "tasks": tasks,
             "levels": levels,
+            "level_rows": level_rows,
             "summary": summary,
         }
     finally:

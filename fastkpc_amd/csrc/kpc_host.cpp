@@ -1464,6 +1464,7 @@ void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
     for (const Task& t : tasks)
       if (t.consumed) push_trace(r, cap, t, ord);
     r->level_tests[ord] = r->n_edgetests[ord];
+    r->level_tasks_planned[ord] = (long long)tasks.size();
     r->level_physical[ord] = eng.stats_.physical_tests - st0.physical_tests;
     r->level_fits[ord] = eng.stats_.fits - st0.fits;
     r->level_components[ord] = eng.stats_.components - st0.components;

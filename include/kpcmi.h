@@ -112,6 +112,11 @@ typedef struct kpc_result {
   double* task_stat;
   double* task_mean;
   double* task_var;
+  /* per level, the tasks of the level plan (every conditioning set of both
+   * orientations of every edge present at the level's start); the
+   * reference's levels data frame column tasks_planned
+   * (fastkpc/src/full_cuda_ci_one_call.cpp:5573, :5634) */
+  long long level_tasks_planned[64];
 } kpc_result;
 
 void kpc_default_options(kpc_options* opt);

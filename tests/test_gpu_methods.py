@@ -213,3 +213,36 @@ def test_skeleton_trace_statistic_and_edge_columns(method):
         assert out[j, 1] == t["method_statistic"][i], (i, out[j], t["method_statistic"][i])
         assert out[j, 2] == t["method_mean"][i] and out[j, 3] == t["method_variance"][i]
         assert out[j, 0] == t["p_used"][i]
+
+
+@pytest.mark.parametrize("cfg", ["sem200x8", "sem351x48"])
+def test_level_rows_match_the_level_plan(cfg):
+    """level_rows (the reference's levels data frame): tasks_planned equals
+    sum over the edges present at the level's start of C(|adj(x)\\{y}|, l) +
+    C(|adj(y)\\{x}|, l), with the adjacency rebuilt from the trace's deletions
+    (pcalg stable: deletions apply at the level's end); tests_replayed is
+    n.edgetests, tasks_ignored_after_delete = planned - replayed >= 0, and the
+    deletions rebuild the final adjacency."""
+    from math import comb
+    import fastkpc_amd as fk
+    from fastkpc_amd.sem import config_data
+    data, _, _ = config_data(cfg)
+    res = fk.fastkpc_compatible_cuda_skeleton(data, 0.1, options={"numCol": 35})
+    rows, t = res["level_rows"], res["tasks"]
+    p = data.shape[1]
+    adj = np.ones((p, p), bool)
+    np.fill_diagonal(adj, False)
+    assert rows["tests_replayed"] == res["n.edgetests"]
+    for l in range(res["levels"]):
+        planned = 0
+        for a in range(p):
+            for b in range(a + 1, p):
+                if adj[a, b]:
+                    planned += comb(int(adj[a].sum()) - 1, l) + comb(int(adj[b].sum()) - 1, l)
+        assert rows["tasks_planned"][l] == planned, (l, rows["tasks_planned"][l], planned)
+        assert rows["tasks_ignored_after_delete"][l] == planned - res["n.edgetests"][l] >= 0
+        sel = (t["level"] == l) & t["native_edge_deleted"]
+        assert rows["deletions"][l] == int(sel.sum())
+        for x, y in zip(t["edge_x"][sel], t["edge_y"][sel]):
+            adj[x - 1, y - 1] = adj[y - 1, x - 1] = False
+    assert np.array_equal(adj, res["adjacency"])
