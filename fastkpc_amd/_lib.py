@@ -73,6 +73,7 @@ class KpcResult(ctypes.Structure):
         ("task_mean", _D),
         ("task_var", _D),
         ("level_tasks_planned", ctypes.c_longlong * 64),
+        ("task_index", ctypes.POINTER(ctypes.c_int)),
     ]
 
 

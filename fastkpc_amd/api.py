@@ -106,6 +106,11 @@ def fastkpc_compatible_cuda_skeleton(data, alpha, labels=None, options=None):
                 "native_edge_deleted": np.ctypeslib.as_array(r.task_deleted, (L,)).astype(bool).copy(),
                 "decision_margin": np.ctypeslib.as_array(r.task_margin, (L,)).copy(),
             }
+            # trace rows in plan order over the consumed tests, numbered across
+            # levels, and each row's 1-based position in its level's plan
+            # (fastkpc/src/full_cuda_ci_one_call.cpp:5535-5537)
+            tasks["canonical_test_order_id"] = np.arange(1, L + 1)
+            tasks["task_index"] = np.ctypeslib.as_array(r.task_index, (L,)).copy()
             stat = np.ctypeslib.as_array(r.task_stat, (L,)).copy()
             mean = np.ctypeslib.as_array(r.task_mean, (L,)).copy()
             var = np.ctypeslib.as_array(r.task_var, (L,)).copy()
@@ -113,6 +118,8 @@ def fastkpc_compatible_cuda_skeleton(data, alpha, labels=None, options=None):
             tasks = {"level": np.zeros(0, int), "x": np.zeros(0, int), "y": np.zeros(0, int),
                      "S_key": [], "conditioning_size": np.zeros(0, int), "p_used": np.zeros(0),
                      "native_edge_deleted": np.zeros(0, bool), "decision_margin": np.zeros(0)}
+            tasks["canonical_test_order_id"] = np.zeros(0, int)
+            tasks["task_index"] = np.zeros(0, int)
             stat = mean = var = np.zeros(0)
         # the tested edge as (smaller, larger) 1-based index, whichever
         # orientation the test conditioned from (the reference trace's

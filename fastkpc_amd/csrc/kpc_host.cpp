@@ -1255,7 +1255,7 @@ kpc_result* new_result(int p) {
   return r;
 }
 
-void push_trace(kpc_result* r, int& cap, const Task& t, int level) {
+void push_trace(kpc_result* r, int& cap, const Task& t, int level, int plan_index) {
   if (r->n_tasks == cap) {
     cap = cap ? cap * 2 : 4096;
     r->task_level = (int*)std::realloc(r->task_level, sizeof(int) * cap);
@@ -1269,6 +1269,7 @@ void push_trace(kpc_result* r, int& cap, const Task& t, int level) {
     r->task_stat = (double*)std::realloc(r->task_stat, sizeof(double) * cap);
     r->task_mean = (double*)std::realloc(r->task_mean, sizeof(double) * cap);
     r->task_var = (double*)std::realloc(r->task_var, sizeof(double) * cap);
+    r->task_index = (int*)std::realloc(r->task_index, sizeof(int) * cap);
   }
   const int i = r->n_tasks++;
   r->task_level[i] = level;
@@ -1282,6 +1283,7 @@ void push_trace(kpc_result* r, int& cap, const Task& t, int level) {
   r->task_stat[i] = t.stat;
   r->task_mean[i] = t.mean;
   r->task_var[i] = t.var;
+  r->task_index[i] = plan_index + 1;
 }
 
 void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
@@ -1461,8 +1463,8 @@ void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
           r->adjacency[(size_t)a * p + b] = 0;
           r->adjacency[(size_t)b * p + a] = 0;
         }
-    for (const Task& t : tasks)
-      if (t.consumed) push_trace(r, cap, t, ord);
+    for (size_t ti = 0; ti < tasks.size(); ++ti)
+      if (tasks[ti].consumed) push_trace(r, cap, tasks[ti], ord, (int)ti);
     r->level_tests[ord] = r->n_edgetests[ord];
     r->level_tasks_planned[ord] = (long long)tasks.size();
     r->level_physical[ord] = eng.stats_.physical_tests - st0.physical_tests;
@@ -1570,6 +1572,7 @@ void kpc_result_free(kpc_result* r) {
   std::free(r->task_stat);
   std::free(r->task_mean);
   std::free(r->task_var);
+  std::free(r->task_index);
   std::free(r);
 }
 

@@ -117,6 +117,11 @@ typedef struct kpc_result {
    * reference's levels data frame column tasks_planned
    * (fastkpc/src/full_cuda_ci_one_call.cpp:5573, :5634) */
   long long level_tasks_planned[64];
+  /* per trace row, the 1-based position of the test in its level's plan
+   * (edges x < y in row-major order, the x orientation's conditioning sets
+   * before the y orientation's, each in lexicographic order); the reference
+   * trace's task_index (fastkpc/src/full_cuda_ci_one_call.cpp:5537) */
+  int* task_index;
 } kpc_result;
 
 void kpc_default_options(kpc_options* opt);

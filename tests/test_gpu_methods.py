@@ -246,3 +246,39 @@ def test_level_rows_match_the_level_plan(cfg):
         for x, y in zip(t["edge_x"][sel], t["edge_y"][sel]):
             adj[x - 1, y - 1] = adj[y - 1, x - 1] = False
     assert np.array_equal(adj, res["adjacency"])
+
+
+def test_trace_task_index_is_the_position_in_the_level_plan():
+    """task_index is each trace row's 1-based position in its level's plan,
+    enumerated here independently in the documented order (edges x < y
+    row-major, the x orientation's conditioning sets, then the y
+    orientation's, each in lexicographic order, over the adjacency at the
+    level's start); canonical_test_order_id numbers the rows 1..N."""
+    from itertools import combinations
+    import fastkpc_amd as fk
+    from fastkpc_amd.sem import config_data
+    data, _, _ = config_data("sem200x8")
+    res = fk.fastkpc_compatible_cuda_skeleton(data, 0.1, options={"numCol": 35})
+    t = res["tasks"]
+    p = data.shape[1]
+    assert np.array_equal(t["canonical_test_order_id"], np.arange(1, len(t["x"]) + 1))
+    adj = np.ones((p, p), bool)
+    np.fill_diagonal(adj, False)
+    for l in range(res["levels"]):
+        plan = {}
+        for a in range(p):
+            for b in range(a + 1, p):
+                if not adj[a, b]:
+                    continue
+                for x, y in ((a, b), (b, a)):
+                    nb = [i for i in range(p) if i != y and adj[x, i]]
+                    for S in combinations(nb, l):
+                        plan[(x + 1, y + 1, "|".join(str(s + 1) for s in S))] = len(plan) + 1
+        assert len(plan) == res["level_rows"]["tasks_planned"][l]
+        rows = np.nonzero(t["level"] == l)[0]
+        got = [int(t["task_index"][i]) for i in rows]
+        want = [plan[(int(t["x"][i]), int(t["y"][i]), t["S_key"][i])] for i in rows]
+        assert got == want and got == sorted(got)
+        sel = rows[t["native_edge_deleted"][rows]]
+        for i in sel:
+            adj[t["edge_x"][i] - 1, t["edge_y"][i] - 1] = adj[t["edge_y"][i] - 1, t["edge_x"][i] - 1] = False
