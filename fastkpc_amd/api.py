@@ -124,6 +124,9 @@ def fastkpc_compatible_cuda_skeleton(data, alpha, labels=None, options=None):
         # the tested edge as (smaller, larger) 1-based index, whichever
         # orientation the test conditioned from (the reference trace's
         # edge_x / edge_y, fastkpc/src/full_cuda_ci_one_call.cpp:5601-5602)
+        # every trace row is a consumed test, so none is ignored (the reference
+        # writes a constant FALSE column, fastkpc/src/full_cuda_ci_one_call.cpp:5610-5611)
+        tasks["native_edge_ignored"] = np.zeros(len(tasks["x"]), bool)
         tasks["edge_x"] = np.minimum(tasks["x"], tasks["y"])
         tasks["edge_y"] = np.maximum(tasks["x"], tasks["y"])
         if options.get("ci_method", "dcc.gamma") == "hsic.gamma":

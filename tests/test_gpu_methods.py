@@ -262,6 +262,7 @@ def test_trace_task_index_is_the_position_in_the_level_plan():
     t = res["tasks"]
     p = data.shape[1]
     assert np.array_equal(t["canonical_test_order_id"], np.arange(1, len(t["x"]) + 1))
+    assert t["native_edge_ignored"].dtype == bool and not t["native_edge_ignored"].any()
     adj = np.ones((p, p), bool)
     np.fill_diagonal(adj, False)
     for l in range(res["levels"]):
