@@ -74,6 +74,9 @@ class KpcResult(ctypes.Structure):
         ("task_var", _D),
         ("level_tasks_planned", ctypes.c_longlong * 64),
         ("task_index", ctypes.POINTER(ctypes.c_int)),
+        ("level_tasks_ignored", ctypes.c_longlong * 64),
+        ("level_rounds", ctypes.c_longlong * 64),
+        ("task_perm_seed", ctypes.POINTER(ctypes.c_ulonglong)),
     ]
 
 

@@ -114,6 +114,7 @@ def fastkpc_compatible_cuda_skeleton(data, alpha, labels=None, options=None):
             stat = np.ctypeslib.as_array(r.task_stat, (L,)).copy()
             mean = np.ctypeslib.as_array(r.task_mean, (L,)).copy()
             var = np.ctypeslib.as_array(r.task_var, (L,)).copy()
+            seeds = np.ctypeslib.as_array(r.task_perm_seed, (L,)).astype(np.uint64)
         else:
             tasks = {"level": np.zeros(0, int), "x": np.zeros(0, int), "y": np.zeros(0, int),
                      "S_key": [], "conditioning_size": np.zeros(0, int), "p_used": np.zeros(0),
@@ -121,6 +122,7 @@ def fastkpc_compatible_cuda_skeleton(data, alpha, labels=None, options=None):
             tasks["canonical_test_order_id"] = np.zeros(0, int)
             tasks["task_index"] = np.zeros(0, int)
             stat = mean = var = np.zeros(0)
+            seeds = np.zeros(0, np.uint64)
         # the tested edge as (smaller, larger) 1-based index, whichever
         # orientation the test conditioned from (the reference trace's
         # edge_x / edge_y, fastkpc/src/full_cuda_ci_one_call.cpp:5601-5602)
@@ -129,29 +131,60 @@ def fastkpc_compatible_cuda_skeleton(data, alpha, labels=None, options=None):
         tasks["native_edge_ignored"] = np.zeros(len(tasks["x"]), bool)
         tasks["edge_x"] = np.minimum(tasks["x"], tasks["y"])
         tasks["edge_y"] = np.maximum(tasks["x"], tasks["y"])
-        if options.get("ci_method", "dcc.gamma") == "hsic.gamma":
-            # fastkpc/src/full_cuda_ci_one_call.cpp:5620-5627
-            tasks["method_statistic"] = stat
-            tasks["method_mean"] = mean
-            tasks["method_variance"] = var
+        # the statistic and its null mean / variance of every row: the
+        # reference trace carries them for hsic.gamma
+        # (fastkpc/src/full_cuda_ci_one_call.cpp:5620-5627); here every method
+        # has them (dcc.gamma: nV^2 and its gamma moments; *.perm: the observed
+        # statistic and the replicates' mean / variance)
+        tasks["method_statistic"] = stat
+        tasks["method_mean"] = mean
+        tasks["method_variance"] = var
+        method = options.get("ci_method", "dcc.gamma")
+        if method in ("dcc.perm", "hsic.perm"):
+            # fastkpc/src/full_cuda_ci_one_call.cpp:5613-5616 (uint64 here:
+            # the seed of the counter-based index generator)
+            tasks["permutation_seed"] = seeds
         levels = r.levels
-        # the reference's levels data frame (fastkpc/src/full_cuda_ci_one_call.cpp:5632-5640):
-        # every planned task is either replayed or ignored once its edge is deleted
-        planned = [int(v) for v in list(r.level_tasks_planned)[:levels]]
-        replayed = [int(v) for v in list(r.n_edgetests)[:levels]]
+        # the reference's levels data frame (fastkpc/src/full_cuda_ci_one_call.cpp:5632-5640)
         lv, dl = tasks["level"], tasks["native_edge_deleted"]
         level_rows = {
             "level": list(range(levels)),
-            "tasks_planned": planned,
-            "tests_replayed": replayed,
-            "tasks_ignored_after_delete": [a - b for a, b in zip(planned, replayed)],
+            "tasks_planned": [int(v) for v in list(r.level_tasks_planned)[:levels]],
+            "tests_replayed": [int(v) for v in list(r.n_edgetests)[:levels]],
+            "tasks_ignored_after_delete": [int(v) for v in list(r.level_tasks_ignored)[:levels]],
             "deletions": [int(np.count_nonzero(dl[lv == l])) for l in range(levels)],
+            "rounds": [int(v) for v in list(r.level_rounds)[:levels]],
             "elapsed_ms": [1e3 * v for v in list(r.level_seconds)[:levels]],
         }
+        # the reference's summary list, core fields first
+        # (fastkpc/src/full_cuda_ci_one_call.cpp:6017-6060), then this
+        # library's own counters
         summary = {
+            "run_status": "ok",
             "entrypoint": "kpc_skeleton_run",
-            "n": n, "p": p, "alpha": float(alpha),
-            "ci_method": options.get("ci_method", "dcc.gamma"),
+            "compatible_cuda_route": "compatible.cuda",
+            "compatible_cuda_strict": True,
+            "native_call_count": 1,
+            "p": p, "n": n, "alpha": float(alpha),
+            "index": float(o.index),
+            "numCol": int(o.num_col),
+            "ci_method": method,
+            "ci_backend": "kpcmi-gfx950",
+            "hsic_sig": float(o.hsic_sig),
+            "permutation_replicates": int(o.replicates) if method in ("dcc.perm", "hsic.perm") else 0,
+            "permutation_include_observed": bool(o.include_observed),
+            "permutation_has_seed": bool(o.has_seed),
+            "permutation_seed": float(o.seed) if o.has_seed else float("nan"),
+            "scheduler": "kpcmi-frontier-waves",
+            "frontier_batch_count": int(sum(level_rows["rounds"])),
+            "levels": levels,
+            "tasks_planned": int(sum(level_rows["tasks_planned"])),
+            "logical_tests_consumed": int(sum(level_rows["tests_replayed"])),
+            "physical_tests_evaluated": int(r.physical_tests),
+            "speculative_tests_ignored": int(r.physical_tests) - int(sum(level_rows["tests_replayed"])),
+            "tasks_ignored_after_delete": int(sum(level_rows["tasks_ignored_after_delete"])),
+            "deletions": int(sum(level_rows["deletions"])),
+            "r_callback_count": 0,
             "mode": options.get("mode", "batched"),
             "max_conditioning_size_resolved": o.max_conditioning_size,
             "wall_seconds": r.wall_seconds,

@@ -1270,6 +1270,8 @@ void push_trace(kpc_result* r, int& cap, const Task& t, int level, int plan_inde
     r->task_mean = (double*)std::realloc(r->task_mean, sizeof(double) * cap);
     r->task_var = (double*)std::realloc(r->task_var, sizeof(double) * cap);
     r->task_index = (int*)std::realloc(r->task_index, sizeof(int) * cap);
+    r->task_perm_seed =
+        (unsigned long long*)std::realloc(r->task_perm_seed, sizeof(unsigned long long) * cap);
   }
   const int i = r->n_tasks++;
   r->task_level[i] = level;
@@ -1302,6 +1304,9 @@ void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
     if (!any) break;
     done = true;
     std::memcpy(snap.data(), r->adjacency, (size_t)p * p);
+    // the level's time includes building its plan, as the reference's
+    // level_started precedes make_layer_plan
+    const double level_t0 = now_s();
     // ---- level plan
     std::vector<Task> tasks;
     std::vector<EdgeRun> edges;
@@ -1338,7 +1343,6 @@ void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
         }
         edges.push_back(er);
       }
-    const double level_t0 = now_s();
     double stage0[5];
     for (int st = 0; st < 5; ++st) stage0[st] = eng.stage(st);
     // Levels whose whole plan needs at most this many residual fits are
@@ -1402,6 +1406,7 @@ void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
         }
         if (plain_now && !batch.empty()) break;
       }
+      ++r->level_rounds[ord];
       if (plain_now) {
         // one launch sequence per test: fresh caches for every test
         eng.begin_level(ord);
@@ -1463,8 +1468,23 @@ void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
           r->adjacency[(size_t)a * p + b] = 0;
           r->adjacency[(size_t)b * p + a] = 0;
         }
-    for (size_t ti = 0; ti < tasks.size(); ++ti)
-      if (tasks[ti].consumed) push_trace(r, cap, tasks[ti], ord, (int)ti);
+    // tasks left on an edge after the test that deleted it (both
+    // orientations' tasks form the edge's run in the plan)
+    for (const EdgeRun& e : edges) {
+      if (e.state != 1) continue;
+      const int total = e.o[0].count + e.o[1].count;
+      const int pos = (e.side ? e.o[0].count : 0) + e.cursor;  // the deleting task
+      r->level_tasks_ignored[ord] += total - (pos + 1);
+    }
+    const bool perm = opt.ci_method == KPC_DCC_PERM || opt.ci_method == KPC_HSIC_PERM;
+    const uint64_t user_seed = opt.has_seed ? (uint64_t)opt.seed : 0ULL;
+    for (size_t ti = 0; ti < tasks.size(); ++ti) {
+      if (!tasks[ti].consumed) continue;
+      push_trace(r, cap, tasks[ti], ord, (int)ti);
+      const Task& t = tasks[ti];
+      r->task_perm_seed[r->n_tasks - 1] =
+          perm ? task_seed(user_seed, opt.ci_method, t.x, t.y, t.S, t.slen) : 0ULL;
+    }
     r->level_tests[ord] = r->n_edgetests[ord];
     r->level_tasks_planned[ord] = (long long)tasks.size();
     r->level_physical[ord] = eng.stats_.physical_tests - st0.physical_tests;
@@ -1573,6 +1593,7 @@ void kpc_result_free(kpc_result* r) {
   std::free(r->task_mean);
   std::free(r->task_var);
   std::free(r->task_index);
+  std::free(r->task_perm_seed);
   std::free(r);
 }
 

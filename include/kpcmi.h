@@ -122,6 +122,17 @@ typedef struct kpc_result {
    * before the y orientation's, each in lexicographic order); the reference
    * trace's task_index (fastkpc/src/full_cuda_ci_one_call.cpp:5537) */
   int* task_index;
+  /* per level, the planned tasks left on each edge after the test that
+   * deleted it, summed (the levels data frame column
+   * tasks_ignored_after_delete, fastkpc/src/full_cuda_ci_one_call.cpp:5486-5488),
+   * and the device waves the level took (its column rounds, :5637: the
+   * reference's frontier batches) */
+  long long level_tasks_ignored[64];
+  long long level_rounds[64];
+  /* per trace row, the seed the row's permutation indices were drawn from
+   * (dcc.perm / hsic.perm; 0 for the other methods); the trace column
+   * permutation_seed (fastkpc/src/full_cuda_ci_one_call.cpp:5613-5616) */
+  unsigned long long* task_perm_seed;
 } kpc_result;
 
 void kpc_default_options(kpc_options* opt);

@@ -89,6 +89,12 @@ int orc_magic_eval(const orc_setup* s, const double* y, const double* lsp, doubl
                    double* score, double* grad, double* hess);
 int orc_regrxons(const double* data, int n, int ld, int target, const int* S, int s_len,
                  const orc_magic_control* ctl, double* resid, orc_magic_result* res);
+/* The final fit of orc_regrxons at a given log(sp) instead of the optimiser's
+ * choice: residuals and the GCV score / rss / edf there.  Used to replay a
+ * fit whose optimiser took a rounding-decided step at the end point another
+ * implementation selected. */
+int orc_regrxons_at_sp(const double* data, int n, int ld, int target, const int* S, int s_len,
+                       const double* log_sp, double rank_tol, double* resid, double out[3]);
 
 /* ---- pcalg stable skeleton (fastkpc/src/skeleton_engine.cpp:96-227) ---- */
 enum { ORC_DCC_GAMMA = 0, ORC_HSIC_GAMMA = 1, ORC_DCC_PERM = 2, ORC_HSIC_PERM = 3 };
@@ -118,6 +124,10 @@ typedef struct {
   double* trace_p;
   unsigned char* trace_del;
   double* trace_margin; /* min decision margin of the test's two fits */
+  /* per trace row: statistic and null mean / variance (dcc.gamma: nV^2 and
+   * its gamma moments; hsic.gamma: HSIC and its moments; *.perm: the observed
+   * statistic and the replicates' mean / variance) */
+  double* trace_stat; double* trace_mean; double* trace_var;
 } orc_skeleton_result;
 
 int orc_skeleton(const double* data, int n, int p, const orc_skeleton_options* opt,

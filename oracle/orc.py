@@ -40,7 +40,8 @@ class OrcSkeletonResult(ctypes.Structure):
                 ("trace_len", ctypes.c_int), ("trace_cap", ctypes.c_int),
                 ("trace_level", _I), ("trace_x", _I), ("trace_y", _I), ("trace_slen", _I),
                 ("trace_S", _I), ("trace_p", _D), ("trace_del", ctypes.POINTER(ctypes.c_ubyte)),
-                ("trace_margin", _D)]
+                ("trace_margin", _D), ("trace_stat", _D), ("trace_mean", _D),
+                ("trace_var", _D)]
 
 
 class OrcSetup(ctypes.Structure):
@@ -91,6 +92,8 @@ def lib():
         L.orc_regrxons.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I, ctypes.c_int,
                                    ctypes.POINTER(OrcMagicControl), _D,
                                    ctypes.POINTER(OrcMagicResult)]
+        L.orc_regrxons_at_sp.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I,
+                                         ctypes.c_int, _D, ctypes.c_double, _D, _D]
         L.orc_ci_test.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I,
                                   ctypes.c_int, ctypes.POINTER(OrcSkeletonOptions), _D]
         _U = ctypes.POINTER(ctypes.c_uint)
@@ -173,6 +176,22 @@ def regrxons(data, target, S, control=None):
                    "log_sp": np.array(res.log_sp[:m]), "iterations": res.iterations,
                    "evaluations": res.evaluations, "step_failed": res.step_failed,
                    "margin": res.margin}
+
+
+def regrxons_at_sp(data, target, S, log_sp, rank_tol=1.4901161193847656e-8):
+    """The oracle's final regrXonS fit at a given log(sp): (residual, {score, rss, edf})."""
+    d = np.asfortranarray(data, dtype=np.float64)
+    n, p = d.shape
+    Sa = np.ascontiguousarray(sorted(S), dtype=np.int32)
+    m = len(Sa) if len(Sa) > 2 else 1
+    lsp = np.ascontiguousarray(log_sp, dtype=np.float64).reshape(m)
+    resid = np.zeros(n)
+    out = np.zeros(3)
+    rc = lib().orc_regrxons_at_sp(_d(d), n, n, int(target), Sa.ctypes.data_as(_I), len(Sa),
+                                  _d(lsp), rank_tol, _d(resid), _d(out))
+    if rc != 0:
+        raise RuntimeError("orc_regrxons_at_sp failed %d" % rc)
+    return resid, {"score": out[0], "rss": out[1], "edf": out[2]}
 
 
 def gcv_eval(data, target, S, log_sp, rank_tol=1.4901161193847656e-8):
@@ -329,6 +348,9 @@ def skeleton(data, alpha=0.1, m_max=None, num_col=35, threads=8, method="dcc.gam
             "p_used": np.ctypeslib.as_array(res.trace_p, (L,)).copy(),
             "native_edge_deleted": np.ctypeslib.as_array(res.trace_del, (L,)).astype(bool).copy(),
             "margin": np.ctypeslib.as_array(res.trace_margin, (L,)).copy(),
+            "statistic": np.ctypeslib.as_array(res.trace_stat, (L,)).copy(),
+            "mean": np.ctypeslib.as_array(res.trace_mean, (L,)).copy(),
+            "variance": np.ctypeslib.as_array(res.trace_var, (L,)).copy(),
         }
         return {"adjacency": adj, "pMax": pmax, "sepsets": sepsets,
                 "n.edgetests": list(res.n_edgetests)[:res.levels], "tasks": tasks,

@@ -1119,3 +1119,34 @@ int orc_regrxons(const double* data, int n, int ld, int target, const int* S, in
   orc_setup_free(&s);
   return rc;
 }
+
+int orc_regrxons_at_sp(const double* data, int n, int ld, int target, const int* S, int s_len,
+                       const double* log_sp, double rank_tol, double* resid, double out[3]) {
+  orc_setup s;
+  int rc = orc_regr_setup(data, n, ld, S, s_len, &s);
+  if (rc != 0) return rc;
+  magic_problem P;
+  rc = magic_prepare(&s, rank_tol, &P);
+  if (rc == 0) {
+    const double* y = data + (size_t)target * ld;
+    double* y0 = (double*)malloc(sizeof(double) * n);
+    memcpy(y0, y, sizeof(double) * n);
+    orc_apply_qt(P.qr, n, s.q, n, P.tau, y0);
+    double yn = 0.0;
+    for (int i = 0; i < n; ++i) yn += y[i] * y[i];
+    magic_core c;
+    memset(&c, 0, sizeof(c));
+    rc = magic_core_eval(&P, y0, yn, log_sp, &c);
+    if (rc == 0) {
+      magic_fit(&P, &c, y, resid);
+      out[0] = c.score;
+      out[1] = c.rss;
+      out[2] = c.edf;
+    }
+    core_free(&c);
+    free(y0);
+  }
+  magic_problem_free(&P);
+  orc_setup_free(&s);
+  return rc;
+}

@@ -218,7 +218,7 @@ static int next_comb(int* idx, int k, int len) {
 }
 
 static void trace_push(orc_skeleton_result* r, int level, int x, int y, const int* S, int s_len,
-                       double pv, int del, double margin) {
+                       double pv, int del, const double out[5]) {
   if (r->trace_len == r->trace_cap) {
     int nc = r->trace_cap ? r->trace_cap * 2 : 4096;
     r->trace_level = (int*)realloc(r->trace_level, sizeof(int) * nc);
@@ -229,6 +229,9 @@ static void trace_push(orc_skeleton_result* r, int level, int x, int y, const in
     r->trace_p = (double*)realloc(r->trace_p, sizeof(double) * nc);
     r->trace_del = (unsigned char*)realloc(r->trace_del, nc);
     r->trace_margin = (double*)realloc(r->trace_margin, sizeof(double) * nc);
+    r->trace_stat = (double*)realloc(r->trace_stat, sizeof(double) * nc);
+    r->trace_mean = (double*)realloc(r->trace_mean, sizeof(double) * nc);
+    r->trace_var = (double*)realloc(r->trace_var, sizeof(double) * nc);
     r->trace_cap = nc;
   }
   int t = r->trace_len++;
@@ -239,7 +242,10 @@ static void trace_push(orc_skeleton_result* r, int level, int x, int y, const in
   for (int i = 0; i < s_len; ++i) r->trace_S[(size_t)t * 64 + i] = S[i];
   r->trace_p[t] = pv;
   r->trace_del[t] = (unsigned char)del;
-  r->trace_margin[t] = margin;
+  r->trace_margin[t] = out[4];
+  r->trace_stat[t] = out[1];
+  r->trace_mean[t] = out[2];
+  r->trace_var[t] = out[3];
 }
 
 /* Fill the component cache for every (target, S) the level could request. */
@@ -349,7 +355,7 @@ int orc_skeleton(const double* data, int n, int p, const orc_skeleton_options* o
             ++tests;
             if (r->pmax[x * p + y] < pv) r->pmax[x * p + y] = pv;
             int deleted = pv >= opt->alpha;
-            trace_push(r, ord, x, y, S, ord, pv, deleted, out[4]);
+            trace_push(r, ord, x, y, S, ord, pv, deleted, out);
             if (deleted) {
               del[a * p + b] = del[b * p + a] = 1;
               r->sepset_len[a * p + b] = r->sepset_len[b * p + a] = ord;
@@ -382,5 +388,6 @@ void orc_skeleton_result_free(orc_skeleton_result* r) {
   free(r->adjacency); free(r->pmax); free(r->sepset_len); free(r->sepset);
   free(r->trace_level); free(r->trace_x); free(r->trace_y); free(r->trace_slen);
   free(r->trace_S); free(r->trace_p); free(r->trace_del); free(r->trace_margin);
+  free(r->trace_stat); free(r->trace_mean); free(r->trace_var);
   memset(r, 0, sizeof(*r));
 }

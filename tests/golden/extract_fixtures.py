@@ -75,6 +75,7 @@ def extract_dev500(root):
     sep = np.array(sep_rows, dtype=np.int32).reshape(-1, 11)
     lt = fx["oracle"]["logical_trace"]
     dt = fx["oracle"]["deletion_trace"]
+    bs = fx["baseline_summary"]
 
     def skey(vals):
         return np.array([s if s is not None else "" for s in vals])
@@ -92,6 +93,8 @@ def extract_dev500(root):
         lt_S=skey(lt["S_key"].value),
         lt_p=np.array(lt["p_value"].value, dtype=np.float64),
         lt_del=np.array([bool(v) for v in lt["deletes_edge"].value]),
+        lt_seq=np.array(lt["logical_sequence_id"].value, dtype=np.int32),
+        lt_task_index=np.array(lt["source_task_index"].value, dtype=np.int32),
         dt_level=np.array(dt["level"].value, dtype=np.int32),
         dt_edge_x=np.array(dt["edge_x"].value, dtype=np.int32),
         dt_edge_y=np.array(dt["edge_y"].value, dtype=np.int32),
@@ -103,6 +106,12 @@ def extract_dev500(root):
         num_col=np.int32(_scalar(cfg["num_col"])),
         resolved_max=np.int32(_scalar(cfg["resolved_max_conditioning_size"])),
         baseline_elapsed_sec=np.float64(_scalar(fx["baseline_elapsed_sec"])),
+        # whole-run totals of the reference's levels data frame
+        bs_levels=np.int32(_scalar(bs["levels"])),
+        bs_tasks_planned=np.int64(_scalar(bs["tasks_planned"])),
+        bs_tests_replayed=np.int64(_scalar(bs["tests_replayed"])),
+        bs_tasks_ignored_after_delete=np.int64(_scalar(bs["tasks_ignored_after_delete"])),
+        bs_deletions=np.int64(_scalar(bs["deletions"])),
     )
     print("wrote dev500x50_v2.npz")
 

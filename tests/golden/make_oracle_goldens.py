@@ -3,7 +3,8 @@
     python tests/golden/make_oracle_goldens.py [name ...]
 
 Writes tests/golden/oracle_<name>.npz (adjacency, sepsets, n.edgetests and the
-canonical trace with p-values and decision margins) for each workload in
+canonical trace with p-values, statistics, null means and variances, and
+decision margins) for each workload in
 WORKLOADS (data source, CI method, permutation replicates).  The GPU parity tests compare the HIP path
 against these; tests/test_oracle_goldens.py re-runs the oracle on the small
 workloads to show the stored traces are the oracle's current output.
@@ -36,7 +37,8 @@ def save(name, data, res):
                         adjacency=res["adjacency"], pmax=res["pMax"], sepsets=sep,
                         n_edgetests=np.array(res["n.edgetests"], dtype=np.int32),
                         level=t["level"], x=t["x"], y=t["y"], S=np.array(t["S_key"]),
-                        p=t["p_used"], deleted=t["native_edge_deleted"], margin=t["margin"])
+                        p=t["p_used"], deleted=t["native_edge_deleted"], margin=t["margin"],
+                        stat=t["statistic"], mean=t["mean"], var=t["variance"])
 
 
 # name -> (data source, ci_method, replicates); permutation workloads use seed 2024
@@ -44,6 +46,7 @@ WORKLOADS = {
     "dev500x50": ("dev500x50", "dcc.gamma", 0),
     "sem351x48": ("sem351x48", "dcc.gamma", 0),
     "sem351x48_hsic_gamma": ("sem351x48", "hsic.gamma", 0),
+    "sem500x50": ("sem500x50", "dcc.gamma", 0),
     "sem200x8_hsic_gamma": ("sem200x8", "hsic.gamma", 0),
     "sem200x8_hsic_perm": ("sem200x8", "hsic.perm", 100),
     "sem200x8_dcc_perm": ("sem200x8", "dcc.perm", 100),

@@ -10,7 +10,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, TIE_MARGIN, compare_skeleton, p_close, trace_map
+from conftest import GOLDEN, TIE_MARGIN, compare_skeleton, p_close, rel_diff, replay_at_gpu_sp
 
 pytestmark = pytest.mark.gpu
 
@@ -93,7 +93,7 @@ def test_dcov_with_tied_values_matches_oracle():
     x = np.round(rng.standard_normal(200), 1)  # many ties -> repeated eigenvalues
     y = np.round(x + rng.standard_normal(200), 1)
     out = f.ci_batch(np.column_stack([x, y]), [0], [1], np.zeros((1, 0), np.int32))
-    ref = o.dcov_gamma(x, y, 20 if False else 35)
+    ref = o.dcov_gamma(x, y, 35)
     assert p_close(out[0, 0], ref["p.value"]), (out[0, 0], ref["p.value"])
 
 
@@ -142,7 +142,17 @@ def test_gcv_gradient_hessian_vs_oracle_and_finite_differences(dev500, S):
 
 
 def test_skeleton_500x50_vs_reference_fixture_and_oracle(dev500):
+    """The reference's own 500x50 dcc.gamma skeleton (kpcalg + mgcv): exact
+    adjacency, sepsets, n.edgetests, logical trace keys in order with their
+    task_index / logical sequence ids and deletion flags, the ordered deletion
+    trace, and the levels-frame totals; p-values within the oracle's own
+    measured agreement with the fixture (2.4e-10 absolute, 6.0e-9 relative)
+    plus 1e-9 relative, except rows whose fits took a rounding-decided
+    optimiser step (1e-6: mgcv, the oracle and the GPU may each take either
+    side), which are replayed on the oracle at the GPU's smoothing parameters
+    at 1e-9; every value against the oracle at 1e-9."""
     f = fk()
+    o = orc()
     d = dev500
     res = f.fastkpc_compatible_cuda_skeleton(d["data"], 0.1, options={"numCol": 35})
     assert res["n.edgetests"] == [int(v) for v in d["n_edgetests"]]
@@ -153,29 +163,80 @@ def test_skeleton_500x50_vs_reference_fixture_and_oracle(dev500):
             continue
         assert res["sepsets"][i][j] == [int(v) for v in row[3:3 + ln]]
     t = res["tasks"]
-    mine = trace_map(t["level"], t["x"], t["y"], t["S_key"], t["p_used"])
-    ref = trace_map(d["lt_level"], d["lt_x"], d["lt_y"], d["lt_S"], d["lt_p"])
-    assert set(mine) == set(ref)
-    # the fixture (kpcalg/mgcv) carries RSpectra tol-1e-10 eigenpairs and mgcv's
-    # own rounding-level optimiser decisions
-    assert max(abs(mine[k] - ref[k]) for k in ref) <= 1e-7
+    keys = list(zip(t["level"].tolist(), t["x"].tolist(), t["y"].tolist(), t["S_key"]))
+    fkeys = list(zip(d["lt_level"].tolist(), d["lt_x"].tolist(), d["lt_y"].tolist(),
+                     [str(v) for v in d["lt_S"]]))
+    assert keys == fkeys
+    assert np.array_equal(t["task_index"], d["lt_task_index"])
+    assert np.array_equal(t["canonical_test_order_id"], d["lt_seq"])
+    assert np.array_equal(t["native_edge_deleted"], d["lt_del"])
+    # ordered deletion trace: level, edge, tested orientation, S, p
+    sel = np.nonzero(t["native_edge_deleted"])[0]
+    assert len(sel) == len(d["dt_p"])
+    assert np.array_equal(t["level"][sel], d["dt_level"])
+    assert np.array_equal(t["edge_x"][sel], d["dt_edge_x"])
+    assert np.array_equal(t["edge_y"][sel], d["dt_edge_y"])
+    assert np.array_equal(t["x"][sel], d["dt_tested_x"])
+    assert np.array_equal(t["y"][sel], d["dt_tested_y"])
+    assert [t["S_key"][i] for i in sel] == [str(v) for v in d["dt_S"]]
+    gold = np.load(os.path.join(GOLDEN, "oracle_dev500x50.npz"))
+    # rows whose fits took a rounding-decided optimiser step (on the GPU or in
+    # the oracle) are replayed below at the GPU's smoothing parameters; every
+    # other row is held to the oracle's own agreement with the fixture
+    tie = np.minimum(t["decision_margin"], gold["margin"]) < TIE_MARGIN
+    dp = np.abs(t["p_used"] - d["lt_p"])
+    lim = 2.4e-10 + 6.0e-9 * np.abs(d["lt_p"]) + 1e-9 * np.abs(d["lt_p"])
+    over = (dp > lim) & ~tie
+    assert not over.any(), (dp[over].max(), np.nonzero(over)[0][:5])
+    assert np.all(np.abs(t["p_used"][sel] - d["dt_p"]) <= np.where(tie[sel], 1e-6, lim[sel]))
+    lr = res["level_rows"]
+    assert len(lr["level"]) == int(d["bs_levels"])
+    assert sum(lr["tasks_planned"]) == int(d["bs_tasks_planned"])
+    assert sum(lr["tests_replayed"]) == int(d["bs_tests_replayed"])
+    assert sum(lr["tasks_ignored_after_delete"]) == int(d["bs_tasks_ignored_after_delete"])
+    assert sum(lr["deletions"]) == int(d["bs_deletions"])
     rep = {}
-    compare_skeleton(res, np.load(os.path.join(GOLDEN, "oracle_dev500x50.npz")), p_close, rep)
-    print("500x50 vs oracle:", rep)
+    ties = compare_skeleton(res, gold, p_close, rep)
+    rep["tie_rows_replayed"] = len(ties)
+    rep["tie_score_excess"] = replay_at_gpu_sp(f, o, d["data"], "dcc.gamma", ties)
+    rep["max_abs_vs_fixture_non_tie"] = float(dp[~tie].max())
+    rep["max_abs_vs_fixture_tie"] = float(dp[tie].max()) if tie.any() else 0.0
+    print("500x50 vs oracle:", rep, flush=True)
 
 
-def test_skeleton_sem351x48_vs_oracle():
+@pytest.mark.parametrize("cfg", ["sem351x48", "sem500x50"])
+def test_skeleton_vs_oracle(cfg):
+    """BASELINE configs 2 (n=351, p=48) and 3 (n=500, p=50, 10% t3 noise):
+    the batched dcc.gamma skeleton against the oracle's full run -- exact
+    decisions, trace and deletion order; p, nV^2, mean and variance of every
+    row at 1e-9 (rounding-tie rows replayed at the GPU's smoothing
+    parameters)."""
     f = fk()
+    o = orc()
     from fastkpc_amd.sem import config_data
-    data, _, _ = config_data("sem351x48")
-    gold = np.load(os.path.join(GOLDEN, "oracle_sem351x48.npz"))
+    data, _, _ = config_data(cfg)
+    gold = np.load(os.path.join(GOLDEN, "oracle_%s.npz" % cfg))
     res = f.fastkpc_compatible_cuda_skeleton(data, 0.1, options={"numCol": 35})
     rep = {}
-    compare_skeleton(res, gold, p_close, rep)
-    print("sem351x48 vs oracle:", rep)
-    # pMax: max p-value per edge over the consumed tests (tie rows included)
-    fin = np.isfinite(gold["pmax"])
-    np.testing.assert_allclose(res["pMax"][fin], gold["pmax"][fin], rtol=1e-6, atol=1e-12)
+    ties = compare_skeleton(res, gold, p_close, rep)
+    rep["tie_rows_replayed"] = len(ties)
+    rep["tie_score_excess"] = replay_at_gpu_sp(f, o, data, "dcc.gamma", ties)
+    print(cfg, "vs oracle:", rep, flush=True)
+    # pMax: the largest p over each edge's consumed tests (every trace row
+    # is checked above), symmetric, -Inf for untested pairs, 1 on the diagonal
+    p = data.shape[1]
+    want = np.full((p, p), -np.inf)
+    np.fill_diagonal(want, 1.0)
+    t = res["tasks"]
+    for a, b, pv in zip(t["edge_x"] - 1, t["edge_y"] - 1, t["p_used"]):
+        want[a, b] = want[b, a] = max(want[a, b], pv)
+    assert np.array_equal(res["pMax"], want)
+    assert np.array_equal(np.isfinite(res["pMax"]), np.isfinite(gold["pmax"]))
+    for i in range(p):
+        for j in range(p):
+            ln = gold["sepsets"][i, j, 0]
+            want = None if ln < 0 else [int(v) for v in gold["sepsets"][i, j, 1:1 + ln]]
+            assert res["sepsets"][i][j] == want, (i, j)
 
 
 def test_plain_form_equals_batched_form():
@@ -191,31 +252,39 @@ def test_plain_form_equals_batched_form():
     assert sum(rp["summary"]["plain_tests"]) == sum(rp["n.edgetests"])
 
 
-def test_ci_batch_vs_oracle_random_sets():
+@pytest.mark.parametrize("s_len", [0, 1, 2, 3, 5, 7, 8])
+def test_ci_batch_vs_oracle_random_sets(s_len):
+    """ci_batch against the oracle's kernelCItest on random tests of
+    sem351x48 with |S| up to 8 (the batched form's conditioning sizes): p,
+    nV^2, mean and variance at 1e-9; a test whose fits hit a rounding tie is
+    replayed at the GPU's smoothing parameters."""
     f = fk()
     o = orc()
     from fastkpc_amd.sem import config_data
     data, _, _ = config_data("sem351x48")
-    rng = np.random.default_rng(17)
-    for s_len in (0, 1, 2, 3, 5):
-        xs, ys, Ss = [], [], []
-        for _ in range(3):
-            perm = rng.permutation(48)
-            xs.append(int(perm[0]))
-            ys.append(int(perm[1]))
-            Ss.append(sorted(int(v) for v in perm[2:2 + s_len]))
-        out = f.ci_batch(data, xs, ys, np.array(Ss, dtype=np.int32).reshape(3, s_len))
-        for i in range(3):
-            ref = o.ci_test(data, xs[i], ys[i], Ss[i])
-            tie = False
-            if s_len:
-                for t in (xs[i], ys[i]):
-                    tie |= o.regrxons(data, t, Ss[i])[1]["margin"] < TIE_MARGIN
-                    _, gi = f.regrxons_batch(data, [t], [Ss[i]])
-                    tie |= gi[0, 30] < TIE_MARGIN
-            tol = 1e-6 if tie else 1e-9
-            assert abs(out[i, 0] - ref["p.value"]) <= tol * ref["p.value"] + 1e-12, \
-                (s_len, xs[i], ys[i], Ss[i], out[i, 0], ref)
+    rng = np.random.default_rng(17 + s_len)
+    xs, ys, Ss = [], [], []
+    for _ in range(3):
+        perm = rng.permutation(48)
+        xs.append(int(perm[0]))
+        ys.append(int(perm[1]))
+        Ss.append(sorted(int(v) for v in perm[2:2 + s_len]))
+    out = f.ci_batch(data, xs, ys, np.array(Ss, dtype=np.int32).reshape(3, s_len))
+    for i in range(3):
+        ref = o.ci_test(data, xs[i], ys[i], Ss[i])
+        tie = False
+        if s_len:
+            for t in (xs[i], ys[i]):
+                tie |= o.regrxons(data, t, Ss[i])[1]["margin"] < TIE_MARGIN
+                _, gi = f.regrxons_batch(data, [t], [Ss[i]])
+                tie |= gi[0, 30] < TIE_MARGIN
+        vals = (ref["p.value"], ref["nV2"], ref["mean"], ref["variance"])
+        ok = p_close(out[i, 0], vals[0]) and all(rel_diff(a, b) <= 1e-9
+                                                 for a, b in zip(out[i, 1:], vals[1:]))
+        if not ok:
+            assert tie, (s_len, xs[i], ys[i], Ss[i], out[i], vals)
+            key = (s_len, xs[i] + 1, ys[i] + 1, "|".join(str(v + 1) for v in Ss[i]))
+            replay_at_gpu_sp(f, o, data, "dcc.gamma", {key: tuple(out[i])})
 
 
 def test_large_additive_designs_vs_oracle():
