@@ -84,7 +84,7 @@ EXPORTS = (
     "kpc_default_options", "kpc_skeleton_run", "kpc_result_free", "kpc_ci_batch",
     "kpc_regrxons_batch", "kpc_regrxons_mixed", "kpc_gcv_grad_hess", "kpc_dcov_components",
     "kpc_last_fit_timing", "kpc_tuning_report", "kpc_task_seed", "kpc_permutation_table", "kpc_last_error",
-    "kpc_device_count", "kpc_empty_cache",
+    "kpc_device_count", "kpc_empty_cache", "kpc_dcov_components_noalloc",
 )
 
 _lib = None

@@ -48,6 +48,15 @@ namespace {
 
 thread_local std::string g_last_error;
 
+}  // namespace
+
+namespace kpc {
+// kpc_last_error() text for entry points defined in other translation units
+void set_last_error(const char* msg) { g_last_error = msg; }
+}  // namespace kpc
+
+namespace {
+
 struct KpcError : std::runtime_error {
   int code;
   KpcError(int c, const std::string& m) : std::runtime_error(m), code(c) {}

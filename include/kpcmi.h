@@ -187,6 +187,20 @@ int kpc_gcv_grad_hess(const double* data, int n, int p, int count, const int* ta
 int kpc_dcov_components(const double* x, int n, int count, int k, double* values,
                         double* vectors, double* dsum, double* moment);
 
+/* Allocation-free kpc_dcov_components.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (HIP graph capture,
+ * serving loops that forbid hipMalloc).  x (n x count, column-major), values
+ * (count x k), vectors (count x k x n: vector e of component i at
+ * (i * k + e) * n), dsum and moment (count) are DEVICE pointers of the caller;
+ * the work is enqueued on `stream` (a hipStream_t; NULL = the default stream)
+ * and the call returns without synchronising, so the outputs are ready when
+ * the stream reaches that point.  Working memory is a 64 MiB device array
+ * per GPU held by the library until the process ends; count <= 65536.
+ * Calls that overlap on one GPU share that array and race.  Results equal
+ * kpc_dcov_components bit for bit. */
+int kpc_dcov_components_noalloc(const double* x, int n, int count, int k, double* values,
+                                double* vectors, double* dsum, double* moment, void* stream);
+
 /* Seed of the permutation test ({x, y}, S) for ci_method: SplitMix64 finaliser
  * folded over (user_seed, ci_method, min(x,y), max(x,y), s_len, S ascending).
  * Host only. */
