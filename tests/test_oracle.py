@@ -101,7 +101,8 @@ def test_dcov_gamma_invariances():
         assert abs(r["p.value"] - base["p.value"]) <= 1e-10
 
 
-@pytest.mark.parametrize("S", [[2], [3, 7], [1, 5, 9]])
+@pytest.mark.parametrize("S", [[2], [3, 7], [1, 5, 9], [2, 4, 6, 8], [1, 3, 5, 7, 9, 11],
+                               [2, 6, 10, 14, 18, 22, 26, 30]])
 def test_gcv_derivatives_match_central_differences(dev500, S):
     data = dev500["data"]
     m = 1 if len(S) <= 2 else len(S)
@@ -120,19 +121,27 @@ def test_gcv_derivatives_match_central_differences(dev500, S):
         np.testing.assert_allclose(fdh, hess[i], rtol=1e-4, atol=1e-6 * (np.abs(hess).max() + score))
 
 
-@pytest.mark.parametrize("S", [[2], [3, 7], [1, 5, 9]])
-def test_gcv_selection_is_a_local_minimum_and_fits_unpenalised_terms(dev500, S):
+@pytest.mark.parametrize("S", [[2], [3, 7], [1, 5, 9], [2, 4, 6, 8], [1, 3, 5, 7, 9, 11],
+                               [2, 6, 10, 14, 18, 22, 26, 30]])
+def test_gcv_selection_meets_the_stopping_rule_and_fits_unpenalised_terms(dev500, S):
     data = dev500["data"]
     resid, info = orc.regrxons(data, 0, S)
     lsp = info["log_sp"]
-    s0, _, _ = orc.gcv_eval(data, 0, S, lsp)
+    s0, g, _ = orc.gcv_eval(data, 0, S, lsp)
     assert abs(s0 - info["score"]) <= 1e-12 * s0
-    for i in range(len(lsp)):
-        for d in (-0.05, 0.05):
-            e = np.zeros(len(lsp))
-            e[i] = d
-            s1, _, _ = orc.gcv_eval(data, 0, S, lsp + e)
-            assert s1 >= s0 * (1 - 1e-9)
+    # magic's stopping rule on its own gradient (mgcv_multi_penalty_cpp.cpp:1225-1244)
+    assert np.linalg.norm(g) <= 1e-7 ** (1.0 / 3.0) * (1.0 + abs(s0))
+    if len(S) <= 3:
+        # full-rank end points here: a local minimum along every coordinate.
+        # (With m >= 4 some log sp run to ~24, the numerical rank drops and
+        # magic's truncated-basis gradient is no longer the score's exact
+        # derivative, so mgcv stops where the exact slope is ~1e-3.)
+        for i in range(len(lsp)):
+            for d in (-0.05, 0.05):
+                e = np.zeros(len(lsp))
+                e[i] = d
+                s1, _, _ = orc.gcv_eval(data, 0, S, lsp + e)
+                assert s1 >= s0 * (1 - 1e-9)
     # intercept and the linear null-space terms are unpenalised: residual is
     # orthogonal to them
     y = data[:, 0]
@@ -173,3 +182,30 @@ def test_oracle_skeleton_reproduces_500x50_fixture(dev500):
     worst = max(abs(mine[k] - ref[k]) for k in ref)
     # the fixture's own eigenpairs carry RSpectra's tol = 1e-10 noise
     assert worst <= 1e-8, worst
+
+
+@pytest.mark.parametrize("S", [[2, 4, 6, 8], [1, 3, 5, 7, 9, 11], [2, 6, 10, 14, 18, 22, 26, 30]])
+def test_multi_penalty_fit_is_equivariant_under_column_order_and_response_scale(dev500, S):
+    """|S| >= 4 (m >= 4 penalties), where no reference fixture reaches: the
+    additive fit does not depend on the order of the conditioning columns
+    (reversing them permutes the penalty blocks and the selected log sp the
+    same way, residuals unchanged) and scales with the response (y -> 3 y:
+    same log sp, residuals x 3, score x 9).  A block-offset or penalty-index
+    mistake breaks the first, a response-dependent initial.sp the second."""
+    data = np.array(dev500["data"])
+    base, info = orc.regrxons(data, 0, S)
+    # reverse the conditioning columns in place: sorted S now names them in
+    # the opposite order
+    d2 = data.copy()
+    d2[:, S] = data[:, S[::-1]]
+    r2, i2 = orc.regrxons(d2, 0, S)
+    np.testing.assert_allclose(r2, base, rtol=0, atol=1e-7 * np.abs(base).max())
+    assert abs(i2["score"] - info["score"]) <= 1e-9 * info["score"]
+    flat = np.abs(info["log_sp"]) > 12  # effectively unpenalised or fully penalised blocks
+    np.testing.assert_allclose(i2["log_sp"][::-1][~flat], info["log_sp"][~flat], atol=1e-4)
+    d3 = data.copy()
+    d3[:, 0] *= 3.0
+    r3, i3 = orc.regrxons(d3, 0, S)
+    np.testing.assert_allclose(r3, 3.0 * base, rtol=0, atol=1e-8 * np.abs(3.0 * base).max())
+    assert abs(i3["score"] - 9.0 * info["score"]) <= 1e-9 * i3["score"]
+    np.testing.assert_allclose(i3["log_sp"], info["log_sp"], atol=1e-6)
