@@ -28,6 +28,8 @@ class KpcOptions(ctypes.Structure):
         ("seed", ctypes.c_ulonglong),
         ("mode", ctypes.c_int),
         ("plain_test_limit", ctypes.c_int),
+        ("perm_stream", ctypes.c_int),
+        ("r_state", ctypes.POINTER(ctypes.c_int)),
     ]
 
 
@@ -77,6 +79,7 @@ class KpcResult(ctypes.Structure):
         ("level_tasks_ignored", ctypes.c_longlong * 64),
         ("level_rounds", ctypes.c_longlong * 64),
         ("task_perm_seed", ctypes.POINTER(ctypes.c_ulonglong)),
+        ("r_state_out", ctypes.c_int * 625),
     ]
 
 
@@ -84,7 +87,7 @@ EXPORTS = (
     "kpc_default_options", "kpc_skeleton_run", "kpc_result_free", "kpc_ci_batch",
     "kpc_regrxons_batch", "kpc_regrxons_mixed", "kpc_gcv_grad_hess", "kpc_dcov_components",
     "kpc_last_fit_timing", "kpc_tuning_report", "kpc_task_seed", "kpc_permutation_table", "kpc_last_error",
-    "kpc_device_count", "kpc_empty_cache", "kpc_dcov_components_noalloc",
+    "kpc_device_count", "kpc_empty_cache", "kpc_dcov_components_noalloc", "kpc_ci_batch_rs",
 )
 
 _lib = None
@@ -110,6 +113,8 @@ def lib():
     L.kpc_result_free.restype = None
     L.kpc_ci_batch.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.POINTER(KpcOptions),
                                ctypes.c_int, _I, _I, ctypes.c_int, _I, _D]
+    L.kpc_ci_batch_rs.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.POINTER(KpcOptions),
+                                  ctypes.c_int, _I, _I, ctypes.c_int, _I, _D, _I]
     L.kpc_regrxons_batch.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I,
                                      ctypes.c_int, _I, _D, _D]
     L.kpc_regrxons_mixed.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I, _I, _I, _I,

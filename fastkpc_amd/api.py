@@ -59,6 +59,22 @@ def _options(n, p, alpha, options):
         raise ValueError("mode must be 'batched' or 'plain'")
     o.mode = 0 if mode == "batched" else 1
     o.plain_test_limit = int(options.get("plain_test_limit", 0))
+    # permutation indices: "philox" -- the counter-based generator keyed by
+    # each test's seed (default); "R" -- R's global Mersenne-Twister stream
+    # after set.seed(seed) (or from r_state, .Random.seed[2:626]), drawn test
+    # by test in pcalg's execution order, as kpcalg and the reference's strict
+    # route draw them (fastkpc/src/full_cuda_ci_one_call.cpp:464-567)
+    rng = perm.get("rng", "philox")
+    if rng not in ("philox", "R"):
+        raise ValueError("permutation_params$rng must be 'philox' or 'R'")
+    o.perm_stream = 1 if rng == "R" else 0
+    if rng == "R" and perm.get("r_state") is not None:
+        st = np.ascontiguousarray(np.asarray(perm["r_state"], dtype=np.int64).astype(np.uint32)
+                                  .view(np.int32))
+        if st.shape != (625,):
+            raise ValueError("r_state must hold 625 integers (.Random.seed[2:626])")
+        o._r_state_keep = st  # keeps the array alive while the options are in use
+        o.r_state = iptr(st)
     return o
 
 
@@ -213,6 +229,9 @@ def fastkpc_compatible_cuda_skeleton(data, alpha, labels=None, options=None):
             "graph_launches": r.stage_launches[7],
             "labels": list(labels),
         }
+        if o.perm_stream == 1 and method in ("dcc.perm", "hsic.perm"):
+            # the stream after the run (.Random.seed[2:626]), as unsigned words
+            summary["r_state_out"] = [int(v) & 0xFFFFFFFF for v in list(r.r_state_out)]
         return {
             "adjacency": adj,
             "sepsets": sepsets,
@@ -228,22 +247,30 @@ def fastkpc_compatible_cuda_skeleton(data, alpha, labels=None, options=None):
 
 
 def ci_batch(data, x, y, S, num_col=35, ci_method="dcc.gamma", alpha=0.1, sig=1.0,
-             replicates=100, seed=None, index=1.0, include_observed=True):
+             replicates=100, seed=None, index=1.0, include_observed=True, rng="philox",
+             r_state=None, return_state=False):
     """Batch of CI tests (x_i, y_i | S_i) sharing |S| (0-based indices):
-    returns (count, 4) {p, statistic, mean, variance}."""
+    returns (count, 4) {p, statistic, mean, variance}.  rng="R": the
+    permutation tests draw from R's stream after set.seed(seed) (or r_state)
+    in batch order, as consecutive kernelCItest calls in R; return_state also
+    returns the stream afterwards (625 unsigned words)."""
     d = _as_data(data)
     n, p = d.shape
     x = np.ascontiguousarray(x, dtype=np.int32)
     y = np.ascontiguousarray(y, dtype=np.int32)
     S = np.ascontiguousarray(np.asarray(S, dtype=np.int32).reshape(len(x), -1))
-    perm = {"replicates": replicates, "include_observed": include_observed}
+    perm = {"replicates": replicates, "include_observed": include_observed, "rng": rng,
+            "r_state": r_state}
     if seed is not None:
         perm["seed"] = seed
     o = _options(n, p, alpha, {"numCol": num_col, "ci_method": ci_method, "index": index,
                                "hsic_params": {"sig": sig}, "permutation_params": perm})
     out = np.zeros((len(x), 4))
-    check(_lib.lib().kpc_ci_batch(dptr(d), n, p, o, len(x), iptr(x), iptr(y), S.shape[1],
-                                  iptr(S), dptr(out)))
+    st = np.zeros(625, dtype=np.int32)
+    check(_lib.lib().kpc_ci_batch_rs(dptr(d), n, p, o, len(x), iptr(x), iptr(y), S.shape[1],
+                                     iptr(S), dptr(out), iptr(st)))
+    if return_state:
+        return out, [int(v) & 0xFFFFFFFF for v in st]
     return out
 
 

@@ -24,6 +24,7 @@
 #include "../../include/kpcmi.h"
 #include "kpc_common.h"
 #include "kpc_device.h"
+#include "kpc_rstream.h"
 #include "kpc_tune.h"
 
 #include <hip/hip_runtime.h>
@@ -294,6 +295,7 @@ struct Task {
   uint64_t mask;
   double p = 0.0, stat = 0.0, mean = 0.0, var = 0.0;
   double margin = 1e300;  // smallest decision margin (fits, inchol pivots, permutation ties)
+  long long rtab = -1;    // R-stream permutation table (offset in the engine's host tables)
   bool computed = false;
   bool consumed = false;
   bool deleted = false;
@@ -360,6 +362,15 @@ class Engine {
     // energy::dcov.test falls back to index 1 outside [0, 2]
     index_ = (opt.index >= 0.0 && opt.index <= 2.0) ? opt.index : 1.0;
     seed_ = opt.has_seed ? (uint64_t)opt.seed : 0ULL;
+    if (opt.perm_stream != KPC_PERM_COUNTER && opt.perm_stream != KPC_PERM_R_STREAM)
+      throw KpcError(-1, "perm_stream must be KPC_PERM_COUNTER or KPC_PERM_R_STREAM");
+    rmode_ = opt.perm_stream == KPC_PERM_R_STREAM &&
+             (method_ == KPC_DCC_PERM || method_ == KPC_HSIC_PERM);
+    if (rmode_) {
+      if (opt.r_state) rs_.load(opt.r_state);
+      else if (opt.has_seed) rs_.set_seed((uint32_t)opt.seed);
+      else throw KpcError(-1, "R-stream permutations need a seed or a stream state");
+    }
     comp_k_ = method_ == KPC_DCC_PERM ? 2 : opt.num_col;
     for (long long i = 0; i < (long long)n * p; ++i)
       if (!std::isfinite(h_data[i])) throw KpcError(-1, "data must be finite");
@@ -402,6 +413,33 @@ class Engine {
     if (stream2_) (void)hipStreamSynchronize(stream2_);
     if (d_data_) pool_release(d_data_, data_bytes_);
     if (res_) res_->in_use = false;  // stream and events stay with the thread
+  }
+
+  // ---- R's global stream (KPC_PERM_R_STREAM) --------------------------
+  bool r_stream() const { return rmode_; }
+  kpc::RStream::State rstate() const { return rs_.state(); }
+  void set_rstate(const kpc::RStream::State& s) { rs_.restore(s); }
+  void save_rstate(int* out) const { rs_.save(out); }
+  void clear_tables() { rtab_host_.clear(); }
+  // the test's R x n index table, drawn now from the stream: dcc.perm as
+  // energy dcov.test (one index vector mutated by every replicate),
+  // hsic.perm as sample(n) per replicate
+  void draw_table(Task& t) {
+    const int R = opt_.replicates;
+    t.rtab = (long long)rtab_host_.size();
+    rtab_host_.resize(rtab_host_.size() + (size_t)R * n_);
+    unsigned short* out = rtab_host_.data() + t.rtab;
+    pool_.resize((size_t)n_);
+    if (method_ == KPC_DCC_PERM) {
+      std::vector<int>& J = pool_;
+      for (int i = 0; i < n_; ++i) J[i] = i;
+      for (int r = 0; r < R; ++r) {
+        rs_.energy_permute(n_, J.data());
+        for (int i = 0; i < n_; ++i) out[(size_t)r * n_ + i] = (unsigned short)J[i];
+      }
+    } else {
+      for (int r = 0; r < R; ++r) rs_.sample(n_, pool_.data(), out + (size_t)r * n_);
+    }
   }
 
   // ---- per-level arenas / caches -------------------------------------
@@ -460,14 +498,23 @@ class Engine {
   }
 
   // Evaluate p-values for the given tasks (all with |S| == level_).
-  void evaluate(std::vector<Task*>& all) {
+  // with_pairs = false: fits and components only (the tests' statistics are
+  // computed by a later call)
+  void evaluate(std::vector<Task*>& all, bool with_pairs = true) {
     if (all.empty()) return;
     const double t0 = now_s();
     // orientation-free identity: reuse a twin's result, evaluate each once
+    // (not for R-stream tests: each draws its own indices and permutes y)
     std::vector<Task*> tasks;
     std::vector<std::pair<Task*, size_t>> dups;  // (task, index into tasks)
     in_batch_.clear();
+    pair_rtab_.clear();
     for (Task* t : all) {
+      if (rmode_ || !with_pairs) {
+        if (with_pairs) pair_rtab_.push_back(t->rtab);
+        tasks.push_back(t);
+        continue;
+      }
       const auto key = twin_key(*t);
       if (const TestResult* done = twin_map_.find(key)) {
         apply_result(*t, *done);
@@ -530,8 +577,10 @@ class Engine {
       int rslot[2] = {-1, -1};
       if (perm) seeds.push_back(task_seed(seed_, method_, t->x, t->y, t->S, t->slen));
       for (int side = 0; side < 2; ++side) {
-        // side 0: min(x, y), side 1: max(x, y) (the permuted variable)
-        const int tgt = side ? std::max(t->x, t->y) : std::min(t->x, t->y);
+        // side 0: min(x, y), side 1: max(x, y) (the permuted variable); R
+        // stream: x, then y (kpcalg permutes the test's y)
+        const int tgt = rmode_ ? (side ? t->y : t->x)
+                               : (side ? std::max(t->x, t->y) : std::min(t->x, t->y));
         auto key = std::make_pair(tgt, t->mask);
         if (const int* it = comp_map_.find(key)) {
           slots[side] = *it;
@@ -571,7 +620,7 @@ class Engine {
       stats_.components += (long long)W[li].comp_dst.size();
       stats_.setups += (long long)W[li].geo_items.size();
     }
-    stats_.physical_tests += (long long)tasks.size();
+    if (with_pairs) stats_.physical_tests += (long long)tasks.size();
     stage_host_ += now_s() - t0;
     const double* comp_source = level_ == 0 ? d_data_ : resid_.ptr;
     // uploads from pageable memory first, while no lane has work queued
@@ -636,7 +685,7 @@ class Engine {
       if (!graph) hip_check(hipEventRecord(L.ev[3], L.stream), "event");
       if (!w.comp_dst.empty()) launch_components(L, comp_source, w.comp_dst.size());
       if (!graph) hip_check(hipEventRecord(L.ev[4], L.stream), "event");
-      launch_pairs(L, (int)w.pair_x.size());
+      if (with_pairs) launch_pairs(L, (int)w.pair_x.size());
       if (graph) {
         hip_check(hipStreamEndCapture(L.stream, &wave_graph), "end capture");
         hipGraphExec_t exec = nullptr;
@@ -655,6 +704,7 @@ class Engine {
     for (int li = 0; li < nlanes; ++li) {
       const Lane L = lane(li);
       LaneWork& w = W[li];
+      if (!with_pairs) continue;
       w.out.resize(w.pair_x.size() * kpc::kPairOut);
       hip_check(hipMemcpyAsync(w.out.data(), L.pair_out->ptr, sizeof(double) * w.out.size(),
                                hipMemcpyDeviceToHost, L.stream),
@@ -668,7 +718,7 @@ class Engine {
       const Lane L = lane(li);
       const LaneWork& w = W[li];
       const bool launched[5] = {w.nsm > 0, !w.geo_items.empty(), !w.fit_items.empty(),
-                                !w.comp_dst.empty(), true};
+                                !w.comp_dst.empty(), with_pairs};
       if (graph) {
         // one graph per wave: only the whole pipeline is timed
         hip_check(hipEventElapsedTime(&ms, L.ev[0], L.ev[5]), "event elapsed");
@@ -696,6 +746,7 @@ class Engine {
       check_device_status(fits, geos, W[0].nsm + W[1].nsm, &rinfo_h);
       check_component_status(dst);
     }
+    if (!with_pairs) return;
     std::vector<double> cmargin_h;
     if (method_ == KPC_HSIC_GAMMA || method_ == KPC_HSIC_PERM) {
       cmargin_h.resize((size_t)n_comp_);
@@ -717,7 +768,7 @@ class Engine {
         if (!cmargin_h.empty())
           res.margin = std::min(res.margin, std::min(cmargin_h[w.pair_x[j]], cmargin_h[w.pair_y[j]]));
         apply_result(*t, res);
-        twin_map_.emplace(twin_key(*t), res);
+        if (!rmode_) twin_map_.emplace(twin_key(*t), res);
       }
     }
     for (auto& d : dups) apply_result(*d.first, *twin_map_.find(twin_key(*tasks[d.second])));
@@ -819,6 +870,10 @@ class Engine {
 
   void perm_pairs(int count) {
     const bool hsic = method_ == KPC_HSIC_PERM;
+    if (rmode_)
+      for (int t = 0; t < count; ++t)
+        if ((size_t)t >= pair_rtab_.size() || pair_rtab_[t] < 0)
+          throw KpcError(-4, "R-stream test has no drawn permutation table");
     const int R = opt_.replicates;
     const double scale = hsic ? 1.0 : 1.0 / (double)n_;
     const long long budget = perm_chunk_replicates();
@@ -885,10 +940,27 @@ class Engine {
         d_perm_units_.upload(segs, stream_);
         const kpc::PermUnit* U = d_perm_units_.ptr;
         hip_check(hipEventRecord(tev_[0], stream_), "event");
-        hip_check(kpc::launch_perm_generate(U + ns, (int)ng, d_seeds_.ptr, n_, gw,
-                                            perm_scratch_.ptr, (int)std::min<size_t>(ng, 8192),
-                                            stream_),
-                  "permutation generate launch");
+        std::vector<unsigned short> staged;
+        if (rmode_) {
+          // the R stream's tables, drawn on the host in execution order:
+          // replicates r .. r + take of test t go to slots slot0 ..
+          staged.resize((size_t)used * n_);
+          for (size_t i = 0; i < ns; ++i) {
+            const kpc::PermUnit& sg = segs[i];
+            std::memcpy(staged.data() + (size_t)sg.slot * n_,
+                        rtab_host_.data() + pair_rtab_[sg.test] + (size_t)sg.r0 * n_,
+                        sizeof(unsigned short) * (size_t)sg.nr * n_);
+          }
+          hip_check(hipMemcpyAsync(perm_scratch_.ptr, staged.data(),
+                                   sizeof(unsigned short) * staged.size(), hipMemcpyHostToDevice,
+                                   stream_),
+                    "R-stream permutation upload");
+        } else {
+          hip_check(kpc::launch_perm_generate(U + ns, (int)ng, d_seeds_.ptr, n_, gw,
+                                              perm_scratch_.ptr, (int)std::min<size_t>(ng, 8192),
+                                              stream_),
+                    "permutation generate launch");
+        }
         hip_check(hipEventRecord(tev_[1], stream_), "event");
         perm_values(U + ns + ng, (int)nv, perm_scratch_.ptr, perm_vals_.ptr, vb);
         hip_check(hipEventRecord(tev_[2], stream_), "event");
@@ -900,7 +972,7 @@ class Engine {
         const double reps = (double)used;
         if ((gtrial || vtrial) && reps * n_ >= 4e6) {
           float ms = 0.f;
-          if (gtrial) {
+          if (gtrial && !rmode_) {
             hip_check(hipEventElapsedTime(&ms, tev_[0], tev_[1]), "event elapsed");
             tuner.record(gkey, gw, ms * 1e-3, reps);
           }
@@ -1202,6 +1274,13 @@ class Engine {
   DevArray<double> comp_rows_, perm_vals_, perm_obs_, perm_state_;
   DevArray<unsigned short> perm_scratch_;
   DevArray<kpc::PermUnit> d_perm_units_;
+  // R's global stream (KPC_PERM_R_STREAM): the stream, the tables drawn for
+  // the current wave and each pair's table offset
+  bool rmode_ = false;
+  kpc::RStream rs_;
+  std::vector<unsigned short> rtab_host_;
+  std::vector<long long> pair_rtab_;
+  std::vector<int> pool_;
   double stage_[5] = {0, 0, 0, 0, 0};
   long long launches_[5] = {0, 0, 0, 0, 0};
   double stage_host_ = 0.0;
@@ -1373,103 +1452,186 @@ void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
       eng.reserve_level(ud.size(), ur.size());
       level_unique_resid = ur.size();
     }
-    // ---- frontier waves
-    for (EdgeRun& e : edges) {
-      e.side = 0;
-      e.cursor = 0;
-      if (e.o[0].nbrs > ord) done = false;
-      if (e.o[0].count == 0) {
-        e.side = 1;
-        if (e.o[1].nbrs > ord) done = false;
-        if (e.o[1].count == 0) e.state = 2;
-      }
-    }
-    // frontier chunk: tasks taken per undecided orientation in the first
-    // wave, doubled every wave (KPC_WAVE_CHUNK overrides; <= 0 evaluates the
-    // whole level speculatively in one wave)
-    int chunk = level_unique_resid <= speculate_limit ? 0 : 4;
-    if (const char* env = std::getenv("KPC_WAVE_CHUNK")) chunk = std::atoi(env);
-    if (chunk <= 0) chunk = 1 << 30;
-    const int batched_chunk = chunk;
-    bool plain_now = plain;
-    if (plain_now) chunk = 1;
-    bool speculate_all = chunk >= (1 << 30);
-    long long level_plain = 0;
-    bool active = true;
-    std::vector<Task*> batch;
-    while (active) {
-      batch.clear();
-      for (EdgeRun& e : edges) {
-        if (e.state != 0) continue;
-        for (int side = e.side; side < 2; ++side) {
-          const Orientation& o = e.o[side];
-          int taken = 0;
-          for (int c = side == e.side ? e.cursor : 0; c < o.count && taken < chunk; ++c) {
-            Task& t = tasks[(size_t)o.first + c];
-            if (!t.computed) {
-              batch.push_back(&t);
-              ++taken;
-            }
-          }
-          if (!speculate_all) break;
-        }
-        if (plain_now && !batch.empty()) break;
-      }
+    if (eng.r_stream()) {
+      // R-stream permutations: the tests run in pcalg's execution order
+      // (orientations (x, y) by x, then y; each orientation's conditioning
+      // sets in lexicographic order; fastkpc/src/full_cuda_ci_one_call.cpp:
+      // 5301-5330 with pcalg_task_less :326-336) and each draws its
+      // replicates from the stream as it runs.  Every fit and component of the
+      // level is computed first in one wave; then a wave takes the next
+      // `chunk` tests of one orientation, drawn as if all of them ran, and the
+      // stream is rewound to the end of the last test consumed (a deletion
+      // ends the orientation and drops the tests drawn after it).
+      std::vector<Task*> all;
+      all.reserve(tasks.size());
+      for (Task& t : tasks) all.push_back(&t);
+      eng.evaluate(all, false);
       ++r->level_rounds[ord];
-      if (plain_now) {
-        // one launch sequence per test: fresh caches for every test
-        eng.begin_level(ord);
-        const double tp0 = now_s();
-        eng.evaluate(batch);
-        r->plain_seconds[ord] += now_s() - tp0;
-        r->plain_tests[ord] += (long long)batch.size();
-        ++level_plain;
-        plain_tests += (long long)batch.size();
-        if (opt.plain_test_limit > 0 && level_plain >= opt.plain_test_limit) {
-          plain_now = false;  // rest of the level runs batched
-          chunk = batched_chunk;
-          speculate_all = chunk >= (1 << 30);
-          eng.begin_level(ord);
-        }
-      } else {
-        eng.evaluate(batch);
-      }
-      // replay in canonical order
-      active = false;
-      for (EdgeRun& e : edges) {
-        while (e.state == 0) {
-          const Orientation& o = e.o[e.side];
-          if (e.cursor < o.count) {
-            Task& t = tasks[(size_t)o.first + e.cursor];
-            if (!t.computed) break;
+      std::vector<std::pair<int, int>> order;  // (edge, side)
+      for (size_t ei = 0; ei < edges.size(); ++ei)
+        for (int side = 0; side < 2; ++side) order.emplace_back((int)ei, side);
+      std::sort(order.begin(), order.end(),
+                [&](const std::pair<int, int>& u, const std::pair<int, int>& v) {
+                  const Orientation& a = edges[u.first].o[u.second];
+                  const Orientation& b = edges[v.first].o[v.second];
+                  return a.x != b.x ? a.x < b.x : a.y < b.y;
+                });
+      std::vector<Task*> batch;
+      std::vector<kpc::RStream::State> snaps;
+      for (const auto& os : order) {
+        EdgeRun& e = edges[os.first];
+        if (e.state == 1) continue;  // deleted by its other orientation
+        const Orientation& o = e.o[os.second];
+        if (o.count == 0) continue;  // fewer neighbours than the level
+        if (o.nbrs > ord) done = false;
+        int c = 0, chunk = 4;
+        bool deleted = false;
+        while (c < o.count && !deleted) {
+          const int take = std::min(chunk, o.count - c);
+          batch.clear();
+          snaps.clear();
+          eng.clear_tables();
+          for (int j = 0; j < take; ++j) {
+            Task& t = tasks[(size_t)o.first + c + j];
+            snaps.push_back(eng.rstate());
+            eng.draw_table(t);
+            batch.push_back(&t);
+          }
+          snaps.push_back(eng.rstate());
+          eng.evaluate(batch);
+          ++r->level_rounds[ord];
+          int used = take;
+          for (int j = 0; j < take; ++j)
+            if (batch[j]->p >= opt.alpha) {
+              used = j + 1;
+              deleted = true;
+              break;
+            }
+          eng.set_rstate(snaps[used]);
+          for (int j = 0; j < used; ++j) {
+            Task& t = *batch[j];
             t.consumed = true;
             ++r->n_edgetests[ord];
             double& pm = r->pmax[(size_t)t.x * p + t.y];
             if (pm < t.p) pm = t.p;
-            if (t.p >= opt.alpha) {
-              t.deleted = true;
-              e.state = 1;
-              r->adjacency[(size_t)e.a * p + e.b] = 0;  // applied after the level via snap
-              r->sepset_len[(size_t)e.a * p + e.b] = r->sepset_len[(size_t)e.b * p + e.a] = ord;
-              for (int i = 0; i < ord; ++i) {
-                r->sepset[((size_t)e.a * p + e.b) * 64 + i] = t.S[i];
-                r->sepset[((size_t)e.b * p + e.a) * 64 + i] = t.S[i];
-              }
-              break;
-            }
-            ++e.cursor;
-          } else if (e.side == 0) {
-            e.side = 1;
-            e.cursor = 0;
-            if (e.o[1].nbrs > ord) done = false;
-            if (e.o[1].count == 0) e.state = 2;
-          } else {
-            e.state = 2;
           }
+          for (int j = used; j < take; ++j) batch[j]->computed = false;  // drawn, not run
+          if (deleted) {
+            const Task& t = *batch[used - 1];
+            batch[used - 1]->deleted = true;
+            e.state = 1;
+            e.side = os.second;
+            e.cursor = c + used - 1;
+            r->adjacency[(size_t)e.a * p + e.b] = 0;  // applied after the level via snap
+            r->sepset_len[(size_t)e.a * p + e.b] = r->sepset_len[(size_t)e.b * p + e.a] = ord;
+            for (int i = 0; i < ord; ++i) {
+              r->sepset[((size_t)e.a * p + e.b) * 64 + i] = t.S[i];
+              r->sepset[((size_t)e.b * p + e.a) * 64 + i] = t.S[i];
+            }
+          }
+          c += used;
+          chunk = std::min(chunk * 2, 1 << 12);
         }
-        if (e.state == 0) active = true;
       }
-      if (!plain) chunk = std::min(chunk * 2, 1 << 20);
+    } else {
+      // ---- frontier waves
+      for (EdgeRun& e : edges) {
+        e.side = 0;
+        e.cursor = 0;
+        if (e.o[0].nbrs > ord) done = false;
+        if (e.o[0].count == 0) {
+          e.side = 1;
+          if (e.o[1].nbrs > ord) done = false;
+          if (e.o[1].count == 0) e.state = 2;
+        }
+      }
+      // frontier chunk: tasks taken per undecided orientation in the first
+      // wave, doubled every wave (KPC_WAVE_CHUNK overrides; <= 0 evaluates the
+      // whole level speculatively in one wave)
+      int chunk = level_unique_resid <= speculate_limit ? 0 : 4;
+      if (const char* env = std::getenv("KPC_WAVE_CHUNK")) chunk = std::atoi(env);
+      if (chunk <= 0) chunk = 1 << 30;
+      const int batched_chunk = chunk;
+      bool plain_now = plain;
+      if (plain_now) chunk = 1;
+      bool speculate_all = chunk >= (1 << 30);
+      long long level_plain = 0;
+      bool active = true;
+      std::vector<Task*> batch;
+      while (active) {
+        batch.clear();
+        for (EdgeRun& e : edges) {
+          if (e.state != 0) continue;
+          for (int side = e.side; side < 2; ++side) {
+            const Orientation& o = e.o[side];
+            int taken = 0;
+            for (int c = side == e.side ? e.cursor : 0; c < o.count && taken < chunk; ++c) {
+              Task& t = tasks[(size_t)o.first + c];
+              if (!t.computed) {
+                batch.push_back(&t);
+                ++taken;
+              }
+            }
+            if (!speculate_all) break;
+          }
+          if (plain_now && !batch.empty()) break;
+        }
+        ++r->level_rounds[ord];
+        if (plain_now) {
+          // one launch sequence per test: fresh caches for every test
+          eng.begin_level(ord);
+          const double tp0 = now_s();
+          eng.evaluate(batch);
+          r->plain_seconds[ord] += now_s() - tp0;
+          r->plain_tests[ord] += (long long)batch.size();
+          ++level_plain;
+          plain_tests += (long long)batch.size();
+          if (opt.plain_test_limit > 0 && level_plain >= opt.plain_test_limit) {
+            plain_now = false;  // rest of the level runs batched
+            chunk = batched_chunk;
+            speculate_all = chunk >= (1 << 30);
+            eng.begin_level(ord);
+          }
+        } else {
+          eng.evaluate(batch);
+        }
+        // replay in canonical order
+        active = false;
+        for (EdgeRun& e : edges) {
+          while (e.state == 0) {
+            const Orientation& o = e.o[e.side];
+            if (e.cursor < o.count) {
+              Task& t = tasks[(size_t)o.first + e.cursor];
+              if (!t.computed) break;
+              t.consumed = true;
+              ++r->n_edgetests[ord];
+              double& pm = r->pmax[(size_t)t.x * p + t.y];
+              if (pm < t.p) pm = t.p;
+              if (t.p >= opt.alpha) {
+                t.deleted = true;
+                e.state = 1;
+                r->adjacency[(size_t)e.a * p + e.b] = 0;  // applied after the level via snap
+                r->sepset_len[(size_t)e.a * p + e.b] = r->sepset_len[(size_t)e.b * p + e.a] = ord;
+                for (int i = 0; i < ord; ++i) {
+                  r->sepset[((size_t)e.a * p + e.b) * 64 + i] = t.S[i];
+                  r->sepset[((size_t)e.b * p + e.a) * 64 + i] = t.S[i];
+                }
+                break;
+              }
+              ++e.cursor;
+            } else if (e.side == 0) {
+              e.side = 1;
+              e.cursor = 0;
+              if (e.o[1].nbrs > ord) done = false;
+              if (e.o[1].count == 0) e.state = 2;
+            } else {
+              e.state = 2;
+            }
+          }
+          if (e.state == 0) active = true;
+        }
+        if (!plain) chunk = std::min(chunk * 2, 1 << 20);
+      }
     }
     for (int a = 0; a < p; ++a)
       for (int b = 0; b < p; ++b)
@@ -1571,6 +1733,7 @@ int kpc_skeleton_run(const double* data, int n, int p, const kpc_options* opt, k
     r->stage_seconds[5] = eng.stage_host();
     r->stage_seconds[7] = eng.graph_seconds();
     r->stage_launches[7] = eng.graph_launches();
+    if (eng.r_stream()) eng.save_rstate(r->r_state_out);
     r->physical_tests = eng.stats_.physical_tests;
     r->residual_fits = eng.stats_.fits;
     r->components = eng.stats_.components;
@@ -1608,6 +1771,12 @@ void kpc_result_free(kpc_result* r) {
 
 int kpc_ci_batch(const double* data, int n, int p, const kpc_options* opt, int count, const int* x,
                  const int* y, int s_len, const int* S, double* out) {
+  return kpc_ci_batch_rs(data, n, p, opt, count, x, y, s_len, S, out, nullptr);
+}
+
+int kpc_ci_batch_rs(const double* data, int n, int p, const kpc_options* opt, int count,
+                    const int* x, const int* y, int s_len, const int* S, double* out,
+                    int* r_state_out) {
   try {
     if (!data || !opt || !out || count < 0 || s_len < 0 || s_len > 62)
       throw KpcError(-1, "invalid arguments");
@@ -1631,8 +1800,12 @@ int kpc_ci_batch(const double* data, int n, int p, const kpc_options* opt, int c
     }
     eng.reserve_level((size_t)count, (size_t)count * 2);
     std::vector<Task*> ptrs;
-    for (Task& t : tasks) ptrs.push_back(&t);
+    for (Task& t : tasks) {
+      if (eng.r_stream()) eng.draw_table(t);  // batch order = draw order
+      ptrs.push_back(&t);
+    }
     eng.evaluate(ptrs);
+    if (eng.r_stream() && r_state_out) eng.save_rstate(r_state_out);
     for (int i = 0; i < count; ++i) {
       out[4 * i + 0] = tasks[i].p;
       out[4 * i + 1] = tasks[i].stat;

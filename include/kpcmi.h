@@ -38,6 +38,12 @@ extern "C" {
 
 enum kpc_ci_method { KPC_DCC_GAMMA = 0, KPC_HSIC_GAMMA = 1, KPC_DCC_PERM = 2, KPC_HSIC_PERM = 3 };
 enum kpc_exec_mode { KPC_BATCHED = 0, KPC_PLAIN = 1 };
+/* where the permutation methods take their indices from */
+enum kpc_perm_stream {
+  KPC_PERM_COUNTER = 0,  /* Philox4x32-10 keyed by the task seed (any order, on the device) */
+  KPC_PERM_R_STREAM = 1  /* R's global Mersenne-Twister stream, drawn test by test in
+                            pcalg's execution order as kpcalg does (host generated) */
+};
 
 typedef struct kpc_options {
   int ci_method;             /* kpc_ci_method */
@@ -52,6 +58,16 @@ typedef struct kpc_options {
   unsigned long long seed;
   int mode;                  /* kpc_exec_mode */
   int plain_test_limit;      /* KPC_PLAIN only: per-level cap on plain tests (0 = all) */
+  /* KPC_PERM_R_STREAM replaces fastkpc's strict-route tables
+   * (fastkpc/src/full_cuda_ci_one_call.cpp:464-567, drawn from R::unif_rand()
+   * after the R facade's set.seed(seed), fastkpc/R/cuda_native.R:2291-2293):
+   * dcc.perm draws energy dcov.test's index vector (one per test, mutated by
+   * every replicate), hsic.perm draws sample(n) per replicate (R >= 3.6
+   * "Rejection" sampling), and the variable permuted is the test's y. */
+  int perm_stream;           /* kpc_perm_stream */
+  /* KPC_PERM_R_STREAM: the stream to start from as .Random.seed[2:626]
+   * (position, then the 624 state words); NULL = set.seed(seed) */
+  const int* r_state;
 } kpc_options;
 
 typedef struct kpc_result {
@@ -133,6 +149,9 @@ typedef struct kpc_result {
    * (dcc.perm / hsic.perm; 0 for the other methods); the trace column
    * permutation_seed (fastkpc/src/full_cuda_ci_one_call.cpp:5613-5616) */
   unsigned long long* task_perm_seed;
+  /* KPC_PERM_R_STREAM: the stream after the last test (.Random.seed[2:626]
+   * layout), for the caller to store back as R's RNGScope does */
+  int r_state_out[625];
 } kpc_result;
 
 void kpc_default_options(kpc_options* opt);
@@ -141,9 +160,15 @@ void kpc_result_free(kpc_result* r);
 
 /* One batch of CI tests (x_i, y_i | S_i), S given as count x s_len 0-based
  * indices (all tests share s_len).  out: count x 4 {p.value, statistic, mean,
- * variance}. */
+ * variance}.  With KPC_PERM_R_STREAM the tests draw from the stream in batch
+ * order, as consecutive kernelCItest calls in R would. */
 int kpc_ci_batch(const double* data, int n, int p, const kpc_options* opt, int count,
                  const int* x, const int* y, int s_len, const int* S, double* out);
+/* kpc_ci_batch that also returns the R stream after the batch (r_state_out:
+ * 625 ints, .Random.seed[2:626]; untouched unless KPC_PERM_R_STREAM). */
+int kpc_ci_batch_rs(const double* data, int n, int p, const kpc_options* opt, int count,
+                    const int* x, const int* y, int s_len, const int* S, double* out,
+                    int* r_state_out);
 
 /* regrXonS residuals of `targets` on S (count x s_len).  resid: count x n;
  * info: count x 32 {status, score, rss, edf, iterations, evaluations,

@@ -51,6 +51,32 @@ void orc_permutation(unsigned long long seed, int replicate, int n, int* perm);
 /* out = {p.value, n V^2, mean(rep), var(rep), margin} */
 int orc_dcc_perm(const double* x, const double* y, int n, double index, int R,
                  int include_observed, unsigned long long seed, double out[5]);
+/* the same with an explicit R x n index table (y permuted as y[table[r][i]]) */
+int orc_dcc_perm_table(const double* x, const double* y, int n, double index, int R,
+                       int include_observed, unsigned long long seed, const int* table,
+                       double out[5]);
+void orc_hsic_perm_table(const double* gx, int rx, const double* gy, int ry, int n, int R,
+                         int include_observed, unsigned long long seed, const int* table,
+                         double out[5]);
+
+/* ---- R's global stream (Mersenne-Twister, set.seed) and kpcalg's
+ *      permutation draws (fastkpc/src/full_cuda_ci_one_call.cpp:464-567) ---- */
+typedef struct {
+  unsigned int mt[624];
+  int mti;
+} orc_rng;
+void orc_rng_set_seed(orc_rng* r, unsigned int seed);
+unsigned int orc_rng_word(orc_rng* r);
+double orc_unif_rand(orc_rng* r);
+/* R_unif_index(dn): rejection ("Rejection" sample.kind) or floor(dn u) */
+double orc_unif_index(orc_rng* r, double dn, int rounding);
+/* sample(n), 0-based; pool: n ints of scratch */
+void orc_r_sample(orc_rng* r, int n, int rounding, int* pool, int* out);
+/* energy's permute(J, n) */
+void orc_energy_permute(orc_rng* r, int* J, int n);
+/* the R x n index table one permutation test draws (method ORC_DCC_PERM:
+ * dcov.test; ORC_HSIC_PERM: hsic.perm); pool: n ints of scratch */
+void orc_r_perm_table(orc_rng* r, int method, int n, int R, int* pool, int* table);
 
 /* ---- regrXonS (kpcalg/R/regrXonS.R:33-61, mgcv gam GCV.Cp) ---- */
 typedef struct {
@@ -110,6 +136,10 @@ typedef struct {
   int threads;      /* OpenMP threads for the per-level precompute (0 = serial) */
   double index;     /* dcc.perm distance exponent */
   int include_observed;
+  /* 1: permutation indices from R's global stream after set.seed(perm_seed),
+   * tests run in pcalg's order (orientations by x, then y); 0: the
+   * counter-based indices of the task seed */
+  int r_stream;
 } orc_skeleton_options;
 
 typedef struct {
@@ -128,6 +158,7 @@ typedef struct {
    * its gamma moments; hsic.gamma: HSIC and its moments; *.perm: the observed
    * statistic and the replicates' mean / variance) */
   double* trace_stat; double* trace_mean; double* trace_var;
+  orc_rng rng_out;      /* r_stream: the stream after the last test */
 } orc_skeleton_result;
 
 int orc_skeleton(const double* data, int n, int p, const orc_skeleton_options* opt,

@@ -21,16 +21,20 @@ class OrcSkeletonOptions(ctypes.Structure):
                 ("num_col", ctypes.c_int), ("hsic_sig", ctypes.c_double),
                 ("replicates", ctypes.c_int), ("perm_seed", ctypes.c_ulonglong),
                 ("threads", ctypes.c_int), ("index", ctypes.c_double),
-                ("include_observed", ctypes.c_int)]
+                ("include_observed", ctypes.c_int), ("r_stream", ctypes.c_int)]
 
 
 METHODS = {"dcc.gamma": 0, "hsic.gamma": 1, "dcc.perm": 2, "hsic.perm": 3}
 
 
 def options(method="dcc.gamma", alpha=0.1, m_max=62, num_col=35, sig=1.0, replicates=100,
-            seed=0, threads=0, index=1.0, include_observed=True):
+            seed=0, threads=0, index=1.0, include_observed=True, r_stream=False):
     return OrcSkeletonOptions(METHODS[method], alpha, m_max, num_col, sig, replicates, seed,
-                              threads, index, 1 if include_observed else 0)
+                              threads, index, 1 if include_observed else 0, 1 if r_stream else 0)
+
+
+class OrcRng(ctypes.Structure):
+    _fields_ = [("mt", ctypes.c_uint * 624), ("mti", ctypes.c_int)]
 
 
 class OrcSkeletonResult(ctypes.Structure):
@@ -41,7 +45,7 @@ class OrcSkeletonResult(ctypes.Structure):
                 ("trace_level", _I), ("trace_x", _I), ("trace_y", _I), ("trace_slen", _I),
                 ("trace_S", _I), ("trace_p", _D), ("trace_del", ctypes.POINTER(ctypes.c_ubyte)),
                 ("trace_margin", _D), ("trace_stat", _D), ("trace_mean", _D),
-                ("trace_var", _D)]
+                ("trace_var", _D), ("rng_out", OrcRng)]
 
 
 class OrcSetup(ctypes.Structure):
@@ -115,6 +119,21 @@ def lib():
                                                  _D]
         L.orc_dcc_perm.argtypes = [_D, _D, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                    ctypes.c_int, ctypes.c_ulonglong, _D]
+        L.orc_rng_set_seed.argtypes = [ctypes.POINTER(OrcRng), ctypes.c_uint]
+        L.orc_rng_word.argtypes = [ctypes.POINTER(OrcRng)]
+        L.orc_rng_word.restype = ctypes.c_uint
+        L.orc_unif_rand.argtypes = [ctypes.POINTER(OrcRng)]
+        L.orc_unif_rand.restype = ctypes.c_double
+        L.orc_unif_index.argtypes = [ctypes.POINTER(OrcRng), ctypes.c_double, ctypes.c_int]
+        L.orc_unif_index.restype = ctypes.c_double
+        L.orc_r_sample.argtypes = [ctypes.POINTER(OrcRng), ctypes.c_int, ctypes.c_int, _I, _I]
+        L.orc_energy_permute.argtypes = [ctypes.POINTER(OrcRng), _I, ctypes.c_int]
+        L.orc_r_perm_table.argtypes = [ctypes.POINTER(OrcRng), ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int, _I, _I]
+        L.orc_dcc_perm_table.argtypes = [_D, _D, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_ulonglong, _I, _D]
+        L.orc_hsic_perm_table.argtypes = [_D, ctypes.c_int, _D, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong, _I, _D]
         _lib = L
     return _lib
 
@@ -320,11 +339,13 @@ def dcc_perm(x, y, index=1.0, replicates=100, seed=0, include_observed=True):
 
 
 def skeleton(data, alpha=0.1, m_max=None, num_col=35, threads=8, method="dcc.gamma", sig=1.0,
-             replicates=100, seed=0, index=1.0, include_observed=True):
+             replicates=100, seed=0, index=1.0, include_observed=True, r_stream=False):
+    """pcalg stable skeleton; r_stream: permutation indices from R's global
+    stream after set.seed(seed), tests in pcalg's execution order."""
     d = np.asfortranarray(data, dtype=np.float64)
     n, p = d.shape
     opt = options(method, alpha, p - 2 if m_max is None else m_max, num_col, sig, replicates,
-                  seed, threads, index, include_observed)
+                  seed, threads, index, include_observed, r_stream)
     res = OrcSkeletonResult()
     rc = lib().orc_skeleton(_d(d), n, p, ctypes.byref(opt), ctypes.byref(res))
     if rc != 0:
@@ -352,8 +373,81 @@ def skeleton(data, alpha=0.1, m_max=None, num_col=35, threads=8, method="dcc.gam
             "mean": np.ctypeslib.as_array(res.trace_mean, (L,)).copy(),
             "variance": np.ctypeslib.as_array(res.trace_var, (L,)).copy(),
         }
-        return {"adjacency": adj, "pMax": pmax, "sepsets": sepsets,
-                "n.edgetests": list(res.n_edgetests)[:res.levels], "tasks": tasks,
-                "levels": res.levels}
+        out = {"adjacency": adj, "pMax": pmax, "sepsets": sepsets,
+               "n.edgetests": list(res.n_edgetests)[:res.levels], "tasks": tasks,
+               "levels": res.levels}
+        if r_stream:
+            out["r_state"] = [int(res.rng_out.mti)] + [int(v) for v in res.rng_out.mt]
+        return out
     finally:
         lib().orc_skeleton_result_free(ctypes.byref(res))
+
+
+class RStream:
+    """R's global stream after set.seed(seed) (Mersenne-Twister, Inversion,
+    Rejection), on the oracle's restatement."""
+
+    def __init__(self, seed):
+        self.r = OrcRng()
+        lib().orc_rng_set_seed(ctypes.byref(self.r), int(seed) & 0xFFFFFFFF)
+
+    def word(self):
+        return int(lib().orc_rng_word(ctypes.byref(self.r)))
+
+    def runif(self, k=1):
+        return [lib().orc_unif_rand(ctypes.byref(self.r)) for _ in range(k)]
+
+    def unif_index(self, dn, rounding=False):
+        return lib().orc_unif_index(ctypes.byref(self.r), float(dn), 1 if rounding else 0)
+
+    def sample(self, n, rounding=False):
+        pool = np.zeros(n, dtype=np.int32)
+        out = np.zeros(n, dtype=np.int32)
+        lib().orc_r_sample(ctypes.byref(self.r), n, 1 if rounding else 0,
+                           pool.ctypes.data_as(_I), out.ctypes.data_as(_I))
+        return out
+
+    def energy_permute(self, J):
+        J = np.ascontiguousarray(J, dtype=np.int32)
+        lib().orc_energy_permute(ctypes.byref(self.r), J.ctypes.data_as(_I), len(J))
+        return J
+
+    def perm_table(self, method, n, R):
+        pool = np.zeros(n, dtype=np.int32)
+        out = np.zeros(R * n, dtype=np.int32)
+        lib().orc_r_perm_table(ctypes.byref(self.r), METHODS[method], n, R,
+                               pool.ctypes.data_as(_I), out.ctypes.data_as(_I))
+        return out.reshape(R, n)
+
+    def state(self):
+        """.Random.seed[2:626]: position, then the 624 words (as unsigned)."""
+        return [int(self.r.mti)] + [int(v) for v in self.r.mt]
+
+
+def dcc_perm_table(x, y, table, index=1.0, include_observed=True):
+    """dcc.perm with an explicit R x n index table (y permuted as y[table[r]])."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    tab = np.ascontiguousarray(table, dtype=np.int32)
+    out = np.zeros(5)
+    rc = lib().orc_dcc_perm_table(_d(x), _d(y), len(x), float(index), tab.shape[0],
+                                  1 if include_observed else 0, 0, tab.ctypes.data_as(_I), _d(out))
+    if rc != 0:
+        raise RuntimeError("orc_dcc_perm_table failed %d" % rc)
+    return {"p.value": out[0], "statistic": out[1], "mean": out[2], "variance": out[3],
+            "margin": out[4]}
+
+
+def hsic_perm_table(x, y, table, sig=1.0, num_col=35, include_observed=True):
+    """hsic.perm with an explicit R x n index table (rows of y's factor permuted)."""
+    gx, _, _, _ = hsic_component(x, sig, num_col)
+    gy, _, _, _ = hsic_component(y, sig, num_col)
+    n = len(x)
+    tab = np.ascontiguousarray(table, dtype=np.int32)
+    out = np.zeros(5)
+    fx = np.ascontiguousarray(gx.T).ravel()
+    fy = np.ascontiguousarray(gy.T).ravel()
+    lib().orc_hsic_perm_table(_d(fx), gx.shape[1], _d(fy), gy.shape[1], n, tab.shape[0],
+                              1 if include_observed else 0, 0, tab.ctypes.data_as(_I), _d(out))
+    return {"p.value": out[0], "statistic": out[1], "mean": out[2], "variance": out[3],
+            "margin": out[4]}

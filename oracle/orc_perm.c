@@ -136,19 +136,25 @@ static void perm_finish(double observed, double stat_scale, const double* rep, i
   out[4] = margin;
 }
 
-int orc_dcc_perm(const double* x, const double* y, int n, double index, int R,
-                 int include_observed, unsigned long long seed, double out[5]) {
+/* dcc.perm with replicate r permuting y by table[r] (R x n, 0-based; the
+ * permuted sample is y[table[r][i]]); table NULL: the counter-based indices
+ * of `seed` (orc_permutation). */
+int orc_dcc_perm_table(const double* x, const double* y, int n, double index, int R,
+                       int include_observed, unsigned long long seed, const int* table,
+                       double out[5]) {
   if (n < 2 || R < 0) return -1;
   double* a = (double*)malloc(sizeof(double) * (size_t)n * n);
   double* b = (double*)malloc(sizeof(double) * (size_t)n * n);
   double* rep = (double*)malloc(sizeof(double) * (R > 0 ? R : 1));
-  int* perm = (int*)malloc(sizeof(int) * n);
+  int* own = (int*)malloc(sizeof(int) * n);
   centred_distance(x, n, index, a);
   centred_distance(y, n, index, b);
   double observed = 0.0;
   for (size_t c = 0; c < (size_t)n * n; ++c) observed += a[c] * b[c];
   for (int r = 0; r < R; ++r) {
-    orc_permutation(seed, r, n, perm);
+    const int* perm = own;
+    if (table) perm = table + (size_t)r * n;
+    else orc_permutation(seed, r, n, own);
     double s = 0.0;
     for (int i = 0; i < n; ++i) {
       const double* br = b + (size_t)perm[i] * n;
@@ -157,26 +163,40 @@ int orc_dcc_perm(const double* x, const double* y, int n, double index, int R,
     rep[r] = s;
   }
   perm_finish(observed, 1.0 / (double)n, rep, R, include_observed, 1, out);
-  free(a); free(b); free(rep); free(perm);
+  free(a); free(b); free(rep); free(own);
   return 0;
 }
 
+int orc_dcc_perm(const double* x, const double* y, int n, double index, int R,
+                 int include_observed, unsigned long long seed, double out[5]) {
+  return orc_dcc_perm_table(x, y, n, index, R, include_observed, seed, NULL, out);
+}
+
 /* hsic.perm on precomputed centred factors (kpcalg/R/hsicperm.R:77-88; the
- * rows of the second factor are permuted). */
-void orc_hsic_perm_from_factors(const double* gx, int rx, const double* gy, int ry, int n,
-                                int R, int include_observed, unsigned long long seed,
-                                double out[5]) {
+ * rows of the second factor are permuted by table[r], or by the counter-based
+ * indices of `seed` when table is NULL). */
+void orc_hsic_perm_table(const double* gx, int rx, const double* gy, int ry, int n, int R,
+                         int include_observed, unsigned long long seed, const int* table,
+                         double out[5]) {
   if (n < 1) return;
   double* rep = (double*)malloc(sizeof(double) * (R > 0 ? R : 1));
-  int* perm = (int*)malloc(sizeof(int) * n);
+  int* own = (int*)malloc(sizeof(int) * n);
   int* ident = (int*)malloc(sizeof(int) * n);
   for (int i = 0; i < n; ++i) ident[i] = i;
   const double n2 = (double)n * (double)n;
   double observed = orc_hsic_cross(gx, rx, gy, ry, n, ident) / n2;
   for (int r = 0; r < R; ++r) {
-    orc_permutation(seed, r, n, perm);
+    const int* perm = own;
+    if (table) perm = table + (size_t)r * n;
+    else orc_permutation(seed, r, n, own);
     rep[r] = orc_hsic_cross(gx, rx, gy, ry, n, perm) / n2;
   }
   perm_finish(observed, 1.0, rep, R, include_observed, 0, out);
-  free(rep); free(perm); free(ident);
+  free(rep); free(own); free(ident);
+}
+
+void orc_hsic_perm_from_factors(const double* gx, int rx, const double* gy, int ry, int n,
+                                int R, int include_observed, unsigned long long seed,
+                                double out[5]) {
+  orc_hsic_perm_table(gx, rx, gy, ry, n, R, include_observed, seed, NULL, out);
 }

@@ -155,12 +155,15 @@ static int get_component(comp_cache* c, const double* data, int n, int target, c
 }
 
 /* One CI test on cached components; out = {p, statistic, mean, variance,
- * margin}.  Permutation tests permute the variable with the larger index and
- * draw their indices from the task seed of (x, y, S). */
+ * margin}.  Permutation tests permute one variable: by default the one with
+ * the larger index, with indices from the task seed of (x, y, S); with an R
+ * stream (rng != NULL) the orientation's y, as kpcalg's kernelCItest passes
+ * (x, y) to dcov.test / hsic.perm, with the test's table drawn from the
+ * stream (fastkpc/src/full_cuda_ci_one_call.cpp:464-567). */
 static int ci_pvalue(comp_cache* c, const double* data, int n, int x, int y, const int* S,
-                     int s_len, const orc_skeleton_options* opt, double out[5]) {
+                     int s_len, const orc_skeleton_options* opt, orc_rng* rng, double out[5]) {
   if (opt->ci_method < ORC_DCC_GAMMA || opt->ci_method > ORC_HSIC_PERM) return -20;
-  const int lo = x < y ? x : y, hi = x < y ? y : x;
+  const int lo = rng ? x : (x < y ? x : y), hi = rng ? y : (x < y ? y : x);
   comp_entry *ex, *ey;
   int rc = get_component(c, data, n, lo, S, s_len, opt, &ex);
   if (rc) return rc;
@@ -168,6 +171,12 @@ static int ci_pvalue(comp_cache* c, const double* data, int n, int x, int y, con
   if (rc) return rc;
   const unsigned long long seed =
       orc_task_seed(opt->perm_seed, opt->ci_method, x, y, S, s_len);
+  int* table = NULL;
+  if (rng && (opt->ci_method == ORC_DCC_PERM || opt->ci_method == ORC_HSIC_PERM)) {
+    const int R = opt->replicates > 0 ? opt->replicates : 0;
+    table = (int*)malloc(sizeof(int) * ((size_t)R * n + n));
+    orc_r_perm_table(rng, opt->ci_method, n, R, table + (size_t)R * n, table);
+  }
   double o[5];
   switch (opt->ci_method) {
     case ORC_DCC_GAMMA: {
@@ -181,15 +190,16 @@ static int ci_pvalue(comp_cache* c, const double* data, int n, int x, int y, con
                                   ey->dsum, ey->moment, n, o);
       break;
     case ORC_HSIC_PERM:
-      orc_hsic_perm_from_factors(ex->vecs, ex->rank, ey->vecs, ey->rank, n, opt->replicates,
-                                 opt->include_observed, seed, o);
+      orc_hsic_perm_table(ex->vecs, ex->rank, ey->vecs, ey->rank, n, opt->replicates,
+                          opt->include_observed, seed, table, o);
       break;
     default:
-      rc = orc_dcc_perm(ex->vecs, ey->vecs, n, opt->index, opt->replicates,
-                        opt->include_observed, seed, o);
-      if (rc) return rc;
+      rc = orc_dcc_perm_table(ex->vecs, ey->vecs, n, opt->index, opt->replicates,
+                              opt->include_observed, seed, table, o);
       break;
   }
+  free(table);
+  if (rc) return rc;
   for (int i = 0; i < 4; ++i) out[i] = o[i];
   double mg = o[4];
   if (ex->margin < mg) mg = ex->margin;
@@ -202,7 +212,7 @@ int orc_ci_test(const double* data, int n, int p, int x, int y, const int* S, in
                 const orc_skeleton_options* opt, double out[5]) {
   (void)p;
   comp_cache c = {0};
-  int rc = ci_pvalue(&c, data, n, x, y, S, s_len, opt, out);
+  int rc = ci_pvalue(&c, data, n, x, y, S, s_len, opt, NULL, out);
   cache_free(&c);
   return rc;
 }
@@ -306,6 +316,42 @@ static int prefetch_level(comp_cache* c, const double* data, int n, int p, const
   return rc;
 }
 
+/* One orientation (x, y) of an edge a < b at a level: its neighbours
+ * adj(x) \ {y} in the level's snapshot and its tasks' span in the canonical
+ * plan (edges a < b row-major, the (a, b) orientation's conditioning sets
+ * before the (b, a) orientation's, each in lexicographic order). */
+typedef struct {
+  int a, b, x, y, len, first, count;
+  int nb[64];
+} orient_t;
+
+typedef struct {
+  int task;  /* canonical plan position */
+  int x, y, s_len, del;
+  int S[64];
+  double out[5];
+} done_test;
+
+static long long n_choose(int n, int k) {
+  if (k < 0 || k > n) return 0;
+  long long r = 1;
+  for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+  return r;
+}
+
+static int cmp_pcalg(const void* l, const void* r) {
+  const orient_t* u = *(const orient_t* const*)l;
+  const orient_t* v = *(const orient_t* const*)r;
+  if (u->x != v->x) return u->x < v->x ? -1 : 1;
+  return u->y < v->y ? -1 : (u->y > v->y);
+}
+
+static int cmp_task(const void* l, const void* r) {
+  const done_test* u = (const done_test*)l;
+  const done_test* v = (const done_test*)r;
+  return u->task < v->task ? -1 : (u->task > v->task);
+}
+
 int orc_skeleton(const double* data, int n, int p, const orc_skeleton_options* opt,
                  orc_skeleton_result* r) {
   memset(r, 0, sizeof(*r));
@@ -319,8 +365,16 @@ int orc_skeleton(const double* data, int n, int p, const orc_skeleton_options* o
   for (int i = 0; i < p; ++i) { r->adjacency[i * p + i] = 0; r->pmax[i * p + i] = 1.0; }
   unsigned char* snap = (unsigned char*)malloc((size_t)p * p);
   unsigned char* del = (unsigned char*)malloc((size_t)p * p);
+  orient_t* ors = (orient_t*)malloc(sizeof(orient_t) * (size_t)p * (p - 1));
+  orient_t** order = (orient_t**)malloc(sizeof(orient_t*) * (size_t)p * (p - 1));
+  orc_rng rng;
+  orc_rng* rp = NULL;
+  if (opt->r_stream) {
+    orc_rng_set_seed(&rng, (unsigned int)opt->perm_seed);
+    rp = &rng;
+  }
   comp_cache cache = {0};
-  int nb[64], idx[64], S[64];
+  int idx[64], S[64];
   int done = 0, ord = 0, rc = 0;
   while (!done && ord <= opt->m_max) {
     int any = 0;
@@ -333,43 +387,73 @@ int orc_skeleton(const double* data, int n, int p, const orc_skeleton_options* o
       rc = prefetch_level(&cache, data, n, p, snap, ord, opt);
       if (rc) break;
     }
-    int tests = 0;
-    for (int a = 0; a < p - 1 && !rc; ++a)
-      for (int b = a + 1; b < p && !rc; ++b) {
+    /* the level plan's orientations, canonical order */
+    int no = 0;
+    long long planned = 0;
+    for (int a = 0; a < p - 1; ++a)
+      for (int b = a + 1; b < p; ++b) {
         if (!snap[a * p + b]) continue;
-        for (int side = 0; side < 2 && !rc; ++side) {
-          int x = side ? b : a, y = side ? a : b;
-          if (del[a * p + b]) break;
-          int len = 0;
-          for (int i = 0; i < p; ++i) if (i != y && snap[i * p + x]) nb[len++] = i;
-          if (len < ord) continue;
-          if (len > ord) done = 0;
-          for (int i = 0; i < ord; ++i) idx[i] = i;
-          do {
-            for (int i = 0; i < ord; ++i) S[i] = nb[idx[i]];
-            double out[5];
-            rc = ci_pvalue(&cache, data, n, x, y, S, ord, opt, out);
-            if (rc) break;
-            double pv = out[0];
-            if (isnan(pv)) pv = 1.0;
-            ++tests;
-            if (r->pmax[x * p + y] < pv) r->pmax[x * p + y] = pv;
-            int deleted = pv >= opt->alpha;
-            trace_push(r, ord, x, y, S, ord, pv, deleted, out);
-            if (deleted) {
-              del[a * p + b] = del[b * p + a] = 1;
-              r->sepset_len[a * p + b] = r->sepset_len[b * p + a] = ord;
-              for (int i = 0; i < ord; ++i) {
-                r->sepset[((size_t)a * p + b) * 64 + i] = S[i];
-                r->sepset[((size_t)b * p + a) * 64 + i] = S[i];
-              }
-              break;
-            }
-          } while (ord > 0 && next_comb(idx, ord, len));
+        for (int side = 0; side < 2; ++side) {
+          orient_t* o = &ors[no++];
+          o->a = a; o->b = b;
+          o->x = side ? b : a;
+          o->y = side ? a : b;
+          o->len = 0;
+          for (int i = 0; i < p; ++i) if (i != o->y && snap[i * p + o->x]) o->nb[o->len++] = i;
+          o->first = (int)planned;
+          o->count = (int)n_choose(o->len, ord);
+          planned += o->count;
         }
       }
+    /* execution order: canonical, or pcalg's (x, then y) for the R stream --
+     * the order kpcalg runs the tests in and so draws their permutations */
+    for (int i = 0; i < no; ++i) order[i] = &ors[i];
+    if (rp) qsort(order, no, sizeof(orient_t*), cmp_pcalg);
+    done_test* res = NULL;
+    int nres = 0, cres = 0;
+    for (int oi = 0; oi < no && !rc; ++oi) {
+      const orient_t* o = order[oi];
+      if (del[o->a * p + o->b]) continue;  /* deleted by its other orientation */
+      if (o->len < ord) continue;
+      if (o->len > ord) done = 0;
+      for (int i = 0; i < ord; ++i) idx[i] = i;
+      int k = 0;
+      do {
+        for (int i = 0; i < ord; ++i) S[i] = o->nb[idx[i]];
+        if (nres == cres) {
+          cres = cres ? 2 * cres : 1024;
+          res = (done_test*)realloc(res, sizeof(done_test) * cres);
+        }
+        done_test* d = &res[nres++];
+        rc = ci_pvalue(&cache, data, n, o->x, o->y, S, ord, opt, rp, d->out);
+        if (rc) break;
+        double pv = d->out[0];
+        if (isnan(pv)) pv = 1.0;
+        d->out[0] = pv;
+        d->task = o->first + k++;
+        d->x = o->x; d->y = o->y; d->s_len = ord;
+        memcpy(d->S, S, sizeof(int) * ord);
+        if (r->pmax[o->x * p + o->y] < pv) r->pmax[o->x * p + o->y] = pv;
+        d->del = pv >= opt->alpha;
+        if (d->del) {
+          del[o->a * p + o->b] = del[o->b * p + o->a] = 1;
+          r->sepset_len[o->a * p + o->b] = r->sepset_len[o->b * p + o->a] = ord;
+          for (int i = 0; i < ord; ++i) {
+            r->sepset[((size_t)o->a * p + o->b) * 64 + i] = S[i];
+            r->sepset[((size_t)o->b * p + o->a) * 64 + i] = S[i];
+          }
+          break;
+        }
+      } while (ord > 0 && next_comb(idx, ord, o->len));
+    }
+    /* trace rows in canonical plan order */
+    qsort(res, nres, sizeof(done_test), cmp_task);
+    for (int i = 0; i < nres && !rc; ++i)
+      trace_push(r, ord, res[i].x, res[i].y, res[i].S, res[i].s_len, res[i].out[0], res[i].del,
+                 res[i].out);
+    free(res);
     for (int i = 0; i < p * p; ++i) if (del[i]) r->adjacency[i] = 0;
-    r->n_edgetests[ord] = tests;
+    r->n_edgetests[ord] = nres;
     ++ord;
   }
   r->levels = ord;
@@ -378,9 +462,12 @@ int orc_skeleton(const double* data, int n, int p, const orc_skeleton_options* o
       double m = r->pmax[i * p + j] > r->pmax[j * p + i] ? r->pmax[i * p + j] : r->pmax[j * p + i];
       r->pmax[i * p + j] = r->pmax[j * p + i] = m;
     }
+  if (rp) r->rng_out = rng;
   cache_free(&cache);
   free(snap);
   free(del);
+  free(ors);
+  free(order);
   return rc;
 }
 

@@ -23,7 +23,7 @@ from fastkpc_amd.sem import config_data  # noqa: E402
 from oracle import orc  # noqa: E402
 
 
-def save(name, data, res):
+def save(name, data, res, extra=None):
     p = data.shape[1]
     sep = np.full((p, p, 65), -1, dtype=np.int32)
     for i in range(p):
@@ -38,7 +38,7 @@ def save(name, data, res):
                         n_edgetests=np.array(res["n.edgetests"], dtype=np.int32),
                         level=t["level"], x=t["x"], y=t["y"], S=np.array(t["S_key"]),
                         p=t["p_used"], deleted=t["native_edge_deleted"], margin=t["margin"],
-                        stat=t["statistic"], mean=t["mean"], var=t["variance"])
+                        stat=t["statistic"], mean=t["mean"], var=t["variance"], **(extra or {}))
 
 
 # name -> (data source, ci_method, replicates); permutation workloads use seed 2024
@@ -52,6 +52,12 @@ WORKLOADS = {
     "sem200x8_dcc_perm": ("sem200x8", "dcc.perm", 100),
     "mixed200x14_hsic_perm": ("mixed200x14", "hsic.perm", 100),
     "mixed200x14_dcc_perm": ("mixed200x14", "dcc.perm", 100),
+    # permutation indices from R's global stream after set.seed(PERM_SEED),
+    # tests in pcalg's execution order (the reference's strict route)
+    "sem200x8_dcc_perm_R": ("sem200x8", "dcc.perm", 100, "R"),
+    "sem200x8_hsic_perm_R": ("sem200x8", "hsic.perm", 100, "R"),
+    "mixed200x14_dcc_perm_R": ("mixed200x14", "dcc.perm", 100, "R"),
+    "mixed200x14_hsic_perm_R": ("mixed200x14", "hsic.perm", 100, "R"),
 }
 PERM_SEED = 2024
 
@@ -68,11 +74,13 @@ def workload_data(src):
 def main(names):
     for name in names:
         t0 = time.time()
-        src, method, reps = WORKLOADS[name]
+        src, method, reps = WORKLOADS[name][:3]
+        r_stream = len(WORKLOADS[name]) > 3
         data = workload_data(src)
         res = orc.skeleton(data, alpha=0.1, m_max=data.shape[1] - 2, num_col=35, threads=8,
-                           method=method, replicates=reps, seed=PERM_SEED)
-        save(name, data, res)
+                           method=method, replicates=reps, seed=PERM_SEED, r_stream=r_stream)
+        save(name, data, res,
+             {"r_state": np.array(res["r_state"], dtype=np.uint32)} if r_stream else None)
         print(name, "tests", sum(res["n.edgetests"]), res["n.edgetests"], "%.1fs" % (time.time() - t0))
 
 

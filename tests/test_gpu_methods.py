@@ -155,7 +155,7 @@ def test_skeleton_methods_vs_oracle(name):
     import sys
     sys.path.insert(0, GOLDEN)
     from make_oracle_goldens import PERM_SEED, WORKLOADS, workload_data
-    src, method, reps = WORKLOADS[name]
+    src, method, reps = WORKLOADS[name][:3]
     data = workload_data(src)
     gold = np.load(os.path.join(GOLDEN, "oracle_%s.npz" % name))
     opts = {"numCol": 35, "ci_method": method}
