@@ -30,6 +30,7 @@ class KpcOptions(ctypes.Structure):
         ("plain_test_limit", ctypes.c_int),
         ("perm_stream", ctypes.c_int),
         ("r_state", ctypes.POINTER(ctypes.c_int)),
+        ("use_cache", ctypes.c_int),
     ]
 
 
@@ -80,7 +81,17 @@ class KpcResult(ctypes.Structure):
         ("level_rounds", ctypes.c_longlong * 64),
         ("task_perm_seed", ctypes.POINTER(ctypes.c_ulonglong)),
         ("r_state_out", ctypes.c_int * 625),
+        ("result_cache_hits", ctypes.c_longlong),
+        ("target_cache_hits", ctypes.c_longlong),
     ]
+
+
+class KpcCacheInfo(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_longlong) for name in (
+        "capacity", "entries", "dataset_entries", "requests", "hits", "misses", "inserts",
+        "evictions", "generation", "target_capacity", "target_entries", "target_dataset_entries",
+        "target_requests", "target_hits", "target_misses", "target_inserts", "target_evictions",
+        "target_generation", "epoch")] + [("dataset_key", ctypes.c_char * 33)]
 
 
 EXPORTS = (
@@ -88,6 +99,7 @@ EXPORTS = (
     "kpc_regrxons_batch", "kpc_regrxons_mixed", "kpc_gcv_grad_hess", "kpc_dcov_components",
     "kpc_last_fit_timing", "kpc_tuning_report", "kpc_task_seed", "kpc_permutation_table", "kpc_last_error",
     "kpc_device_count", "kpc_empty_cache", "kpc_dcov_components_noalloc", "kpc_ci_batch_rs",
+    "kpc_cache_control", "kpc_cache_state",
 )
 
 _lib = None
@@ -128,6 +140,8 @@ def lib():
                                 ctypes.c_int, ctypes.POINTER(ctypes.c_ulonglong)]
     L.kpc_permutation_table.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_ulonglong), _I]
+    L.kpc_cache_control.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.POINTER(KpcCacheInfo)]
+    L.kpc_cache_state.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.POINTER(KpcCacheInfo)]
     L.kpc_last_error.restype = ctypes.c_char_p
     L.kpc_device_count.restype = ctypes.c_int
     L.kpc_empty_cache.restype = ctypes.c_int

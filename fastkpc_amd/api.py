@@ -18,7 +18,7 @@ ALLOWED_OPTIONS = ("max_conditioning_size", "index", "numCol", "trace_level", "d
                    "mgcv_residual_backend", "mgcv_residual_backend_native_s_size_limit",
                    "mgcv_residual_backend_condition_threshold", "route",
                    "compatible_cuda_strict", "ci_method", "hsic_params", "permutation_params",
-                   "mode", "plain_test_limit")
+                   "mode", "plain_test_limit", "cache")
 
 
 def resolve_max_conditioning_size(max_conditioning_size, p):
@@ -68,6 +68,9 @@ def _options(n, p, alpha, options):
     if rng not in ("philox", "R"):
         raise ValueError("permutation_params$rng must be 'philox' or 'R'")
     o.perm_stream = 1 if rng == "R" else 0
+    # the process-wide result and target-state caches (the reference's
+    # one-call route always uses them); off unless asked for
+    o.use_cache = 1 if options.get("cache", False) else 0
     if rng == "R" and perm.get("r_state") is not None:
         st = np.ascontiguousarray(np.asarray(perm["r_state"], dtype=np.int64).astype(np.uint32)
                                   .view(np.int32))
@@ -201,6 +204,8 @@ def fastkpc_compatible_cuda_skeleton(data, alpha, labels=None, options=None):
             "tasks_ignored_after_delete": int(sum(level_rows["tasks_ignored_after_delete"])),
             "deletions": int(sum(level_rows["deletions"])),
             "r_callback_count": 0,
+            "result_cache_hit_count": int(r.result_cache_hits),
+            "target_cache_hit_count": int(r.target_cache_hits),
             "mode": options.get("mode", "batched"),
             "max_conditioning_size_resolved": o.max_conditioning_size,
             "wall_seconds": r.wall_seconds,
@@ -411,3 +416,44 @@ def dcov_components(x, k):
     check(_lib.lib().kpc_dcov_components(dptr(xx), n, c, k, dptr(vals), dptr(vecs), dptr(dsum),
                                          dptr(mom)))
     return vals, vecs.transpose(0, 2, 1), dsum, mom
+
+
+CACHE_ACTIONS = {"info": 0, "reset": 1, "configure": 2, "configure_target": 3}
+
+
+def _cache_dict(ci, action=None):
+    out = {"schema_version": "kpcmi-cache-control-v1"}
+    if action is not None:
+        out["action"] = action
+    for name, _ in _lib.KpcCacheInfo._fields_:
+        if name == "dataset_key":
+            continue
+        key = "cache_epoch" if name == "epoch" else name
+        key = key if not key.startswith("target_") else key
+        out[key] = int(getattr(ci, name))
+    return out
+
+
+def cache_control(action="info", capacity=None):
+    """The result / target-state caches' statistics, reset or capacity
+    (fastkpc/src/full_cuda_ci_one_call.cpp:6812-6866: actions info, reset,
+    configure, configure_target)."""
+    if action not in CACHE_ACTIONS:
+        raise ValueError("action must be one of " + ", ".join(CACHE_ACTIONS))
+    ci = _lib.KpcCacheInfo()
+    check(_lib.lib().kpc_cache_control(CACHE_ACTIONS[action],
+                                       -1 if capacity is None else int(capacity),
+                                       _lib.ctypes.byref(ci)))
+    return _cache_dict(ci, action)
+
+
+def cache_state(data):
+    """The caches' statistics with the entries that belong to `data`
+    (fastkpc/src/full_cuda_ci_one_call.cpp:6869-6900)."""
+    d = _as_data(data)
+    n, p = d.shape
+    ci = _lib.KpcCacheInfo()
+    check(_lib.lib().kpc_cache_state(dptr(d), n, p, _lib.ctypes.byref(ci)))
+    out = _cache_dict(ci)
+    out["dataset_key"] = ci.dataset_key.decode()
+    return out

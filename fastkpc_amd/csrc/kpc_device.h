@@ -65,9 +65,13 @@ hipError_t launch_geometry(const GeometryItem* items, int count, int n, int q, i
 long long magic_scratch_doubles(int q, int m, int rk, int n);
 size_t magic_lds_bytes(int n, int q, int m, bool aug_in_lds);
 bool magic_aug_fits_lds(int n, int q, int m);
+// fixed_info: fit records (kMagicInfo doubles each) of cached fits; an item
+// with pad > 0 replays the final fit at the log(sp) of record pad - 1 and
+// writes that record as its info (pad == 0: the full optimiser)
 hipError_t launch_magic(const MagicItem* items, int count, int n, int q, int m, int rk,
                         const double* data, const double* design, double* resid, double* info,
-                        double* scratch, int grid, const MagicControl& ctl, hipStream_t stream);
+                        double* scratch, int grid, const MagicControl& ctl, hipStream_t stream,
+                        const double* fixed_info = nullptr);
 
 }  // namespace kpc
 

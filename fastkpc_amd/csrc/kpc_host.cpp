@@ -25,6 +25,7 @@
 #include "kpc_common.h"
 #include "kpc_device.h"
 #include "kpc_rstream.h"
+#include "kpc_cache.h"
 #include "kpc_tune.h"
 
 #include <hip/hip_runtime.h>
@@ -374,6 +375,29 @@ class Engine {
     comp_k_ = method_ == KPC_DCC_PERM ? 2 : opt.num_col;
     for (long long i = 0; i < (long long)n * p; ++i)
       if (!std::isfinite(h_data[i])) throw KpcError(-1, "data must be finite");
+    if (opt.use_cache) {
+      // a test's value depends on the data, the method and that method's
+      // own parameters; a fit's only on the data, S and the target
+      dataset_ = kpc::dataset_digest(h_data, n, p);
+      use_targets_ = true;
+      use_results_ = !rmode_;
+      uint64_t h = kpc::mix64(0x726573756c74ULL ^ (uint64_t)method_);
+      auto fold = [&h](uint64_t v) { h = kpc::mix64(h ^ v); };
+      auto bits = [](double v) {
+        uint64_t b;
+        std::memcpy(&b, &v, sizeof(b));
+        return b;
+      };
+      if (method_ != KPC_DCC_PERM) fold((uint64_t)opt.num_col);
+      if (method_ == KPC_HSIC_GAMMA || method_ == KPC_HSIC_PERM) fold(bits(opt.hsic_sig));
+      if (method_ == KPC_DCC_PERM) fold(bits(index_));
+      if (method_ == KPC_DCC_PERM || method_ == KPC_HSIC_PERM) {
+        fold((uint64_t)opt.replicates);
+        fold((uint64_t)opt.include_observed);
+        fold(seed_);
+      }
+      opt_hash_ = h;
+    }
     data_bytes_ = sizeof(double) * (size_t)n * p;
     d_data_ = (double*)pool_acquire(data_bytes_);
     hip_check(hipMemcpy(d_data_, h_data, sizeof(double) * (size_t)n * p, hipMemcpyHostToDevice),
@@ -510,6 +534,14 @@ class Engine {
     in_batch_.clear();
     pair_rtab_.clear();
     for (Task* t : all) {
+      if (use_results_ && with_pairs) {
+        kpc::CachedResult cr;
+        if (kpc::result_cache().lookup(result_key(*t), &cr)) {
+          apply_result(*t, TestResult{cr.p, cr.stat, cr.mean, cr.var, cr.margin});
+          ++result_hits_;
+          continue;
+        }
+      }
       if (rmode_ || !with_pairs) {
         if (with_pairs) pair_rtab_.push_back(t->rtab);
         tasks.push_back(t);
@@ -555,6 +587,8 @@ class Engine {
       std::vector<int> smooth_items;  // d, c0, c1, slot
       std::vector<kpc::GeometryItem> geo_items;
       std::vector<kpc::MagicItem> fit_items;
+      std::vector<uint64_t> fit_mask;  // conditioning set of each fit (target-state cache)
+      std::vector<double> fixed;       // cached fit records the fits with pad > 0 replay
       std::vector<long long> comp_src;
       std::vector<int> comp_dst;
       std::vector<int> pair_x, pair_y;
@@ -604,7 +638,16 @@ class Engine {
         mi.target = tgt;
         mi.resid_slot = rs;
         mi.pad = 0;
+        if (use_targets_) {
+          kpc::CachedFit cf;
+          if (kpc::target_cache().lookup(target_key(t->mask, tgt), &cf)) {
+            mi.pad = 1 + (int)(w.fixed.size() / kpc::kMagicInfo);
+            w.fixed.insert(w.fixed.end(), cf.record.begin(), cf.record.end());
+            ++target_hits_;
+          }
+        }
         w.fit_items.push_back(mi);
+        w.fit_mask.push_back(t->mask);
         w.comp_src.push_back((long long)rs * n_);
       }
       w.pair_x.push_back(slots[0]);
@@ -630,6 +673,7 @@ class Engine {
       L.smooth_items->upload(w.smooth_items, L.stream);
       L.geo_items->upload(w.geo_items, L.stream);
       L.fit_items->upload(w.fit_items, L.stream);
+      L.fixed->upload(w.fixed, L.stream);
       L.comp_src->upload(w.comp_src, L.stream);
       L.comp_dst->upload(w.comp_dst, L.stream);
       L.pair_x->upload(w.pair_x, L.stream);
@@ -679,7 +723,8 @@ class Engine {
         L.scratch3->reserve((size_t)grid * stride);
         hip_check(kpc::launch_magic(L.fit_items->ptr, (int)w.fit_items.size(), n_, q_, m_, rk_,
                                     d_data_, design_.ptr, resid_.ptr, rinfo_.ptr,
-                                    L.scratch3->ptr, grid, ctl_, L.stream),
+                                    L.scratch3->ptr, grid, ctl_, L.stream,
+                                    w.fixed.empty() ? nullptr : L.fixed->ptr),
                   "magic launch");
       }
       if (!graph) hip_check(hipEventRecord(L.ev[3], L.stream), "event");
@@ -746,6 +791,17 @@ class Engine {
       check_device_status(fits, geos, W[0].nsm + W[1].nsm, &rinfo_h);
       check_component_status(dst);
     }
+    if (use_targets_ && !rinfo_h.empty())
+      for (int li = 0; li < nlanes; ++li)
+        for (size_t i = 0; i < W[li].fit_items.size(); ++i) {
+          const kpc::MagicItem& f = W[li].fit_items[i];
+          if (f.pad > 0) continue;
+          kpc::CachedFit cf;
+          std::memcpy(cf.record.data(), rinfo_h.data() + (size_t)f.resid_slot * kpc::kMagicInfo,
+                      sizeof(double) * kpc::kMagicInfo);
+          if (m_ <= kpc::kMagicInfo - 10)  // the record holds every log sp
+            kpc::target_cache().insert(target_key(W[li].fit_mask[i], f.target), cf);
+        }
     if (!with_pairs) return;
     std::vector<double> cmargin_h;
     if (method_ == KPC_HSIC_GAMMA || method_ == KPC_HSIC_PERM) {
@@ -769,10 +825,38 @@ class Engine {
           res.margin = std::min(res.margin, std::min(cmargin_h[w.pair_x[j]], cmargin_h[w.pair_y[j]]));
         apply_result(*t, res);
         if (!rmode_) twin_map_.emplace(twin_key(*t), res);
+        if (use_results_) {
+          // both orientations: outside the R stream a test's value does not
+          // depend on which variable conditions first
+          const kpc::CachedResult cr{res.p, res.stat, res.mean, res.var, res.margin};
+          Task twin = *t;
+          std::swap(twin.x, twin.y);
+          kpc::result_cache().insert(result_key(*t), cr);
+          kpc::result_cache().insert(result_key(twin), cr);
+        }
       }
     }
     for (auto& d : dups) apply_result(*d.first, *twin_map_.find(twin_key(*tasks[d.second])));
   }
+
+  kpc::CacheKey result_key(const Task& t) const {
+    kpc::CacheKey k;
+    k.dataset = dataset_;
+    k.a = opt_hash_;
+    k.b = ((uint64_t)(uint32_t)t.x << 32) | (uint32_t)t.y;
+    k.c = t.mask;
+    return k;
+  }
+  kpc::CacheKey target_key(uint64_t mask, int target) const {
+    kpc::CacheKey k;
+    k.dataset = dataset_;
+    k.a = 0x746172676574ULL;  // a fit, not a test
+    k.b = mask;
+    k.c = (uint64_t)target;
+    return k;
+  }
+  long long result_hits() const { return result_hits_; }
+  long long target_hits() const { return target_hits_; }
 
   static std::pair<uint64_t, uint64_t> twin_key(const Task& t) {
     const uint64_t a = (uint64_t)std::min(t.x, t.y), b = (uint64_t)std::max(t.x, t.y);
@@ -808,15 +892,18 @@ class Engine {
     DevArray<double>* scratch2;  // geometry
     DevArray<double>* scratch3;  // GCV fits
     DevArray<double>* scratch4;  // components
+    DevArray<double>* fixed;     // cached fit records
   };
   Lane lane(int i) {
     if (i == 0)
       return Lane{stream_,        ev_,          &d_smooth_items_, &d_geo_items_, &d_fit_items_,
                   &d_comp_src_,   &d_comp_dst_, &d_pair_x_,       &d_pair_y_,    &d_pair_out_,
-                  &scratch_,      &scratch2_,   &scratch3_,       &scratch4_};
+                  &scratch_,      &scratch2_,   &scratch3_,       &scratch4_,
+                  &d_fixed_};
     return Lane{stream2_,        ev2_,          &l1_smooth_items_, &l1_geo_items_, &l1_fit_items_,
                 &l1_comp_src_,   &l1_comp_dst_, &l1_pair_x_,       &l1_pair_y_,    &l1_pair_out_,
-                &l1_scratch_,    &l1_scratch2_, &l1_scratch3_,     &l1_scratch4_};
+                &l1_scratch_,    &l1_scratch2_, &l1_scratch3_,     &l1_scratch4_,
+                &l1_fixed_};
   }
 
   void launch_pairs(int count) { launch_pairs(lane(0), count); }
@@ -1197,7 +1284,7 @@ class Engine {
       for (const auto& f : fits) {
         const double* o = h.data() + (size_t)f.resid_slot * kpc::kMagicInfo;
         if (o[0] != 0.0) throw KpcError(-6, "GCV optimiser did not converge");
-        stats_.evals += (long long)o[5];
+        if (f.pad == 0) stats_.evals += (long long)o[5];
       }
       if (want_info) rinfo_out->swap(h);
     }
@@ -1281,6 +1368,12 @@ class Engine {
   std::vector<unsigned short> rtab_host_;
   std::vector<long long> pair_rtab_;
   std::vector<int> pool_;
+  // process-wide result / target-state caches (use_cache)
+  bool use_results_ = false, use_targets_ = false;
+  kpc::Digest dataset_;
+  uint64_t opt_hash_ = 0;
+  long long result_hits_ = 0, target_hits_ = 0;
+  DevArray<double> d_fixed_, l1_fixed_;
   double stage_[5] = {0, 0, 0, 0, 0};
   long long launches_[5] = {0, 0, 0, 0, 0};
   double stage_host_ = 0.0;
@@ -1734,6 +1827,8 @@ int kpc_skeleton_run(const double* data, int n, int p, const kpc_options* opt, k
     r->stage_seconds[7] = eng.graph_seconds();
     r->stage_launches[7] = eng.graph_launches();
     if (eng.r_stream()) eng.save_rstate(r->r_state_out);
+    r->result_cache_hits = eng.result_hits();
+    r->target_cache_hits = eng.target_hits();
     r->physical_tests = eng.stats_.physical_tests;
     r->residual_fits = eng.stats_.fits;
     r->components = eng.stats_.components;
@@ -2072,6 +2167,80 @@ int kpc_dcov_components(const double* x, int n, int count, int k, double* values
     if (vectors) hip_check(hipMemcpy(vectors, d_vecs.ptr, sizeof(double) * (size_t)count * n * k, hipMemcpyDeviceToHost), "D2H");
     if (dsum) hip_check(hipMemcpy(dsum, d_dsum.ptr, sizeof(double) * count, hipMemcpyDeviceToHost), "D2H");
     if (moment) hip_check(hipMemcpy(moment, d_mom.ptr, sizeof(double) * count, hipMemcpyDeviceToHost), "D2H");
+    return 0;
+  } catch (const std::exception& e) {
+    return fill_error(e);
+  }
+}
+
+namespace {
+void fill_cache_info(kpc_cache_info* out, const kpc::Digest* dataset) {
+  const kpc::CacheStats a = kpc::result_cache().stats(dataset);
+  const kpc::CacheStats b = kpc::target_cache().stats(dataset);
+  std::memset(out, 0, sizeof(*out));
+  out->capacity = a.capacity;
+  out->entries = a.entries;
+  out->dataset_entries = a.dataset_entries;
+  out->requests = a.requests;
+  out->hits = a.hits;
+  out->misses = a.misses;
+  out->inserts = a.inserts;
+  out->evictions = a.evictions;
+  out->generation = a.generation;
+  out->target_capacity = b.capacity;
+  out->target_entries = b.entries;
+  out->target_dataset_entries = b.dataset_entries;
+  out->target_requests = b.requests;
+  out->target_hits = b.hits;
+  out->target_misses = b.misses;
+  out->target_inserts = b.inserts;
+  out->target_evictions = b.evictions;
+  out->target_generation = b.generation;
+  out->epoch = kpc::cache_epoch().load();
+  if (dataset)
+    std::snprintf(out->dataset_key, sizeof(out->dataset_key), "%016llx%016llx",
+                  (unsigned long long)dataset->hi, (unsigned long long)dataset->lo);
+}
+}  // namespace
+
+int kpc_cache_control(int action, long long capacity, kpc_cache_info* out) {
+  try {
+    if (!out) throw KpcError(-1, "null argument");
+    switch (action) {
+      case KPC_CACHE_INFO:
+        if (capacity != -1) throw KpcError(-1, "cache info does not accept a capacity");
+        break;
+      case KPC_CACHE_RESET:
+        if (capacity != -1) throw KpcError(-1, "cache reset does not accept a capacity");
+        kpc::result_cache().reset();
+        kpc::target_cache().reset();
+        kpc::cache_epoch().fetch_add(1);
+        break;
+      case KPC_CACHE_CONFIGURE:
+        if (!kpc::result_cache().configure(capacity))
+          throw KpcError(-1, "result cache capacity is outside [1, 1048576]");
+        break;
+      case KPC_CACHE_CONFIGURE_TARGET:
+        if (!kpc::target_cache().configure(capacity))
+          throw KpcError(-1, "target-state cache capacity is outside [1, 524288]");
+        break;
+      default:
+        throw KpcError(-1, "unknown cache control action");
+    }
+    fill_cache_info(out, nullptr);
+    return 0;
+  } catch (const std::exception& e) {
+    return fill_error(e);
+  }
+}
+
+int kpc_cache_state(const double* data, int n, int p, kpc_cache_info* out) {
+  try {
+    if (!data || !out || n <= 0 || p <= 0) throw KpcError(-1, "invalid arguments");
+    for (long long i = 0; i < (long long)n * p; ++i)
+      if (!std::isfinite(data[i])) throw KpcError(-1, "data must be finite");
+    const kpc::Digest d = kpc::dataset_digest(data, n, p);
+    fill_cache_info(out, &d);
     return 0;
   } catch (const std::exception& e) {
     return fill_error(e);

@@ -1747,7 +1747,8 @@ __global__ __launch_bounds__(kMagicThreads, 2) void magic_kernel(
     const MagicItem* __restrict__ items, int count, int n, int q, int m, int rk,
     const double* __restrict__ data, const double* __restrict__ design,
     double* __restrict__ resid, double* __restrict__ info, double* __restrict__ scratch,
-    long long scratch_stride, int aug_in_lds, MagicControl ctl) {
+    long long scratch_stride, int aug_in_lds, MagicControl ctl,
+    const double* __restrict__ fixed_info) {
   extern __shared__ double lds_dyn[];
   const int tid = threadIdx.x;
   double* gs = scratch + (size_t)blockIdx.x * scratch_stride;
@@ -1760,6 +1761,10 @@ __global__ __launch_bounds__(kMagicThreads, 2) void magic_kernel(
     const double* D = design + (size_t)it.design_slot * DL.stride;
     const double* y = data + (size_t)it.target * n;
     double* out = info + (size_t)it.resid_slot * kMagicInfo;
+    // it.pad > 0: a cached fit -- the final fit at the log(sp) of the fit
+    // record fixed_info[it.pad - 1] (no optimiser), then that record restored
+    const double* finfo =
+        it.pad > 0 && fixed_info ? fixed_info + (size_t)(it.pad - 1) * kMagicInfo : nullptr;
     const long long t_start = wall_clock64();
     for (int r = tid; r < n; r += blockDim.x) { L.y[r] = y[r]; L.yorig[r] = y[r]; }
     if (tid == 0) {
@@ -1783,11 +1788,16 @@ __global__ __launch_bounds__(kMagicThreads, 2) void magic_kernel(
     double part = 0.0;
     for (int r = tid; r < n; r += blockDim.x) part += L.yorig[r] * L.yorig[r];
     const double ynorm2 = block_sum(part, L.red);
-    if (tid < m) L.lsp[tid] = D[DL.lsp0 + tid];
+    if (tid < m) L.lsp[tid] = finfo ? finfo[8 + tid] : D[DL.lsp0 + tid];
     for (int i = tid; i < 2 + 3 * m; i += blockDim.x) L.nrm[i] = D[DL.norms + i];
     if (tid == 0) L.dctl[22] = (double)(wall_clock64() - t_start);  // Q'y prologue
     __syncthreads();
     const int max_half = m == 1 ? ctl.max_half_single : ctl.max_half_multi;
+    int iter = 0;
+    if (finfo) {
+      if (tid == 0) L.ictl[4] = 1;
+      __syncthreads();
+    } else {
     magic_eval(L, S, D, DL, n, q, m, rk, L.lsp, ynorm2, ctl.rank_tol);
     magic_derivs(L, S, D, DL, n, q, m, rk, L.lsp);
     // optimizer state (lane 0 decides, LDS broadcasts)
@@ -1804,7 +1814,6 @@ __global__ __launch_bounds__(kMagicThreads, 2) void magic_kernel(
       for (int i = 0; i < m; ++i) { L.grad[i] = 0.0; L.nstep[i] = 0.0; L.sstep[i] = 0.0; }
     }
     __syncthreads();
-    int iter;
     for (iter = 1; iter <= ctl.max_iterations; ++iter) {
       if (tid == 0) ic[6] = 0;
       __syncthreads();
@@ -1898,9 +1907,8 @@ __global__ __launch_bounds__(kMagicThreads, 2) void magic_kernel(
       __syncthreads();
       if (ic[4]) break;
     }
-    const int converged = ic[4];
     // infinity probes per coordinate
-    if (converged) {
+    if (ic[4]) {
       for (int c = 0; c < m; ++c) {
         const double dir = L.grad[c] < 0.0 ? 1.0 : -1.0;
         for (int pr = 1; pr <= ctl.max_probes; ++pr) {
@@ -1924,6 +1932,10 @@ __global__ __launch_bounds__(kMagicThreads, 2) void magic_kernel(
         }
       }
     }
+    }  // optimiser (skipped for a cached fit)
+    const int* ic = L.ictl;
+    const double* st = L.dctl;
+    const int converged = ic[4];
     // final fit at the selected log(sp)
     const long long t_final = wall_clock64();
     magic_eval(L, S, D, DL, n, q, m, rk, L.lsp, ynorm2, ctl.rank_tol);
@@ -1953,7 +1965,9 @@ __global__ __launch_bounds__(kMagicThreads, 2) void magic_kernel(
         for (int c = 0; c < q; ++c) f += X[(size_t)c * n + r] * beta[c];
         ro[r] = L.yorig[r] - f;
       }
-      if (tid == 0) {
+      if (finfo) {
+        for (int i = tid; i < kMagicInfo; i += blockDim.x) out[i] = finfo[i];
+      } else if (tid == 0) {
         out[0] = converged ? 0.0 : -7.0;
         out[1] = L.dctl[2];
         out[2] = L.dctl[0];
@@ -2015,7 +2029,8 @@ hipError_t launch_geometry(const GeometryItem* items, int count, int n, int q, i
 
 hipError_t launch_magic(const MagicItem* items, int count, int n, int q, int m, int rk,
                         const double* data, const double* design, double* resid, double* info,
-                        double* scratch, int grid, const MagicControl& ctl, hipStream_t stream) {
+                        double* scratch, int grid, const MagicControl& ctl, hipStream_t stream,
+                        const double* fixed_info) {
   if (count <= 0) return hipSuccess;
   const bool in_lds = magic_aug_fits_lds(n, q, m);
   const size_t lds = magic_lds_bytes(n, q, m, in_lds);
@@ -2026,11 +2041,11 @@ hipError_t launch_magic(const MagicItem* items, int count, int n, int q, int m, 
   if (all_lds)
     hipLaunchKernelGGL(magic_kernel<true>, dim3(grid), dim3(magic_threads(q)), lds, stream, items,
                        count, n, q, m, rk, data, design, resid, info, scratch,
-                       magic_scratch_doubles(q, m, rk, n), 1, ctl);
+                       magic_scratch_doubles(q, m, rk, n), 1, ctl, fixed_info);
   else
     hipLaunchKernelGGL(magic_kernel<false>, dim3(grid), dim3(magic_threads(q)), lds, stream, items,
                        count, n, q, m, rk, data, design, resid, info, scratch,
-                       magic_scratch_doubles(q, m, rk, n), in_lds ? 1 : 0, ctl);
+                       magic_scratch_doubles(q, m, rk, n), in_lds ? 1 : 0, ctl, fixed_info);
   return hipGetLastError();
 }
 

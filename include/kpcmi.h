@@ -68,6 +68,13 @@ typedef struct kpc_options {
   /* KPC_PERM_R_STREAM: the stream to start from as .Random.seed[2:626]
    * (position, then the 624 state words); NULL = set.seed(seed) */
   const int* r_state;
+  /* 1: consult and fill the process-wide result and target-state caches
+   * (the reference's one-call route always does: CompactResultCache and
+   * TargetStateCache, fastkpc/src/full_cuda_ci_one_call.cpp:1443-1755); a
+   * replayed skeleton then runs no test.  0 (default): neither.  R-stream
+   * permutation results are never cached (their indices depend on the
+   * stream), their fits are. */
+  int use_cache;
 } kpc_options;
 
 typedef struct kpc_result {
@@ -152,6 +159,10 @@ typedef struct kpc_result {
   /* KPC_PERM_R_STREAM: the stream after the last test (.Random.seed[2:626]
    * layout), for the caller to store back as R's RNGScope does */
   int r_state_out[625];
+  /* use_cache: tests answered by the result cache, fits replayed from the
+   * target-state cache */
+  long long result_cache_hits;
+  long long target_cache_hits;
 } kpc_result;
 
 void kpc_default_options(kpc_options* opt);
@@ -239,6 +250,26 @@ int kpc_task_seed(unsigned long long user_seed, int ci_method, int x, int y, con
  * counter {i >> 1, r, 0x4b504331, 0}. */
 int kpc_permutation_table(int n, int replicates, int count, const unsigned long long* seeds,
                           int* out);
+
+/* Cache control, the counterpart of C_full_cuda_ci_one_call_cache_control
+ * and _cache_state (fastkpc/src/full_cuda_ci_one_call.cpp:6812-6900). */
+enum kpc_cache_action {
+  KPC_CACHE_INFO = 0,             /* statistics only (capacity must be -1) */
+  KPC_CACHE_RESET = 1,            /* empty both caches, zero their counters, new epoch */
+  KPC_CACHE_CONFIGURE = 2,        /* result-cache capacity, 1 .. 1048576 */
+  KPC_CACHE_CONFIGURE_TARGET = 3  /* target-state capacity, 1 .. 524288 */
+};
+typedef struct kpc_cache_info {
+  long long capacity, entries, dataset_entries, requests, hits, misses, inserts, evictions,
+      generation;
+  long long target_capacity, target_entries, target_dataset_entries, target_requests,
+      target_hits, target_misses, target_inserts, target_evictions, target_generation;
+  long long epoch;
+  char dataset_key[33]; /* kpc_cache_state: the data's 128-bit digest, hex */
+} kpc_cache_info;
+int kpc_cache_control(int action, long long capacity, kpc_cache_info* out);
+/* Statistics plus the entries that belong to `data` (n x p, column-major). */
+int kpc_cache_state(const double* data, int n, int p, kpc_cache_info* out);
 
 /* Frees the device blocks the library keeps cached between calls (its arenas
  * are reused by later calls of the same thread instead of being freed).
