@@ -86,6 +86,45 @@ class KpcResult(ctypes.Structure):
     ]
 
 
+class KpcOrientOptions(ctypes.Structure):
+    _fields_ = [
+        ("ci_method", ctypes.c_int),
+        ("alpha", ctypes.c_double),
+        ("num_col", ctypes.c_int),
+        ("hsic_sig", ctypes.c_double),
+        ("index", ctypes.c_double),
+        ("replicates", ctypes.c_int),
+        ("include_observed", ctypes.c_int),
+        ("has_seed", ctypes.c_int),
+        ("seed", ctypes.c_ulonglong),
+        ("perm_stream", ctypes.c_int),
+        ("r_state", ctypes.POINTER(ctypes.c_int)),
+        ("orient_collider", ctypes.c_int),
+        ("solve_confl", ctypes.c_int),
+        ("rules", ctypes.c_int * 3),
+    ]
+
+
+class KpcOrientResult(ctypes.Structure):
+    _fields_ = [
+        ("p", ctypes.c_int),
+        ("pdag", _I),
+        ("n_calls", ctypes.c_int),
+        ("call_v", _I),
+        ("call_slen", _I),
+        ("call_S", _I),
+        ("call_plen", _I),
+        ("call_parents", _I),
+        ("call_reject", _I),
+        ("n_tests", ctypes.c_int),
+        ("test_call", _I),
+        ("test_other", _I),
+        ("test_out", _D),
+        ("r_state_out", ctypes.c_int * 625),
+        ("wall_seconds", ctypes.c_double),
+    ]
+
+
 class KpcCacheInfo(ctypes.Structure):
     _fields_ = [(name, ctypes.c_longlong) for name in (
         "capacity", "entries", "dataset_entries", "requests", "hits", "misses", "inserts",
@@ -99,7 +138,8 @@ EXPORTS = (
     "kpc_regrxons_batch", "kpc_regrxons_mixed", "kpc_gcv_grad_hess", "kpc_dcov_components",
     "kpc_last_fit_timing", "kpc_tuning_report", "kpc_task_seed", "kpc_permutation_table", "kpc_last_error",
     "kpc_device_count", "kpc_empty_cache", "kpc_dcov_components_noalloc", "kpc_ci_batch_rs",
-    "kpc_cache_control", "kpc_cache_state",
+    "kpc_cache_control", "kpc_cache_state", "kpc_default_orient_options", "kpc_wanpdag_run",
+    "kpc_orient_result_free",
 )
 
 _lib = None
@@ -142,6 +182,13 @@ def lib():
                                         ctypes.POINTER(ctypes.c_ulonglong), _I]
     L.kpc_cache_control.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.POINTER(KpcCacheInfo)]
     L.kpc_cache_state.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.POINTER(KpcCacheInfo)]
+    L.kpc_default_orient_options.argtypes = [ctypes.POINTER(KpcOrientOptions)]
+    L.kpc_default_orient_options.restype = None
+    L.kpc_wanpdag_run.argtypes = [_D, ctypes.c_int, ctypes.c_int, _U8, _I, _I,
+                                  ctypes.POINTER(KpcOrientOptions),
+                                  ctypes.POINTER(ctypes.POINTER(KpcOrientResult))]
+    L.kpc_orient_result_free.argtypes = [ctypes.POINTER(KpcOrientResult)]
+    L.kpc_orient_result_free.restype = None
     L.kpc_last_error.restype = ctypes.c_char_p
     L.kpc_device_count.restype = ctypes.c_int
     L.kpc_empty_cache.restype = ctypes.c_int

@@ -457,3 +457,117 @@ def cache_state(data):
     out = _cache_dict(ci)
     out["dataset_key"] = ci.dataset_key.decode()
     return out
+
+
+def _sepset_arrays(sepsets, p):
+    slen = np.full(p * p, -1, dtype=np.int32)
+    sep = np.zeros(p * p * 64, dtype=np.int32)
+    for i in range(p):
+        for j in range(p):
+            ss = sepsets[i][j]
+            if ss is not None:
+                slen[i * p + j] = len(ss)
+                sep[(i * p + j) * 64:(i * p + j) * 64 + len(ss)] = [int(v) - 1 for v in ss]
+    return slen, sep
+
+
+def fastkpc_orient_wanpdag(skeleton, data, alpha=0.1, index=1.0, numCol=None,
+                           ci_method="dcc.gamma", hsic_params=None, permutation_params=None,
+                           orient_collider=True, solve_confl=False, rules=(True, True, True),
+                           rng_state=None):
+    """kpcalg-authority WAN-PDAG orientation of a skeleton result
+    (fastkpc/R/fast_kpc.R:263-445: kpcalg udag2wanpdag + regrVonPS) on the GPU.
+    skeleton: the dict fastkpc_compatible_cuda_skeleton returns (adjacency,
+    sepsets with 1-based members).  Permutation methods use the counter-based
+    indices unless permutation_params$rng == "R" (R's stream from rng_state,
+    .Random.seed[2:626] -- e.g. the skeleton's summary r_state_out -- or from
+    set.seed(seed)).  Returns pdag (p x p ints), the regrVonPS call trace
+    (0-based) and the residual CI trace."""
+    d = _as_data(data)
+    n, p = d.shape
+    adj = np.ascontiguousarray(np.asarray(skeleton["adjacency"], dtype=np.uint8).reshape(p, p))
+    slen, sep = _sepset_arrays(skeleton["sepsets"], p)
+    o = _lib.KpcOrientOptions()
+    _lib.lib().kpc_default_orient_options(o)
+    if ci_method not in CI_METHODS:
+        raise ValueError("ci_method must be one of " + ", ".join(CI_METHODS))
+    o.ci_method = CI_METHODS[ci_method]
+    o.alpha = float(alpha)
+    o.num_col = int(n // 10 if numCol is None else numCol)
+    o.hsic_sig = float((hsic_params or {}).get("sig", 1.0))
+    o.index = float(index)
+    perm = dict(permutation_params or {})
+    o.replicates = int(perm.get("replicates", 100))
+    o.include_observed = 1 if perm.get("include_observed", True) else 0
+    if perm.get("seed") is not None:
+        o.has_seed = 1
+        o.seed = int(perm["seed"])
+    rng = perm.get("rng", "philox")
+    if rng not in ("philox", "R"):
+        raise ValueError("permutation_params$rng must be 'philox' or 'R'")
+    o.perm_stream = 1 if rng == "R" else 0
+    keep = None
+    if rng == "R" and rng_state is not None:
+        keep = np.ascontiguousarray(np.asarray(rng_state, dtype=np.int64).astype(np.uint32)
+                                    .view(np.int32))
+        if keep.shape != (625,):
+            raise ValueError("rng_state must hold 625 integers (.Random.seed[2:626])")
+        o.r_state = iptr(keep)
+    o.orient_collider = 1 if orient_collider else 0
+    o.solve_confl = 1 if solve_confl else 0
+    for i in range(3):
+        o.rules[i] = 1 if rules[i] else 0
+    res = _lib.ctypes.POINTER(_lib.KpcOrientResult)()
+    check(_lib.lib().kpc_wanpdag_run(dptr(d), n, p, adj.ctypes.data_as(_lib._U8), iptr(slen),
+                                     iptr(sep), o, _lib.ctypes.byref(res)))
+    try:
+        r = res.contents
+        pdag = np.ctypeslib.as_array(r.pdag, (p * p,)).reshape(p, p).copy()
+        calls = [{"target": r.call_v[k],
+                  "subset": [r.call_S[k * 64 + i] for i in range(r.call_slen[k])],
+                  "parents": [r.call_parents[k * 64 + i] for i in range(r.call_plen[k])],
+                  "reject_count": r.call_reject[k]} for k in range(r.n_calls)]
+        tests = [{"call": r.test_call[k], "other": r.test_other[k],
+                  "p.value": r.test_out[4 * k], "statistic": r.test_out[4 * k + 1],
+                  "mean": r.test_out[4 * k + 2], "variance": r.test_out[4 * k + 3],
+                  "rejected": r.test_out[4 * k] < alpha} for k in range(r.n_tests)]
+        out = {"pdag": pdag, "events": calls, "ci_trace": tests,
+               "counts": {"regrvonps_calls": r.n_calls, "ci_test_count": r.n_tests},
+               "orientation_elapsed_sec": r.wall_seconds,
+               "authority": "kpcalg::udag2wanpdag (kpcmi GPU regrVonPS)"}
+        if rng == "R" and ci_method in ("dcc.perm", "hsic.perm"):
+            out["rng_end_state"] = [int(v) & 0xFFFFFFFF for v in list(r.r_state_out)]
+        return out
+    finally:
+        _lib.lib().kpc_orient_result_free(res)
+
+
+def fastkpc_compatible_cuda_wanpdag(data, alpha=0.1, labels=None, options=None):
+    """Skeleton then kpcalg-authority orientation
+    (fastkpc/R/fast_kpc.R:447-604): options as fastkpc_compatible_cuda_skeleton
+    plus orient_collider, solve_confl and rules."""
+    options = dict(options or {})
+    orient = {k: options.pop(k) for k in ("orient_collider", "solve_confl", "rules")
+              if k in options}
+    d = _as_data(data)
+    n, p = d.shape
+    sk = fastkpc_compatible_cuda_skeleton(d, alpha, labels=labels, options=options)
+    method = options.get("ci_method", "dcc.gamma")
+    perm = options.get("permutation_params", {})
+    orientation = fastkpc_orient_wanpdag(
+        sk, d, alpha=alpha, index=options.get("index", 1.0),
+        numCol=options.get("numCol", n // 10), ci_method=method,
+        hsic_params=options.get("hsic_params"), permutation_params=perm,
+        rng_state=sk["summary"].get("r_state_out"), **orient)
+    pd = orientation["pdag"]
+    directed = int(np.count_nonzero((pd == 1) & (pd.T == 0)))
+    undirected = int(np.count_nonzero((pd == 1) & (pd.T == 1)) // 2)
+    bidirected = int(np.count_nonzero((pd == 2) & (pd.T == 2)) // 2)
+    return {"schema_version": "fastkpc-compatible-cuda-wanpdag-v1",
+            "skeleton": sk, "orientation": orientation,
+            "metrics": {"skeleton_edge_count": int(np.count_nonzero(np.triu(sk["adjacency"]))),
+                        "directed_edge_count": directed,
+                        "undirected_edge_count": undirected,
+                        "bidirected_edge_count": bidirected,
+                        "skeleton_ci_test_count": int(sum(sk["n.edgetests"])),
+                        "orientation_ci_test_count": len(orientation["ci_trace"])}}

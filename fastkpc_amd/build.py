@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libkpcmi.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["kpc_dcov.hip", "kpc_dcov_lanczos.hip", "kpc_smooth.hip", "kpc_magic.hip", "kpc_hsic.hip",
-           "kpc_perm.hip", "kpc_noalloc.hip",
+           "kpc_perm.hip", "kpc_noalloc.hip", "kpc_orient.cpp",
            "kpc_host.cpp", "kpc_tune.cpp"]
 COMMON = [
     "-std=c++17", "-O3", "--offload-arch=gfx950", "-fPIC",

@@ -1930,7 +1930,10 @@ std::vector<Task> fit_sets(int p, int count, const int* targets, int s_len,
     for (int j = 0; j < s_len; ++j) {
       t.S[j] = S_rows[i][j];
       if (t.S[j] < 0 || t.S[j] >= p) throw KpcError(-1, "S out of range");
-      if (j > 0 && t.S[j] <= t.S[j - 1]) throw KpcError(-1, "S must be increasing");
+      // any order of distinct columns: the order of the smooth terms in the
+      // GAM formula (kpcalg's regrVonPS regresses on c(S, parents))
+      for (int i = 0; i < j; ++i)
+        if (t.S[i] == t.S[j]) throw KpcError(-1, "S must hold distinct columns");
       if (t.S[j] == t.x) throw KpcError(-1, "target must not be in S");
       key[j] = t.S[j];
     }

@@ -189,7 +189,8 @@ int kpc_regrxons_batch(const double* data, int n, int p, int count, const int* t
                        int s_len, const int* S, double* resid, double* info);
 
 /* regrXonS fits with per-fit conditioning sets of mixed size: fit i regresses
- * column targets[i] on the s_len[i] (1..62) increasing 0-based columns
+ * column targets[i] on the s_len[i] (1..62) distinct 0-based columns (in the
+ * given order: the order of the GAM's smooth terms)
  * S[s_off[i] .. s_off[i] + s_len[i]).  Fits are grouped by |S| (design width
  * q = 10, 30 or 1 + 9|S|): one design batch and one GCV launch per group, the
  * data uploaded once.  resid: count x n; info: count x 32 as in
@@ -250,6 +251,64 @@ int kpc_task_seed(unsigned long long user_seed, int ci_method, int x, int y, con
  * counter {i >> 1, r, 0x4b504331, 0}. */
 int kpc_permutation_table(int n, int replicates, int count, const unsigned long long* seeds,
                           int* out);
+
+/* ---- kpcalg-authority WAN-PDAG orientation ----
+ * Replaces fastkpc_orient_wanpdag_kpcalg_authority (fastkpc/R/fast_kpc.R:
+ * 263-445), i.e. kpcalg's udag2wanpdag (kpcalg/R/udag2wanpdag.R:104-436) with
+ * regrVonPS (kpcalg/R/regrvonps.R:17-39): the collider step, the generalised
+ * transitive step and rules 1-3, each regrVonPS call one GPU fit of V on
+ * c(S, parents(V)) (in that term order) and one batch of GPU CI tests of the
+ * residual against the raw columns of S.  Permutation methods draw either the
+ * counter-based indices (per call from (seed, method, V, sorted c(S,
+ * parents)), test i keyed (0, 1 + i)) or R's stream (KPC_PERM_R_STREAM,
+ * continuing from r_state, e.g. the skeleton's r_state_out, as kpcalg does
+ * after pcalg's skeleton). */
+typedef struct kpc_orient_options {
+  int ci_method;            /* kpc_ci_method */
+  double alpha;             /* regrVonPS rejects a residual test at p < alpha */
+  int num_col;
+  double hsic_sig;
+  double index;
+  int replicates;
+  int include_observed;
+  int has_seed;
+  unsigned long long seed;
+  int perm_stream;          /* kpc_perm_stream */
+  const int* r_state;       /* 625 ints (.Random.seed[2:626]) or NULL = set.seed(seed) */
+  int orient_collider;      /* udag2wanpdag orientCollider */
+  int solve_confl;          /* udag2wanpdag solve.confl (2 marks a bidirected edge) */
+  int rules[3];             /* rules 1-3 on/off */
+} kpc_orient_options;
+
+typedef struct kpc_orient_result {
+  int p;
+  int* pdag;                /* p x p row-major: pdag[i * p + j] = 1 for i -> j or i - j */
+  /* regrVonPS calls in order: target V, the subset S, the parents of V at
+   * the call, the number of residual tests with p < alpha */
+  int n_calls;
+  int* call_v;
+  int* call_slen;
+  int* call_S;              /* n_calls x 64, 0-based */
+  int* call_plen;
+  int* call_parents;        /* n_calls x 64, 0-based */
+  int* call_reject;
+  /* the residual tests in order: call index, the S member tested, and
+   * {p, statistic, mean, variance} */
+  int n_tests;
+  int* test_call;
+  int* test_other;
+  double* test_out;         /* n_tests x 4 */
+  int r_state_out[625];     /* KPC_PERM_R_STREAM permutation methods: the stream afterwards */
+  double wall_seconds;
+} kpc_orient_result;
+
+void kpc_default_orient_options(kpc_orient_options* opt);
+/* adjacency: p x p symmetric 0/1 (row-major, e.g. kpc_result.adjacency);
+ * sepset_len / sepset as in kpc_result (p x p, -1 = none; p x p x 64). */
+int kpc_wanpdag_run(const double* data, int n, int p, const unsigned char* adjacency,
+                    const int* sepset_len, const int* sepset, const kpc_orient_options* opt,
+                    kpc_orient_result** out);
+void kpc_orient_result_free(kpc_orient_result* r);
 
 /* Cache control, the counterpart of C_full_cuda_ci_one_call_cache_control
  * and _cache_state (fastkpc/src/full_cuda_ci_one_call.cpp:6812-6900). */

@@ -165,6 +165,40 @@ int orc_skeleton(const double* data, int n, int p, const orc_skeleton_options* o
                  orc_skeleton_result* res);
 void orc_skeleton_result_free(orc_skeleton_result* r);
 
+/* ---- kpcalg WAN-PDAG orientation (kpcalg/R/udag2wanpdag.R,
+ *      kpcalg/R/regrvonps.R; fastkpc/R/fast_kpc.R:263-445) ---- */
+typedef struct {
+  int ci_method;
+  double alpha;
+  int num_col;
+  double hsic_sig;
+  int replicates;
+  int include_observed;
+  double index;
+  unsigned long long perm_seed; /* counter-based indices: orc_orient_seed(perm_seed, ...) */
+  int r_stream;                 /* 1: R's stream, from r_state_in or set.seed(perm_seed) */
+  const orc_rng* r_state_in;
+  int orient_collider, solve_confl;
+  int rules[3];
+} orc_orient_options;
+
+typedef struct {
+  int* pdag; /* p x p row-major: 1 / 0 (2 = bidirected with solve_confl) */
+  int n_calls, call_cap;
+  int* call_v; int* call_slen; int* call_S; int* call_plen; int* call_parents; int* call_reject;
+  int n_tests, test_cap;
+  int* test_call; int* test_other;
+  double* test_out; /* n_tests x 4 {p, statistic, mean, variance} */
+  orc_rng rng_out;
+} orc_orient_result;
+
+unsigned long long orc_orient_seed(unsigned long long user_seed, int method, int V, int i,
+                                   const int* cond, int c_len);
+int orc_wanpdag(const double* data, int n, int p, const unsigned char* adjacency,
+                const int* sepset_len, const int* sepset, const orc_orient_options* opt,
+                orc_orient_result* res);
+void orc_orient_result_free(orc_orient_result* r);
+
 /* single CI test (kernelCItest semantics):
  * out = {p.value, statistic, mean, variance, decision margin} */
 int orc_ci_test(const double* data, int n, int p, int x, int y, const int* S, int s_len,

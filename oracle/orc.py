@@ -179,10 +179,13 @@ def default_control():
     return c
 
 
-def regrxons(data, target, S, control=None):
+def regrxons(data, target, S, control=None, ordered=False):
+    """regrXonS of column `target` on the columns S (0-based); S is sorted
+    unless ordered=True, which keeps the given column order (the order of the
+    smooth terms in the GAM formula)."""
     d = np.asfortranarray(data, dtype=np.float64)
     n, p = d.shape
-    Sa = np.ascontiguousarray(sorted(S), dtype=np.int32)
+    Sa = np.ascontiguousarray(list(S) if ordered else sorted(S), dtype=np.int32)
     resid = np.zeros(n)
     res = OrcMagicResult()
     ctl = control or default_control()
@@ -451,3 +454,86 @@ def hsic_perm_table(x, y, table, sig=1.0, num_col=35, include_observed=True):
                               1 if include_observed else 0, 0, tab.ctypes.data_as(_I), _d(out))
     return {"p.value": out[0], "statistic": out[1], "mean": out[2], "variance": out[3],
             "margin": out[4]}
+
+
+class OrcOrientOptions(ctypes.Structure):
+    _fields_ = [("ci_method", ctypes.c_int), ("alpha", ctypes.c_double), ("num_col", ctypes.c_int),
+                ("hsic_sig", ctypes.c_double), ("replicates", ctypes.c_int),
+                ("include_observed", ctypes.c_int), ("index", ctypes.c_double),
+                ("perm_seed", ctypes.c_ulonglong), ("r_stream", ctypes.c_int),
+                ("r_state_in", ctypes.POINTER(OrcRng)), ("orient_collider", ctypes.c_int),
+                ("solve_confl", ctypes.c_int), ("rules", ctypes.c_int * 3)]
+
+
+class OrcOrientResult(ctypes.Structure):
+    _fields_ = [("pdag", _I), ("n_calls", ctypes.c_int), ("call_cap", ctypes.c_int),
+                ("call_v", _I), ("call_slen", _I), ("call_S", _I), ("call_plen", _I),
+                ("call_parents", _I), ("call_reject", _I), ("n_tests", ctypes.c_int),
+                ("test_cap", ctypes.c_int), ("test_call", _I), ("test_other", _I),
+                ("test_out", _D), ("rng_out", OrcRng)]
+
+
+def sepset_arrays(sepsets, p):
+    """1-based sepset lists (None = no sepset) -> (len p x p, 0-based p x p x 64)."""
+    slen = np.full(p * p, -1, dtype=np.int32)
+    sep = np.zeros(p * p * 64, dtype=np.int32)
+    for i in range(p):
+        for j in range(p):
+            s = sepsets[i][j]
+            if s is not None:
+                slen[i * p + j] = len(s)
+                sep[(i * p + j) * 64:(i * p + j) * 64 + len(s)] = [int(v) - 1 for v in s]
+    return slen, sep
+
+
+def wanpdag(data, adjacency, sepsets, method="dcc.gamma", alpha=0.1, num_col=35, sig=1.0,
+            replicates=100, seed=0, index=1.0, include_observed=True, r_stream=False,
+            r_state=None, orient_collider=True, solve_confl=False, rules=(True, True, True)):
+    """kpcalg udag2wanpdag on the oracle: pdag (p x p ints), the regrVonPS
+    calls and their residual tests (0-based variables)."""
+    d = np.asfortranarray(data, dtype=np.float64)
+    n, p = d.shape
+    adj = np.ascontiguousarray(np.asarray(adjacency, dtype=np.uint8).reshape(p, p))
+    slen, sep = sepset_arrays(sepsets, p)
+    L = lib()
+    if not hasattr(L, "_orient_ready"):
+        L.orc_wanpdag.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_ubyte),
+                                  _I, _I, ctypes.POINTER(OrcOrientOptions),
+                                  ctypes.POINTER(OrcOrientResult)]
+        L.orc_orient_result_free.argtypes = [ctypes.POINTER(OrcOrientResult)]
+        L._orient_ready = True
+    st = None
+    if r_state is not None:
+        st = OrcRng()
+        st.mti = int(r_state[0])
+        for i in range(624):
+            st.mt[i] = int(r_state[1 + i]) & 0xFFFFFFFF
+    o = OrcOrientOptions(METHODS[method], alpha, num_col, sig, replicates,
+                         1 if include_observed else 0, index, seed, 1 if r_stream else 0,
+                         ctypes.pointer(st) if st is not None else None,
+                         1 if orient_collider else 0, 1 if solve_confl else 0,
+                         (ctypes.c_int * 3)(*[1 if r else 0 for r in rules]))
+    res = OrcOrientResult()
+    rc = L.orc_wanpdag(_d(d), n, p, adj.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)),
+                       slen.ctypes.data_as(_I), sep.ctypes.data_as(_I), ctypes.byref(o),
+                       ctypes.byref(res))
+    try:
+        if rc != 0:
+            raise RuntimeError("orc_wanpdag failed %d" % rc)
+        pdag = np.ctypeslib.as_array(res.pdag, (p * p,)).reshape(p, p).copy()
+        calls = []
+        for k in range(res.n_calls):
+            calls.append({"V": res.call_v[k],
+                          "S": [res.call_S[k * 64 + i] for i in range(res.call_slen[k])],
+                          "parents": [res.call_parents[k * 64 + i] for i in range(res.call_plen[k])],
+                          "reject": res.call_reject[k]})
+        tests = []
+        for k in range(res.n_tests):
+            tests.append({"call": res.test_call[k], "other": res.test_other[k],
+                          "out": [res.test_out[k * 4 + i] for i in range(4)]})
+        out = {"pdag": pdag, "calls": calls, "tests": tests}
+        if r_stream:
+            out["r_state"] = [int(res.rng_out.mti)] + [int(v) for v in res.rng_out.mt]
+        return out
+    finally:
+        L.orc_orient_result_free(ctypes.byref(res))
