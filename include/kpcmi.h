@@ -304,7 +304,8 @@ typedef struct kpc_orient_result {
 
 void kpc_default_orient_options(kpc_orient_options* opt);
 /* adjacency: p x p symmetric 0/1 (row-major, e.g. kpc_result.adjacency);
- * sepset_len / sepset as in kpc_result (p x p, -1 = none; p x p x 64). */
+ * sepset_len / sepset laid out as the kpc_result fields: p x p (-1 = none) and
+ * p x p x 64. */
 int kpc_wanpdag_run(const double* data, int n, int p, const unsigned char* adjacency,
                     const int* sepset_len, const int* sepset, const kpc_orient_options* opt,
                     kpc_orient_result** out);
