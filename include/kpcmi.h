@@ -26,8 +26,10 @@
  * Conventions: every entry returns 0 on success and a negative status on
  * failure (then kpc_last_error() describes it); inputs are host pointers to
  * column-major n x p doubles unless the name says _device; outputs are caller
- * allocated except kpc_result, which kpc_result_free releases.  One context
- * per host thread; calls on one GPU from several threads serialise.
+ * allocated except kpc_result, which kpc_result_free releases.  Calls from
+ * several host threads may run at the same time on one GPU: each call has its
+ * own streams and device arenas (a thread reuses only the arenas it released
+ * itself), and the process-wide result / target-state caches are locked.
  */
 #ifndef KPCMI_H
 #define KPCMI_H

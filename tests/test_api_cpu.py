@@ -126,3 +126,18 @@ def test_regression_batch_generator():
         assert X.min() >= -2.0 and X.max() <= 2.0
         noise = d[:, tg] - np.sum(np.sin(2 * X) + 0.3 * X * X, axis=1)
         assert abs(noise.std() - 0.3) < 0.12
+
+
+def test_plain_c_consumer_compiles_and_runs(tmp_path):
+    """tests/c_consumer.c includes kpcmi.h as a C99 program, links
+    libkpcmi.so and calls the host-only entry points (defaults, task seeds,
+    cache control, argument validation with kpc_last_error)."""
+    exe = str(tmp_path / "c_consumer")
+    libdir = os.path.join(ROOT, "fastkpc_amd")
+    cmd = ["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "c_consumer.c"), "-L", libdir, "-lkpcmi",
+           "-Wl,-rpath," + libdir, "-o", exe]
+    subprocess.run(cmd, check=True, capture_output=True, text=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "all checks passed" in r.stdout
