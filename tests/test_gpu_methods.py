@@ -12,7 +12,8 @@ import os
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, TIE_MARGIN, compare_skeleton, p_close, replay_at_gpu_sp
+from conftest import (GOLDEN, TIE_MARGIN, check_level_rows, compare_skeleton, p_close,
+                      replay_at_gpu_sp)
 
 pytestmark = pytest.mark.gpu
 
@@ -229,54 +230,16 @@ def test_skeleton_trace_statistic_and_edge_columns(method):
 
 @pytest.mark.parametrize("cfg", ["sem200x8", "sem351x48"])
 def test_level_rows_match_the_level_plan(cfg):
-    """level_rows (the reference's levels data frame): tasks_planned equals
-    sum over the edges present at the level's start of C(|adj(x)\\{y}|, l) +
-    C(|adj(y)\\{x}|, l), with the adjacency rebuilt from the trace's deletions
-    (pcalg stable: deletions apply at the level's end); tests_replayed is
-    n.edgetests, and the
-    deletions rebuild the final adjacency.  tasks_ignored_after_delete is
-    counted edge by edge: the planned tasks after each deleting row."""
-    from math import comb
+    """level_rows (the reference's levels data frame) against the plan
+    rebuilt from the trace alone (conftest.check_level_rows): tasks planned,
+    tasks ignored after each deleting row, deletions and the final
+    adjacency."""
     import fastkpc_amd as fk
     from fastkpc_amd.sem import config_data
     data, _, _ = config_data(cfg)
     res = fk.fastkpc_compatible_cuda_skeleton(data, 0.1, options={"numCol": 35})
-    rows, t = res["level_rows"], res["tasks"]
-    p = data.shape[1]
-    adj = np.ones((p, p), bool)
-    np.fill_diagonal(adj, False)
-    assert rows["tests_replayed"] == res["n.edgetests"]
-    for l in range(res["levels"]):
-        planned = 0
-        for a in range(p):
-            for b in range(a + 1, p):
-                if adj[a, b]:
-                    planned += comb(int(adj[a].sum()) - 1, l) + comb(int(adj[b].sum()) - 1, l)
-        assert rows["tasks_planned"][l] == planned, (l, rows["tasks_planned"][l], planned)
-        # per edge: an edge still present replays every planned task; a deleted
-        # edge ignores the planned tasks after its deleting row
-        lv = t["level"] == l
-        ignored = 0
-        for a in range(p):
-            for b in range(a + 1, p):
-                if not adj[a, b]:
-                    continue
-                n_plan = comb(int(adj[a].sum()) - 1, l) + comb(int(adj[b].sum()) - 1, l)
-                er = np.nonzero(lv & (t["edge_x"] == a + 1) & (t["edge_y"] == b + 1))[0]
-                dl = er[t["native_edge_deleted"][er]]
-                if len(dl) == 0:
-                    assert len(er) == n_plan, (l, a, b, len(er), n_plan)
-                else:
-                    assert len(dl) == 1 and dl[0] == er[-1]
-                    ignored += n_plan - len(er)
-        assert rows["tasks_ignored_after_delete"][l] == ignored, (l, rows, ignored)
-        assert rows["tasks_planned"][l] == res["n.edgetests"][l] + ignored
-        assert rows["rounds"][l] >= 1
-        sel = (t["level"] == l) & t["native_edge_deleted"]
-        assert rows["deletions"][l] == int(sel.sum())
-        for x, y in zip(t["edge_x"][sel], t["edge_y"][sel]):
-            adj[x - 1, y - 1] = adj[y - 1, x - 1] = False
-    assert np.array_equal(adj, res["adjacency"])
+    check_level_rows(res, data.shape[1])
+    assert all(r >= 1 for r in res["level_rows"]["rounds"])
 
 
 def test_trace_task_index_is_the_position_in_the_level_plan():
