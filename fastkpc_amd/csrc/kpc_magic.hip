@@ -13,6 +13,9 @@
 //   D^-1 V1' rS_i so every product is nr x rank_i), Newton / steepest descent
 //   with step halving, the mgcv convergence test and the final infinity
 //   probes (:1118-1346), then residuals y - X beta.
+#include <algorithm>
+#include <cstdlib>
+
 #include "kpc_common.h"
 #include "kpc_device.h"
 #include "kpc_wave_linalg.h"
@@ -580,17 +583,53 @@ struct MagicLds {
   double* syA;    // m x m: Hessian eigen work
   double* syZ;    // m x m: eigenvectors
   double* syw;    // m: eigenvalues
+  // trial slots (null unless the launch scores step-halving trials on
+  // several waves at once): the fit's constant inputs staged once per fit
+  // and each slot's own outputs
+  double* Rst;     // q x q: the design's R (column-major)
+  double* rSst;    // m rk x q: penalty roots rS
+  double* yst;     // q: Q'y
+  double* trials;  // kSpecTries x m: one iteration's trial points in try order
+  double* sres;    // 4 x 8: per-slot {score, rss, edf, rank margin, certified}
+  double* sy1;     // 4 x q: y1 of slots 1..3
+  double* sQ3;     // q x q: Q1 of slot 3 (slots 1 and 2 use X4 and V)
+  int nslot;
 };
 
-__host__ __device__ inline size_t magic_lds_doubles(int n, int q, int m, bool aug_in_lds) {
+// Trial slots: a multi-penalty additive design (rank-8 penalty blocks,
+// q = 1 + 9 m, 3 <= m <= 5) on a 256-lane workgroup whose working set is all
+// in LDS scores up to nslot step-halving trials at once, one per wave.
+constexpr int kSpecTries = 32;
+__host__ __device__ inline bool magic_slot_shape(int q, int m, int rk) {
+  return rk == 8 && m >= 3 && m <= 5 && q == 1 + 9 * m;
+}
+__host__ __device__ inline long long magic_spec_doubles(int q, int m, int nslot) {
+  if (nslot <= 1) return 0;
+  return (long long)q * q + 8LL * m * q + q + (long long)kSpecTries * m + 32 + 4LL * q +
+         (nslot == 4 ? (long long)q * q : 0);
+}
+
+__host__ __device__ inline size_t magic_lds_doubles(int n, int q, int m, bool aug_in_lds,
+                                                    int nslot = 1) {
   (void)n;
   size_t s = (aug_in_lds ? (size_t)magic_big_doubles(q) : 0) + (size_t)magic_small_doubles(q, m);
   if (magic_deriv_in_lds(q, m)) s += (size_t)magic_deriv_doubles(q, m);
-  return s;
+  return s + (size_t)magic_spec_doubles(q, m, nslot);
+}
+
+// Slots for (n, q, m, rk): 4 while the workgroup's LDS stays within half a
+// CU (two fits per CU), else 3 (slots 1 and 2 reuse V and X4) within the
+// single-fit budget, else 1 (trials scored one at a time).
+inline int magic_spec_slots(int n, int q, int m, int rk) {
+  if (!magic_slot_shape(q, m, rk) || q <= 16 || !magic_deriv_in_lds(q, m)) return 1;
+  const size_t base = magic_lds_doubles(n, q, m, true);
+  if ((base + (size_t)magic_spec_doubles(q, m, 4)) * 8 <= 80 * 1024) return 4;
+  if ((base + (size_t)magic_spec_doubles(q, m, 3)) * 8 <= 150 * 1024) return 3;
+  return 1;
 }
 
 __device__ __attribute__((always_inline)) MagicLds carve_magic(double* base, int n, int q, int m, bool aug_in_lds,
-                                double* gscratch, bool deriv_in_lds) {
+                                double* gscratch, bool deriv_in_lds, int nslot = 1) {
   MagicLds L;
   double* p = base;
   // response vectors live in the global slab after the working set
@@ -634,6 +673,19 @@ __device__ __attribute__((always_inline)) MagicLds carve_magic(double* base, int
   L.ord = ip; ip += q;
   L.piv = ip; ip += q;
   L.ictl = ip;
+  ip += 32;
+  L.nslot = nslot > 1 && aug_in_lds && deriv_in_lds ? nslot : 1;
+  L.Rst = L.rSst = L.yst = L.trials = L.sres = L.sy1 = L.sQ3 = nullptr;
+  if (L.nslot > 1) {
+    double* sp = reinterpret_cast<double*>(ip);  // 2 q + 32 ints: 8-byte aligned
+    L.Rst = sp; sp += (size_t)q * q;
+    L.rSst = sp; sp += (size_t)8 * m * q;
+    L.yst = sp; sp += q;
+    L.trials = sp; sp += (size_t)kSpecTries * m;
+    L.sres = sp; sp += 32;
+    L.sy1 = sp; sp += 4 * (size_t)q;
+    L.sQ3 = L.nslot == 4 ? sp : nullptr;
+  }
   return L;
 }
 
@@ -800,8 +852,10 @@ __device__ __attribute__((always_inline)) void jacobi_svd(double* A, int rows, i
 // quads, then the half-row mirror.
 template <int kCtrl>
 __device__ __forceinline__ double dpp_move(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), kCtrl, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), kCtrl, 0xF, 0xF, false);
+  // every lane of an 8-lane group is active whenever a group sums, so no
+  // lane reads a disabled source and the moves need no fill value
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), kCtrl, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), kCtrl, 0xF, 0xF, true);
   return __hiloint2double(hi, lo);
 }
 
@@ -1204,6 +1258,276 @@ __device__ __attribute__((always_inline)) void upper_tri_inverse_fast(const doub
     case 4: upper_tri_inverse_regs<4>(R, ld, q, X, dinv); break;
     case 6: upper_tri_inverse_regs<6>(R, ld, q, X, dinv); break;
     default: upper_tri_inverse(R, ld, q, X, dinv);
+  }
+}
+
+// ---- one-wave certified-QR evaluation (additive designs: m rank-8 penalty
+// blocks, q = 1 + 9 m).  Operation for operation the arithmetic of the
+// workgroup path of magic_eval: stacked_qr_cols' per-column dots, updates and
+// reflectors (8 lanes per column, lane sub holding E rows sub + 8 t, which
+// for rank-8 blocks is penalty block t), Q1 formed alongside, y1 = Q1' y0 per
+// column, and the closing block_sum3 of a 256-lane workgroup replayed virtual
+// wave by virtual wave (the same partial sums in the same lanes, the same
+// shuffle tree, the four wave totals added in the same order).  Only where
+// values travel changes: columns c = g + 8 i belong to lane group g, the raw
+// tail of column j + 1 is read across lanes from its group once step j has
+// updated it and every group forms the reflector itself, so the whole
+// factorisation runs without a workgroup barrier and the waves of a
+// workgroup can score different trial points at once.
+
+template <int K>
+__device__ __forceinline__ void wave_reflector(const double (&ur)[K], double alpha, double (&u)[K],
+                                               double& tj, double& beta) {
+  double s2 = 0.0;
+#pragma unroll
+  for (int t = 0; t < K; ++t) s2 += ur[t] * ur[t];
+  s2 = sum8(s2);
+  double scal = 0.0;
+  tj = 0.0;
+  beta = alpha;
+  if (s2 > 0.0) {
+    const double h = fma(alpha, alpha, s2);
+    const double r = nrsq(h);
+    const double nrm = h * r;
+    beta = -copysign(nrm, alpha);
+    tj = fma(fabs(alpha), r, 1.0);
+    scal = copysign(nrcp(fabs(alpha) + nrm), alpha);
+  }
+#pragma unroll
+  for (int t = 0; t < K; ++t) u[t] = ur[t] * scal;
+}
+
+// sum8 of every v[i], i in [I0, I1), stage by stage (the same lane order per
+// value as sum8; the stages of different values interleave).
+template <int I0, int I1, int N>
+__device__ __forceinline__ void sum8_range(double (&v)[N]) {
+#pragma unroll
+  for (int i = I0; i < I1; ++i) v[i] += dpp_move<0xB1>(v[i]);
+#pragma unroll
+  for (int i = I0; i < I1; ++i) v[i] += dpp_move<0x4E>(v[i]);
+#pragma unroll
+  for (int i = I0; i < I1; ++i) v[i] += dpp_move<0x141>(v[i]);
+}
+
+// Step j (in segment S = j / 8) of the structured QR on one wave: reflector
+// j (u, tj) onto the E columns c > j of slots i >= S (row j of the top block,
+// R2[j][c] out) and onto the X columns c <= j of slots i <= S (row j of Q1
+// out), then reflector j + 1 from its column's updated tail.  Per column the
+// operations and their order are stacked_qr_cols' / qr_cols_apply_x's;
+// columns are computed side by side without branches.
+template <int K, int NC, int Q, int S>
+__device__ __forceinline__ void wave_qr_step(int j, int g, int sub, double (&E)[NC][K],
+                                             double (&X)[NC][K], double (&u)[K], double& tj,
+                                             const double* Rst, double* R2, int ldr, double* Q1,
+                                             int ldq) {
+  if constexpr (S < NC) {
+    double rjc[NC], dot[NC];
+#pragma unroll
+    for (int i = S; i < NC; ++i) {
+      const int c = g + 8 * i;
+      rjc[i] = c < Q ? Rst[c * Q + j] : 0.0;
+      dot[i] = 0.0;
+      if (sub == 0) dot[i] = rjc[i];
+    }
+#pragma unroll
+    for (int t = 0; t < K; ++t)
+#pragma unroll
+      for (int i = S; i < NC; ++i) dot[i] += u[t] * E[i][t];
+    sum8_range<S, NC>(dot);
+    // (columns c <= j and the padding are dead: their E may change freely)
+    const bool upd = tj != 0.0;
+#pragma unroll
+    for (int i = S; i < NC; ++i) {
+      const int c = g + 8 * i;
+      const double d = dot[i] * tj;
+      if (upd) {
+#pragma unroll
+        for (int t = 0; t < K; ++t) E[i][t] -= d * u[t];
+      }
+      if (R2 && c > j && c < Q && sub == 0) R2[(size_t)c * ldr + j] = upd ? rjc[i] - d : rjc[i];
+    }
+    double dx[NC];
+#pragma unroll
+    for (int i = 0; i <= S; ++i) {
+      const int c = g + 8 * i;
+      dx[i] = 0.0;
+      if (sub == 0) dx[i] = c == j ? 1.0 : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < K; ++t)
+#pragma unroll
+      for (int i = 0; i <= S; ++i) dx[i] += u[t] * X[i][t];
+    sum8_range<0, S + 1>(dx);
+    // (a column c > j still holds exact +0 and keeps it: d = +0, +0 - (+-0)
+    // = +0, and its Q1 entry is the +0 written up front)
+#pragma unroll
+    for (int i = 0; i <= S; ++i) {
+      const int c = g + 8 * i;
+      const double d = dx[i] * tj;
+      if (c < Q && sub == 0) Q1[(size_t)j * ldq + c] = (c == j ? 1.0 : 0.0) - d;
+#pragma unroll
+      for (int t = 0; t < K; ++t) X[i][t] -= d * u[t];
+    }
+    if (j + 1 < Q) {
+      const int jn = j + 1, gn = jn & 7;
+      const bool next_slot = (jn >> 3) != S;  // column j + 1 opens the next slot
+      double ur[K], beta;
+#pragma unroll
+      for (int t = 0; t < K; ++t) {
+        double v = E[S][t];
+        if constexpr (S + 1 < NC) v = next_slot ? E[S + 1][t] : v;
+        ur[t] = __shfl(v, gn * 8 + sub, KPC_WAVE);
+      }
+      wave_reflector<K>(ur, Rst[jn * Q + jn], u, tj, beta);
+      if (R2 && g == 0 && sub == 0) R2[(size_t)jn * ldr + jn] = beta;
+    }
+  }
+}
+
+// Score of the design at log(sp) = lspv by the calling wave.  Rst (q x q), rSst
+// (8m x q) and y0 (Q'y, q) are the fit's staged inputs; nrm the design norms.
+// Outputs: R2 (upper triangle, ld ldr; null: not kept), Q1 (ld ldq), y1, and
+// res = {score, rss, edf, rank margin, 1} -- or res[4] = 0 when the stacked
+// root or the stage-1 rank certificate does not hold (the caller then takes
+// the workgroup path, which decides those cases).
+template <int K>
+__device__ __attribute__((always_inline)) void wave_eval_additive(
+    const double* Rst, const double* rSst, const double* y0, const double* nrm,
+    const double* lspv, int n, double ynorm2, double rank_tol, double* R2, int ldr, double* Q1,
+    int ldq, double* y1, double* res, double* tacc = nullptr) {
+  constexpr int Q = 1 + 9 * K;
+  constexpr int NC = (Q + 7) / 8;
+  const int lane = threadIdx.x & (KPC_WAVE - 1);
+  const long long tw0 = tacc ? wall_clock64() : 0;
+  const int g = lane >> 3, sub = lane & 7;
+  double sp[K], sq[K];
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    sp[i] = exp(lspv[i]);
+    sq[i] = sqrt(sp[i]);
+  }
+  double dmax = 0.0;
+#pragma unroll
+  for (int i = 0; i < K; ++i) dmax = fmax(dmax, sp[i] * nrm[2 + K + i]);
+  const double dstop = (double)Q * kEps * dmax;
+  bool stacked = true;
+#pragma unroll
+  for (int i = 0; i < K; ++i) stacked = stacked && sp[i] * nrm[2 + 2 * K + i] > 1e3 * dstop;
+  double emax = nrm[0];
+#pragma unroll
+  for (int i = 0; i < K; ++i) emax += sp[i] * nrm[2 + i];
+  const double lb = 1.0 / sqrt(nrm[1] * emax);
+  if (!(stacked && isfinite(lb) && lb > 2.0 * rank_tol)) {
+    if (lane == 0) res[4] = 0.0;
+    return;
+  }
+  double E[NC][K], X[NC][K];
+#pragma unroll
+  for (int i = 0; i < NC; ++i) {
+    const int c = g + 8 * i;
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+      E[i][t] = c < Q ? sq[t] * rSst[(sub + 8 * t) * Q + c] : 0.0;
+      X[i][t] = 0.0;
+    }
+  }
+  // Q1 is upper triangular (reflector j touches row j of the top block
+  // only): row c of Q1' [I; 0] stays exactly +0 before step c, so those
+  // entries are written once here and column c of X joins at step c.
+#pragma unroll
+  for (int i = 0; i < NC; ++i) {
+    const int c = g + 8 * i;
+    if (c < Q)
+      for (int j = sub; j < c; j += 8) Q1[(size_t)j * ldq + c] = 0.0;
+  }
+  double u[K], tj, beta;
+  {
+    double ur[K];
+#pragma unroll
+    for (int t = 0; t < K; ++t) ur[t] = __shfl(E[0][t], sub, KPC_WAVE);
+    wave_reflector<K>(ur, Rst[0], u, tj, beta);
+    if (R2 && lane == 0) R2[0] = beta;
+  }
+  const long long tw1 = tacc ? wall_clock64() : 0;
+  for (int j = 0; j < Q; ++j) {
+    // step j runs in segment j / 8: E columns of slots >= j / 8, X columns of
+    // slots <= j / 8 (compile-time ranges; the rest are finished or zero)
+    switch (j >> 3) {
+      case 0: wave_qr_step<K, NC, Q, 0>(j, g, sub, E, X, u, tj, Rst, R2, ldr, Q1, ldq); break;
+      case 1: wave_qr_step<K, NC, Q, 1>(j, g, sub, E, X, u, tj, Rst, R2, ldr, Q1, ldq); break;
+      case 2: wave_qr_step<K, NC, Q, 2>(j, g, sub, E, X, u, tj, Rst, R2, ldr, Q1, ldq); break;
+      case 3: wave_qr_step<K, NC, Q, 3>(j, g, sub, E, X, u, tj, Rst, R2, ldr, Q1, ldq); break;
+      case 4: wave_qr_step<K, NC, Q, 4>(j, g, sub, E, X, u, tj, Rst, R2, ldr, Q1, ldq); break;
+      default: wave_qr_step<K, NC, Q, 5>(j, g, sub, E, X, u, tj, Rst, R2, ldr, Q1, ldq); break;
+    }
+  }
+  wave_sync();
+  const long long tw2 = tacc ? wall_clock64() : 0;
+  // y1 = Q1' y0, one 8-lane group per column
+#pragma unroll
+  for (int i = 0; i < NC; ++i) {
+    const int c = g + 8 * i;
+    if (c < Q) {
+      double sacc = 0.0;
+      for (int r = sub; r < Q; r += 8) sacc += Q1[(size_t)c * ldq + r] * y0[r];
+      sacc = sum8(sacc);
+      if (sub == 0) y1[c] = sacc;
+    }
+  }
+  wave_sync();
+  // y'AAy, edf, y'Ay: the 256-lane partial sums and block_sum3, wave by wave
+  double ta = 0.0, tb = 0.0, tc = 0.0;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    const int vt = 64 * w + lane, cgv = vt >> 3;
+    double pa = 0.0, pe = 0.0, py = 0.0;
+    for (int r = cgv; r < Q; r += 32) {
+      double v = 0.0;
+      for (int c = sub; c < Q; c += 8) {
+        const double uu = Q1[(size_t)c * ldq + r];
+        v += uu * y1[c];
+        pe += uu * uu;
+      }
+      v = sum8(v);
+      if (sub == 0) pa += v * v;
+    }
+    if (vt < Q) py += y1[vt] * y1[vt];
+    pa = wave_sum(pa);
+    pe = wave_sum(pe);
+    py = wave_sum(py);
+    ta += pa;
+    tb += pe;
+    tc += py;
+  }
+  if (lane == 0) {
+    const double yAAy = ta, edf = tb, yAy = tc;
+    double rss = ynorm2 - 2.0 * yAy + yAAy;
+    if (rss < 0.0) rss = 0.0;
+    const double delta = (double)n - edf;
+    res[0] = delta > 1e-8 ? (double)n * rss / (delta * delta) : __builtin_inf();
+    res[1] = rss;
+    res[2] = edf;
+    res[3] = lb / rank_tol - 1.0;
+    res[4] = 1.0;
+    if (tacc) {  // phase ticks: set-up, QR with Q1, y1 and the sums
+      const long long tw3 = wall_clock64();
+      tacc[0] += (double)(tw1 - tw0);
+      tacc[1] += (double)(tw2 - tw1);
+      tacc[2] += (double)(tw3 - tw2);
+    }
+  }
+  wave_sync();
+}
+
+__device__ __attribute__((always_inline)) void wave_eval_slot(
+    const MagicLds& L, int m, const double* lspv, int n, double ynorm2, double rank_tol,
+    double* R2, int ldr, double* Q1, int ldq, double* y1, double* res) {
+  switch (m) {
+    case 3: wave_eval_additive<3>(L.Rst, L.rSst, L.yst, L.nrm, lspv, n, ynorm2, rank_tol, R2, ldr, Q1, ldq, y1, res); break;
+    case 4: wave_eval_additive<4>(L.Rst, L.rSst, L.yst, L.nrm, lspv, n, ynorm2, rank_tol, R2, ldr, Q1, ldq, y1, res); break;
+    case 5: wave_eval_additive<5>(L.Rst, L.rSst, L.yst, L.nrm, lspv, n, ynorm2, rank_tol, R2, ldr, Q1, ldq, y1, res); break;
+    default:
+      if ((threadIdx.x & (KPC_WAVE - 1)) == 0) res[4] = 0.0;
   }
 }
 
@@ -1742,6 +2066,173 @@ __device__ __attribute__((always_inline)) void small_syev_wave(const double* Hin
 // Every LDS pointer then derives from the dynamic shared array on every path,
 // so the compiler addresses it as LDS (ds_read / ds_write with immediate
 // offsets) instead of through generic flat pointers.
+// End of an optimiser iteration (mgcv magic): the convergence test from
+// iteration 4 on (score reduction and gradient norm against tol; a step
+// halving that ran out counts as converged, flagged as a failed step), the
+// gradient of the accepted point taken as the current one, the Hessian's
+// eigen-decomposition, and the next Newton step (capped at max_newton_step;
+// steepest descent when the Hessian is not positive definite) and
+// steepest-descent step.  Workgroup-wide; decisions on lane 0.
+__device__ __attribute__((always_inline)) void magic_conv_and_step(const MagicLds& L,
+                                                                   const MagicControl& ctl, int m,
+                                                                   int iter, int max_half) {
+  const int tid = threadIdx.x;
+  double* st = L.dctl;
+  int* ic = L.ictl;
+  if (tid == 0) {
+    if (iter > 3) {
+      double gn = 0.0;
+      for (int i = 0; i < m; ++i) gn += L.grad[i] * L.grad[i];
+      gn = sqrt(gn);
+      const double t1 = ctl.tol * (1.0 + st[3]);
+      const double t2 = pow(ctl.tol, 1.0 / 3.0) * (1.0 + fabs(st[3]));
+      int conv = st[4] <= t1 && gn <= t2;
+      st[5] = fmin(st[5], fmin(fabs(st[4] - t1) / t1, fabs(gn - t2) / t2));
+      if (ic[6] == max_half) { conv = 1; ic[7] = 1; }
+      ic[4] = conv;
+    }
+    for (int i = 0; i < m; ++i) L.grad[i] = L.cgrad[i];
+  }
+  __syncthreads();
+  if (tid < KPC_WAVE) small_syev_wave(L.hess, m, L.syw, L.syZ, L.syA);
+  __syncthreads();
+  if (tid == 0) {
+    double* w = L.syw;
+    double* Z = L.syZ;
+    int sd = 0;
+    for (int i = 0; i < m; ++i) {
+      if (w[i] < 0.0) sd = 1;
+      st[5] = fmin(st[5], fabs(w[i]) / st[3]);
+    }
+    ic[3] = sd;
+    if (!sd) {
+      for (int r = 0; r < m; ++r) {
+        double v = 0.0;
+        for (int c = 0; c < m; ++c) {
+          double pc = 0.0;
+          for (int k = 0; k < m; ++k) pc += Z[c * m + k] * L.grad[k];
+          v += Z[c * m + r] * (pc / w[c]);
+        }
+        L.nstep[r] = -v;
+      }
+      double mx = fabs(L.nstep[0]);
+      for (int i = 1; i < m; ++i) mx = fmax(mx, fabs(L.nstep[i]));
+      if (mx > ctl.max_newton_step)
+        for (int i = 0; i < m; ++i) L.nstep[i] *= ctl.max_newton_step / mx;
+    }
+    double mg = fabs(L.grad[0]);
+    for (int i = 1; i < m; ++i) mg = fmax(mg, fabs(L.grad[i]));
+    for (int i = 0; i < m; ++i) L.sstep[i] = mg == 0.0 ? 0.0 : -L.grad[i] / mg;
+  }
+  __syncthreads();
+}
+
+// Per-fit prologue of the GCV kernels: y into the fit's slab (and a copy),
+// the timing fields cleared, y0 = Q'y, |y|^2, the starting log(sp) (the
+// cached fit's when finfo is set) and the design norms.  Returns |y|^2.
+__device__ __attribute__((always_inline)) double magic_prologue(const MagicLds& L, const double* D,
+                                                                const DesignLayout& DL,
+                                                                const double* y, int n, int q,
+                                                                int m, const double* finfo) {
+  const int tid = threadIdx.x;
+  for (int r = tid; r < n; r += blockDim.x) { L.y[r] = y[r]; L.yorig[r] = y[r]; }
+  if (tid == 0) {
+    L.dctl[26] = L.dctl[27] = L.dctl[29] = 0.0;
+    L.dctl[20] = L.dctl[21] = L.dctl[22] = L.dctl[23] = 0.0;
+    L.dctl[8] = L.dctl[9] = L.dctl[10] = L.dctl[11] = 0.0;
+    L.dctl[12] = L.dctl[13] = 0.0;
+    L.ictl[28] = 0;
+  }
+  __syncthreads();
+  // y0 = Q' y (reflectors in order)
+  for (int i = 0; i < q; ++i) {
+    const double* v = D + DL.QR + (size_t)i * n;
+    const double tau = D[DL.tau + i];
+    double part = 0.0;
+    for (int r = i + tid; r < n; r += blockDim.x) part += (r == i ? 1.0 : v[r]) * L.y[r];
+    const double s = block_sum(part, L.red) * tau;
+    for (int r = i + tid; r < n; r += blockDim.x) L.y[r] -= s * (r == i ? 1.0 : v[r]);
+    __syncthreads();
+  }
+  double part = 0.0;
+  for (int r = tid; r < n; r += blockDim.x) part += L.yorig[r] * L.yorig[r];
+  const double ynorm2 = block_sum(part, L.red);
+  if (tid < m) L.lsp[tid] = finfo ? finfo[8 + tid] : D[DL.lsp0 + tid];
+  for (int i = tid; i < 2 + 3 * m; i += blockDim.x) L.nrm[i] = D[DL.norms + i];
+  return ynorm2;
+}
+
+// Residuals y - X beta at the point last evaluated (the selected log(sp))
+// and the fit record: the cached record for a replayed fit, else status,
+// score, rss, edf, iterations, evaluations, step failure, rank, log(sp),
+// decision margin and the timing fields.
+__device__ __attribute__((always_inline)) void magic_finish(
+    const MagicLds& L, const MagicScratch& S, const double* D, const DesignLayout& DL, int n,
+    int q, int m, double* ro, double* out, const double* finfo, int converged, int iter,
+    long long t_start, long long t_final) {
+  const int tid = threadIdx.x;
+  const int* ic = L.ictl;
+  const double* st = L.dctl;
+  const int nr = L.ictl[0];
+  const bool qr = L.ictl[13] != 0;
+  if (qr && !L.ictl[15]) upper_tri_inverse_fast(L.aug, q + L.ictl[1], q, L.V, L.wk);
+  double* theta = S.theta;
+  for (int r = tid; r < q; r += blockDim.x) {
+    double v = 0.0;
+    if (qr) {
+      for (int c = r; c < q; ++c) v += L.V[(size_t)c * q + r] * L.y1[c];  // R2^-1 y1
+    } else {
+      for (int c = 0; c < nr; ++c)
+        v += L.V[(size_t)L.ord[c] * q + r] * (L.y1[c] / L.d[L.ord[c]]);
+    }
+    theta[r] = v;
+  }
+  __syncthreads();
+  double* beta = S.beta;
+  for (int i = tid; i < q; i += blockDim.x) beta[(int)D[DL.piv + i]] = theta[i];
+  __syncthreads();
+  const double* X = D + DL.X;
+  for (int r = tid; r < n; r += blockDim.x) {
+    double f = 0.0;
+    for (int c = 0; c < q; ++c) f += X[(size_t)c * n + r] * beta[c];
+    ro[r] = L.yorig[r] - f;
+  }
+  if (finfo) {
+    for (int i = tid; i < kMagicInfo; i += blockDim.x) out[i] = finfo[i];
+  } else if (tid == 0) {
+    out[0] = converged ? 0.0 : -7.0;
+    out[1] = L.dctl[2];
+    out[2] = L.dctl[0];
+    out[3] = L.dctl[1];
+    out[4] = (double)iter;
+    out[5] = (double)ic[9];
+    out[6] = (double)ic[7];
+    out[7] = (double)nr;
+    for (int i = 0; i < m && i < kMagicInfo - 10; ++i) out[8 + i] = L.lsp[i];
+    out[kMagicInfo - 2] = st[5];
+    out[kMagicInfo - 3] = (double)(wall_clock64() - t_start);  // wall-clock ticks
+    out[kMagicInfo - 4] = L.dctl[26];  // ticks in the pivoted Cholesky
+    out[kMagicInfo - 5] = L.dctl[27];  // ticks in the SVD
+    out[kMagicInfo - 6] = L.dctl[29];  // ticks in the derivatives
+    out[kMagicInfo - 7] = (double)L.ictl[28];  // Jacobi sweeps
+    if (8 + m <= 14) {  // evaluation sub-phases where the log(sp) slots leave room
+      out[14] = L.dctl[8];
+      out[15] = L.dctl[9];
+      out[16] = L.dctl[10];
+      out[17] = L.dctl[11];
+      out[18] = L.dctl[12];
+      out[19] = L.dctl[13];
+    }
+    if (8 + m <= 20) {  // phase ticks where the log(sp) slots leave room
+      out[20] = L.dctl[20];  // all objective evaluations
+      out[21] = L.dctl[21];  // all derivative evaluations
+      out[22] = L.dctl[22];  // Q'y prologue
+      out[23] = (double)(wall_clock64() - t_final);  // final fit and residuals
+    }
+  }
+  __syncthreads();
+}
+
 template <bool kLds>
 __global__ __launch_bounds__(kMagicThreads, 2) void magic_kernel(
     const MagicItem* __restrict__ items, int count, int n, int q, int m, int rk,
@@ -1766,30 +2257,7 @@ __global__ __launch_bounds__(kMagicThreads, 2) void magic_kernel(
     const double* finfo =
         it.pad > 0 && fixed_info ? fixed_info + (size_t)(it.pad - 1) * kMagicInfo : nullptr;
     const long long t_start = wall_clock64();
-    for (int r = tid; r < n; r += blockDim.x) { L.y[r] = y[r]; L.yorig[r] = y[r]; }
-    if (tid == 0) {
-      L.dctl[26] = L.dctl[27] = L.dctl[29] = 0.0;
-      L.dctl[20] = L.dctl[21] = L.dctl[22] = L.dctl[23] = 0.0;
-      L.dctl[8] = L.dctl[9] = L.dctl[10] = L.dctl[11] = 0.0;
-      L.dctl[12] = L.dctl[13] = 0.0;
-      L.ictl[28] = 0;
-    }
-    __syncthreads();
-    // y0 = Q' y (reflectors in order)
-    for (int i = 0; i < q; ++i) {
-      const double* v = D + DL.QR + (size_t)i * n;
-      const double tau = D[DL.tau + i];
-      double part = 0.0;
-      for (int r = i + tid; r < n; r += blockDim.x) part += (r == i ? 1.0 : v[r]) * L.y[r];
-      const double s = block_sum(part, L.red) * tau;
-      for (int r = i + tid; r < n; r += blockDim.x) L.y[r] -= s * (r == i ? 1.0 : v[r]);
-      __syncthreads();
-    }
-    double part = 0.0;
-    for (int r = tid; r < n; r += blockDim.x) part += L.yorig[r] * L.yorig[r];
-    const double ynorm2 = block_sum(part, L.red);
-    if (tid < m) L.lsp[tid] = finfo ? finfo[8 + tid] : D[DL.lsp0 + tid];
-    for (int i = tid; i < 2 + 3 * m; i += blockDim.x) L.nrm[i] = D[DL.norms + i];
+    const double ynorm2 = magic_prologue(L, D, DL, y, n, q, m, finfo);
     if (tid == 0) L.dctl[22] = (double)(wall_clock64() - t_start);  // Q'y prologue
     __syncthreads();
     const int max_half = m == 1 ? ctl.max_half_single : ctl.max_half_multi;
@@ -1859,52 +2327,7 @@ __global__ __launch_bounds__(kMagicThreads, 2) void magic_kernel(
           __syncthreads();
         }
       }
-      if (tid == 0) {
-        if (iter > 3) {
-          double gn = 0.0;
-          for (int i = 0; i < m; ++i) gn += L.grad[i] * L.grad[i];
-          gn = sqrt(gn);
-          const double t1 = ctl.tol * (1.0 + st[3]);
-          const double t2 = pow(ctl.tol, 1.0 / 3.0) * (1.0 + fabs(st[3]));
-          int conv = st[4] <= t1 && gn <= t2;
-          st[5] = fmin(st[5], fmin(fabs(st[4] - t1) / t1, fabs(gn - t2) / t2));
-          if (ic[6] == max_half) { conv = 1; ic[7] = 1; }
-          ic[4] = conv;
-        }
-        for (int i = 0; i < m; ++i) L.grad[i] = L.cgrad[i];
-      }
-      __syncthreads();
-      if (tid < KPC_WAVE) small_syev_wave(L.hess, m, L.syw, L.syZ, L.syA);
-      __syncthreads();
-      if (tid == 0) {
-        double* w = L.syw;
-        double* Z = L.syZ;
-        int sd = 0;
-        for (int i = 0; i < m; ++i) {
-          if (w[i] < 0.0) sd = 1;
-          st[5] = fmin(st[5], fabs(w[i]) / st[3]);
-        }
-        ic[3] = sd;
-        if (!sd) {
-          for (int r = 0; r < m; ++r) {
-            double v = 0.0;
-            for (int c = 0; c < m; ++c) {
-              double pc = 0.0;
-              for (int k = 0; k < m; ++k) pc += Z[c * m + k] * L.grad[k];
-              v += Z[c * m + r] * (pc / w[c]);
-            }
-            L.nstep[r] = -v;
-          }
-          double mx = fabs(L.nstep[0]);
-          for (int i = 1; i < m; ++i) mx = fmax(mx, fabs(L.nstep[i]));
-          if (mx > ctl.max_newton_step)
-            for (int i = 0; i < m; ++i) L.nstep[i] *= ctl.max_newton_step / mx;
-        }
-        double mg = fabs(L.grad[0]);
-        for (int i = 1; i < m; ++i) mg = fmax(mg, fabs(L.grad[i]));
-        for (int i = 0; i < m; ++i) L.sstep[i] = mg == 0.0 ? 0.0 : -L.grad[i] / mg;
-      }
-      __syncthreads();
+      magic_conv_and_step(L, ctl, m, iter, max_half);
       if (ic[4]) break;
     }
     // infinity probes per coordinate
@@ -1933,73 +2356,259 @@ __global__ __launch_bounds__(kMagicThreads, 2) void magic_kernel(
       }
     }
     }  // optimiser (skipped for a cached fit)
-    const int* ic = L.ictl;
-    const double* st = L.dctl;
-    const int converged = ic[4];
+    const int converged = L.ictl[4];
     // final fit at the selected log(sp)
     const long long t_final = wall_clock64();
     magic_eval(L, S, D, DL, n, q, m, rk, L.lsp, ynorm2, ctl.rank_tol);
-    {
-      const int nr = L.ictl[0];
-      const bool qr = L.ictl[13] != 0;
-      if (qr && !L.ictl[15]) upper_tri_inverse_fast(L.aug, q + L.ictl[1], q, L.V, L.wk);
-      double* theta = S.theta;
-      for (int r = tid; r < q; r += blockDim.x) {
-        double v = 0.0;
-        if (qr) {
-          for (int c = r; c < q; ++c) v += L.V[(size_t)c * q + r] * L.y1[c];  // R2^-1 y1
-        } else {
-          for (int c = 0; c < nr; ++c)
-            v += L.V[(size_t)L.ord[c] * q + r] * (L.y1[c] / L.d[L.ord[c]]);
-        }
-        theta[r] = v;
-      }
-      __syncthreads();
-      double* beta = S.beta;
-      for (int i = tid; i < q; i += blockDim.x) beta[(int)D[DL.piv + i]] = theta[i];
-      __syncthreads();
-      double* ro = resid + (size_t)it.resid_slot * n;
-      const double* X = D + DL.X;
-      for (int r = tid; r < n; r += blockDim.x) {
-        double f = 0.0;
-        for (int c = 0; c < q; ++c) f += X[(size_t)c * n + r] * beta[c];
-        ro[r] = L.yorig[r] - f;
-      }
-      if (finfo) {
-        for (int i = tid; i < kMagicInfo; i += blockDim.x) out[i] = finfo[i];
-      } else if (tid == 0) {
-        out[0] = converged ? 0.0 : -7.0;
-        out[1] = L.dctl[2];
-        out[2] = L.dctl[0];
-        out[3] = L.dctl[1];
-        out[4] = (double)iter;
-        out[5] = (double)ic[9];
-        out[6] = (double)ic[7];
-        out[7] = (double)nr;
-        for (int i = 0; i < m && i < kMagicInfo - 10; ++i) out[8 + i] = L.lsp[i];
-        out[kMagicInfo - 2] = st[5];
-        out[kMagicInfo - 3] = (double)(wall_clock64() - t_start);  // wall-clock ticks
-        out[kMagicInfo - 4] = L.dctl[26];  // ticks in the pivoted Cholesky
-        out[kMagicInfo - 5] = L.dctl[27];  // ticks in the SVD
-        out[kMagicInfo - 6] = L.dctl[29];  // ticks in the derivatives
-        out[kMagicInfo - 7] = (double)L.ictl[28];  // Jacobi sweeps
-        if (8 + m <= 14) {  // evaluation sub-phases where the log(sp) slots leave room
-          out[14] = L.dctl[8];
-          out[15] = L.dctl[9];
-          out[16] = L.dctl[10];
-          out[17] = L.dctl[11];
-          out[18] = L.dctl[12];
-          out[19] = L.dctl[13];
-        }
-        if (8 + m <= 20) {  // phase ticks where the log(sp) slots leave room
-          out[20] = L.dctl[20];  // all objective evaluations
-          out[21] = L.dctl[21];  // all derivative evaluations
-          out[22] = L.dctl[22];  // Q'y prologue
-          out[23] = (double)(wall_clock64() - t_final);  // final fit and residuals
-        }
-      }
-      __syncthreads();
+    magic_finish(L, S, D, DL, n, q, m, resid + (size_t)it.resid_slot * n, out, finfo, converged,
+                 iter, t_start, t_final);
+  }
+}
+
+// ---------------------------------------------------------------- slots
+// magic for additive designs with m = K rank-8 penalty blocks (q = 1 + 9 K):
+// the optimiser of magic_kernel with every score routed through one site.
+// The waves of the workgroup score up to nslot points of the current
+// sequence at once -- one iteration's step-halving trials, one coordinate's
+// infinity probes, or a single point -- on the one-wave certified-QR path
+// (wave_eval_additive, the arithmetic of magic_eval's QR path); a point not
+// certified for that path is scored by magic_eval on the whole workgroup.
+// The points of a sequence are fixed before any is scored (try t uses the
+// step the serial loop reaches after t - 1 failures; probe p the point after
+// p - 1 accepted probes) and are consumed strictly in order by the serial
+// accept / count / margin rules, so the trajectory, the evaluation count and
+// every output equal the one-at-a-time optimiser's.
+enum : int { kPhFirst = 0, kPhHalve = 1, kPhProbe = 2, kPhRecomp = 3, kPhFinal = 4, kPhDone = 5 };
+// Control words in ictl beyond magic_kernel's: [16] phase, [17] pending work
+// (1: derivatives at lsp, 2: convergence test and next step), [18] points in
+// the sequence, [19] next point, [20] probe coordinate, [22] point for the
+// workgroup path (-1: none).
+
+// Probe sequence of coordinate c (the final fit once every coordinate is done).
+__device__ __forceinline__ void slot_start_probe(const MagicLds& L, const MagicControl& ctl, int m,
+                                                 int c) {
+  int* ic = L.ictl;
+  if (c >= m || ctl.max_probes <= 0) {
+    for (int i = 0; i < m; ++i) L.trials[i] = L.lsp[i];
+    ic[16] = kPhFinal;
+    ic[18] = 1;
+    ic[19] = 0;
+    return;
+  }
+  const double dir = L.grad[c] < 0.0 ? 1.0 : -1.0;
+  for (int pr = 0; pr < ctl.max_probes; ++pr) {
+    double* t = L.trials + (size_t)pr * m;
+    for (int i = 0; i < m; ++i) t[i] = L.lsp[i];
+    t[c] = (pr == 0 ? L.lsp[c] : L.trials[(size_t)(pr - 1) * m + c]) + dir * ctl.probe_step;
+  }
+  ic[16] = kPhProbe;
+  ic[18] = ctl.max_probes;
+  ic[19] = 0;
+  ic[20] = c;
+}
+
+// Step-halving trials of the next iteration in try order.
+__device__ __forceinline__ void slot_start_halving(const MagicLds& L, int m, int max_half) {
+  int* ic = L.ictl;
+  double* sv = L.step;
+  int sd = ic[3];
+  for (int i = 0; i < m; ++i) sv[i] = sd ? L.sstep[i] : L.nstep[i];
+  for (int t = 1; t <= max_half; ++t) {
+    if (t == 4 && !sd) {
+      sd = 1;
+      for (int i = 0; i < m; ++i) sv[i] = L.sstep[i];
     }
+    for (int i = 0; i < m; ++i) L.trials[(size_t)(t - 1) * m + i] = L.lsp[i] + sv[i];
+    for (int i = 0; i < m; ++i) sv[i] /= 2.0;
+    if (t == max_half - 1)
+      for (int i = 0; i < m; ++i) sv[i] = 0.0;
+  }
+  ic[16] = kPhHalve;
+  ic[18] = max_half;
+  ic[19] = 0;
+}
+
+// Slot 0 scored its point into the workgroup's buffers: its results become
+// the workgroup's last evaluation (as magic_eval would leave them).
+__device__ __forceinline__ void slot_load0(const MagicLds& L, int q, int mr, const double* r) {
+  L.ictl[0] = q;
+  L.ictl[1] = mr;
+  L.ictl[13] = 1;
+  L.ictl[15] = 0;
+  L.ictl[2] = 0;
+  L.dctl[0] = r[1];
+  L.dctl[1] = r[2];
+  L.dctl[2] = r[0];
+  L.dctl[6] = r[3];
+}
+
+// Lane 0: the score of point k of the current sequence; in_l when the point's
+// factors are in the workgroup's buffers.  False when the sequence ends here.
+__device__ __forceinline__ bool slot_consume(const MagicLds& L, const MagicControl& ctl, int m,
+                                             int max_half, int k, double score, double margin,
+                                             bool in_l) {
+  int* ic = L.ictl;
+  double* st = L.dctl;  // st[3] min score, st[4] score reduction, st[5] margin
+  switch (ic[16]) {
+    case kPhFirst:
+      st[3] = score;
+      st[4] = 1e10;
+      ic[3] = ic[4] = ic[7] = 0;
+      ic[9] = 1;
+      st[5] = margin;
+      for (int i = 0; i < m; ++i) L.grad[i] = L.nstep[i] = L.sstep[i] = 0.0;
+      ic[5] = 1;
+      ic[6] = 0;
+      ic[17] = 3;
+      return false;
+    case kPhHalve: {
+      const int t = k + 1;
+      ic[6] = t;
+      ic[9] += 1;
+      ic[19] = t;
+      st[5] = fmin(st[5], fmin(margin, fabs(score - st[3]) / st[3]));
+      if (score < st[3]) {
+        st[4] = st[3] - score;
+        st[3] = score;
+        for (int i = 0; i < m; ++i) L.lsp[i] = L.trials[(size_t)k * m + i];
+        if (in_l) {
+          ic[17] = 3;
+        } else {
+          // re-score the accepted point into the workgroup's buffers first
+          for (int i = 0; i < m; ++i) L.trials[i] = L.lsp[i];
+          ic[16] = kPhRecomp;
+          ic[18] = 1;
+          ic[19] = 0;
+        }
+        return false;
+      }
+      if (t == max_half) {
+        ic[17] = 2;
+        return false;
+      }
+      return true;
+    }
+    case kPhRecomp:
+      ic[17] = 3;
+      return false;
+    case kPhProbe: {
+      ic[9] += 1;
+      ic[19] = k + 1;
+      const bool acc = score < st[3];
+      st[5] = fmin(st[5], fmin(margin, fabs(score - st[3]) / st[3]));
+      if (acc) {
+        for (int i = 0; i < m; ++i) L.lsp[i] = L.trials[(size_t)k * m + i];
+        st[3] = score;
+        if (k + 1 < ic[18]) return true;
+      }
+      slot_start_probe(L, ctl, m, ic[20] + 1);
+      return false;
+    }
+    default:  // kPhFinal
+      ic[16] = kPhDone;
+      return false;
+  }
+}
+
+template <int K>
+__global__ __launch_bounds__(kMagicThreads, 2) void magic_slots_kernel(
+    const MagicItem* __restrict__ items, int count, int n, const double* __restrict__ data,
+    const double* __restrict__ design, double* __restrict__ resid, double* __restrict__ info,
+    double* __restrict__ scratch, long long scratch_stride, MagicControl ctl,
+    const double* __restrict__ fixed_info, int nslot) {
+  constexpr int m = K, rk = 8, q = 1 + 9 * K, mr = 8 * K;
+  extern __shared__ double lds_dyn[];
+  const int tid = threadIdx.x;
+  const int wid = tid / KPC_WAVE;
+  double* gs = scratch + (size_t)blockIdx.x * scratch_stride;
+  const MagicLds L = carve_magic(lds_dyn, n, q, m, true, gs, true, nslot);
+  const MagicScratch S = carve_magic_scratch(gs, q, m, rk);
+  const DesignLayout DL = design_layout(n, q, m, rk);
+  const int max_half = ctl.max_half_multi;
+  int* ic = L.ictl;
+  for (int item = blockIdx.x; item < count; item += gridDim.x) {
+    const MagicItem it = items[item];
+    const double* D = design + (size_t)it.design_slot * DL.stride;
+    const double* y = data + (size_t)it.target * n;
+    double* out = info + (size_t)it.resid_slot * kMagicInfo;
+    const double* finfo =
+        it.pad > 0 && fixed_info ? fixed_info + (size_t)(it.pad - 1) * kMagicInfo : nullptr;
+    const long long t_start = wall_clock64();
+    const double ynorm2 = magic_prologue(L, D, DL, y, n, q, m, finfo);
+    // the fit's constant inputs of the one-wave path, staged once
+    for (int e = tid; e < q * q; e += blockDim.x) L.Rst[e] = D[DL.R + e];
+    for (int e = tid; e < mr * q; e += blockDim.x) L.rSst[e] = D[DL.rS + e];
+    for (int r = tid; r < q; r += blockDim.x) L.yst[r] = L.y[r];
+    if (tid == 0) {
+      L.dctl[22] = (double)(wall_clock64() - t_start);  // Q'y prologue
+      for (int i = 0; i < m; ++i) L.trials[i] = L.lsp[i];
+      ic[16] = finfo ? kPhFinal : kPhFirst;
+      ic[18] = 1;
+      ic[19] = 0;
+      ic[17] = 0;
+      ic[4] = finfo ? 1 : 0;
+      ic[5] = 0;
+    }
+    __syncthreads();
+    // every round consumes a point or ends a sequence; the bound is a guard
+    const int max_rounds = 4 * (ctl.max_iterations * (max_half + 2) + m * ctl.max_probes + 8);
+    for (int round = 0; ic[16] != kPhDone && round < max_rounds; ++round) {
+      const int phase = ic[16];
+      const int base = ic[19];
+      const int cnt =
+          min(phase == kPhHalve || phase == kPhProbe ? nslot : 1, ic[18] - base);
+      const long long tr0 = wall_clock64();
+      if (wid < cnt) {
+        double* Q1 = wid == 0 ? L.Qm : wid == 1 ? L.X4 : wid == 2 ? L.V : L.sQ3;
+        wave_eval_additive<K>(L.Rst, L.rSst, L.yst, L.nrm, L.trials + (size_t)(base + wid) * m,
+                              n, ynorm2, ctl.rank_tol, wid == 0 ? L.aug : nullptr, q + mr, Q1,
+                              wid == 0 ? q + mr : q, wid == 0 ? L.y1 : L.sy1 + (size_t)wid * q,
+                              L.sres + 8 * wid, wid == 0 ? L.dctl + 8 : nullptr);
+      }
+      __syncthreads();
+      if (tid == 0) {
+        L.dctl[20] += (double)(wall_clock64() - tr0);
+        ic[17] = 0;
+        ic[22] = -1;
+        for (int s = 0; s < cnt; ++s) {
+          const double* r = L.sres + 8 * s;
+          if (r[4] == 0.0) {
+            ic[22] = base + s;
+            break;
+          }
+          if (s == 0) slot_load0(L, q, mr, r);
+          if (!slot_consume(L, ctl, m, max_half, base + s, r[0], r[3], s == 0)) break;
+        }
+      }
+      __syncthreads();
+      if (ic[22] >= 0) {
+        const int k = ic[22];
+        magic_eval(L, S, D, DL, n, q, m, rk, L.trials + (size_t)k * m, ynorm2, ctl.rank_tol);
+        if (tid == 0) slot_consume(L, ctl, m, max_half, k, L.dctl[2], L.dctl[6], true);
+        __syncthreads();
+      }
+      const int work = ic[17];
+      if (work & 1) magic_derivs(L, S, D, DL, n, q, m, rk, L.lsp);
+      if (work & 2) {
+        magic_conv_and_step(L, ctl, m, ic[5], max_half);
+        if (tid == 0) {
+          if (ic[4]) {
+            slot_start_probe(L, ctl, m, 0);
+          } else if (ic[5] + 1 > ctl.max_iterations) {
+            ic[5] += 1;
+            slot_start_probe(L, ctl, m, m);  // out of iterations: the final fit
+          } else {
+            ic[5] += 1;
+            ic[6] = 0;
+            slot_start_halving(L, m, max_half);
+          }
+        }
+        __syncthreads();
+      }
+    }
+    const long long t_final = wall_clock64();
+    magic_finish(L, S, D, DL, n, q, m, resid + (size_t)it.resid_slot * n, out, finfo, ic[4], ic[5],
+                 t_start, t_final);
   }
 }
 
@@ -2027,25 +2636,59 @@ hipError_t launch_geometry(const GeometryItem* items, int count, int n, int q, i
   return hipGetLastError();
 }
 
+// Trial slots per launch: KPC_MAGIC_SLOTS=1 scores step-halving trials one
+// at a time on the workgroup path (outputs are the same either way).
+static int magic_slot_limit() {
+  static const int lim = [] {
+    const char* e = getenv("KPC_MAGIC_SLOTS");
+    const int v = e ? atoi(e) : 4;
+    return v < 1 ? 1 : v > 4 ? 4 : v;
+  }();
+  return lim;
+}
+
 hipError_t launch_magic(const MagicItem* items, int count, int n, int q, int m, int rk,
                         const double* data, const double* design, double* resid, double* info,
                         double* scratch, int grid, const MagicControl& ctl, hipStream_t stream,
                         const double* fixed_info) {
   if (count <= 0) return hipSuccess;
   const bool in_lds = magic_aug_fits_lds(n, q, m);
-  const size_t lds = magic_lds_bytes(n, q, m, in_lds);
   const bool all_lds = in_lds && magic_deriv_in_lds(q, m);
+  const int nslot = all_lds && ctl.max_half_multi >= 1 && ctl.max_half_multi <= kSpecTries &&
+                            ctl.max_probes <= kSpecTries
+                        ? std::min(magic_spec_slots(n, q, m, rk), magic_slot_limit())
+                        : 1;
+  const long long stride = magic_scratch_doubles(q, m, rk, n);
+  if (nslot > 1) {
+    const size_t lds = magic_lds_doubles(n, q, m, true, nslot) * sizeof(double);
+    const void* fn = m == 3 ? (const void*)magic_slots_kernel<3>
+                   : m == 4 ? (const void*)magic_slots_kernel<4>
+                            : (const void*)magic_slots_kernel<5>;
+    hipError_t err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (err != hipSuccess) return err;
+    if (m == 3)
+      hipLaunchKernelGGL(magic_slots_kernel<3>, dim3(grid), dim3(256), lds, stream, items, count, n,
+                         data, design, resid, info, scratch, stride, ctl, fixed_info, nslot);
+    else if (m == 4)
+      hipLaunchKernelGGL(magic_slots_kernel<4>, dim3(grid), dim3(256), lds, stream, items, count, n,
+                         data, design, resid, info, scratch, stride, ctl, fixed_info, nslot);
+    else
+      hipLaunchKernelGGL(magic_slots_kernel<5>, dim3(grid), dim3(256), lds, stream, items, count, n,
+                         data, design, resid, info, scratch, stride, ctl, fixed_info, nslot);
+    return hipGetLastError();
+  }
+  const size_t lds = magic_lds_bytes(n, q, m, in_lds);
   const void* fn = all_lds ? (const void*)magic_kernel<true> : (const void*)magic_kernel<false>;
   hipError_t err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (err != hipSuccess) return err;
   if (all_lds)
     hipLaunchKernelGGL(magic_kernel<true>, dim3(grid), dim3(magic_threads(q)), lds, stream, items,
-                       count, n, q, m, rk, data, design, resid, info, scratch,
-                       magic_scratch_doubles(q, m, rk, n), 1, ctl, fixed_info);
+                       count, n, q, m, rk, data, design, resid, info, scratch, stride, 1, ctl,
+                       fixed_info);
   else
     hipLaunchKernelGGL(magic_kernel<false>, dim3(grid), dim3(magic_threads(q)), lds, stream, items,
-                       count, n, q, m, rk, data, design, resid, info, scratch,
-                       magic_scratch_doubles(q, m, rk, n), in_lds ? 1 : 0, ctl, fixed_info);
+                       count, n, q, m, rk, data, design, resid, info, scratch, stride,
+                       in_lds ? 1 : 0, ctl, fixed_info);
   return hipGetLastError();
 }
 
