@@ -2637,14 +2637,12 @@ hipError_t launch_geometry(const GeometryItem* items, int count, int n, int q, i
 }
 
 // Trial slots per launch: KPC_MAGIC_SLOTS=1 scores step-halving trials one
-// at a time on the workgroup path (outputs are the same either way).
+// at a time on the workgroup path (outputs are the same either way); read at
+// every launch.
 static int magic_slot_limit() {
-  static const int lim = [] {
-    const char* e = getenv("KPC_MAGIC_SLOTS");
-    const int v = e ? atoi(e) : 4;
-    return v < 1 ? 1 : v > 4 ? 4 : v;
-  }();
-  return lim;
+  const char* e = getenv("KPC_MAGIC_SLOTS");
+  const int v = e ? atoi(e) : 4;
+  return v < 1 ? 1 : v > 4 ? 4 : v;
 }
 
 hipError_t launch_magic(const MagicItem* items, int count, int n, int q, int m, int rk,
