@@ -70,7 +70,7 @@ def pair_flops(n, k, pairs):
 def read_traffic(kernel, workload_tag):
     """Calibrated HBM bytes per launch of `kernel` from the committed counter
     profile, when that profile was taken on this workload (else None)."""
-    path = os.path.join(ROOT, "profiles", "traffic_%s.json" % kernel)
+    path = os.path.join(ROOT, "profiles", "traffic_%s.json" % kernel.replace("+", "_"))
     if not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -257,7 +257,7 @@ def bench_regression(args):
                    "setup_s_per_step": round(setup_s / args.steps, 4),
                    "gcv_s_per_step": round(ktime, 4)},
         "plain_gpu": plain,
-        "roofline": {"bound": "mfma", "kernel": "magic_kernel", "achieved": round(achieved, 4),
+        "roofline": {"bound": "mfma", "kernel": "magic_kernel+magic_slots_kernel", "achieved": round(achieved, 4),
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_PEAK_TFLOPS, 6),
                      "traffic": None, "launches_per_step": launches / args.steps,
@@ -350,8 +350,11 @@ def main():
 
     # ---- roofline of the dominant kernel (HIP events inside the library)
     kernels = {
-        "gcv": ("magic_kernel", sum(gcv_flops(n, l, s["level_fits"][l], s["level_evaluations"][l])
-                                    for l in range(res["levels"]))),
+        # the GCV stage: magic_kernel (thin-plate designs, |S| <= 2) and
+        # magic_slots_kernel<m> (additive designs, |S| = 3..5)
+        "gcv": ("magic_kernel+magic_slots_kernel",
+                sum(gcv_flops(n, l, s["level_fits"][l], s["level_evaluations"][l])
+                    for l in range(res["levels"]))),
     }
     if args.method == "dcc.gamma":
         kernels["dcov_component"] = ("dcov_lanczos_kernel",
@@ -378,9 +381,12 @@ def main():
     achieved = flops_per_run / ktime / 1e12
     roofline = {
         # FP64 compute roof (vector and matrix FP64 share the 78.6 TFLOP/s
-        # peak); the GCV kernel's limiter is the latency of its sequential
-        # optimiser steps, see the occupancy / stall counters in profiles/
+        # peak; the path's HBM bytes are far below the HBM roof).  The
+        # kernels' limiter is the latency of their sequential steps: the
+        # occupancy / stall counters are in profiles/counters_sem351x48.md
         "bound": "mfma",
+        "limiter": "latency (sequential optimiser and Lanczos steps; waves wait in 58-69% of "
+                   "their resident cycles)",
         "kernel": kname,
         "achieved": round(achieved, 4),
         "peak": FP64_PEAK_TFLOPS,
