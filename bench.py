@@ -350,8 +350,8 @@ def main():
 
     # ---- roofline of the dominant kernel (HIP events inside the library)
     kernels = {
-        # the GCV stage: magic_kernel (thin-plate designs, |S| <= 2) and
-        # magic_slots_kernel<m> (additive designs, |S| = 3..5)
+        # the GCV stage: magic_slots_kernel<m> (additive designs, |S| = 3, 4)
+        # and magic_kernel (every other design)
         "gcv": ("magic_kernel+magic_slots_kernel",
                 sum(gcv_flops(n, l, s["level_fits"][l], s["level_evaluations"][l])
                     for l in range(res["levels"]))),

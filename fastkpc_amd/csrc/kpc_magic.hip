@@ -597,11 +597,14 @@ struct MagicLds {
 };
 
 // Trial slots: a multi-penalty additive design (rank-8 penalty blocks,
-// q = 1 + 9 m, 3 <= m <= 5) on a 256-lane workgroup whose working set is all
-// in LDS scores up to nslot step-halving trials at once, one per wave.
+// q = 1 + 9 m, m = 3 or 4) on a 256-lane workgroup whose working set is all
+// in LDS scores up to nslot step-halving trials at once, one per wave.  (At
+// m = 5 the one-wave evaluation holds 60 doubles of E and X per lane and
+// measured twice as slow as the workgroup path on fits without halving
+// tails, so m = 5 keeps magic_kernel.)
 constexpr int kSpecTries = 32;
 __host__ __device__ inline bool magic_slot_shape(int q, int m, int rk) {
-  return rk == 8 && m >= 3 && m <= 5 && q == 1 + 9 * m;
+  return rk == 8 && m >= 3 && m <= 4 && q == 1 + 9 * m;
 }
 __host__ __device__ inline long long magic_spec_doubles(int q, int m, int nslot) {
   if (nslot <= 1) return 0;
@@ -1519,18 +1522,6 @@ __device__ __attribute__((always_inline)) void wave_eval_additive(
   wave_sync();
 }
 
-__device__ __attribute__((always_inline)) void wave_eval_slot(
-    const MagicLds& L, int m, const double* lspv, int n, double ynorm2, double rank_tol,
-    double* R2, int ldr, double* Q1, int ldq, double* y1, double* res) {
-  switch (m) {
-    case 3: wave_eval_additive<3>(L.Rst, L.rSst, L.yst, L.nrm, lspv, n, ynorm2, rank_tol, R2, ldr, Q1, ldq, y1, res); break;
-    case 4: wave_eval_additive<4>(L.Rst, L.rSst, L.yst, L.nrm, lspv, n, ynorm2, rank_tol, R2, ldr, Q1, ldq, y1, res); break;
-    case 5: wave_eval_additive<5>(L.Rst, L.rSst, L.yst, L.nrm, lspv, n, ynorm2, rank_tol, R2, ldr, Q1, ldq, y1, res); break;
-    default:
-      if ((threadIdx.x & (KPC_WAVE - 1)) == 0) res[4] = 0.0;
-  }
-}
-
 // Total penalty sum_i sp_i S_i = sum_i sp_i rS_i rS_i' (q x q, column-major).
 __device__ __attribute__((always_inline)) void aggregate_penalty(const double* D, const DesignLayout& DL, int q, int m, int rk,
                                   const double* spv, double* agg) {
@@ -2366,7 +2357,8 @@ __global__ __launch_bounds__(kMagicThreads, 2) void magic_kernel(
 }
 
 // ---------------------------------------------------------------- slots
-// magic for additive designs with m = K rank-8 penalty blocks (q = 1 + 9 K):
+// magic for additive designs with m = K rank-8 penalty blocks (q = 1 + 9 K,
+// K = 3 or 4):
 // the optimiser of magic_kernel with every score routed through one site.
 // The waves of the workgroup score up to nslot points of the current
 // sequence at once -- one iteration's step-halving trials, one coordinate's
@@ -2659,19 +2651,15 @@ hipError_t launch_magic(const MagicItem* items, int count, int n, int q, int m, 
   const long long stride = magic_scratch_doubles(q, m, rk, n);
   if (nslot > 1) {
     const size_t lds = magic_lds_doubles(n, q, m, true, nslot) * sizeof(double);
-    const void* fn = m == 3 ? (const void*)magic_slots_kernel<3>
-                   : m == 4 ? (const void*)magic_slots_kernel<4>
-                            : (const void*)magic_slots_kernel<5>;
+    const void* fn =
+        m == 3 ? (const void*)magic_slots_kernel<3> : (const void*)magic_slots_kernel<4>;
     hipError_t err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (err != hipSuccess) return err;
     if (m == 3)
       hipLaunchKernelGGL(magic_slots_kernel<3>, dim3(grid), dim3(256), lds, stream, items, count, n,
                          data, design, resid, info, scratch, stride, ctl, fixed_info, nslot);
-    else if (m == 4)
-      hipLaunchKernelGGL(magic_slots_kernel<4>, dim3(grid), dim3(256), lds, stream, items, count, n,
-                         data, design, resid, info, scratch, stride, ctl, fixed_info, nslot);
     else
-      hipLaunchKernelGGL(magic_slots_kernel<5>, dim3(grid), dim3(256), lds, stream, items, count, n,
+      hipLaunchKernelGGL(magic_slots_kernel<4>, dim3(grid), dim3(256), lds, stream, items, count, n,
                          data, design, resid, info, scratch, stride, ctl, fixed_info, nslot);
     return hipGetLastError();
   }

@@ -1,4 +1,4 @@
-"""GCV trial slots on MI355X: the additive |S| = 3..5 fits whose step-halving
+"""GCV trial slots on MI355X: the additive |S| = 3, 4 fits whose step-halving
 trials and infinity probes are scored several at a time (one per wave) must
 equal the one-at-a-time optimiser bit for bit.
 
