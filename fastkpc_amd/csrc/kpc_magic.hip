@@ -715,6 +715,120 @@ long long magic_scratch_doubles(int q, int m, int rk, int n) {
 
 // One-sided Jacobi SVD of the rows x q matrix A (column-major, ld rows) with
 // round-robin parallel ordering; V accumulates the rotations.
+// Value of lane (lane ^ OFF) within aligned groups of 2 OFF lanes: DPP moves
+// for OFF < 16 (quad permutations, row shifts by 4 / 8 with a select), a
+// cross-lane read otherwise.
+template <int OFF>
+__device__ __forceinline__ double xor_lanes(double v);
+
+// Sweeps of jacobi_svd with the pair width lpp = LPP and at most R rows per
+// lane (rows <= LPP R), every pair of a round in one pass: each lane keeps its
+// rows of the pair's columns (and of V) in registers from the dot products to
+// the rotation, the round-robin partners come from conditional subtractions,
+// and the xor reductions are DPP moves.  Same operations in the same order as
+// the general sweep loop (same rows per lane, same reduction tree, same
+// rotation), so the results are bit-identical.
+template <int LPP, int R>
+__device__ __attribute__((always_inline)) void jacobi_sweeps_fixed(double* A, int rows, int q,
+                                                                   double* V, int* ictl) {
+  constexpr int kJc = 8;
+  const int tid = threadIdx.x;
+  const int qe = q + (q & 1);
+  const int npairs = qe / 2;
+  const int qm = qe - 1;
+  const int pair = tid / LPP, sub = tid % LPP;
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    if (tid == 0) ictl[8] = 0;
+    __syncthreads();
+    for (int rnd = 0; rnd < qm; ++rnd) {
+      int a = -1, b = -1;
+      if (pair < npairs) {
+        if (pair == 0) {
+          a = rnd;
+          b = qe - 1;
+        } else {
+          a = rnd + pair;
+          if (a >= qm) a -= qm;
+          b = rnd - pair;
+          if (b < 0) b += qm;
+        }
+        if (a > b) {
+          const int t = a;
+          a = b;
+          b = t;
+        }
+      }
+      const bool active = pair < npairs && b < q;
+      double xs[R], ys[R], vx[R], vy[R];
+      double al = 0.0, be = 0.0, ga = 0.0;
+      if (active) {
+        const double* ca = A + (size_t)a * rows;
+        const double* cb = A + (size_t)b * rows;
+        const double* va = V + (size_t)a * q;
+        const double* vb = V + (size_t)b * q;
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+          const int r = sub + u * LPP;
+          xs[u] = r < rows ? ca[r] : 0.0;
+          ys[u] = r < rows ? cb[r] : 0.0;
+          vx[u] = r < q ? va[r] : 0.0;
+          vy[u] = r < q ? vb[r] : 0.0;
+        }
+        // (the general loop runs chunks of kJc rows; with R <= kJc that is one
+        // chunk plus rows past the column, which add +0)
+#pragma unroll
+        for (int u = 0; u < kJc; ++u) {
+          const double x = u < R ? xs[u] : 0.0, y = u < R ? ys[u] : 0.0;
+          al += x * x;
+          be += y * y;
+          ga += x * y;
+        }
+      }
+#pragma unroll
+      for (int off = LPP >> 1; off > 0; off >>= 1) {
+        double ta, tb, tg;
+        switch (off) {
+          case 1: ta = xor_lanes<1>(al); tb = xor_lanes<1>(be); tg = xor_lanes<1>(ga); break;
+          case 2: ta = xor_lanes<2>(al); tb = xor_lanes<2>(be); tg = xor_lanes<2>(ga); break;
+          case 4: ta = xor_lanes<4>(al); tb = xor_lanes<4>(be); tg = xor_lanes<4>(ga); break;
+          case 8: ta = xor_lanes<8>(al); tb = xor_lanes<8>(be); tg = xor_lanes<8>(ga); break;
+          default: ta = xor_lanes<16>(al); tb = xor_lanes<16>(be); tg = xor_lanes<16>(ga); break;
+        }
+        al += ta;
+        be += tb;
+        ga += tg;
+      }
+      if (active && ga != 0.0 && fabs(ga) > kEps * sqrt(al * be)) {
+        const double zeta = (be - al) / (2.0 * ga);
+        const double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        const double c = 1.0 / sqrt(1.0 + t * t);
+        const double sn = c * t;
+        double* ca = A + (size_t)a * rows;
+        double* cb = A + (size_t)b * rows;
+        double* va = V + (size_t)a * q;
+        double* vb = V + (size_t)b * q;
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+          const int r = sub + u * LPP;
+          if (r < rows) {
+            ca[r] = c * xs[u] - sn * ys[u];
+            cb[r] = sn * xs[u] + c * ys[u];
+          }
+          if (r < q) {
+            va[r] = c * vx[u] - sn * vy[u];
+            vb[r] = sn * vx[u] + c * vy[u];
+          }
+        }
+        if (sub == 0) ictl[8] = 1;
+      }
+      __syncthreads();
+    }
+    if (tid == 0) ictl[28] += 1;
+    if (!ictl[8]) break;
+    __syncthreads();
+  }
+}
+
 __device__ __attribute__((always_inline)) void jacobi_svd(double* A, int rows, int q, double* V, double* d, int* ord,
                            int* ictl) {
   const int tid = threadIdx.x;
@@ -731,7 +845,11 @@ __device__ __attribute__((always_inline)) void jacobi_svd(double* A, int rows, i
   // pairs per group (more than one only for q > 2 blockDim), uniform count
   const int passes = (npairs + groups - 1) / groups;
   __syncthreads();
-  for (int sweep = 0; sweep < 60; ++sweep) {
+  const bool fixed8 = lpp == 8 && passes == 1 && rows <= 8 * 8;
+  const bool fixed16 = lpp == 16 && passes == 1 && rows <= 16 * 4;
+  if (fixed8) jacobi_sweeps_fixed<8, 8>(A, rows, q, V, ictl);
+  if (fixed16) jacobi_sweeps_fixed<16, 4>(A, rows, q, V, ictl);
+  for (int sweep = 0; sweep < 60 && !fixed8 && !fixed16; ++sweep) {
     if (tid == 0) ictl[8] = 0;
     __syncthreads();
     // round-robin schedule: pair 0 meets the fixed column qe-1, pair p>0
@@ -860,6 +978,21 @@ __device__ __forceinline__ double dpp_move(double v) {
   const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), kCtrl, 0xF, 0xF, true);
   const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), kCtrl, 0xF, 0xF, true);
   return __hiloint2double(hi, lo);
+}
+
+template <int OFF>
+__device__ __forceinline__ double xor_lanes(double v) {
+  if constexpr (OFF == 1) {
+    return dpp_move<0xB1>(v);  // quad_perm [1,0,3,2]
+  } else if constexpr (OFF == 2) {
+    return dpp_move<0x4E>(v);  // quad_perm [2,3,0,1]
+  } else if constexpr (OFF == 4 || OFF == 8) {
+    const double up = dpp_move<0x100 + OFF>(v);  // row_shl: lane + OFF
+    const double dn = dpp_move<0x110 + OFF>(v);  // row_shr: lane - OFF
+    return (threadIdx.x & OFF) ? dn : up;
+  } else {
+    return __shfl_xor(v, OFF, KPC_WAVE);
+  }
 }
 
 __device__ __forceinline__ double sum8(double v) {
