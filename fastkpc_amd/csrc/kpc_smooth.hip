@@ -16,11 +16,17 @@
 //            S <- S / (||S||_1 / ||X||_inf^2)
 //
 // Device design:
-//  * E_ij = eta(|x_i - x_j|) is never stored: every Lanczos step recomputes
-//    E q from the unique locations held in LDS (nu^2 eta evaluations, no
-//    memory traffic).  Lanes own rows, waves own (row tile, column chunk)
-//    units, four evaluations are in flight per lane, and the chunk partial
-//    sums are added in a fixed order.
+//  * E_ij = eta(|x_i - x_j|) is evaluated once per smooth, not once per
+//    Lanczos step: the upper triangle of E in 64 x 64 tiles (T(T+1)/2 tiles,
+//    T = ceil(nu / 64), zero past nu) goes to the workgroup's slab, each tile
+//    stored by diagonals (entry t * 64 + l is E[64 I + l][64 J + (l + t) mod
+//    64]) so a wave reads it with 64 coalesced loads.  A step then streams
+//    the tiles once: lane l adds E_ij q_j to its row, and an off-diagonal tile
+//    serves its transpose as well through a column accumulator that rotates
+//    one lane per diagonal (wave_rol:1), so lane l meets column (l + t) mod
+//    64 at diagonal t.  Tile partials are summed per row in tile order
+//    (deterministic).  A step costs 8 bytes and ~4 instructions per stored
+//    entry instead of ~48 FP64 instructions per entry of E.
 //  * The eta of d = 2 uses tp_log (atanh series, within 2 ulp) instead of
 //    the library logarithm, whose double-double compensation triples the
 //    cost of the product.
@@ -137,7 +143,7 @@ struct SmoothLds {
   double* al;    // Lanczos alpha
   double* be;    // Lanczos beta
   double* e2;    // beta^2 for the Sturm counts
-  double* w;     // kSmoothWork: product partials, CGS coefficients,
+  double* w;     // kSmoothWork: tile partials of E q, CGS coefficients,
                  // twisted pivots, penalty work
   double* red;   // 16
   double* dctl;  // 8
@@ -187,45 +193,95 @@ __device__ inline SmoothLds carve_smooth_lds(double* base, int np2) {
   return L;
 }
 
-// Chunk partials of E q (E implicit) into L.w[chunk * nu + row]; columns are
-// read in blocks of eight (LDS broadcasts) ahead of their evaluation.
+// Tiles of the upper triangle of E: T = ceil(nu / 64) row blocks, tile s =
+// (I, J) with I <= J in row-major order, 4096 doubles each.
+__host__ __device__ inline int smooth_tile_dim(int nu) { return (nu + 63) / 64; }
+__host__ __device__ inline int smooth_tile_count(int nu) {
+  const int T = smooth_tile_dim(nu);
+  return T * (T + 1) / 2;
+}
+__device__ __forceinline__ int tile_index(int I, int J, int T) {
+  return I * T - I * (I - 1) / 2 + (J - I);
+}
+__device__ __forceinline__ void tile_decode(int s, int T, int& I, int& J) {
+  I = 0;
+  while (s >= T - I) {
+    s -= T - I;
+    ++I;
+  }
+  J = I + s;
+}
+
+// Value of lane (lane + 1) mod 64 (DPP wave_rol:1; every lane is active).
+__device__ __forceinline__ double wave_rol1(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x134, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x134, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// E tiles into Et, stored by diagonals: Et[s][t][l] = E[64 I + l][64 J +
+// (l + t) mod 64], zero outside nu x nu.  Each wave builds whole tiles.
 template <int D>
-__device__ void tp_product(const SmoothLds& L, int nu, int nchunk, int cs, double eta) {
+__device__ void tp_build_tiles(const SmoothLds& L, int nu, double eta, double* __restrict__ Et) {
   const int lane = threadIdx.x & (KPC_WAVE - 1);
   const int wid = threadIdx.x / KPC_WAVE;
   const int nw = blockDim.x / KPC_WAVE;
-  const int ntile = (nu + KPC_WAVE - 1) / KPC_WAVE;
-  const double* __restrict__ ka = L.ka;
-  const double* __restrict__ kbv = L.kb;
-  const double* __restrict__ q = L.qv;
-  for (int u = wid; u < ntile * nchunk; u += nw) {
-    const int tile = u % ntile, chunk = u / ntile;
-    const int i = tile * KPC_WAVE + lane;
-    if (i >= nu) continue;
-    const int c0 = chunk * cs;
-    const int c1 = min(nu, c0 + cs);
-    const double xi = ka[i];
-    const double yi = D == 2 ? kbv[i] : 0.0;
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    int j = c0;
-    for (; j + 8 <= c1; j += 8) {
-      double xs[8], ys[8], qs[8];
+  const int T = smooth_tile_dim(nu), ntile = smooth_tile_count(nu);
+  for (int s = wid; s < ntile; s += nw) {
+    int I, J;
+    tile_decode(s, T, I, J);
+    const int i = 64 * I + lane;
+    const bool iv = i < nu;
+    const double xi = iv ? L.ka[i] : 0.0;
+    const double yi = iv && D == 2 ? L.kb[i] : 0.0;
+    double* out = Et + (size_t)s * 4096 + lane;
+    for (int t = 0; t < 64; ++t) {
+      const int j = 64 * J + ((lane + t) & 63);
+      double e = 0.0;
+      if (iv && j < nu) e = tp_eta<D>(xi - L.ka[j], D == 2 ? yi - L.kb[j] : 0.0, eta);
+      out[64 * t] = e;
+    }
+  }
+}
+
+// Tile partials of E q: part[s][0][l] = sum_t Et[s][t][l] q[64 J + (l+t) mod
+// 64] (rows 64 I + l), and for I < J part[s][1][l] = sum_t Et[s][t][(l - t)
+// mod 64] q[64 I + (l - t) mod 64] (rows 64 J + l, the transpose) through the
+// rotating column accumulator.  q must be zero from nu to 64 T.
+__device__ void tp_tile_product(const double* __restrict__ Et, const double* q, int nu,
+                                double* part) {
+  const int lane = threadIdx.x & (KPC_WAVE - 1);
+  const int wid = threadIdx.x / KPC_WAVE;
+  const int nw = blockDim.x / KPC_WAVE;
+  const int T = smooth_tile_dim(nu), ntile = smooth_tile_count(nu);
+  for (int s = wid; s < ntile; s += nw) {
+    int I, J;
+    tile_decode(s, T, I, J);
+    const double* __restrict__ tile = Et + (size_t)s * 4096 + lane;
+    const double* qJ = q + 64 * J;
+    const double qi = q[64 * I + lane];
+    double row = 0.0, col = 0.0;
+    // the tile's diagonals in two batches of 32 loads, all issued before the
+    // first is used
+    for (int t0 = 0; t0 < 64; t0 += 32) {
+      double e[32];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        xs[k] = ka[j + k];
-        ys[k] = D == 2 ? kbv[j + k] : 0.0;
-        qs[k] = q[j + k];
-      }
+      for (int u = 0; u < 32; ++u) e[u] = tile[64 * (t0 + u)];
+      if (I == J) {
 #pragma unroll
-      for (int k = 0; k < 8; k += 4) {
-        a0 += tp_eta<D>(xi - xs[k], yi - ys[k], eta) * qs[k];
-        a1 += tp_eta<D>(xi - xs[k + 1], yi - ys[k + 1], eta) * qs[k + 1];
-        a2 += tp_eta<D>(xi - xs[k + 2], yi - ys[k + 2], eta) * qs[k + 2];
-        a3 += tp_eta<D>(xi - xs[k + 3], yi - ys[k + 3], eta) * qs[k + 3];
+        for (int u = 0; u < 32; ++u) row = fma(e[u], qJ[(lane + t0 + u) & 63], row);
+      } else {
+#pragma unroll
+        for (int u = 0; u < 32; ++u) {
+          row = fma(e[u], qJ[(lane + t0 + u) & 63], row);
+          if (t0 + u > 0) col = wave_rol1(col);
+          col = fma(e[u], qi, col);
+        }
       }
     }
-    for (; j < c1; ++j) a0 += tp_eta<D>(xi - ka[j], D == 2 ? yi - kbv[j] : 0.0, eta) * q[j];
-    L.w[chunk * nu + i] = (a0 + a1) + (a2 + a3);
+    if (I != J) col = wave_rol1(col);
+    part[(size_t)s * 128 + lane] = row;
+    part[(size_t)s * 128 + 64 + lane] = col;
   }
 }
 
@@ -235,7 +291,7 @@ __global__ __launch_bounds__(kSmoothThreads) void smooth_setup_kernel(
     double* __restrict__ scratch, long long scratch_stride) {
   extern __shared__ double lds_raw[];
   constexpr double kEps = 2.220446049250313e-16;
-  int np2 = 1;
+  int np2 = 64;  // at least one tile row of q (zero past nu)
   while (np2 < n) np2 <<= 1;
   SmoothLds L = carve_smooth_lds(lds_raw, np2);
   const int tid = threadIdx.x;
@@ -324,7 +380,20 @@ __global__ __launch_bounds__(kSmoothThreads) void smooth_setup_kernel(
     double* Zs = Qb + (size_t)nu * (nu + 1);         // kMaxBasis x nu Ritz coefficients
     double* U = Zs + (size_t)kMaxBasis * nu;         // nu x kb Ritz vectors
     double* Xs = U + (size_t)nu * kMaxBasis;         // n x kb (row-major) before absorption
+    double* Et = Xs + (size_t)n * kMaxBasis;         // E tiles (4096 each)
+    const int ntile = smooth_tile_count(nu);
+    // tile partials of E q and the Gram-Schmidt coefficients: in the LDS work
+    // area when both fit, else partials in the slab after the tiles
+    const bool part_lds = (long long)ntile * 128 + nu <= kSmoothWork;
+    double* tpart = part_lds ? L.w : Et + (size_t)ntile * 4096;
+    double* hc = part_lds ? L.w + (size_t)ntile * 128 : L.w;  // Gram-Schmidt coefficients (nu)
     const double eta0 = eta_constant(order, d);
+    const double eta = d == 2 ? 0.5 * eta0 : eta0;
+    if (d == 2)
+      tp_build_tiles<2>(L, nu, eta, Et);
+    else
+      tp_build_tiles<1>(L, nu, eta, Et);
+    for (int r = nu + tid; r < 64 * smooth_tile_dim(nu); r += blockDim.x) L.qv[r] = 0.0;
     // ---- Lanczos start vector (mgcv LCG)
     if (tid == 0) {
       unsigned long st = 1;
@@ -350,27 +419,21 @@ __global__ __launch_bounds__(kSmoothThreads) void smooth_setup_kernel(
     int tenth = (int)floor(nu / 10.0);
     if (tenth < 1) tenth = 1;
     if (tenth < check) check = tenth;
-    // product decomposition: (row tile, column chunk) units spread evenly;
-    // the partials and the Gram-Schmidt coefficients share the work area
-    const int ntile = (nu + KPC_WAVE - 1) / KPC_WAVE;
-    int nchunk = 1;
-    while (nchunk < 8 && (ntile * nchunk) % nw != 0 && (2 * nchunk + 1) * nu <= kSmoothWork)
-      nchunk *= 2;
-    const int cs = ((nu + nchunk - 1) / nchunk + 7) & ~7;
-    double* hc = L.w + (size_t)nchunk * nu;  // Gram-Schmidt coefficients (nu)
-    const double eta = d == 2 ? 0.5 * eta0 : eta0;
+    const int T = smooth_tile_dim(nu);
     int upper = kb, lower = 0, completed = nu;
     for (int it = 0; it < nu; ++it) {
-      if (d == 2)
-        tp_product<2>(L, nu, nchunk, cs, eta);
-      else
-        tp_product<1>(L, nu, nchunk, cs, eta);
+      __syncthreads();  // q complete (and, at it = 0, the tiles)
+      tp_tile_product(Et, L.qv, nu, tpart);
       __syncthreads();
-      // z = E q_it (sum of the chunk partials), alpha, three-term recurrence
+      // z = E q_it (tile partials of row block R in tile order: the transposed
+      // parts of tiles (J, R), J < R, then the row parts of tiles (R, J)),
+      // alpha, three-term recurrence
       double part = 0.0;
       for (int r = tid; r < nu; r += blockDim.x) {
+        const int R = r >> 6, l = r & 63;
         double sz = 0.0;
-        for (int c = 0; c < nchunk; ++c) sz += L.w[c * nu + r];
+        for (int J = 0; J < R; ++J) sz += tpart[(size_t)tile_index(J, R, T) * 128 + 64 + l];
+        for (int J = R; J < T; ++J) sz += tpart[(size_t)tile_index(R, J, T) * 128 + l];
         L.z[r] = sz;
         part += L.qv[r] * sz;
       }
@@ -811,16 +874,17 @@ __global__ __launch_bounds__(kSmoothThreads) void smooth_setup_kernel(
 }
 
 long long smooth_scratch_doubles(int n) {
-  // Q basis + Ritz coefficient vectors + U + X (row-major)
+  // Q basis + Ritz coefficient vectors + U + X (row-major) + E tiles + tile
+  // partials (when they exceed the LDS work area)
   long long nu = n;
   long long s = nu * (nu + 1) + (long long)kMaxBasis * nu + nu * kMaxBasis +
-                (long long)n * kMaxBasis + 64;
+                (long long)n * kMaxBasis + (long long)smooth_tile_count(n) * (4096 + 128) + 64;
   if (s < 2LL * n) s = 2LL * n;
   return (s + 31) / 32 * 32;
 }
 
 size_t smooth_lds_bytes(int n) {
-  int np2 = 1;
+  int np2 = 64;
   while (np2 < n) np2 <<= 1;
   return smooth_lds_size(np2);
 }
