@@ -590,15 +590,17 @@ struct MagicLds {
   double* rSst;    // m rk x q: penalty roots rS
   double* yst;     // q: Q'y
   double* trials;  // kSpecTries x m: one iteration's trial points in try order
-  double* sres;    // 4 x 8: per-slot {score, rss, edf, rank margin, certified}
-  double* sy1;     // 4 x q: y1 of slots 1..3
-  double* sQ3;     // q x q: Q1 of slot 3 (slots 1 and 2 use X4 and V)
+  double* sres;    // 5 x 8: per-point {score, rss, edf, rank margin, certified}
+  double* sy1;     // 4 x q: y1 of the points scored outside the workgroup's buffers
+  double* sQ3;     // q x q: Q1 of the third such point (the first two use X4 and V)
+  double* sQ4;     // q x q: Q1 of the fourth (wave 0's second point; five slots)
   int nslot;
 };
 
 // Trial slots: a multi-penalty additive design (rank-8 penalty blocks,
 // q = 1 + 9 m, m = 3 or 4) on a 256-lane workgroup whose working set is all
-// in LDS scores up to nslot step-halving trials at once, one per wave.  (At
+// in LDS scores up to nslot step-halving trials at once, one per wave, and at
+// m = 3 a fifth on wave 0 beside its first (wave_eval_additive2).  (At
 // m = 5 the one-wave evaluation holds 60 doubles of E and X per lane and
 // measured twice as slow as the workgroup path on fits without halving
 // tails, so m = 5 keeps magic_kernel.)
@@ -608,8 +610,8 @@ __host__ __device__ inline bool magic_slot_shape(int q, int m, int rk) {
 }
 __host__ __device__ inline long long magic_spec_doubles(int q, int m, int nslot) {
   if (nslot <= 1) return 0;
-  return (long long)q * q + 8LL * m * q + q + (long long)kSpecTries * m + 32 + 4LL * q +
-         (nslot == 4 ? (long long)q * q : 0);
+  return (long long)q * q + 8LL * m * q + q + (long long)kSpecTries * m + 40 + 4LL * q +
+         (nslot >= 4 ? (long long)q * q : 0) + (nslot >= 5 ? (long long)q * q : 0);
 }
 
 __host__ __device__ inline size_t magic_lds_doubles(int n, int q, int m, bool aug_in_lds,
@@ -620,12 +622,14 @@ __host__ __device__ inline size_t magic_lds_doubles(int n, int q, int m, bool au
   return s + (size_t)magic_spec_doubles(q, m, nslot);
 }
 
-// Slots for (n, q, m, rk): 4 while the workgroup's LDS stays within half a
-// CU (two fits per CU), else 3 (slots 1 and 2 reuse V and X4) within the
-// single-fit budget, else 1 (trials scored one at a time).
+// Slots for (n, q, m, rk): 5 at m = 3 and 4 otherwise while the workgroup's
+// LDS stays within half a CU (two fits per CU), else 3 (slots 1 and 2 reuse
+// V and X4) within the single-fit budget, else 1 (trials scored one at a
+// time).
 inline int magic_spec_slots(int n, int q, int m, int rk) {
   if (!magic_slot_shape(q, m, rk) || q <= 16 || !magic_deriv_in_lds(q, m)) return 1;
   const size_t base = magic_lds_doubles(n, q, m, true);
+  if (m == 3 && (base + (size_t)magic_spec_doubles(q, m, 5)) * 8 <= 80 * 1024) return 5;
   if ((base + (size_t)magic_spec_doubles(q, m, 4)) * 8 <= 80 * 1024) return 4;
   if ((base + (size_t)magic_spec_doubles(q, m, 3)) * 8 <= 150 * 1024) return 3;
   return 1;
@@ -678,16 +682,18 @@ __device__ __attribute__((always_inline)) MagicLds carve_magic(double* base, int
   L.ictl = ip;
   ip += 32;
   L.nslot = nslot > 1 && aug_in_lds && deriv_in_lds ? nslot : 1;
-  L.Rst = L.rSst = L.yst = L.trials = L.sres = L.sy1 = L.sQ3 = nullptr;
+  L.Rst = L.rSst = L.yst = L.trials = L.sres = L.sy1 = L.sQ3 = L.sQ4 = nullptr;
   if (L.nslot > 1) {
     double* sp = reinterpret_cast<double*>(ip);  // 2 q + 32 ints: 8-byte aligned
     L.Rst = sp; sp += (size_t)q * q;
     L.rSst = sp; sp += (size_t)8 * m * q;
     L.yst = sp; sp += q;
     L.trials = sp; sp += (size_t)kSpecTries * m;
-    L.sres = sp; sp += 32;
+    L.sres = sp; sp += 40;
     L.sy1 = sp; sp += 4 * (size_t)q;
-    L.sQ3 = L.nslot == 4 ? sp : nullptr;
+    L.sQ3 = L.nslot >= 4 ? sp : nullptr;
+    if (L.nslot >= 4) sp += (size_t)q * q;
+    L.sQ4 = L.nslot >= 5 ? sp : nullptr;
   }
   return L;
 }
@@ -1655,6 +1661,262 @@ __device__ __attribute__((always_inline)) void wave_eval_additive(
   wave_sync();
 }
 
+// Two trial points on one wave: wave_qr_step for points a and b side by
+// side (the design's R entries are loaded once for both).  Each point's
+// operations and their order are exactly wave_qr_step's, so each point's
+// outputs equal a one-point evaluation's; the two dependency chains give the
+// wave twice the independent work per step.  R2 is written for point a only.
+template <int K, int NC, int Q, int S>
+__device__ __forceinline__ void wave_qr_step2(int j, int g, int sub, double (&E)[2][NC][K],
+                                              double (&X)[2][NC][K], double (&u)[2][K],
+                                              double (&tj)[2], const double* Rst, double* R2,
+                                              int ldr, double* Q1a, int lda, double* Q1b,
+                                              int ldb) {
+  if constexpr (S < NC) {
+    double rjc[NC], dot[2][NC];
+#pragma unroll
+    for (int i = S; i < NC; ++i) {
+      const int c = g + 8 * i;
+      rjc[i] = c < Q ? Rst[c * Q + j] : 0.0;
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        dot[p][i] = 0.0;
+        if (sub == 0) dot[p][i] = rjc[i];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < K; ++t)
+#pragma unroll
+      for (int i = S; i < NC; ++i)
+#pragma unroll
+        for (int p = 0; p < 2; ++p) dot[p][i] += u[p][t] * E[p][i][t];
+    sum8_range<S, NC>(dot[0]);
+    sum8_range<S, NC>(dot[1]);
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const bool upd = tj[p] != 0.0;
+#pragma unroll
+      for (int i = S; i < NC; ++i) {
+        const int c = g + 8 * i;
+        const double d = dot[p][i] * tj[p];
+        if (upd) {
+#pragma unroll
+          for (int t = 0; t < K; ++t) E[p][i][t] -= d * u[p][t];
+        }
+        if (p == 0 && R2 && c > j && c < Q && sub == 0)
+          R2[(size_t)c * ldr + j] = upd ? rjc[i] - d : rjc[i];
+      }
+    }
+    double dx[2][NC];
+#pragma unroll
+    for (int i = 0; i <= S; ++i) {
+      const int c = g + 8 * i;
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        dx[p][i] = 0.0;
+        if (sub == 0) dx[p][i] = c == j ? 1.0 : 0.0;
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < K; ++t)
+#pragma unroll
+      for (int i = 0; i <= S; ++i)
+#pragma unroll
+        for (int p = 0; p < 2; ++p) dx[p][i] += u[p][t] * X[p][i][t];
+    sum8_range<0, S + 1>(dx[0]);
+    sum8_range<0, S + 1>(dx[1]);
+#pragma unroll
+    for (int i = 0; i <= S; ++i) {
+      const int c = g + 8 * i;
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        const double d = dx[p][i] * tj[p];
+        if (c < Q && sub == 0) {
+          if (p == 0)
+            Q1a[(size_t)j * lda + c] = (c == j ? 1.0 : 0.0) - d;
+          else
+            Q1b[(size_t)j * ldb + c] = (c == j ? 1.0 : 0.0) - d;
+        }
+#pragma unroll
+        for (int t = 0; t < K; ++t) X[p][i][t] -= d * u[p][t];
+      }
+    }
+    if (j + 1 < Q) {
+      const int jn = j + 1, gn = jn & 7;
+      const bool next_slot = (jn >> 3) != S;  // column j + 1 opens the next slot
+      const double rdiag = Rst[jn * Q + jn];
+      double ur[2][K], beta[2];
+#pragma unroll
+      for (int t = 0; t < K; ++t)
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+          double v = E[p][S][t];
+          if constexpr (S + 1 < NC) v = next_slot ? E[p][S + 1][t] : v;
+          ur[p][t] = __shfl(v, gn * 8 + sub, KPC_WAVE);
+        }
+      wave_reflector<K>(ur[0], rdiag, u[0], tj[0], beta[0]);
+      wave_reflector<K>(ur[1], rdiag, u[1], tj[1], beta[1]);
+      if (R2 && g == 0 && sub == 0) R2[(size_t)jn * ldr + jn] = beta[0];
+    }
+  }
+}
+
+// wave_eval_additive for two points at once (lspa, lspb), each with its own
+// Q1 / y1 / res outputs (R2 and the phase ticks for point a only); each
+// point's outputs are bit-identical to a one-point evaluation of it.
+template <int K>
+__device__ __attribute__((always_inline)) void wave_eval_additive2(
+    const double* Rst, const double* rSst, const double* y0, const double* nrm,
+    const double* lspa, const double* lspb, int n, double ynorm2, double rank_tol, double* R2,
+    int ldr, double* Q1a, int lda, double* y1a, double* resa, double* tacc, double* Q1b, int ldb,
+    double* y1b, double* resb) {
+  constexpr int Q = 1 + 9 * K;
+  constexpr int NC = (Q + 7) / 8;
+  const int lane = threadIdx.x & (KPC_WAVE - 1);
+  const long long tw0 = tacc ? wall_clock64() : 0;
+  const int g = lane >> 3, sub = lane & 7;
+  const double* lspv[2] = {lspa, lspb};
+  double* res[2] = {resa, resb};
+  double lb[2];
+  bool ok[2];
+  double E[2][NC][K], X[2][NC][K];
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    double sp[K], sq[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      sp[i] = exp(lspv[p][i]);
+      sq[i] = sqrt(sp[i]);
+    }
+    double dmax = 0.0;
+#pragma unroll
+    for (int i = 0; i < K; ++i) dmax = fmax(dmax, sp[i] * nrm[2 + K + i]);
+    const double dstop = (double)Q * kEps * dmax;
+    bool stacked = true;
+#pragma unroll
+    for (int i = 0; i < K; ++i) stacked = stacked && sp[i] * nrm[2 + 2 * K + i] > 1e3 * dstop;
+    double emax = nrm[0];
+#pragma unroll
+    for (int i = 0; i < K; ++i) emax += sp[i] * nrm[2 + i];
+    lb[p] = 1.0 / sqrt(nrm[1] * emax);
+    ok[p] = stacked && isfinite(lb[p]) && lb[p] > 2.0 * rank_tol;
+    if (!ok[p] && lane == 0) res[p][4] = 0.0;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+      const int c = g + 8 * i;
+#pragma unroll
+      for (int t = 0; t < K; ++t) {
+        E[p][i][t] = c < Q ? sq[t] * rSst[(sub + 8 * t) * Q + c] : 0.0;
+        X[p][i][t] = 0.0;
+      }
+    }
+  }
+  if (!ok[0] && !ok[1]) return;
+#pragma unroll
+  for (int i = 0; i < NC; ++i) {
+    const int c = g + 8 * i;
+    if (c < Q)
+      for (int j = sub; j < c; j += 8) {
+        Q1a[(size_t)j * lda + c] = 0.0;
+        Q1b[(size_t)j * ldb + c] = 0.0;
+      }
+  }
+  double u[2][K], tj[2];
+  {
+    double ur[2][K], beta[2];
+#pragma unroll
+    for (int t = 0; t < K; ++t)
+#pragma unroll
+      for (int p = 0; p < 2; ++p) ur[p][t] = __shfl(E[p][0][t], sub, KPC_WAVE);
+    wave_reflector<K>(ur[0], Rst[0], u[0], tj[0], beta[0]);
+    wave_reflector<K>(ur[1], Rst[0], u[1], tj[1], beta[1]);
+    if (R2 && lane == 0) R2[0] = beta[0];
+  }
+  const long long tw1 = tacc ? wall_clock64() : 0;
+  for (int j = 0; j < Q; ++j) {
+    switch (j >> 3) {
+      case 0: wave_qr_step2<K, NC, Q, 0>(j, g, sub, E, X, u, tj, Rst, R2, ldr, Q1a, lda, Q1b, ldb); break;
+      case 1: wave_qr_step2<K, NC, Q, 1>(j, g, sub, E, X, u, tj, Rst, R2, ldr, Q1a, lda, Q1b, ldb); break;
+      case 2: wave_qr_step2<K, NC, Q, 2>(j, g, sub, E, X, u, tj, Rst, R2, ldr, Q1a, lda, Q1b, ldb); break;
+      case 3: wave_qr_step2<K, NC, Q, 3>(j, g, sub, E, X, u, tj, Rst, R2, ldr, Q1a, lda, Q1b, ldb); break;
+      case 4: wave_qr_step2<K, NC, Q, 4>(j, g, sub, E, X, u, tj, Rst, R2, ldr, Q1a, lda, Q1b, ldb); break;
+      default: wave_qr_step2<K, NC, Q, 5>(j, g, sub, E, X, u, tj, Rst, R2, ldr, Q1a, lda, Q1b, ldb); break;
+    }
+  }
+  wave_sync();
+  const long long tw2 = tacc ? wall_clock64() : 0;
+  double* Q1v[2] = {Q1a, Q1b};
+  const int ldv[2] = {lda, ldb};
+  double* y1v[2] = {y1a, y1b};
+  // y1 = Q1' y0, one 8-lane group per column
+#pragma unroll
+  for (int i = 0; i < NC; ++i) {
+    const int c = g + 8 * i;
+    if (c < Q) {
+      double sacc[2] = {0.0, 0.0};
+      for (int r = sub; r < Q; r += 8)
+#pragma unroll
+        for (int p = 0; p < 2; ++p) sacc[p] += Q1v[p][(size_t)c * ldv[p] + r] * y0[r];
+      sacc[0] = sum8(sacc[0]);
+      sacc[1] = sum8(sacc[1]);
+      if (sub == 0) {
+        y1a[c] = sacc[0];
+        y1b[c] = sacc[1];
+      }
+    }
+  }
+  wave_sync();
+  // y'AAy, edf, y'Ay: the 256-lane partial sums and block_sum3, wave by wave
+  double ta[2] = {0.0, 0.0}, tb[2] = {0.0, 0.0}, tc[2] = {0.0, 0.0};
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    const int vt = 64 * w + lane, cgv = vt >> 3;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      double pa = 0.0, pe = 0.0, py = 0.0;
+      for (int r = cgv; r < Q; r += 32) {
+        double v = 0.0;
+        for (int c = sub; c < Q; c += 8) {
+          const double uu = Q1v[p][(size_t)c * ldv[p] + r];
+          v += uu * y1v[p][c];
+          pe += uu * uu;
+        }
+        v = sum8(v);
+        if (sub == 0) pa += v * v;
+      }
+      if (vt < Q) py += y1v[p][vt] * y1v[p][vt];
+      pa = wave_sum(pa);
+      pe = wave_sum(pe);
+      py = wave_sum(py);
+      ta[p] += pa;
+      tb[p] += pe;
+      tc[p] += py;
+    }
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      if (!ok[p]) continue;
+      const double yAAy = ta[p], edf = tb[p], yAy = tc[p];
+      double rss = ynorm2 - 2.0 * yAy + yAAy;
+      if (rss < 0.0) rss = 0.0;
+      const double delta = (double)n - edf;
+      res[p][0] = delta > 1e-8 ? (double)n * rss / (delta * delta) : __builtin_inf();
+      res[p][1] = rss;
+      res[p][2] = edf;
+      res[p][3] = lb[p] / rank_tol - 1.0;
+      res[p][4] = 1.0;
+    }
+    if (tacc) {  // phase ticks: set-up, QR with Q1, y1 and the sums
+      const long long tw3 = wall_clock64();
+      tacc[0] += (double)(tw1 - tw0);
+      tacc[1] += (double)(tw2 - tw1);
+      tacc[2] += (double)(tw3 - tw2);
+    }
+  }
+  wave_sync();
+}
+
 // Total penalty sum_i sp_i S_i = sum_i sp_i rS_i rS_i' (q x q, column-major).
 __device__ __attribute__((always_inline)) void aggregate_penalty(const double* D, const DesignLayout& DL, int q, int m, int rk,
                                   const double* spv, double* agg) {
@@ -2128,6 +2390,76 @@ __device__ __forceinline__ bool bits_differ(double x, double y) {
   return __double_as_longlong(x) != __double_as_longlong(y);
 }
 
+// small_syev_wave for a compile-time M <= 4 with A and Z in registers: every
+// lane of the wave runs the same rotations with the same operations in the
+// same order (each rotation's column and row updates, the stop test's sums,
+// the bitwise fixed point), so w and Z are bit-identical to the LDS form
+// without its per-rotation LDS round trips and wave syncs.
+template <int M>
+__device__ __attribute__((always_inline)) void small_syev_regs(const double* Hin, double* w,
+                                                               double* Z) {
+  const int lane = threadIdx.x & (KPC_WAVE - 1);
+  double A[M * M], V[M * M];
+#pragma unroll
+  for (int i = 0; i < M * M; ++i) {
+    A[i] = Hin[i];
+    V[i] = (i % M == i / M) ? 1.0 : 0.0;
+  }
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, tot = 0.0;
+#pragma unroll
+    for (int i = 0; i < M * M; ++i) {
+      tot += A[i] * A[i];
+      if (i % M != i / M) off += A[i] * A[i];
+    }
+    if (off <= 1e-300 || off <= kEps * kEps * tot * 1e-4) break;
+    bool changed = false;
+#pragma unroll
+    for (int p = 0; p < M - 1; ++p)
+#pragma unroll
+      for (int q2 = p + 1; q2 < M; ++q2) {
+        const double apq = A[q2 * M + p];
+        if (apq == 0.0) continue;
+        const double app = A[p * M + p], aqq = A[q2 * M + q2];
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double t = copysign(1.0, theta) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+          const double akp = A[p * M + k], akq = A[q2 * M + k];
+          const double np = c * akp - s * akq, nq = s * akp + c * akq;
+          changed |= bits_differ(np, akp) || bits_differ(nq, akq);
+          A[p * M + k] = np;
+          A[q2 * M + k] = nq;
+        }
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+          const double apk = A[k * M + p], aqk = A[k * M + q2];
+          const double np = c * apk - s * aqk, nq = s * apk + c * aqk;
+          changed |= bits_differ(np, apk) || bits_differ(nq, aqk);
+          A[k * M + p] = np;
+          A[k * M + q2] = nq;
+        }
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+          const double zkp = V[p * M + k], zkq = V[q2 * M + k];
+          const double np = c * zkp - s * zkq, nq = s * zkp + c * zkq;
+          changed |= bits_differ(np, zkp) || bits_differ(nq, zkq);
+          V[p * M + k] = np;
+          V[q2 * M + k] = nq;
+        }
+      }
+    if (!changed) break;
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < M; ++i) w[i] = A[i * M + i];
+#pragma unroll
+    for (int i = 0; i < M * M; ++i) Z[i] = V[i];
+  }
+  wave_sync();
+}
+
 __device__ __attribute__((always_inline)) void small_syev_wave(const double* Hin, int m, double* w, double* Z, double* A) {
   const int lane = threadIdx.x & (KPC_WAVE - 1);
   for (int i = lane; i < m * m; i += KPC_WAVE) {
@@ -2218,7 +2550,15 @@ __device__ __attribute__((always_inline)) void magic_conv_and_step(const MagicLd
     for (int i = 0; i < m; ++i) L.grad[i] = L.cgrad[i];
   }
   __syncthreads();
-  if (tid < KPC_WAVE) small_syev_wave(L.hess, m, L.syw, L.syZ, L.syA);
+  if (tid < KPC_WAVE) {
+    switch (m) {
+      case 1: small_syev_regs<1>(L.hess, L.syw, L.syZ); break;
+      case 2: small_syev_regs<2>(L.hess, L.syw, L.syZ); break;
+      case 3: small_syev_regs<3>(L.hess, L.syw, L.syZ); break;
+      case 4: small_syev_regs<4>(L.hess, L.syw, L.syZ); break;
+      default: small_syev_wave(L.hess, m, L.syw, L.syZ, L.syA); break;
+    }
+  }
   __syncthreads();
   if (tid == 0) {
     double* w = L.syw;
@@ -2507,7 +2847,7 @@ enum : int { kPhFirst = 0, kPhHalve = 1, kPhProbe = 2, kPhRecomp = 3, kPhFinal =
 // Control words in ictl beyond magic_kernel's: [16] phase, [17] pending work
 // (1: derivatives at lsp, 2: convergence test and next step), [18] points in
 // the sequence, [19] next point, [20] probe coordinate, [22] point for the
-// workgroup path (-1: none).
+// workgroup path (-1: none), [23] the try accepted at the last iteration.
 
 // Probe sequence of coordinate c (the final fit once every coordinate is done).
 __device__ __forceinline__ void slot_start_probe(const MagicLds& L, const MagicControl& ctl, int m,
@@ -2595,6 +2935,7 @@ __device__ __forceinline__ bool slot_consume(const MagicLds& L, const MagicContr
       if (score < st[3]) {
         st[4] = st[3] - score;
         st[3] = score;
+        ic[23] = k;  // the next iteration's predicted accepted try
         for (int i = 0; i < m; ++i) L.lsp[i] = L.trials[(size_t)k * m + i];
         if (in_l) {
           ic[17] = 3;
@@ -2636,7 +2977,7 @@ __device__ __forceinline__ bool slot_consume(const MagicLds& L, const MagicContr
 }
 
 template <int K>
-__global__ __launch_bounds__(kMagicThreads, 2) void magic_slots_kernel(
+__global__ __launch_bounds__(kMagicThreads, K == 3 ? 1 : 2) void magic_slots_kernel(
     const MagicItem* __restrict__ items, int count, int n, const double* __restrict__ data,
     const double* __restrict__ design, double* __restrict__ resid, double* __restrict__ info,
     double* __restrict__ scratch, long long scratch_stride, MagicControl ctl,
@@ -2673,6 +3014,7 @@ __global__ __launch_bounds__(kMagicThreads, 2) void magic_slots_kernel(
       ic[17] = 0;
       ic[4] = finfo ? 1 : 0;
       ic[5] = 0;
+      ic[23] = 0;
     }
     __syncthreads();
     // every round consumes a point or ends a sequence; the bound is a guard
@@ -2682,13 +3024,37 @@ __global__ __launch_bounds__(kMagicThreads, 2) void magic_slots_kernel(
       const int base = ic[19];
       const int cnt =
           min(phase == kPhHalve || phase == kPhProbe ? nslot : 1, ic[18] - base);
+      // Point pi of the round is scored into the workgroup's buffers by wave 0:
+      // in a halving sequence the try accepted at the last iteration (the
+      // likeliest to be accepted again, so its derivatives need no re-score);
+      // the other points, in order, go to waves 1-3 and a fifth to wave 0
+      // beside pi.  Point s's results are at sres[8 s] whichever wave scored it.
+      const int pred = phase == kPhHalve ? ic[23] - base : 0;
+      const int pi = pred < 0 ? 0 : (pred >= cnt ? cnt - 1 : pred);
       const long long tr0 = wall_clock64();
-      if (wid < cnt) {
-        double* Q1 = wid == 0 ? L.Qm : wid == 1 ? L.X4 : wid == 2 ? L.V : L.sQ3;
-        wave_eval_additive<K>(L.Rst, L.rSst, L.yst, L.nrm, L.trials + (size_t)(base + wid) * m,
-                              n, ynorm2, ctl.rank_tol, wid == 0 ? L.aug : nullptr, q + mr, Q1,
-                              wid == 0 ? q + mr : q, wid == 0 ? L.y1 : L.sy1 + (size_t)wid * q,
-                              L.sres + 8 * wid, wid == 0 ? L.dctl + 8 : nullptr);
+      if (wid == 0) {
+        const double* ta = L.trials + (size_t)(base + pi) * m;
+        bool two = false;
+        if constexpr (K == 3) {
+          two = cnt >= 5;
+          if (two) {
+            const int sb = 3 < pi ? 3 : 4;  // the fourth point other than pi
+            wave_eval_additive2<K>(L.Rst, L.rSst, L.yst, L.nrm, ta,
+                                   L.trials + (size_t)(base + sb) * m, n, ynorm2, ctl.rank_tol,
+                                   L.aug, q + mr, L.Qm, q + mr, L.y1, L.sres + 8 * pi,
+                                   L.dctl + 8, L.sQ4, q, L.sy1, L.sres + 8 * sb);
+          }
+        }
+        if (!two)
+          wave_eval_additive<K>(L.Rst, L.rSst, L.yst, L.nrm, ta, n, ynorm2, ctl.rank_tol, L.aug,
+                                q + mr, L.Qm, q + mr, L.y1, L.sres + 8 * pi, L.dctl + 8);
+      } else if (wid < cnt) {
+        const int o = wid - 1;               // this wave's rank among the other points
+        const int sp = o < pi ? o : o + 1;   // its point
+        double* Q1 = wid == 1 ? L.X4 : wid == 2 ? L.V : L.sQ3;
+        wave_eval_additive<K>(L.Rst, L.rSst, L.yst, L.nrm, L.trials + (size_t)(base + sp) * m,
+                              n, ynorm2, ctl.rank_tol, nullptr, q + mr, Q1, q,
+                              L.sy1 + (size_t)wid * q, L.sres + 8 * sp, nullptr);
       }
       __syncthreads();
       if (tid == 0) {
@@ -2701,8 +3067,8 @@ __global__ __launch_bounds__(kMagicThreads, 2) void magic_slots_kernel(
             ic[22] = base + s;
             break;
           }
-          if (s == 0) slot_load0(L, q, mr, r);
-          if (!slot_consume(L, ctl, m, max_half, base + s, r[0], r[3], s == 0)) break;
+          if (s == pi) slot_load0(L, q, mr, r);
+          if (!slot_consume(L, ctl, m, max_half, base + s, r[0], r[3], s == pi)) break;
         }
       }
       __syncthreads();
@@ -2766,8 +3132,8 @@ hipError_t launch_geometry(const GeometryItem* items, int count, int n, int q, i
 // every launch.
 static int magic_slot_limit() {
   const char* e = getenv("KPC_MAGIC_SLOTS");
-  const int v = e ? atoi(e) : 4;
-  return v < 1 ? 1 : v > 4 ? 4 : v;
+  const int v = e ? atoi(e) : 5;
+  return v < 1 ? 1 : v > 5 ? 5 : v;
 }
 
 hipError_t launch_magic(const MagicItem* items, int count, int n, int q, int m, int rk,

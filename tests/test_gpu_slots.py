@@ -2,8 +2,11 @@
 trials and infinity probes are scored several at a time (one per wave) must
 equal the one-at-a-time optimiser bit for bit.
 
-KPC_MAGIC_SLOTS (read at every GCV launch) switches between the two:
-1 runs the serial kernel, 4 (the default) the slot kernel.  Compared on
+KPC_MAGIC_SLOTS (read at every GCV launch) switches between them: 1 runs
+the serial kernel, 4 the slot kernel with one point per wave, 5 (the
+default) adds a fifth point on wave 0 at |S| = 3 (two points interleaved on
+one wave), whose work goes to the workgroup's buffers for the try accepted
+at the previous iteration.  Compared on
 sem351x48 fits that include the slowest fits of its skeleton (123 Newton
 iterations with five tries in most of them), on cached-fit replays, and on
 a whole skeleton: residuals, every optimiser field of the fit record
@@ -81,21 +84,24 @@ def test_slot_fits_equal_serial(sem351):
         r1, i1 = api.regrxons_mixed(sem351, T, SL)
     with slots(4):
         r4, i4 = api.regrxons_mixed(sem351, T, SL)
+    with slots(5):
+        r5, i5 = api.regrxons_mixed(sem351, T, SL)
     print("fits", len(T), "evaluations", int(i1[:, 5].sum()), "max", int(i1[:, 5].max()))
     assert np.all(i1[:, 0] == 0)
     assert i1[:, 5].max() >= 500  # the 583-evaluation fit is in the set
-    assert np.array_equal(bits(r1), bits(r4))
-    for a, b in zip(record_fields(i1, SL), record_fields(i4, SL)):
-        assert np.array_equal(bits(a), bits(b))
+    for r, i in ((r4, i4), (r5, i5)):
+        assert np.array_equal(bits(r1), bits(r))
+        for a, b in zip(record_fields(i1, SL), record_fields(i, SL)):
+            assert np.array_equal(bits(a), bits(b))
 
 
 def test_slot_skeleton_equals_serial(sem351):
     api = fk()
     out = {}
-    for v in (1, 4):
+    for v in (1, 5):
         with slots(v):
             out[v] = api.fastkpc_compatible_cuda_skeleton(sem351, 0.1)
-    a, b = out[1], out[4]
+    a, b = out[1], out[5]
     assert np.array_equal(np.asarray(a["adjacency"]), np.asarray(b["adjacency"]))
     assert np.array_equal(bits(a["pMax"]), bits(b["pMax"]))
     assert a["summary"]["magic_evaluations"] == b["summary"]["magic_evaluations"]
@@ -113,14 +119,14 @@ def test_slot_cached_replay_equals_serial(sem351):
     # on the slot kernel's replay path and on the serial kernel's
     api = fk()
     res = {}
-    for v in (1, 4):
+    for v in (1, 5):
         with slots(v):
             api.cache_control("reset")
             api.fastkpc_compatible_cuda_skeleton(sem351, 0.1, options={"numCol": 35, "cache": True})
             res[v] = api.fastkpc_compatible_cuda_skeleton(sem351, 0.1,
                                                           options={"numCol": 30, "cache": True})
     api.cache_control("reset")
-    a, b = res[1], res[4]
+    a, b = res[1], res[5]
     assert a["summary"]["target_cache_hit_count"] > 0
     assert a["summary"]["target_cache_hit_count"] == b["summary"]["target_cache_hit_count"]
     assert np.array_equal(bits(a["pMax"]), bits(b["pMax"]))
