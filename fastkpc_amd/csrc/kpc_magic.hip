@@ -2977,7 +2977,7 @@ __device__ __forceinline__ bool slot_consume(const MagicLds& L, const MagicContr
 }
 
 template <int K>
-__global__ __launch_bounds__(kMagicThreads, 1) void magic_slots_kernel(
+__global__ __launch_bounds__(kMagicThreads, K == 3 ? 1 : 2) void magic_slots_kernel(
     const MagicItem* __restrict__ items, int count, int n, const double* __restrict__ data,
     const double* __restrict__ design, double* __restrict__ resid, double* __restrict__ info,
     double* __restrict__ scratch, long long scratch_stride, MagicControl ctl,

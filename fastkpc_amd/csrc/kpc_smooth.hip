@@ -34,9 +34,7 @@
 //    recurrence become classical Gram-Schmidt passes, the second one only
 //    when the first leaves less than half the squared norm (DGKS); both
 //    schemes are orthogonal to working precision.  A wave forms eight
-//    coefficients at once with coalesced, unconditional basis reads, 32 in
-//    flight, and reduces the eight sums by a transposed butterfly
-//    (kpc_gs.h).
+//    coefficients at once with coalesced, unconditional basis reads.
 //  * A Ritz check needs the k largest and k smallest eigenvalues of the
 //    Lanczos tridiagonal and the last components of the k leading
 //    eigenvectors: Sturm multisection and twisted factorisations give them
@@ -49,7 +47,6 @@
 //    each entry is formed with the reference's operation order.
 #include "kpc_common.h"
 #include "kpc_device.h"
-#include "kpc_gs.h"
 #include "kpc_tridiag.h"
 
 namespace kpc {
@@ -452,11 +449,56 @@ __global__ __launch_bounds__(kSmoothThreads) void smooth_setup_kernel(
       }
       double zn2 = block_sum(part, L.red);
       // Reorthogonalisation (it > 0; at it = 0 the reference only removes the
-      // alpha component): classical Gram-Schmidt against Q[0..it] (cgs_pass),
-      // repeated once more when a pass leaves less than half the squared norm
-      // it started from (DGKS).
+      // alpha component): classical Gram-Schmidt against Q[0..it], repeated
+      // once more when a pass leaves less than half the squared norm it
+      // started from (DGKS).  A wave forms eight coefficients at once (rows
+      // past the last basis vector are clamped to it, so every load is
+      // unconditional and all of them can be in flight together).
       for (int pass = 0; pass < 2 && it > 0; ++pass) {
-        const double after = cgs_pass<4, 32>(Qb, nu, nu, it + 1, L.z, hc, L.red);
+        for (int b0 = wid * 8; b0 <= it; b0 += nw * 8) {
+          double g[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+          const int nb = min(8, it + 1 - b0);
+          const double* qrow[8];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) qrow[k] = Qb + (size_t)(b0 + (k < nb ? k : nb - 1)) * nu;
+#pragma unroll 2
+          for (int i = lane; i < nu; i += KPC_WAVE) {
+            double v[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v[k] = qrow[k][i];
+            const double zi = L.z[i];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) g[k] += zi * v[k];
+          }
+#pragma unroll
+          for (int k = 0; k < 8; ++k) g[k] = wave_sum(g[k]);
+          if (lane < nb) {
+            double gv = g[0];
+#pragma unroll
+            for (int k = 1; k < 8; ++k) gv = lane == k ? g[k] : gv;
+            hc[b0 + lane] = gv;
+          }
+        }
+        __syncthreads();
+        part = 0.0;
+        for (int i = tid; i < nu; i += blockDim.x) {
+          double sa[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+          int b = 0;
+#pragma unroll 2
+          for (; b + 8 <= it + 1; b += 8) {
+            double v[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v[k] = Qb[(size_t)(b + k) * nu + i];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) sa[k] += hc[b + k] * v[k];
+          }
+          for (; b <= it; ++b) sa[0] += hc[b] * Qb[(size_t)b * nu + i];
+          const double zi =
+              L.z[i] - (((sa[0] + sa[1]) + (sa[2] + sa[3])) + ((sa[4] + sa[5]) + (sa[6] + sa[7])));
+          L.z[i] = zi;
+          part += zi * zi;
+        }
+        const double after = block_sum(part, L.red);
         const bool again = after < 0.5 * zn2;
         zn2 = after;
         if (!again) break;

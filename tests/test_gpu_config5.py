@@ -18,7 +18,14 @@ import pytest
 
 from conftest import GOLDEN, TIE_MARGIN, check_level_rows, rel_diff, replay_at_gpu_sp
 
-pytestmark = pytest.mark.gpu
+pytestmark = [
+    pytest.mark.gpu,
+    pytest.mark.skip(reason="off the GPU: a run of this test (dcc.perm) left the GPU with a memory "
+                            "fault after changes to the thin-plate set-up's Gram-Schmidt pass and "
+                            "X / penalty stage; those changes were taken out, but the fault's cause "
+                            "is not shown yet, so the n = 2000 skeletons stay off the GPU until it "
+                            "is"),
+]
 
 
 @pytest.mark.parametrize("method", ["dcc.perm", "hsic.perm"])
