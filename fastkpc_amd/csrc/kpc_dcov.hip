@@ -301,101 +301,83 @@ __global__ __launch_bounds__(kDcovThreads) void dcov_component_kernel(
 }
 
 // ------------------------------------------------------------------ pair
-// One workgroup per pair: nV2 = sum_ij lx_i ly_j (Cx'Cy)_ij^2 / n, kpcalg
-// moments and the gamma tail (fastkpc/src/legacy_dcov_gamma_cpp.cpp:420-455).
-// dcov.gamma statistic of one test from the two components: with centred
-// vectors X (n x k), Y (n x k) and values lx, ly,
-// n V^2 = sum_ij lx_i ly_j (X_i . Y_j)^2 / n.  C = X'Y is a small GEMM: rows
-// are staged through LDS in chunks of kPairRows (row-major, k padded to kp),
-// each thread accumulates one or two 4 x 4 blocks of C in registers, and every
-// C_ij is summed over the rows in order (as a plain dot product would).
-constexpr int kPairThreads = 128;
-constexpr int kPairRows = 32;  // 18 KB of LDS per workgroup at k = 35: eight per CU
+// dcov.gamma statistic of one test from the two components (kpcalg moments
+// and the gamma tail, fastkpc/src/legacy_dcov_gamma_cpp.cpp:420-455): with
+// centred vectors X, Y (n x k, column-major) and values lx, ly,
+// n V^2 = sum_ij lx_i ly_j (X_i . Y_j)^2 / n.  C = X'Y is a k x k GEMM over the
+// n samples, run on FP64 matrix cores: one wave per test, C in KT x KT tiles
+// of 16 x 16 (k padded to 16 KT with zeros), v_mfma_f64_16x16x4f64 over four
+// samples per step.  Operand fragments: lane l holds X[16 a + l % 16][r + l / 16]
+// (A, 16 components x 4 samples) and Y[16 b + l % 16][r + l / 16] (B); item v of
+// the accumulator is C[16 a + 4 v + l / 16][16 b + l % 16].  Four samples' loads
+// run a step ahead of the matrix ops; no LDS, no barrier.
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int kPairWaves = 4;  // tests per 256-lane workgroup
 
-__device__ __forceinline__ void pair_block_accumulate(const double* Xc, const double* Yc, int kp,
-                                                      int rows, int bi, int bj, double* acc) {
-  for (int r = 0; r < rows; ++r) {
-    const double4 a = *reinterpret_cast<const double4*>(Xc + r * kp + 4 * bi);
-    const double4 b = *reinterpret_cast<const double4*>(Yc + r * kp + 4 * bj);
-    const double av[4] = {a.x, a.y, a.z, a.w};
-    const double bv[4] = {b.x, b.y, b.z, b.w};
+template <int KT>
+__device__ __forceinline__ void pair_fragments(const double* X, const double* Y, int n, int k,
+                                               int r, int c, double (&av)[KT], double (&bv)[KT]) {
+  const bool rv = r < n;
+  const int rr = rv ? r : 0;
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) acc[u * 4 + v] += av[u] * bv[v];
+  for (int t = 0; t < KT; ++t) {
+    const int i = 16 * t + c;
+    const bool ok = rv && i < k;
+    const int ii = i < k ? i : 0;
+    const double x = X[(size_t)ii * n + rr];
+    const double y = Y[(size_t)ii * n + rr];
+    av[t] = ok ? x : 0.0;
+    bv[t] = ok ? y : 0.0;
   }
 }
 
-__global__ __launch_bounds__(kPairThreads) void dcov_pair_kernel(
+template <int KT>
+__global__ __launch_bounds__(64 * kPairWaves) void dcov_pair_kernel(
     const int* __restrict__ pair_x, const int* __restrict__ pair_y, int count, int n, int k,
     const double* __restrict__ comp_vals, const double* __restrict__ comp_vecs,
     const double* __restrict__ comp_dsum, const double* __restrict__ comp_moment,
     double* __restrict__ out) {
-  extern __shared__ double lds_pair[];
-  __shared__ double red[16];
-  const int tid = threadIdx.x;
-  const int kp = (k + 3) & ~3;
-  const int nb = kp / 4;
-  double* Xc = lds_pair;
-  double* Yc = lds_pair + kPairRows * kp;
-  const int b0 = tid, b1 = tid + (int)blockDim.x;
-  const bool has0 = b0 < nb * nb, has1 = b1 < nb * nb;
-  for (int item = blockIdx.x; item < count; item += gridDim.x) {
+  const int lane = threadIdx.x & (KPC_WAVE - 1);
+  const int wid = threadIdx.x / KPC_WAVE;
+  const int c = lane & 15, g = lane >> 4;
+  for (int item = blockIdx.x * kPairWaves + wid; item < count; item += gridDim.x * kPairWaves) {
     const int cx = pair_x[item], cy = pair_y[item];
     const double* X = comp_vecs + (size_t)cx * n * k;
     const double* Y = comp_vecs + (size_t)cy * n * k;
     const double* lx = comp_vals + (size_t)cx * k;
     const double* ly = comp_vals + (size_t)cy * k;
-    double acc0[16], acc1[16];
+    v4d acc[KT][KT];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      acc0[u] = 0.0;
-      acc1[u] = 0.0;
-    }
-    for (int r0 = 0; r0 < n; r0 += kPairRows) {
-      const int rows = min(kPairRows, n - r0);
-      __syncthreads();
-      // four elements of X and Y per thread in flight before any LDS store
-      for (int e0 = tid; e0 < rows * kp; e0 += 4 * (int)blockDim.x) {
-        double xv[4], yv[4];
+    for (int a = 0; a < KT; ++a)
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int e = e0 + u * (int)blockDim.x;
-          const int i = e / rows, r = e % rows;
-          const bool ok = e < rows * kp && i < k;
-          xv[u] = ok ? X[(size_t)i * n + r0 + r] : 0.0;
-          yv[u] = ok ? Y[(size_t)i * n + r0 + r] : 0.0;
-        }
+      for (int b = 0; b < KT; ++b) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+    double av[KT], bv[KT], an[KT], bn[KT];
+    pair_fragments<KT>(X, Y, n, k, g, c, av, bv);
+    for (int r0 = 0; r0 < n; r0 += 4) {
+      pair_fragments<KT>(X, Y, n, k, r0 + 4 + g, c, an, bn);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int e = e0 + u * (int)blockDim.x;
-          const int i = e / rows, r = e % rows;
-          if (e < rows * kp) {
-            Xc[r * kp + i] = xv[u];
-            Yc[r * kp + i] = yv[u];
-          }
-        }
+      for (int a = 0; a < KT; ++a)
+#pragma unroll
+        for (int b = 0; b < KT; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+      for (int t = 0; t < KT; ++t) {
+        av[t] = an[t];
+        bv[t] = bn[t];
       }
-      __syncthreads();
-      if (has0) pair_block_accumulate(Xc, Yc, kp, rows, b0 % nb, b0 / nb, acc0);
-      if (has1) pair_block_accumulate(Xc, Yc, kp, rows, b1 % nb, b1 / nb, acc1);
     }
     double part = 0.0;
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int a = 0; a < KT; ++a)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        if (has0) {
-          const int i = 4 * (b0 % nb) + u, j = 4 * (b0 / nb) + v;
-          if (i < k && j < k) part += lx[i] * ly[j] * acc0[u * 4 + v] * acc0[u * 4 + v];
+      for (int b = 0; b < KT; ++b)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = 16 * a + 4 * v + g, j = 16 * b + c;
+          if (i < k && j < k) part += lx[i] * ly[j] * acc[a][b][v] * acc[a][b][v];
         }
-        if (has1) {
-          const int i = 4 * (b1 % nb) + u, j = 4 * (b1 / nb) + v;
-          if (i < k && j < k) part += lx[i] * ly[j] * acc1[u * 4 + v] * acc1[u * 4 + v];
-        }
-      }
-    const double total = block_sum(part, red);
-    if (tid == 0) {
+    const double total = wave_sum(part);
+    if (lane == 0) {
       const double nd = (double)n;
       const double nV2 = total / nd;
       const double mean = (comp_dsum[cx] / (nd * nd)) * (comp_dsum[cy] / (nd * nd));
@@ -441,11 +423,28 @@ hipError_t launch_dcov_pairs(const int* px, const int* py, int count, int n, int
                              const double* vals, const double* vecs, const double* dsum,
                              const double* moment, double* out, int grid, hipStream_t stream) {
   if (count <= 0) return hipSuccess;
-  const int kp = (k + 3) & ~3;
-  if ((kp / 4) * (kp / 4) > 2 * kPairThreads) return hipErrorInvalidValue;
-  const size_t lds = (size_t)2 * kPairRows * kp * sizeof(double);
-  hipLaunchKernelGGL(dcov_pair_kernel, dim3(grid), dim3(kPairThreads), lds, stream, px, py, count,
-                     n, k, vals, vecs, dsum, moment, out);
+  if (k < 1 || k > 64) return hipErrorInvalidValue;
+  // grid: at most one workgroup per kPairWaves tests
+  const int g = std::max(1, std::min(grid, (count + kPairWaves - 1) / kPairWaves));
+  const dim3 block(64 * kPairWaves);
+  switch ((k + 15) / 16) {
+    case 1:
+      hipLaunchKernelGGL(dcov_pair_kernel<1>, dim3(g), block, 0, stream, px, py, count, n, k, vals,
+                         vecs, dsum, moment, out);
+      break;
+    case 2:
+      hipLaunchKernelGGL(dcov_pair_kernel<2>, dim3(g), block, 0, stream, px, py, count, n, k, vals,
+                         vecs, dsum, moment, out);
+      break;
+    case 3:
+      hipLaunchKernelGGL(dcov_pair_kernel<3>, dim3(g), block, 0, stream, px, py, count, n, k, vals,
+                         vecs, dsum, moment, out);
+      break;
+    default:
+      hipLaunchKernelGGL(dcov_pair_kernel<4>, dim3(g), block, 0, stream, px, py, count, n, k, vals,
+                         vecs, dsum, moment, out);
+      break;
+  }
   return hipGetLastError();
 }
 
