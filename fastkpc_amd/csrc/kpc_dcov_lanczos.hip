@@ -37,7 +37,7 @@ __host__ __device__ inline int lanczos_ivw(int n, int mmax, int k) {
   const int nl = (n + 63) & ~63;
   const int ml = (mmax + 64) & ~63;
   const int base = 4096 > 6 * mmax + 64 ? 4096 : 6 * mmax + 64;
-  const int room = (81920 - (3 * kMaxNumCol + 8) * 4) / 8 - (4 * nl + 4 * ml + 16 + 8 * kMaxNumCol);
+  const int room = (81920 - (3 * kMaxNumCol + 8) * 4) / 8 - (4 * nl + 4 * ml + 32 + 8 * kMaxNumCol);
   const int want = 2 * (2 * k + 6) * k < room ? 2 * (2 * k + 6) * k : room;
   return base > want ? base : want;
 }
@@ -95,6 +95,75 @@ __device__ inline void ritz_assemble(const double* __restrict__ Q, const double*
   }
 }
 
+// Wave sum by DPP moves: quad xor 1, quad xor 2, half-row mirror and row
+// mirror give every lane its 16-lane row total in registers, then the four
+// row totals are read (lanes 0, 16, 32, 48) and added as (r0 + r1) + (r2 + r3),
+// the same value in every lane.  All 64 lanes must be active.
+template <int kCtrl>
+__device__ __forceinline__ double lz_dpp(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), kCtrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), kCtrl, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lz_readlane(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+  v += lz_dpp<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += lz_dpp<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += lz_dpp<0x141>(v);  // row_half_mirror
+  v += lz_dpp<0x140>(v);  // row_mirror
+  return (lz_readlane(v, 0) + lz_readlane(v, 16)) + (lz_readlane(v, 32) + lz_readlane(v, 48));
+}
+
+// Inclusive prefix sums over the wave of two values at once by DPP moves:
+// row_shr 1, 2, 4, 8 within each 16-lane row (lanes past the row start read
+// 0), then row_bcast 15 (row 0's total into row 1, row 2's into row 3) and
+// row_bcast 31 (the total of rows 0-1 into rows 2 and 3).  All 64 lanes must
+// be active.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ double lz_dpp_masked(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), kCtrl, kRowMask, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), kCtrl, kRowMask, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ void wave_scan2_dpp(double& a, double& b) {
+  a += lz_dpp_masked<0x111, 0xF>(a);  // row_shr:1
+  b += lz_dpp_masked<0x111, 0xF>(b);
+  a += lz_dpp_masked<0x112, 0xF>(a);  // row_shr:2
+  b += lz_dpp_masked<0x112, 0xF>(b);
+  a += lz_dpp_masked<0x114, 0xF>(a);  // row_shr:4
+  b += lz_dpp_masked<0x114, 0xF>(b);
+  a += lz_dpp_masked<0x118, 0xF>(a);  // row_shr:8
+  b += lz_dpp_masked<0x118, 0xF>(b);
+  a += lz_dpp_masked<0x142, 0xA>(a);  // row_bcast:15 into rows 1, 3
+  b += lz_dpp_masked<0x142, 0xA>(b);
+  a += lz_dpp_masked<0x143, 0xC>(a);  // row_bcast:31 into rows 2, 3
+  b += lz_dpp_masked<0x143, 0xC>(b);
+}
+
+// block_sum with one barrier: the waves' partial sums go to one of two slot
+// sets, alternating call by call (par), so a call never overwrites slots that a
+// thread may still be reading: the call before it (other set) ends with a
+// barrier that every thread reaches only after its reads of this set two
+// calls back.  Same wave sums and the same order over the waves as
+// block_sum.  All threads make the same sequence of calls.
+__device__ __forceinline__ double block_sum_alt(double v, double* slots, int& par) {
+  const int lane = threadIdx.x & (KPC_WAVE - 1);
+  const int wid = threadIdx.x / KPC_WAVE;
+  const int nw = (blockDim.x + KPC_WAVE - 1) / KPC_WAVE;
+  double* r = slots + (par ? 8 : 0);
+  par ^= 1;
+  v = wave_sum_dpp(v);
+  if (lane == 0) r[wid] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < nw; ++w) t += r[w];
+  return t;
+}
+
 __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
     const double* __restrict__ src_base, const long long* __restrict__ src_off,
     const int* __restrict__ dst_slot, int count, int n, int k, double* __restrict__ comp_vals,
@@ -112,7 +181,8 @@ __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
   double* e2 = be + ml;                // ml
   double* cg = e2 + ml;                // ml (CGS coefficients)
   double* red = cg + ml;               // 16
-  double* cval = red + 16;             // 2 kMaxNumCol
+  double* rsum = red + 16;             // 2 x 8: alternating block-sum slots of the steps
+  double* cval = rsum + 16;            // 2 kMaxNumCol
   double* sval = cval + 2 * kMaxNumCol;   // kMaxNumCol
   double* sres = sval + kMaxNumCol;       // kMaxNumCol
   int* cidx = reinterpret_cast<int*>(sres + kMaxNumCol);  // 2 kMaxNumCol
@@ -203,6 +273,7 @@ __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
     }
     __syncthreads();
     int m_final = mmax;
+    int par = 0;  // slot set of the next block_sum_alt
     // first convergence check at 2k + 6 steps: top-k |lambda| components of
     // distance matrices converge there for n from 200 to 2000 (one check per
     // component); later checks follow the observed residual decay
@@ -227,14 +298,7 @@ __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
           sx += e.x * e.y;
         }
         double iv = sv, ix = sx;
-#pragma unroll
-        for (int off = 1; off < KPC_WAVE; off <<= 1) {
-          const double tv = __shfl_up(iv, off, KPC_WAVE), tx = __shfl_up(ix, off, KPC_WAVE);
-          if (lane >= off) {
-            iv += tv;
-            ix += tx;
-          }
-        }
+        wave_scan2_dpp(iv, ix);
         double ev = __shfl_up(iv, 1, KPC_WAVE), ex = __shfl_up(ix, 1, KPC_WAVE);
         if (lane == 0) ev = ex = 0.0;
         double* scan = ivl;  // free between convergence checks
@@ -264,7 +328,7 @@ __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
       __syncthreads();
       double part = 0.0;
       for (int i = tid; i < n; i += blockDim.x) part += rq[i].y * z[i];
-      const double aj = block_sum(part, red);
+      const double aj = block_sum_alt(part, rsum, par);
       // three-term recurrence
       const double bprev = j > 0 ? be[j - 1] : 0.0;
       const double* qp = Q + (size_t)(j > 0 ? j - 1 : 0) * n;
@@ -274,7 +338,7 @@ __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
         z[i] = v;
         part += v * v;
       }
-      double zn2 = block_sum(part, red);
+      double zn2 = block_sum_alt(part, rsum, par);
       // Classical Gram-Schmidt against Q[0..j], repeated once more when a pass
       // leaves less than half the squared norm it started from (DGKS).  A
       // wave forms eight coefficients at once (lanes over rows, coalesced;
@@ -297,7 +361,7 @@ __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
             for (int c = 0; c < 8; ++c) g[c] += zi * v[c];
           }
 #pragma unroll
-          for (int c = 0; c < 8; ++c) g[c] = wave_sum(g[c]);
+          for (int c = 0; c < 8; ++c) g[c] = wave_sum_dpp(g[c]);
           if (lane < nb) {
             double gv = g[0];
 #pragma unroll
@@ -324,7 +388,7 @@ __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
           z[i] = zi;
           part += zi * zi;
         }
-        const double after = block_sum(part, red);
+        const double after = block_sum_alt(part, rsum, par);
         const bool again = after < 0.5 * zn2;
         zn2 = after;
         if (!again) break;
@@ -518,7 +582,7 @@ int lanczos_mmax(int n) { return n < 1024 ? n : 1024; }
 size_t dcov_lanczos_lds_bytes(int n, int k, int mmax) {
   const int nl = (n + 63) & ~63;
   const int ml = (mmax + 64) & ~63;
-  return (size_t)(4 * nl + 4 * ml + 16 + 4 * kMaxNumCol + 4 * kMaxNumCol + lanczos_ivw(n, mmax, k)) *
+  return (size_t)(4 * nl + 4 * ml + 32 + 4 * kMaxNumCol + 4 * kMaxNumCol + lanczos_ivw(n, mmax, k)) *
              sizeof(double) +
          (size_t)(3 * kMaxNumCol + 8) * sizeof(int);
 }
