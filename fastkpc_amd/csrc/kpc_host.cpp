@@ -306,9 +306,29 @@ double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// Page-locked host block (hipHostMalloc) that grows on demand.  Copies to or
+// from it are asynchronous DMA; copies from pageable memory block the host
+// while the driver stages them (~15-30 us per work list, ~0.3 ms for a
+// level's fit records).  Its owner reuses it only after synchronising the
+// streams its earlier copies ran on.
+struct PinnedBuf {
+  unsigned char* ptr = nullptr;
+  size_t cap = 0;
+  void reserve(size_t bytes) {
+    if (bytes <= cap) return;
+    const size_t c = std::max(bytes + bytes / 2, (size_t)1 << 16);
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+    hip_check(hipHostMalloc((void**)&ptr, c, hipHostMallocDefault), "hipHostMalloc");
+    cap = c;
+  }
+};
+
 // HIP stream and timing events of an engine, reused by the later engines of
 // the same thread and device (creating and destroying them costs ~0.4 ms per
-// call).  They are kept until the process ends; an engine created while
+// call), and the page-locked blocks of its uploads (one per launch lane) and
+// read-backs.  They are kept until the process ends; an engine created while
 // another engine of the same thread holds them gets a set of its own.
 struct EngineHipResources {
   int device = -1;
@@ -316,6 +336,8 @@ struct EngineHipResources {
   hipStream_t stream = nullptr, stream2 = nullptr;
   hipEvent_t ev[11] = {};
   hipEvent_t ev2[6] = {};
+  PinnedBuf up[2];
+  PinnedBuf down;
 };
 thread_local std::deque<EngineHipResources>* g_engine_res = nullptr;
 
@@ -594,7 +616,6 @@ class Engine {
       std::vector<int> pair_x, pair_y;
       std::vector<int> fit_x, fit_y;  // residual slot of each task's fits (-1: raw column)
       std::vector<size_t> task_idx;
-      std::vector<double> out;
       int nsm = 0;
     };
     LaneWork W[2];
@@ -666,19 +687,46 @@ class Engine {
     if (with_pairs) stats_.physical_tests += (long long)tasks.size();
     stage_host_ += now_s() - t0;
     const double* comp_source = level_ == 0 ? d_data_ : resid_.ptr;
-    // uploads from pageable memory first, while no lane has work queued
+    // uploads: a lane's work lists are packed into its page-locked block and
+    // copied from there on the lane's stream, ahead of its launches (the
+    // previous wave's copies from the block ended at its synchronisation)
     for (int li = 0; li < nlanes; ++li) {
       const Lane L = lane(li);
       const LaneWork& w = W[li];
-      L.smooth_items->upload(w.smooth_items, L.stream);
-      L.geo_items->upload(w.geo_items, L.stream);
-      L.fit_items->upload(w.fit_items, L.stream);
-      L.fixed->upload(w.fixed, L.stream);
-      L.comp_src->upload(w.comp_src, L.stream);
-      L.comp_dst->upload(w.comp_dst, L.stream);
-      L.pair_x->upload(w.pair_x, L.stream);
-      L.pair_y->upload(w.pair_y, L.stream);
-      if (li == 0) d_seeds_.upload(seeds, stream_);
+      PinnedBuf& pb = res_->up[li];
+      size_t need = 0;
+      auto room = [&need](const auto& v) {
+        need += (v.size() * sizeof(v[0]) + 255) & ~(size_t)255;
+      };
+      room(w.smooth_items);
+      room(w.geo_items);
+      room(w.fit_items);
+      room(w.fixed);
+      room(w.comp_src);
+      room(w.comp_dst);
+      room(w.pair_x);
+      room(w.pair_y);
+      if (li == 0) room(seeds);
+      pb.reserve(need);
+      size_t off = 0;
+      auto put = [&](auto& dev, const auto& v) {
+        dev.reserve(v.size());
+        if (v.empty()) return;
+        const size_t bytes = v.size() * sizeof(v[0]);
+        std::memcpy(pb.ptr + off, v.data(), bytes);
+        hip_check(hipMemcpyAsync(dev.ptr, pb.ptr + off, bytes, hipMemcpyHostToDevice, L.stream),
+                  "hipMemcpyAsync H2D");
+        off += (bytes + 255) & ~(size_t)255;
+      };
+      put(*L.smooth_items, w.smooth_items);
+      put(*L.geo_items, w.geo_items);
+      put(*L.fit_items, w.fit_items);
+      put(*L.fixed, w.fixed);
+      put(*L.comp_src, w.comp_src);
+      put(*L.comp_dst, w.comp_dst);
+      put(*L.pair_x, w.pair_x);
+      put(*L.pair_y, w.pair_y);
+      if (li == 0) put(d_seeds_, seeds);
       L.pair_out->reserve(w.pair_x.size() * kpc::kPairOut);
     }
     for (int li = 0; li < nlanes; ++li) {
@@ -744,18 +792,47 @@ class Engine {
         hip_check(hipEventRecord(L.ev[5], L.stream), "event");
       }
     }
-    // copies into pageable memory return only when done, so they follow every
-    // lane's launches
-    for (int li = 0; li < nlanes; ++li) {
-      const Lane L = lane(li);
-      LaneWork& w = W[li];
-      if (!with_pairs) continue;
-      w.out.resize(w.pair_x.size() * kpc::kPairOut);
-      hip_check(hipMemcpyAsync(w.out.data(), L.pair_out->ptr, sizeof(double) * w.out.size(),
-                               hipMemcpyDeviceToHost, L.stream),
-                "D2H p-values");
+    // read-backs into the page-locked block, queued behind the kernels: each
+    // lane's test outputs on its own stream, then on lane 0 (after lane 1's
+    // last kernel) the level's status words; one synchronisation of the lanes
+    const std::vector<kpc::MagicItem>* fits = &W[0].fit_items;
+    const std::vector<kpc::GeometryItem>* geos = &W[0].geo_items;
+    const std::vector<int>* dst = &W[0].comp_dst;
+    std::vector<kpc::MagicItem> fits2;
+    std::vector<kpc::GeometryItem> geos2;
+    std::vector<int> dst2;
+    if (nlanes > 1) {
+      fits2 = W[0].fit_items;
+      fits2.insert(fits2.end(), W[1].fit_items.begin(), W[1].fit_items.end());
+      geos2 = W[0].geo_items;
+      geos2.insert(geos2.end(), W[1].geo_items.begin(), W[1].geo_items.end());
+      dst2 = W[0].comp_dst;
+      dst2.insert(dst2.end(), W[1].comp_dst.begin(), W[1].comp_dst.end());
+      fits = &fits2;
+      geos = &geos2;
+      dst = &dst2;
     }
-    for (int li = 0; li < nlanes; ++li)
+    const bool hsic = method_ == KPC_HSIC_GAMMA || method_ == KPC_HSIC_PERM;
+    const StatusReadBack rb = plan_status(*fits, *geos, W[0].nsm + W[1].nsm, dst, true,
+                                          with_pairs && hsic, 0);
+    size_t out_off[2] = {0, 0};
+    size_t down_end = rb.end;
+    for (int li = 0; li < nlanes && with_pairs; ++li) {
+      out_off[li] = down_end;
+      down_end += aligned256(W[li].pair_x.size() * kpc::kPairOut * sizeof(double));
+    }
+    res_->down.reserve(down_end);
+    for (int li = 0; li < nlanes && with_pairs; ++li) {
+      const Lane L = lane(li);
+      const size_t bytes = W[li].pair_x.size() * kpc::kPairOut * sizeof(double);
+      if (bytes)
+        hip_check(hipMemcpyAsync(res_->down.ptr + out_off[li], L.pair_out->ptr, bytes,
+                                 hipMemcpyDeviceToHost, L.stream),
+                  "D2H p-values");
+    }
+    if (nlanes > 1) hip_check(hipStreamWaitEvent(stream_, ev2_[5], 0), "stream wait");
+    queue_status(rb, stream_);
+    for (int li = nlanes - 1; li >= 0; --li)
       hip_check(hipStreamSynchronize(lane(li).stream), "wave synchronize");
     // stage times are summed over the lanes (they overlap in wall time)
     float ms;
@@ -777,51 +854,34 @@ class Engine {
       for (int s = 0; s < 5; ++s)
         if (launched[s]) ++launches_[s];
     }
-    std::vector<double> rinfo_h;
-    if (nlanes == 1) {
-      check_device_status(W[0].fit_items, W[0].geo_items, W[0].nsm, &rinfo_h);
-      check_component_status(W[0].comp_dst);
-    } else {
-      std::vector<kpc::MagicItem> fits(W[0].fit_items);
-      fits.insert(fits.end(), W[1].fit_items.begin(), W[1].fit_items.end());
-      std::vector<kpc::GeometryItem> geos(W[0].geo_items);
-      geos.insert(geos.end(), W[1].geo_items.begin(), W[1].geo_items.end());
-      std::vector<int> dst(W[0].comp_dst);
-      dst.insert(dst.end(), W[1].comp_dst.begin(), W[1].comp_dst.end());
-      check_device_status(fits, geos, W[0].nsm + W[1].nsm, &rinfo_h);
-      check_component_status(dst);
-    }
-    if (use_targets_ && !rinfo_h.empty())
+    const double* rinfo_h = verify_status(rb, *fits, *geos, dst);
+    if (use_targets_ && rinfo_h)
       for (int li = 0; li < nlanes; ++li)
         for (size_t i = 0; i < W[li].fit_items.size(); ++i) {
           const kpc::MagicItem& f = W[li].fit_items[i];
           if (f.pad > 0) continue;
           kpc::CachedFit cf;
-          std::memcpy(cf.record.data(), rinfo_h.data() + (size_t)f.resid_slot * kpc::kMagicInfo,
+          std::memcpy(cf.record.data(), rinfo_h + (size_t)f.resid_slot * kpc::kMagicInfo,
                       sizeof(double) * kpc::kMagicInfo);
           if (m_ <= kpc::kMagicInfo - 10)  // the record holds every log sp
             kpc::target_cache().insert(target_key(W[li].fit_mask[i], f.target), cf);
         }
     if (!with_pairs) return;
-    std::vector<double> cmargin_h;
-    if (method_ == KPC_HSIC_GAMMA || method_ == KPC_HSIC_PERM) {
-      cmargin_h.resize((size_t)n_comp_);
-      hip_check(hipMemcpy(cmargin_h.data(), comp_margin_.ptr, sizeof(double) * cmargin_h.size(),
-                          hipMemcpyDeviceToHost),
-                "component margins");
-    }
+    const double* cmargin_h =
+        rb.cmargin_n ? reinterpret_cast<const double*>(res_->down.ptr + rb.cmargin_off) : nullptr;
     for (int li = 0; li < nlanes; ++li) {
       const LaneWork& w = W[li];
+      const double* out = reinterpret_cast<const double*>(res_->down.ptr + out_off[li]);
       for (size_t j = 0; j < w.task_idx.size(); ++j) {
         Task* t = tasks[w.task_idx[j]];
-        const double* o = w.out.data() + kpc::kPairOut * j;
+        const double* o = out + kpc::kPairOut * j;
         TestResult res{o[0], o[1], o[2], o[3], o[4]};
         if (std::isnan(res.p)) res.p = 1.0;  // NAdelete = TRUE
         for (int rs : {w.fit_x[j], w.fit_y[j]})
           if (rs >= 0)
             res.margin = std::min(res.margin,
                                   rinfo_h[(size_t)rs * kpc::kMagicInfo + kpc::kMagicInfo - 2]);
-        if (!cmargin_h.empty())
+        if (cmargin_h)
           res.margin = std::min(res.margin, std::min(cmargin_h[w.pair_x[j]], cmargin_h[w.pair_y[j]]));
         apply_result(*t, res);
         if (!rmode_) twin_map_.emplace(twin_key(*t), res);
@@ -1242,68 +1302,130 @@ class Engine {
     return ds;
   }
 
-  // Status words of a wave's smooth setups, designs and fits, each read in
-  // one copy; the fit info of every residual slot of the level is returned
-  // through rinfo_out when given.
-  void check_device_status(const std::vector<kpc::MagicItem>& fits,
-                           const std::vector<kpc::GeometryItem>& geos, int nsm,
-                           std::vector<double>* rinfo_out = nullptr) {
+  // Status read-back of a wave: smooth set-up info, the design status words
+  // (one slot range), the fit records of every residual slot of the level,
+  // component status and HSIC component margins.  plan_status lays the
+  // regions out in the page-locked read-back block from byte `base` on,
+  // queue_status copies them there on one stream (behind that stream's
+  // kernels), and verify_status checks them once that stream is synchronised.
+  struct StatusReadBack {
+    size_t end = 0;                          // first byte past the regions
+    size_t smooth_off = 0, smooth_n = 0;     // ints, 4 per smooth slot
+    size_t design_off = 0, design_n = 0;     // doubles
+    int design_lo = 0;
+    size_t rinfo_off = 0, rinfo_n = 0;       // doubles, kMagicInfo per residual slot
+    size_t cstat_off = 0, cstat_n = 0;       // ints
+    size_t cmargin_off = 0, cmargin_n = 0;   // doubles
+    bool smooth_uni = false;
+  };
+  static size_t aligned256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+  StatusReadBack plan_status(const std::vector<kpc::MagicItem>& fits,
+                             const std::vector<kpc::GeometryItem>& geos, int nsm,
+                             const std::vector<int>* dst, bool want_info, bool want_margins,
+                             size_t base) const {
+    StatusReadBack rb;
+    size_t off = base;
     if (nsm > 0) {
-      const bool uni = level_ > 2;
-      DevArray<int>& info = uni ? usmooth_info_ : lsmooth_info_;
-      const size_t cnt = uni ? (size_t)p_ : (size_t)n_lsmooth_;
-      std::vector<int> h(cnt * 4);
-      hip_check(hipMemcpy(h.data(), info.ptr, sizeof(int) * h.size(), hipMemcpyDeviceToHost),
-                "smooth info");
-      for (size_t i = 0; i < cnt; ++i) {
-        if (uni && uni_slot_[i] < 0) continue;
-        if (h[4 * i] != 0)
-          throw KpcError(-5, "thin-plate setup failed (fewer unique locations than basis size)");
-      }
+      rb.smooth_uni = level_ > 2;
+      rb.smooth_n = (rb.smooth_uni ? (size_t)p_ : (size_t)n_lsmooth_) * 4;
+      rb.smooth_off = off;
+      off += aligned256(rb.smooth_n * sizeof(int));
     }
     if (!geos.empty()) {
-      // one copy of the compact status words of every design slot in range
       int lo = geos[0].design_slot, hi = lo;
       for (const auto& g : geos) {
         lo = std::min(lo, g.design_slot);
         hi = std::max(hi, g.design_slot);
       }
-      std::vector<double> st((size_t)(hi - lo + 1));
-      hip_check(hipMemcpy(st.data(), design_status_.ptr + lo, sizeof(double) * st.size(),
-                          hipMemcpyDeviceToHost),
-                "design status");
-      for (const auto& g : geos)
-        if (st[(size_t)(g.design_slot - lo)] != 0.0)
-          throw KpcError(-5, "design geometry failed (penalty rank or initial.sp)");
+      rb.design_lo = lo;
+      rb.design_n = (size_t)(hi - lo + 1);
+      rb.design_off = off;
+      off += aligned256(rb.design_n * sizeof(double));
     }
-    const bool want_info = rinfo_out && level_ > 0 && n_resid_ > 0;
-    if (!fits.empty() || want_info) {
-      std::vector<double> h((size_t)n_resid_ * kpc::kMagicInfo);
-      hip_check(hipMemcpy(h.data(), rinfo_.ptr, sizeof(double) * h.size(), hipMemcpyDeviceToHost),
-                "fit info");
-      for (const auto& f : fits) {
-        const double* o = h.data() + (size_t)f.resid_slot * kpc::kMagicInfo;
-        if (o[0] != 0.0) throw KpcError(-6, "GCV optimiser did not converge");
-        if (f.pad == 0) stats_.evals += (long long)o[5];
-      }
-      if (want_info) rinfo_out->swap(h);
+    if (!fits.empty() || (want_info && level_ > 0 && n_resid_ > 0)) {
+      rb.rinfo_n = (size_t)n_resid_ * kpc::kMagicInfo;
+      rb.rinfo_off = off;
+      off += aligned256(rb.rinfo_n * sizeof(double));
     }
+    if (dst && !dst->empty() && method_ != KPC_DCC_PERM &&
+        !(method_ == KPC_DCC_GAMMA && dense_components_)) {
+      rb.cstat_n = (size_t)n_comp_;
+      rb.cstat_off = off;
+      off += aligned256(rb.cstat_n * sizeof(int));
+    }
+    if (want_margins) {
+      rb.cmargin_n = (size_t)n_comp_;
+      rb.cmargin_off = off;
+      off += aligned256(rb.cmargin_n * sizeof(double));
+    }
+    rb.end = off;
+    return rb;
   }
 
-  void check_component_status(const std::vector<int>& dst) {
-    if (dst.empty() || method_ == KPC_DCC_PERM) return;
-    if (method_ == KPC_DCC_GAMMA && dense_components_) return;
-    std::vector<int> h((size_t)n_comp_);
-    hip_check(hipMemcpy(h.data(), comp_status_.ptr, sizeof(int) * h.size(), hipMemcpyDeviceToHost),
-              "component status");
-    for (int s : dst) {
-      if (method_ != KPC_DCC_GAMMA) {
-        if (h[s] <= 0 || h[s] > k_) throw KpcError(-7, "incomplete Cholesky produced no factor");
-        continue;
+  // the read-back block must already hold rb.end bytes
+  void queue_status(const StatusReadBack& rb, hipStream_t s) {
+    unsigned char* h = res_->down.ptr;
+    auto down = [&](size_t off, const void* src, size_t bytes) {
+      if (bytes)
+        hip_check(hipMemcpyAsync(h + off, src, bytes, hipMemcpyDeviceToHost, s), "status read-back");
+    };
+    down(rb.smooth_off, (rb.smooth_uni ? usmooth_info_ : lsmooth_info_).ptr,
+         rb.smooth_n * sizeof(int));
+    down(rb.design_off, design_status_.ptr + rb.design_lo, rb.design_n * sizeof(double));
+    down(rb.rinfo_off, rinfo_.ptr, rb.rinfo_n * sizeof(double));
+    down(rb.cstat_off, comp_status_.ptr, rb.cstat_n * sizeof(int));
+    down(rb.cmargin_off, comp_margin_.ptr, rb.cmargin_n * sizeof(double));
+  }
+
+  // Returns the fit records of the level's residual slots (in the read-back
+  // block, valid until the next read-back) or null when none were read.
+  const double* verify_status(const StatusReadBack& rb, const std::vector<kpc::MagicItem>& fits,
+                              const std::vector<kpc::GeometryItem>& geos,
+                              const std::vector<int>* dst) {
+    const unsigned char* h = res_->down.ptr;
+    if (rb.smooth_n) {
+      const int* info = reinterpret_cast<const int*>(h + rb.smooth_off);
+      for (size_t i = 0; i < rb.smooth_n / 4; ++i) {
+        if (rb.smooth_uni && uni_slot_[i] < 0) continue;
+        if (info[4 * i] != 0)
+          throw KpcError(-5, "thin-plate setup failed (fewer unique locations than basis size)");
       }
-      if (h[s] <= 0) throw KpcError(-7, "dcov component eigensolver did not converge");
-      stats_.lanczos_steps += h[s];
     }
+    if (rb.design_n) {
+      const double* st = reinterpret_cast<const double*>(h + rb.design_off);
+      for (const auto& g : geos)
+        if (st[(size_t)(g.design_slot - rb.design_lo)] != 0.0)
+          throw KpcError(-5, "design geometry failed (penalty rank or initial.sp)");
+    }
+    const double* rinfo = rb.rinfo_n ? reinterpret_cast<const double*>(h + rb.rinfo_off) : nullptr;
+    for (const auto& f : fits) {
+      const double* o = rinfo + (size_t)f.resid_slot * kpc::kMagicInfo;
+      if (o[0] != 0.0) throw KpcError(-6, "GCV optimiser did not converge");
+      if (f.pad == 0) stats_.evals += (long long)o[5];
+    }
+    if (rb.cstat_n) {
+      const int* cs = reinterpret_cast<const int*>(h + rb.cstat_off);
+      for (int sl : *dst) {
+        if (method_ != KPC_DCC_GAMMA) {
+          if (cs[sl] <= 0 || cs[sl] > k_) throw KpcError(-7, "incomplete Cholesky produced no factor");
+          continue;
+        }
+        if (cs[sl] <= 0) throw KpcError(-7, "dcov component eigensolver did not converge");
+        stats_.lanczos_steps += cs[sl];
+      }
+    }
+    return rinfo;
+  }
+
+  // Status words of set-ups run on the engine's own stream.
+  void check_device_status(const std::vector<kpc::MagicItem>& fits,
+                           const std::vector<kpc::GeometryItem>& geos, int nsm) {
+    const StatusReadBack rb = plan_status(fits, geos, nsm, nullptr, false, false, 0);
+    res_->down.reserve(rb.end);
+    queue_status(rb, stream_);
+    hip_check(hipStreamSynchronize(stream_), "status synchronize");
+    verify_status(rb, fits, geos, nullptr);
   }
 
   int n_, p_, k_;
