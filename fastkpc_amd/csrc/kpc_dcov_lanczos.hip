@@ -30,16 +30,23 @@ namespace kpc {
 
 constexpr int kLzThreads = 512;
 
-// Doubles of LDS for the twisted-factorisation pivots (2 m per eigenpair):
-// enough for all k eigenpairs of the first convergence check (m = 2k + 6) in
-// one batch while two workgroups still share a CU's 160 KB of LDS.
+// Dynamic LDS bytes of one workgroup when two share a CU's 160 KB: 80 KB less
+// 512 bytes for the kernel's static LDS (256 bytes).
+constexpr int kLzLdsBudget = 81920 - 512;
+
+// Doubles of the LDS workspace: the sort keys (2 P, P = n rounded up to a
+// power of two, at most 4096), the twisted-factorisation pivots of a
+// convergence check (2 m per eigenpair), the Ritz coefficients and the
+// moment's row blocks, and between convergence checks the first basis
+// columns.  It takes the rest of the budget, at least `base`.
 __host__ __device__ inline int lanczos_ivw(int n, int mmax, int k) {
+  (void)k;
   const int nl = (n + 63) & ~63;
   const int ml = (mmax + 64) & ~63;
   const int base = 4096 > 6 * mmax + 64 ? 4096 : 6 * mmax + 64;
-  const int room = (81920 - (3 * kMaxNumCol + 8) * 4) / 8 - (4 * nl + 4 * ml + 32 + 8 * kMaxNumCol);
-  const int want = 2 * (2 * k + 6) * k < room ? 2 * (2 * k + 6) * k : room;
-  return base > want ? base : want;
+  const int room =
+      (kLzLdsBudget - (3 * kMaxNumCol + 8) * 4) / 8 - (4 * nl + 4 * ml + 32 + 8 * kMaxNumCol);
+  return base > room ? base : room;
 }
 
 // Top-kk candidates by |value| (ties toward the higher ascending index): a
@@ -164,6 +171,39 @@ __device__ __forceinline__ double block_sum_alt(double v, double* slots, int& pa
   return t;
 }
 
+// Coefficients of z against basis columns b0 .. b0 + nb - 1 (lane partial
+// sums over rows lane, lane + 64, ...; columns past nb repeat the last one)
+// with the columns read from qbase, the slab or its LDS copy.
+__device__ __forceinline__ void gs_block_coeffs(const double* qbase, const double* z, int n,
+                                                int b0, int nb, int lane, double g[8]) {
+  const double* qrow[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) qrow[c] = qbase + (size_t)(b0 + (c < nb ? c : nb - 1)) * n;
+#pragma unroll 2
+  for (int i = lane; i < n; i += KPC_WAVE) {
+    double v[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) v[c] = qrow[c][i];
+    const double zi = z[i];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) g[c] += zi * v[c];
+  }
+}
+
+// Row i's combination sum_b cg[b] q_b[i] over whole blocks b0 <= b < b1 of
+// eight columns read from qbase, accumulated into sa[b mod 8].
+__device__ __forceinline__ void gs_block_sums(const double* qbase, const double* cg, int n, int i,
+                                              int b0, int b1, double sa[8]) {
+#pragma unroll 2
+  for (int b = b0; b + 8 <= b1; b += 8) {
+    double v[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) v[c] = qbase[(size_t)(b + c) * n + i];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) sa[c] += cg[b + c] * v[c];
+  }
+}
+
 __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
     const double* __restrict__ src_base, const long long* __restrict__ src_off,
     const int* __restrict__ dst_slot, int count, int n, int k, double* __restrict__ comp_vals,
@@ -199,6 +239,14 @@ __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
   const int nw = blockDim.x / KPC_WAVE;
   double* Q = scratch + (size_t)blockIdx.x * scratch_stride;   // n x (mmax + 1)
   double* S = Q + (size_t)n * (mmax + 1);                      // k x mmax (Ritz coefficients)
+  // Between convergence checks the workspace past the scan slots also holds
+  // basis columns 0 .. ncache0 - 1 (whole Gram-Schmidt blocks of eight), which
+  // both Gram-Schmidt passes then read from LDS instead of the slab: the same
+  // values, so the same results.  A convergence check overwrites them.
+  double* qc = ivl + 64;
+  int ncache0 = (ivw - 64) / n;
+  if (ncache0 > mmax + 1) ncache0 = mmax + 1;
+  ncache0 &= ~7;
   for (int item = blockIdx.x; item < count; item += gridDim.x) {
     const double* x = src_base + src_off[item];
     const int slot = dst_slot[item];
@@ -266,10 +314,12 @@ __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
     __syncthreads();
     const double inv0 = red[8];
     __syncthreads();
+    int ncache = ncache0;
     for (int i = tid; i < n; i += blockDim.x) {
       const double v = z[perm[i]] * inv0;
       rq[i].y = v;
       Q[i] = v;
+      if (ncache > 0) qc[i] = v;
     }
     __syncthreads();
     int m_final = mmax;
@@ -348,18 +398,10 @@ __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
         for (int b0 = wid * 8; b0 <= j; b0 += nw * 8) {
           double g[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
           const int nb = min(8, j + 1 - b0);
-          const double* qrow[8];
-#pragma unroll
-          for (int c = 0; c < 8; ++c) qrow[c] = Q + (size_t)(b0 + (c < nb ? c : nb - 1)) * n;
-#pragma unroll 2
-          for (int i = lane; i < n; i += KPC_WAVE) {
-            double v[8];
-#pragma unroll
-            for (int c = 0; c < 8; ++c) v[c] = qrow[c][i];
-            const double zi = z[i];
-#pragma unroll
-            for (int c = 0; c < 8; ++c) g[c] += zi * v[c];
-          }
+          if (b0 < ncache)
+            gs_block_coeffs(qc, z, n, b0, nb, lane, g);
+          else
+            gs_block_coeffs(Q, z, n, b0, nb, lane, g);
 #pragma unroll
           for (int c = 0; c < 8; ++c) g[c] = wave_sum_dpp(g[c]);
           if (lane < nb) {
@@ -373,7 +415,9 @@ __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
         part = 0.0;
         for (int i = tid; i < n; i += blockDim.x) {
           double sa[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-          int b = 0;
+          const int bc = min((j + 1) & ~7, ncache);  // whole blocks held in LDS
+          gs_block_sums(qc, cg, n, i, 0, bc, sa);
+          int b = bc;
 #pragma unroll 2
           for (; b + 8 <= j + 1; b += 8) {
             double v[8];
@@ -382,7 +426,7 @@ __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
 #pragma unroll
             for (int c = 0; c < 8; ++c) sa[c] += cg[b + c] * v[c];
           }
-          for (; b <= j; ++b) sa[0] += cg[b] * Q[(size_t)b * n + i];
+          for (; b <= j; ++b) sa[0] += cg[b] * (b < ncache ? qc[(size_t)b * n + i] : Q[(size_t)b * n + i]);
           const double zi =
               z[i] - (((sa[0] + sa[1]) + (sa[2] + sa[3])) + ((sa[4] + sa[5]) + (sa[6] + sa[7])));
           z[i] = zi;
@@ -405,6 +449,7 @@ __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
         const double v = z[i] * inv;
         rq[i].y = v;
         Q[(size_t)(j + 1) * n + i] = v;
+        if (j + 1 < ncache) qc[(size_t)(j + 1) * n + i] = v;
       }
       __syncthreads();
       const int m = j + 1;
@@ -491,6 +536,7 @@ __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
       }
       __syncthreads();
       if (ictl[0]) { m_final = m; break; }
+      ncache = 0;  // the check's pivots overwrote the cached columns
       prev_m = m;
       prev_worst = red[12];
       next_check = ictl[3];
