@@ -71,34 +71,104 @@ __device__ void lz_select(const double* cval, int ncand, int nlow, int hs, int k
   __syncthreads();
 }
 
-// Z[:, e] = Q S[:, e] for e < k (n x m basis, coefficient vectors at stride
-// sst): a thread takes one row and up to kRitzGroup vectors, so every basis
-// entry is loaded once per group (at k = 35, two groups: two passes of the
-// block over the rows); the basis loop is unrolled sixteen deep so the loads
-// overlap, and each coefficient sum runs over the basis in order.
-constexpr int kRitzGroup = 18;
-__device__ inline void ritz_assemble(const double* __restrict__ Q, const double* Sv, int sst,
-                                     int n, int m, int k, const int* perm,
-                                     double* __restrict__ Z) {
-  const int ng = (k + kRitzGroup - 1) / kRitzGroup;
-  for (int u = threadIdx.x; u < n * ng; u += blockDim.x) {
-    const int i = u % n, g = u / n;
-    const int e0 = kRitzGroup * g;
-    const int ne = min(kRitzGroup, k - e0);
-    const double* sv = Sv + (size_t)e0 * sst;
-    double acc[kRitzGroup];
+// Z[:, e] = Q S[:, e] for e < k on FP64 matrix cores (v_mfma_f64_16x16x4f64):
+// a wave takes tiles of 16 rows and runs K over the basis four vectors per
+// step; the A operand (lane l: row l % 16, basis vector l / 16 of the step)
+// is loaded once per step and shared by the NTC tiles of 16 vectors (B: lane
+// l holds coefficient b0 + l / 16 of vector l % 16).  Accumulator item v of
+// lane l is row 4 v + l / 16, vector l % 16.  Rows past n load a valid row
+// (not stored); basis vectors past m and vectors past k load 0.
+typedef double lz_v4d __attribute__((ext_vector_type(4)));
+template <int NTC>
+__device__ inline void ritz_mfma(const double* Q, const double* Sv, int sst, int n, int m, int k,
+                                 const int* perm, double* Z) {
+  const int lane = threadIdx.x & (KPC_WAVE - 1);
+  const int wid = threadIdx.x / KPC_WAVE;
+  const int nw = (int)blockDim.x / KPC_WAVE;
+  const int r16 = lane & 15, kq = lane >> 4;
+  const int ntr = (n + 15) >> 4;
+  for (int tr = wid; tr < ntr; tr += nw) {
+    const int row = tr * 16 + r16;
+    const double* qr = Q + (row < n ? row : n - 1);
+    lz_v4d acc[NTC];
 #pragma unroll
-    for (int c = 0; c < kRitzGroup; ++c) acc[c] = 0.0;
-#pragma unroll 16
-    for (int b = 0; b < m; ++b) {
-      const double qb = Q[(size_t)b * n + i];
+    for (int t = 0; t < NTC; ++t) acc[t] = lz_v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int b0 = 0; b0 < m; b0 += 4) {
+      const int b = b0 + kq;
+      const double a = b < m ? qr[(size_t)b * n] : 0.0;
 #pragma unroll
-      for (int c = 0; c < kRitzGroup; ++c)
-        if (c < ne) acc[c] += qb * sv[(size_t)c * sst + b];
+      for (int t = 0; t < NTC; ++t) {
+        const int c = t * 16 + r16;
+        const double bv = (b < m && c < k) ? Sv[(size_t)c * sst + b] : 0.0;
+        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[t], 0, 0, 0);
+      }
     }
 #pragma unroll
-    for (int c = 0; c < kRitzGroup; ++c)
-      if (c < ne) Z[(size_t)(e0 + c) * n + perm[i]] = acc[c];
+    for (int t = 0; t < NTC; ++t) {
+      const int c = t * 16 + r16;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = tr * 16 + 4 * v + kq;
+        if (i < n && c < k) Z[(size_t)c * n + perm[i]] = acc[t][v];
+      }
+    }
+  }
+}
+__device__ inline void ritz_assemble_mfma(const double* Q, const double* Sv, int sst, int n, int m,
+                                          int k, const int* perm, double* Z) {
+  switch ((k + 15) >> 4) {
+    case 1: ritz_mfma<1>(Q, Sv, sst, n, m, k, perm, Z); break;
+    case 2: ritz_mfma<2>(Q, Sv, sst, n, m, k, perm, Z); break;
+    case 3: ritz_mfma<3>(Q, Sv, sst, n, m, k, perm, Z); break;
+    default: ritz_mfma<4>(Q, Sv, sst, n, m, k, perm, Z); break;
+  }
+}
+
+// Gram matrix G = Z' Z of the k vectors (columns of Z, n rows) on FP64
+// matrix cores, into G (k x k, LDS): tiles of 16 x 16 with i <= j, one per
+// wave in turn, K over the rows four per step; an entry's sum runs over rows
+// 0 .. n - 1 as one chain of FMAs.  Vectors past k and rows past n load 0
+// (vector indices clamped to a valid one).
+template <int NTC>
+__device__ inline void gram_mfma(const double* Z, int n, int k, double* G) {
+  const int lane = threadIdx.x & (KPC_WAVE - 1);
+  const int wid = threadIdx.x / KPC_WAVE;
+  const int nw = (int)blockDim.x / KPC_WAVE;
+  const int r16 = lane & 15, kq = lane >> 4;
+  int tile = 0;
+  for (int ti = 0; ti < NTC; ++ti)
+    for (int tj = ti; tj < NTC; ++tj, ++tile) {
+      if (tile % nw != wid) continue;
+      const int ci = ti * 16 + r16, cj = tj * 16 + r16;
+      const double* zi = Z + (size_t)(ci < k ? ci : k - 1) * n;
+      const double* zj = Z + (size_t)(cj < k ? cj : k - 1) * n;
+      const bool oki = ci < k, okj = cj < k;
+      lz_v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+      for (int r0 = 0; r0 < n; r0 += 4) {
+        const int r = r0 + kq;
+        const int rr = r < n ? r : n - 1;
+        const double a = (r < n && oki) ? zi[rr] : 0.0;
+        const double b = (r < n && okj) ? zj[rr] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int gi = ti * 16 + 4 * v + kq, gj = tj * 16 + r16;
+        if (gi < k && gj < k) {
+          G[gi * k + gj] = acc[v];
+          G[gj * k + gi] = acc[v];
+        }
+      }
+    }
+}
+__device__ inline void gram_assemble_mfma(const double* Z, int n, int k, double* G) {
+  switch ((k + 15) >> 4) {
+    case 1: gram_mfma<1>(Z, n, k, G); break;
+    case 2: gram_mfma<2>(Z, n, k, G); break;
+    case 3: gram_mfma<3>(Z, n, k, G); break;
+    default: gram_mfma<4>(Z, n, k, G); break;
   }
 }
 
@@ -554,9 +624,9 @@ __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
       for (int e = tid; e < k * m_final; e += blockDim.x)
         ivl[e] = S[(size_t)(e / m_final) * mmax + e % m_final];
       __syncthreads();
-      ritz_assemble(Q, ivl, m_final, n, m_final, k, perm, Z);
+      ritz_assemble_mfma(Q, ivl, m_final, n, m_final, k, perm, Z);
     } else {
-      ritz_assemble(Q, S, mmax, n, m_final, k, perm, Z);
+      ritz_assemble_mfma(Q, S, mmax, n, m_final, k, perm, Z);
     }
     __syncthreads();
     for (int c = wid; c < k; c += nw) {
@@ -568,47 +638,21 @@ __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
     }
     __syncthreads();
     // self moment sum_ij lambda_i lambda_j (z_i . z_j)^2 over the centred
-    // vectors: the Gram entries (i <= j) accumulate over blocks of 64 rows
-    // staged in LDS (row-major 64 x k), in row order
-    double gacc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    // vectors: the Gram entries (rows in order) on matrix cores into the
+    // workspace, then one thread per pair (i <= j)
+    gram_assemble_mfma(Z, n, k, ivl);
+    __syncthreads();
     const int npairs = k * (k + 1) / 2;
-    for (int r0 = 0; r0 < n; r0 += 64) {
-      const int rows = min(64, n - r0);
-      for (int e = tid; e < rows * k; e += blockDim.x) {
-        const int c = e / rows, r = e % rows;
-        ivl[r * k + c] = Z[(size_t)c * n + r0 + r];
-      }
-      __syncthreads();
-#pragma unroll
-      for (int t = 0; t < 5; ++t) {
-        int pidx = tid + t * (int)blockDim.x;
-        if (pidx < npairs) {
-          int i = 0;
-          while (pidx >= k - i) {
-            pidx -= k - i;
-            ++i;
-          }
-          const int jj = i + pidx;
-          double sacc = gacc[t];
-          for (int r = 0; r < rows; ++r) sacc += ivl[r * k + i] * ivl[r * k + jj];
-          gacc[t] = sacc;
-        }
-      }
-      __syncthreads();
-    }
     double mpart = 0.0;
-#pragma unroll
-    for (int t = 0; t < 5; ++t) {
-      int pidx = tid + t * (int)blockDim.x;
-      if (pidx < npairs) {
-        int i = 0;
-        while (pidx >= k - i) {
-          pidx -= k - i;
-          ++i;
-        }
-        const int jj = i + pidx;
-        mpart += (jj == i ? 1.0 : 2.0) * (sval[i] * sval[jj] * gacc[t] * gacc[t]);
+    for (int pidx0 = tid; pidx0 < npairs; pidx0 += blockDim.x) {
+      int pidx = pidx0, i = 0;
+      while (pidx >= k - i) {
+        pidx -= k - i;
+        ++i;
       }
+      const int jj = i + pidx;
+      const double gv = ivl[i * k + jj];
+      mpart += (jj == i ? 1.0 : 2.0) * (sval[i] * sval[jj] * gv * gv);
     }
     const double moment = block_sum(mpart, red);
     if (tid == 0) {

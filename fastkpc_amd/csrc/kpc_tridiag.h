@@ -19,12 +19,29 @@ __device__ __forceinline__ double lz_rcp(double x) {
 }
 
 // Number of eigenvalues of the tridiagonal (d, e2 = e^2) strictly below x.
+// Four rows per trip: their loads and shifts d_i - x are issued ahead of the
+// pivot chain, which then waits on no memory (same operations per row).
 __device__ inline int lz_sturm(const double* d, const double* e2, int n, double x, double pivmin) {
   int cnt = 0;
   double q = d[0] - x;
   if (fabs(q) < pivmin) q = -pivmin;
   cnt += q < 0.0;
-  for (int i = 1; i < n; ++i) {
+  int i = 1;
+  for (; i + 4 <= n; i += 4) {
+    double dx[4], ev[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      dx[u] = d[i + u] - x;
+      ev[u] = e2[i - 1 + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      q = dx[u] - ev[u] * lz_rcp1(q);
+      if (fabs(q) < pivmin) q = -pivmin;
+      cnt += q < 0.0;
+    }
+  }
+  for (; i < n; ++i) {
     q = (d[i] - x) - e2[i - 1] * lz_rcp1(q);
     if (fabs(q) < pivmin) q = -pivmin;
     cnt += q < 0.0;
@@ -42,10 +59,33 @@ __device__ inline int lz_sturm(const double* d, const double* e2, int n, double 
 // non-null; returns its last component.
 __device__ inline double lz_twisted(const double* al, const double* be, int m, double th, double pivmin,
                              double* rd, double* rr, double* z) {
-  // forward and backward pivot sweeps, interleaved for ILP
+  // forward and backward pivot sweeps, interleaved for ILP; four steps per
+  // trip with their diagonal shifts and squared off-diagonals loaded first
+  // (the pivot stores to LDS would otherwise keep later loads behind them)
   double d = al[0] - th;
   double r = al[m - 1] - th;
-  for (int i = 0; i < m; ++i) {
+  int i = 0;
+  for (; i + 4 < m; i += 4) {
+    double fa[4], fb[4], ba[4], bb[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      fa[u] = al[i + u + 1] - th;
+      fb[u] = be[i + u] * be[i + u];
+      ba[u] = al[m - 2 - i - u] - th;
+      bb[u] = be[m - 2 - i - u] * be[m - 2 - i - u];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (fabs(d) < pivmin) d = -pivmin;
+      if (fabs(r) < pivmin) r = -pivmin;
+      const double id = lz_rcp(d), ir = lz_rcp(r);
+      rd[i + u] = id;
+      rr[m - 1 - i - u] = ir;
+      d = fa[u] - fb[u] * id;
+      r = ba[u] - bb[u] * ir;
+    }
+  }
+  for (; i < m; ++i) {
     if (fabs(d) < pivmin) d = -pivmin;
     if (fabs(r) < pivmin) r = -pivmin;
     const double id = lz_rcp(d), ir = lz_rcp(r);
