@@ -166,6 +166,20 @@ __host__ __device__ inline size_t smooth_lds_size(int np2) {
          16 * sizeof(int);
 }
 
+// Dynamic LDS of a set-up (one workgroup per CU) is capped at this: the
+// fixed areas above, then a cache of the first Lanczos basis columns (column
+// stride n, whole Gram-Schmidt blocks of eight; none when under eight fit).
+constexpr size_t kSmoothLdsBudget = 163840 - 1024;
+
+__host__ __device__ inline int smooth_cache_cols(int n) {
+  int np2 = 64;
+  while (np2 < n) np2 <<= 1;
+  const size_t base = smooth_lds_size(np2);
+  if (base >= kSmoothLdsBudget) return 0;
+  const int c = (int)((kSmoothLdsBudget - base) / (sizeof(double) * (size_t)n));
+  return (c < 64 ? c : 64) & ~7;
+}
+
 __device__ inline SmoothLds carve_smooth_lds(double* base, int np2) {
   SmoothLds L;
   double* d = base;
@@ -285,6 +299,39 @@ __device__ void tp_tile_product(const double* __restrict__ Et, const double* q, 
   }
 }
 
+// Gram-Schmidt coefficient partial sums of z against basis columns b0 ..
+// b0 + nb - 1 (lanes over rows; columns past nb repeat the last one), the
+// columns read from qbase: the slab or its LDS copy.
+__device__ __forceinline__ void smooth_block_coeffs(const double* qbase, const double* z, int nu,
+                                                    int b0, int nb, int lane, double g[8]) {
+  const double* qrow[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) qrow[k] = qbase + (size_t)(b0 + (k < nb ? k : nb - 1)) * nu;
+#pragma unroll 2
+  for (int i = lane; i < nu; i += KPC_WAVE) {
+    double v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = qrow[k][i];
+    const double zi = z[i];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) g[k] += zi * v[k];
+  }
+}
+
+// Row i's sum over whole blocks b0 <= b < b1 of eight basis columns read from
+// qbase, accumulated into sa[b mod 8].
+__device__ __forceinline__ void smooth_block_sums(const double* qbase, const double* hc, int nu,
+                                                  int i, int b0, int b1, double sa[8]) {
+#pragma unroll 2
+  for (int b = b0; b + 8 <= b1; b += 8) {
+    double v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = qbase[(size_t)(b + k) * nu + i];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) sa[k] += hc[b + k] * v[k];
+  }
+}
+
 __global__ __launch_bounds__(kSmoothThreads) void smooth_setup_kernel(
     const double* __restrict__ data, int n, int p, const int* __restrict__ items, int count,
     double* __restrict__ out_X, double* __restrict__ out_S, int* __restrict__ out_info,
@@ -294,6 +341,10 @@ __global__ __launch_bounds__(kSmoothThreads) void smooth_setup_kernel(
   int np2 = 64;  // at least one tile row of q (zero past nu)
   while (np2 < n) np2 <<= 1;
   SmoothLds L = carve_smooth_lds(lds_raw, np2);
+  // basis columns 0 .. ncache - 1 of the current set-up, column stride nu:
+  // the Gram-Schmidt passes read them here instead of the slab (same values)
+  double* qc = lds_raw + smooth_lds_size(np2) / sizeof(double);
+  const int ncache0 = smooth_cache_cols(n);
   const int tid = threadIdx.x;
   const int lane = tid & (KPC_WAVE - 1);
   const int wid = tid / KPC_WAVE;
@@ -407,10 +458,12 @@ __global__ __launch_bounds__(kSmoothThreads) void smooth_setup_kernel(
       L.dctl[0] = sqrt(nrm);
     }
     __syncthreads();
+    const int ncache = ncache0 < nu + 1 ? ncache0 : (nu + 1) & ~7;
     for (int r = tid; r < nu; r += blockDim.x) {
       const double v = L.z[r] / L.dctl[0];
       L.qv[r] = v;
       Qb[r] = v;
+      if (ncache > 0) qc[r] = v;
     }
     __syncthreads();
     const double tol = pow(kEps, 0.7);
@@ -458,18 +511,10 @@ __global__ __launch_bounds__(kSmoothThreads) void smooth_setup_kernel(
         for (int b0 = wid * 8; b0 <= it; b0 += nw * 8) {
           double g[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
           const int nb = min(8, it + 1 - b0);
-          const double* qrow[8];
-#pragma unroll
-          for (int k = 0; k < 8; ++k) qrow[k] = Qb + (size_t)(b0 + (k < nb ? k : nb - 1)) * nu;
-#pragma unroll 2
-          for (int i = lane; i < nu; i += KPC_WAVE) {
-            double v[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) v[k] = qrow[k][i];
-            const double zi = L.z[i];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) g[k] += zi * v[k];
-          }
+          if (b0 < ncache)
+            smooth_block_coeffs(qc, L.z, nu, b0, nb, lane, g);
+          else
+            smooth_block_coeffs(Qb, L.z, nu, b0, nb, lane, g);
 #pragma unroll
           for (int k = 0; k < 8; ++k) g[k] = wave_sum(g[k]);
           if (lane < nb) {
@@ -483,7 +528,9 @@ __global__ __launch_bounds__(kSmoothThreads) void smooth_setup_kernel(
         part = 0.0;
         for (int i = tid; i < nu; i += blockDim.x) {
           double sa[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-          int b = 0;
+          const int bc = min((it + 1) & ~7, ncache);  // whole blocks held in LDS
+          smooth_block_sums(qc, hc, nu, i, 0, bc, sa);
+          int b = bc;
 #pragma unroll 2
           for (; b + 8 <= it + 1; b += 8) {
             double v[8];
@@ -492,7 +539,8 @@ __global__ __launch_bounds__(kSmoothThreads) void smooth_setup_kernel(
 #pragma unroll
             for (int k = 0; k < 8; ++k) sa[k] += hc[b + k] * v[k];
           }
-          for (; b <= it; ++b) sa[0] += hc[b] * Qb[(size_t)b * nu + i];
+          for (; b <= it; ++b)
+            sa[0] += hc[b] * (b < ncache ? qc[(size_t)b * nu + i] : Qb[(size_t)b * nu + i]);
           const double zi =
               L.z[i] - (((sa[0] + sa[1]) + (sa[2] + sa[3])) + ((sa[4] + sa[5]) + (sa[6] + sa[7])));
           L.z[i] = zi;
@@ -511,6 +559,7 @@ __global__ __launch_bounds__(kSmoothThreads) void smooth_setup_kernel(
           const double v = L.z[r] / rn;
           L.qv[r] = v;
           qn[r] = v;
+          if (it + 1 < ncache) qc[(size_t)(it + 1) * nu + r] = v;
         }
       }
       __syncthreads();
@@ -626,27 +675,9 @@ __global__ __launch_bounds__(kSmoothThreads) void smooth_setup_kernel(
       __syncthreads();
       break;
     }
-    // ---- Ritz vectors U = Q Zs (nu x K), eight vectors per thread
-    {
-      const int K = upper + lower;
-      const int ng = (K + 7) / 8;
-      for (int u = tid; u < nu * ng; u += blockDim.x) {
-        const int r = u % nu, g = u / nu;
-        const int e0 = 8 * g;
-        const int ne = min(8, K - e0);
-        double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        const double* zs = Zs + (size_t)e0 * completed;
-        for (int b = 0; b < completed; ++b) {
-          const double qb = Qb[(size_t)b * nu + r];
-#pragma unroll
-          for (int k = 0; k < 8; ++k)
-            if (k < ne) acc[k] += qb * zs[(size_t)k * completed + b];
-        }
-#pragma unroll
-        for (int k = 0; k < 8; ++k)
-          if (k < ne) U[(size_t)(e0 + k) * nu + r] = acc[k];
-      }
-    }
+    // ---- Ritz vectors U = Q Zs (nu x K) on matrix cores (each entry's sum
+    // over the basis in order, as a chain of FMAs)
+    ritz_assemble_mfma(Qb, Zs, completed, nu, completed, upper + lower, nullptr, U);
     __syncthreads();
     // ---- T columns [1, x (, y)]; TU = T' U (M x kb)
     for (int e = wid; e < M * kb; e += nw) {
@@ -886,7 +917,7 @@ long long smooth_scratch_doubles(int n) {
 size_t smooth_lds_bytes(int n) {
   int np2 = 64;
   while (np2 < n) np2 <<= 1;
-  return smooth_lds_size(np2);
+  return smooth_lds_size(np2) + (size_t)smooth_cache_cols(n) * n * sizeof(double);
 }
 
 hipError_t launch_smooth_setup(const double* data, int n, int p, const int* items, int count,

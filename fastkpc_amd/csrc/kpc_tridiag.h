@@ -1,6 +1,7 @@
 // Symmetric tridiagonal eigen-solvers shared by the Lanczos kernels (dcov
 // components, thin-plate smooth setup): Sturm counts, multisection for
-// selected eigenvalues and twisted factorisations for their eigenvectors.
+// selected eigenvalues, twisted factorisations for their eigenvectors, and the
+// Ritz vectors Q S on FP64 matrix cores.
 // Dependent FP64 operations cost tens of cycles each on CDNA4, so the serial
 // recurrences use hardware reciprocal estimates refined by Newton steps.
 #pragma once
@@ -196,6 +197,61 @@ __device__ inline void lz_multisect(const double* al, const double* e2, int m, d
     cval[c] = 0.5 * (lo + hi);
   }
   __syncthreads();
+}
+
+// Z[:, e] = Q S[:, e] for e < k on FP64 matrix cores (v_mfma_f64_16x16x4f64):
+// a wave takes tiles of 16 rows and runs K over the basis four vectors per
+// step; the A operand (lane l: row l % 16, basis vector l / 16 of the step)
+// is loaded once per step and shared by the NTC tiles of 16 vectors (B: lane
+// l holds coefficient b0 + l / 16 of vector l % 16).  Accumulator item v of
+// lane l is row 4 v + l / 16, vector l % 16.  Rows past n load a valid row
+// (not stored); basis vectors past m and vectors past k load 0.  Row i of the
+// result goes to row perm[i] of Z (row i when perm is null).
+typedef double lz_v4d __attribute__((ext_vector_type(4)));
+template <int NTC>
+__device__ inline void ritz_mfma(const double* Q, const double* Sv, int sst, int n, int m, int k,
+                                 const int* perm, double* Z) {
+  const int lane = threadIdx.x & (KPC_WAVE - 1);
+  const int wid = threadIdx.x / KPC_WAVE;
+  const int nw = (int)blockDim.x / KPC_WAVE;
+  const int r16 = lane & 15, kq = lane >> 4;
+  const int ntr = (n + 15) >> 4;
+  for (int tr = wid; tr < ntr; tr += nw) {
+    const int row = tr * 16 + r16;
+    const double* qr = Q + (row < n ? row : n - 1);
+    lz_v4d acc[NTC];
+#pragma unroll
+    for (int t = 0; t < NTC; ++t) acc[t] = lz_v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int b0 = 0; b0 < m; b0 += 4) {
+      const int b = b0 + kq;
+      const double a = b < m ? qr[(size_t)b * n] : 0.0;
+#pragma unroll
+      for (int t = 0; t < NTC; ++t) {
+        const int c = t * 16 + r16;
+        const double bv = (b < m && c < k) ? Sv[(size_t)c * sst + b] : 0.0;
+        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[t], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < NTC; ++t) {
+      const int c = t * 16 + r16;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = tr * 16 + 4 * v + kq;
+        if (i < n && c < k) Z[(size_t)c * n + (perm ? perm[i] : i)] = acc[t][v];
+      }
+    }
+  }
+}
+__device__ inline void ritz_assemble_mfma(const double* Q, const double* Sv, int sst, int n, int m,
+                                          int k, const int* perm, double* Z) {
+  switch ((k + 15) >> 4) {
+    case 1: ritz_mfma<1>(Q, Sv, sst, n, m, k, perm, Z); break;
+    case 2: ritz_mfma<2>(Q, Sv, sst, n, m, k, perm, Z); break;
+    case 3: ritz_mfma<3>(Q, Sv, sst, n, m, k, perm, Z); break;
+    default: ritz_mfma<4>(Q, Sv, sst, n, m, k, perm, Z); break;
+  }
 }
 
 }  // namespace kpc
