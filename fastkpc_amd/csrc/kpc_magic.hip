@@ -241,6 +241,10 @@ __device__ __attribute__((always_inline)) void wg_gemm_tn(const double* A, int l
 __device__ __attribute__((always_inline)) void upper_tri_inverse_fast(const double* R, int ld, int q,
                                                                       double* X, double* dinv);
 
+// kLdsQR: the pivoted QR works on an LDS copy of X (n x q, dynamic LDS) and
+// stores the factored columns to the design afterwards; otherwise it works on
+// the design's QR block in device memory.  Same operations either way.
+template <bool kLdsQR>
 __global__ __launch_bounds__(kGeomThreads) void geometry_kernel(
     const GeometryItem* __restrict__ items, int count, int n, int q, int m, int dim, int rk,
     const double* __restrict__ smooth_X, const double* __restrict__ smooth_S,
@@ -255,6 +259,11 @@ __global__ __launch_bounds__(kGeomThreads) void geometry_kernel(
   // per-lane scratch memory
   __shared__ double isp_pd[kMaxQ], isp_vals[kMaxQ], isp_sp[kMaxPen], isp_sx[32], isp_spd[32];
   __shared__ unsigned char isp_pen[kMaxQ];
+  __shared__ double isp_B[29 * 29];  // one smooth's penalty, staged for the serial section
+  __shared__ double isp_rmn[29], isp_cmn[29];  // its row / column means of |B|
+  // one smooth's penalty factor (dim <= 29), pivoted-Cholesky work and pivots
+  __shared__ double pen_F[29 * 29], pen_wk[2 * 29 + 8];
+  __shared__ int pen_pv[29];
   const int tid = threadIdx.x;
   const int lane = tid & (KPC_WAVE - 1);
   const int wid = tid / KPC_WAVE;
@@ -265,7 +274,8 @@ __global__ __launch_bounds__(kGeomThreads) void geometry_kernel(
     const GeometryItem it = items[item];
     double* D = design + (size_t)it.design_slot * DL.stride;
     double* X = D + DL.X;
-    double* QR = D + DL.QR;
+    extern __shared__ double geo_dyn[];
+    double* QR = kLdsQR ? geo_dyn : D + DL.QR;
     // assemble X
     for (int r = tid; r < n; r += blockDim.x) X[r] = 1.0;
     for (int s = 0; s < m; ++s) {
@@ -354,7 +364,9 @@ __global__ __launch_bounds__(kGeomThreads) void geometry_kernel(
       }
       __syncthreads();
     }
-    // R and pivots
+    // R and pivots (and the factored columns, from the LDS copy)
+    if (kLdsQR)
+      for (int e = tid; e < n * q; e += blockDim.x) D[DL.QR + e] = QR[e];
     for (int e = tid; e < q * q; e += blockDim.x) {
       const int r = e % q, c = e / q;
       D[DL.R + e] = r <= c ? QR[(size_t)c * n + r] : 0.0;
@@ -365,9 +377,9 @@ __global__ __launch_bounds__(kGeomThreads) void geometry_kernel(
     int status = 0;
     for (int s = 0; s < m; ++s) {
       const double* Sb = smooth_S + (size_t)it.smooth_slot[s] * 29 * 29;
-      double* F = ws;               // dim x dim
-      double* wk = ws + dim * dim;  // 2 dim
-      int* pv = reinterpret_cast<int*>(ws + dim * dim + 2 * dim + 8);
+      double* F = pen_F;   // dim x dim
+      double* wk = pen_wk;  // 2 dim
+      int* pv = pen_pv;
       for (int e = tid; e < dim * dim; e += blockDim.x) {
         const int r = e % dim, c = e / dim;
         F[e] = r <= c ? Sb[(size_t)c * dim + r] : 0.0;
@@ -426,29 +438,43 @@ __global__ __launch_bounds__(kGeomThreads) void geometry_kernel(
       if (lane == 0) colv[c] = s.hi + s.lo;
     }
     __syncthreads();
-    if (tid == 0) {
-      double* pd = isp_pd;  // q
-      unsigned char* pen = isp_pen;
-      double* sp = isp_sp;
+    double* pd = isp_pd;  // q
+    unsigned char* pen = isp_pen;
+    double* sp = isp_sp;
+    if (tid == 0)
       for (int c = 0; c < q; ++c) { pd[c] = 0.0; pen[c] = 0; }
-      const double thr_scale = pow(kEps, 0.8);
-      double* sx = isp_sx;
-      double* sp_d = isp_spd;
-      for (int s = 0; s < m; ++s) {
-        const double* B = smooth_S + (size_t)it.smooth_slot[s] * 29 * 29;
+    const double thr_scale = pow(kEps, 0.8);
+    double* sx = isp_sx;
+    double* sp_d = isp_spd;
+    // per smooth: the workgroup stages its penalty in LDS, then thread 0 runs
+    // the serial part from there (the same operations as reading it directly)
+    for (int s = 0; s < m; ++s) {
+      const double* Bg = smooth_S + (size_t)it.smooth_slot[s] * 29 * 29;
+      __syncthreads();  // colv ready; the previous smooth's serial part is done
+      for (int e = tid; e < dim * dim; e += blockDim.x) isp_B[e] = Bg[e];
+      __syncthreads();
+      // row and column means of |B| in double-double, a thread per row (each
+      // sum in column order), and max |B| (exact in any order)
+      for (int r = tid; r < dim; r += blockDim.x) {
+        dd rt{0.0, 0.0}, ct{0.0, 0.0};
+        for (int o = 0; o < dim; ++o) {
+          rt = dd_add(rt, dd{fabs(isp_B[(size_t)o * dim + r]), 0.0});
+          ct = dd_add(ct, dd{fabs(isp_B[(size_t)r * dim + o]), 0.0});
+        }
+        const dd rm = dd_div_int(rt, (double)dim), cmn = dd_div_int(ct, (double)dim);
+        isp_rmn[r] = rm.hi + rm.lo;
+        isp_cmn[r] = cmn.hi + cmn.lo;
+      }
+      double mxp = 0.0;
+      for (int e = tid; e < dim * dim; e += blockDim.x) mxp = fmax(mxp, fabs(isp_B[e]));
+      const double mx = block_max(mxp, red);
+      if (tid == 0) {
+        const double* B = isp_B;
         const int off = 1 + s * dim;
-        double mx = 0.0;
-        for (int e = 0; e < dim * dim; ++e) mx = fmax(mx, fabs(B[e]));
         const double thr = thr_scale * mx;
         int cnt = 0;
         for (int r = 0; r < dim; ++r) {
-          dd rt{0.0, 0.0}, ct{0.0, 0.0};
-          for (int o = 0; o < dim; ++o) {
-            rt = dd_add(rt, dd{fabs(B[(size_t)o * dim + r]), 0.0});
-            ct = dd_add(ct, dd{fabs(B[(size_t)r * dim + o]), 0.0});
-          }
-          dd rm = dd_div_int(rt, (double)dim), cmn = dd_div_int(ct, (double)dim);
-          const double rmn = rm.hi + rm.lo, cmv2 = cmn.hi + cmn.lo;
+          const double rmn = isp_rmn[r], cmv2 = isp_cmn[r];
           const double dg = fabs(B[(size_t)r * dim + r]);
           if (rmn > thr && cmv2 > thr && dg > thr) {
             sx[cnt] = colv[off + r];
@@ -457,10 +483,16 @@ __global__ __launch_bounds__(kGeomThreads) void geometry_kernel(
             ++cnt;
           }
         }
-        if (cnt == 0) { status = -8; sp[s] = 1.0; continue; }
-        sp[s] = serial_r_mean(sx, cnt) / serial_r_mean(sp_d, cnt);
-        for (int r = 0; r < dim; ++r) pd[off + r] += sp[s] * B[(size_t)r * dim + r];
+        if (cnt == 0) {
+          status = -8;
+          sp[s] = 1.0;
+        } else {
+          sp[s] = serial_r_mean(sx, cnt) / serial_r_mean(sp_d, cnt);
+          for (int r = 0; r < dim; ++r) pd[off + r] += sp[s] * B[(size_t)r * dim + r];
+        }
       }
+    }
+    if (tid == 0) {
       double* vals = isp_vals;
       for (int guard = 0; guard < 4000; ++guard) {
         int cnt = 0;
@@ -3121,9 +3153,23 @@ hipError_t launch_geometry(const GeometryItem* items, int count, int n, int q, i
                            double* status_out, long long smooth_x_stride, double* scratch,
                            int grid, hipStream_t stream) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(geometry_kernel, dim3(grid), dim3(kGeomThreads), 0, stream, items, count, n,
-                     q, m, dim, rk, smooth_X, smooth_S, design, status_out, smooth_x_stride,
-                     scratch, geometry_scratch_doubles(q));
+  // the QR in LDS when X (n x q) fits beside the kernel's static LDS in a CU
+  hipFuncAttributes fa;
+  hipError_t err = hipFuncGetAttributes(&fa, (const void*)geometry_kernel<true>);
+  if (err != hipSuccess) return err;
+  const size_t dyn = (size_t)n * q * sizeof(double);
+  if (dyn + fa.sharedSizeBytes <= 163840) {
+    err = hipFuncSetAttribute((const void*)geometry_kernel<true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(geometry_kernel<true>, dim3(grid), dim3(kGeomThreads), dyn, stream, items,
+                       count, n, q, m, dim, rk, smooth_X, smooth_S, design, status_out,
+                       smooth_x_stride, scratch, geometry_scratch_doubles(q));
+  } else {
+    hipLaunchKernelGGL(geometry_kernel<false>, dim3(grid), dim3(kGeomThreads), 0, stream, items,
+                       count, n, q, m, dim, rk, smooth_X, smooth_S, design, status_out,
+                       smooth_x_stride, scratch, geometry_scratch_doubles(q));
+  }
   return hipGetLastError();
 }
 
