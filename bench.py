@@ -385,7 +385,7 @@ def main():
         # kernels' limiter is the latency of their sequential steps: the
         # occupancy / stall counters are in profiles/counters_sem351x48.md
         "bound": "mfma",
-        "limiter": "latency (sequential optimiser and Lanczos steps; waves wait in 58-69% of "
+        "limiter": "latency (sequential optimiser and Lanczos steps; waves wait in 60-68% of "
                    "their resident cycles)",
         "kernel": kname,
         "achieved": round(achieved, 4),
