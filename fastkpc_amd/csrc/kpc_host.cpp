@@ -516,6 +516,13 @@ class Engine {
     }
   }
 
+  // Univariate smooths (levels 1 and >= 3, the same set-up for a column at
+  // either level) go to the per-column cache, built once per call; level 1
+  // keeps per-design slots when its tests run one launch sequence each (the
+  // plain form).
+  void set_share_univariate(bool v) { share_uni_l1_ = v; }
+  bool uni_smooths() const { return level_ > 2 || (level_ == 1 && share_uni_l1_); }
+
   void reserve_level(size_t max_designs, size_t max_resid) {
     const size_t nx = (size_t)n_ * 29;
     if (level_ > 0 && level_ <= 2) {
@@ -523,7 +530,7 @@ class Engine {
       lsmooth_S_.reserve(max_designs * 29 * 29);
       lsmooth_info_.reserve(max_designs * 4);
     }
-    if (level_ > 2) {
+    if (level_ == 1 || level_ > 2) {
       usmooth_X_.reserve((size_t)p_ * nx);
       usmooth_S_.reserve((size_t)p_ * 29 * 29);
       usmooth_info_.reserve((size_t)p_ * 4);
@@ -621,7 +628,7 @@ class Engine {
     LaneWork W[2];
     // above level 2 the smooths are univariate and shared between sets: they
     // run on lane 0 first and lane 1 waits for them
-    const bool uni = level_ > 2;
+    const bool uni = uni_smooths();
     std::vector<int> shared_smooth;
     std::vector<unsigned long long> seeds;
     for (size_t ti = 0; ti < tasks.size(); ++ti) {
@@ -1226,7 +1233,7 @@ class Engine {
     d_smooth_items_.upload(smooth_items, stream_);
     d_geo_items_.upload(geo_items, stream_);
     const int nsm = (int)smooth_items.size() / 4;
-    const bool uni = level_ > 2;
+    const bool uni = uni_smooths();
     if (nsm > 0) {
       const long long stride = kpc::smooth_scratch_doubles(n_);
       int grid = std::min(nsm, grid_limit(stride, 512));
@@ -1277,7 +1284,7 @@ class Engine {
     kpc::GeometryItem gi;
     std::memset(&gi, 0, sizeof(gi));
     gi.design_slot = ds;
-    if (level_ <= 2) {
+    if (!uni_smooths()) {
       const int ss = n_lsmooth_++;
       smooth_items.push_back(level_);
       smooth_items.push_back(t.S[0]);
@@ -1327,7 +1334,7 @@ class Engine {
     StatusReadBack rb;
     size_t off = base;
     if (nsm > 0) {
-      rb.smooth_uni = level_ > 2;
+      rb.smooth_uni = uni_smooths();
       rb.smooth_n = (rb.smooth_uni ? (size_t)p_ : (size_t)n_lsmooth_) * 4;
       rb.smooth_off = off;
       off += aligned256(rb.smooth_n * sizeof(int));
@@ -1437,6 +1444,7 @@ class Engine {
   EngineHipResources* res_ = nullptr;
   hipStream_t stream_ = nullptr, stream2_ = nullptr;
   int lanes_ = 2;
+  bool share_uni_l1_ = true;  // level-1 smooths in the per-column cache
   double* d_data_ = nullptr;
   size_t data_bytes_ = 0;
   hipEvent_t ev_[8];
@@ -1794,6 +1802,7 @@ void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
         ++r->level_rounds[ord];
         if (plain_now) {
           // one launch sequence per test: fresh caches for every test
+          eng.set_share_univariate(false);
           eng.begin_level(ord);
           const double tp0 = now_s();
           eng.evaluate(batch);
@@ -1805,6 +1814,7 @@ void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
             plain_now = false;  // rest of the level runs batched
             chunk = batched_chunk;
             speculate_all = chunk >= (1 << 30);
+            eng.set_share_univariate(true);
             eng.begin_level(ord);
           }
         } else {
