@@ -336,8 +336,10 @@ struct EngineHipResources {
   hipStream_t stream = nullptr, stream2 = nullptr;
   hipEvent_t ev[11] = {};
   hipEvent_t ev2[6] = {};
+  hipEvent_t evp = nullptr;  // end of the univariate smooths built ahead
   PinnedBuf up[2];
   PinnedBuf down;
+  PinnedBuf pre;  // their work list
 };
 thread_local std::deque<EngineHipResources>* g_engine_res = nullptr;
 
@@ -356,6 +358,7 @@ EngineHipResources* acquire_engine_resources() {
   hip_check(hipStreamCreate(&r.stream2), "hipStreamCreate");
   for (int e = 0; e < 11; ++e) hip_check(hipEventCreate(&r.ev[e]), "hipEventCreate");
   for (int e = 0; e < 6; ++e) hip_check(hipEventCreate(&r.ev2[e]), "hipEventCreate");
+  hip_check(hipEventCreate(&r.evp), "hipEventCreate");
   r.in_use = true;
   g_engine_res->push_back(r);
   return &g_engine_res->back();
@@ -523,6 +526,45 @@ class Engine {
   void set_share_univariate(bool v) { share_uni_l1_ = v; }
   bool uni_smooths() const { return level_ > 2 || (level_ == 1 && share_uni_l1_); }
 
+  // Builds every column's univariate smooth ahead, on lane 1, while level 0
+  // runs on lane 0 (they depend on the data alone).  A design claims one in
+  // ensure_design; only claimed smooths are checked (a failed set-up is an
+  // error only where a design uses it), and the first wave that claims one
+  // orders lane 0 after this launch.  Its work list and scratch stay with the
+  // engine until it is destroyed.
+  void prefetch_univariate() {
+    if (uni_slot_.empty()) uni_slot_.assign(p_, -1);
+    uni_ready_.assign(p_, 0);
+    std::vector<int> items;
+    for (int c = 0; c < p_; ++c)
+      if (uni_slot_[c] < 0) {
+        items.insert(items.end(), {1, c, c, c});
+        uni_ready_[c] = 1;
+      }
+    if (items.empty()) return;
+    const size_t nx = (size_t)n_ * 29;
+    usmooth_X_.reserve((size_t)p_ * nx);
+    usmooth_S_.reserve((size_t)p_ * 29 * 29);
+    usmooth_info_.reserve((size_t)p_ * 4);
+    PinnedBuf& pb = res_->pre;
+    pb.reserve(items.size() * sizeof(int));
+    std::memcpy(pb.ptr, items.data(), items.size() * sizeof(int));
+    pre_items_.reserve(items.size());
+    hip_check(hipMemcpyAsync(pre_items_.ptr, pb.ptr, items.size() * sizeof(int),
+                             hipMemcpyHostToDevice, stream2_),
+              "hipMemcpyAsync H2D");
+    const int nsm = (int)items.size() / 4;
+    const long long stride = kpc::smooth_scratch_doubles(n_);
+    const int grid = std::min(nsm, grid_limit(stride, 512));
+    pre_scratch_.reserve((size_t)grid * stride);
+    hip_check(kpc::launch_smooth_setup(d_data_, n_, p_, pre_items_.ptr, nsm, usmooth_X_.ptr,
+                                       usmooth_S_.ptr, usmooth_info_.ptr, pre_scratch_.ptr, grid,
+                                       stream2_),
+              "smooth setup launch");
+    hip_check(hipEventRecord(res_->evp, stream2_), "event");
+    prefetch_pending_ = true;
+  }
+
   void reserve_level(size_t max_designs, size_t max_resid) {
     const size_t nx = (size_t)n_ * 29;
     if (level_ > 0 && level_ <= 2) {
@@ -561,6 +603,7 @@ class Engine {
     std::vector<Task*> tasks;
     std::vector<std::pair<Task*, size_t>> dups;  // (task, index into tasks)
     in_batch_.clear();
+    wave_claimed_ = false;
     pair_rtab_.clear();
     for (Task* t : all) {
       if (use_results_ && with_pairs) {
@@ -736,6 +779,11 @@ class Engine {
       if (li == 0) put(d_seeds_, seeds);
       L.pair_out->reserve(w.pair_x.size() * kpc::kPairOut);
     }
+    // designs on smooths built ahead on lane 1: lane 0 follows that launch
+    if (wave_claimed_ && prefetch_pending_) {
+      hip_check(hipStreamWaitEvent(stream_, res_->evp, 0), "stream wait");
+      prefetch_pending_ = false;
+    }
     for (int li = 0; li < nlanes; ++li) {
       const Lane L = lane(li);
       LaneWork& w = W[li];
@@ -820,7 +868,7 @@ class Engine {
       dst = &dst2;
     }
     const bool hsic = method_ == KPC_HSIC_GAMMA || method_ == KPC_HSIC_PERM;
-    const StatusReadBack rb = plan_status(*fits, *geos, W[0].nsm + W[1].nsm, dst, true,
+    const StatusReadBack rb = plan_status(*fits, *geos, W[0].nsm + W[1].nsm + (wave_claimed_ ? 1 : 0), dst, true,
                                           with_pairs && hsic, 0);
     size_t out_off[2] = {0, 0};
     size_t down_end = rb.end;
@@ -1297,10 +1345,15 @@ class Engine {
         if (uni_slot_.empty()) uni_slot_.assign(p_, -1);
         if (uni_slot_[col] < 0) {
           uni_slot_[col] = col;
-          smooth_items.push_back(1);
-          smooth_items.push_back(col);
-          smooth_items.push_back(col);
-          smooth_items.push_back(col);
+          if (!uni_ready_.empty() && uni_ready_[col]) {
+            uni_ready_[col] = 0;  // built ahead
+            wave_claimed_ = true;
+          } else {
+            smooth_items.push_back(1);
+            smooth_items.push_back(col);
+            smooth_items.push_back(col);
+            smooth_items.push_back(col);
+          }
         }
         gi.smooth_slot[i] = col;
       }
@@ -1445,6 +1498,11 @@ class Engine {
   hipStream_t stream_ = nullptr, stream2_ = nullptr;
   int lanes_ = 2;
   bool share_uni_l1_ = true;  // level-1 smooths in the per-column cache
+  std::vector<char> uni_ready_;  // univariate smooths built ahead, not yet claimed
+  bool prefetch_pending_ = false;  // lane 0 not yet ordered after that launch
+  bool wave_claimed_ = false;      // the current wave claimed one (status read-back)
+  DevArray<int> pre_items_;
+  DevArray<double> pre_scratch_;
   double* d_data_ = nullptr;
   size_t data_bytes_ = 0;
   hipEvent_t ev_[8];
@@ -1609,6 +1667,8 @@ void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
   const bool plain = opt.mode == KPC_PLAIN;
   long long plain_tests = 0;
   (void)plain_tests;
+  // level 1's univariate smooths, built on lane 1 while level 0 runs
+  if (!plain && mmax >= 1) eng.prefetch_univariate();
   while (!done && ord <= mmax) {
     bool any = false;
     for (int i = 0; i < p * p; ++i) any |= r->adjacency[i] != 0;
