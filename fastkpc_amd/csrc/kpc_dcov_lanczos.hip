@@ -344,11 +344,18 @@ __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
     // distance matrices converge there for n from 200 to 2000 (one check per
     // component); later checks follow the observed residual decay
     int next_check = 2 * k + 6 < mmax ? 2 * k + 6 : mmax, prev_m = 0;
+    // n <= blockDim: thread t owns row t in every per-row loop of a step (the
+    // matvec's runs are single rows), so the barriers after the matvec and
+    // after the normalisation guard nothing: alpha's partial is formed in the
+    // matvec loop (the same product), and every cross-row read is behind the
+    // block sums' barriers
+    const bool own_rows = n <= (int)blockDim.x;
     double prev_worst = 0.0;
     double tmax = 0.0;  // running max of |alpha_j| + |beta_j| (scale of T)
     if (tid == 0) { ictl[1] = 0; ictl[2] = 2; }
     __syncthreads();
     for (int j = 0; j < mmax; ++j) {
+      double apart = 0.0;  // alpha's partial sum
       // z = D q_j on the sorted residual: with exclusive prefix sums
       // P_i = sum_{l<i} q_l and X_i = sum_{l<i} x_l q_l and totals Tq, Tx,
       // (D q)_i = x_i (2 P_i - Tq) - 2 X_i + Tx.  A thread scans a run of
@@ -386,15 +393,19 @@ __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
         double pv = bv + ev, px = bx + ex;
         for (int i = i0; i < i1; ++i) {
           const double2 e = rq[i];
-          z[i] = e.x * (2.0 * pv - tq) - 2.0 * px + tx;
+          const double zi = e.x * (2.0 * pv - tq) - 2.0 * px + tx;
+          z[i] = zi;
+          if (own_rows) apart += e.y * zi;
           pv += e.y;
           px += e.x * e.y;
         }
       }
-      __syncthreads();
-      double part = 0.0;
-      for (int i = tid; i < n; i += blockDim.x) part += rq[i].y * z[i];
-      const double aj = block_sum_alt(part, rsum, par);
+      if (!own_rows) {
+        __syncthreads();
+        for (int i = tid; i < n; i += blockDim.x) apart += rq[i].y * z[i];
+      }
+      const double aj = block_sum_alt(apart, rsum, par);
+      double part;
       // three-term recurrence
       const double bprev = j > 0 ? be[j - 1] : 0.0;
       const double* qp = Q + (size_t)(j > 0 ? j - 1 : 0) * n;
@@ -467,9 +478,9 @@ __global__ __launch_bounds__(kLzThreads, 4) void dcov_lanczos_kernel(
         Q[(size_t)(j + 1) * n + i] = v;
         if (j + 1 < ncache) qc[(size_t)(j + 1) * n + i] = v;
       }
-      __syncthreads();
       const int m = j + 1;
       const bool check = m >= next_check || m == mmax || breakdown;
+      if (!own_rows || check) __syncthreads();
       if (!check) continue;
       // Gershgorin bounds of T
       for (int i = tid; i < m - 1; i += blockDim.x) e2[i] = be[i] * be[i];
