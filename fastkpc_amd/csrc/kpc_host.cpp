@@ -231,19 +231,25 @@ class FlatMap {
     size_ = 0;
   }
   size_t size() const { return size_; }
+  // Room for n keys without growing (capacity a power of two >= 2 n).
+  void reserve(size_t n) {
+    size_t cap = used_.empty() ? 64 : used_.size();
+    while (cap < 2 * n) cap *= 2;
+    if (cap > used_.size()) rehash(cap);
+  }
 
  private:
   size_t slot(const K& k) const {
     return (size_t)(((uint64_t)H()(k) * 0x9E3779B97F4A7C15ULL) >> shift_);
   }
-  void grow() {
+  void grow() { rehash(used_.empty() ? 64 : 2 * used_.size()); }
+  void rehash(size_t cap) {
     std::vector<K> ok;
     std::vector<V> ov;
     std::vector<unsigned char> ou;
     ok.swap(keys_);
     ov.swap(vals_);
     ou.swap(used_);
-    const size_t cap = ou.empty() ? 64 : 2 * ou.size();
     keys_.resize(cap);
     vals_.resize(cap);
     used_.assign(cap, 0);
@@ -566,6 +572,9 @@ class Engine {
   }
 
   void reserve_level(size_t max_designs, size_t max_resid) {
+    design_map_.reserve(max_designs);
+    resid_map_.reserve(max_resid);
+    comp_map_.reserve(max_resid);
     const size_t nx = (size_t)n_ * 29;
     if (level_ > 0 && level_ <= 2) {
       lsmooth_X_.reserve(max_designs * nx);
@@ -603,6 +612,8 @@ class Engine {
     std::vector<Task*> tasks;
     std::vector<std::pair<Task*, size_t>> dups;  // (task, index into tasks)
     in_batch_.clear();
+    in_batch_.reserve(all.size());
+    twin_map_.reserve(twin_map_.size() + all.size());
     wave_claimed_ = false;
     pair_rtab_.clear();
     for (Task* t : all) {
@@ -643,6 +654,7 @@ class Engine {
     int nlanes = 1;
     if (lanes_ > 1 && !graph && !perm && level_ > 0 && tasks.size() >= 64) {
       lane_map_.clear();
+      lane_map_.reserve(tasks.size());
       int distinct = 0;
       for (size_t i = 0; i < tasks.size(); ++i) {
         if (const int* l = lane_map_.find(tasks[i]->mask)) {
@@ -1682,6 +1694,20 @@ void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
     std::vector<Task> tasks;
     std::vector<EdgeRun> edges;
     int nb[64], idx[64];
+    {
+      // the plan's size first (sum over orientations of C(neighbours, ord)),
+      // so the task vector is allocated once
+      double count = 0.0;
+      for (int a = 0; a < p; ++a) {
+        int deg = 0;
+        for (int i = 0; i < p; ++i) deg += i != a && snap[(size_t)i * p + a];
+        if (deg - 1 < ord) continue;
+        double c = 1.0;
+        for (int i = 0; i < ord; ++i) c = c * (double)(deg - 1 - i) / (double)(i + 1);
+        count += deg * c;
+      }
+      if (count < 1e7) tasks.reserve((size_t)count);
+    }
     for (int a = 0; a < p - 1; ++a)
       for (int b = a + 1; b < p; ++b) {
         if (!snap[(size_t)a * p + b]) continue;
@@ -1727,6 +1753,8 @@ void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
     {
       FlatMap<uint64_t, int, U64Hash> ud;
       FlatMap<std::pair<int, uint64_t>, int, KeyHash> ur;
+      ud.reserve(tasks.size());
+      ur.reserve(2 * tasks.size());
       for (const Task& t : tasks) {
         ud.emplace(t.mask, 0);
         ur.emplace(std::make_pair(t.x, t.mask), 0);
