@@ -500,6 +500,7 @@ class Engine {
   // ---- per-level arenas / caches -------------------------------------
   void begin_level(int level) {
     level_ = level;
+    l1_claim_ = false;
     design_map_.clear();
     resid_map_.clear();
     comp_map_.clear();
@@ -532,13 +533,16 @@ class Engine {
   void set_share_univariate(bool v) { share_uni_l1_ = v; }
   bool uni_smooths() const { return level_ > 2 || (level_ == 1 && share_uni_l1_); }
 
-  // Builds every column's univariate smooth ahead, on lane 1, while level 0
-  // runs on lane 0 (they depend on the data alone).  A design claims one in
-  // ensure_design; only claimed smooths are checked (a failed set-up is an
-  // error only where a design uses it), and the first wave that claims one
-  // orders lane 0 after this launch.  Its work list and scratch stay with the
-  // engine until it is destroyed.
-  void prefetch_univariate() {
+  // Built ahead on lane 1 while level 0 runs on lane 0 (all of it depends on
+  // the data alone): every column's univariate smooth and, outside the R
+  // stream, the caches and graph capture, level 1's designs (S = {s}, design
+  // slot s) and residual fits ((t, {s}), residual slot t p + s).  Level 1
+  // claims what its tests use (ensure_design, evaluate): only claimed work is
+  // checked (a failed set-up or fit is an error only where a test uses it) and
+  // counted, and the first wave that claims anything orders lane 0 after this
+  // chain.  Work lists, slabs and the level-1 arenas are sized here once;
+  // level 1's own reservations never exceed them.
+  void prefetch_ahead() {
     if (uni_slot_.empty()) uni_slot_.assign(p_, -1);
     uni_ready_.assign(p_, 0);
     std::vector<int> items;
@@ -548,16 +552,36 @@ class Engine {
         uni_ready_[c] = 1;
       }
     if (items.empty()) return;
+    const int q1 = 10, m1 = 1, dim1 = 9, rk1 = 8;
+    const bool with_level1 =
+        !rmode_ && !use_targets_ && !use_results_ && !use_graph_ && n_ > q1 && (int)items.size() == 4 * p_;
+    std::vector<kpc::GeometryItem> geo;
+    std::vector<kpc::MagicItem> fits;
+    if (with_level1) {
+      for (int c = 0; c < p_; ++c) {
+        kpc::GeometryItem gi;
+        std::memset(&gi, 0, sizeof(gi));
+        gi.design_slot = c;
+        gi.smooth_slot[0] = c;
+        geo.push_back(gi);
+      }
+      for (int t = 0; t < p_; ++t)
+        for (int c = 0; c < p_; ++c)
+          if (t != c) fits.push_back(kpc::MagicItem{c, t, t * p_ + c, 0});
+    }
     const size_t nx = (size_t)n_ * 29;
     usmooth_X_.reserve((size_t)p_ * nx);
     usmooth_S_.reserve((size_t)p_ * 29 * 29);
     usmooth_info_.reserve((size_t)p_ * 4);
+    // one page-locked block: the smooth, design and fit work lists
+    const size_t b0 = items.size() * sizeof(int), b1 = geo.size() * sizeof(kpc::GeometryItem),
+                 b2 = fits.size() * sizeof(kpc::MagicItem);
+    const size_t o1 = aligned256(b0), o2 = o1 + aligned256(b1);
     PinnedBuf& pb = res_->pre;
-    pb.reserve(items.size() * sizeof(int));
-    std::memcpy(pb.ptr, items.data(), items.size() * sizeof(int));
+    pb.reserve(o2 + aligned256(b2));
+    std::memcpy(pb.ptr, items.data(), b0);
     pre_items_.reserve(items.size());
-    hip_check(hipMemcpyAsync(pre_items_.ptr, pb.ptr, items.size() * sizeof(int),
-                             hipMemcpyHostToDevice, stream2_),
+    hip_check(hipMemcpyAsync(pre_items_.ptr, pb.ptr, b0, hipMemcpyHostToDevice, stream2_),
               "hipMemcpyAsync H2D");
     const int nsm = (int)items.size() / 4;
     const long long stride = kpc::smooth_scratch_doubles(n_);
@@ -567,8 +591,48 @@ class Engine {
                                        usmooth_S_.ptr, usmooth_info_.ptr, pre_scratch_.ptr, grid,
                                        stream2_),
               "smooth setup launch");
+    if (with_level1) {
+      const kpc::DesignLayout lay = kpc::host_design_layout(n_, q1, m1, rk1);
+      design_.reserve((size_t)p_ * (size_t)lay.stride);
+      design_status_.reserve(p_);
+      resid_.reserve((size_t)p_ * p_ * n_);
+      rinfo_.reserve((size_t)p_ * p_ * kpc::kMagicInfo);
+      std::memcpy(pb.ptr + o1, geo.data(), b1);
+      std::memcpy(pb.ptr + o2, fits.data(), b2);
+      pre_geo_items_.reserve(geo.size());
+      pre_fit_items_.reserve(fits.size());
+      hip_check(hipMemcpyAsync(pre_geo_items_.ptr, pb.ptr + o1, b1, hipMemcpyHostToDevice, stream2_),
+                "hipMemcpyAsync H2D");
+      hip_check(hipMemcpyAsync(pre_fit_items_.ptr, pb.ptr + o2, b2, hipMemcpyHostToDevice, stream2_),
+                "hipMemcpyAsync H2D");
+      const long long gstride = kpc::geometry_scratch_doubles(q1);
+      const int ggrid = std::min((int)geo.size(), grid_limit(gstride, 2048));
+      pre_scratch2_.reserve((size_t)ggrid * gstride);
+      hip_check(kpc::launch_geometry(pre_geo_items_.ptr, (int)geo.size(), n_, q1, m1, dim1, rk1,
+                                     usmooth_X_.ptr, usmooth_S_.ptr, design_.ptr,
+                                     design_status_.ptr, (long long)n_ * 29, pre_scratch2_.ptr,
+                                     ggrid, stream2_),
+                "geometry launch");
+      const long long mstride = kpc::magic_scratch_doubles(q1, m1, rk1, n_);
+      const int mgrid = std::min((int)fits.size(), grid_limit(mstride, 65535));
+      pre_scratch3_.reserve((size_t)mgrid * mstride);
+      hip_check(kpc::launch_magic(pre_fit_items_.ptr, (int)fits.size(), n_, q1, m1, rk1, d_data_,
+                                  design_.ptr, resid_.ptr, rinfo_.ptr, pre_scratch3_.ptr, mgrid,
+                                  ctl_, stream2_, nullptr),
+                "magic launch");
+      l1_ahead_ = true;
+    }
     hip_check(hipEventRecord(res_->evp, stream2_), "event");
     prefetch_pending_ = true;
+  }
+
+  // Level 1 with its designs and fits built ahead: slots are the fixed ones
+  // of prefetch_ahead (design s, residual t p + s), none is allocated.
+  void adopt_level1() {
+    if (!l1_ahead_ || level_ != 1) return;
+    l1_claim_ = true;
+    n_design_ = p_;
+    n_resid_ = p_ * p_;
   }
 
   void reserve_level(size_t max_designs, size_t max_resid) {
@@ -615,6 +679,8 @@ class Engine {
     in_batch_.reserve(all.size());
     twin_map_.reserve(twin_map_.size() + all.size());
     wave_claimed_ = false;
+    claimed_geos_.clear();
+    claimed_fits_.clear();
     pair_rtab_.clear();
     for (Task* t : all) {
       if (use_results_ && with_pairs) {
@@ -713,6 +779,16 @@ class Engine {
           continue;
         }
         int dslot = ensure_design(*t, smooth_items, w.geo_items);
+        if (l1_claim_) {
+          // fit built ahead at residual slot t p + s
+          const int rs = tgt * p_ + t->S[0];
+          resid_map_.set(key, rs);
+          rslot[side] = rs;
+          claimed_fits_.push_back(kpc::MagicItem{dslot, tgt, rs, 0});
+          wave_claimed_ = true;
+          w.comp_src.push_back((long long)rs * n_);
+          continue;
+        }
         const int rs = n_resid_++;
         resid_map_.set(key, rs);
         rslot[side] = rs;
@@ -746,6 +822,8 @@ class Engine {
       stats_.components += (long long)W[li].comp_dst.size();
       stats_.setups += (long long)W[li].geo_items.size();
     }
+    stats_.fits += (long long)claimed_fits_.size();
+    stats_.setups += (long long)claimed_geos_.size();
     if (with_pairs) stats_.physical_tests += (long long)tasks.size();
     stage_host_ += now_s() - t0;
     const double* comp_source = level_ == 0 ? d_data_ : resid_.ptr;
@@ -868,16 +946,20 @@ class Engine {
     std::vector<kpc::MagicItem> fits2;
     std::vector<kpc::GeometryItem> geos2;
     std::vector<int> dst2;
-    if (nlanes > 1) {
+    if (nlanes > 1 || !claimed_fits_.empty() || !claimed_geos_.empty()) {
       fits2 = W[0].fit_items;
-      fits2.insert(fits2.end(), W[1].fit_items.begin(), W[1].fit_items.end());
       geos2 = W[0].geo_items;
-      geos2.insert(geos2.end(), W[1].geo_items.begin(), W[1].geo_items.end());
-      dst2 = W[0].comp_dst;
-      dst2.insert(dst2.end(), W[1].comp_dst.begin(), W[1].comp_dst.end());
+      if (nlanes > 1) {
+        fits2.insert(fits2.end(), W[1].fit_items.begin(), W[1].fit_items.end());
+        geos2.insert(geos2.end(), W[1].geo_items.begin(), W[1].geo_items.end());
+        dst2 = W[0].comp_dst;
+        dst2.insert(dst2.end(), W[1].comp_dst.begin(), W[1].comp_dst.end());
+        dst = &dst2;
+      }
+      fits2.insert(fits2.end(), claimed_fits_.begin(), claimed_fits_.end());
+      geos2.insert(geos2.end(), claimed_geos_.begin(), claimed_geos_.end());
       fits = &fits2;
       geos = &geos2;
-      dst = &dst2;
     }
     const bool hsic = method_ == KPC_HSIC_GAMMA || method_ == KPC_HSIC_PERM;
     const StatusReadBack rb = plan_status(*fits, *geos, W[0].nsm + W[1].nsm + (wave_claimed_ ? 1 : 0), dst, true,
@@ -1339,6 +1421,22 @@ class Engine {
   int ensure_design(const Task& t, std::vector<int>& smooth_items,
                     std::vector<kpc::GeometryItem>& geo_items) {
     if (const int* it = design_map_.find(t.mask)) return *it;
+    if (l1_claim_) {
+      // built ahead: design slot s, on column s's univariate smooth
+      const int c = t.S[0];
+      design_map_.set(t.mask, c);
+      if (uni_slot_[c] < 0) {
+        uni_slot_[c] = c;
+        uni_ready_[c] = 0;
+      }
+      wave_claimed_ = true;
+      kpc::GeometryItem gi;
+      std::memset(&gi, 0, sizeof(gi));
+      gi.design_slot = c;
+      gi.smooth_slot[0] = c;
+      claimed_geos_.push_back(gi);
+      return c;
+    }
     const int ds = n_design_++;
     design_map_.set(t.mask, ds);
     kpc::GeometryItem gi;
@@ -1515,6 +1613,13 @@ class Engine {
   bool wave_claimed_ = false;      // the current wave claimed one (status read-back)
   DevArray<int> pre_items_;
   DevArray<double> pre_scratch_;
+  DevArray<kpc::GeometryItem> pre_geo_items_;
+  DevArray<kpc::MagicItem> pre_fit_items_;
+  DevArray<double> pre_scratch2_, pre_scratch3_;
+  bool l1_ahead_ = false;  // level 1's designs and fits built ahead
+  bool l1_claim_ = false;  // this level claims them (level 1, adopted)
+  std::vector<kpc::GeometryItem> claimed_geos_;  // the wave's claimed designs
+  std::vector<kpc::MagicItem> claimed_fits_;     // and fits (status read-back)
   double* d_data_ = nullptr;
   size_t data_bytes_ = 0;
   hipEvent_t ev_[8];
@@ -1679,8 +1784,8 @@ void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
   const bool plain = opt.mode == KPC_PLAIN;
   long long plain_tests = 0;
   (void)plain_tests;
-  // level 1's univariate smooths, built on lane 1 while level 0 runs
-  if (!plain && mmax >= 1) eng.prefetch_univariate();
+  // level 1's smooths, designs and fits, built on lane 1 while level 0 runs
+  if (!plain && mmax >= 1) eng.prefetch_ahead();
   while (!done && ord <= mmax) {
     bool any = false;
     for (int i = 0; i < p * p; ++i) any |= r->adjacency[i] != 0;
@@ -1763,6 +1868,7 @@ void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
       eng.reserve_level(ud.size(), ur.size());
       level_unique_resid = ur.size();
     }
+    if (ord == 1 && !plain) eng.adopt_level1();
     if (eng.r_stream()) {
       // R-stream permutations: the tests run in pcalg's execution order
       // (orientations (x, y) by x, then y; each orientation's conditioning
