@@ -1,12 +1,13 @@
-"""Univariate thin-plate smooths built ahead while level 0 runs.
+"""Level-1 work built ahead while level 0 runs.
 
-Every column's univariate smooth (k = 10) is set up on the second launch lane
-during level 0, and level 1's designs (and later levels' additive terms)
-claim them.  A set-up fails when its column has fewer unique values than the
-basis size (mgcv's "fewer unique covariate combinations" error); such a
-failure must be reported exactly where a design uses the column, as when the
-smooth was built on demand: a column cut off at level 0 never raises, a
-column still adjacent at level 1 does.
+Every column's univariate smooth (k = 10), level 1's designs S = {s} and its
+residual fits (t | {s}) are computed on the second launch lane during level
+0; level 1's tests (and later levels' additive terms) claim them.  A set-up
+fails when its column has fewer unique values than the basis size (mgcv's
+"fewer unique covariate combinations" error); such a failure must be
+reported exactly where a design uses the column, as when the smooth was built
+on demand: a column cut off at level 0 never raises, a column still adjacent
+at level 1 does.  The plain form builds everything on demand and must agree.
 
 The data make the level-0 decisions certain: alpha = 1e-12 deletes every
 edge of an independent column (its dcov p-values cannot come that close to
