@@ -333,11 +333,30 @@ __global__ __launch_bounds__(kGeomThreads) void geometry_kernel(
         D[DL.tau + i] = tau;
       }
       __syncthreads();
-      // apply H_i to trailing columns
+      // apply H_i to trailing columns, a wave's columns j and j + nw side by
+      // side (each column's sums in the same order as alone)
       if (tau != 0.0) {
-        for (int j = i + 1 + wid; j < q; j += nw) {
+        const double* vi = QR + (size_t)i * n;
+        int j = i + 1 + wid;
+        for (; j + nw < q; j += 2 * nw) {
+          double* c0 = QR + (size_t)j * n;
+          double* c1 = QR + (size_t)(j + nw) * n;
+          double s0 = 0.0, s1 = 0.0;
+          for (int r = i + lane; r < n; r += KPC_WAVE) {
+            const double v = r == i ? 1.0 : vi[r];
+            s0 += v * c0[r];
+            s1 += v * c1[r];
+          }
+          s0 = wave_sum(s0) * tau;
+          s1 = wave_sum(s1) * tau;
+          for (int r = i + lane; r < n; r += KPC_WAVE) {
+            const double v = r == i ? 1.0 : vi[r];
+            c0[r] -= s0 * v;
+            c1[r] -= s1 * v;
+          }
+        }
+        if (j < q) {
           double* cj = QR + (size_t)j * n;
-          const double* vi = QR + (size_t)i * n;
           double s = 0.0;
           for (int r = i + lane; r < n; r += KPC_WAVE) s += (r == i ? 1.0 : vi[r]) * cj[r];
           s = wave_sum(s) * tau;
