@@ -351,19 +351,31 @@ __global__ __launch_bounds__(64 * kPairWaves) void dcov_pair_kernel(
     for (int a = 0; a < KT; ++a)
 #pragma unroll
       for (int b = 0; b < KT; ++b) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
-    double av[KT], bv[KT], an[KT], bn[KT];
+    // two 4-row steps per trip with the next two steps' loads in flight; the
+    // accumulation runs over the rows in order (a step past n adds exact
+    // zeros: its operands are zero-padded)
+    double av[KT], bv[KT], aw[KT], bw[KT], an[KT], bn[KT], ao[KT], bo[KT];
     pair_fragments<KT>(X, Y, n, k, g, c, av, bv);
-    for (int r0 = 0; r0 < n; r0 += 4) {
-      pair_fragments<KT>(X, Y, n, k, r0 + 4 + g, c, an, bn);
+    pair_fragments<KT>(X, Y, n, k, 4 + g, c, aw, bw);
+    for (int r0 = 0; r0 < n; r0 += 8) {
+      pair_fragments<KT>(X, Y, n, k, r0 + 8 + g, c, an, bn);
+      pair_fragments<KT>(X, Y, n, k, r0 + 12 + g, c, ao, bo);
 #pragma unroll
       for (int a = 0; a < KT; ++a)
 #pragma unroll
         for (int b = 0; b < KT; ++b)
           acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
 #pragma unroll
+      for (int a = 0; a < KT; ++a)
+#pragma unroll
+        for (int b = 0; b < KT; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(aw[a], bw[b], acc[a][b], 0, 0, 0);
+#pragma unroll
       for (int t = 0; t < KT; ++t) {
         av[t] = an[t];
         bv[t] = bn[t];
+        aw[t] = ao[t];
+        bw[t] = bo[t];
       }
     }
     double part = 0.0;
