@@ -645,6 +645,7 @@ hipError_t launch_dcov_lanczos(const double* src_base, const long long* src_off,
                                double* scratch, int grid, int mmax, double tol,
                                hipStream_t stream) {
   if (count <= 0) return hipSuccess;
+  if (n < 1 || n > kMaxN || k < 1 || k > kMaxNumCol) return hipErrorInvalidValue;
   const size_t lds = dcov_lanczos_lds_bytes(n, k, mmax);
   hipError_t err = hipFuncSetAttribute((const void*)dcov_lanczos_kernel,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
