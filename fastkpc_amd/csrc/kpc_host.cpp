@@ -343,9 +343,14 @@ struct EngineHipResources {
   hipEvent_t ev[11] = {};
   hipEvent_t ev2[6] = {};
   hipEvent_t evp = nullptr;  // end of the univariate smooths built ahead
+  // third stream: the next level's designs and fits built ahead (evs[0..1]
+  // bracket the geometry launch, evs[2] ends the chain)
+  hipStream_t stream3 = nullptr;
+  hipEvent_t evs[3] = {};
   PinnedBuf up[2];
   PinnedBuf down;
-  PinnedBuf pre;  // their work list
+  PinnedBuf pre;    // their work list
+  PinnedBuf ahead;  // the next level's work lists
 };
 thread_local std::deque<EngineHipResources>* g_engine_res = nullptr;
 
@@ -365,9 +370,20 @@ EngineHipResources* acquire_engine_resources() {
   for (int e = 0; e < 11; ++e) hip_check(hipEventCreate(&r.ev[e]), "hipEventCreate");
   for (int e = 0; e < 6; ++e) hip_check(hipEventCreate(&r.ev2[e]), "hipEventCreate");
   hip_check(hipEventCreate(&r.evp), "hipEventCreate");
+  hip_check(hipStreamCreate(&r.stream3), "hipStreamCreate");
+  for (int e = 0; e < 3; ++e) hip_check(hipEventCreate(&r.evs[e]), "hipEventCreate");
   r.in_use = true;
   g_engine_res->push_back(r);
   return &g_engine_res->back();
+}
+
+uint64_t mask_of(const int* S, int len);
+bool next_comb(int* idx, int k, int len);
+
+template <class T>
+void swap_arena(DevArray<T>& a, DevArray<T>& b) {
+  std::swap(a.ptr, b.ptr);
+  std::swap(a.cap, b.cap);
 }
 
 class Engine {
@@ -455,6 +471,7 @@ class Engine {
     res_ = acquire_engine_resources();
     stream_ = res_->stream;
     stream2_ = res_->stream2;
+    stream3_ = res_->stream3;
     for (int e = 0; e < 8; ++e) ev_[e] = res_->ev[e];
     for (int e = 0; e < 3; ++e) tev_[e] = res_->ev[8 + e];
     for (int e = 0; e < 6; ++e) ev2_[e] = res_->ev2[e];
@@ -466,6 +483,7 @@ class Engine {
     // arenas go back to the pool after this body: nothing may still use them
     if (stream_) (void)hipStreamSynchronize(stream_);
     if (stream2_) (void)hipStreamSynchronize(stream2_);
+    if (stream3_) (void)hipStreamSynchronize(stream3_);
     if (d_data_) pool_release(d_data_, data_bytes_);
     if (res_) res_->in_use = false;  // stream and events stay with the thread
   }
@@ -501,6 +519,7 @@ class Engine {
   void begin_level(int level) {
     level_ = level;
     l1_claim_ = false;
+    ahead_claim_ = false;
     design_map_.clear();
     resid_map_.clear();
     comp_map_.clear();
@@ -633,6 +652,169 @@ class Engine {
     l1_claim_ = true;
     n_design_ = p_;
     n_resid_ = p_ * p_;
+  }
+
+  // Level `next`'s additive designs and residual fits, built ahead on the
+  // third stream while the current level runs.  Adjacency only shrinks during
+  // a level, so every (target, S) that level `next` can fit comes from an edge
+  // x - y of the level-start graph `snap` with S a `next`-subset of x's other
+  // neighbours there; that superset is built when it has at most `limit` fits
+  // and every column's univariate smooth exists.  Level `next` claims what its
+  // tests use (adopt_ahead); unclaimed work is neither checked nor counted.
+  // The lists are launched behind the first wave's launches (launch_ahead).
+  void plan_ahead(const unsigned char* snap, int next, size_t limit) {
+    ahead_queued_ = false;
+    ahead_level_ = -1;
+    if (next < 3 || limit == 0 || rmode_ || use_targets_ || use_results_ || use_graph_) return;
+    const int q = 1 + 9 * next;
+    if (q >= n_ || q > kpc::kMaxQ || next > kpc::kMaxPen || next * 8 > q) return;
+    if (uni_slot_.size() != (size_t)p_ || uni_ready_.size() != (size_t)p_) return;
+    for (int c = 0; c < p_; ++c)
+      if (uni_slot_[c] < 0 && !uni_ready_[c]) return;  // a smooth not built
+    const int p = p_;
+    double count = 0.0;  // orientations x subsets (each gives at most two fits)
+    for (int a = 0; a < p; ++a) {
+      int deg = 0;
+      for (int i = 0; i < p; ++i) deg += i != a && snap[(size_t)i * p + a];
+      if (deg - 1 < next) continue;
+      double c = 1.0;
+      for (int i = 0; i < next; ++i) c = c * (double)(deg - 1 - i) / (double)(i + 1);
+      count += deg * c;
+    }
+    if (count == 0.0 || count > 4.0 * (double)limit) return;
+    ahead_geo_.clear();
+    ahead_fits_.clear();
+    ahead_design_map_.clear();
+    ahead_resid_map_.clear();
+    ahead_design_map_.reserve((size_t)count);
+    ahead_resid_map_.reserve(std::min((size_t)(2.0 * count), limit + 2));
+    int nb[64], idx[64], S[64];
+    for (int x = 0; x < p; ++x)
+      for (int y = 0; y < p; ++y) {
+        if (x == y || !snap[(size_t)y * p + x]) continue;
+        int len = 0;
+        for (int i = 0; i < p; ++i)
+          if (i != y && snap[(size_t)i * p + x]) nb[len++] = i;
+        if (len < next) continue;
+        for (int i = 0; i < next; ++i) idx[i] = i;
+        do {
+          for (int i = 0; i < next; ++i) S[i] = nb[idx[i]];
+          const uint64_t mask = mask_of(S, next);
+          int ds;
+          if (const int* d = ahead_design_map_.find(mask)) {
+            ds = *d;
+          } else {
+            ds = (int)ahead_geo_.size();
+            ahead_design_map_.set(mask, ds);
+            kpc::GeometryItem gi;
+            std::memset(&gi, 0, sizeof(gi));
+            gi.design_slot = ds;
+            for (int i = 0; i < next; ++i) gi.smooth_slot[i] = S[i];
+            ahead_geo_.push_back(gi);
+          }
+          for (int side = 0; side < 2; ++side) {
+            const auto key = std::make_pair(side ? y : x, mask);
+            if (ahead_resid_map_.find(key)) continue;
+            const int rs = (int)ahead_fits_.size();
+            if ((size_t)rs >= limit) return;
+            ahead_resid_map_.set(key, rs);
+            ahead_fits_.push_back(kpc::MagicItem{ds, key.first, rs, 0});
+          }
+        } while (next_comb(idx, next, len));
+      }
+    ahead_level_ = next;
+    ahead_queued_ = true;
+  }
+
+  // stage times of an adopted chain (its events are re-recorded by the next)
+  void account_ahead() {
+    if (!ahead_time_pending_) return;
+    ahead_time_pending_ = false;
+    hip_check(hipEventSynchronize(res_->evs[2]), "event synchronize");
+    float ms;
+    hip_check(hipEventElapsedTime(&ms, res_->evs[0], res_->evs[1]), "event elapsed");
+    stage_[1] += ms * 1e-3;
+    hip_check(hipEventElapsedTime(&ms, res_->evs[1], res_->evs[2]), "event elapsed");
+    stage_[2] += ms * 1e-3;
+    ++launches_[1];
+    ++launches_[2];
+  }
+
+  void launch_ahead() {
+    ahead_queued_ = false;
+    account_ahead();
+    // the previous chain's lists, arenas and slabs are reused below
+    if (ahead_launched_) hip_check(hipEventSynchronize(res_->evs[2]), "event synchronize");
+    const int next = ahead_level_;
+    const int q = 1 + 9 * next, m = next, dim = 9, rk = 8;
+    const kpc::DesignLayout lay = kpc::host_design_layout(n_, q, m, rk);
+    const size_t nd = ahead_geo_.size(), nf = ahead_fits_.size();
+    // room for as many of the level's own designs and fits again
+    ahead_design_.reserve(2 * nd * (size_t)lay.stride);
+    ahead_design_status_.reserve(2 * nd);
+    ahead_resid_.reserve(2 * nf * (size_t)n_);
+    ahead_rinfo_.reserve(2 * nf * kpc::kMagicInfo);
+    const size_t b1 = nd * sizeof(kpc::GeometryItem), b2 = nf * sizeof(kpc::MagicItem);
+    const size_t o2 = aligned256(b1);
+    PinnedBuf& pb = res_->ahead;
+    pb.reserve(o2 + aligned256(b2));
+    std::memcpy(pb.ptr, ahead_geo_.data(), b1);
+    std::memcpy(pb.ptr + o2, ahead_fits_.data(), b2);
+    ahead_geo_items_.reserve(nd);
+    ahead_fit_items_.reserve(nf);
+    // the univariate smooths built ahead on lane 1 (the others were built by
+    // waves already synchronised)
+    hip_check(hipStreamWaitEvent(stream3_, res_->evp, 0), "stream wait");
+    hip_check(hipMemcpyAsync(ahead_geo_items_.ptr, pb.ptr, b1, hipMemcpyHostToDevice, stream3_),
+              "hipMemcpyAsync H2D");
+    hip_check(hipMemcpyAsync(ahead_fit_items_.ptr, pb.ptr + o2, b2, hipMemcpyHostToDevice, stream3_),
+              "hipMemcpyAsync H2D");
+    hip_check(hipEventRecord(res_->evs[0], stream3_), "event");
+    const long long gstride = kpc::geometry_scratch_doubles(q);
+    const int ggrid = std::min((int)nd, grid_limit(gstride, 2048));
+    ahead_scratch2_.reserve((size_t)ggrid * gstride);
+    hip_check(kpc::launch_geometry(ahead_geo_items_.ptr, (int)nd, n_, q, m, dim, rk, usmooth_X_.ptr,
+                                   usmooth_S_.ptr, ahead_design_.ptr, ahead_design_status_.ptr,
+                                   (long long)n_ * 29, ahead_scratch2_.ptr, ggrid, stream3_),
+              "geometry launch");
+    hip_check(hipEventRecord(res_->evs[1], stream3_), "event");
+    const long long mstride = kpc::magic_scratch_doubles(q, m, rk, n_);
+    const int mgrid = std::min((int)nf, grid_limit(mstride, 65535));
+    ahead_scratch3_.reserve((size_t)mgrid * mstride);
+    hip_check(kpc::launch_magic(ahead_fit_items_.ptr, (int)nf, n_, q, m, rk, d_data_, ahead_design_.ptr,
+                                ahead_resid_.ptr, ahead_rinfo_.ptr, ahead_scratch3_.ptr, mgrid, ctl_,
+                                stream3_, nullptr),
+              "magic launch");
+    hip_check(hipEventRecord(res_->evs[2], stream3_), "event");
+    ahead_launched_ = true;
+    ahead_ready_ = true;
+  }
+
+  bool claims_ahead() const { return ahead_claim_; }
+
+  // Level `next` (this level) takes the arenas built ahead for it: design
+  // and residual slots of the superset first, the level's own after them.
+  void adopt_ahead(size_t max_designs, size_t max_resid) {
+    const bool ready = ahead_ready_ && ahead_level_ == level_ && level_ > 0;
+    ahead_ready_ = false;
+    if (!ready) return;
+    const size_t nd = ahead_geo_.size(), nf = ahead_fits_.size();
+    if ((nd + max_designs) * (size_t)layout_.stride > ahead_design_.cap ||
+        nd + max_designs > ahead_design_status_.cap || (nf + max_resid) * (size_t)n_ > ahead_resid_.cap ||
+        (nf + max_resid) * (size_t)kpc::kMagicInfo > ahead_rinfo_.cap)
+      return;
+    swap_arena(design_, ahead_design_);
+    swap_arena(design_status_, ahead_design_status_);
+    swap_arena(resid_, ahead_resid_);
+    swap_arena(rinfo_, ahead_rinfo_);
+    // the maps move too: the lists are refilled for the level after this one
+    std::swap(claim_design_map_, ahead_design_map_);
+    std::swap(claim_resid_map_, ahead_resid_map_);
+    n_design_ = (int)nd;
+    n_resid_ = (int)nf;
+    ahead_claim_ = true;
+    ahead_wait_ = true;
+    ahead_time_pending_ = true;
   }
 
   void reserve_level(size_t max_designs, size_t max_resid) {
@@ -779,6 +961,17 @@ class Engine {
           continue;
         }
         int dslot = ensure_design(*t, smooth_items, w.geo_items);
+        if (ahead_claim_) {
+          if (const int* ahead = claim_resid_map_.find(key)) {
+            // fit built ahead during the previous level
+            const int rs = *ahead;
+            resid_map_.set(key, rs);
+            rslot[side] = rs;
+            claimed_fits_.push_back(kpc::MagicItem{dslot, tgt, rs, 0});
+            w.comp_src.push_back((long long)rs * n_);
+            continue;
+          }
+        }
         if (l1_claim_) {
           // fit built ahead at residual slot t p + s
           const int rs = tgt * p_ + t->S[0];
@@ -874,6 +1067,12 @@ class Engine {
       hip_check(hipStreamWaitEvent(stream_, res_->evp, 0), "stream wait");
       prefetch_pending_ = false;
     }
+    // arenas built ahead on the third stream: both lanes follow that chain
+    if (ahead_wait_) {
+      hip_check(hipStreamWaitEvent(stream_, res_->evs[2], 0), "stream wait");
+      hip_check(hipStreamWaitEvent(stream2_, res_->evs[2], 0), "stream wait");
+      ahead_wait_ = false;
+    }
     for (int li = 0; li < nlanes; ++li) {
       const Lane L = lane(li);
       LaneWork& w = W[li];
@@ -937,6 +1136,8 @@ class Engine {
         hip_check(hipEventRecord(L.ev[5], L.stream), "event");
       }
     }
+    // the next level's work, queued behind this wave's launches
+    if (ahead_queued_) launch_ahead();
     // read-backs into the page-locked block, queued behind the kernels: each
     // lane's test outputs on its own stream, then on lane 0 (after lane 1's
     // last kernel) the level's status words; one synchronisation of the lanes
@@ -985,6 +1186,7 @@ class Engine {
       hip_check(hipStreamSynchronize(lane(li).stream), "wave synchronize");
     // stage times are summed over the lanes (they overlap in wall time)
     float ms;
+    account_ahead();
     for (int li = 0; li < nlanes; ++li) {
       const Lane L = lane(li);
       const LaneWork& w = W[li];
@@ -1421,6 +1623,29 @@ class Engine {
   int ensure_design(const Task& t, std::vector<int>& smooth_items,
                     std::vector<kpc::GeometryItem>& geo_items) {
     if (const int* it = design_map_.find(t.mask)) return *it;
+    if (ahead_claim_) {
+      if (const int* ahead = claim_design_map_.find(t.mask)) {
+        // built ahead during the previous level on these columns' smooths
+        const int ds = *ahead;
+        design_map_.set(t.mask, ds);
+        kpc::GeometryItem gi;
+        std::memset(&gi, 0, sizeof(gi));
+        gi.design_slot = ds;
+        for (int i = 0; i < t.slen; ++i) {
+          const int col = t.S[i];
+          if (uni_slot_[col] < 0) {
+            uni_slot_[col] = col;
+            if (uni_ready_[col]) {
+              uni_ready_[col] = 0;  // built ahead on lane 1
+              wave_claimed_ = true;
+            }
+          }
+          gi.smooth_slot[i] = col;
+        }
+        claimed_geos_.push_back(gi);
+        return ds;
+      }
+    }
     if (l1_claim_) {
       // built ahead: design slot s, on column s's univariate smooth
       const int c = t.S[0];
@@ -1618,6 +1843,23 @@ class Engine {
   DevArray<double> pre_scratch2_, pre_scratch3_;
   bool l1_ahead_ = false;  // level 1's designs and fits built ahead
   bool l1_claim_ = false;  // this level claims them (level 1, adopted)
+  // the next level's designs and fits built ahead on the third stream
+  hipStream_t stream3_ = nullptr;
+  int ahead_level_ = -1;         // the level the lists are for
+  bool ahead_queued_ = false;    // lists built, launch behind the next wave
+  bool ahead_launched_ = false;  // a chain was launched (evs[2] recorded)
+  bool ahead_ready_ = false;     // launched for ahead_level_, not yet adopted
+  bool ahead_claim_ = false;     // this level claims them
+  bool ahead_wait_ = false;      // lanes not yet ordered after the chain
+  std::vector<kpc::GeometryItem> ahead_geo_;
+  std::vector<kpc::MagicItem> ahead_fits_;
+  bool ahead_time_pending_ = false;  // an adopted chain's stage times
+  FlatMap<uint64_t, int, U64Hash> ahead_design_map_, claim_design_map_;
+  FlatMap<std::pair<int, uint64_t>, int, KeyHash> ahead_resid_map_, claim_resid_map_;
+  DevArray<double> ahead_design_, ahead_design_status_, ahead_resid_, ahead_rinfo_;
+  DevArray<kpc::GeometryItem> ahead_geo_items_;
+  DevArray<kpc::MagicItem> ahead_fit_items_;
+  DevArray<double> ahead_scratch2_, ahead_scratch3_;
   std::vector<kpc::GeometryItem> claimed_geos_;  // the wave's claimed designs
   std::vector<kpc::MagicItem> claimed_fits_;     // and fits (status read-back)
   double* d_data_ = nullptr;
@@ -1853,6 +2095,10 @@ void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
     size_t level_unique_resid = 0;
     size_t speculate_limit = 16384;
     if (const char* env = std::getenv("KPC_SPECULATE_LIMIT")) speculate_limit = std::atol(env);
+    // the next level is built ahead when its superset needs at most this many
+    // fits (KPC_AHEAD_LIMIT, 0 turns it off)
+    size_t ahead_limit = 512;
+    if (const char* env = std::getenv("KPC_AHEAD_LIMIT")) ahead_limit = std::atol(env);
     const Engine::Stats st0 = eng.stats_;
     eng.begin_level(ord);
     {
@@ -1867,8 +2113,15 @@ void run_skeleton(Engine& eng, const kpc_options& opt, kpc_result* r) {
       }
       eng.reserve_level(ud.size(), ur.size());
       level_unique_resid = ur.size();
+      if (!plain) eng.adopt_ahead(ud.size(), ur.size());
     }
     if (ord == 1 && !plain) eng.adopt_level1();
+    // the next level's designs and fits, built ahead from this level's graph
+    // behind this level's own GCV work (a level that claimed its fits from
+    // such a chain is too short to hide another one: if the skeleton ended
+    // there, its end would wait for the whole chain)
+    if (!plain && ord + 1 <= mmax && !eng.claims_ahead())
+      eng.plan_ahead(snap.data(), ord + 1, ahead_limit);
     if (eng.r_stream()) {
       // R-stream permutations: the tests run in pcalg's execution order
       // (orientations (x, y) by x, then y; each orientation's conditioning
