@@ -343,6 +343,7 @@ struct EngineHipResources {
   hipEvent_t ev[11] = {};
   hipEvent_t ev2[6] = {};
   hipEvent_t evp = nullptr;  // end of the univariate smooths built ahead
+  hipEvent_t evq[4] = {};    // stage boundaries of that chain (smooth, geometry, fits)
   // third stream: the next level's designs and fits built ahead (evs[0..1]
   // bracket the geometry launch, evs[2] ends the chain)
   hipStream_t stream3 = nullptr;
@@ -370,6 +371,7 @@ EngineHipResources* acquire_engine_resources() {
   for (int e = 0; e < 11; ++e) hip_check(hipEventCreate(&r.ev[e]), "hipEventCreate");
   for (int e = 0; e < 6; ++e) hip_check(hipEventCreate(&r.ev2[e]), "hipEventCreate");
   hip_check(hipEventCreate(&r.evp), "hipEventCreate");
+  for (int e = 0; e < 4; ++e) hip_check(hipEventCreate(&r.evq[e]), "hipEventCreate");
   hip_check(hipStreamCreate(&r.stream3), "hipStreamCreate");
   for (int e = 0; e < 3; ++e) hip_check(hipEventCreate(&r.evs[e]), "hipEventCreate");
   r.in_use = true;
@@ -606,10 +608,12 @@ class Engine {
     const long long stride = kpc::smooth_scratch_doubles(n_);
     const int grid = std::min(nsm, grid_limit(stride, 512));
     pre_scratch_.reserve((size_t)grid * stride);
+    hip_check(hipEventRecord(res_->evq[0], stream2_), "event");
     hip_check(kpc::launch_smooth_setup(d_data_, n_, p_, pre_items_.ptr, nsm, usmooth_X_.ptr,
                                        usmooth_S_.ptr, usmooth_info_.ptr, pre_scratch_.ptr, grid,
                                        stream2_),
               "smooth setup launch");
+    hip_check(hipEventRecord(res_->evq[1], stream2_), "event");
     if (with_level1) {
       const kpc::DesignLayout lay = kpc::host_design_layout(n_, q1, m1, rk1);
       design_.reserve((size_t)p_ * (size_t)lay.stride);
@@ -632,6 +636,7 @@ class Engine {
                                      design_status_.ptr, (long long)n_ * 29, pre_scratch2_.ptr,
                                      ggrid, stream2_),
                 "geometry launch");
+      hip_check(hipEventRecord(res_->evq[2], stream2_), "event");
       const long long mstride = kpc::magic_scratch_doubles(q1, m1, rk1, n_);
       const int mgrid = std::min((int)fits.size(), grid_limit(mstride, 65535));
       pre_scratch3_.reserve((size_t)mgrid * mstride);
@@ -639,10 +644,12 @@ class Engine {
                                   design_.ptr, resid_.ptr, rinfo_.ptr, pre_scratch3_.ptr, mgrid,
                                   ctl_, stream2_, nullptr),
                 "magic launch");
+      hip_check(hipEventRecord(res_->evq[3], stream2_), "event");
       l1_ahead_ = true;
     }
     hip_check(hipEventRecord(res_->evp, stream2_), "event");
     prefetch_pending_ = true;
+    prefetch_time_pending_ = true;
   }
 
   // Level 1 with its designs and fits built ahead: slots are the fixed ones
@@ -724,6 +731,26 @@ class Engine {
       }
     ahead_level_ = next;
     ahead_queued_ = true;
+  }
+
+  // stage times of the chain built during level 0, counted with level 1's
+  // first wave (the chain has ended by then: lane 1 ran it)
+  void account_prefetch() {
+    if (!prefetch_time_pending_ || level_ < 1) return;
+    prefetch_time_pending_ = false;
+    hip_check(hipEventSynchronize(res_->evp), "event synchronize");
+    float ms;
+    hip_check(hipEventElapsedTime(&ms, res_->evq[0], res_->evq[1]), "event elapsed");
+    stage_[0] += ms * 1e-3;
+    ++launches_[0];
+    if (l1_ahead_) {
+      hip_check(hipEventElapsedTime(&ms, res_->evq[1], res_->evq[2]), "event elapsed");
+      stage_[1] += ms * 1e-3;
+      hip_check(hipEventElapsedTime(&ms, res_->evq[2], res_->evq[3]), "event elapsed");
+      stage_[2] += ms * 1e-3;
+      ++launches_[1];
+      ++launches_[2];
+    }
   }
 
   // stage times of an adopted chain (its events are re-recorded by the next)
@@ -1187,6 +1214,7 @@ class Engine {
     // stage times are summed over the lanes (they overlap in wall time)
     float ms;
     account_ahead();
+    account_prefetch();
     for (int li = 0; li < nlanes; ++li) {
       const Lane L = lane(li);
       const LaneWork& w = W[li];
@@ -1835,6 +1863,7 @@ class Engine {
   bool share_uni_l1_ = true;  // level-1 smooths in the per-column cache
   std::vector<char> uni_ready_;  // univariate smooths built ahead, not yet claimed
   bool prefetch_pending_ = false;  // lane 0 not yet ordered after that launch
+  bool prefetch_time_pending_ = false;  // its stage times not yet counted
   bool wave_claimed_ = false;      // the current wave claimed one (status read-back)
   DevArray<int> pre_items_;
   DevArray<double> pre_scratch_;
