@@ -343,15 +343,17 @@ struct EngineHipResources {
   hipEvent_t ev[11] = {};
   hipEvent_t ev2[6] = {};
   hipEvent_t evp = nullptr;  // end of the univariate smooths built ahead
-  hipEvent_t evq[4] = {};    // stage boundaries of that chain (smooth, geometry, fits)
-  // third stream: the next level's designs and fits built ahead (evs[0..1]
-  // bracket the geometry launch, evs[2] ends the chain)
+  hipEvent_t evq[5] = {};    // stage boundaries of that chain (smooth, geometry, fits, components)
+  // third stream: the next level's designs, fits and components built ahead
+  // (evs[0..3] bound the geometry, GCV and component launches; evs[3] ends
+  // the chain)
   hipStream_t stream3 = nullptr;
-  hipEvent_t evs[3] = {};
+  hipEvent_t evs[4] = {};
   PinnedBuf up[2];
   PinnedBuf down;
   PinnedBuf pre;    // their work list
   PinnedBuf ahead;  // the next level's work lists
+  PinnedBuf chainc;  // a chain's component lists
 };
 thread_local std::deque<EngineHipResources>* g_engine_res = nullptr;
 
@@ -371,9 +373,9 @@ EngineHipResources* acquire_engine_resources() {
   for (int e = 0; e < 11; ++e) hip_check(hipEventCreate(&r.ev[e]), "hipEventCreate");
   for (int e = 0; e < 6; ++e) hip_check(hipEventCreate(&r.ev2[e]), "hipEventCreate");
   hip_check(hipEventCreate(&r.evp), "hipEventCreate");
-  for (int e = 0; e < 4; ++e) hip_check(hipEventCreate(&r.evq[e]), "hipEventCreate");
+  for (int e = 0; e < 5; ++e) hip_check(hipEventCreate(&r.evq[e]), "hipEventCreate");
   hip_check(hipStreamCreate(&r.stream3), "hipStreamCreate");
-  for (int e = 0; e < 3; ++e) hip_check(hipEventCreate(&r.evs[e]), "hipEventCreate");
+  for (int e = 0; e < 4; ++e) hip_check(hipEventCreate(&r.evs[e]), "hipEventCreate");
   r.in_use = true;
   g_engine_res->push_back(r);
   return &g_engine_res->back();
@@ -387,6 +389,13 @@ void swap_arena(DevArray<T>& a, DevArray<T>& b) {
   std::swap(a.ptr, b.ptr);
   std::swap(a.cap, b.cap);
 }
+
+// Component arenas (per component slot: k values, n x comp_k vectors, sums,
+// moment, status, margin; HSIC permutation rows)
+struct CompSet {
+  DevArray<double> vals, vecs, dsum, mom, margin, rows;
+  DevArray<int> status;
+};
 
 class Engine {
  public:
@@ -478,6 +487,7 @@ class Engine {
     for (int e = 0; e < 3; ++e) tev_[e] = res_->ev[8 + e];
     for (int e = 0; e < 6; ++e) ev2_[e] = res_->ev2[e];
     if (const char* env = std::getenv("KPC_LANES")) lanes_ = std::atoi(env) == 1 ? 1 : 2;
+    if (const char* env = std::getenv("KPC_AHEAD_COMPONENTS")) comps_ahead_ = std::atoi(env) != 0;
   }
   Engine(const Engine&) = delete;
   Engine& operator=(const Engine&) = delete;
@@ -522,6 +532,7 @@ class Engine {
     level_ = level;
     l1_claim_ = false;
     ahead_claim_ = false;
+    claim_comps_ = false;
     design_map_.clear();
     resid_map_.clear();
     comp_map_.clear();
@@ -646,6 +657,10 @@ class Engine {
                 "magic launch");
       hip_check(hipEventRecord(res_->evq[3], stream2_), "event");
       l1_ahead_ = true;
+      // their components at component slot = residual slot (t p + s), into
+      // the chain's component arenas (level 0 uses the engine's)
+      l1_comps_ = launch_chain_components(stream2_, fits, resid_.ptr, (size_t)2 * p_ * p_);
+      hip_check(hipEventRecord(res_->evq[4], stream2_), "event");
     }
     hip_check(hipEventRecord(res_->evp, stream2_), "event");
     prefetch_pending_ = true;
@@ -659,6 +674,11 @@ class Engine {
     l1_claim_ = true;
     n_design_ = p_;
     n_resid_ = p_ * p_;
+    if (l1_comps_ && comps_hold(chain_comp_, (size_t)2 * p_ * p_)) {
+      swap_comps(chain_comp_);
+      n_comp_ = p_ * p_;
+      claim_comps_ = true;
+    }
   }
 
   // Level `next`'s additive designs and residual fits, built ahead on the
@@ -733,6 +753,68 @@ class Engine {
     ahead_queued_ = true;
   }
 
+  // Component arenas of a chain built ahead (the engine's own serve the
+  // running level); swapped in by the level that claims them.
+  void reserve_comps(CompSet& c, size_t slots) {
+    c.vals.reserve(slots * k_);
+    c.vecs.reserve(slots * (size_t)n_ * comp_k_);
+    c.dsum.reserve(slots);
+    c.mom.reserve(slots);
+    c.status.reserve(slots);
+    c.margin.reserve(slots);
+    if (method_ == KPC_HSIC_PERM) c.rows.reserve(slots * (size_t)n_ * 16 * hsic_kt());
+  }
+  bool comps_hold(const CompSet& c, size_t slots) const {
+    return c.vals.cap >= slots * k_ && c.vecs.cap >= slots * (size_t)n_ * comp_k_ &&
+           c.dsum.cap >= slots && c.mom.cap >= slots && c.status.cap >= slots &&
+           c.margin.cap >= slots &&
+           (method_ != KPC_HSIC_PERM || c.rows.cap >= slots * (size_t)n_ * 16 * hsic_kt());
+  }
+  void swap_comps(CompSet& c) {
+    swap_arena(comp_vals_, c.vals);
+    swap_arena(comp_vecs_, c.vecs);
+    swap_arena(comp_dsum_, c.dsum);
+    swap_arena(comp_mom_, c.mom);
+    swap_arena(comp_status_, c.status);
+    swap_arena(comp_margin_, c.margin);
+    swap_arena(comp_rows_, c.rows);
+  }
+  // The components of a chain's fits (component slot = residual slot), read
+  // from `src`, into chain_comp_ with room for `slots`; returns whether they
+  // were launched (KPC_AHEAD_COMPONENTS=0 leaves them to the claiming level).
+  bool launch_chain_components(hipStream_t stream, const std::vector<kpc::MagicItem>& fits,
+                               const double* src, size_t slots) {
+    chain_comps_ = false;
+    if (!comps_ahead_ || fits.empty()) return false;
+    std::vector<long long> csrc(fits.size());
+    std::vector<int> cdst(fits.size());
+    for (size_t i = 0; i < fits.size(); ++i) {
+      csrc[i] = (long long)fits[i].resid_slot * n_;
+      cdst[i] = fits[i].resid_slot;
+    }
+    reserve_comps(chain_comp_, slots);
+    const size_t b1 = csrc.size() * sizeof(long long), b2 = cdst.size() * sizeof(int);
+    PinnedBuf& pb = res_->chainc;
+    pb.reserve(aligned256(b1) + aligned256(b2));
+    std::memcpy(pb.ptr, csrc.data(), b1);
+    std::memcpy(pb.ptr + aligned256(b1), cdst.data(), b2);
+    chain_comp_src_.reserve(csrc.size());
+    chain_comp_dst_.reserve(cdst.size());
+    hip_check(hipMemcpyAsync(chain_comp_src_.ptr, pb.ptr, b1, hipMemcpyHostToDevice, stream),
+              "hipMemcpyAsync H2D");
+    hip_check(hipMemcpyAsync(chain_comp_dst_.ptr, pb.ptr + aligned256(b1), b2, hipMemcpyHostToDevice,
+                             stream),
+              "hipMemcpyAsync H2D");
+    const Lane L{stream,           nullptr, nullptr, nullptr, nullptr, &chain_comp_src_,
+                 &chain_comp_dst_, nullptr, nullptr, nullptr, nullptr, nullptr,
+                 nullptr,          &chain_scratch4_, nullptr};
+    swap_comps(chain_comp_);
+    launch_components(L, src, fits.size());
+    swap_comps(chain_comp_);
+    chain_comps_ = true;
+    return true;
+  }
+
   // stage times of the chain built during level 0, counted with level 1's
   // first wave (the chain has ended by then: lane 1 ran it)
   void account_prefetch() {
@@ -750,6 +832,11 @@ class Engine {
       stage_[2] += ms * 1e-3;
       ++launches_[1];
       ++launches_[2];
+      if (l1_comps_) {
+        hip_check(hipEventElapsedTime(&ms, res_->evq[3], res_->evq[4]), "event elapsed");
+        stage_[3] += ms * 1e-3;
+        ++launches_[3];
+      }
     }
   }
 
@@ -757,7 +844,7 @@ class Engine {
   void account_ahead() {
     if (!ahead_time_pending_) return;
     ahead_time_pending_ = false;
-    hip_check(hipEventSynchronize(res_->evs[2]), "event synchronize");
+    hip_check(hipEventSynchronize(res_->evs[3]), "event synchronize");
     float ms;
     hip_check(hipEventElapsedTime(&ms, res_->evs[0], res_->evs[1]), "event elapsed");
     stage_[1] += ms * 1e-3;
@@ -765,13 +852,19 @@ class Engine {
     stage_[2] += ms * 1e-3;
     ++launches_[1];
     ++launches_[2];
+    if (chain_comps_) {
+      hip_check(hipEventElapsedTime(&ms, res_->evs[2], res_->evs[3]), "event elapsed");
+      stage_[3] += ms * 1e-3;
+      ++launches_[3];
+    }
   }
 
   void launch_ahead() {
     ahead_queued_ = false;
     account_ahead();
-    // the previous chain's lists, arenas and slabs are reused below
-    if (ahead_launched_) hip_check(hipEventSynchronize(res_->evs[2]), "event synchronize");
+    // the previous chains' lists, arenas and slabs are reused below
+    if (ahead_launched_) hip_check(hipEventSynchronize(res_->evs[3]), "event synchronize");
+    if (prefetch_pending_) hip_check(hipEventSynchronize(res_->evp), "event synchronize");
     const int next = ahead_level_;
     const int q = 1 + 9 * next, m = next, dim = 9, rk = 8;
     const kpc::DesignLayout lay = kpc::host_design_layout(n_, q, m, rk);
@@ -813,6 +906,8 @@ class Engine {
                                 stream3_, nullptr),
               "magic launch");
     hip_check(hipEventRecord(res_->evs[2], stream3_), "event");
+    launch_chain_components(stream3_, ahead_fits_, ahead_resid_.ptr, 2 * nf);
+    hip_check(hipEventRecord(res_->evs[3], stream3_), "event");
     ahead_launched_ = true;
     ahead_ready_ = true;
   }
@@ -830,8 +925,14 @@ class Engine {
         nd + max_designs > ahead_design_status_.cap || (nf + max_resid) * (size_t)n_ > ahead_resid_.cap ||
         (nf + max_resid) * (size_t)kpc::kMagicInfo > ahead_rinfo_.cap)
       return;
+    const bool with_comps = chain_comps_ && comps_hold(chain_comp_, nf + max_resid);
     swap_arena(design_, ahead_design_);
     swap_arena(design_status_, ahead_design_status_);
+    if (with_comps) {
+      swap_comps(chain_comp_);
+      n_comp_ = (int)nf;
+      claim_comps_ = true;
+    }
     swap_arena(resid_, ahead_resid_);
     swap_arena(rinfo_, ahead_rinfo_);
     // the maps move too: the lists are refilled for the level after this one
@@ -890,6 +991,7 @@ class Engine {
     wave_claimed_ = false;
     claimed_geos_.clear();
     claimed_fits_.clear();
+    claimed_comps_.clear();
     pair_rtab_.clear();
     for (Task* t : all) {
       if (use_results_ && with_pairs) {
@@ -979,6 +1081,26 @@ class Engine {
           if (const int* rt = resid_map_.find(key)) rslot[side] = *rt;
           continue;
         }
+        if (claim_comps_) {
+          int rs = -1;
+          if (ahead_claim_) {
+            if (const int* ahead = claim_resid_map_.find(key)) rs = *ahead;
+          } else if (l1_claim_) {
+            rs = tgt * p_ + t->S[0];
+          }
+          if (rs >= 0) {
+            // fit and component built ahead (component slot = residual slot)
+            const int dslot = ensure_design(*t, smooth_items, w.geo_items);
+            comp_map_.set(key, rs);
+            resid_map_.set(key, rs);
+            slots[side] = rs;
+            rslot[side] = rs;
+            claimed_fits_.push_back(kpc::MagicItem{dslot, tgt, rs, 0});
+            claimed_comps_.push_back(rs);
+            if (l1_claim_) wave_claimed_ = true;
+            continue;
+          }
+        }
         const int cs = n_comp_++;
         comp_map_.set(key, cs);
         slots[side] = cs;
@@ -1044,6 +1166,7 @@ class Engine {
     }
     stats_.fits += (long long)claimed_fits_.size();
     stats_.setups += (long long)claimed_geos_.size();
+    stats_.components += (long long)claimed_comps_.size();
     if (with_pairs) stats_.physical_tests += (long long)tasks.size();
     stage_host_ += now_s() - t0;
     const double* comp_source = level_ == 0 ? d_data_ : resid_.ptr;
@@ -1096,8 +1219,8 @@ class Engine {
     }
     // arenas built ahead on the third stream: both lanes follow that chain
     if (ahead_wait_) {
-      hip_check(hipStreamWaitEvent(stream_, res_->evs[2], 0), "stream wait");
-      hip_check(hipStreamWaitEvent(stream2_, res_->evs[2], 0), "stream wait");
+      hip_check(hipStreamWaitEvent(stream_, res_->evs[3], 0), "stream wait");
+      hip_check(hipStreamWaitEvent(stream2_, res_->evs[3], 0), "stream wait");
       ahead_wait_ = false;
     }
     for (int li = 0; li < nlanes; ++li) {
@@ -1180,8 +1303,11 @@ class Engine {
       if (nlanes > 1) {
         fits2.insert(fits2.end(), W[1].fit_items.begin(), W[1].fit_items.end());
         geos2.insert(geos2.end(), W[1].geo_items.begin(), W[1].geo_items.end());
+      }
+      if (nlanes > 1 || !claimed_comps_.empty()) {
         dst2 = W[0].comp_dst;
-        dst2.insert(dst2.end(), W[1].comp_dst.begin(), W[1].comp_dst.end());
+        if (nlanes > 1) dst2.insert(dst2.end(), W[1].comp_dst.begin(), W[1].comp_dst.end());
+        dst2.insert(dst2.end(), claimed_comps_.begin(), claimed_comps_.end());
         dst = &dst2;
       }
       fits2.insert(fits2.end(), claimed_fits_.begin(), claimed_fits_.end());
@@ -1883,6 +2009,15 @@ class Engine {
   std::vector<kpc::GeometryItem> ahead_geo_;
   std::vector<kpc::MagicItem> ahead_fits_;
   bool ahead_time_pending_ = false;  // an adopted chain's stage times
+  CompSet chain_comp_;  // components of the chain built ahead
+  DevArray<long long> chain_comp_src_;
+  DevArray<int> chain_comp_dst_;
+  DevArray<double> chain_scratch4_;
+  bool comps_ahead_ = true;    // chains build their fits' components
+  bool chain_comps_ = false;   // the last chain launched did
+  bool l1_comps_ = false;      // the level-1 chain did
+  bool claim_comps_ = false;   // this level claims components built ahead
+  std::vector<int> claimed_comps_;  // the wave's claimed components (status)
   FlatMap<uint64_t, int, U64Hash> ahead_design_map_, claim_design_map_;
   FlatMap<std::pair<int, uint64_t>, int, KeyHash> ahead_resid_map_, claim_resid_map_;
   DevArray<double> ahead_design_, ahead_design_status_, ahead_resid_, ahead_rinfo_;

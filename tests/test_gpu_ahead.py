@@ -3,9 +3,12 @@
 During level L (L >= 2) the skeleton may build level L + 1's additive designs
 and residual fits for every (target, S) the level-start graph allows (a
 superset of what level L + 1 can use, since adjacency only shrinks within a
-level); level L + 1 claims the slots its tests use.  The fits are the same
-computations on the same inputs, so every output must equal the skeleton run
-with the chain off (KPC_AHEAD_LIMIT=0), bit for bit.
+level); level L + 1 claims the slots its tests use.  Each chain (this one and
+level 1's, built during level 0) also computes its fits' components (Lanczos,
+incomplete Cholesky or dcc.perm row means), claimed with the fits.  The work
+is the same computations on the same inputs, so every output must equal the
+skeleton run with the chains off (KPC_AHEAD_LIMIT=0, KPC_AHEAD_COMPONENTS=0),
+bit for bit.
 """
 import os
 
@@ -15,19 +18,22 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def _run(data, method, limit):
+def _run(data, method, limit, components=True, num_col=35):
     import fastkpc_amd as fk
 
-    old = os.environ.get("KPC_AHEAD_LIMIT")
-    os.environ["KPC_AHEAD_LIMIT"] = str(limit)
+    env = {"KPC_AHEAD_LIMIT": str(limit), "KPC_AHEAD_COMPONENTS": "1" if components else "0"}
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
     try:
         return fk.fastkpc_compatible_cuda_skeleton(
-            data, 0.1, options={"numCol": 35, "ci_method": method, "index": 1})
+            data, 0.1, options={"numCol": num_col, "ci_method": method, "index": 1,
+                                "permutation_params": {"replicates": 60, "seed": 7}})
     finally:
-        if old is None:
-            del os.environ["KPC_AHEAD_LIMIT"]
-        else:
-            os.environ["KPC_AHEAD_LIMIT"] = old
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
 
 
 def _same(a, b):
@@ -64,3 +70,13 @@ def test_every_level_built_ahead_equals_on_demand():
 
     data = config_data("sem351x48")[0]
     _same(_run(data, "dcc.gamma", 1 << 20), _run(data, "dcc.gamma", 0))
+
+
+@pytest.mark.parametrize("cfg,method", [("sem351x48", "dcc.gamma"), ("sem200x8", "hsic.gamma"),
+                                        ("sem200x8", "hsic.perm"), ("sem200x8", "dcc.perm")])
+def test_components_built_ahead_equal_on_demand(cfg, method):
+    from fastkpc_amd.sem import config_data
+
+    data = config_data(cfg)[0]
+    nc = 35 if cfg == "sem351x48" else 20
+    _same(_run(data, method, 512, True, nc), _run(data, method, 0, False, nc))
